@@ -1,0 +1,194 @@
+/*
+ * phase_hip.h -- C ABI of the MI355X (gfx950) discretise-and-solve library.
+ *
+ * It sits behind two interfaces of the reference (obrienadam/Phase; paths are
+ * relative to its root):
+ *
+ *  1. the sparse-solver plugin class SparseMatrixSolver
+ *     (src/Math/SparseMatrixSolver.h:11-62), selected by the string
+ *     LinearAlgebra.<eqn>.lib through SparseMatrixSolverFactory::create
+ *     (src/Math/SparseMatrixSolverFactory.cpp:26-41).  A C++ adapter class
+ *     (phase_amd/host/HipSparseMatrixSolver.h) implements that class on top of
+ *     the phase_hip_* solver entry points below; its factory name is "hip".
+ *
+ *  2. the fv:: operator API (src/2D/Unstructured/FiniteVolume/Discretization/
+ *     Laplacian.h:7-167) over a FiniteVolumeGrid2D built by
+ *     FiniteVolumeGrid2D::init(nodes, cptr, cind)
+ *     (src/2D/Unstructured/FiniteVolumeGrid2D/FiniteVolumeGrid2D.cpp:20-35).
+ *     phase_hip_mesh_create takes exactly those arrays; the assembly entry
+ *     points build the equation's CSR and right-hand side on the device.
+ *
+ * Conventions (the reference's, unless stated):
+ *  - Index = int32, Scalar = double (src/Types/Types.h:1-14).
+ *  - An equation stores A and rhs with A x + rhs = 0; phase_hip_solve solves
+ *    A x = -rhs exactly as FiniteVolumeEquation<T>::solve passes -rhs_ to the
+ *    solver (src/2D/Unstructured/FiniteVolume/Equation/FiniteVolumeEquation.tpp:73).
+ *    phase_hip_set_rhs, like SparseMatrixSolver::setRhs, takes the right-hand
+ *    side b of A x = b directly.
+ *  - Host arrays are caller-owned and copied; device memory is owned by the
+ *    handle.  A handle is not thread-safe; each has its own HIP stream.
+ *  - Every function returns 0 on success or a negative PHASE_HIP_E* code; the
+ *    message is available from phase_hip_last_error(handle) (or, for a failed
+ *    create, phase_hip_last_error(NULL)).  The C++ adapter turns it into the
+ *    reference's Exception(class, method, description) (src/System/Exception.h).
+ *  - Boundary types use the values of FiniteVolumeField<T>::BoundaryType
+ *    (src/2D/Unstructured/FiniteVolume/Field/FiniteVolumeField.h:12).
+ */
+#ifndef PHASE_HIP_H
+#define PHASE_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct phase_hip_solver phase_hip_solver;
+typedef struct phase_hip_mesh phase_hip_mesh;
+
+enum {
+  PHASE_HIP_OK = 0,
+  PHASE_HIP_EINVAL = -1,     /* bad argument or inconsistent sizes */
+  PHASE_HIP_EDEVICE = -2,    /* HIP runtime error */
+  PHASE_HIP_ESTATE = -3,     /* call out of order (no pattern, no mesh ...) */
+  PHASE_HIP_ENOCONV = -4,    /* solve stopped at maxIters above tolerance */
+  PHASE_HIP_EBREAKDOWN = -5, /* zero pivot or Krylov breakdown */
+  PHASE_HIP_EBOUNDARY = -6   /* unsupported boundary type for the operator */
+};
+
+enum {
+  PHASE_HIP_BC_FIXED = 0,
+  PHASE_HIP_BC_NORMAL_GRADIENT = 1,
+  PHASE_HIP_BC_SYMMETRY = 2,
+  PHASE_HIP_BC_OUTFLOW = 3,
+  PHASE_HIP_BC_PARTIAL_SLIP = 4
+};
+
+/* Solve modes (phase_hip_setup key "mode"). */
+enum {
+  PHASE_HIP_MODE_FP64 = 0,   /* BiCGStab + ILU(0), all FP64 */
+  PHASE_HIP_MODE_MIXED = 1,  /* FP32 BiCGStab + ILU(0) inside FP64 iterative refinement */
+  PHASE_HIP_MODE_LEARNED = 2 /* MIXED with the learned initial guess / flexible preconditioner */
+};
+
+const char *phase_hip_last_error(const phase_hip_solver *s);
+int phase_hip_device_count(int *count);
+
+/* ------------------------------------------------------------------ solver */
+
+/* SparseMatrixSolverFactory::create("hip") -- SparseMatrixSolverFactory.cpp:26-41 */
+int phase_hip_solver_create(phase_hip_solver **s, int device);
+int phase_hip_solver_destroy(phase_hip_solver *s);
+
+/* SparseMatrixSolver::setup(ptree) -- TrilinosBelosSparseMatrixSolver.cpp:44-86.
+ * Keys: "tolerance" (1e-8), "maxIters" (500), "mode" (fp64|mixed|learned),
+ * "innerTolerance" (mixed: relative reduction per refinement pass, 1e-3),
+ * "blockRows" (rows per ILU block, <= 64), "fallbackRatio" (learned: minimum
+ * per-iteration residual reduction before falling back, 0.5).  Unknown keys
+ * are ignored, as the reference ignores ptree keys it does not read. */
+int phase_hip_solver_setup(phase_hip_solver *s, const char *key, const char *value);
+
+/* SparseMatrixSolver::setRank(int) -- SparseMatrixSolver.h:23 */
+int phase_hip_solver_set_rank(phase_hip_solver *s, int rank);
+
+/* SparseMatrixSolver::set(rowPtr, colInd, vals) -- SparseMatrixSolver.h:30-32,
+ * TrilinosSparseMatrixSolver.cpp:71-96.  nslots = rowPtr[n]; colInd < 0 marks
+ * an empty slot and is skipped.  When the pattern equals the previous call's
+ * only the values are uploaded; otherwise the ordering (RCM, ILU blocks and
+ * block colouring) is rebuilt. */
+int phase_hip_solver_set(phase_hip_solver *s, int n, const int *rowPtr, const int *colInd,
+                         const double *vals);
+
+/* SparseMatrixSolver::setRhs / setGuess -- SparseMatrixSolver.h:36-38 */
+int phase_hip_solver_set_rhs(phase_hip_solver *s, const double *b);
+int phase_hip_solver_set_guess(phase_hip_solver *s, const double *x0);
+
+/* SparseMatrixSolver::solve() -- SparseMatrixSolver.h:40; returns error() via
+ * *relres: the FP64 true relative residual ||b - A x||_2 / ||b||_2. */
+int phase_hip_solver_solve(phase_hip_solver *s, double *relres);
+
+/* SparseMatrixSolver::x(Index) for all rows, nIters(), error() --
+ * SparseMatrixSolver.h:46-54 */
+int phase_hip_solver_get_x(phase_hip_solver *s, double *x);
+int phase_hip_solver_n_iters(const phase_hip_solver *s);
+double phase_hip_solver_error(const phase_hip_solver *s);
+
+/* Per-solve statistics: inner iterations, refinement passes, fallbacks taken,
+ * device milliseconds of the last solve (HIP events on the handle's stream). */
+int phase_hip_solver_stats(const phase_hip_solver *s, int *innerIters, int *outerPasses,
+                           int *fallbacks, double *solveMs);
+
+/* Export the system held on the device, in the ORIGINAL row/column numbering
+ * and compact CSR form (empty slots dropped), each row in the solver's slot
+ * order.  nnz is queried with rowPtr == NULL.  Used for parity checks. */
+int phase_hip_solver_export(phase_hip_solver *s, int *nnz, int *rowPtr, int *colInd,
+                            double *vals, double *rhs);
+
+/* Device SpMV y = A x on the held system (original numbering in and out).
+ * FP64 when single == 0, FP32 values and vectors otherwise. */
+int phase_hip_solver_spmv(phase_hip_solver *s, int single, const double *x, double *y);
+
+/* The ILU(0) factor computed on the device, in SOLVER order (rows and columns
+ * permuted), compact CSR with sorted columns: L strictly below the diagonal
+ * (unit diagonal implied), U on and above.  perm[k] = original row of solver
+ * row k.  nnz is queried with rowPtr == NULL. */
+int phase_hip_solver_ilu_export(phase_hip_solver *s, int *nnz, int *perm, int *rowPtr,
+                                int *colInd, double *vals);
+
+/* Timing hooks for the benchmark: run `reps` device SpMVs / ILU applies on the
+ * resident system and return the mean device time of one launch (ms) from HIP
+ * events on the handle's stream. */
+int phase_hip_solver_time_kernels(phase_hip_solver *s, int reps, double *spmv64Ms,
+                                  double *spmv32Ms, double *ilu64Ms, double *ilu32Ms);
+
+/* ------------------------------------------------------------------ mesh */
+
+/* FiniteVolumeGrid2D::init(nodes, cptr, cind) -- FiniteVolumeGrid2D.cpp:20-35,
+ * createCell :84-114, init() :396-418.  xy: 2*nNodes doubles; cell c has nodes
+ * cind[cptr[c]..cptr[c+1]) counter-clockwise. */
+int phase_hip_mesh_create(phase_hip_mesh **m, int device, int nNodes, const double *xy,
+                          int nCells, const int *cptr, const int *cind);
+int phase_hip_mesh_destroy(phase_hip_mesh *m);
+const char *phase_hip_mesh_last_error(const phase_hip_mesh *m);
+int phase_hip_mesh_sizes(const phase_hip_mesh *m, int *nCells, int *nFaces,
+                         int *nInteriorFaces, int *nBoundaryFaces);
+/* Face topology (face ids in the reference's first-encounter order). */
+int phase_hip_mesh_faces(const phase_hip_mesh *m, int *lNode, int *rNode, int *lCell,
+                         int *rCell);
+/* Geometry computed on the device: cell volume / centroid (Cell.cpp:8-29),
+ * face centroid / unnormalised normal (Face.cpp:9-18). */
+int phase_hip_mesh_geometry(const phase_hip_mesh *m, double *cellVolume,
+                            double *cellCentroid, double *faceCentroid, double *faceSf);
+
+/* ------------------------------------------------------------------ operators */
+
+/* fv::laplacian into the solver's system (pattern = the mesh's cell graph):
+ *   variant 0: laplacian(Scalar gamma, phi, theta)  Laplacian.h:7-47
+ *   variant 1: laplacian(Scalar gamma, phi)         Laplacian.h:49-83
+ *   variant 2: laplacian(field gamma, phi, theta)   Laplacian.h:85-130
+ *   variant 3: laplacian(field gamma, phi)          Laplacian.h:132-167
+ * gammaFace / gamma0Face: per face (variants 2,3); bcType: per face (interior
+ * entries ignored; NULL = NORMAL_GRADIENT everywhere, the reference default,
+ * FiniteVolumeField.tpp:104-109); phiFace: boundary values; phi0Cell/phi0Face:
+ * previous time level (theta variants).  Pointers may be NULL when unused.
+ * The first call binds the solver to the mesh (its pattern); later calls
+ * replace values and rhs in place. */
+int phase_hip_assemble_laplacian(phase_hip_solver *s, phase_hip_mesh *m, int variant,
+                                 double gamma, const double *gammaFace,
+                                 const double *gamma0Face, double theta, const int *bcType,
+                                 const double *phiFace, const double *phi0Cell,
+                                 const double *phi0Face);
+
+/* CrsEquation::operator==(Vector) / operator-=(Vector) -- CrsEquation.cpp:282-311:
+ * rhs -= v (v per row, original numbering). */
+int phase_hip_equation_sub_rhs(phase_hip_solver *s, const double *v);
+
+/* The held equation in the reference's padded CrsEquation layout
+ * (CrsEquation.cpp:9-14: 5 slots per row, -1 = empty, first-touch column order
+ * of the last assembled variant) with the stored rhs (A x + rhs = 0).
+ * nslots is queried with rowPtr == NULL. */
+int phase_hip_equation_export_padded(phase_hip_solver *s, int *nslots, int *rowPtr,
+                                     int *colInd, double *vals, double *rhs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

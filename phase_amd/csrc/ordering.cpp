@@ -1,0 +1,414 @@
+// Host-side ordering of a sparsity pattern for the device solver.
+//
+// The reference hands its CSR (rows in cell order, src/Math/CrsEquation.cpp:9-14)
+// to Ifpack2's RILUK inside an additive Schwarz preconditioner
+// (src/Math/TrilinosBelosSparseMatrixSolver.cpp:69-82), which factors rows in
+// their natural order -- a strictly sequential recurrence.  For the GPU the
+// rows are renumbered once per pattern:
+//
+//   1. reverse Cuthill-McKee over the symmetrised graph (locality);
+//   2. rows grouped into blocks of <= 64 graph-connected rows, grown
+//      breadth-first from seeds taken in RCM order (one wavefront per block);
+//   3. blocks coloured greedily so that blocks of one colour share no
+//      coupling;
+//   4. solver order = blocks by (colour, creation index), rows inside a block
+//      in RCM order.
+//
+// ILU(0) is then computed exactly (no dropped couplings) in that order.  A
+// row depends only on earlier-coloured neighbour blocks and on earlier rows of
+// its own block, so one wavefront per block can run the in-block recurrence by
+// levels while blocks wait for their (few) predecessor blocks.
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <queue>
+
+#include "phase_internal.h"
+
+namespace phase {
+
+namespace {
+
+struct Graph {
+  std::vector<int> ptr, adj;
+  int deg(int i) const { return ptr[i + 1] - ptr[i]; }
+};
+
+Graph symmetrise(int n, const std::vector<int> &rp, const std::vector<int> &ci) {
+  std::vector<int> cnt(n, 0);
+  for (int i = 0; i < n; ++i)
+    for (int j = rp[i]; j < rp[i + 1]; ++j)
+      if (ci[j] != i) { ++cnt[i]; ++cnt[ci[j]]; }
+  Graph g;
+  g.ptr.assign(n + 1, 0);
+  for (int i = 0; i < n; ++i) g.ptr[i + 1] = g.ptr[i] + cnt[i];
+  std::vector<int> fill(g.ptr.begin(), g.ptr.end() - 1);
+  g.adj.resize(g.ptr[n]);
+  for (int i = 0; i < n; ++i)
+    for (int j = rp[i]; j < rp[i + 1]; ++j)
+      if (ci[j] != i) { g.adj[fill[i]++] = ci[j]; g.adj[fill[ci[j]]++] = i; }
+  // sort + unique each row, then compact
+  std::vector<int> ptr2(n + 1, 0);
+  int w = 0;
+  for (int i = 0; i < n; ++i) {
+    int b = g.ptr[i], e = g.ptr[i + 1];
+    std::sort(g.adj.begin() + b, g.adj.begin() + e);
+    int last = -1;
+    ptr2[i] = w;
+    for (int k = b; k < e; ++k)
+      if (g.adj[k] != last) { last = g.adj[k]; g.adj[w++] = last; }
+  }
+  ptr2[n] = w;
+  g.adj.resize(w);
+  g.ptr = ptr2;
+  return g;
+}
+
+// BFS level structure from root; returns the last level's nodes and depth.
+int bfsDepth(const Graph &g, int root, std::vector<int> &mark, int stamp, std::vector<int> &last) {
+  std::vector<int> cur{root}, nxt;
+  mark[root] = stamp;
+  int depth = 0;
+  while (true) {
+    nxt.clear();
+    for (int u : cur)
+      for (int k = g.ptr[u]; k < g.ptr[u + 1]; ++k) {
+        int w = g.adj[k];
+        if (mark[w] != stamp) { mark[w] = stamp; nxt.push_back(w); }
+      }
+    if (nxt.empty()) break;
+    cur.swap(nxt);
+    ++depth;
+  }
+  last = cur;
+  return depth;
+}
+
+std::vector<int> rcm(const Graph &g, int n) {
+  std::vector<int> order;
+  order.reserve(n);
+  std::vector<char> visited(n, 0);
+  std::vector<int> mark(n, -1), last;
+  int stamp = 0;
+  std::vector<int> byDeg(n);
+  std::iota(byDeg.begin(), byDeg.end(), 0);
+  std::stable_sort(byDeg.begin(), byDeg.end(), [&](int a, int b) { return g.deg(a) < g.deg(b); });
+  std::vector<int> nb;
+  for (int cand : byDeg) {
+    if (visited[cand]) continue;
+    // pseudo-peripheral root (George & Liu): walk to a far, low-degree node
+    int root = cand;
+    int depth = bfsDepth(g, root, mark, stamp++, last);
+    for (int rep = 0; rep < 4; ++rep) {
+      int best = last[0];
+      for (int v : last)
+        if (g.deg(v) < g.deg(best)) best = v;
+      int d2 = bfsDepth(g, best, mark, stamp++, last);
+      if (d2 <= depth) break;
+      depth = d2;
+      root = best;
+    }
+    size_t head = order.size();
+    order.push_back(root);
+    visited[root] = 1;
+    while (head < order.size()) {
+      int u = order[head++];
+      nb.clear();
+      for (int k = g.ptr[u]; k < g.ptr[u + 1]; ++k)
+        if (!visited[g.adj[k]]) { visited[g.adj[k]] = 1; nb.push_back(g.adj[k]); }
+      std::stable_sort(nb.begin(), nb.end(), [&](int a, int b) { return g.deg(a) < g.deg(b); });
+      order.insert(order.end(), nb.begin(), nb.end());
+    }
+  }
+  std::reverse(order.begin(), order.end());
+  return order;
+}
+
+}  // namespace
+
+void buildOrdering(Ordering &o, int n, const int *rowPtr, const int *colInd, int blockRows) {
+  if (n <= 0) fail(PHASE_HIP_EINVAL, "rank must be positive");
+  if (blockRows < 1 || blockRows > kLanes) fail(PHASE_HIP_EINVAL, "blockRows must be in [1,64]");
+  o = Ordering();
+  o.n = n;
+  // 1. compact pattern
+  o.cRowPtr.assign(n + 1, 0);
+  for (int i = 0; i < n; ++i) {
+    if (rowPtr[i + 1] < rowPtr[i]) fail(PHASE_HIP_EINVAL, "rowPtr is not monotone");
+    for (int j = rowPtr[i]; j < rowPtr[i + 1]; ++j) {
+      int c = colInd[j];
+      if (c < 0) continue;
+      if (c >= n) fail(PHASE_HIP_EINVAL, "column index out of range in row " + std::to_string(i));
+      o.cCol.push_back(c);
+      o.compactSrc.push_back(j);
+    }
+    o.cRowPtr[i + 1] = (int)o.cCol.size();
+  }
+  o.nnz = (int)o.cCol.size();
+
+  // 2. RCM
+  Graph g = symmetrise(n, o.cRowPtr, o.cCol);
+  std::vector<int> rorder = rcm(g, n);
+  std::vector<int> rpos(n);
+  for (int k = 0; k < n; ++k) rpos[rorder[k]] = k;
+
+  // 3. breadth-first aggregation into blocks of <= blockRows rows
+  std::vector<int> blk(n, -1);
+  std::vector<std::vector<int>> members;
+  std::vector<int> queue, nbs;
+  for (int s : rorder) {
+    if (blk[s] >= 0) continue;
+    int b = (int)members.size();
+    members.emplace_back();
+    auto &mem = members.back();
+    mem.push_back(s);
+    blk[s] = b;
+    queue.assign(1, s);
+    size_t head = 0;
+    while (head < queue.size() && (int)mem.size() < blockRows) {
+      int u = queue[head++];
+      nbs.clear();
+      for (int k = g.ptr[u]; k < g.ptr[u + 1]; ++k)
+        if (blk[g.adj[k]] < 0) nbs.push_back(g.adj[k]);
+      std::sort(nbs.begin(), nbs.end(), [&](int a, int c) { return rpos[a] < rpos[c]; });
+      for (int w : nbs) {
+        if ((int)mem.size() >= blockRows) break;
+        blk[w] = b;
+        mem.push_back(w);
+        queue.push_back(w);
+      }
+    }
+    std::sort(mem.begin(), mem.end(), [&](int a, int c) { return rpos[a] < rpos[c]; });
+  }
+  int nb = (int)members.size();
+
+  // 4. greedy colouring of the block graph
+  std::vector<int> colour(nb, -1), seen(nb + 64, -1);
+  int nColours = 0;
+  std::vector<int> used;
+  for (int b = 0; b < nb; ++b) {
+    used.clear();
+    for (int u : members[b])
+      for (int k = g.ptr[u]; k < g.ptr[u + 1]; ++k) {
+        int ob = blk[g.adj[k]];
+        if (ob != b && colour[ob] >= 0) used.push_back(colour[ob]);
+      }
+    std::sort(used.begin(), used.end());
+    int c = 0;
+    for (int x : used) {
+      if (x == c) ++c;
+      else if (x > c) break;
+    }
+    colour[b] = c;
+    nColours = std::max(nColours, c + 1);
+  }
+  o.nColours = nColours;
+
+  // 5. solver order
+  std::vector<int> border(nb);
+  std::iota(border.begin(), border.end(), 0);
+  std::stable_sort(border.begin(), border.end(), [&](int a, int c) { return colour[a] < colour[c]; });
+  o.perm.clear();
+  o.perm.reserve(n);
+  o.nBlocks = nb;
+  o.blocks.assign(nb, BlockInfo{});
+  std::vector<int> blockOfRow(n);
+  for (int bi = 0; bi < nb; ++bi) {
+    auto &mem = members[border[bi]];
+    o.blocks[bi].rowStart = (int)o.perm.size();
+    o.blocks[bi].nRows = (int)mem.size();
+    for (int r : mem) {
+      blockOfRow[o.perm.size()] = bi;
+      o.perm.push_back(r);
+    }
+  }
+  o.iperm.assign(n, 0);
+  for (int k = 0; k < n; ++k) o.iperm[o.perm[k]] = k;
+
+  // 6. SELL-64 layout of A over solver rows, original in-row order
+  o.nSlices = (n + kLanes - 1) / kLanes;
+  o.sliceOff.assign(o.nSlices + 1, 0);
+  for (int s = 0; s < o.nSlices; ++s) {
+    int K = 0;
+    for (int l = 0; l < kLanes && s * kLanes + l < n; ++l) {
+      int i = o.perm[s * kLanes + l];
+      K = std::max(K, o.cRowPtr[i + 1] - o.cRowPtr[i]);
+    }
+    o.sliceOff[s + 1] = o.sliceOff[s] + K * kLanes;
+  }
+  o.sellCol.assign(o.sliceOff[o.nSlices], -1);
+  o.sellSrc.assign(o.sliceOff[o.nSlices], -1);
+  for (int k = 0; k < n; ++k) {
+    int s = k / kLanes, l = k % kLanes, i = o.perm[k];
+    for (int e = 0, j = o.cRowPtr[i]; j < o.cRowPtr[i + 1]; ++j, ++e) {
+      int idx = o.sliceOff[s] + e * kLanes + l;
+      o.sellCol[idx] = o.iperm[o.cCol[j]];
+      o.sellSrc[idx] = o.compactSrc[j];
+    }
+  }
+
+  // 7. ILU(0) factor layout
+  struct Ent { int col, src; };
+  std::vector<std::vector<Ent>> srow(n);
+  o.rows.assign(n, RowInfo{});
+  std::vector<int> diagSrc(n, -1);
+  for (int k = 0; k < n; ++k) {
+    int i = o.perm[k];
+    auto &r = srow[k];
+    for (int j = o.cRowPtr[i]; j < o.cRowPtr[i + 1]; ++j) r.push_back({o.iperm[o.cCol[j]], o.compactSrc[j]});
+    std::sort(r.begin(), r.end(), [](const Ent &a, const Ent &b) { return a.col < b.col; });
+    // merge duplicate columns (possible only in hand-made patterns): keep first
+    std::vector<Ent> u;
+    for (auto &e : r)
+      if (u.empty() || u.back().col != e.col) u.push_back(e);
+    r.swap(u);
+  }
+  int lRun = 0, uRun = 0;
+  for (int b = 0; b < nb; ++b) {
+    BlockInfo &B = o.blocks[b];
+    int bs = B.rowStart, be = bs + B.nRows;
+    int kL = 0, kU = 0, nLev = 0, nULev = 0;
+    for (int k = bs; k < be; ++k) {
+      int nL = 0, nLoff = 0, nU = 0, nUin = 0, lev = 0;
+      for (auto &e : srow[k]) {
+        if (e.col < k) {
+          ++nL;
+          if (e.col < bs) ++nLoff;
+          else lev = std::max(lev, (int)o.rows[e.col].lev + 1);
+        } else if (e.col > k) {
+          ++nU;
+          if (e.col < be) ++nUin;
+        } else {
+          diagSrc[k] = e.src;
+        }
+      }
+      if (nL > 255 || nU > 255) fail(PHASE_HIP_EINVAL, "row too long for the ILU layout");
+      if (nU - nUin > 8)
+        fail(PHASE_HIP_EINVAL, "row couples to more than 8 rows of later ILU blocks");
+      o.rows[k].nL = (uint8_t)nL;
+      o.rows[k].nLoff = (uint8_t)nLoff;
+      o.rows[k].nU = (uint8_t)nU;
+      o.rows[k].nUin = (uint8_t)nUin;
+      o.rows[k].lev = (uint8_t)lev;
+      kL = std::max(kL, nL);
+      kU = std::max(kU, nU);
+      nLev = std::max(nLev, lev + 1);
+    }
+    for (int k = be - 1; k >= bs; --k) {
+      int ulev = 0;
+      for (auto &e : srow[k])
+        if (e.col > k && e.col < be) ulev = std::max(ulev, (int)o.rows[e.col].ulev + 1);
+      o.rows[k].ulev = (uint8_t)ulev;
+      nULev = std::max(nULev, ulev + 1);
+    }
+    B.lOff = lRun;
+    B.kL = kL;
+    B.uOff = uRun;
+    B.kU = kU;
+    B.nLev = nLev;
+    B.nULev = nULev;
+    lRun += kL * kLanes;
+    uRun += kU * kLanes;
+  }
+  o.nLSlots = lRun;
+  o.nUSlots = uRun;
+  o.lCol.assign(lRun, -1);
+  o.uCol.assign(uRun, -1);
+  o.facSrc.assign((size_t)lRun + uRun + n, -1);
+  // fac index of entry (row k, column c), or -1
+  auto facIndex = [&](int k, int c) -> int {
+    const BlockInfo &B = o.blocks[blockOfRow[k]];
+    int lane = k - B.rowStart, eL = 0, eU = 0;
+    for (auto &e : srow[k]) {
+      if (e.col < k) {
+        if (e.col == c) return B.lOff + eL * kLanes + lane;
+        ++eL;
+      } else if (e.col > k) {
+        if (e.col == c) return lRun + B.uOff + eU * kLanes + lane;
+        ++eU;
+      } else if (c == k) {
+        return lRun + uRun + k;
+      }
+    }
+    return -1;
+  };
+  for (int k = 0; k < n; ++k) {
+    const BlockInfo &B = o.blocks[blockOfRow[k]];
+    int lane = k - B.rowStart, eL = 0, eU = 0;
+    for (auto &e : srow[k]) {
+      if (e.col < k) {
+        int s = B.lOff + eL * kLanes + lane;
+        o.lCol[s] = e.col;
+        o.facSrc[s] = e.src;
+        ++eL;
+      } else if (e.col > k) {
+        int s = B.uOff + eU * kLanes + lane;
+        o.uCol[s] = e.col;
+        o.facSrc[lRun + s] = e.src;
+        ++eU;
+      }
+    }
+    o.facSrc[(size_t)lRun + uRun + k] = diagSrc[k];
+  }
+  // pred / succ block lists
+  std::vector<int> tmp;
+  for (int b = 0; b < nb; ++b) {
+    BlockInfo &B = o.blocks[b];
+    int bs = B.rowStart, be = bs + B.nRows;
+    tmp.clear();
+    for (int k = bs; k < be; ++k)
+      for (auto &e : srow[k])
+        if (e.col < bs) tmp.push_back(blockOfRow[e.col]);
+    std::sort(tmp.begin(), tmp.end());
+    tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+    B.predBeg = (int)o.pred.size();
+    o.pred.insert(o.pred.end(), tmp.begin(), tmp.end());
+    B.predEnd = (int)o.pred.size();
+    tmp.clear();
+    for (int k = bs; k < be; ++k)
+      for (auto &e : srow[k])
+        if (e.col >= be) tmp.push_back(blockOfRow[e.col]);
+    std::sort(tmp.begin(), tmp.end());
+    tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+    B.succBeg = (int)o.succ.size();
+    o.succ.insert(o.succ.end(), tmp.begin(), tmp.end());
+    B.succEnd = (int)o.succ.size();
+    if (B.predEnd - B.predBeg > kLanes || B.succEnd - B.succBeg > kLanes)
+      fail(PHASE_HIP_EINVAL, "block has more than 64 neighbour blocks");
+  }
+  // IKJ update lists: for L entry (k, c) in ascending c, every U entry (c, j)
+  // of row c whose column j is also in row k, in ascending j.
+  auto visitPairs = [&](auto &&emit) {
+    for (int k = 0; k < n; ++k) {
+      const BlockInfo &B = o.blocks[blockOfRow[k]];
+      int lane = k - B.rowStart, eL = 0;
+      for (auto &e : srow[k]) {
+        if (e.col >= k) break;
+        int slot = B.lOff + eL * kLanes + lane;
+        ++eL;
+        int c = e.col;
+        const BlockInfo &C = o.blocks[blockOfRow[c]];
+        int laneC = c - C.rowStart, eU = 0;
+        for (auto &f : srow[c]) {
+          if (f.col <= c) continue;
+          int src = lRun + C.uOff + eU * kLanes + laneC;
+          ++eU;
+          int dst = facIndex(k, f.col);
+          if (dst >= 0) emit(slot, src, dst);
+        }
+      }
+    }
+  };
+  o.updPtr.assign(lRun + 1, 0);
+  visitPairs([&](int slot, int, int) { ++o.updPtr[slot + 1]; });
+  for (int s = 0; s < lRun; ++s) o.updPtr[s + 1] += o.updPtr[s];
+  o.updSrc.resize(o.updPtr[lRun]);
+  o.updDst.resize(o.updPtr[lRun]);
+  std::vector<int> fillPos(o.updPtr.begin(), o.updPtr.end() - 1);
+  visitPairs([&](int slot, int src, int dst) {
+    o.updSrc[fillPos[slot]] = src;
+    o.updDst[fillPos[slot]++] = dst;
+  });
+}
+
+}  // namespace phase

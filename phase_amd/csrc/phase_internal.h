@@ -1,0 +1,129 @@
+// Internal declarations of libphase_hip: device buffers, the solver ordering
+// and the kernel launchers shared by solver.hip and mesh.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/phase_hip.h"
+
+namespace phase {
+
+struct HipError {
+  int code;
+  std::string msg;
+};
+
+#define PH_CHECK(expr)                                                                   \
+  do {                                                                                   \
+    hipError_t _e = (expr);                                                              \
+    if (_e != hipSuccess)                                                                \
+      throw ::phase::HipError{PHASE_HIP_EDEVICE,                                         \
+                              std::string(#expr) + ": " + hipGetErrorString(_e)};        \
+  } while (0)
+
+[[noreturn]] inline void fail(int code, const std::string &msg) { throw HipError{code, msg}; }
+
+// Owning device allocation (hipMalloc / hipFree), movable.
+template <class T>
+struct DBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  DBuf(DBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DBuf &operator=(DBuf &&o) noexcept {
+    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    return *this;
+  }
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t count) {
+    if (count == n && p) return;
+    release();
+    if (count) PH_CHECK(hipMalloc(&p, count * sizeof(T)));
+    n = count;
+  }
+  void upload(const std::vector<T> &h, hipStream_t s) {
+    alloc(h.size());
+    if (n) PH_CHECK(hipMemcpyAsync(p, h.data(), n * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+  void zero(hipStream_t s) {
+    if (n) PH_CHECK(hipMemsetAsync(p, 0, n * sizeof(T), s));
+  }
+};
+
+constexpr int kLanes = 64;  // rows per ILU block and per SpMV slice (one wavefront)
+
+// Block metadata of the ILU ordering (one wavefront per block).
+struct BlockInfo {
+  int rowStart;   // first solver row
+  int nRows;      // <= 64
+  int lOff, kL;   // L slots: lOff + e*64 + lane, e < kL
+  int uOff, kU;   // U slots
+  int nLev, nULev;  // in-block levels of the forward / backward sweeps
+  int predBeg, predEnd;  // blocks the forward sweep waits for (pred[])
+  int succBeg, succEnd;  // blocks the backward sweep waits for (succ[])
+};
+
+// Per-row ILU metadata: counts of L / U entries and the in-/off-block split.
+struct RowInfo {
+  uint8_t nL, nLoff, nU, nUin;
+  uint8_t lev, ulev, pad0, pad1;
+};
+
+// Host-built ordering of one sparsity pattern.  "compact" = the caller's CSR
+// with empty slots dropped, original numbering and in-row order.
+struct Ordering {
+  int n = 0;
+  int nnz = 0;                       // compact entries
+  std::vector<int> cRowPtr, cCol;    // compact pattern, original numbering
+  std::vector<int> compactSrc;       // compact entry -> caller slot (padded index)
+  std::vector<int> perm, iperm;      // solver row k <-> original row perm[k]
+  int nBlocks = 0, nColours = 0;
+  std::vector<BlockInfo> blocks;
+  std::vector<int> pred, succ;
+  std::vector<RowInfo> rows;          // per solver row
+  // SELL-64 SpMV layout over solver rows: slice s covers rows [64s, 64s+64)
+  int nSlices = 0;
+  std::vector<int> sliceOff;          // nSlices+1, in units of slots
+  std::vector<int> sellCol;           // solver column or -1
+  std::vector<int> sellSrc;           // compact entry index or -1
+  // ILU factor layout: fac[] = [L slots | U slots | diagonal (n)]
+  int nLSlots = 0, nUSlots = 0;
+  std::vector<int> lCol, uCol;        // solver column, -1 padding
+  std::vector<int> facSrc;            // per fac entry: compact entry index or -1
+  std::vector<int> updPtr;            // per L slot (nLSlots+1)
+  std::vector<int> updSrc, updDst;    // fac indices
+};
+
+void buildOrdering(Ordering &o, int n, const int *rowPtr, const int *colInd, int blockRows);
+
+}  // namespace phase
+
+namespace phase {
+
+// Device-resident scalars of one BiCGStab run.  Written only by the last
+// workgroup of each reduction kernel (deterministic fixed-order sums).
+struct Scalars {
+  double rho, alpha, omega, beta;
+  double rr, tol2;        // ||r||^2 of the recursive residual and the stop threshold
+  double rv, ts, tt;      // last (r0,v), (t,s), (t,t)
+  double rrPrev;          // ||r||^2 one iteration earlier (fallback monitor)
+  int done;               // 0 running, 1 converged, 2 breakdown, 3 max iterations, 4 stagnation
+  int iters, maxIters, pad;
+  double minRatio;        // learned mode: smallest accepted ||r_k||/||r_{k-1}|| ... (<=0 disables)
+};
+
+enum { kDoneRunning = 0, kDoneConverged = 1, kDoneBreakdown = 2, kDoneMaxIters = 3,
+       kDoneStagnation = 4 };
+
+}  // namespace phase

@@ -1,0 +1,92 @@
+// The solver handle behind the phase_hip_solver_* C entry points.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "phase_internal.h"
+
+namespace phase {
+
+template <class T>
+struct KrylovVecs {
+  DBuf<T> r, r0, p, v, s, t, ph, sh, y, x;
+  void alloc(int n) {
+    for (DBuf<T> *b : {&r, &r0, &p, &v, &s, &t, &ph, &sh, &y, &x}) b->alloc(n);
+  }
+};
+
+struct MeshBinding;  // mesh.hip: link -> slot maps of an assembled operator
+
+}  // namespace phase
+
+struct phase_hip_solver {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string err;
+
+  // setup() keys
+  double tol = 1e-8;
+  int maxIters = 500;
+  int mode = PHASE_HIP_MODE_FP64;
+  double innerTol = 1e-3;
+  int blockRows = 64;
+  double fallbackRatio = 0.5;
+  int maxOuter = 100;
+  int chunk = 8;  // iterations enqueued between host checks
+
+  // pattern and ordering
+  int rank = 0;
+  bool havePattern = false;
+  std::vector<int> patRowPtr, patColInd;
+  phase::Ordering ord;
+
+  phase::DBuf<phase::BlockInfo> dBlocks;
+  phase::DBuf<phase::RowInfo> dRows;
+  phase::DBuf<int> dPred, dSucc, dLCol, dUCol, dUpdPtr, dUpdSrc, dUpdDst;
+  phase::DBuf<int> dSliceOff, dSellCol, dSellSrc, dFacSrc, dPerm;
+
+  // values: caller slots (FP64), SpMV copies, ILU factors
+  phase::DBuf<double> dVals;
+  phase::DBuf<double> dSell64, dFac64;
+  phase::DBuf<float> dSell32, dFac32;
+  bool haveValues = false, sell32Valid = false, fac64Valid = false, fac32Valid = false;
+
+  // right-hand side / solution, solver order
+  phase::DBuf<double> dB, dX, dR64;
+  bool haveRhs = false, haveGuess = false, haveSolution = false;
+  phase::KrylovVecs<double> k64;
+  phase::KrylovVecs<float> k32;
+
+  // synchronisation and reductions
+  phase::DBuf<int> dFlags, dErr;
+  phase::DBuf<unsigned long long> dTicket;
+  unsigned long long ticketBase = 0;
+  int epoch = 0;
+  phase::DBuf<double> dPartials;
+  phase::DBuf<unsigned> dCounter;
+  phase::DBuf<phase::Scalars> dSc;  // [0] inner run, [1] outer residual, [2] misc
+  phase::Scalars *hSc = nullptr;    // pinned mirror of dSc
+  phase::DBuf<double> dMisc;        // [0] ||b||^2
+
+  // results of the last solve
+  int nIters = 0, innerIters = 0, outerPasses = 0, fallbacks = 0;
+  double error = 0.0, solveMs = 0.0;
+
+  // mesh-assembled operator (mesh.hip)
+  phase::MeshBinding *binding = nullptr;
+  phase_hip_mesh *boundMesh = nullptr;
+  int lastVariant = -1;
+};
+
+namespace phase {
+// Upload a new pattern (rebuilding the ordering) -- used by set() and by the
+// mesh assembly path.
+void solverSetPattern(phase_hip_solver *s, int n, const int *rowPtr, const int *colInd);
+// dVals changed on the device: refresh SpMV copies, invalidate factors.
+void solverValuesChanged(phase_hip_solver *s);
+void solverFreeBinding(phase_hip_solver *s);
+}  // namespace phase

@@ -1,0 +1,700 @@
+// Sparse solver behind SparseMatrixSolver (src/Math/SparseMatrixSolver.h:11-62):
+// right-preconditioned BiCGStab with ILU(0), the method the reference runs
+// through Belos + Ifpack2 (src/Math/TrilinosBelosSparseMatrixSolver.cpp:18-86),
+// in FP64 or as FP32 inner iterations inside FP64 iterative refinement.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+
+#include "solver.h"
+#include "solver_kernels.h"
+
+using namespace phase;
+
+namespace {
+
+thread_local std::string gLastCreateError;
+
+inline int gridFor(long n, int block, int cap) {
+  long g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  return (int)std::min<long>(g, cap);
+}
+
+constexpr int kElemCap = 4096;
+constexpr int kRedCap = 1024;
+
+template <class F>
+int guarded(phase_hip_solver *s, F &&f) {
+  if (!s) return PHASE_HIP_EINVAL;
+  try {
+    f();
+    return PHASE_HIP_OK;
+  } catch (const HipError &e) {
+    s->err = e.msg;
+    return e.code;
+  } catch (const std::exception &e) {
+    s->err = e.what();
+    return PHASE_HIP_EINVAL;
+  }
+}
+
+void setDevice(phase_hip_solver *s) { PH_CHECK(hipSetDevice(s->device)); }
+
+template <class T>
+IluDev<T> iluDev(phase_hip_solver *s, T *fac) {
+  IluDev<T> L;
+  L.blocks = s->dBlocks.p;
+  L.rows = s->dRows.p;
+  L.pred = s->dPred.p;
+  L.succ = s->dSucc.p;
+  L.lCol = s->dLCol.p;
+  L.uCol = s->dUCol.p;
+  L.fac = fac;
+  L.nb = s->ord.nBlocks;
+  L.nLSlots = s->ord.nLSlots;
+  L.nUSlots = s->ord.nUSlots;
+  L.n = s->rank;
+  return L;
+}
+
+SyncDev syncDev(phase_hip_solver *s) { return SyncDev{s->dFlags.p, s->dTicket.p, s->dErr.p}; }
+
+// Every sweep launch takes nb tickets and a fresh epoch.
+void nextSweep(phase_hip_solver *s, unsigned long long &base, int &epoch) {
+  base = s->ticketBase;
+  s->ticketBase += (unsigned long long)s->ord.nBlocks;
+  epoch = ++s->epoch;
+}
+
+int spmvGrid(phase_hip_solver *s) { return std::max(1, (s->ord.nSlices + 3) / 4); }
+
+void checkDeviceErr(phase_hip_solver *s) {
+  int e = 0;
+  PH_CHECK(hipMemcpyAsync(&e, s->dErr.p, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+  PH_CHECK(hipStreamSynchronize(s->stream));
+  if (e & 1) {
+    PH_CHECK(hipMemsetAsync(s->dErr.p, 0, sizeof(int), s->stream));
+    fail(PHASE_HIP_EDEVICE, "ILU sweep dependency wait timed out");
+  }
+  if (e & 2) {
+    PH_CHECK(hipMemsetAsync(s->dErr.p, 0, sizeof(int), s->stream));
+    fail(PHASE_HIP_EBREAKDOWN, "zero pivot in ILU(0) factorisation");
+  }
+}
+
+template <class T>
+void factorize(phase_hip_solver *s, DBuf<T> &fac) {
+  const int m = (int)fac.n;
+  hipLaunchKernelGGL((kGather<T>), dim3(gridFor(m, 256, kElemCap)), dim3(256), 0, s->stream, m,
+                     s->dFacSrc.p, s->dVals.p, fac.p);
+  unsigned long long base;
+  int ep;
+  nextSweep(s, base, ep);
+  hipLaunchKernelGGL((kIluFactor<T>), dim3(s->ord.nBlocks), dim3(64), 0, s->stream, iluDev<T>(s, fac.p),
+                     s->dUpdPtr.p, s->dUpdSrc.p, s->dUpdDst.p, syncDev(s), base, ep);
+  PH_CHECK(hipGetLastError());
+}
+
+void ensureFactor64(phase_hip_solver *s) {
+  if (s->fac64Valid) return;
+  s->dFac64.alloc((size_t)s->ord.nLSlots + s->ord.nUSlots + s->rank);
+  factorize<double>(s, s->dFac64);
+  checkDeviceErr(s);
+  s->fac64Valid = true;
+}
+
+void ensureSingle(phase_hip_solver *s) {
+  if (!s->sell32Valid) {
+    s->dSell32.alloc(s->dSell64.n);
+    const int m = (int)s->dSell64.n;
+    hipLaunchKernelGGL((kConvert<double, float>), dim3(gridFor(m, 256, kElemCap)), dim3(256), 0, s->stream,
+                       m, s->dSell64.p, s->dSell32.p);
+    s->sell32Valid = true;
+  }
+  if (!s->fac32Valid) {
+    s->dFac32.alloc((size_t)s->ord.nLSlots + s->ord.nUSlots + s->rank);
+    factorize<float>(s, s->dFac32);
+    checkDeviceErr(s);
+    s->fac32Valid = true;
+  }
+}
+
+template <class T>
+void iluApply(phase_hip_solver *s, DBuf<T> &fac, const T *in, T *tmp, T *out, const Scalars *sc) {
+  unsigned long long base;
+  int ep;
+  nextSweep(s, base, ep);
+  hipLaunchKernelGGL((kLSolve<T, kProNone>), dim3(s->ord.nBlocks), dim3(64), 0, s->stream,
+                     iluDev<T>(s, fac.p), syncDev(s), base, ep, sc, in, tmp, (T *)nullptr,
+                     (const T *)nullptr, (const T *)nullptr);
+  nextSweep(s, base, ep);
+  hipLaunchKernelGGL((kUSolve<T>), dim3(s->ord.nBlocks), dim3(64), 0, s->stream, iluDev<T>(s, fac.p),
+                     syncDev(s), base, ep, sc, tmp, out);
+}
+
+// One right-preconditioned BiCGStab iteration, all scalars on the device.
+template <class T>
+void iterate(phase_hip_solver *s, KrylovVecs<T> &K, DBuf<T> &fac, const T *sell, T *x, Scalars *sc) {
+  const int n = s->rank;
+  unsigned long long base;
+  int ep;
+  IluDev<T> L = iluDev<T>(s, fac.p);
+  SyncDev sy = syncDev(s);
+  // p = r + beta (p - omega v) ; ph = M^{-1} p
+  nextSweep(s, base, ep);
+  hipLaunchKernelGGL((kLSolve<T, kProP>), dim3(s->ord.nBlocks), dim3(64), 0, s->stream, L, sy, base, ep,
+                     sc, (const T *)nullptr, K.y.p, K.p.p, K.r.p, K.v.p);
+  nextSweep(s, base, ep);
+  hipLaunchKernelGGL((kUSolve<T>), dim3(s->ord.nBlocks), dim3(64), 0, s->stream, L, sy, base, ep, sc, K.y.p,
+                     K.ph.p);
+  // v = A ph ; alpha
+  hipLaunchKernelGGL((kSpmvDotV<T>), dim3(spmvGrid(s)), dim3(kRedBlock), 0, s->stream, n, s->ord.nSlices,
+                     s->dSliceOff.p, s->dSellCol.p, sell, K.ph.p, K.v.p, K.r0.p, sc, s->dPartials.p,
+                     s->dCounter.p);
+  // s = r - alpha v ; sh = M^{-1} s
+  nextSweep(s, base, ep);
+  hipLaunchKernelGGL((kLSolve<T, kProS>), dim3(s->ord.nBlocks), dim3(64), 0, s->stream, L, sy, base, ep,
+                     sc, (const T *)nullptr, K.y.p, K.s.p, K.r.p, K.v.p);
+  nextSweep(s, base, ep);
+  hipLaunchKernelGGL((kUSolve<T>), dim3(s->ord.nBlocks), dim3(64), 0, s->stream, L, sy, base, ep, sc, K.y.p,
+                     K.sh.p);
+  // t = A sh ; omega
+  hipLaunchKernelGGL((kSpmvDotT<T>), dim3(spmvGrid(s)), dim3(kRedBlock), 0, s->stream, n, s->ord.nSlices,
+                     s->dSliceOff.p, s->dSellCol.p, sell, K.sh.p, K.t.p, K.s.p, sc, s->dPartials.p,
+                     s->dCounter.p);
+  // x, r, rho, beta, convergence
+  hipLaunchKernelGGL((kUpdate<T>), dim3(gridFor(n, kRedBlock, kRedCap)), dim3(kRedBlock), 0, s->stream, n,
+                     sc, x, K.ph.p, K.sh.p, K.r.p, K.s.p, K.t.p, K.r0.p, s->dPartials.p, s->dCounter.p);
+}
+
+void pullScalars(phase_hip_solver *s) {
+  PH_CHECK(hipMemcpyAsync(s->hSc, s->dSc.p, 3 * sizeof(Scalars), hipMemcpyDeviceToHost, s->stream));
+  PH_CHECK(hipStreamSynchronize(s->stream));
+}
+
+double normB2(phase_hip_solver *s) {
+  const int n = s->rank;
+  hipLaunchKernelGGL((kNorm2<double>), dim3(gridFor(n, kRedBlock, kRedCap)), dim3(kRedBlock), 0, s->stream,
+                     n, s->dB.p, s->dMisc.p, s->dPartials.p, s->dCounter.p);
+  double b2 = 0.;
+  PH_CHECK(hipMemcpyAsync(&b2, s->dMisc.p, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  PH_CHECK(hipStreamSynchronize(s->stream));
+  return b2;
+}
+
+// Run BiCGStab chunks until the device says done; returns iterations taken.
+template <class T>
+int runKrylov(phase_hip_solver *s, KrylovVecs<T> &K, DBuf<T> &fac, const T *sell, T *x, int budget) {
+  Scalars *sc = s->dSc.p;  // slot 0
+  int it = 0;
+  while (true) {
+    for (int c = 0; c < s->chunk; ++c) iterate<T>(s, K, fac, sell, x, sc);
+    PH_CHECK(hipGetLastError());
+    pullScalars(s);
+    it = s->hSc[0].iters;
+    if (s->hSc[0].done != kDoneRunning || it >= budget) break;
+  }
+  checkDeviceErr(s);
+  return it;
+}
+
+void solveFp64(phase_hip_solver *s) {
+  const int n = s->rank;
+  ensureFactor64(s);
+  KrylovVecs<double> &K = s->k64;
+  K.alloc(n);
+  const double b2 = normB2(s);
+  s->innerIters = 0;
+  s->outerPasses = 0;
+  if (b2 == 0.) {
+    PH_CHECK(hipMemsetAsync(s->dX.p, 0, n * sizeof(double), s->stream));
+    s->error = 0.;
+    return;
+  }
+  const double tol2 = s->tol * s->tol * b2;
+  Scalars *sc = s->dSc.p;
+  int total = 0;
+  while (true) {
+    // r = b - A x (true residual), r0 = r
+    hipLaunchKernelGGL((kResidual<double>), dim3(spmvGrid(s)), dim3(kRedBlock), 0, s->stream, n,
+                       s->ord.nSlices, s->dSliceOff.p, s->dSellCol.p, s->dSell64.p, s->dX.p, s->dB.p, K.r.p,
+                       K.r0.p, sc, s->dPartials.p, s->dCounter.p);
+    hipLaunchKernelGGL((kBicgInit<double>), dim3(gridFor(n, 256, kElemCap)), dim3(256), 0, s->stream, n, sc,
+                       K.p.p, K.v.p, (double *)nullptr, tol2, s->maxIters - total, 0.0);
+    pullScalars(s);
+    s->error = std::sqrt(s->hSc[0].rr / b2);
+    ++s->outerPasses;
+    if (s->hSc[0].done == kDoneConverged || total >= s->maxIters) break;
+    int it = runKrylov<double>(s, K, s->dFac64, s->dSell64.p, s->dX.p, s->maxIters - total);
+    total += it;
+    if (s->hSc[0].done == kDoneBreakdown && it == 0) break;
+  }
+  s->innerIters = total;
+  s->nIters = total;
+}
+
+void solveMixed(phase_hip_solver *s) {
+  const int n = s->rank;
+  ensureSingle(s);
+  KrylovVecs<float> &K = s->k32;
+  K.alloc(n);
+  s->dR64.alloc(n);
+  const double b2 = normB2(s);
+  s->innerIters = 0;
+  s->outerPasses = 0;
+  if (b2 == 0.) {
+    PH_CHECK(hipMemsetAsync(s->dX.p, 0, n * sizeof(double), s->stream));
+    s->error = 0.;
+    return;
+  }
+  Scalars *inner = s->dSc.p, *outer = s->dSc.p + 1;
+  int total = 0;
+  for (int pass = 0; pass <= s->maxOuter; ++pass) {
+    hipLaunchKernelGGL((kResidual<double>), dim3(spmvGrid(s)), dim3(kRedBlock), 0, s->stream, n,
+                       s->ord.nSlices, s->dSliceOff.p, s->dSellCol.p, s->dSell64.p, s->dX.p, s->dB.p,
+                       s->dR64.p, (double *)nullptr, outer, s->dPartials.p, s->dCounter.p);
+    pullScalars(s);
+    s->error = std::sqrt(s->hSc[1].rr / b2);
+    if (s->error <= s->tol || total >= s->maxIters || pass == s->maxOuter) break;
+    ++s->outerPasses;
+    hipLaunchKernelGGL((kNormalise<float>), dim3(gridFor(n, 256, kElemCap)), dim3(256), 0, s->stream, n,
+                       s->dR64.p, K.r.p, K.r0.p, outer);
+    hipLaunchKernelGGL((kNorm2<float>), dim3(gridFor(n, kRedBlock, kRedCap)), dim3(kRedBlock), 0, s->stream,
+                       n, K.r.p, &inner->rr, s->dPartials.p, s->dCounter.p);
+    // stop the inner run at innerTol relative reduction, or at the FP64 goal
+    const double want = std::max(s->innerTol, 0.5 * s->tol / s->error);
+    hipLaunchKernelGGL((kBicgInit<float>), dim3(gridFor(n, 256, kElemCap)), dim3(256), 0, s->stream, n, inner,
+                       K.p.p, K.v.p, K.x.p, want * want, s->maxIters - total, 0.0);
+    int it = runKrylov<float>(s, K, s->dFac32, s->dSell32.p, K.x.p, s->maxIters - total);
+    total += it;
+    hipLaunchKernelGGL((kAccumulate<float>), dim3(gridFor(n, 256, kElemCap)), dim3(256), 0, s->stream, n,
+                       s->dX.p, K.x.p, outer);
+    if (it == 0) break;
+  }
+  s->innerIters = total;
+  s->nIters = total;
+}
+
+void allocCommon(phase_hip_solver *s) {
+  const int n = s->rank;
+  s->dB.alloc(n);
+  s->dX.alloc(n);
+  s->dFlags.alloc(std::max(1, s->ord.nBlocks));
+  s->dFlags.zero(s->stream);
+  s->epoch = 0;
+  s->dTicket.alloc(1);
+  s->dTicket.zero(s->stream);
+  s->ticketBase = 0;
+  s->dErr.alloc(1);
+  s->dErr.zero(s->stream);
+  const int maxGrid = std::max(spmvGrid(s), kRedCap);
+  s->dPartials.alloc(2 * (size_t)maxGrid);
+  s->dCounter.alloc(1);
+  s->dCounter.zero(s->stream);
+  s->dSc.alloc(3);
+  s->dSc.zero(s->stream);
+  s->dMisc.alloc(4);
+  if (!s->hSc) PH_CHECK(hipHostMalloc((void **)&s->hSc, 3 * sizeof(Scalars), hipHostMallocDefault));
+}
+
+}  // namespace
+
+namespace phase {
+
+void solverSetPattern(phase_hip_solver *s, int n, const int *rowPtr, const int *colInd) {
+  setDevice(s);
+  if (s->rank != 0 && n != s->rank) fail(PHASE_HIP_EINVAL, "pattern size differs from setRank");
+  buildOrdering(s->ord, n, rowPtr, colInd, s->blockRows);
+  s->rank = n;
+  const Ordering &o = s->ord;
+  hipStream_t st = s->stream;
+  s->dBlocks.upload(o.blocks, st);
+  s->dRows.upload(o.rows, st);
+  s->dPred.upload(o.pred.empty() ? std::vector<int>{0} : o.pred, st);
+  s->dSucc.upload(o.succ.empty() ? std::vector<int>{0} : o.succ, st);
+  s->dLCol.upload(o.lCol.empty() ? std::vector<int>{-1} : o.lCol, st);
+  s->dUCol.upload(o.uCol.empty() ? std::vector<int>{-1} : o.uCol, st);
+  s->dUpdPtr.upload(o.updPtr, st);
+  s->dUpdSrc.upload(o.updSrc.empty() ? std::vector<int>{0} : o.updSrc, st);
+  s->dUpdDst.upload(o.updDst.empty() ? std::vector<int>{0} : o.updDst, st);
+  s->dSliceOff.upload(o.sliceOff, st);
+  s->dSellCol.upload(o.sellCol, st);
+  s->dSellSrc.upload(o.sellSrc, st);
+  s->dFacSrc.upload(o.facSrc, st);
+  s->dPerm.upload(o.perm, st);
+  s->dSell64.alloc(o.sellCol.size());
+  s->patRowPtr.assign(rowPtr, rowPtr + n + 1);
+  s->patColInd.assign(colInd, colInd + rowPtr[n]);
+  s->dVals.alloc(std::max<size_t>(1, (size_t)rowPtr[n]));
+  s->havePattern = true;
+  s->haveValues = false;
+  s->fac64Valid = s->fac32Valid = s->sell32Valid = false;
+  allocCommon(s);
+  PH_CHECK(hipStreamSynchronize(st));
+}
+
+void solverValuesChanged(phase_hip_solver *s) {
+  const int m = (int)s->dSell64.n;
+  hipLaunchKernelGGL((kGather<double>), dim3(gridFor(m, 256, kElemCap)), dim3(256), 0, s->stream, m,
+                     s->dSellSrc.p, s->dVals.p, s->dSell64.p);
+  PH_CHECK(hipGetLastError());
+  s->haveValues = true;
+  s->fac64Valid = s->fac32Valid = s->sell32Valid = false;
+}
+
+}  // namespace phase
+
+extern "C" {
+
+const char *phase_hip_last_error(const phase_hip_solver *s) {
+  return s ? s->err.c_str() : gLastCreateError.c_str();
+}
+
+int phase_hip_device_count(int *count) {
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  if (e != hipSuccess) c = 0;
+  if (count) *count = c;
+  return PHASE_HIP_OK;
+}
+
+int phase_hip_solver_create(phase_hip_solver **out, int device) {
+  if (!out) return PHASE_HIP_EINVAL;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+    gLastCreateError = "no HIP device available";
+    return PHASE_HIP_EDEVICE;
+  }
+  if (device < 0 || device >= count) {
+    gLastCreateError = "device index out of range";
+    return PHASE_HIP_EINVAL;
+  }
+  phase_hip_solver *s = new phase_hip_solver();
+  s->device = device;
+  int rc = guarded(s, [&] {
+    setDevice(s);
+    PH_CHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    PH_CHECK(hipEventCreate(&s->ev0));
+    PH_CHECK(hipEventCreate(&s->ev1));
+  });
+  if (rc != PHASE_HIP_OK) {
+    gLastCreateError = s->err;
+    delete s;
+    return rc;
+  }
+  *out = s;
+  return PHASE_HIP_OK;
+}
+
+int phase_hip_solver_destroy(phase_hip_solver *s) {
+  if (!s) return PHASE_HIP_OK;
+  (void)hipSetDevice(s->device);
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  solverFreeBinding(s);
+  s->dBlocks.release();
+  if (s->hSc) (void)hipHostFree(s->hSc);
+  if (s->ev0) (void)hipEventDestroy(s->ev0);
+  if (s->ev1) (void)hipEventDestroy(s->ev1);
+  // device buffers are released by their destructors before the stream goes
+  hipStream_t st = s->stream;
+  delete s;
+  if (st) (void)hipStreamDestroy(st);
+  return PHASE_HIP_OK;
+}
+
+int phase_hip_solver_setup(phase_hip_solver *s, const char *key, const char *value) {
+  return guarded(s, [&] {
+    if (!key || !value) fail(PHASE_HIP_EINVAL, "null key or value");
+    std::string k(key), v(value);
+    std::string lv = v;
+    std::transform(lv.begin(), lv.end(), lv.begin(), ::tolower);
+    if (k == "tolerance") s->tol = std::stod(v);
+    else if (k == "maxIters") s->maxIters = std::stoi(v);
+    else if (k == "mode") {
+      if (lv == "fp64" || lv == "0") s->mode = PHASE_HIP_MODE_FP64;
+      else if (lv == "mixed" || lv == "1") s->mode = PHASE_HIP_MODE_MIXED;
+      else if (lv == "learned" || lv == "2") s->mode = PHASE_HIP_MODE_LEARNED;
+      else fail(PHASE_HIP_EINVAL, "unknown mode '" + v + "'");
+    } else if (k == "innerTolerance") s->innerTol = std::stod(v);
+    else if (k == "blockRows") {
+      int br = std::stoi(v);
+      if (br < 1 || br > 64) fail(PHASE_HIP_EINVAL, "blockRows must be in [1,64]");
+      if (br != s->blockRows && s->havePattern) s->havePattern = false;  // rebuild on next set
+      s->blockRows = br;
+    } else if (k == "fallbackRatio") s->fallbackRatio = std::stod(v);
+    else if (k == "maxOuter") s->maxOuter = std::stoi(v);
+    else if (k == "chunk") s->chunk = std::max(1, std::stoi(v));
+  });
+}
+
+int phase_hip_solver_set_rank(phase_hip_solver *s, int rank) {
+  return guarded(s, [&] {
+    if (rank <= 0) fail(PHASE_HIP_EINVAL, "rank must be positive");
+    if (rank != s->rank) {
+      s->havePattern = false;
+      s->haveRhs = s->haveGuess = s->haveSolution = false;
+      s->rank = 0;
+      solverFreeBinding(s);
+    }
+    s->rank = rank;
+  });
+}
+
+int phase_hip_solver_set(phase_hip_solver *s, int n, const int *rowPtr, const int *colInd,
+                         const double *vals) {
+  return guarded(s, [&] {
+    if (n <= 0 || !rowPtr || !colInd || !vals) fail(PHASE_HIP_EINVAL, "bad arguments to set()");
+    if (s->rank && n != s->rank) fail(PHASE_HIP_EINVAL, "set(): size differs from setRank");
+    setDevice(s);
+    const int nslots = rowPtr[n];
+    bool same = s->havePattern && (int)s->patRowPtr.size() == n + 1 &&
+                std::memcmp(s->patRowPtr.data(), rowPtr, (n + 1) * sizeof(int)) == 0 &&
+                (int)s->patColInd.size() == nslots &&
+                std::memcmp(s->patColInd.data(), colInd, nslots * sizeof(int)) == 0;
+    if (!same) {
+      solverFreeBinding(s);
+      s->rank = 0;
+      solverSetPattern(s, n, rowPtr, colInd);
+    }
+    PH_CHECK(hipMemcpyAsync(s->dVals.p, vals, nslots * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    solverValuesChanged(s);
+  });
+}
+
+int phase_hip_solver_set_rhs(phase_hip_solver *s, const double *b) {
+  return guarded(s, [&] {
+    if (!s->havePattern) fail(PHASE_HIP_ESTATE, "setRhs before set()");
+    setDevice(s);
+    const int n = s->rank;
+    s->dR64.alloc(n);
+    PH_CHECK(hipMemcpyAsync(s->dR64.p, b, n * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    hipLaunchKernelGGL((kPermuteIn<double, double>), dim3(gridFor(n, 256, kElemCap)), dim3(256), 0, s->stream, n,
+                       s->dPerm.p, s->dR64.p, s->dB.p, 1.0);
+    PH_CHECK(hipGetLastError());
+    s->haveRhs = true;
+  });
+}
+
+int phase_hip_solver_set_guess(phase_hip_solver *s, const double *x0) {
+  return guarded(s, [&] {
+    if (!s->havePattern) fail(PHASE_HIP_ESTATE, "setGuess before set()");
+    setDevice(s);
+    const int n = s->rank;
+    s->dR64.alloc(n);
+    PH_CHECK(hipMemcpyAsync(s->dR64.p, x0, n * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    hipLaunchKernelGGL((kPermuteIn<double, double>), dim3(gridFor(n, 256, kElemCap)), dim3(256), 0, s->stream, n,
+                       s->dPerm.p, s->dR64.p, s->dX.p, 1.0);
+    PH_CHECK(hipGetLastError());
+    s->haveGuess = true;
+  });
+}
+
+int phase_hip_solver_solve(phase_hip_solver *s, double *relres) {
+  return guarded(s, [&] {
+    if (!s->havePattern || !s->haveValues) fail(PHASE_HIP_ESTATE, "solve before set()");
+    if (!s->haveRhs) fail(PHASE_HIP_ESTATE, "solve before setRhs()");
+    setDevice(s);
+    // Warm start: like the reference's Tpetra x_ (TrilinosSparseMatrixSolver.cpp:31-38),
+    // the previous solution is the initial guess unless setGuess() gave one.
+    if (!s->haveGuess && !s->haveSolution) PH_CHECK(hipMemsetAsync(s->dX.p, 0, s->rank * sizeof(double), s->stream));
+    PH_CHECK(hipEventRecord(s->ev0, s->stream));
+    s->fallbacks = 0;
+    if (s->mode == PHASE_HIP_MODE_FP64) solveFp64(s);
+    else solveMixed(s);
+    PH_CHECK(hipEventRecord(s->ev1, s->stream));
+    PH_CHECK(hipEventSynchronize(s->ev1));
+    float ms = 0.f;
+    PH_CHECK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+    s->solveMs = ms;
+    s->haveGuess = false;
+    s->haveSolution = true;
+    if (relres) *relres = s->error;
+    if (!(s->error <= s->tol))
+      fail(PHASE_HIP_ENOCONV, "BiCGStab stopped at relative residual " + std::to_string(s->error) +
+                                  " after " + std::to_string(s->nIters) + " iterations");
+  });
+}
+
+int phase_hip_solver_get_x(phase_hip_solver *s, double *x) {
+  return guarded(s, [&] {
+    if (!s->havePattern) fail(PHASE_HIP_ESTATE, "no system");
+    setDevice(s);
+    const int n = s->rank;
+    s->dR64.alloc(n);
+    hipLaunchKernelGGL((kPermuteOut<double>), dim3(gridFor(n, 256, kElemCap)), dim3(256), 0, s->stream, n,
+                       s->dPerm.p, s->dX.p, s->dR64.p);
+    PH_CHECK(hipMemcpyAsync(x, s->dR64.p, n * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    PH_CHECK(hipStreamSynchronize(s->stream));
+  });
+}
+
+int phase_hip_solver_n_iters(const phase_hip_solver *s) { return s ? s->nIters : 0; }
+double phase_hip_solver_error(const phase_hip_solver *s) { return s ? s->error : 0.0; }
+
+int phase_hip_solver_stats(const phase_hip_solver *s, int *innerIters, int *outerPasses, int *fallbacks,
+                           double *solveMs) {
+  if (!s) return PHASE_HIP_EINVAL;
+  if (innerIters) *innerIters = s->innerIters;
+  if (outerPasses) *outerPasses = s->outerPasses;
+  if (fallbacks) *fallbacks = s->fallbacks;
+  if (solveMs) *solveMs = s->solveMs;
+  return PHASE_HIP_OK;
+}
+
+int phase_hip_solver_export(phase_hip_solver *s, int *nnz, int *rowPtr, int *colInd, double *vals,
+                            double *rhs) {
+  return guarded(s, [&] {
+    if (!s->havePattern || !s->haveValues) fail(PHASE_HIP_ESTATE, "no system");
+    const Ordering &o = s->ord;
+    if (nnz) *nnz = o.nnz;
+    if (!rowPtr) return;
+    setDevice(s);
+    std::vector<double> hv(s->dVals.n);
+    PH_CHECK(hipMemcpyAsync(hv.data(), s->dVals.p, hv.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    PH_CHECK(hipStreamSynchronize(s->stream));
+    std::copy(o.cRowPtr.begin(), o.cRowPtr.end(), rowPtr);
+    std::copy(o.cCol.begin(), o.cCol.end(), colInd);
+    for (int j = 0; j < o.nnz; ++j) vals[j] = hv[o.compactSrc[j]];
+    if (rhs) {
+      const int n = s->rank;
+      std::vector<double> hb(n, 0.0);
+      if (s->haveRhs) {
+        s->dR64.alloc(n);
+        hipLaunchKernelGGL((kPermuteOut<double>), dim3(gridFor(n, 256, kElemCap)), dim3(256), 0, s->stream, n,
+                           s->dPerm.p, s->dB.p, s->dR64.p);
+        PH_CHECK(hipMemcpyAsync(hb.data(), s->dR64.p, n * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+        PH_CHECK(hipStreamSynchronize(s->stream));
+      }
+      for (int i = 0; i < n; ++i) rhs[i] = -hb[i];
+    }
+  });
+}
+
+int phase_hip_solver_spmv(phase_hip_solver *s, int single, const double *x, double *y) {
+  return guarded(s, [&] {
+    if (!s->havePattern || !s->haveValues) fail(PHASE_HIP_ESTATE, "no system");
+    setDevice(s);
+    const int n = s->rank;
+    s->dR64.alloc(n);
+    PH_CHECK(hipMemcpyAsync(s->dR64.p, x, n * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    if (!single) {
+      s->k64.alloc(n);
+      hipLaunchKernelGGL((kPermuteIn<double, double>), dim3(gridFor(n, 256, kElemCap)), dim3(256), 0, s->stream, n,
+                         s->dPerm.p, s->dR64.p, s->k64.y.p, 1.0);
+      hipLaunchKernelGGL((kSpmv<double>), dim3(spmvGrid(s)), dim3(kRedBlock), 0, s->stream, n, s->ord.nSlices,
+                         s->dSliceOff.p, s->dSellCol.p, s->dSell64.p, s->k64.y.p, s->k64.t.p);
+      hipLaunchKernelGGL((kPermuteOut<double>), dim3(gridFor(n, 256, kElemCap)), dim3(256), 0, s->stream, n,
+                         s->dPerm.p, s->k64.t.p, s->dR64.p);
+    } else {
+      ensureSingle(s);
+      s->k32.alloc(n);
+      hipLaunchKernelGGL((kPermuteIn<double, float>), dim3(gridFor(n, 256, kElemCap)), dim3(256), 0, s->stream, n,
+                         s->dPerm.p, s->dR64.p, s->k32.y.p, 1.0);
+      hipLaunchKernelGGL((kSpmv<float>), dim3(spmvGrid(s)), dim3(kRedBlock), 0, s->stream, n, s->ord.nSlices,
+                         s->dSliceOff.p, s->dSellCol.p, s->dSell32.p, s->k32.y.p, s->k32.t.p);
+      hipLaunchKernelGGL((kPermuteOut<float>), dim3(gridFor(n, 256, kElemCap)), dim3(256), 0, s->stream, n,
+                         s->dPerm.p, s->k32.t.p, s->dR64.p);
+    }
+    PH_CHECK(hipGetLastError());
+    PH_CHECK(hipMemcpyAsync(y, s->dR64.p, n * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    PH_CHECK(hipStreamSynchronize(s->stream));
+  });
+}
+
+int phase_hip_solver_ilu_export(phase_hip_solver *s, int *nnz, int *perm, int *rowPtr, int *colInd,
+                                double *vals) {
+  return guarded(s, [&] {
+    if (!s->havePattern || !s->haveValues) fail(PHASE_HIP_ESTATE, "no system");
+    setDevice(s);
+    const Ordering &o = s->ord;
+    const int n = s->rank;
+    long total = 0;
+    for (int k = 0; k < n; ++k) total += o.rows[k].nL + o.rows[k].nU + 1;
+    if (nnz) *nnz = (int)total;
+    if (!rowPtr) return;
+    ensureFactor64(s);
+    std::vector<double> f(s->dFac64.n);
+    PH_CHECK(hipMemcpyAsync(f.data(), s->dFac64.p, f.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    PH_CHECK(hipStreamSynchronize(s->stream));
+    std::copy(o.perm.begin(), o.perm.end(), perm);
+    int w = 0;
+    rowPtr[0] = 0;
+    for (int b = 0; b < o.nBlocks; ++b) {
+      const BlockInfo &B = o.blocks[b];
+      for (int lane = 0; lane < B.nRows; ++lane) {
+        const int k = B.rowStart + lane;
+        const RowInfo &ri = o.rows[k];
+        for (int e = 0; e < ri.nL; ++e) {
+          int sl = B.lOff + e * kLanes + lane;
+          colInd[w] = o.lCol[sl];
+          vals[w++] = f[sl];
+        }
+        colInd[w] = k;
+        vals[w++] = f[(size_t)o.nLSlots + o.nUSlots + k];
+        for (int e = 0; e < ri.nU; ++e) {
+          int sl = B.uOff + e * kLanes + lane;
+          colInd[w] = o.uCol[sl];
+          vals[w++] = f[(size_t)o.nLSlots + sl];
+        }
+        rowPtr[k + 1] = w;
+      }
+    }
+  });
+}
+
+int phase_hip_solver_time_kernels(phase_hip_solver *s, int reps, double *spmv64Ms, double *spmv32Ms,
+                                  double *ilu64Ms, double *ilu32Ms) {
+  return guarded(s, [&] {
+    if (!s->havePattern || !s->haveValues) fail(PHASE_HIP_ESTATE, "no system");
+    setDevice(s);
+    const int n = s->rank;
+    reps = std::max(1, reps);
+    ensureFactor64(s);
+    ensureSingle(s);
+    s->k64.alloc(n);
+    s->k32.alloc(n);
+    PH_CHECK(hipMemsetAsync(s->k64.ph.p, 0, n * sizeof(double), s->stream));
+    PH_CHECK(hipMemsetAsync(s->k32.ph.p, 0, n * sizeof(float), s->stream));
+    hipLaunchKernelGGL((kConvert<double, double>), dim3(gridFor(n, 256, kElemCap)), dim3(256), 0, s->stream, n,
+                       s->dB.p, s->k64.ph.p);
+    hipLaunchKernelGGL((kConvert<double, float>), dim3(gridFor(n, 256, kElemCap)), dim3(256), 0, s->stream, n,
+                       s->dB.p, s->k32.ph.p);
+    auto timeIt = [&](auto &&launch) {
+      launch();  // warm
+      PH_CHECK(hipEventRecord(s->ev0, s->stream));
+      for (int r = 0; r < reps; ++r) launch();
+      PH_CHECK(hipEventRecord(s->ev1, s->stream));
+      PH_CHECK(hipEventSynchronize(s->ev1));
+      float ms = 0.f;
+      PH_CHECK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+      return (double)ms / reps;
+    };
+    double a = timeIt([&] {
+      hipLaunchKernelGGL((kSpmv<double>), dim3(spmvGrid(s)), dim3(kRedBlock), 0, s->stream, n, s->ord.nSlices,
+                         s->dSliceOff.p, s->dSellCol.p, s->dSell64.p, s->k64.ph.p, s->k64.t.p);
+    });
+    double b = timeIt([&] {
+      hipLaunchKernelGGL((kSpmv<float>), dim3(spmvGrid(s)), dim3(kRedBlock), 0, s->stream, n, s->ord.nSlices,
+                         s->dSliceOff.p, s->dSellCol.p, s->dSell32.p, s->k32.ph.p, s->k32.t.p);
+    });
+    double c = timeIt([&] {
+      iluApply<double>(s, s->dFac64, s->k64.ph.p, s->k64.y.p, s->k64.sh.p, (const Scalars *)nullptr);
+    });
+    double d = timeIt([&] {
+      iluApply<float>(s, s->dFac32, s->k32.ph.p, s->k32.y.p, s->k32.sh.p, (const Scalars *)nullptr);
+    });
+    checkDeviceErr(s);
+    if (spmv64Ms) *spmv64Ms = a;
+    if (spmv32Ms) *spmv32Ms = b;
+    if (ilu64Ms) *ilu64Ms = c;
+    if (ilu32Ms) *ilu32Ms = d;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,490 @@
+// Device kernels of the sparse solver (included by solver.hip only).
+//
+// Data layout (all in "solver order", see ordering.cpp):
+//  * A in SELL-64: slice s = rows [64s, 64s+64); entry e of lane l at
+//    sliceOff[s] + 64*e + l; column -1 = padding.  Rows keep the caller's
+//    in-row order, so y = A x sums each row in the same order as a plain CSR
+//    loop over the caller's slots.
+//  * ILU(0) factor fac = [L slots | U slots | diagonal]; block b (<= 64 rows,
+//    one wavefront) keeps its L / U entries at lOff / uOff + 64*e + lane,
+//    ascending column per row.
+//  * Krylov vectors are dense arrays in solver order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "phase_internal.h"
+
+namespace phase {
+
+constexpr int kMaxOffU = 8;  // off-block U entries of one row held in registers
+constexpr int kRedBlock = 256;
+
+enum { kProNone = 0, kProP = 1, kProS = 2 };
+
+template <class T>
+struct IluDev {
+  const BlockInfo *blocks;
+  const RowInfo *rows;
+  const int *pred, *succ, *lCol, *uCol;
+  T *fac;
+  int nb, nLSlots, nUSlots, n;
+};
+
+struct SyncDev {
+  int *flags;
+  unsigned long long *ticket;
+  int *err;
+};
+
+__device__ __forceinline__ int ldAcquire(const int *p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void stRelease(int *p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Bounded wait for a block's completion flag: every wait ends (after about a
+// second at worst, flagging an error) so a wrong dependency can never hang.
+__device__ __forceinline__ void waitFlag(const int *f, int epoch, int *err) {
+  for (int it = 0; it < (1 << 24); ++it) {
+    if (ldAcquire(f) >= epoch) return;
+    __builtin_amdgcn_s_sleep(1);
+  }
+  atomicOr(err, 1);
+}
+
+// Take the next block of a sweep.  Tickets are handed out in launch order, so
+// every block a ticket holder waits on belongs to an earlier ticket holder that
+// is already running: the sweep cannot deadlock, whatever the occupancy.
+__device__ __forceinline__ long long takeTicket(unsigned long long *ticket,
+                                                unsigned long long base) {
+  __shared__ unsigned long long sTicket;
+  if (threadIdx.x == 0) sTicket = atomicAdd(ticket, 1ULL);
+  __syncthreads();
+  return (long long)(sTicket - base);
+}
+
+__device__ __forceinline__ double waveSum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Deterministic grid reduction: per-block fixed-order sums go to partials; the
+// last block to finish (atomic ticket) sums the partials in index order.
+// Returns true in the last block; thread 0 then holds the totals in v.
+template <int NV>
+__device__ bool reduceLast(double (&v)[NV], double *partials, unsigned *counter) {
+  __shared__ double sred[NV][kRedBlock / 64];
+  __shared__ bool sLast;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    double s = waveSum(v[q]);
+    if (lane == 0) sred[q][wave] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      double s = 0.;
+      for (int w = 0; w < kRedBlock / 64; ++w) s += sred[q][w];
+      partials[q * gridDim.x + blockIdx.x] = s;
+    }
+    __threadfence();
+    unsigned t = atomicAdd(counter, 1u);
+    sLast = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!sLast) return false;
+  __threadfence();
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    double s = 0.;
+    for (int i = threadIdx.x; i < (int)gridDim.x; i += kRedBlock) s += partials[q * gridDim.x + i];
+    s = waveSum(s);
+    __syncthreads();
+    if (lane == 0) sred[q][wave] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double tsum = 0.;
+      for (int w = 0; w < kRedBlock / 64; ++w) tsum += sred[q][w];
+      v[q] = tsum;
+    }
+  }
+  if (threadIdx.x == 0) *counter = 0u;
+  return true;
+}
+
+// ---------------------------------------------------------------- data movement
+template <class T>
+__global__ void kGather(int m, const int *__restrict__ map, const double *__restrict__ src,
+                        T *__restrict__ dst) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
+    const int s = map[i];
+    dst[i] = s >= 0 ? (T)src[s] : (T)0;
+  }
+}
+
+// out[k] = in[perm[k]] (original -> solver order)
+template <class TI, class TO>
+__global__ void kPermuteIn(int n, const int *__restrict__ perm, const TI *__restrict__ in,
+                           TO *__restrict__ out, double scale) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+    out[k] = (TO)(scale * (double)in[perm[k]]);
+}
+
+// out[perm[k]] = in[k] (solver -> original order)
+template <class T>
+__global__ void kPermuteOut(int n, const int *__restrict__ perm, const T *__restrict__ in,
+                            double *__restrict__ out) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+    out[perm[k]] = (double)in[k];
+}
+
+template <class TI, class TO>
+__global__ void kConvert(int n, const TI *__restrict__ in, TO *__restrict__ out) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    out[i] = (TO)in[i];
+}
+
+// ---------------------------------------------------------------- SpMV (SELL-64)
+template <class T>
+__device__ __forceinline__ T sellRow(int slice, int lane, const int *__restrict__ sliceOff,
+                                     const int *__restrict__ col, const T *__restrict__ val,
+                                     const T *__restrict__ x) {
+  const int o = sliceOff[slice];
+  const int K = (sliceOff[slice + 1] - o) >> 6;
+  T acc = 0;
+  for (int e = 0; e < K; ++e) {
+    const int idx = o + (e << 6) + lane;
+    const int c = col[idx];
+    if (c >= 0) acc = fma(val[idx], x[c], acc);
+  }
+  return acc;
+}
+
+// y = A x
+template <class T>
+__global__ __launch_bounds__(kRedBlock) void kSpmv(int n, int nSlices, const int *__restrict__ sliceOff,
+                                                   const int *__restrict__ col, const T *__restrict__ val,
+                                                   const T *__restrict__ x, T *__restrict__ y) {
+  const int slice = blockIdx.x * (kRedBlock / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (slice >= nSlices) return;
+  const int row = slice * 64 + lane;
+  if (row >= n) return;
+  y[row] = sellRow(slice, lane, sliceOff, col, val, x);
+}
+
+// v = A ph ; alpha = rho / (r0, v)
+template <class T>
+__global__ __launch_bounds__(kRedBlock) void kSpmvDotV(int n, int nSlices, const int *__restrict__ sliceOff,
+                                                       const int *__restrict__ col, const T *__restrict__ val,
+                                                       const T *__restrict__ ph, T *__restrict__ v,
+                                                       const T *__restrict__ r0, Scalars *sc,
+                                                       double *partials, unsigned *counter) {
+  if (sc->done) return;
+  const int slice = blockIdx.x * (kRedBlock / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int row = slice * 64 + lane;
+  double d[1] = {0.};
+  if (slice < nSlices && row < n) {
+    const T y = sellRow(slice, lane, sliceOff, col, val, ph);
+    v[row] = y;
+    d[0] = (double)r0[row] * (double)y;
+  }
+  if (reduceLast<1>(d, partials, counter) && threadIdx.x == 0) {
+    sc->rv = d[0];
+    if (d[0] == 0.) sc->done = kDoneBreakdown;
+    else sc->alpha = sc->rho / d[0];
+  }
+}
+
+// t = A sh ; omega = (t, s) / (t, t)
+template <class T>
+__global__ __launch_bounds__(kRedBlock) void kSpmvDotT(int n, int nSlices, const int *__restrict__ sliceOff,
+                                                       const int *__restrict__ col, const T *__restrict__ val,
+                                                       const T *__restrict__ sh, T *__restrict__ t,
+                                                       const T *__restrict__ s, Scalars *sc,
+                                                       double *partials, unsigned *counter) {
+  if (sc->done) return;
+  const int slice = blockIdx.x * (kRedBlock / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int row = slice * 64 + lane;
+  double d[2] = {0., 0.};
+  if (slice < nSlices && row < n) {
+    const T y = sellRow(slice, lane, sliceOff, col, val, sh);
+    t[row] = y;
+    d[0] = (double)y * (double)s[row];
+    d[1] = (double)y * (double)y;
+  }
+  if (reduceLast<2>(d, partials, counter) && threadIdx.x == 0) {
+    sc->ts = d[0];
+    sc->tt = d[1];
+    sc->omega = d[1] > 0. ? d[0] / d[1] : 0.;
+  }
+}
+
+// r = b - A x (optionally r0 = r); sc->rr = (r, r)
+template <class T>
+__global__ __launch_bounds__(kRedBlock) void kResidual(int n, int nSlices, const int *__restrict__ sliceOff,
+                                                       const int *__restrict__ col, const T *__restrict__ val,
+                                                       const T *__restrict__ x, const T *__restrict__ b,
+                                                       T *__restrict__ r, T *__restrict__ r0, Scalars *sc,
+                                                       double *partials, unsigned *counter) {
+  const int slice = blockIdx.x * (kRedBlock / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int row = slice * 64 + lane;
+  double d[1] = {0.};
+  if (slice < nSlices && row < n) {
+    const T y = sellRow(slice, lane, sliceOff, col, val, x);
+    const T ri = b[row] - y;
+    r[row] = ri;
+    if (r0) r0[row] = ri;
+    d[0] = (double)ri * (double)ri;
+  }
+  if (reduceLast<1>(d, partials, counter) && threadIdx.x == 0) sc->rr = d[0];
+}
+
+// *out = (v, v)
+template <class T>
+__global__ __launch_bounds__(kRedBlock) void kNorm2(int n, const T *__restrict__ v, double *out,
+                                                    double *partials, unsigned *counter) {
+  double d[1] = {0.};
+  for (int i = blockIdx.x * kRedBlock + threadIdx.x; i < n; i += gridDim.x * kRedBlock)
+    d[0] += (double)v[i] * (double)v[i];
+  if (reduceLast<1>(d, partials, counter) && threadIdx.x == 0) *out = d[0];
+}
+
+// ---------------------------------------------------------------- BiCGStab vector steps
+template <class T>
+__global__ void kBicgInit(int n, Scalars *sc, T *p, T *v, T *x, double tol2, int maxIters,
+                          double minRatio) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    p[i] = 0;
+    v[i] = 0;
+    if (x) x[i] = 0;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->rho = sc->rr;
+    sc->rrPrev = sc->rr;
+    sc->alpha = 1.;
+    sc->omega = 1.;
+    sc->beta = 0.;
+    sc->tol2 = tol2;
+    sc->iters = 0;
+    sc->maxIters = maxIters;
+    sc->minRatio = minRatio;
+    sc->done = sc->rr <= tol2 ? kDoneConverged : kDoneRunning;
+  }
+}
+
+// x += alpha ph + omega sh ; r = s - omega t ; rho' = (r0, r) ; rr = (r, r)
+template <class T>
+__global__ __launch_bounds__(kRedBlock) void kUpdate(int n, Scalars *sc, T *__restrict__ x,
+                                                     const T *__restrict__ ph, const T *__restrict__ sh,
+                                                     T *__restrict__ r, const T *__restrict__ s,
+                                                     const T *__restrict__ t, const T *__restrict__ r0,
+                                                     double *partials, unsigned *counter) {
+  if (sc->done) return;
+  const T al = (T)sc->alpha, om = (T)sc->omega;
+  double d[2] = {0., 0.};
+  for (int i = blockIdx.x * kRedBlock + threadIdx.x; i < n; i += gridDim.x * kRedBlock) {
+    x[i] = x[i] + (al * ph[i] + om * sh[i]);
+    const T ri = s[i] - om * t[i];
+    r[i] = ri;
+    d[0] += (double)r0[i] * (double)ri;
+    d[1] += (double)ri * (double)ri;
+  }
+  if (reduceLast<2>(d, partials, counter) && threadIdx.x == 0) {
+    const double rhoNew = d[0], rr = d[1];
+    sc->rrPrev = sc->rr;
+    sc->rr = rr;
+    sc->beta = (rhoNew / sc->rho) * (sc->alpha / sc->omega);
+    sc->rho = rhoNew;
+    sc->iters += 1;
+    if (rr <= sc->tol2) sc->done = kDoneConverged;
+    else if (rhoNew == 0. || sc->omega == 0.) sc->done = kDoneBreakdown;
+    else if (sc->minRatio > 0. && rr > sc->minRatio * sc->minRatio * sc->rrPrev) sc->done = kDoneStagnation;
+    else if (sc->iters >= sc->maxIters) sc->done = kDoneMaxIters;
+  }
+}
+
+// x64 += scale * x32 with scale = sqrt(*rrOuter) (undoing the inner normalisation)
+template <class T>
+__global__ void kAccumulate(int n, double *__restrict__ x, const T *__restrict__ e, const Scalars *outer) {
+  const double sc = sqrt(outer->rr);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    x[i] += sc * (double)e[i];
+}
+
+// r32 = r64 / ||r64|| ; r0 = r32
+template <class T>
+__global__ void kNormalise(int n, const double *__restrict__ r64, T *__restrict__ r, T *__restrict__ r0,
+                           const Scalars *outer) {
+  const double inv = 1.0 / sqrt(outer->rr);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const T v = (T)(r64[i] * inv);
+    r[i] = v;
+    r0[i] = v;
+  }
+}
+
+// ---------------------------------------------------------------- ILU(0)
+// Factorisation, IKJ order per row (the oracle's orc_ilu0): for each L entry
+// (i,k) in ascending k: l = a_ik / u_kk, then a_ij -= l * u_kj over U(k) in
+// ascending j.  Rows of earlier blocks are final once their flag is set; rows
+// of the same block are final after their level.
+template <class T>
+__global__ __launch_bounds__(64) void kIluFactor(IluDev<T> L, const int *__restrict__ updPtr,
+                                                 const int *__restrict__ updSrc,
+                                                 const int *__restrict__ updDst, SyncDev sy,
+                                                 unsigned long long base, int epoch) {
+  const long long bi = takeTicket(sy.ticket, base);
+  if (bi < 0 || bi >= L.nb) return;
+  const int b = (int)bi;
+  const BlockInfo B = L.blocks[b];
+  const int lane = threadIdx.x;
+  const int np = B.predEnd - B.predBeg;
+  if (lane < np) waitFlag(&sy.flags[L.pred[B.predBeg + lane]], epoch, sy.err);
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  __syncthreads();
+  const bool act = lane < B.nRows;
+  const int row = B.rowStart + lane;
+  const RowInfo ri = act ? L.rows[row] : RowInfo{};
+  T *fac = L.fac;
+  const int dbase = L.nLSlots + L.nUSlots;
+  for (int lev = 0; lev < B.nLev; ++lev) {
+    if (act && ri.lev == lev) {
+      for (int e = 0; e < ri.nL; ++e) {
+        const int s = B.lOff + (e << 6) + lane;
+        const int k = L.lCol[s];
+        const T lik = fac[s] / fac[dbase + k];
+        fac[s] = lik;
+        for (int u = updPtr[s]; u < updPtr[s + 1]; ++u) {
+          const int dst = updDst[u];
+          fac[dst] = fma(-lik, fac[updSrc[u]], fac[dst]);
+        }
+      }
+      if (fac[dbase + row] == (T)0) atomicOr(sy.err, 2);
+    }
+    __syncthreads();
+  }
+  __syncthreads();
+  if (lane == 0) stRelease(&sy.flags[b], epoch);
+}
+
+// Forward sweep y = L^{-1} b (unit L).  PRO selects the fused BiCGStab
+// prologue that forms b on the fly: kProP: p = r + beta (p - omega v);
+// kProS: s = r - alpha v (both stored).
+template <class T, int PRO>
+__global__ __launch_bounds__(64) void kLSolve(IluDev<T> L, SyncDev sy, unsigned long long base, int epoch,
+                                              const Scalars *sc, const T *__restrict__ in, T *__restrict__ y,
+                                              T *__restrict__ pv, const T *__restrict__ r,
+                                              const T *__restrict__ v) {
+  __shared__ T ylds[64];
+  const long long bi = takeTicket(sy.ticket, base);
+  if (bi < 0 || bi >= L.nb) return;
+  if (sc && sc->done) return;
+  const int b = (int)bi;
+  const BlockInfo B = L.blocks[b];
+  const int lane = threadIdx.x;
+  const bool act = lane < B.nRows;
+  const int row = B.rowStart + lane;
+  const RowInfo ri = act ? L.rows[row] : RowInfo{};
+  T acc = 0;
+  if (act) {
+    if (PRO == kProP) {
+      const T beta = (T)sc->beta, omega = (T)sc->omega;
+      const T p = r[row] + beta * (pv[row] - omega * v[row]);
+      pv[row] = p;
+      acc = p;
+    } else if (PRO == kProS) {
+      const T alpha = (T)sc->alpha;
+      const T s = r[row] - alpha * v[row];
+      pv[row] = s;
+      acc = s;
+    } else {
+      acc = in[row];
+    }
+  }
+  const int np = B.predEnd - B.predBeg;
+  if (lane < np) waitFlag(&sy.flags[L.pred[B.predBeg + lane]], epoch, sy.err);
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  __syncthreads();
+  if (act) {
+    for (int e = 0; e < ri.nLoff; ++e) {
+      const int s = B.lOff + (e << 6) + lane;
+      acc = fma(-L.fac[s], y[L.lCol[s]], acc);
+    }
+  }
+  for (int lev = 0; lev < B.nLev; ++lev) {
+    if (act && ri.lev == lev) {
+      for (int e = ri.nLoff; e < ri.nL; ++e) {
+        const int s = B.lOff + (e << 6) + lane;
+        acc = fma(-L.fac[s], ylds[L.lCol[s] - B.rowStart], acc);
+      }
+      ylds[lane] = acc;
+    }
+    __syncthreads();
+  }
+  if (act) y[row] = acc;
+  __syncthreads();
+  if (lane == 0) stRelease(&sy.flags[b], epoch);
+}
+
+// Backward sweep x = U^{-1} y, blocks in reverse order.
+template <class T>
+__global__ __launch_bounds__(64) void kUSolve(IluDev<T> L, SyncDev sy, unsigned long long base, int epoch,
+                                              const Scalars *sc, const T *__restrict__ y, T *__restrict__ x) {
+  __shared__ T xlds[64];
+  const long long bi = takeTicket(sy.ticket, base);
+  if (bi < 0 || bi >= L.nb) return;
+  if (sc && sc->done) return;
+  const int b = L.nb - 1 - (int)bi;
+  const BlockInfo B = L.blocks[b];
+  const int lane = threadIdx.x;
+  const bool act = lane < B.nRows;
+  const int row = B.rowStart + lane;
+  const RowInfo ri = act ? L.rows[row] : RowInfo{};
+  const T *facU = L.fac + L.nLSlots;
+  T acc = act ? y[row] : (T)0;
+  const T d = act ? L.fac[L.nLSlots + L.nUSlots + row] : (T)1;
+  const int ns = B.succEnd - B.succBeg;
+  if (lane < ns) waitFlag(&sy.flags[L.succ[B.succBeg + lane]], epoch, sy.err);
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  __syncthreads();
+  const int nOff = act ? ri.nU - ri.nUin : 0;
+  T ou[kMaxOffU], ox[kMaxOffU];
+#pragma unroll
+  for (int q = 0; q < kMaxOffU; ++q) {
+    ou[q] = 0;
+    ox[q] = 0;
+    if (q < nOff) {
+      const int s = B.uOff + ((ri.nUin + q) << 6) + lane;
+      ou[q] = facU[s];
+      ox[q] = x[L.uCol[s]];
+    }
+  }
+  for (int lev = 0; lev < B.nULev; ++lev) {
+    if (act && ri.ulev == lev) {
+      for (int e = 0; e < ri.nUin; ++e) {
+        const int s = B.uOff + (e << 6) + lane;
+        acc = fma(-facU[s], xlds[L.uCol[s] - B.rowStart], acc);
+      }
+#pragma unroll
+      for (int q = 0; q < kMaxOffU; ++q)
+        if (q < nOff) acc = fma(-ou[q], ox[q], acc);
+      acc = acc / d;
+      xlds[lane] = acc;
+    }
+    __syncthreads();
+  }
+  if (act) x[row] = acc;
+  __syncthreads();
+  if (lane == 0) stRelease(&sy.flags[b], epoch);
+}
+
+}  // namespace phase

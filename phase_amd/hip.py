@@ -1,0 +1,278 @@
+"""ctypes binding of libphase_hip.so (include/phase_hip.h)."""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "_lib", "libphase_hip.so")
+
+# FiniteVolumeField<T>::BoundaryType (FiniteVolumeField.h:12)
+BC_FIXED, BC_NORMAL_GRADIENT, BC_SYMMETRY, BC_OUTFLOW, BC_PARTIAL_SLIP = 0, 1, 2, 3, 4
+MODE_FP64, MODE_MIXED, MODE_LEARNED = 0, 1, 2
+
+_ERRORS = {-1: "EINVAL", -2: "EDEVICE", -3: "ESTATE", -4: "ENOCONV", -5: "EBREAKDOWN",
+           -6: "EBOUNDARY"}
+
+
+class PhaseHipError(RuntimeError):
+    """The reference's Exception(class, method, description) (src/System/Exception.h)."""
+
+    def __init__(self, code, method, msg):
+        self.code = code
+        super().__init__("HipSparseMatrixSolver::%s [%s]: %s" % (method, _ERRORS.get(code, code), msg))
+
+
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_vp = ctypes.c_void_p
+_ip = ctypes.POINTER(ctypes.c_int)
+_dp = ctypes.POINTER(ctypes.c_double)
+
+_SIGS = {
+    "phase_hip_last_error": (ctypes.c_char_p, [_vp]),
+    "phase_hip_device_count": (ctypes.c_int, [_ip]),
+    "phase_hip_solver_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int]),
+    "phase_hip_solver_destroy": (ctypes.c_int, [_vp]),
+    "phase_hip_solver_setup": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_char_p]),
+    "phase_hip_solver_set_rank": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "phase_hip_solver_set": (ctypes.c_int, [_vp, ctypes.c_int, _i32p, _i32p, _f64p]),
+    "phase_hip_solver_set_rhs": (ctypes.c_int, [_vp, _f64p]),
+    "phase_hip_solver_set_guess": (ctypes.c_int, [_vp, _f64p]),
+    "phase_hip_solver_solve": (ctypes.c_int, [_vp, _dp]),
+    "phase_hip_solver_get_x": (ctypes.c_int, [_vp, _f64p]),
+    "phase_hip_solver_n_iters": (ctypes.c_int, [_vp]),
+    "phase_hip_solver_error": (ctypes.c_double, [_vp]),
+    "phase_hip_solver_stats": (ctypes.c_int, [_vp, _ip, _ip, _ip, _dp]),
+    "phase_hip_solver_export": (ctypes.c_int, [_vp, _ip, _vp, _vp, _vp, _vp]),
+    "phase_hip_solver_spmv": (ctypes.c_int, [_vp, ctypes.c_int, _f64p, _f64p]),
+    "phase_hip_solver_ilu_export": (ctypes.c_int, [_vp, _ip, _vp, _vp, _vp, _vp]),
+    "phase_hip_solver_time_kernels": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, _dp]),
+    "phase_hip_mesh_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, _f64p,
+                                             ctypes.c_int, _i32p, _i32p]),
+    "phase_hip_mesh_destroy": (ctypes.c_int, [_vp]),
+    "phase_hip_mesh_last_error": (ctypes.c_char_p, [_vp]),
+    "phase_hip_mesh_sizes": (ctypes.c_int, [_vp, _ip, _ip, _ip, _ip]),
+    "phase_hip_mesh_faces": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "phase_hip_mesh_geometry": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "phase_hip_assemble_laplacian": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_double, _vp, _vp,
+                                                    ctypes.c_double, _vp, _vp, _vp, _vp]),
+    "phase_hip_equation_sub_rhs": (ctypes.c_int, [_vp, _f64p]),
+    "phase_hip_equation_export_padded": (ctypes.c_int, [_vp, _ip, _vp, _vp, _vp, _vp]),
+}
+
+_lib = None
+
+
+def library_path():
+    return _LIB
+
+
+def load_library():
+    """Load libphase_hip.so; raises if it has not been built (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB):
+        raise ImportError("libphase_hip.so is not built (%s); run __graft_entry__.build()" % _LIB)
+    lib = ctypes.CDLL(_LIB)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def device_count():
+    c = ctypes.c_int(0)
+    load_library().phase_hip_device_count(ctypes.byref(c))
+    return c.value
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _f64(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.int32)
+
+
+class HipMesh:
+    """Unstructured 2D mesh held on the device (FiniteVolumeGrid2D::init)."""
+
+    def __init__(self, xy, cptr, cind, device=0):
+        lib = load_library()
+        xy = _f64(np.asarray(xy).reshape(-1))
+        cptr, cind = _i32(cptr), _i32(cind)
+        h = _vp()
+        rc = lib.phase_hip_mesh_create(ctypes.byref(h), device, len(xy) // 2, xy, len(cptr) - 1, cptr, cind)
+        if rc != 0:
+            raise PhaseHipError(rc, "meshCreate", "mesh creation failed")
+        self._h = h
+        a, b, c, d = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        lib.phase_hip_mesh_sizes(h, a, b, c, d)
+        self.n_cells, self.n_faces, self.n_interior, self.n_boundary = a.value, b.value, c.value, d.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().phase_hip_mesh_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def faces(self):
+        F = self.n_faces
+        out = {k: np.empty(F, np.int32) for k in ("lnode", "rnode", "lcell", "rcell")}
+        load_library().phase_hip_mesh_faces(self._h, _p(out["lnode"]), _p(out["rnode"]), _p(out["lcell"]),
+                                            _p(out["rcell"]))
+        return out
+
+    def geometry(self):
+        N, F = self.n_cells, self.n_faces
+        vol, cc, fc, sf = np.empty(N), np.empty(2 * N), np.empty(2 * F), np.empty(2 * F)
+        rc = load_library().phase_hip_mesh_geometry(self._h, _p(vol), _p(cc), _p(fc), _p(sf))
+        if rc != 0:
+            raise PhaseHipError(rc, "geometry", load_library().phase_hip_mesh_last_error(self._h).decode())
+        return vol, cc.reshape(N, 2), fc.reshape(F, 2), sf.reshape(F, 2)
+
+
+class HipSparseMatrixSolver:
+    """SparseMatrixSolver plugin "hip" (src/Math/SparseMatrixSolver.h:11-62)."""
+
+    def __init__(self, device=0, **setup):
+        lib = load_library()
+        h = _vp()
+        rc = lib.phase_hip_solver_create(ctypes.byref(h), device)
+        if rc != 0:
+            raise PhaseHipError(rc, "create", lib.phase_hip_last_error(None).decode())
+        self._h = h
+        self.rank = 0
+        for k, v in setup.items():
+            self.setup(k, v)
+
+    def _check(self, rc, method):
+        if rc != 0:
+            raise PhaseHipError(rc, method, load_library().phase_hip_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().phase_hip_solver_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- SparseMatrixSolver interface
+    def setup(self, key, value):
+        self._check(load_library().phase_hip_solver_setup(self._h, key.encode(), str(value).encode()), "setup")
+
+    def set_rank(self, rank):
+        self._check(load_library().phase_hip_solver_set_rank(self._h, int(rank)), "setRank")
+        self.rank = int(rank)
+
+    def set(self, row_ptr, col_ind, vals):
+        row_ptr, col_ind, vals = _i32(row_ptr), _i32(col_ind), _f64(vals)
+        n = len(row_ptr) - 1
+        self._check(load_library().phase_hip_solver_set(self._h, n, row_ptr, col_ind, vals), "set")
+        self.rank = n
+
+    def set_rhs(self, b):
+        self._check(load_library().phase_hip_solver_set_rhs(self._h, _f64(b)), "setRhs")
+
+    def set_guess(self, x0):
+        self._check(load_library().phase_hip_solver_set_guess(self._h, _f64(x0)), "setGuess")
+
+    def solve(self, raise_on_noconv=True):
+        rr = ctypes.c_double()
+        rc = load_library().phase_hip_solver_solve(self._h, ctypes.byref(rr))
+        if rc != 0 and (raise_on_noconv or rc != -4):
+            self._check(rc, "solve")
+        return rr.value
+
+    def x(self):
+        out = np.empty(self.rank)
+        self._check(load_library().phase_hip_solver_get_x(self._h, out), "x")
+        return out
+
+    def n_iters(self):
+        return load_library().phase_hip_solver_n_iters(self._h)
+
+    def error(self):
+        return load_library().phase_hip_solver_error(self._h)
+
+    def stats(self):
+        a, b, c, d = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        self._check(load_library().phase_hip_solver_stats(self._h, a, b, c, d), "stats")
+        return dict(inner_iters=a.value, outer_passes=b.value, fallbacks=c.value, solve_ms=d.value)
+
+    # --- device-side helpers used by tests and the benchmark
+    def export(self):
+        nnz = ctypes.c_int()
+        self._check(load_library().phase_hip_solver_export(self._h, nnz, None, None, None, None), "export")
+        rp = np.empty(self.rank + 1, np.int32)
+        ci = np.empty(nnz.value, np.int32)
+        v = np.empty(nnz.value)
+        rhs = np.empty(self.rank)
+        self._check(load_library().phase_hip_solver_export(self._h, nnz, _p(rp), _p(ci), _p(v), _p(rhs)),
+                    "export")
+        return rp, ci, v, rhs
+
+    def spmv(self, x, single=False):
+        y = np.empty(self.rank)
+        self._check(load_library().phase_hip_solver_spmv(self._h, int(bool(single)), _f64(x), y), "spmv")
+        return y
+
+    def ilu_export(self):
+        nnz = ctypes.c_int()
+        self._check(load_library().phase_hip_solver_ilu_export(self._h, nnz, None, None, None, None), "ilu")
+        perm = np.empty(self.rank, np.int32)
+        rp = np.empty(self.rank + 1, np.int32)
+        ci = np.empty(nnz.value, np.int32)
+        v = np.empty(nnz.value)
+        self._check(load_library().phase_hip_solver_ilu_export(self._h, nnz, _p(perm), _p(rp), _p(ci), _p(v)),
+                    "ilu")
+        return perm, rp, ci, v
+
+    def time_kernels(self, reps=20):
+        a, b, c, d = ctypes.c_double(), ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        self._check(load_library().phase_hip_solver_time_kernels(self._h, int(reps), a, b, c, d), "time")
+        return dict(spmv64_ms=a.value, spmv32_ms=b.value, ilu64_ms=c.value, ilu32_ms=d.value)
+
+    # --- fv:: operators on a HipMesh
+    def assemble_laplacian(self, mesh, variant, gamma=1.0, gamma_face=None, gamma0_face=None, theta=1.0,
+                           bc_type=None, phi_face=None, phi0_cell=None, phi0_face=None):
+        args = [_f64(gamma_face), _f64(gamma0_face), _i32(bc_type), _f64(phi_face), _f64(phi0_cell),
+                _f64(phi0_face)]
+        rc = load_library().phase_hip_assemble_laplacian(
+            self._h, mesh._h, int(variant), float(gamma), _p(args[0]), _p(args[1]), float(theta),
+            _p(args[2]), _p(args[3]), _p(args[4]), _p(args[5]))
+        self._check(rc, "assembleLaplacian")
+        self.rank = mesh.n_cells
+
+    def sub_rhs(self, v):
+        """CrsEquation::operator==(Vector): stored rhs -= v."""
+        self._check(load_library().phase_hip_equation_sub_rhs(self._h, _f64(v)), "operator==")
+
+    def export_padded(self):
+        ns = ctypes.c_int()
+        self._check(load_library().phase_hip_equation_export_padded(self._h, ns, None, None, None, None),
+                    "exportPadded")
+        rp = np.empty(self.rank + 1, np.int32)
+        ci = np.empty(ns.value, np.int32)
+        v = np.empty(ns.value)
+        rhs = np.empty(self.rank)
+        self._check(load_library().phase_hip_equation_export_padded(self._h, ns, _p(rp), _p(ci), _p(v),
+                                                                    _p(rhs)), "exportPadded")
+        return rp, ci, v, rhs

@@ -1,0 +1,233 @@
+// CPU emulation of the device solver's layout and kernels (test-only).
+//
+// Builds the product's ordering (phase_amd/csrc/ordering.cpp) and replays the
+// SELL-64 SpMV, the ILU(0) factorisation and both triangular sweeps exactly as
+// the gfx950 kernels in solver_kernels.h do -- block by block in ticket order,
+// level by level inside a block -- while checking every array index against
+// its bounds and every dependency against the schedule (a predecessor block
+// must already be finished, an in-block dependency must sit at a lower level).
+// The results are compared with the oracle by tests/test_ordering_emul.py.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../phase_amd/csrc/phase_internal.h"
+
+using namespace phase;
+
+namespace {
+struct Bad {
+  std::string msg;
+};
+#define REQ(c, m) \
+  do {            \
+    if (!(c)) throw Bad{m}; \
+  } while (0)
+}  // namespace
+
+extern "C" int emul_run(int n, const int *rowPtr, const int *colInd, const double *vals, int blockRows,
+                        int *perm, int *facRowPtr, int *facCol, double *facVal, int cap, const double *rhsIn,
+                        double *y, double *x, const double *xin, double *spmvOut, int *info, char *err,
+                        int errLen) {
+  try {
+    Ordering o;
+    try {
+      buildOrdering(o, n, rowPtr, colInd, blockRows);
+    } catch (const HipError &e) {
+      throw Bad{e.msg};
+    }
+    const int nslots = rowPtr[n];
+    const int nb = o.nBlocks;
+    info[0] = nb;
+    info[1] = o.nColours;
+    info[2] = o.nLSlots;
+    info[3] = o.nUSlots;
+    int maxLev = 0, maxULev = 0;
+    for (auto &B : o.blocks) {
+      maxLev = std::max(maxLev, B.nLev);
+      maxULev = std::max(maxULev, B.nULev);
+    }
+    info[4] = maxLev;
+    info[5] = maxULev;
+    for (int k = 0; k < n; ++k) perm[k] = o.perm[k];
+    // --- value gathers (kGather)
+    const size_t nSell = o.sellCol.size();
+    REQ(o.sellSrc.size() == nSell, "sellSrc size");
+    std::vector<double> sell(nSell);
+    for (size_t i = 0; i < nSell; ++i) {
+      REQ(o.sellSrc[i] < nslots, "sellSrc out of range");
+      sell[i] = o.sellSrc[i] >= 0 ? vals[o.sellSrc[i]] : 0.;
+    }
+    const size_t nFac = (size_t)o.nLSlots + o.nUSlots + n;
+    REQ(o.facSrc.size() == nFac, "facSrc size");
+    std::vector<double> fac(nFac);
+    for (size_t i = 0; i < nFac; ++i) {
+      REQ(o.facSrc[i] < nslots, "facSrc out of range");
+      fac[i] = o.facSrc[i] >= 0 ? vals[o.facSrc[i]] : 0.;
+    }
+    // --- SpMV (kSpmv)
+    REQ((int)o.sliceOff.size() == o.nSlices + 1, "sliceOff size");
+    for (int k = 0; k < n; ++k) {
+      int s = k / 64, lane = k % 64;
+      int off = o.sliceOff[s], K = (o.sliceOff[s + 1] - off) / 64;
+      REQ((o.sliceOff[s + 1] - off) % 64 == 0, "slice not a multiple of 64");
+      double acc = 0.;
+      for (int e = 0; e < K; ++e) {
+        size_t idx = off + e * 64 + lane;
+        REQ(idx < nSell, "sell index out of range");
+        int c = o.sellCol[idx];
+        REQ(c < n, "sell column out of range");
+        if (c >= 0) acc = std::fma(sell[idx], xin[c], acc);
+      }
+      spmvOut[k] = acc;
+    }
+    const int dbase = o.nLSlots + o.nUSlots;
+    auto lslot = [&](const BlockInfo &B, int e, int lane) {
+      int s = B.lOff + e * 64 + lane;
+      REQ(s >= 0 && s < o.nLSlots, "L slot out of range");
+      return s;
+    };
+    auto uslot = [&](const BlockInfo &B, int e, int lane) {
+      int s = B.uOff + e * 64 + lane;
+      REQ(s >= 0 && s < o.nUSlots, "U slot out of range");
+      return s;
+    };
+    std::vector<char> doneBlk(nb, 0), rowDone(n, 0);
+    // --- factorisation (kIluFactor), blocks in ticket order
+    for (int b = 0; b < nb; ++b) {
+      const BlockInfo &B = o.blocks[b];
+      REQ(B.nRows >= 1 && B.nRows <= 64, "block size");
+      REQ(B.predEnd - B.predBeg <= 64, "too many preds");
+      for (int q = B.predBeg; q < B.predEnd; ++q) {
+        REQ(o.pred[q] < b, "pred block not earlier");
+        REQ(doneBlk[o.pred[q]], "pred block not finished");
+      }
+      for (int lev = 0; lev < B.nLev; ++lev) {
+        for (int lane = 0; lane < B.nRows; ++lane) {
+          const int row = B.rowStart + lane;
+          const RowInfo &ri = o.rows[row];
+          if (ri.lev != lev) continue;
+          for (int e = 0; e < ri.nL; ++e) {
+            const int s = lslot(B, e, lane);
+            const int k = o.lCol[s];
+            REQ(k >= 0 && k < row, "L column not below diagonal");
+            if (e < ri.nLoff) REQ(k < B.rowStart, "off-block L entry inside block");
+            else REQ(k >= B.rowStart && o.rows[k].lev < lev, "in-block L dependency not at a lower level");
+            REQ(rowDone[k], "L dependency row not finished");
+            const double lik = fac[s] / fac[dbase + k];
+            fac[s] = lik;
+            REQ(o.updPtr[s] <= o.updPtr[s + 1], "updPtr");
+            for (int u = o.updPtr[s]; u < o.updPtr[s + 1]; ++u) {
+              const int dst = o.updDst[u], src = o.updSrc[u];
+              REQ(dst >= 0 && (size_t)dst < nFac && src >= o.nLSlots && src < dbase, "update index");
+              fac[dst] = std::fma(-lik, fac[src], fac[dst]);
+            }
+          }
+          rowDone[row] = 1;
+        }
+      }
+      for (int lane = 0; lane < B.nRows; ++lane) REQ(rowDone[B.rowStart + lane], "row left unprocessed");
+      doneBlk[b] = 1;
+    }
+    // export factor (solver order, sorted columns)
+    int w = 0;
+    facRowPtr[0] = 0;
+    for (int b = 0; b < nb; ++b) {
+      const BlockInfo &B = o.blocks[b];
+      for (int lane = 0; lane < B.nRows; ++lane) {
+        const int k = B.rowStart + lane;
+        const RowInfo &ri = o.rows[k];
+        REQ(w + ri.nL + ri.nU + 1 <= cap, "factor capacity");
+        for (int e = 0; e < ri.nL; ++e) { int s = lslot(B, e, lane); facCol[w] = o.lCol[s]; facVal[w++] = fac[s]; }
+        facCol[w] = k;
+        facVal[w++] = fac[dbase + k];
+        for (int e = 0; e < ri.nU; ++e) {
+          int s = uslot(B, e, lane);
+          facCol[w] = o.uCol[s];
+          facVal[w++] = fac[o.nLSlots + s];
+        }
+        facRowPtr[k + 1] = w;
+      }
+    }
+    // --- forward sweep (kLSolve)
+    std::fill(doneBlk.begin(), doneBlk.end(), 0);
+    std::fill(rowDone.begin(), rowDone.end(), 0);
+    for (int b = 0; b < nb; ++b) {
+      const BlockInfo &B = o.blocks[b];
+      for (int q = B.predBeg; q < B.predEnd; ++q) REQ(doneBlk[o.pred[q]], "fwd pred not finished");
+      std::vector<double> acc(64, 0.), lds(64, 0.);
+      for (int lane = 0; lane < B.nRows; ++lane) {
+        const int row = B.rowStart + lane;
+        const RowInfo &ri = o.rows[row];
+        acc[lane] = rhsIn[row];
+        for (int e = 0; e < ri.nLoff; ++e) {
+          const int s = lslot(B, e, lane);
+          REQ(rowDone[o.lCol[s]], "fwd off-block value not ready");
+          acc[lane] = std::fma(-fac[s], y[o.lCol[s]], acc[lane]);
+        }
+      }
+      for (int lev = 0; lev < B.nLev; ++lev) {
+        for (int lane = 0; lane < B.nRows; ++lane) {
+          const int row = B.rowStart + lane;
+          const RowInfo &ri = o.rows[row];
+          if (ri.lev != lev) continue;
+          for (int e = ri.nLoff; e < ri.nL; ++e) {
+            const int s = lslot(B, e, lane);
+            const int li = o.lCol[s] - B.rowStart;
+            REQ(li >= 0 && li < lane && rowDone[o.lCol[s]], "fwd in-block value not ready");
+            acc[lane] = std::fma(-fac[s], lds[li], acc[lane]);
+          }
+          lds[lane] = acc[lane];
+          y[row] = acc[lane];
+        }
+        for (int lane = 0; lane < B.nRows; ++lane)
+          if (o.rows[B.rowStart + lane].lev == lev) rowDone[B.rowStart + lane] = 1;
+      }
+      doneBlk[b] = 1;
+    }
+    // --- backward sweep (kUSolve), blocks in reverse
+    std::fill(doneBlk.begin(), doneBlk.end(), 0);
+    std::fill(rowDone.begin(), rowDone.end(), 0);
+    for (int t = 0; t < nb; ++t) {
+      const int b = nb - 1 - t;
+      const BlockInfo &B = o.blocks[b];
+      for (int q = B.succBeg; q < B.succEnd; ++q) {
+        REQ(o.succ[q] > b, "succ block not later");
+        REQ(doneBlk[o.succ[q]], "bwd succ not finished");
+      }
+      std::vector<double> acc(64, 0.), lds(64, 0.);
+      for (int lane = 0; lane < B.nRows; ++lane) acc[lane] = y[B.rowStart + lane];
+      for (int lev = 0; lev < B.nULev; ++lev) {
+        for (int lane = 0; lane < B.nRows; ++lane) {
+          const int row = B.rowStart + lane;
+          const RowInfo &ri = o.rows[row];
+          if (ri.ulev != lev) continue;
+          REQ(ri.nU - ri.nUin <= 8, "too many off-block U entries");
+          for (int e = 0; e < ri.nUin; ++e) {
+            const int s = uslot(B, e, lane);
+            const int li = o.uCol[s] - B.rowStart;
+            REQ(li > lane && li < B.nRows && rowDone[o.uCol[s]], "bwd in-block value not ready");
+            acc[lane] = std::fma(-fac[o.nLSlots + s], lds[li], acc[lane]);
+          }
+          for (int e = ri.nUin; e < ri.nU; ++e) {
+            const int s = uslot(B, e, lane);
+            REQ(o.uCol[s] >= B.rowStart + B.nRows && rowDone[o.uCol[s]], "bwd off-block value not ready");
+            acc[lane] = std::fma(-fac[o.nLSlots + s], x[o.uCol[s]], acc[lane]);
+          }
+          acc[lane] = acc[lane] / fac[dbase + row];
+          lds[lane] = acc[lane];
+          x[row] = acc[lane];
+        }
+        for (int lane = 0; lane < B.nRows; ++lane)
+          if (o.rows[B.rowStart + lane].ulev == lev) rowDone[B.rowStart + lane] = 1;
+      }
+      doneBlk[b] = 1;
+    }
+    return 0;
+  } catch (const Bad &e) {
+    std::snprintf(err, errLen, "%s", e.msg.c_str());
+    return 1;
+  }
+}
