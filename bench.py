@@ -1,0 +1,197 @@
+#!/usr/bin/env python3
+"""Pressure-correction solve benchmark on one MI355X.
+
+Workload (one "step" = one pressure-correction solve of a time step):
+standalone pCorr Poisson on a 1,000,000-cell unstructured triangular mesh of
+[0,1]^2 (Delaunay of 512 points per edge + 498,977 uniform interior points,
+seed 1), face coefficient rho*d lognormal sigma=1 (seed 2), right-hand side
+i.i.d. N(0,1) projected to zero mean (seed 3), pCorr = 0 (FIXED) on the y-
+edge, zero-gradient elsewhere.  Each step assembles fv::laplacian(gamma_f, p)
+== b on the device from HBM-resident inputs, factors ILU(0) and solves to an
+FP64 true relative residual of 1e-8 from a zero initial guess.
+
+Prints ONE JSON line.  `value` is ms per step of the fastest method
+("mixed": FP32 BiCGStab+ILU(0) in FP64 iterative refinement); the plain FP64
+BiCGStab+ILU(0) time is reported beside it, as is the FP64 SpMV bandwidth.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+METRIC = "pCorr solve ms/step at 1e-8 rel. residual, learned+mixed vs plain; SpMV GB/s"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_problem(n_cells, k, sigma):
+    from phase_amd import meshgen as G
+
+    ni = G.delaunay_cells_for(n_cells, k)
+    xy, cptr, cind = G.delaunay_square(k, ni, seed=1)
+    return xy, cptr, cind
+
+
+def spmv_bytes(n, nnz, s):
+    return nnz * (s + 4) + (n + 1) * 4 + 2 * n * s
+
+
+def sweep_bytes(n, nnz_tri, s, diag):
+    # triangular part values+columns, row metadata, rhs in, result out (+ diagonal)
+    return nnz_tri * (s + 4) + (n + 1) * 4 + 2 * n * s + (n * s if diag else 0)
+
+
+def cpu_baseline(xy, cptr, cind, bc, g, b, iters_target):
+    """The FP64 oracle (single thread) on a bounded sample of the same solve:
+    assembly + ILU(0) + 10 and 60 BiCGStab iterations; ms/step extrapolated to
+    `iters_target` iterations (the GPU FP64 count; natural-order ILU needs at
+    least as many)."""
+    from oracle import oracle as O
+
+    O.build()
+    t0 = time.perf_counter()
+    m = O.Mesh(xy, cptr, cind)
+    e = O.laplacian(m, 3, gamma_face=g, bc_type=bc, phi_face=np.zeros(m.n_faces))
+    e.eq(b)  # stored rhs = -b  ->  A x = b
+    rp, ci, v, rhs = e.get()
+    t_asm = time.perf_counter() - t0
+    times = {}
+    for it in (10, 60):
+        t0 = time.perf_counter()
+        O.bicgstab_ilu0(rp, ci, v, -rhs, tol=1e-30, max_iters=it)
+        times[it] = time.perf_counter() - t0
+    per_iter = (times[60] - times[10]) / 50.0
+    setup = max(times[10] - 10 * per_iter, 0.0)
+    ms = 1e3 * (t_asm + setup + per_iter * iters_target)
+    return dict(value=round(ms, 1), unit="ms/step", cores=1, kind="port",
+                sample="oracle FP64 BiCGStab+ILU(0), 1 thread: assembly %.2fs + ILU setup %.2fs + "
+                       "%.1f ms/iteration measured over 10 and 60 iterations, scaled to %d iterations"
+                       % (t_asm, setup, 1e3 * per_iter, iters_target))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cells", type=int, default=1_000_000)
+    ap.add_argument("--edge", type=int, default=512)
+    ap.add_argument("--sigma", type=float, default=1.0)
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--inner-tol", type=float, default=1e-3)
+    ap.add_argument("--modes", default="fp64,mixed")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--kernel-reps", type=int, default=20)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; --gpus %d is not supported (the path would shard by "
+              "cell partition, see DESIGN.md)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import phase_amd as P
+    from phase_amd import meshgen as G
+
+    t0 = time.perf_counter()
+    xy, cptr, cind = build_problem(args.cells, args.edge, args.sigma)
+    mesh = P.HipMesh(xy, cptr, cind)
+    f = mesh.faces()
+    pid = G.boundary_patch(f["lnode"], f["rnode"], f["rcell"], xy)
+    bc = np.where(pid == 2, P.BC_FIXED, P.BC_NORMAL_GRADIENT).astype(np.int32)
+    g = G.lognormal_face_coeffs(mesh.n_faces, args.sigma, seed=2)
+    b = G.zero_mean_normal(mesh.n_cells, seed=3)
+    d_g, d_bc, d_pf, d_b = (P.DeviceArray(g), P.DeviceArray(bc), P.DeviceArray(np.zeros(mesh.n_faces)),
+                            P.DeviceArray(b))
+    log("setup: %d cells, %d faces, %.1fs" % (mesh.n_cells, mesh.n_faces, time.perf_counter() - t0))
+
+    results = {}
+    kernels = None
+    nnz = None
+    for mode in args.modes.split(","):
+        s = P.HipSparseMatrixSolver(0, tolerance=args.tol, maxIters=50000, mode=mode,
+                                    innerTolerance=args.inner_tol)
+
+        def step():
+            s.assemble_laplacian_device(mesh, 3, gamma_face=d_g, bc_type=d_bc, phi_face=d_pf)
+            s.sub_rhs_device(d_b)
+            s.zero_guess()
+            return s.solve()
+
+        for _ in range(args.warmup):
+            step()
+        t0 = time.perf_counter()
+        rel = []
+        for _ in range(args.steps):
+            rel.append(step())
+        dt = (time.perf_counter() - t0) / args.steps
+        st = s.stats()
+        results[mode] = dict(ms_per_step=1e3 * dt, relres=max(rel), iters=st["inner_iters"],
+                             passes=st["outer_passes"], solve_ms=st["solve_ms"])
+        log("%s: %.2f ms/step, relres %.2e, %d iterations, %d passes" % (mode, 1e3 * dt, max(rel),
+                                                                          st["inner_iters"], st["outer_passes"]))
+        if kernels is None:
+            kernels = s.time_kernels(args.kernel_reps)
+            nnz = len(s.export()[1])
+        s.close()
+
+    n = mesh.n_cells
+    nnz_tri = (nnz - n) // 2
+    kb = {
+        "spmv64": spmv_bytes(n, nnz, 8), "spmv32": spmv_bytes(n, nnz, 4),
+        "lsweep64": sweep_bytes(n, nnz_tri, 8, False), "usweep64": sweep_bytes(n, nnz_tri, 8, True),
+        "lsweep32": sweep_bytes(n, nnz_tri, 4, False), "usweep32": sweep_bytes(n, nnz_tri, 4, True),
+    }
+    gbs = {k: kb[k] / (kernels[k] * 1e-3) / 1e9 for k in kb}
+    log("kernels (ms): " + ", ".join("%s %.4f" % (k, v) for k, v in kernels.items()))
+    log("kernels (GB/s): " + ", ".join("%s %.0f" % (k, v) for k, v in gbs.items()))
+
+    head = "mixed" if "mixed" in results else list(results)[0]
+    # dominant kernel of the headline solve: per BiCGStab iteration 2 forward
+    # sweeps, 2 backward sweeps and 2 SpMVs of the inner precision
+    suf = "32" if head == "mixed" else "64"
+    per_iter = {k: 2 * kernels[k + suf] for k in ("lsweep", "usweep", "spmv")}
+    dom = max(per_iter, key=per_iter.get) + suf
+    roof = dict(kernel=dom, bound="hbm", achieved=round(gbs[dom], 1), peak=PEAK_HBM_GBS, unit="GB/s",
+                frac=round(gbs[dom] / PEAK_HBM_GBS, 4), traffic=None, algorithmic_bytes=kb[dom],
+                avg_ms=kernels[dom])
+
+    out = {
+        "metric": METRIC,
+        "value": round(results[head]["ms_per_step"], 3),
+        "unit": "ms/step",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(results[head]["ms_per_step"], 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-inner/f64-refine" if head == "mixed" else "f64",
+        "data": "synthetic",
+        "config": {"workload": "pcorr-poisson-delaunay-1M", "cells": n, "faces": mesh.n_faces, "nnz": nnz,
+                   "sigma": args.sigma, "tol": args.tol, "method": head},
+        "modes": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                  for k, v in results.items()},
+        "speedup_vs_plain_fp64": (round(results["fp64"]["ms_per_step"] / results[head]["ms_per_step"], 3)
+                                  if "fp64" in results else None),
+        "spmv_gbs": {"fp64": round(gbs["spmv64"], 1), "fp32": round(gbs["spmv32"], 1)},
+        "kernel_ms": {k: round(v, 5) for k, v in kernels.items()},
+        "roofline": roof,
+    }
+    if not args.no_cpu_baseline:
+        it = results.get("fp64", results[head])["iters"]
+        out["cpu_baseline"] = cpu_baseline(xy, cptr, cind, bc, g, b, it)
+        log("cpu baseline: %s" % out["cpu_baseline"])
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

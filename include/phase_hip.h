@@ -37,6 +37,8 @@
 #ifndef PHASE_HIP_H
 #define PHASE_HIP_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -72,6 +74,12 @@ enum {
 const char *phase_hip_last_error(const phase_hip_solver *s);
 int phase_hip_device_count(int *count);
 
+/* Device memory for callers that keep operator inputs resident in HBM (the
+ * *_device entry points).  toDevice: 1 host->device, 0 device->host. */
+int phase_hip_malloc(int device, size_t bytes, void **ptr);
+int phase_hip_free(void *ptr);
+int phase_hip_memcpy(void *dst, const void *src, size_t bytes, int toDevice);
+
 /* ------------------------------------------------------------------ solver */
 
 /* SparseMatrixSolverFactory::create("hip") -- SparseMatrixSolverFactory.cpp:26-41 */
@@ -100,6 +108,9 @@ int phase_hip_solver_set(phase_hip_solver *s, int n, const int *rowPtr, const in
 /* SparseMatrixSolver::setRhs / setGuess -- SparseMatrixSolver.h:36-38 */
 int phase_hip_solver_set_rhs(phase_hip_solver *s, const double *b);
 int phase_hip_solver_set_guess(phase_hip_solver *s, const double *x0);
+
+/* setGuess(0) without a host copy: the next solve starts from x = 0. */
+int phase_hip_solver_zero_guess(phase_hip_solver *s);
 
 /* SparseMatrixSolver::solve() -- SparseMatrixSolver.h:40; returns error() via
  * *relres: the FP64 true relative residual ||b - A x||_2 / ||b||_2. */
@@ -133,11 +144,13 @@ int phase_hip_solver_spmv(phase_hip_solver *s, int single, const double *x, doub
 int phase_hip_solver_ilu_export(phase_hip_solver *s, int *nnz, int *perm, int *rowPtr,
                                 int *colInd, double *vals);
 
-/* Timing hooks for the benchmark: run `reps` device SpMVs / ILU applies on the
- * resident system and return the mean device time of one launch (ms) from HIP
- * events on the handle's stream. */
-int phase_hip_solver_time_kernels(phase_hip_solver *s, int reps, double *spmv64Ms,
-                                  double *spmv32Ms, double *ilu64Ms, double *ilu32Ms);
+/* Timing hook for the benchmark: runs each kernel `reps` times on the
+ * resident system and writes the mean device time of one launch (ms, HIP
+ * events on the handle's stream) to ms[0..nms): 0 SpMV FP64, 1 SpMV FP32,
+ * 2 ILU forward sweep FP64, 3 backward sweep FP64, 4 forward FP32,
+ * 5 backward FP32, 6 ILU(0) factorisation FP64 (value gather + factor),
+ * 7 factorisation FP32. */
+int phase_hip_solver_time_kernels(phase_hip_solver *s, int reps, double *ms, int nms);
 
 /* ------------------------------------------------------------------ mesh */
 
@@ -177,9 +190,20 @@ int phase_hip_assemble_laplacian(phase_hip_solver *s, phase_hip_mesh *m, int var
                                  const double *phiFace, const double *phi0Cell,
                                  const double *phi0Face);
 
+/* As phase_hip_assemble_laplacian, with every array argument a DEVICE pointer
+ * (e.g. from phase_hip_malloc) so operator inputs stay resident in HBM.  The
+ * boundary-type check then runs in the kernel and an unsupported type is
+ * reported (PHASE_HIP_EBOUNDARY) by the next solve or export. */
+int phase_hip_assemble_laplacian_device(phase_hip_solver *s, phase_hip_mesh *m, int variant,
+                                        double gamma, const double *gammaFace,
+                                        const double *gamma0Face, double theta,
+                                        const int *bcType, const double *phiFace,
+                                        const double *phi0Cell, const double *phi0Face);
+
 /* CrsEquation::operator==(Vector) / operator-=(Vector) -- CrsEquation.cpp:282-311:
- * rhs -= v (v per row, original numbering). */
+ * rhs -= v (v per row, original numbering).  _device: v is a device pointer. */
 int phase_hip_equation_sub_rhs(phase_hip_solver *s, const double *v);
+int phase_hip_equation_sub_rhs_device(phase_hip_solver *s, const double *v);
 
 /* The held equation in the reference's padded CrsEquation layout
  * (CrsEquation.cpp:9-14: 5 slots per row, -1 = empty, first-touch column order

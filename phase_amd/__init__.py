@@ -16,11 +16,11 @@ interfaces:
 There is no CPU fallback: loading fails loudly when the library is absent.
 """
 from .hip import (BC_FIXED, BC_NORMAL_GRADIENT, BC_OUTFLOW, BC_PARTIAL_SLIP, BC_SYMMETRY,
-                  MODE_FP64, MODE_LEARNED, MODE_MIXED, HipMesh, HipSparseMatrixSolver,
+                  MODE_FP64, MODE_LEARNED, MODE_MIXED, DeviceArray, HipMesh, HipSparseMatrixSolver,
                   PhaseHipError, device_count, library_path, load_library)
 
 __all__ = [
     "BC_FIXED", "BC_NORMAL_GRADIENT", "BC_OUTFLOW", "BC_PARTIAL_SLIP", "BC_SYMMETRY",
-    "MODE_FP64", "MODE_LEARNED", "MODE_MIXED", "HipMesh", "HipSparseMatrixSolver",
+    "MODE_FP64", "MODE_LEARNED", "MODE_MIXED", "DeviceArray", "HipMesh", "HipSparseMatrixSolver",
     "PhaseHipError", "device_count", "library_path", "load_library",
 ]

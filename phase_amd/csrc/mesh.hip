@@ -54,7 +54,8 @@ struct MeshBinding {
   DBuf<double> dStage[6];            // uploaded operator inputs
   DBuf<int> dBc;
   DBuf<int> dErr;
-  std::vector<int> lastBc;           // boundary types of the last assembly
+  std::vector<int> lastBc;           // boundary types of the last assembly (host copy)
+  bool bcOnHost = false;
 };
 
 void solverFreeBinding(phase_hip_solver *s) {
@@ -185,6 +186,10 @@ __global__ void kAssembleLaplacian(int nCells, int variant, double gamma, const 
       const int f = bface[k];
       const int bt = bcType ? bcType[f] : PHASE_HIP_BC_NORMAL_GRADIENT;
       if (bt == PHASE_HIP_BC_FIXED) {
+        if (!phiF || (hasTheta && !phi0F)) {
+          atomicOr(err, 2);
+          continue;
+        }
         const double g = field ? gammaF[f] : gamma;
         const double coeff = g * bdot[k] / bms[k];
         double dg;
@@ -461,88 +466,161 @@ int phase_hip_mesh_geometry(const phase_hip_mesh *mc, double *cellVolume, double
   });
 }
 
+}  // extern "C"
+
+namespace {
+
+void bindMesh(phase_hip_solver *s, phase_hip_mesh *m) {
+  if (s->boundMesh == m && s->binding && s->havePattern) return;
+  solverFreeBinding(s);
+  auto *b = new MeshBinding();
+  buildBinding(m, *b);
+  s->rank = 0;
+  solverSetPattern(s, m->nCells, b->rowPtr.data(), b->colInd.data());
+  b->dIslot.upload(b->islot.empty() ? std::vector<int>{0} : b->islot, s->stream);
+  b->dIfirst.upload(b->ifirst.empty() ? std::vector<uint8_t>{0} : b->ifirst, s->stream);
+  b->dDslot.upload(b->dslot, s->stream);
+  b->dRhs.alloc(m->nCells);
+  b->dErr.alloc(1);
+  b->dErr.zero(s->stream);
+  s->binding = b;
+  s->boundMesh = m;
+  s->haveSolution = false;
+}
+
+// Shared body of the host- and device-pointer assembly entry points.  With
+// onDevice the operator inputs are device pointers and the boundary-type check
+// happens in the kernel (reported by the next solve or export).
+void assembleLaplacian(phase_hip_solver *s, phase_hip_mesh *m, int variant, double gamma, const double *gammaFace,
+                       const double *gamma0Face, double theta, const int *bcType, const double *phiFace,
+                       const double *phi0Cell, const double *phi0Face, bool onDevice) {
+  if (!m) fail(PHASE_HIP_EINVAL, "null mesh");
+  if (variant < 0 || variant > 3) fail(PHASE_HIP_EINVAL, "laplacian variant must be 0..3");
+  if (m->device != s->device) fail(PHASE_HIP_EINVAL, "mesh and solver live on different devices");
+  const bool field = variant >= 2, hasTheta = (variant == 0 || variant == 2);
+  if (field && !gammaFace) fail(PHASE_HIP_EINVAL, "field-gamma laplacian needs gammaFace");
+  if (hasTheta && !phi0Cell) fail(PHASE_HIP_EINVAL, "theta laplacian needs phi0Cell");
+  PH_CHECK(hipSetDevice(s->device));
+  const int N = m->nCells, F = m->nFaces;
+  if (!onDevice && bcType) {
+    bool needFixed = false, bad = false;
+    for (int f = 0; f < F; ++f) {
+      if (m->fR[f] >= 0) continue;
+      const int bt = bcType[f];
+      if (bt == PHASE_HIP_BC_FIXED) needFixed = true;
+      else if (bt != PHASE_HIP_BC_NORMAL_GRADIENT && bt != PHASE_HIP_BC_SYMMETRY) bad = true;
+    }
+    if (bad) fail(PHASE_HIP_EBOUNDARY, "fv::laplacian: unrecognized or unspecified boundary type.");
+    if (needFixed && !phiFace) fail(PHASE_HIP_EINVAL, "FIXED boundaries need phiFace");
+    if (needFixed && hasTheta && !phi0Face)
+      fail(PHASE_HIP_EINVAL, "theta laplacian with FIXED boundaries needs phi0Face");
+  }
+  bindMesh(s, m);
+  MeshBinding &b = *s->binding;
+  hipStream_t st = s->stream;
+  PH_CHECK(hipStreamSynchronize(m->stream));
+  const double *dG = gammaFace, *dG0 = gamma0Face ? gamma0Face : gammaFace, *dPhiF = phiFace,
+               *dPhi0C = phi0Cell, *dPhi0F = phi0Face;
+  const int *dBc = bcType;
+  if (!onDevice) {
+    uploadOpt(b.dStage[0], gammaFace, F, st);
+    uploadOpt(b.dStage[1], dG0, F, st);
+    uploadOpt(b.dStage[2], phiFace, F, st);
+    uploadOpt(b.dStage[3], phi0Cell, N, st);
+    uploadOpt(b.dStage[4], phi0Face, F, st);
+    dG = gammaFace ? b.dStage[0].p : nullptr;
+    dG0 = dG0 ? b.dStage[1].p : nullptr;
+    dPhiF = phiFace ? b.dStage[2].p : nullptr;
+    dPhi0C = phi0Cell ? b.dStage[3].p : nullptr;
+    dPhi0F = phi0Face ? b.dStage[4].p : nullptr;
+  }
+  if (bcType) {
+    b.dBc.alloc(F);
+    PH_CHECK(hipMemcpyAsync(b.dBc.p, bcType, F * sizeof(int),
+                            onDevice ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+    dBc = b.dBc.p;
+  } else {
+    b.dBc.release();
+    dBc = nullptr;
+  }
+  b.bcOnHost = false;
+  hipLaunchKernelGGL(kAssembleLaplacian, dim3(grid1(N)), dim3(256), 0, st, N, variant, gamma, field ? dG : nullptr,
+                     (field && hasTheta) ? dG0 : nullptr, theta, dBc, dPhiF, hasTheta ? dPhi0C : nullptr,
+                     hasTheta ? dPhi0F : nullptr, m->dIptr.p, m->dIface.p, m->dInb.p, m->dIDot.p, m->dIMs.p,
+                     b.dIslot.p, b.dIfirst.p, m->dBptr.p, m->dBface.p, m->dBDot.p, m->dBMs.p, b.dDslot.p, s->dVals.p,
+                     b.dRhs.p, b.dErr.p);
+  PH_CHECK(hipGetLastError());
+  solverValuesChanged(s);
+  hipLaunchKernelGGL(kRhsToB, dim3(grid1(N)), dim3(256), 0, st, N, s->dPerm.p, b.dRhs.p, s->dB.p);
+  PH_CHECK(hipGetLastError());
+  s->haveRhs = true;
+  s->lastVariant = variant;
+  if (!onDevice) checkAssembly(s);
+}
+
+void subRhs(phase_hip_solver *s, const double *v, bool onDevice) {
+  if (!s->binding) fail(PHASE_HIP_ESTATE, "no assembled equation");
+  PH_CHECK(hipSetDevice(s->device));
+  const int N = s->rank;
+  MeshBinding &b = *s->binding;
+  const double *dv = v;
+  if (!onDevice) {
+    b.dStage[5].alloc(N);
+    PH_CHECK(hipMemcpyAsync(b.dStage[5].p, v, N * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    dv = b.dStage[5].p;
+  }
+  hipLaunchKernelGGL(kSubRhs, dim3(grid1(N)), dim3(256), 0, s->stream, N, s->dPerm.p, dv, b.dRhs.p, s->dB.p);
+  PH_CHECK(hipGetLastError());
+  if (!onDevice) PH_CHECK(hipStreamSynchronize(s->stream));
+}
+
+}  // namespace
+
+namespace phase {
+
+// Report a boundary-type error flagged by the assembly kernel (and clear it).
+void checkAssembly(phase_hip_solver *s) {
+  if (!s->binding) return;
+  MeshBinding &b = *s->binding;
+  int e = 0;
+  PH_CHECK(hipMemcpyAsync(&e, b.dErr.p, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+  PH_CHECK(hipStreamSynchronize(s->stream));
+  if (e) {
+    b.dErr.zero(s->stream);
+    fail(PHASE_HIP_EBOUNDARY, "fv::laplacian: unrecognized or unspecified boundary type.");
+  }
+}
+
+}  // namespace phase
+
+extern "C" {
+
 int phase_hip_assemble_laplacian(phase_hip_solver *s, phase_hip_mesh *m, int variant, double gamma,
                                  const double *gammaFace, const double *gamma0Face, double theta,
                                  const int *bcType, const double *phiFace, const double *phi0Cell,
                                  const double *phi0Face) {
   return guardedSolver(s, [&] {
-    if (!m) fail(PHASE_HIP_EINVAL, "null mesh");
-    if (variant < 0 || variant > 3) fail(PHASE_HIP_EINVAL, "laplacian variant must be 0..3");
-    if (m->device != s->device) fail(PHASE_HIP_EINVAL, "mesh and solver live on different devices");
-    const bool field = variant >= 2, hasTheta = (variant == 0 || variant == 2);
-    if (field && !gammaFace) fail(PHASE_HIP_EINVAL, "field-gamma laplacian needs gammaFace");
-    if (hasTheta && (!phi0Cell)) fail(PHASE_HIP_EINVAL, "theta laplacian needs phi0Cell");
-    PH_CHECK(hipSetDevice(s->device));
-    const int N = m->nCells, F = m->nFaces;
-    bool needFixed = false, bad = false;
-    if (bcType)
-      for (int f = 0; f < F; ++f) {
-        if (m->fR[f] >= 0) continue;
-        const int bt = bcType[f];
-        if (bt == PHASE_HIP_BC_FIXED) needFixed = true;
-        else if (bt != PHASE_HIP_BC_NORMAL_GRADIENT && bt != PHASE_HIP_BC_SYMMETRY) bad = true;
-      }
-    if (bad) fail(PHASE_HIP_EBOUNDARY, "fv::laplacian: unrecognized or unspecified boundary type.");
-    if (needFixed && !phiFace) fail(PHASE_HIP_EINVAL, "FIXED boundaries need phiFace");
-    if (needFixed && hasTheta && !phi0Face) fail(PHASE_HIP_EINVAL, "theta laplacian with FIXED boundaries needs phi0Face");
-    if (s->boundMesh != m || !s->binding || !s->havePattern) {
-      solverFreeBinding(s);
-      auto *b = new MeshBinding();
-      buildBinding(m, *b);
-      s->rank = 0;
-      solverSetPattern(s, N, b->rowPtr.data(), b->colInd.data());
-      b->dIslot.upload(b->islot.empty() ? std::vector<int>{0} : b->islot, s->stream);
-      b->dIfirst.upload(b->ifirst.empty() ? std::vector<uint8_t>{0} : b->ifirst, s->stream);
-      b->dDslot.upload(b->dslot, s->stream);
-      b->dRhs.alloc(N);
-      b->dErr.alloc(1);
-      s->binding = b;
-      s->boundMesh = m;
-      s->haveSolution = false;
-    }
-    MeshBinding &b = *s->binding;
-    hipStream_t st = s->stream;
-    PH_CHECK(hipStreamSynchronize(m->stream));
-    uploadOpt(b.dStage[0], gammaFace, F, st);
-    uploadOpt(b.dStage[1], gamma0Face ? gamma0Face : gammaFace, F, st);
-    uploadOpt(b.dStage[2], phiFace, F, st);
-    uploadOpt(b.dStage[3], phi0Cell, N, st);
-    uploadOpt(b.dStage[4], phi0Face, F, st);
-    uploadOpt(b.dBc, bcType, F, st);
-    b.lastBc.assign(bcType ? bcType : nullptr, bcType ? bcType + F : nullptr);
-    b.dErr.zero(st);
-    hipLaunchKernelGGL(kAssembleLaplacian, dim3(grid1(N)), dim3(256), 0, st, N, variant, gamma,
-                       gammaFace ? b.dStage[0].p : nullptr, field && hasTheta ? b.dStage[1].p : nullptr, theta,
-                       bcType ? b.dBc.p : nullptr, phiFace ? b.dStage[2].p : nullptr,
-                       hasTheta ? b.dStage[3].p : nullptr, (hasTheta && phi0Face) ? b.dStage[4].p : nullptr,
-                       m->dIptr.p, m->dIface.p, m->dInb.p, m->dIDot.p, m->dIMs.p, b.dIslot.p, b.dIfirst.p,
-                       m->dBptr.p, m->dBface.p, m->dBDot.p, m->dBMs.p, b.dDslot.p, s->dVals.p, b.dRhs.p, b.dErr.p);
-    PH_CHECK(hipGetLastError());
-    solverValuesChanged(s);
-    hipLaunchKernelGGL(kRhsToB, dim3(grid1(N)), dim3(256), 0, st, N, s->dPerm.p, b.dRhs.p, s->dB.p);
-    PH_CHECK(hipGetLastError());
-    s->haveRhs = true;
-    s->lastVariant = variant;
-    int e = 0;
-    PH_CHECK(hipMemcpyAsync(&e, b.dErr.p, sizeof(int), hipMemcpyDeviceToHost, st));
-    PH_CHECK(hipStreamSynchronize(st));
-    if (e) fail(PHASE_HIP_EBOUNDARY, "fv::laplacian: unrecognized or unspecified boundary type.");
+    assembleLaplacian(s, m, variant, gamma, gammaFace, gamma0Face, theta, bcType, phiFace, phi0Cell, phi0Face,
+                      false);
+  });
+}
+
+int phase_hip_assemble_laplacian_device(phase_hip_solver *s, phase_hip_mesh *m, int variant, double gamma,
+                                        const double *gammaFace, const double *gamma0Face, double theta,
+                                        const int *bcType, const double *phiFace, const double *phi0Cell,
+                                        const double *phi0Face) {
+  return guardedSolver(s, [&] {
+    assembleLaplacian(s, m, variant, gamma, gammaFace, gamma0Face, theta, bcType, phiFace, phi0Cell, phi0Face,
+                      true);
   });
 }
 
 int phase_hip_equation_sub_rhs(phase_hip_solver *s, const double *v) {
-  return guardedSolver(s, [&] {
-    if (!s->binding) fail(PHASE_HIP_ESTATE, "no assembled equation");
-    PH_CHECK(hipSetDevice(s->device));
-    const int N = s->rank;
-    MeshBinding &b = *s->binding;
-    b.dStage[5].alloc(N);
-    PH_CHECK(hipMemcpyAsync(b.dStage[5].p, v, N * sizeof(double), hipMemcpyHostToDevice, s->stream));
-    hipLaunchKernelGGL(kSubRhs, dim3(grid1(N)), dim3(256), 0, s->stream, N, s->dPerm.p, b.dStage[5].p, b.dRhs.p,
-                       s->dB.p);
-    PH_CHECK(hipGetLastError());
-    PH_CHECK(hipStreamSynchronize(s->stream));
-  });
+  return guardedSolver(s, [&] { subRhs(s, v, false); });
+}
+
+int phase_hip_equation_sub_rhs_device(phase_hip_solver *s, const double *v) {
+  return guardedSolver(s, [&] { subRhs(s, v, true); });
 }
 
 int phase_hip_equation_export_padded(phase_hip_solver *s, int *nslots, int *rowPtr, int *colInd, double *vals,
@@ -554,6 +632,16 @@ int phase_hip_equation_export_padded(phase_hip_solver *s, int *nslots, int *rowP
     MeshBinding &b = *s->binding;
     const int N = m->nCells;
     const bool pFirst = s->lastVariant >= 2;
+    checkAssembly(s);
+    if (!b.bcOnHost) {
+      b.lastBc.clear();
+      if (b.dBc.p) {
+        b.lastBc.resize(m->nFaces);
+        PH_CHECK(hipMemcpyAsync(b.lastBc.data(), b.dBc.p, m->nFaces * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+        PH_CHECK(hipStreamSynchronize(s->stream));
+      }
+      b.bcOnHost = true;
+    }
     // per row: first-touch column order of the last variant (CrsEquation::addCoeff)
     std::vector<int> rp(N + 1, 0);
     std::vector<std::vector<int>> order(N);

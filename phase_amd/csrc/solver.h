@@ -89,4 +89,6 @@ void solverSetPattern(phase_hip_solver *s, int n, const int *rowPtr, const int *
 // dVals changed on the device: refresh SpMV copies, invalidate factors.
 void solverValuesChanged(phase_hip_solver *s);
 void solverFreeBinding(phase_hip_solver *s);
+// Raise a boundary-type error flagged by the last device assembly, if any.
+void checkAssembly(phase_hip_solver *s);
 }  // namespace phase

@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <type_traits>
 
 #include "solver.h"
 #include "solver_kernels.h"
@@ -495,11 +496,35 @@ int phase_hip_solver_set_guess(phase_hip_solver *s, const double *x0) {
   });
 }
 
+int phase_hip_solver_zero_guess(phase_hip_solver *s) {
+  return guarded(s, [&] {
+    if (!s->havePattern) fail(PHASE_HIP_ESTATE, "setGuess before set()");
+    setDevice(s);
+    PH_CHECK(hipMemsetAsync(s->dX.p, 0, s->rank * sizeof(double), s->stream));
+    s->haveGuess = true;
+  });
+}
+
+int phase_hip_malloc(int device, size_t bytes, void **ptr) {
+  if (!ptr) return PHASE_HIP_EINVAL;
+  *ptr = nullptr;
+  if (hipSetDevice(device) != hipSuccess) return PHASE_HIP_EDEVICE;
+  return hipMalloc(ptr, bytes ? bytes : 1) == hipSuccess ? PHASE_HIP_OK : PHASE_HIP_EDEVICE;
+}
+
+int phase_hip_free(void *ptr) { return hipFree(ptr) == hipSuccess ? PHASE_HIP_OK : PHASE_HIP_EDEVICE; }
+
+int phase_hip_memcpy(void *dst, const void *src, size_t bytes, int toDevice) {
+  hipError_t e = hipMemcpy(dst, src, bytes, toDevice ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost);
+  return e == hipSuccess ? PHASE_HIP_OK : PHASE_HIP_EDEVICE;
+}
+
 int phase_hip_solver_solve(phase_hip_solver *s, double *relres) {
   return guarded(s, [&] {
     if (!s->havePattern || !s->haveValues) fail(PHASE_HIP_ESTATE, "solve before set()");
     if (!s->haveRhs) fail(PHASE_HIP_ESTATE, "solve before setRhs()");
     setDevice(s);
+    checkAssembly(s);
     // Warm start: like the reference's Tpetra x_ (TrilinosSparseMatrixSolver.cpp:31-38),
     // the previous solution is the initial guess unless setGuess() gave one.
     if (!s->haveGuess && !s->haveSolution) PH_CHECK(hipMemsetAsync(s->dX.p, 0, s->rank * sizeof(double), s->stream));
@@ -648,10 +673,10 @@ int phase_hip_solver_ilu_export(phase_hip_solver *s, int *nnz, int *perm, int *r
   });
 }
 
-int phase_hip_solver_time_kernels(phase_hip_solver *s, int reps, double *spmv64Ms, double *spmv32Ms,
-                                  double *ilu64Ms, double *ilu32Ms) {
+int phase_hip_solver_time_kernels(phase_hip_solver *s, int reps, double *ms, int nms) {
   return guarded(s, [&] {
     if (!s->havePattern || !s->haveValues) fail(PHASE_HIP_ESTATE, "no system");
+    if (!ms || nms < 1) fail(PHASE_HIP_EINVAL, "no output");
     setDevice(s);
     const int n = s->rank;
     reps = std::max(1, reps);
@@ -659,8 +684,6 @@ int phase_hip_solver_time_kernels(phase_hip_solver *s, int reps, double *spmv64M
     ensureSingle(s);
     s->k64.alloc(n);
     s->k32.alloc(n);
-    PH_CHECK(hipMemsetAsync(s->k64.ph.p, 0, n * sizeof(double), s->stream));
-    PH_CHECK(hipMemsetAsync(s->k32.ph.p, 0, n * sizeof(float), s->stream));
     hipLaunchKernelGGL((kConvert<double, double>), dim3(gridFor(n, 256, kElemCap)), dim3(256), 0, s->stream, n,
                        s->dB.p, s->k64.ph.p);
     hipLaunchKernelGGL((kConvert<double, float>), dim3(gridFor(n, 256, kElemCap)), dim3(256), 0, s->stream, n,
@@ -671,29 +694,45 @@ int phase_hip_solver_time_kernels(phase_hip_solver *s, int reps, double *spmv64M
       for (int r = 0; r < reps; ++r) launch();
       PH_CHECK(hipEventRecord(s->ev1, s->stream));
       PH_CHECK(hipEventSynchronize(s->ev1));
-      float ms = 0.f;
-      PH_CHECK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
-      return (double)ms / reps;
+      float t = 0.f;
+      PH_CHECK(hipEventElapsedTime(&t, s->ev0, s->ev1));
+      return (double)t / reps;
     };
-    double a = timeIt([&] {
+    const Scalars *noSc = nullptr;
+    auto lsweep = [&](auto *fac, auto *in, auto *out) {
+      using T = std::remove_pointer_t<decltype(in)>;
+      unsigned long long base;
+      int ep;
+      nextSweep(s, base, ep);
+      hipLaunchKernelGGL((kLSolve<T, kProNone>), dim3(s->ord.nBlocks), dim3(64), 0, s->stream, iluDev<T>(s, fac),
+                         syncDev(s), base, ep, noSc, (const T *)in, out, (T *)nullptr, (const T *)nullptr,
+                         (const T *)nullptr);
+    };
+    auto usweep = [&](auto *fac, auto *in, auto *out) {
+      using T = std::remove_pointer_t<decltype(in)>;
+      unsigned long long base;
+      int ep;
+      nextSweep(s, base, ep);
+      hipLaunchKernelGGL((kUSolve<T>), dim3(s->ord.nBlocks), dim3(64), 0, s->stream, iluDev<T>(s, fac), syncDev(s),
+                         base, ep, noSc, (const T *)in, out);
+    };
+    double out[8];
+    out[0] = timeIt([&] {
       hipLaunchKernelGGL((kSpmv<double>), dim3(spmvGrid(s)), dim3(kRedBlock), 0, s->stream, n, s->ord.nSlices,
                          s->dSliceOff.p, s->dSellCol.p, s->dSell64.p, s->k64.ph.p, s->k64.t.p);
     });
-    double b = timeIt([&] {
+    out[1] = timeIt([&] {
       hipLaunchKernelGGL((kSpmv<float>), dim3(spmvGrid(s)), dim3(kRedBlock), 0, s->stream, n, s->ord.nSlices,
                          s->dSliceOff.p, s->dSellCol.p, s->dSell32.p, s->k32.ph.p, s->k32.t.p);
     });
-    double c = timeIt([&] {
-      iluApply<double>(s, s->dFac64, s->k64.ph.p, s->k64.y.p, s->k64.sh.p, (const Scalars *)nullptr);
-    });
-    double d = timeIt([&] {
-      iluApply<float>(s, s->dFac32, s->k32.ph.p, s->k32.y.p, s->k32.sh.p, (const Scalars *)nullptr);
-    });
+    out[2] = timeIt([&] { lsweep(s->dFac64.p, s->k64.ph.p, s->k64.y.p); });
+    out[3] = timeIt([&] { usweep(s->dFac64.p, s->k64.y.p, s->k64.sh.p); });
+    out[4] = timeIt([&] { lsweep(s->dFac32.p, s->k32.ph.p, s->k32.y.p); });
+    out[5] = timeIt([&] { usweep(s->dFac32.p, s->k32.y.p, s->k32.sh.p); });
+    out[6] = timeIt([&] { factorize<double>(s, s->dFac64); });
+    out[7] = timeIt([&] { factorize<float>(s, s->dFac32); });
     checkDeviceErr(s);
-    if (spmv64Ms) *spmv64Ms = a;
-    if (spmv32Ms) *spmv32Ms = b;
-    if (ilu64Ms) *ilu64Ms = c;
-    if (ilu32Ms) *ilu32Ms = d;
+    for (int i = 0; i < std::min(nms, 8); ++i) ms[i] = out[i];
   });
 }
 

@@ -47,7 +47,7 @@ _SIGS = {
     "phase_hip_solver_export": (ctypes.c_int, [_vp, _ip, _vp, _vp, _vp, _vp]),
     "phase_hip_solver_spmv": (ctypes.c_int, [_vp, ctypes.c_int, _f64p, _f64p]),
     "phase_hip_solver_ilu_export": (ctypes.c_int, [_vp, _ip, _vp, _vp, _vp, _vp]),
-    "phase_hip_solver_time_kernels": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, _dp]),
+    "phase_hip_solver_time_kernels": (ctypes.c_int, [_vp, ctypes.c_int, _f64p, ctypes.c_int]),
     "phase_hip_mesh_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, _f64p,
                                              ctypes.c_int, _i32p, _i32p]),
     "phase_hip_mesh_destroy": (ctypes.c_int, [_vp]),
@@ -58,6 +58,13 @@ _SIGS = {
     "phase_hip_assemble_laplacian": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_double, _vp, _vp,
                                                     ctypes.c_double, _vp, _vp, _vp, _vp]),
     "phase_hip_equation_sub_rhs": (ctypes.c_int, [_vp, _f64p]),
+    "phase_hip_assemble_laplacian_device": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_double, _vp, _vp,
+                                                           ctypes.c_double, _vp, _vp, _vp, _vp]),
+    "phase_hip_equation_sub_rhs_device": (ctypes.c_int, [_vp, _vp]),
+    "phase_hip_solver_zero_guess": (ctypes.c_int, [_vp]),
+    "phase_hip_malloc": (ctypes.c_int, [ctypes.c_int, ctypes.c_size_t, ctypes.POINTER(_vp)]),
+    "phase_hip_free": (ctypes.c_int, [_vp]),
+    "phase_hip_memcpy": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t, ctypes.c_int]),
     "phase_hip_equation_export_padded": (ctypes.c_int, [_vp, _ip, _vp, _vp, _vp, _vp]),
 }
 
@@ -100,6 +107,38 @@ def _f64(a):
 
 def _i32(a):
     return None if a is None else np.ascontiguousarray(a, dtype=np.int32)
+
+
+class DeviceArray:
+    """A device allocation holding a copy of a host numpy array (HBM-resident
+    operator input for the *_device entry points)."""
+
+    def __init__(self, host, device=0):
+        lib = load_library()
+        self.host = np.ascontiguousarray(host)
+        self.nbytes = self.host.nbytes
+        self.ptr = _vp()
+        rc = lib.phase_hip_malloc(device, self.nbytes, ctypes.byref(self.ptr))
+        if rc != 0:
+            raise PhaseHipError(rc, "malloc", "device allocation failed")
+        rc = lib.phase_hip_memcpy(self.ptr, self.host.ctypes.data_as(_vp), self.nbytes, 1)
+        if rc != 0:
+            raise PhaseHipError(rc, "memcpy", "host to device copy failed")
+
+    def free(self):
+        if getattr(self, "ptr", None) and self.ptr.value:
+            load_library().phase_hip_free(self.ptr)
+            self.ptr = _vp()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _dp_or_none(a):
+    return None if a is None else a.ptr
 
 
 class HipMesh:
@@ -245,10 +284,13 @@ class HipSparseMatrixSolver:
                     "ilu")
         return perm, rp, ci, v
 
+    KERNEL_NAMES = ("spmv64", "spmv32", "lsweep64", "usweep64", "lsweep32", "usweep32", "factor64", "factor32")
+
     def time_kernels(self, reps=20):
-        a, b, c, d = ctypes.c_double(), ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
-        self._check(load_library().phase_hip_solver_time_kernels(self._h, int(reps), a, b, c, d), "time")
-        return dict(spmv64_ms=a.value, spmv32_ms=b.value, ilu64_ms=c.value, ilu32_ms=d.value)
+        """Mean device ms of one launch of each solver kernel (HIP events)."""
+        out = np.zeros(len(self.KERNEL_NAMES))
+        self._check(load_library().phase_hip_solver_time_kernels(self._h, int(reps), out, len(out)), "time")
+        return dict(zip(self.KERNEL_NAMES, out.tolist()))
 
     # --- fv:: operators on a HipMesh
     def assemble_laplacian(self, mesh, variant, gamma=1.0, gamma_face=None, gamma0_face=None, theta=1.0,
@@ -260,6 +302,22 @@ class HipSparseMatrixSolver:
             _p(args[2]), _p(args[3]), _p(args[4]), _p(args[5]))
         self._check(rc, "assembleLaplacian")
         self.rank = mesh.n_cells
+
+    def assemble_laplacian_device(self, mesh, variant, gamma=1.0, gamma_face=None, gamma0_face=None, theta=1.0,
+                                  bc_type=None, phi_face=None, phi0_cell=None, phi0_face=None):
+        """Same as assemble_laplacian with DeviceArray inputs (resident in HBM)."""
+        rc = load_library().phase_hip_assemble_laplacian_device(
+            self._h, mesh._h, int(variant), float(gamma), _dp_or_none(gamma_face), _dp_or_none(gamma0_face),
+            float(theta), _dp_or_none(bc_type), _dp_or_none(phi_face), _dp_or_none(phi0_cell),
+            _dp_or_none(phi0_face))
+        self._check(rc, "assembleLaplacian")
+        self.rank = mesh.n_cells
+
+    def sub_rhs_device(self, v):
+        self._check(load_library().phase_hip_equation_sub_rhs_device(self._h, v.ptr), "operator==")
+
+    def zero_guess(self):
+        self._check(load_library().phase_hip_solver_zero_guess(self._h), "setGuess")
 
     def sub_rhs(self, v):
         """CrsEquation::operator==(Vector): stored rhs -= v."""

@@ -1,0 +1,87 @@
+"""CrsEquation semantics of the oracle (src/Math/CrsEquation.cpp), checked
+against hand-worked known answers."""
+import numpy as np
+
+
+def test_padded_ctor_and_first_touch(oracle):
+    e = oracle.Eqn.create(3, 2)
+    e.add_coeff(0, 2, 1.5)      # claims slot 0 of row 0
+    e.add_coeff(0, 0, -1.0)     # slot 1
+    e.add_coeff(0, 2, 0.25)     # accumulates into slot 0
+    e.add_coeff(2, 1, 4.0)
+    rp, ci, v, rhs = e.get()
+    assert list(rp) == [0, 2, 4, 6]
+    assert list(ci) == [2, 0, -1, -1, 1, -1]
+    assert list(v) == [1.75, -1.0, 0.0, 0.0, 4.0, 0.0]
+
+
+def test_slow_path_insert_grows_row(oracle):
+    e = oracle.Eqn.create(2, 2)
+    for c, val in ((0, 1.0), (1, 2.0), (1, 3.0)):
+        e.add_coeff(0, c, val)
+    e.add_coeff(0, 5, 7.0)      # row full -> inserted at its end, later rows shift
+    e.add_coeff(1, 1, 9.0)
+    rp, ci, v, _ = e.get()
+    assert list(rp) == [0, 3, 5]
+    assert list(ci) == [0, 1, 5, 1, -1]
+    assert list(v) == [1.0, 5.0, 7.0, 9.0, 0.0]
+
+
+def test_set_coeff_overwrites(oracle):
+    e = oracle.Eqn.create(1, 3)
+    e.add_coeff(0, 0, 1.0)
+    e.set_coeff(0, 0, 5.0)
+    e.set_coeff(0, 3, 2.0)
+    rp, ci, v, _ = e.get()
+    assert list(ci) == [0, 3, -1] and list(v) == [5.0, 2.0, 0.0]
+
+
+def test_plus_equals_drops_zeros_and_merges(oracle):
+    a = oracle.Eqn.create(2, 3)
+    a.add_coeff(0, 0, 1.0)
+    a.add_coeff(0, 1, 0.0)      # explicit zero: dropped by +=
+    a.add_coeff(1, 1, -0.0)     # negative zero: dropped too (== 0.)
+    a.add_rhs(0, 1.0)
+    b = oracle.Eqn.create(2, 3)
+    b.add_coeff(0, 2, 3.0)      # new column appended after a's entries
+    b.add_coeff(0, 0, 2.0)      # matches a's column 0
+    b.add_coeff(1, 0, 0.0)      # zero in rhs operand: skipped
+    b.add_rhs(1, 2.0)
+    a += b
+    rp, ci, v, rhs = a.get()
+    assert list(rp) == [0, 2, 2]
+    assert list(ci) == [0, 2] and list(v) == [3.0, 3.0]
+    assert list(rhs) == [1.0, 2.0]
+
+
+def test_minus_equals_and_eq_forms(oracle):
+    a = oracle.Eqn.create(2, 2)
+    a.add_coeff(0, 0, 1.0)
+    b = oracle.Eqn.create(2, 2)
+    b.add_coeff(0, 0, 4.0)
+    b.add_coeff(0, 1, 2.0)
+    b.add_rhs(0, 1.0)
+    a -= b                       # operator-= / operator==(CrsEquation)
+    rp, ci, v, rhs = a.get()
+    assert list(ci) == [0, 1] and list(v) == [-3.0, -2.0] and list(rhs) == [-1.0, 0.0]
+    a.eq(0.0)                    # == 0. leaves rhs untouched
+    assert list(a.get()[3]) == [-1.0, 0.0]
+    a.eq(2.0)                    # == s: rhs -= s
+    assert list(a.get()[3]) == [-3.0, -2.0]
+    a.eq(np.array([1.0, -1.0]))  # == Vector: rhs -= v
+    assert list(a.get()[3]) == [-4.0, -1.0]
+    a.scale(-0.5)                # *= s scales values and rhs
+    rp, ci, v, rhs = a.get()
+    assert list(v) == [1.5, 1.0] and list(rhs) == [2.0, 0.5]
+
+
+def test_relax_restored_formula(oracle):
+    """a_P /= alpha; rhs -= (1 - alpha) a_P phi (ScalarFiniteVolumeEquation.cpp:45-55)."""
+    e = oracle.Eqn.create(2, 2)
+    e.add_coeff(0, 0, 4.0)
+    e.add_coeff(0, 1, -1.0)
+    e.add_coeff(1, 1, 2.0)
+    e.relax(0.5, np.array([1.0, 3.0]))
+    rp, ci, v, rhs = e.get()
+    assert list(v) == [8.0, -1.0, 4.0, 0.0]
+    assert list(rhs) == [-4.0, -6.0]
