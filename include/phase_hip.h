@@ -122,8 +122,9 @@ int phase_hip_solver_get_x(phase_hip_solver *s, double *x);
 int phase_hip_solver_n_iters(const phase_hip_solver *s);
 double phase_hip_solver_error(const phase_hip_solver *s);
 
-/* Per-solve statistics: inner iterations, refinement passes, fallbacks taken,
- * device milliseconds of the last solve (HIP events on the handle's stream). */
+/* Per-solve statistics: inner iterations, refinement passes, fallbacks taken
+ * (learned mode: passes where the network step was rejected), device
+ * milliseconds of the last solve (HIP events on the handle's stream). */
 int phase_hip_solver_stats(const phase_hip_solver *s, int *innerIters, int *outerPasses,
                            int *fallbacks, double *solveMs);
 
@@ -151,6 +152,32 @@ int phase_hip_solver_ilu_export(phase_hip_solver *s, int *nnz, int *perm, int *r
  * 5 backward FP32, 6 ILU(0) factorisation FP64 (value gather + factor),
  * 7 factorisation FP32. */
 int phase_hip_solver_time_kernels(phase_hip_solver *s, int reps, double *ms, int nms);
+
+/* ------------------------------------------------------------------ learned part
+ * No counterpart in the reference.  A message-passing network on the matrix
+ * graph (6 layers, 64 hidden, BF16 MFMA / FP32 accumulate) trained on
+ * (residual, correction) pairs captured from the solver's own FP64 solves; in
+ * mode "learned" each FP64 refinement pass first applies the network's
+ * correction as a minimal-residual step and keeps it only when it reduces the
+ * residual by at least "fallbackRatio" (0.5), otherwise the solve continues
+ * with plain ILU(0) refinement.  Requires a symmetric matrix. */
+
+/* Capture a pair every `everyIters` iterations of FP64 solves (0 = off), up to
+ * maxPairs pairs in device memory (2 * maxPairs * rank floats). */
+int phase_hip_learned_capture(phase_hip_solver *s, int everyIters, int maxPairs);
+int phase_hip_learned_n_pairs(const phase_hip_solver *s, int *count);
+/* `steps` Adam steps (one captured pair each) at learning rate lr; *lastLoss is
+ * the loss of the final step. */
+int phase_hip_learned_train(phase_hip_solver *s, int steps, double lr, int seed, double *lastLoss);
+/* Flat FP32 parameters (checkpoint); count queried with params == NULL. */
+int phase_hip_learned_params(phase_hip_solver *s, float *params, int *count);
+int phase_hip_learned_set_params(phase_hip_solver *s, const float *params, int count);
+/* Forward + backward for residual r, ILU-preconditioned residual z and target
+ * correction (original numbering): output, loss and flat gradient. */
+int phase_hip_learned_eval(phase_hip_solver *s, const double *r, const double *z, const double *target,
+                           double *out, double *loss, float *grad);
+/* e = network correction of residual r (original numbering). */
+int phase_hip_learned_apply(phase_hip_solver *s, const double *r, double *e);
 
 /* ------------------------------------------------------------------ mesh */
 

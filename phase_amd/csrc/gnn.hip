@@ -1,0 +1,941 @@
+// Learned correction for the pressure-correction solve: a message-passing
+// network on the matrix graph, trained on (residual, correction) pairs the
+// solver emits, applied between FP64 refinement passes.  No counterpart in the
+// reference; the network is defined here and restated in FP64 by the oracle
+// (oracle/gnn_oracle.py) that the gradient tests compare against.
+//
+// With d_i = |a_ii|, sd_i = sqrt(d_i), A_hat_ij = a_ij / (sd_i sd_j) (i != j):
+//   X_i  = [r_i/(sd_i s), z_i sd_i / s, (sum_j a_ij)/a_ii, 1],  s = rms(r/sd),
+//          z = ILU(0)^{-1} r
+//   h0   = relu(X W_in + b_in)
+//   h_l  = h_{l-1} + relu([h_{l-1}, A_hat h_{l-1}] Wsm + mean(h_{l-1}) Wg + b)
+//   out  = h_L w_out + b_out,   correction e_i = out_i s / sd_i
+//   loss = mean_i (out_i - e*_i sd_i / s)^2 for a captured pair (r, e*).
+//
+// Dense layers run on BF16 MFMA (v_mfma_f32_16x16x32_bf16, FP32 accumulate):
+// activations are stored as BF16 rows of 64 features, weights as BF16 copies
+// of FP32 master parameters (Adam).  All reductions over rows go through
+// per-block partials summed in a fixed order, so training is deterministic.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <string>
+
+#include "gnn.h"
+#include "solver.h"
+#include "solver_kernels.h"
+
+using namespace phase;
+
+namespace {
+
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(4))) float f32x4;
+
+constexpr int H = kGnnH;
+constexpr int kRowsPerWave = 16;
+constexpr int kWgRows = 64;     // rows per workgroup in row-parallel kernels
+constexpr int kChunkRows = 2048;  // rows per workgroup in weight-gradient kernels
+
+__device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ float bf2f(__bf16 v) { return (float)v; }
+
+// Graph weight of SELL entry idx of solver row `row`: a_ij / (sd_i sd_j), or
+// 0 for the diagonal and padding.
+template <class T>
+__device__ __forceinline__ float ahat(const int *__restrict__ col, const T *__restrict__ val,
+                                      const float *__restrict__ sd, int idx, int row, int &c) {
+  c = col[idx];
+  if (c < 0 || c == row) return 0.f;
+  return (float)val[idx] / (sd[row] * sd[c]);
+}
+
+int gridFor1(long n) { return (int)std::max<long>(1, std::min<long>((n + kRedBlock - 1) / kRedBlock, 1024)); }
+
+// ---------------------------------------------------------------- features
+__global__ void kGnnScale(int n, const float *__restrict__ valDiag, const float *__restrict__ rowsumIn,
+                          float *__restrict__ sd, float *__restrict__ rowsum) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    sd[i] = sqrtf(fabsf(valDiag[i]));
+    rowsum[i] = rowsumIn[i];
+  }
+}
+
+// diag and row-sum ratio from the SELL FP32 copy of A
+__global__ void kGnnDiag(int n, const int *__restrict__ sliceOff, const int *__restrict__ col,
+                         const float *__restrict__ val, float *__restrict__ diag, float *__restrict__ rs) {
+  for (int row = blockIdx.x * blockDim.x + threadIdx.x; row < n; row += gridDim.x * blockDim.x) {
+    const int s = row >> 6, lane = row & 63;
+    const int o = sliceOff[s], K = (sliceOff[s + 1] - o) >> 6;
+    float d = 0.f, sum = 0.f;
+    for (int e = 0; e < K; ++e) {
+      const int idx = o + (e << 6) + lane;
+      const int c = col[idx];
+      if (c < 0) continue;
+      sum += val[idx];
+      if (c == row) d = val[idx];
+    }
+    diag[row] = d;
+    rs[row] = d != 0.f ? sum / d : 0.f;
+  }
+}
+
+// sum_i (r_i / sd_i)^2 -> scal[0] = rms
+__global__ __launch_bounds__(kRedBlock) void kGnnRms(int n, const float *__restrict__ r, const float *__restrict__ sd,
+                                                     double *scal, double *partials, unsigned *counter) {
+  double d[1] = {0.};
+  for (int i = blockIdx.x * kRedBlock + threadIdx.x; i < n; i += gridDim.x * kRedBlock) {
+    const double v = (double)r[i] / (double)sd[i];
+    d[0] += v * v;
+  }
+  if (reduceLast<1>(d, partials, counter) && threadIdx.x == 0) {
+    const double rms = sqrt(d[0] / (double)n);
+    scal[0] = rms > 0. ? rms : 1.;
+  }
+}
+
+__global__ void kGnnFeatures(int n, const float *__restrict__ r, const float *__restrict__ z,
+                             const float *__restrict__ sd, const float *__restrict__ rowsum,
+                             const double *__restrict__ scal, float *__restrict__ X) {
+  const float inv = (float)(1.0 / scal[0]);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    X[4 * i + 0] = r[i] / sd[i] * inv;
+    X[4 * i + 1] = z[i] * sd[i] * inv;
+    X[4 * i + 2] = rowsum[i];
+    X[4 * i + 3] = 1.f;
+  }
+}
+
+// target t_i = e_i sd_i / s
+__global__ void kGnnTarget(int n, const float *__restrict__ e, const float *__restrict__ sd,
+                           const double *__restrict__ scal, float *__restrict__ t) {
+  const float inv = (float)(1.0 / scal[0]);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) t[i] = e[i] * sd[i] * inv;
+}
+
+// ---------------------------------------------------------------- input layer
+__global__ void kGnnIn(int n, const float *__restrict__ X, const float *__restrict__ P, __bf16 *__restrict__ pre0,
+                       __bf16 *__restrict__ h0) {
+  const float *W = P;             // [4][64]
+  const float *b = P + kGnnDin * H;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < (long)n * H; t += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(t / H), f = (int)(t % H);
+    float v = b[f];
+#pragma unroll
+    for (int k = 0; k < kGnnDin; ++k) v = fmaf(X[4 * i + k], W[k * H + f], v);
+    pre0[t] = (__bf16)v;
+    h0[t] = (__bf16)fmaxf(v, 0.f);
+  }
+}
+
+// ---------------------------------------------------------------- column means / sums
+// out[f] = scale * sum_i M[i][f] (M bf16 or f32 rows of 64), deterministic.
+template <class T>
+__global__ __launch_bounds__(256) void kColSum(int n, const T *__restrict__ M, float scale, float *__restrict__ out,
+                                               float *__restrict__ partials, unsigned *counter) {
+  __shared__ float red[4][H];
+  __shared__ bool last;
+  const int f = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int rowsPer = (n + gridDim.x - 1) / gridDim.x;
+  const int r0 = blockIdx.x * rowsPer, r1 = min(n, r0 + rowsPer);
+  float acc = 0.f;
+  for (int i = r0 + q; i < r1; i += 4) acc += (float)M[(size_t)i * H + f];
+  red[q][f] = acc;
+  __syncthreads();
+  if (threadIdx.x < H) {
+    partials[blockIdx.x * H + f] = red[0][f] + red[1][f] + red[2][f] + red[3][f];
+  }
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) last = (atomicAdd(counter, 1u) == gridDim.x - 1);
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  if (threadIdx.x < H) {
+    float s = 0.f;
+    for (int b = 0; b < (int)gridDim.x; ++b) s += partials[b * H + f];
+    out[f] = s * scale;
+  }
+  if (threadIdx.x == 0) *counter = 0u;
+}
+
+// c[f] = b[f] + sum_k g[k] Wg[k][f]
+__global__ void kGnnFold(const float *__restrict__ g, const float *__restrict__ Wg, const float *__restrict__ b,
+                         float *__restrict__ c) {
+  const int f = threadIdx.x;
+  if (f >= H) return;
+  float v = b[f];
+  for (int k = 0; k < H; ++k) v = fmaf(g[k], Wg[k * H + f], v);
+  c[f] = v;
+}
+
+// ---------------------------------------------------------------- message-passing layer
+// One wavefront per 16 rows: A operand = [h | A_hat h] rows (K = 128), B
+// operand = Wsm columns from LDS (stored [out col][k]).  Writes agg, pre and
+// h_new = h + relu(pre + c).
+template <class T>
+__global__ __launch_bounds__(256) void kGnnLayerFwd(int n, const __bf16 *__restrict__ h, const int *__restrict__ sliceOff,
+                                                    const int *__restrict__ col, const T *__restrict__ val,
+                                                    const float *__restrict__ sd, const __bf16 *__restrict__ wT,
+                                                    const float *__restrict__ c, __bf16 *__restrict__ agg,
+                                                    __bf16 *__restrict__ pre, __bf16 *__restrict__ hNew) {
+  __shared__ __attribute__((aligned(16))) __bf16 W[H][128 + 8];
+  __shared__ float cs[H];
+  for (int t = threadIdx.x; t < H * 16; t += 256) {  // 64 cols x 16 chunks of 8
+    const int o = t / 16, k8 = (t % 16) * 8;
+    *(bf16x8 *)&W[o][k8] = *(const bf16x8 *)&wT[o * 128 + k8];
+  }
+  if (threadIdx.x < H) cs[threadIdx.x] = c[threadIdx.x];
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int rowBase = blockIdx.x * kWgRows + wave * kRowsPerWave;
+  if (rowBase >= n) return;
+  const int r = rowBase + (lane & 15), gq = lane >> 4;
+  const bool act = r < n;
+  bf16x8 a[4];
+  // self features: chunks 0,1 ; aggregated: chunks 2,3
+#pragma unroll
+  for (int kc = 0; kc < 2; ++kc) {
+    const int f0 = 32 * kc + 8 * gq;
+    if (act) a[kc] = *(const bf16x8 *)&h[(size_t)r * H + f0];
+    else
+      for (int j = 0; j < 8; ++j) a[kc][j] = (__bf16)0.f;
+    float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (act) {
+      const int s = r >> 6, l = r & 63;
+      const int o = sliceOff[s], K = (sliceOff[s + 1] - o) >> 6;
+      for (int e = 0; e < K; ++e) {
+        int cc;
+        const float w = ahat(col, val, sd, o + (e << 6) + l, r, cc);
+        if (w == 0.f) continue;
+        const bf16x8 hv = *(const bf16x8 *)&h[(size_t)cc * H + f0];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc[j] = fmaf(w, (float)hv[j], acc[j]);
+      }
+    }
+    bf16x8 av;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) av[j] = (__bf16)acc[j];
+    a[2 + kc] = av;
+    if (act) *(bf16x8 *)&agg[(size_t)r * H + f0] = av;
+  }
+  f32x4 out[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    out[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int oc = 16 * t + (lane & 15);
+#pragma unroll
+    for (int kc = 0; kc < 4; ++kc) {
+      const bf16x8 b = *(const bf16x8 *)&W[oc][32 * kc + 8 * gq];
+      out[t] = mfma(a[kc], b, out[t]);
+    }
+  }
+  // D layout: row = 4*gq + v, col = 16*t + lane%16
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int oc = 16 * t + (lane & 15);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = rowBase + 4 * gq + v;
+      if (row < n) {
+        const float p = out[t][v] + cs[oc];
+        pre[(size_t)row * H + oc] = (__bf16)p;
+        hNew[(size_t)row * H + oc] = (__bf16)((float)h[(size_t)row * H + oc] + fmaxf(p, 0.f));
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- output layer
+// out_i = h_i . w_out + b_out ; with a target: loss partials, dout and dh_L.
+__global__ __launch_bounds__(256) void kGnnOut(int n, const __bf16 *__restrict__ h, const float *__restrict__ wo,
+                                               const float *__restrict__ tgt, float *__restrict__ out,
+                                               float *__restrict__ dh, double *scal, double *partials,
+                                               unsigned *counter) {
+  double d[1] = {0.};
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    float v = wo[H];
+    for (int f = 0; f < H; ++f) v = fmaf((float)h[(size_t)i * H + f], wo[f], v);
+    out[i] = v;
+    if (tgt) {
+      const float diff = v - tgt[i];
+      d[0] += (double)diff * diff;
+      const float dout = 2.f * diff / (float)n;
+      for (int f = 0; f < H; ++f) dh[(size_t)i * H + f] = dout * wo[f];
+    }
+  }
+  if (!tgt) return;
+  if (reduceLast<1>(d, partials, counter) && threadIdx.x == 0) scal[1] = d[0] / (double)n;
+}
+
+// grad of w_out / b_out: per-block partials of h^T dout (dout recomputed)
+__global__ __launch_bounds__(256) void kGnnOutGrad(int n, const __bf16 *__restrict__ h, const float *__restrict__ out,
+                                                   const float *__restrict__ tgt, float *__restrict__ partials) {
+  __shared__ float red[4][H + 1];
+  const int f = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int rowsPer = (n + gridDim.x - 1) / gridDim.x;
+  const int r0 = blockIdx.x * rowsPer, r1 = min(n, r0 + rowsPer);
+  float acc = 0.f, accb = 0.f;
+  for (int i = r0 + q; i < r1; i += 4) {
+    const float dout = 2.f * (out[i] - tgt[i]) / (float)n;
+    acc = fmaf((float)h[(size_t)i * H + f], dout, acc);
+    if (f == 0) accb += dout;
+  }
+  red[q][f] = acc;
+  if (f == 0) red[q][H] = accb;
+  __syncthreads();
+  if (threadIdx.x < H + 1) {
+    const int k = threadIdx.x;
+    partials[blockIdx.x * (H + 1) + k] = red[0][k] + red[1][k] + red[2][k] + red[3][k];
+  }
+}
+
+// ---------------------------------------------------------------- backward, data part
+// dpre = dh * (pre > 0) ; dSA[r][k] = sum_c dpre[r][c] W[k][c]  (k < 128)
+__global__ __launch_bounds__(256) void kGnnBwdData(int n, const float *__restrict__ dh, const __bf16 *__restrict__ pre,
+                                                   const __bf16 *__restrict__ wN, __bf16 *__restrict__ dpre,
+                                                   float *__restrict__ dSA) {
+  __shared__ __attribute__((aligned(16))) __bf16 W[128][H + 8];
+  for (int t = threadIdx.x; t < 128 * 8; t += 256) {
+    const int k = t / 8, c8 = (t % 8) * 8;
+    *(bf16x8 *)&W[k][c8] = *(const bf16x8 *)&wN[k * H + c8];
+  }
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int rowBase = blockIdx.x * kWgRows + wave * kRowsPerWave;
+  if (rowBase >= n) return;
+  const int r = rowBase + (lane & 15), gq = lane >> 4;
+  const bool act = r < n;
+  bf16x8 a[2];
+#pragma unroll
+  for (int kc = 0; kc < 2; ++kc) {
+    const int f0 = 32 * kc + 8 * gq;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      float v = 0.f;
+      if (act) {
+        const size_t idx = (size_t)r * H + f0 + j;
+        v = (float)pre[idx] > 0.f ? dh[idx] : 0.f;
+      }
+      a[kc][j] = (__bf16)v;
+    }
+    if (act) *(bf16x8 *)&dpre[(size_t)r * H + f0] = a[kc];
+  }
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int k = 16 * t + (lane & 15);
+#pragma unroll
+    for (int kc = 0; kc < 2; ++kc) acc = mfma(a[kc], *(const bf16x8 *)&W[k][32 * kc + 8 * gq], acc);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = rowBase + 4 * gq + v;
+      if (row < n) dSA[(size_t)row * 128 + k] = acc[v];
+    }
+  }
+}
+
+// ---------------------------------------------------------------- backward, weight part
+// Per workgroup partial of dWsm[k][c] = sum_r [h | agg][r][k] dpre[r][c] over
+// kChunkRows rows (MFMA with K = rows), and of db[c] = sum_r dpre[r][c].
+// Wave w owns k in [32w, 32w+32) (two 16-row tiles) x 4 column tiles.
+__global__ __launch_bounds__(256) void kGnnBwdWeights(int n, const __bf16 *__restrict__ h,
+                                                      const __bf16 *__restrict__ agg,
+                                                      const __bf16 *__restrict__ dpre, float *__restrict__ partials) {
+  __shared__ __attribute__((aligned(16))) __bf16 HA[128][32 + 8];  // [feature][row]
+  __shared__ __attribute__((aligned(16))) __bf16 DP[H][32 + 8];    // [col][row]
+  __shared__ float dbs[4][H];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, gq = lane >> 4;
+  const int r0 = blockIdx.x * kChunkRows, r1 = min(n, r0 + kChunkRows);
+  f32x4 acc[2][4];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[a][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  float dbAcc = 0.f;
+  for (int base = r0; base < r1; base += 32) {
+    __syncthreads();
+    // stage 32 rows, transposed
+    for (int t = threadIdx.x; t < 32 * 128; t += 256) {
+      const int rr = t / 128, k = t % 128, row = base + rr;
+      __bf16 v = (__bf16)0.f;
+      if (row < r1) v = k < H ? h[(size_t)row * H + k] : agg[(size_t)row * H + (k - H)];
+      HA[k][rr] = v;
+    }
+    for (int t = threadIdx.x; t < 32 * H; t += 256) {
+      const int rr = t / H, cc = t % H, row = base + rr;
+      DP[cc][rr] = row < r1 ? dpre[(size_t)row * H + cc] : (__bf16)0.f;
+    }
+    __syncthreads();
+    {
+      // column sums of dpre for db: thread (q, c) sums 8 rows
+      const int cc = threadIdx.x & 63, q = threadIdx.x >> 6;
+      for (int rr = q * 8; rr < q * 8 + 8; ++rr) dbAcc += (float)DP[cc][rr];
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      const int k = 32 * wave + 16 * a + (lane & 15);
+      const bf16x8 av = *(const bf16x8 *)&HA[k][8 * gq];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int cc = 16 * t + (lane & 15);
+        acc[a][t] = mfma(av, *(const bf16x8 *)&DP[cc][8 * gq], acc[a][t]);
+      }
+    }
+  }
+  // partial layout per block: [128][64] dW then [64] db
+  float *P = partials + (size_t)blockIdx.x * (128 * H + H);
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int k = 32 * wave + 16 * a + 4 * gq + v;
+        const int cc = 16 * t + (lane & 15);
+        P[k * H + cc] = acc[a][t][v];
+      }
+  __syncthreads();
+  dbs[threadIdx.x >> 6][threadIdx.x & 63] = dbAcc;
+  __syncthreads();
+  if (threadIdx.x < H) P[128 * H + threadIdx.x] = dbs[0][threadIdx.x] + dbs[1][threadIdx.x] + dbs[2][threadIdx.x] + dbs[3][threadIdx.x];
+}
+
+// Sum per-block partials (fixed order) into grad[dst .. dst+m)
+__global__ void kSumPartials(int nBlocks, int m, const float *__restrict__ partials, float *__restrict__ grad) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x) {
+    float s = 0.f;
+    for (int b = 0; b < nBlocks; ++b) s += partials[(size_t)b * m + k];
+    grad[k] = s;
+  }
+}
+
+// As kSumPartials with the record split: k < split -> dst1[k], else dst2[k - split]
+__global__ void kSumPartialsStrided(int nBlocks, int m, const float *__restrict__ partials, float *__restrict__ dst1,
+                                    int split, float *__restrict__ dst2) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x) {
+    float s = 0.f;
+    for (int b = 0; b < nBlocks; ++b) s += partials[(size_t)b * m + k];
+    if (k < split) dst1[k] = s;
+    else dst2[k - split] = s;
+  }
+}
+
+// dWg[k][c] = g[k] * db[c]; gterm[k] = sum_c Wg[k][c] db[c] / n
+__global__ void kGnnGlobalGrad(int n, const float *__restrict__ g, const float *__restrict__ db,
+                               const float *__restrict__ Wg, float *__restrict__ dWg, float *__restrict__ gterm) {
+  const int k = threadIdx.x;
+  if (k >= H) return;
+  float s = 0.f;
+  for (int c = 0; c < H; ++c) {
+    dWg[k * H + c] = g[k] * db[c];
+    s = fmaf(Wg[k * H + c], db[c], s);
+  }
+  gterm[k] = s / (float)n;
+}
+
+// dh_prev = dh + dself + A_hat^T dagg + gterm   (A_hat symmetric)
+template <class T>
+__global__ void kGnnBwdCombine(int n, float *__restrict__ dh, const float *__restrict__ dSA,
+                               const int *__restrict__ sliceOff, const int *__restrict__ col, const T *__restrict__ val,
+                               const float *__restrict__ sd, const float *__restrict__ gterm) {
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < (long)n * H; t += (long)gridDim.x * blockDim.x) {
+    const int r = (int)(t / H), f = (int)(t % H);
+    float v = dh[t] + dSA[(size_t)r * 128 + f] + gterm[f];
+    const int s = r >> 6, l = r & 63;
+    const int o = sliceOff[s], K = (sliceOff[s + 1] - o) >> 6;
+    for (int e = 0; e < K; ++e) {
+      int cc;
+      const float w = ahat(col, val, sd, o + (e << 6) + l, r, cc);
+      if (w != 0.f) v = fmaf(w, dSA[(size_t)cc * 128 + H + f], v);
+    }
+    dh[t] = v;
+  }
+}
+
+// input layer gradient partials: dW_in[k][c] = sum_r X[r][k] dpre0[r][c], db_in[c]
+__global__ __launch_bounds__(256) void kGnnInGrad(int n, const float *__restrict__ X, const __bf16 *__restrict__ pre0,
+                                                  const float *__restrict__ dh, float *__restrict__ partials) {
+  __shared__ float red[4][(kGnnDin + 1) * H];
+  const int c = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int rowsPer = (n + gridDim.x - 1) / gridDim.x;
+  const int r0 = blockIdx.x * rowsPer, r1 = min(n, r0 + rowsPer);
+  float acc[kGnnDin + 1] = {0, 0, 0, 0, 0};
+  for (int i = r0 + q; i < r1; i += 4) {
+    const size_t idx = (size_t)i * H + c;
+    const float dp = (float)pre0[idx] > 0.f ? dh[idx] : 0.f;
+#pragma unroll
+    for (int k = 0; k < kGnnDin; ++k) acc[k] = fmaf(X[4 * i + k], dp, acc[k]);
+    acc[kGnnDin] += dp;
+  }
+#pragma unroll
+  for (int k = 0; k <= kGnnDin; ++k) red[q][k * H + c] = acc[k];
+  __syncthreads();
+  for (int t = threadIdx.x; t < (kGnnDin + 1) * H; t += 256)
+    partials[(size_t)blockIdx.x * (kGnnDin + 1) * H + t] = red[0][t] + red[1][t] + red[2][t] + red[3][t];
+}
+
+// Adam (Kingma & Ba 2015), then refresh the BF16 weight copies
+__global__ void kAdam(int m, float *__restrict__ p, const float *__restrict__ g, float *__restrict__ mo,
+                      float *__restrict__ ve, float lr, float b1, float b2, float eps, float c1, float c2,
+                      float clip) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
+    float gi = g[i];
+    gi = fminf(fmaxf(gi, -clip), clip);
+    const float mi = b1 * mo[i] + (1.f - b1) * gi;
+    const float vi = b2 * ve[i] + (1.f - b2) * gi * gi;
+    mo[i] = mi;
+    ve[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  }
+}
+
+// wT[l][o][k] = Wsm[l][k][o] ; wN[l][k][o] = Wsm[l][k][o]  (BF16)
+__global__ void kGnnCast(int L, const float *__restrict__ P, size_t layer0, size_t layerSize, __bf16 *__restrict__ wT,
+                         __bf16 *__restrict__ wN) {
+  const int per = 128 * H;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < L * per; t += gridDim.x * blockDim.x) {
+    const int l = t / per, kk = (t % per) / H, o = t % H;
+    const float v = P[layer0 + (size_t)l * layerSize + (size_t)kk * H + o];
+    wT[(size_t)l * per + (size_t)o * 128 + kk] = (__bf16)v;
+    wN[(size_t)l * per + (size_t)kk * H + o] = (__bf16)v;
+  }
+}
+
+__global__ void kGnnScaleOut(int n, const float *__restrict__ out, const float *__restrict__ sd,
+                             const double *__restrict__ scal, double *__restrict__ e) {
+  const double s = scal[0];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    e[i] = (double)out[i] * s / (double)sd[i];
+}
+
+// ---------------------------------------------------------------- host side
+inline int g1(long n, int cap = 8192) { return (int)std::max<long>(1, std::min<long>((n + 255) / 256, cap)); }
+
+void ensureNet(phase_hip_solver *s) {
+  if (!s->gnn) s->gnn = new GnnState();
+  GnnState &G = *s->gnn;
+  if (!G.initialised) {
+    const size_t P = G.lay.total();
+    G.params.alloc(P);
+    G.grad.alloc(P);
+    G.adamM.alloc(P);
+    G.adamV.alloc(P);
+    // He-style initialisation from a fixed seed (deterministic)
+    std::mt19937 rng(1234);
+    std::normal_distribution<float> nd(0.f, 1.f);
+    std::vector<float> h(P, 0.f);
+    for (size_t i = 0; i < (size_t)kGnnDin * H; ++i) h[G.lay.inW() + i] = nd(rng) * std::sqrt(2.f / kGnnDin);
+    for (int l = 0; l < G.lay.L; ++l) {
+      for (size_t i = 0; i < (size_t)128 * H; ++i) h[G.lay.wsm(l) + i] = nd(rng) * (0.5f / std::sqrt((float)H));
+      for (size_t i = 0; i < (size_t)H * H; ++i) h[G.lay.wg(l) + i] = nd(rng) * (0.1f / std::sqrt((float)H));
+    }
+    for (int i = 0; i < H; ++i) h[G.lay.outW() + i] = nd(rng) * (1.f / std::sqrt((float)H));
+    G.params.upload(h, s->stream);
+    G.adamM.zero(s->stream);
+    G.adamV.zero(s->stream);
+    G.step = 0;
+    G.initialised = true;
+  }
+  const int n = s->rank;
+  if (G.n != n) {
+    const int L = G.lay.L;
+    G.n = n;
+    G.sd.alloc(n);
+    G.rowsum.alloc(n);
+    G.X.alloc(4 * (size_t)n);
+    G.out.alloc(n);
+    G.dOut.alloc(n);
+    G.h.alloc((size_t)(L + 1) * n * H);
+    G.agg.alloc((size_t)L * n * H);
+    G.pre.alloc((size_t)L * n * H);
+    G.pre0.alloc((size_t)n * H);
+    G.dpre.alloc((size_t)n * H);
+    G.dh.alloc((size_t)n * H);
+    G.dSA.alloc((size_t)n * 128);
+    G.gmean.alloc((size_t)L * H);
+    G.cvec.alloc((size_t)L * H);
+    const int nbW = (n + kChunkRows - 1) / kChunkRows;
+    G.partials.alloc(std::max<size_t>((size_t)nbW * (128 * H + H), (size_t)2048 * (kGnnDin + 1) * H));
+    G.scal.alloc(4);
+    G.counter.alloc(1);
+    G.counter.zero(s->stream);
+    G.wT.alloc((size_t)L * 128 * H);
+    G.wN.alloc((size_t)L * 128 * H);
+  }
+}
+
+// Graph quantities of the current matrix (FP32 SELL copy): sd, row-sum ratio.
+void prepareGraph(phase_hip_solver *s) {
+  GnnState &G = *s->gnn;
+  const int n = s->rank;
+  hipLaunchKernelGGL(kGnnDiag, dim3(g1(n)), dim3(256), 0, s->stream, n, s->dSliceOff.p, s->dSellCol.p, s->dSell32.p,
+                     G.dOut.p, G.rowsum.p);
+  hipLaunchKernelGGL(kGnnScale, dim3(g1(n)), dim3(256), 0, s->stream, n, G.dOut.p, G.rowsum.p, G.sd.p, G.rowsum.p);
+}
+
+void castWeights(phase_hip_solver *s) {
+  GnnState &G = *s->gnn;
+  hipLaunchKernelGGL(kGnnCast, dim3(g1((long)G.lay.L * 128 * H)), dim3(256), 0, s->stream, G.lay.L, G.params.p,
+                     G.lay.layer(0), GnnLayout::layerSize(), G.wT.p, G.wN.p);
+}
+
+__bf16 *hL(GnnState &G, int l) { return G.h.p + (size_t)l * G.n * H; }
+
+// features of residual r (FP32, solver order) with z = ILU32^{-1} r supplied
+void gnnFeatures(phase_hip_solver *s, const float *r, const float *z) {
+  GnnState &G = *s->gnn;
+  const int n = s->rank;
+  hipLaunchKernelGGL(kGnnRms, dim3(gridFor1(n)), dim3(kRedBlock), 0, s->stream, n, r, G.sd.p, G.scal.p, s->dPartials.p,
+                     s->dCounter.p);
+  hipLaunchKernelGGL(kGnnFeatures, dim3(g1(n)), dim3(256), 0, s->stream, n, r, z, G.sd.p, G.rowsum.p, G.scal.p, G.X.p);
+}
+
+}  // namespace
+
+namespace phase {
+
+void gnnForward(phase_hip_solver *s, const float *target) {
+  GnnState &G = *s->gnn;
+  const int n = s->rank, L = G.lay.L;
+  const float *P = G.params.p;
+  hipLaunchKernelGGL(kGnnIn, dim3(g1((long)n * H)), dim3(256), 0, s->stream, n, G.X.p, P + G.lay.inW(), G.pre0.p,
+                     hL(G, 0));
+  const int nwg = (n + kWgRows - 1) / kWgRows;
+  const int ncs = std::min(1024, std::max(1, (n + 255) / 256));
+  for (int l = 0; l < L; ++l) {
+    hipLaunchKernelGGL((kColSum<__bf16>), dim3(ncs), dim3(256), 0, s->stream, n, (const __bf16 *)hL(G, l), 1.f / n,
+                       G.gmean.p + l * H, G.partials.p, G.counter.p);
+    hipLaunchKernelGGL(kGnnFold, dim3(1), dim3(64), 0, s->stream, G.gmean.p + l * H, P + G.lay.wg(l), P + G.lay.b(l),
+                       G.cvec.p + l * H);
+    hipLaunchKernelGGL((kGnnLayerFwd<float>), dim3(nwg), dim3(256), 0, s->stream, n, (const __bf16 *)hL(G, l),
+                       s->dSliceOff.p, s->dSellCol.p, s->dSell32.p, G.sd.p, G.wT.p + (size_t)l * 128 * H,
+                       G.cvec.p + l * H, G.agg.p + (size_t)l * n * H, G.pre.p + (size_t)l * n * H, hL(G, l + 1));
+  }
+  hipLaunchKernelGGL(kGnnOut, dim3(gridFor1(n)), dim3(256), 0, s->stream, n, (const __bf16 *)hL(G, L),
+                     P + G.lay.outW(), target, G.out.p, G.dh.p, G.scal.p, s->dPartials.p, s->dCounter.p);
+  PH_CHECK(hipGetLastError());
+}
+
+// Gradient of the loss of the last forward (with target) into G.grad.
+void gnnBackward(phase_hip_solver *s, const float *target) {
+  GnnState &G = *s->gnn;
+  const int n = s->rank, L = G.lay.L;
+  const float *P = G.params.p;
+  const int nbO = std::min(1024, std::max(1, (n + 1023) / 1024));
+  hipLaunchKernelGGL(kGnnOutGrad, dim3(nbO), dim3(256), 0, s->stream, n, (const __bf16 *)hL(G, L), G.out.p, target,
+                     G.partials.p);
+  hipLaunchKernelGGL(kSumPartials, dim3(1), dim3(128), 0, s->stream, nbO, H + 1, G.partials.p, G.grad.p + G.lay.outW());
+  const int nwg = (n + kWgRows - 1) / kWgRows;
+  const int nbW = (n + kChunkRows - 1) / kChunkRows;
+  for (int l = L - 1; l >= 0; --l) {
+    hipLaunchKernelGGL(kGnnBwdData, dim3(nwg), dim3(256), 0, s->stream, n, G.dh.p, G.pre.p + (size_t)l * n * H,
+                       G.wN.p + (size_t)l * 128 * H, G.dpre.p, G.dSA.p);
+    hipLaunchKernelGGL(kGnnBwdWeights, dim3(nbW), dim3(256), 0, s->stream, n, (const __bf16 *)hL(G, l),
+                       (const __bf16 *)(G.agg.p + (size_t)l * n * H), (const __bf16 *)G.dpre.p, G.partials.p);
+    // dWsm (128*64) and db (64) are contiguous in the partial record but
+    // separated by Wg in the parameter layout: sum in two pieces
+    hipLaunchKernelGGL(kSumPartialsStrided, dim3(g1(128 * H + H)), dim3(256), 0, s->stream, nbW, 128 * H + H,
+                       G.partials.p, G.grad.p + G.lay.wsm(l), 128 * H, G.grad.p + G.lay.b(l));
+    hipLaunchKernelGGL(kGnnGlobalGrad, dim3(1), dim3(64), 0, s->stream, n, G.gmean.p + l * H, G.grad.p + G.lay.b(l),
+                       P + G.lay.wg(l), G.grad.p + G.lay.wg(l), G.cvec.p + l * H);
+    hipLaunchKernelGGL((kGnnBwdCombine<float>), dim3(g1((long)n * H)), dim3(256), 0, s->stream, n, G.dh.p, G.dSA.p,
+                       s->dSliceOff.p, s->dSellCol.p, s->dSell32.p, G.sd.p, G.cvec.p + l * H);
+  }
+  const int nbI = std::min(2048, std::max(1, (n + 255) / 256));
+  hipLaunchKernelGGL(kGnnInGrad, dim3(nbI), dim3(256), 0, s->stream, n, G.X.p, G.pre0.p, G.dh.p, G.partials.p);
+  hipLaunchKernelGGL(kSumPartials, dim3(g1((kGnnDin + 1) * H)), dim3(256), 0, s->stream, nbI, (kGnnDin + 1) * H,
+                     G.partials.p, G.grad.p + G.lay.inW());
+  PH_CHECK(hipGetLastError());
+}
+
+}  // namespace phase
+
+namespace {
+
+void gnnTrainStep(phase_hip_solver *s, int pair, float lr) {
+  GnnState &G = *s->gnn;
+  const int n = s->rank;
+  const float *r = G.pairR.p + (size_t)pair * n;
+  const float *e = G.pairE.p + (size_t)pair * n;
+  KrylovVecs<float> &K = s->k32;
+  // z = ILU(0)^{-1} r (FP32 factor)
+  {
+    unsigned long long base;
+    int ep;
+    base = s->ticketBase;
+    s->ticketBase += (unsigned long long)s->ord.nBlocks;
+    ep = ++s->epoch;
+    IluDev<float> L;
+    L.blocks = s->dBlocks.p; L.rows = s->dRows.p; L.pred = s->dPred.p; L.succ = s->dSucc.p;
+    L.lCol = s->dLCol.p; L.uCol = s->dUCol.p; L.fac = s->dFac32.p; L.nb = s->ord.nBlocks;
+    L.nLSlots = s->ord.nLSlots; L.nUSlots = s->ord.nUSlots; L.n = n;
+    SyncDev sy{s->dFlags.p, s->dTicket.p, s->dErr.p};
+    hipLaunchKernelGGL((kLSolve<float, kProNone>), dim3(s->ord.nBlocks), dim3(64), 0, s->stream, L, sy, base, ep,
+                       (const Scalars *)nullptr, r, K.y.p, (float *)nullptr, (const float *)nullptr,
+                       (const float *)nullptr);
+    base = s->ticketBase;
+    s->ticketBase += (unsigned long long)s->ord.nBlocks;
+    ep = ++s->epoch;
+    hipLaunchKernelGGL((kUSolve<float>), dim3(s->ord.nBlocks), dim3(64), 0, s->stream, L, sy, base, ep,
+                       (const Scalars *)nullptr, K.y.p, K.sh.p);
+  }
+  gnnFeatures(s, r, K.sh.p);
+  hipLaunchKernelGGL(kGnnTarget, dim3(g1(n)), dim3(256), 0, s->stream, n, e, G.sd.p, G.scal.p, G.dOut.p);
+  gnnForward(s, G.dOut.p);
+  gnnBackward(s, G.dOut.p);
+  ++G.step;
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float c1 = 1.f - std::pow(b1, (float)G.step), c2 = 1.f - std::pow(b2, (float)G.step);
+  const int P = (int)G.lay.total();
+  hipLaunchKernelGGL(kAdam, dim3(g1(P)), dim3(256), 0, s->stream, P, G.params.p, G.grad.p, G.adamM.p, G.adamV.p, lr,
+                     b1, b2, 1e-8f, c1, c2, 10.f);
+  castWeights(s);
+}
+
+// r64 (solver order, device) -> e64 network correction (solver order, device)
+void applyNet(phase_hip_solver *s, const double *r64, double *e64) {
+  GnnState &G = *s->gnn;
+  const int n = s->rank;
+  KrylovVecs<float> &K = s->k32;
+  hipLaunchKernelGGL((kConvert<double, float>), dim3(g1(n)), dim3(256), 0, s->stream, n, r64, K.t.p);
+  unsigned long long base = s->ticketBase;
+  s->ticketBase += (unsigned long long)s->ord.nBlocks;
+  int ep = ++s->epoch;
+  IluDev<float> L;
+  L.blocks = s->dBlocks.p; L.rows = s->dRows.p; L.pred = s->dPred.p; L.succ = s->dSucc.p;
+  L.lCol = s->dLCol.p; L.uCol = s->dUCol.p; L.fac = s->dFac32.p; L.nb = s->ord.nBlocks;
+  L.nLSlots = s->ord.nLSlots; L.nUSlots = s->ord.nUSlots; L.n = n;
+  SyncDev sy{s->dFlags.p, s->dTicket.p, s->dErr.p};
+  hipLaunchKernelGGL((kLSolve<float, kProNone>), dim3(s->ord.nBlocks), dim3(64), 0, s->stream, L, sy, base, ep,
+                     (const Scalars *)nullptr, (const float *)K.t.p, K.y.p, (float *)nullptr, (const float *)nullptr,
+                     (const float *)nullptr);
+  base = s->ticketBase;
+  s->ticketBase += (unsigned long long)s->ord.nBlocks;
+  ep = ++s->epoch;
+  hipLaunchKernelGGL((kUSolve<float>), dim3(s->ord.nBlocks), dim3(64), 0, s->stream, L, sy, base, ep,
+                     (const Scalars *)nullptr, K.y.p, K.sh.p);
+  gnnFeatures(s, K.t.p, K.sh.p);
+  gnnForward(s, nullptr);
+  hipLaunchKernelGGL(kGnnScaleOut, dim3(g1(n)), dim3(256), 0, s->stream, n, G.out.p, G.sd.p, G.scal.p, e64);
+  PH_CHECK(hipGetLastError());
+}
+
+template <class F>
+int guardedG(phase_hip_solver *s, F &&f) {
+  if (!s) return PHASE_HIP_EINVAL;
+  try {
+    f();
+    return PHASE_HIP_OK;
+  } catch (const HipError &e) {
+    s->err = e.msg;
+    return e.code;
+  } catch (const std::exception &e) {
+    s->err = e.what();
+    return PHASE_HIP_EINVAL;
+  }
+}
+
+void requireSystem(phase_hip_solver *s) {
+  if (!s->havePattern || !s->haveValues) fail(PHASE_HIP_ESTATE, "no system: call set() or assemble first");
+}
+
+}  // namespace
+
+namespace phase {
+
+void gnnFree(phase_hip_solver *s) {
+  delete s->gnn;
+  s->gnn = nullptr;
+}
+
+void gnnPrepare(phase_hip_solver *s) {
+  solverEnsureSingle(s);
+  ensureNet(s);
+  prepareGraph(s);
+  castWeights(s);
+}
+
+void gnnApply(phase_hip_solver *s, const double *r64, double *e64) { applyNet(s, r64, e64); }
+
+// Capture hooks (FP64 solves): snapshot (r, x) every `captureEvery`
+// iterations; the correction e = x_final - x is known when the solve ends.
+void gnnCaptureIter(phase_hip_solver *s, const double *r, const double *x, int iter) {
+  GnnState *G = s->gnn;
+  if (!G || G->captureEvery <= 0) return;
+  if (G->nPairs + (int)G->pendingSnap.size() >= G->maxPairs) return;
+  const int n = s->rank;
+  const int slot = G->nPairs + (int)G->pendingSnap.size();
+  const int k = (int)G->pendingSnap.size();
+  if ((size_t)(k + 1) * n > G->snapX.n) {
+    // grow the snapshot store (device-to-device copy of what it holds)
+    DBuf<double> nb;
+    nb.alloc((size_t)(k + 8) * n);
+    if (k) PH_CHECK(hipMemcpyAsync(nb.p, G->snapX.p, (size_t)k * n * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+    G->snapX = std::move(nb);
+  }
+  hipLaunchKernelGGL((kConvert<double, float>), dim3(g1(n)), dim3(256), 0, s->stream, n, r, G->pairR.p + (size_t)slot * n);
+  PH_CHECK(hipMemcpyAsync(G->snapX.p + (size_t)k * n, x, n * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+  G->pendingSnap.push_back(slot);
+  (void)iter;
+}
+
+__global__ void kPairE(int n, const double *__restrict__ xf, const double *__restrict__ xk, float *__restrict__ e) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) e[i] = (float)(xf[i] - xk[i]);
+}
+
+void gnnCaptureFinish(phase_hip_solver *s, const double *xFinal, bool converged) {
+  GnnState *G = s->gnn;
+  if (!G || G->captureEvery <= 0) return;
+  const int n = s->rank;
+  if (converged) {
+    for (size_t k = 0; k < G->pendingSnap.size(); ++k)
+      hipLaunchKernelGGL(kPairE, dim3(g1(n)), dim3(256), 0, s->stream, n, xFinal, G->snapX.p + k * n,
+                         G->pairE.p + (size_t)G->pendingSnap[k] * n);
+    G->nPairs += (int)G->pendingSnap.size();
+  }
+  G->pendingSnap.clear();
+}
+
+}  // namespace phase
+
+extern "C" {
+
+int phase_hip_learned_capture(phase_hip_solver *s, int everyIters, int maxPairs) {
+  return guardedG(s, [&] {
+    requireSystem(s);
+    if (everyIters < 0 || maxPairs < 0) fail(PHASE_HIP_EINVAL, "negative capture settings");
+    PH_CHECK(hipSetDevice(s->device));
+    ensureNet(s);
+    GnnState &G = *s->gnn;
+    G.captureEvery = everyIters;
+    if (maxPairs != G.maxPairs) {
+      G.maxPairs = maxPairs;
+      G.pairR.alloc((size_t)std::max(1, maxPairs) * s->rank);
+      G.pairE.alloc((size_t)std::max(1, maxPairs) * s->rank);
+      G.nPairs = 0;
+    }
+    G.pendingSnap.clear();
+  });
+}
+
+int phase_hip_learned_n_pairs(const phase_hip_solver *s, int *count) {
+  if (!s || !count) return PHASE_HIP_EINVAL;
+  *count = s->gnn ? s->gnn->nPairs : 0;
+  return PHASE_HIP_OK;
+}
+
+int phase_hip_learned_train(phase_hip_solver *s, int steps, double lr, int seed, double *lastLoss) {
+  return guardedG(s, [&] {
+    requireSystem(s);
+    PH_CHECK(hipSetDevice(s->device));
+    if (!s->gnn || s->gnn->nPairs == 0) fail(PHASE_HIP_ESTATE, "no captured (residual, correction) pairs");
+    gnnPrepare(s);
+    GnnState &G = *s->gnn;
+    std::mt19937 rng((unsigned)seed);
+    double loss = 0.;
+    for (int t = 0; t < steps; ++t) {
+      gnnTrainStep(s, (int)(rng() % (unsigned)G.nPairs), (float)lr);
+      if (t + 1 == steps) {
+        PH_CHECK(hipMemcpyAsync(&loss, G.scal.p + 1, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+      }
+    }
+    PH_CHECK(hipStreamSynchronize(s->stream));
+    G.trained = true;
+    if (lastLoss) *lastLoss = loss;
+  });
+}
+
+int phase_hip_learned_params(phase_hip_solver *s, float *params, int *count) {
+  return guardedG(s, [&] {
+    PH_CHECK(hipSetDevice(s->device));
+    if (!s->gnn) s->gnn = new GnnState();
+    GnnState &G = *s->gnn;
+    if (count) *count = (int)G.lay.total();
+    if (!params) return;
+    if (!G.initialised) {
+      if (!s->havePattern) fail(PHASE_HIP_ESTATE, "network not initialised");
+      ensureNet(s);
+    }
+    PH_CHECK(hipMemcpyAsync(params, G.params.p, G.lay.total() * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    PH_CHECK(hipStreamSynchronize(s->stream));
+  });
+}
+
+int phase_hip_learned_set_params(phase_hip_solver *s, const float *params, int count) {
+  return guardedG(s, [&] {
+    requireSystem(s);
+    PH_CHECK(hipSetDevice(s->device));
+    ensureNet(s);
+    GnnState &G = *s->gnn;
+    if (count != (int)G.lay.total()) fail(PHASE_HIP_EINVAL, "parameter count mismatch");
+    PH_CHECK(hipMemcpyAsync(G.params.p, params, count * sizeof(float), hipMemcpyHostToDevice, s->stream));
+    G.adamM.zero(s->stream);
+    G.adamV.zero(s->stream);
+    G.step = 0;
+    G.trained = true;
+    PH_CHECK(hipStreamSynchronize(s->stream));
+  });
+}
+
+// Forward + backward on a given residual r, ILU-preconditioned residual z and
+// correction target e (original numbering): network output (original
+// numbering), loss and flat parameter gradient.  For gradient checks.
+int phase_hip_learned_eval(phase_hip_solver *s, const double *r, const double *z, const double *target, double *out,
+                           double *loss, float *grad) {
+  return guardedG(s, [&] {
+    requireSystem(s);
+    PH_CHECK(hipSetDevice(s->device));
+    gnnPrepare(s);
+    GnnState &G = *s->gnn;
+    const int n = s->rank;
+    KrylovVecs<float> &K = s->k32;
+    K.alloc(n);
+    DBuf<double> tmp;
+    tmp.alloc(3 * (size_t)n);
+    PH_CHECK(hipMemcpyAsync(tmp.p, r, n * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    PH_CHECK(hipMemcpyAsync(tmp.p + n, z, n * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    PH_CHECK(hipMemcpyAsync(tmp.p + 2 * n, target, n * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    hipLaunchKernelGGL((kPermuteIn<double, float>), dim3(g1(n)), dim3(256), 0, s->stream, n, s->dPerm.p, tmp.p, K.t.p, 1.0);
+    hipLaunchKernelGGL((kPermuteIn<double, float>), dim3(g1(n)), dim3(256), 0, s->stream, n, s->dPerm.p, tmp.p + n,
+                       K.sh.p, 1.0);
+    hipLaunchKernelGGL((kPermuteIn<double, float>), dim3(g1(n)), dim3(256), 0, s->stream, n, s->dPerm.p, tmp.p + 2 * n,
+                       K.s.p, 1.0);
+    gnnFeatures(s, K.t.p, K.sh.p);
+    gnnForward(s, K.s.p);
+    gnnBackward(s, K.s.p);
+    hipLaunchKernelGGL((kConvert<float, double>), dim3(g1(n)), dim3(256), 0, s->stream, n, G.out.p, tmp.p);
+    hipLaunchKernelGGL((kPermuteOut<double>), dim3(g1(n)), dim3(256), 0, s->stream, n, s->dPerm.p, tmp.p, tmp.p + n);
+    PH_CHECK(hipMemcpyAsync(out, tmp.p + n, n * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    PH_CHECK(hipMemcpyAsync(loss, G.scal.p + 1, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    PH_CHECK(hipMemcpyAsync(grad, G.grad.p, G.lay.total() * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    PH_CHECK(hipStreamSynchronize(s->stream));
+  });
+}
+
+// e = network correction of residual r (original numbering, FP64 in and out)
+int phase_hip_learned_apply(phase_hip_solver *s, const double *r, double *e) {
+  return guardedG(s, [&] {
+    requireSystem(s);
+    PH_CHECK(hipSetDevice(s->device));
+    gnnPrepare(s);
+    const int n = s->rank;
+    DBuf<double> tmp;
+    tmp.alloc(3 * (size_t)n);
+    PH_CHECK(hipMemcpyAsync(tmp.p, r, n * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    hipLaunchKernelGGL((kPermuteIn<double, double>), dim3(g1(n)), dim3(256), 0, s->stream, n, s->dPerm.p, tmp.p,
+                       tmp.p + n, 1.0);
+    applyNet(s, tmp.p + n, tmp.p + 2 * n);
+    hipLaunchKernelGGL((kPermuteOut<double>), dim3(g1(n)), dim3(256), 0, s->stream, n, s->dPerm.p, tmp.p + 2 * n, tmp.p);
+    PH_CHECK(hipMemcpyAsync(e, tmp.p, n * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    PH_CHECK(hipStreamSynchronize(s->stream));
+  });
+}
+
+}  // extern "C"

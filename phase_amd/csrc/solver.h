@@ -19,6 +19,7 @@ struct KrylovVecs {
 };
 
 struct MeshBinding;  // mesh.hip: link -> slot maps of an assembled operator
+struct GnnState;     // gnn.hip: learned correction network
 
 }  // namespace phase
 
@@ -73,8 +74,13 @@ struct phase_hip_solver {
   phase::DBuf<double> dMisc;        // [0] ||b||^2
 
   // results of the last solve
-  int nIters = 0, innerIters = 0, outerPasses = 0, fallbacks = 0;
+  int nIters = 0, innerIters = 0, outerPasses = 0, fallbacks = 0, learnedSteps = 0;
   double error = 0.0, solveMs = 0.0;
+
+  // learned correction (gnn.hip) and its FP64 work vectors
+  phase::GnnState *gnn = nullptr;
+  phase::DBuf<double> dE64, dQ64;
+  int lastCaptureIter = 0;
 
   // mesh-assembled operator (mesh.hip)
   phase::MeshBinding *binding = nullptr;
@@ -91,4 +97,6 @@ void solverValuesChanged(phase_hip_solver *s);
 void solverFreeBinding(phase_hip_solver *s);
 // Raise a boundary-type error flagged by the last device assembly, if any.
 void checkAssembly(phase_hip_solver *s);
+// FP32 copies of A and the FP32 ILU(0) factor are current.
+void solverEnsureSingle(phase_hip_solver *s);
 }  // namespace phase

@@ -11,6 +11,7 @@
 #include <string>
 #include <type_traits>
 
+#include "gnn.h"
 #include "solver.h"
 #include "solver_kernels.h"
 
@@ -109,6 +110,14 @@ void ensureFactor64(phase_hip_solver *s) {
   s->fac64Valid = true;
 }
 
+}  // namespace
+
+namespace phase {
+void solverEnsureSingle(phase_hip_solver *s);
+}
+
+namespace {
+
 void ensureSingle(phase_hip_solver *s) {
   if (!s->sell32Valid) {
     s->dSell32.alloc(s->dSell64.n);
@@ -189,8 +198,10 @@ double normB2(phase_hip_solver *s) {
 }
 
 // Run BiCGStab chunks until the device says done; returns iterations taken.
-template <class T>
-int runKrylov(phase_hip_solver *s, KrylovVecs<T> &K, DBuf<T> &fac, const T *sell, T *x, int budget) {
+// onChunk(iterationsSoFar) runs on the host after every chunk.
+template <class T, class OnChunk>
+int runKrylov(phase_hip_solver *s, KrylovVecs<T> &K, DBuf<T> &fac, const T *sell, T *x, int budget,
+              OnChunk &&onChunk) {
   Scalars *sc = s->dSc.p;  // slot 0
   int it = 0;
   while (true) {
@@ -199,10 +210,13 @@ int runKrylov(phase_hip_solver *s, KrylovVecs<T> &K, DBuf<T> &fac, const T *sell
     pullScalars(s);
     it = s->hSc[0].iters;
     if (s->hSc[0].done != kDoneRunning || it >= budget) break;
+    onChunk(it);
   }
   checkDeviceErr(s);
   return it;
 }
+
+bool capturing(phase_hip_solver *s) { return s->gnn && s->gnn->captureEvery > 0; }
 
 void solveFp64(phase_hip_solver *s) {
   const int n = s->rank;
@@ -231,12 +245,48 @@ void solveFp64(phase_hip_solver *s) {
     s->error = std::sqrt(s->hSc[0].rr / b2);
     ++s->outerPasses;
     if (s->hSc[0].done == kDoneConverged || total >= s->maxIters) break;
-    int it = runKrylov<double>(s, K, s->dFac64, s->dSell64.p, s->dX.p, s->maxIters - total);
+    if (capturing(s) && total == 0) {
+      gnnCaptureIter(s, K.r.p, s->dX.p, 0);
+      s->lastCaptureIter = 0;
+    }
+    const int base = total;
+    int it = runKrylov<double>(s, K, s->dFac64, s->dSell64.p, s->dX.p, s->maxIters - total, [&](int sofar) {
+      if (capturing(s) && base + sofar - s->lastCaptureIter >= s->gnn->captureEvery) {
+        gnnCaptureIter(s, K.r.p, s->dX.p, base + sofar);
+        s->lastCaptureIter = base + sofar;
+      }
+    });
     total += it;
     if (s->hSc[0].done == kDoneBreakdown && it == 0) break;
   }
   s->innerIters = total;
   s->nIters = total;
+  if (capturing(s)) gnnCaptureFinish(s, s->dX.p, s->error <= s->tol);
+}
+
+// Learned step of a refinement pass: e = network(r); the step is the
+// minimal-residual multiple alpha e of it, kept only when it reduces ||r|| by
+// at least the fallback ratio -- otherwise the rest of the solve runs without
+// the network (plain ILU(0) refinement).
+bool learnedStep(phase_hip_solver *s, double r2) {
+  const int n = s->rank;
+  s->dE64.alloc(n);
+  s->dQ64.alloc(n);
+  Scalars *misc = s->dSc.p + 2, *outer = s->dSc.p + 1;
+  gnnApply(s, s->dR64.p, s->dE64.p);
+  hipLaunchKernelGGL((kSpmvDot2<double>), dim3(spmvGrid(s)), dim3(kRedBlock), 0, s->stream, n, s->ord.nSlices,
+                     s->dSliceOff.p, s->dSellCol.p, s->dSell64.p, s->dE64.p, s->dQ64.p, s->dR64.p, misc,
+                     s->dPartials.p, s->dCounter.p);
+  pullScalars(s);
+  const double rq = s->hSc[2].ts, qq = s->hSc[2].tt;
+  if (!(qq > 0.) || !std::isfinite(rq) || !std::isfinite(qq)) return false;
+  const double newr2 = r2 - rq * rq / qq;
+  const double ratio = std::sqrt(std::max(newr2, 0.) / r2);
+  if (!(ratio <= s->fallbackRatio)) return false;
+  hipLaunchKernelGGL((kMinResUpdate<double>), dim3(gridFor(n, kRedBlock, kRedCap)), dim3(kRedBlock), 0, s->stream, n,
+                     misc, outer, s->dX.p, s->dE64.p, s->dR64.p, s->dQ64.p, s->dPartials.p, s->dCounter.p);
+  PH_CHECK(hipGetLastError());
+  return true;
 }
 
 void solveMixed(phase_hip_solver *s) {
@@ -253,6 +303,8 @@ void solveMixed(phase_hip_solver *s) {
     s->error = 0.;
     return;
   }
+  bool useNet = s->mode == PHASE_HIP_MODE_LEARNED && s->gnn && s->gnn->trained;
+  if (useNet) gnnPrepare(s);
   Scalars *inner = s->dSc.p, *outer = s->dSc.p + 1;
   int total = 0;
   for (int pass = 0; pass <= s->maxOuter; ++pass) {
@@ -263,6 +315,14 @@ void solveMixed(phase_hip_solver *s) {
     s->error = std::sqrt(s->hSc[1].rr / b2);
     if (s->error <= s->tol || total >= s->maxIters || pass == s->maxOuter) break;
     ++s->outerPasses;
+    if (useNet) {
+      if (learnedStep(s, s->hSc[1].rr)) {
+        ++s->learnedSteps;
+      } else {
+        useNet = false;
+        ++s->fallbacks;
+      }
+    }
     hipLaunchKernelGGL((kNormalise<float>), dim3(gridFor(n, 256, kElemCap)), dim3(256), 0, s->stream, n,
                        s->dR64.p, K.r.p, K.r0.p, outer);
     hipLaunchKernelGGL((kNorm2<float>), dim3(gridFor(n, kRedBlock, kRedCap)), dim3(kRedBlock), 0, s->stream,
@@ -271,7 +331,7 @@ void solveMixed(phase_hip_solver *s) {
     const double want = std::max(s->innerTol, 0.5 * s->tol / s->error);
     hipLaunchKernelGGL((kBicgInit<float>), dim3(gridFor(n, 256, kElemCap)), dim3(256), 0, s->stream, n, inner,
                        K.p.p, K.v.p, K.x.p, want * want, s->maxIters - total, 0.0);
-    int it = runKrylov<float>(s, K, s->dFac32, s->dSell32.p, K.x.p, s->maxIters - total);
+    int it = runKrylov<float>(s, K, s->dFac32, s->dSell32.p, K.x.p, s->maxIters - total, [](int) {});
     total += it;
     hipLaunchKernelGGL((kAccumulate<float>), dim3(gridFor(n, 256, kElemCap)), dim3(256), 0, s->stream, n,
                        s->dX.p, K.x.p, outer);
@@ -339,6 +399,8 @@ void solverSetPattern(phase_hip_solver *s, int n, const int *rowPtr, const int *
   PH_CHECK(hipStreamSynchronize(st));
 }
 
+void solverEnsureSingle(phase_hip_solver *s) { ensureSingle(s); }
+
 void solverValuesChanged(phase_hip_solver *s) {
   const int m = (int)s->dSell64.n;
   hipLaunchKernelGGL((kGather<double>), dim3(gridFor(m, 256, kElemCap)), dim3(256), 0, s->stream, m,
@@ -398,6 +460,7 @@ int phase_hip_solver_destroy(phase_hip_solver *s) {
   (void)hipSetDevice(s->device);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   solverFreeBinding(s);
+  gnnFree(s);
   s->dBlocks.release();
   if (s->hSc) (void)hipHostFree(s->hSc);
   if (s->ev0) (void)hipEventDestroy(s->ev0);
@@ -530,6 +593,7 @@ int phase_hip_solver_solve(phase_hip_solver *s, double *relres) {
     if (!s->haveGuess && !s->haveSolution) PH_CHECK(hipMemsetAsync(s->dX.p, 0, s->rank * sizeof(double), s->stream));
     PH_CHECK(hipEventRecord(s->ev0, s->stream));
     s->fallbacks = 0;
+    s->learnedSteps = 0;
     if (s->mode == PHASE_HIP_MODE_FP64) solveFp64(s);
     else solveMixed(s);
     PH_CHECK(hipEventRecord(s->ev1, s->stream));

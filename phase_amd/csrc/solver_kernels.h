@@ -37,8 +37,8 @@ struct SyncDev {
   int *err;
 };
 
-__device__ __forceinline__ int ldAcquire(const int *p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ int ldRelaxed(const int *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void stRelease(int *p, int v) {
   __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
@@ -46,9 +46,12 @@ __device__ __forceinline__ void stRelease(int *p, int v) {
 
 // Bounded wait for a block's completion flag: every wait ends (after about a
 // second at worst, flagging an error) so a wrong dependency can never hang.
+// The poll is a relaxed device-coherent load; callers follow the wait with one
+// agent-scope acquire fence, so the cache invalidation happens once per block
+// rather than once per poll.
 __device__ __forceinline__ void waitFlag(const int *f, int epoch, int *err) {
   for (int it = 0; it < (1 << 24); ++it) {
-    if (ldAcquire(f) >= epoch) return;
+    if (ldRelaxed(f) >= epoch) return;
     __builtin_amdgcn_s_sleep(1);
   }
   atomicOr(err, 1);
@@ -310,6 +313,46 @@ __global__ __launch_bounds__(kRedBlock) void kUpdate(int n, Scalars *sc, T *__re
     else if (sc->minRatio > 0. && rr > sc->minRatio * sc->minRatio * sc->rrPrev) sc->done = kDoneStagnation;
     else if (sc->iters >= sc->maxIters) sc->done = kDoneMaxIters;
   }
+}
+
+// q = A e ; sc->ts = (r, q), sc->tt = (q, q)
+template <class T>
+__global__ __launch_bounds__(kRedBlock) void kSpmvDot2(int n, int nSlices, const int *__restrict__ sliceOff,
+                                                       const int *__restrict__ col, const T *__restrict__ val,
+                                                       const T *__restrict__ e, T *__restrict__ q,
+                                                       const T *__restrict__ r, Scalars *sc, double *partials,
+                                                       unsigned *counter) {
+  const int slice = blockIdx.x * (kRedBlock / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int row = slice * 64 + lane;
+  double d[2] = {0., 0.};
+  if (slice < nSlices && row < n) {
+    const T y = sellRow(slice, lane, sliceOff, col, val, e);
+    q[row] = y;
+    d[0] = (double)r[row] * (double)y;
+    d[1] = (double)y * (double)y;
+  }
+  if (reduceLast<2>(d, partials, counter) && threadIdx.x == 0) {
+    sc->ts = d[0];
+    sc->tt = d[1];
+  }
+}
+
+// alpha = ts / tt ; x += alpha e ; r -= alpha q ; outer->rr = (r, r)
+template <class T>
+__global__ __launch_bounds__(kRedBlock) void kMinResUpdate(int n, const Scalars *step, Scalars *outer,
+                                                           T *__restrict__ x, const T *__restrict__ e,
+                                                           T *__restrict__ r, const T *__restrict__ q,
+                                                           double *partials, unsigned *counter) {
+  const T alpha = (T)(step->ts / step->tt);
+  double d[1] = {0.};
+  for (int i = blockIdx.x * kRedBlock + threadIdx.x; i < n; i += gridDim.x * kRedBlock) {
+    x[i] += alpha * e[i];
+    const T ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    d[0] += (double)ri * (double)ri;
+  }
+  if (reduceLast<1>(d, partials, counter) && threadIdx.x == 0) outer->rr = d[0];
 }
 
 // x64 += scale * x32 with scale = sqrt(*rrOuter) (undoing the inner normalisation)

@@ -66,6 +66,13 @@ _SIGS = {
     "phase_hip_free": (ctypes.c_int, [_vp]),
     "phase_hip_memcpy": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t, ctypes.c_int]),
     "phase_hip_equation_export_padded": (ctypes.c_int, [_vp, _ip, _vp, _vp, _vp, _vp]),
+    "phase_hip_learned_capture": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "phase_hip_learned_n_pairs": (ctypes.c_int, [_vp, _ip]),
+    "phase_hip_learned_train": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp]),
+    "phase_hip_learned_params": (ctypes.c_int, [_vp, _vp, _ip]),
+    "phase_hip_learned_set_params": (ctypes.c_int, [_vp, _vp, ctypes.c_int]),
+    "phase_hip_learned_eval": (ctypes.c_int, [_vp, _f64p, _f64p, _f64p, _f64p, _dp, _vp]),
+    "phase_hip_learned_apply": (ctypes.c_int, [_vp, _f64p, _f64p]),
 }
 
 _lib = None
@@ -291,6 +298,46 @@ class HipSparseMatrixSolver:
         out = np.zeros(len(self.KERNEL_NAMES))
         self._check(load_library().phase_hip_solver_time_kernels(self._h, int(reps), out, len(out)), "time")
         return dict(zip(self.KERNEL_NAMES, out.tolist()))
+
+    # --- learned correction network
+    def learned_capture(self, every_iters, max_pairs):
+        self._check(load_library().phase_hip_learned_capture(self._h, int(every_iters), int(max_pairs)), "capture")
+
+    def learned_n_pairs(self):
+        c = ctypes.c_int()
+        self._check(load_library().phase_hip_learned_n_pairs(self._h, ctypes.byref(c)), "pairs")
+        return c.value
+
+    def learned_train(self, steps, lr=1e-3, seed=0):
+        loss = ctypes.c_double()
+        self._check(load_library().phase_hip_learned_train(self._h, int(steps), float(lr), int(seed),
+                                                           ctypes.byref(loss)), "train")
+        return loss.value
+
+    def learned_params(self):
+        c = ctypes.c_int()
+        self._check(load_library().phase_hip_learned_params(self._h, None, ctypes.byref(c)), "params")
+        out = np.empty(c.value, np.float32)
+        self._check(load_library().phase_hip_learned_params(self._h, _p(out), ctypes.byref(c)), "params")
+        return out
+
+    def learned_set_params(self, params):
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        self._check(load_library().phase_hip_learned_set_params(self._h, _p(params), len(params)), "setParams")
+
+    def learned_eval(self, r, z, target):
+        n = self.rank
+        out = np.empty(n)
+        loss = ctypes.c_double()
+        grad = np.empty(len(self.learned_params()), np.float32)
+        self._check(load_library().phase_hip_learned_eval(self._h, _f64(r), _f64(z), _f64(target), out,
+                                                          ctypes.byref(loss), _p(grad)), "eval")
+        return out, loss.value, grad
+
+    def learned_apply(self, r):
+        e = np.empty(self.rank)
+        self._check(load_library().phase_hip_learned_apply(self._h, _f64(r), e), "apply")
+        return e
 
     # --- fv:: operators on a HipMesh
     def assemble_laplacian(self, mesh, variant, gamma=1.0, gamma_face=None, gamma0_face=None, theta=1.0,

@@ -1,0 +1,134 @@
+"""The learned correction network on the GPU (BF16 MFMA, FP32 accumulate)
+against its FP64 restatement (oracle/gnn_oracle.py), and its use in the
+solver: training lowers the loss, learned mode still converges to the FP64
+goal, and a useless network is rejected by the fallback."""
+import numpy as np
+import pytest
+
+from oracle import gnn_oracle as NO
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+def bf16_round(a):
+    """Round to the nearest bfloat16 (ties to even), returned as float64."""
+    f = np.ascontiguousarray(a, dtype=np.float32)
+    u = f.view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return u.astype(np.uint32).view(np.float32).astype(np.float64).reshape(np.shape(a))
+
+
+def flatten(p):
+    L = NO.n_layers(p)
+    parts = [p["W_in"].ravel(), p["b_in"]]
+    for l in range(L):
+        parts += [np.vstack([p["Ws%d" % l], p["Wm%d" % l]]).ravel(), p["Wg%d" % l].ravel(), p["b%d" % l]]
+    parts += [p["w_out"], p["b_out"]]
+    return np.concatenate(parts)
+
+
+def unflatten_like(flat, p):
+    g, o = {}, 0
+    def take(shape):
+        nonlocal o
+        n = int(np.prod(shape))
+        v = flat[o:o + n].reshape(shape)
+        o += n
+        return v
+    g["W_in"] = take((4, 64))
+    g["b_in"] = take((64,))
+    for l in range(NO.n_layers(p)):
+        wsm = take((128, 64))
+        g["Ws%d" % l], g["Wm%d" % l] = wsm[:64], wsm[64:]
+        g["Wg%d" % l] = take((64, 64))
+        g["b%d" % l] = take((64,))
+    g["w_out"] = take((64,))
+    g["b_out"] = take((1,))
+    return g
+
+
+@pytest.fixture(scope="module")
+def system(oracle):
+    xy, cptr, cind = H.delaunay_case(3000, 24, seed=21)
+    m, (rp, ci, v), b, g, bc = H.pressure_system(oracle, xy, cptr, cind, sigma=1.0)
+    return rp, ci, v, b
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+
+
+def test_forward_backward_match_fp64_oracle(hip, system):
+    rp, ci, v, b = system
+    A = H.compact_sorted(rp, ci, v)
+    s = hip.HipSparseMatrixSolver(0)
+    s.set(rp, ci, v)
+    rng = np.random.default_rng(0)
+    p = NO.init_params(hidden=64, layers=6, seed=3)
+    for k in p:
+        if k.startswith("b"):
+            p[k] = rng.standard_normal(p[k].shape) * 0.05
+        p[k] = bf16_round(p[k])
+    s.learned_set_params(flatten(p).astype(np.float32))
+    n = len(b)
+    r = rng.standard_normal(n)
+    z = rng.standard_normal(n) * 1e-2
+    target = rng.standard_normal(n)
+    Ah, sd, rowsum = NO.scaled_graph(A)
+    X, sc = NO.features(r, z, sd, rowsum)
+    loss_ref, g_ref, out_ref = NO.loss_and_grad(Ah, X, p, target * sd / sc)
+    out, loss, grad = s.learned_eval(r, z, target)
+    assert rel(out, out_ref) < 3e-2
+    assert abs(loss - loss_ref) <= 5e-2 * loss_ref
+    g = unflatten_like(grad.astype(np.float64), p)
+    for k in g_ref:
+        assert rel(g[k], g_ref[k]) < 0.12, k
+
+
+def test_training_and_learned_mode(hip, system):
+    rp, ci, v, b = system
+    n = len(b)
+    s = hip.HipSparseMatrixSolver(0, tolerance=1e-8, maxIters=20000, mode="fp64")
+    s.set(rp, ci, v)
+    s.learned_capture(20, 200)
+    rng = np.random.default_rng(5)
+    for _ in range(6):
+        bb = rng.standard_normal(n)
+        s.set_rhs(bb - bb.mean())
+        s.set_guess(np.zeros(n))
+        s.solve()
+    npairs = s.learned_n_pairs()
+    assert npairs >= 30
+    first = s.learned_train(1, lr=1e-3, seed=1)
+    last = np.mean([s.learned_train(50, lr=1e-3, seed=t) for t in range(8)])
+    assert np.isfinite(last) and last < first
+    # learned mode reaches the same FP64 goal; the FP64 true residual is the judge
+    A = H.compact_sorted(rp, ci, v)
+    for mode in ("mixed", "learned"):
+        s.setup("mode", mode)
+        s.set_rhs(b)
+        s.set_guess(np.zeros(n))
+        rr = s.solve()
+        x = s.x()
+        assert rr <= 1e-8
+        assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) <= 1.5e-8
+
+
+def test_fallback_on_useless_network(hip, system):
+    rp, ci, v, b = system
+    n = len(b)
+    s = hip.HipSparseMatrixSolver(0, tolerance=1e-8, maxIters=20000, mode="learned")
+    s.set(rp, ci, v)
+    params = s.learned_params()
+    s.learned_set_params(np.zeros_like(params))        # network output identically 0
+    s.set_rhs(b)
+    rr = s.solve()
+    assert rr <= 1e-8
+    assert s.stats()["fallbacks"] == 1
+    bad = np.full_like(params, np.nan)                  # poisoned weights
+    s.learned_set_params(bad)
+    s.set_rhs(b)
+    s.set_guess(np.zeros(n))
+    rr = s.solve()
+    assert rr <= 1e-8 and s.stats()["fallbacks"] == 1
