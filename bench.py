@@ -87,7 +87,8 @@ def main():
     ap.add_argument("--edge", type=int, default=512)
     ap.add_argument("--sigma", type=float, default=1.0)
     ap.add_argument("--tol", type=float, default=1e-8)
-    ap.add_argument("--inner-tol", type=float, default=1e-3)
+    ap.add_argument("--inner-tol", type=float, default=0.1)
+    ap.add_argument("--sweep", default="colour")
     ap.add_argument("--modes", default="fp64,mixed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=20)
@@ -117,13 +118,13 @@ def main():
     nnz = None
     for mode in args.modes.split(","):
         s = P.HipSparseMatrixSolver(0, tolerance=args.tol, maxIters=50000, mode=mode,
-                                    innerTolerance=args.inner_tol)
+                                    innerTolerance=args.inner_tol, sweep=args.sweep)
 
         def step():
             s.assemble_laplacian_device(mesh, 3, gamma_face=d_g, bc_type=d_bc, phi_face=d_pf)
             s.sub_rhs_device(d_b)
             s.zero_guess()
-            return s.solve()
+            return s.solve(raise_on_noconv=False)
 
         for _ in range(args.warmup):
             step()

@@ -88,7 +88,9 @@ int phase_hip_solver_destroy(phase_hip_solver *s);
 
 /* SparseMatrixSolver::setup(ptree) -- TrilinosBelosSparseMatrixSolver.cpp:44-86.
  * Keys: "tolerance" (1e-8), "maxIters" (500), "mode" (fp64|mixed|learned),
- * "innerTolerance" (mixed: relative reduction per refinement pass, 1e-3),
+ * "innerTolerance" (mixed: residual reduction between FP64 refinement points, 0.1),
+ * "sweep" (ILU sweeps: "colour" = one launch per block colour, "syncfree" =
+ * one launch with device-scope completion flags),
  * "blockRows" (rows per ILU block, <= 64), "fallbackRatio" (learned: minimum
  * per-iteration residual reduction before falling back, 0.5).  Unknown keys
  * are ignored, as the reference ignores ptree keys it does not read. */
@@ -167,13 +169,15 @@ int phase_hip_solver_time_kernels(phase_hip_solver *s, int reps, double *ms, int
 int phase_hip_learned_capture(phase_hip_solver *s, int everyIters, int maxPairs);
 int phase_hip_learned_n_pairs(const phase_hip_solver *s, int *count);
 /* `steps` Adam steps (one captured pair each) at learning rate lr; *lastLoss is
- * the loss of the final step. */
+ * the mean loss over those steps.  Loss = 1 - cos^2(network output, target
+ * correction): only the direction matters, the solver scales the step. */
 int phase_hip_learned_train(phase_hip_solver *s, int steps, double lr, int seed, double *lastLoss);
 /* Flat FP32 parameters (checkpoint); count queried with params == NULL. */
 int phase_hip_learned_params(phase_hip_solver *s, float *params, int *count);
 int phase_hip_learned_set_params(phase_hip_solver *s, const float *params, int count);
 /* Forward + backward for residual r, ILU-preconditioned residual z and target
- * correction (original numbering): output, loss and flat gradient. */
+ * (in network output units, original numbering): output, loss and flat
+ * gradient. */
 int phase_hip_learned_eval(phase_hip_solver *s, const double *r, const double *z, const double *target,
                            double *out, double *loss, float *grad);
 /* e = network correction of residual r (original numbering). */

@@ -17,7 +17,11 @@ sd_i = sqrt(d_i):
         pre = h W_s + agg W_m + g W_g + b
         h   = h + relu(pre)
     out = h w_out + b_out ;  correction e_i = out_i s / sd_i
-Loss on a captured pair (r, e): mean_i (out_i - e_i sd_i / s)^2.
+Loss on a captured pair (r, e), with t_i = e_i sd_i / s:
+    loss = 1 - (out . t)^2 / (|out|^2 |t|^2)
+i.e. one minus the squared cosine between output and target.  The solver
+applies the correction through a minimal-residual multiple alpha * e, so only
+its direction matters; the loss is invariant to the output's scale and sign.
 """
 import numpy as np
 
@@ -85,9 +89,11 @@ def forward(Ah, X, p):
 def loss_and_grad(Ah, X, p, target):
     out, c = forward(Ah, X, p)
     N = len(out)
-    diff = out - target
-    loss = np.mean(diff * diff)
-    dout = 2.0 * diff / N
+    a = out @ target
+    pp = out @ out
+    q = target @ target
+    loss = 1.0 - a * a / (pp * q)
+    dout = (2.0 * a / (pp * q)) * (a * out / pp - target)
     g = {}
     hL = c["hs"][-1]
     g["w_out"] = hL.T @ dout

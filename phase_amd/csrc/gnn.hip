@@ -253,39 +253,66 @@ __global__ __launch_bounds__(256) void kGnnLayerFwd(int n, const __bf16 *__restr
 }
 
 // ---------------------------------------------------------------- output layer
-// out_i = h_i . w_out + b_out ; with a target: loss partials, dout and dh_L.
+// out_i = h_i . w_out + b_out
 __global__ __launch_bounds__(256) void kGnnOut(int n, const __bf16 *__restrict__ h, const float *__restrict__ wo,
-                                               const float *__restrict__ tgt, float *__restrict__ out,
-                                               float *__restrict__ dh, double *scal, double *partials,
-                                               unsigned *counter) {
-  double d[1] = {0.};
+                                               float *__restrict__ out) {
   for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
     float v = wo[H];
     for (int f = 0; f < H; ++f) v = fmaf((float)h[(size_t)i * H + f], wo[f], v);
     out[i] = v;
-    if (tgt) {
-      const float diff = v - tgt[i];
-      d[0] += (double)diff * diff;
-      const float dout = 2.f * diff / (float)n;
-      for (int f = 0; f < H; ++f) dh[(size_t)i * H + f] = dout * wo[f];
-    }
   }
-  if (!tgt) return;
-  if (reduceLast<1>(d, partials, counter) && threadIdx.x == 0) scal[1] = d[0] / (double)n;
 }
 
-// grad of w_out / b_out: per-block partials of h^T dout (dout recomputed)
-__global__ __launch_bounds__(256) void kGnnOutGrad(int n, const __bf16 *__restrict__ h, const float *__restrict__ out,
-                                                   const float *__restrict__ tgt, float *__restrict__ partials) {
+// a = out.t, p = |out|^2, q = |t|^2 ; loss = 1 - a^2/(p q) -> scal[1..4];
+// scal[5] accumulates losses over training steps
+__global__ __launch_bounds__(kRedBlock) void kGnnLossCos(int n, const float *__restrict__ out,
+                                                         const float *__restrict__ t, double *scal,
+                                                         double *partials, unsigned *counter) {
+  double d[3] = {0., 0., 0.};
+  for (int i = blockIdx.x * kRedBlock + threadIdx.x; i < n; i += gridDim.x * kRedBlock) {
+    const double o = out[i], ti = t[i];
+    d[0] += o * ti;
+    d[1] += o * o;
+    d[2] += ti * ti;
+  }
+  if (reduceLast<3>(d, partials, counter) && threadIdx.x == 0) {
+    const double pq = d[1] * d[2];
+    const double loss = pq > 0. ? 1.0 - d[0] * d[0] / pq : 1.0;
+    scal[1] = loss;
+    scal[2] = d[0];
+    scal[3] = d[1];
+    scal[4] = d[2];
+    scal[5] += loss;
+  }
+}
+
+// dout_i = 2a/(p q) (a out_i / p - t_i) ; dh_L = dout w_out^T
+__global__ void kGnnDout(int n, const float *__restrict__ out, const float *__restrict__ t,
+                         const float *__restrict__ wo, const double *__restrict__ scal, float *__restrict__ dout,
+                         float *__restrict__ dh) {
+  const double a = scal[2], p = scal[3], q = scal[4];
+  const bool ok = p > 0. && q > 0.;
+  const float c1 = ok ? (float)(2.0 * a / (p * q)) : 0.f, c2 = ok ? (float)(a / p) : 0.f;
+  for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < (long)n * H; k += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(k / H), f = (int)(k % H);
+    const float d = c1 * (c2 * out[i] - t[i]);
+    if (f == 0) dout[i] = d;
+    dh[k] = d * wo[f];
+  }
+}
+
+// grad of w_out / b_out: per-block partials of h^T dout and sum(dout)
+__global__ __launch_bounds__(256) void kGnnOutGrad(int n, const __bf16 *__restrict__ h, const float *__restrict__ dout,
+                                                   float *__restrict__ partials) {
   __shared__ float red[4][H + 1];
   const int f = threadIdx.x & 63, q = threadIdx.x >> 6;
   const int rowsPer = (n + gridDim.x - 1) / gridDim.x;
   const int r0 = blockIdx.x * rowsPer, r1 = min(n, r0 + rowsPer);
   float acc = 0.f, accb = 0.f;
   for (int i = r0 + q; i < r1; i += 4) {
-    const float dout = 2.f * (out[i] - tgt[i]) / (float)n;
-    acc = fmaf((float)h[(size_t)i * H + f], dout, acc);
-    if (f == 0) accb += dout;
+    const float d = dout[i];
+    acc = fmaf((float)h[(size_t)i * H + f], d, acc);
+    if (f == 0) accb += d;
   }
   red[q][f] = acc;
   if (f == 0) red[q][H] = accb;
@@ -563,7 +590,8 @@ void ensureNet(phase_hip_solver *s) {
     G.cvec.alloc((size_t)L * H);
     const int nbW = (n + kChunkRows - 1) / kChunkRows;
     G.partials.alloc(std::max<size_t>((size_t)nbW * (128 * H + H), (size_t)2048 * (kGnnDin + 1) * H));
-    G.scal.alloc(4);
+    G.scal.alloc(8);
+    G.scal.zero(s->stream);
     G.counter.alloc(1);
     G.counter.zero(s->stream);
     G.wT.alloc((size_t)L * 128 * H);
@@ -601,7 +629,7 @@ void gnnFeatures(phase_hip_solver *s, const float *r, const float *z) {
 
 namespace phase {
 
-void gnnForward(phase_hip_solver *s, const float *target) {
+void gnnForward(phase_hip_solver *s) {
   GnnState &G = *s->gnn;
   const int n = s->rank, L = G.lay.L;
   const float *P = G.params.p;
@@ -619,17 +647,28 @@ void gnnForward(phase_hip_solver *s, const float *target) {
                        G.cvec.p + l * H, G.agg.p + (size_t)l * n * H, G.pre.p + (size_t)l * n * H, hL(G, l + 1));
   }
   hipLaunchKernelGGL(kGnnOut, dim3(gridFor1(n)), dim3(256), 0, s->stream, n, (const __bf16 *)hL(G, L),
-                     P + G.lay.outW(), target, G.out.p, G.dh.p, G.scal.p, s->dPartials.p, s->dCounter.p);
+                     P + G.lay.outW(), G.out.p);
+  PH_CHECK(hipGetLastError());
+}
+
+// Cosine loss of the last forward against target t; dout and dh_L for backward.
+void gnnLoss(phase_hip_solver *s, const float *target) {
+  GnnState &G = *s->gnn;
+  const int n = s->rank;
+  hipLaunchKernelGGL(kGnnLossCos, dim3(gridFor1(n)), dim3(kRedBlock), 0, s->stream, n, G.out.p, target, G.scal.p,
+                     s->dPartials.p, s->dCounter.p);
+  hipLaunchKernelGGL(kGnnDout, dim3(g1((long)n * H)), dim3(256), 0, s->stream, n, G.out.p, target,
+                     G.params.p + G.lay.outW(), G.scal.p, G.dOut.p, G.dh.p);
   PH_CHECK(hipGetLastError());
 }
 
 // Gradient of the loss of the last forward (with target) into G.grad.
-void gnnBackward(phase_hip_solver *s, const float *target) {
+void gnnBackward(phase_hip_solver *s) {
   GnnState &G = *s->gnn;
   const int n = s->rank, L = G.lay.L;
   const float *P = G.params.p;
   const int nbO = std::min(1024, std::max(1, (n + 1023) / 1024));
-  hipLaunchKernelGGL(kGnnOutGrad, dim3(nbO), dim3(256), 0, s->stream, n, (const __bf16 *)hL(G, L), G.out.p, target,
+  hipLaunchKernelGGL(kGnnOutGrad, dim3(nbO), dim3(256), 0, s->stream, n, (const __bf16 *)hL(G, L), G.dOut.p,
                      G.partials.p);
   hipLaunchKernelGGL(kSumPartials, dim3(1), dim3(128), 0, s->stream, nbO, H + 1, G.partials.p, G.grad.p + G.lay.outW());
   const int nwg = (n + kWgRows - 1) / kWgRows;
@@ -662,34 +701,18 @@ namespace {
 void gnnTrainStep(phase_hip_solver *s, int pair, float lr) {
   GnnState &G = *s->gnn;
   const int n = s->rank;
+  KrylovVecs<float> &K = s->k32;
+  if (pair < 0 || pair >= G.nPairs || K.y.n < (size_t)n || K.sh.n < (size_t)n || !s->dFac32.p)
+    fail(PHASE_HIP_ESTATE, "training state not prepared");
   const float *r = G.pairR.p + (size_t)pair * n;
   const float *e = G.pairE.p + (size_t)pair * n;
-  KrylovVecs<float> &K = s->k32;
   // z = ILU(0)^{-1} r (FP32 factor)
-  {
-    unsigned long long base;
-    int ep;
-    base = s->ticketBase;
-    s->ticketBase += (unsigned long long)s->ord.nBlocks;
-    ep = ++s->epoch;
-    IluDev<float> L;
-    L.blocks = s->dBlocks.p; L.rows = s->dRows.p; L.pred = s->dPred.p; L.succ = s->dSucc.p;
-    L.lCol = s->dLCol.p; L.uCol = s->dUCol.p; L.fac = s->dFac32.p; L.nb = s->ord.nBlocks;
-    L.nLSlots = s->ord.nLSlots; L.nUSlots = s->ord.nUSlots; L.n = n;
-    SyncDev sy{s->dFlags.p, s->dTicket.p, s->dErr.p};
-    hipLaunchKernelGGL((kLSolve<float, kProNone>), dim3(s->ord.nBlocks), dim3(64), 0, s->stream, L, sy, base, ep,
-                       (const Scalars *)nullptr, r, K.y.p, (float *)nullptr, (const float *)nullptr,
-                       (const float *)nullptr);
-    base = s->ticketBase;
-    s->ticketBase += (unsigned long long)s->ord.nBlocks;
-    ep = ++s->epoch;
-    hipLaunchKernelGGL((kUSolve<float>), dim3(s->ord.nBlocks), dim3(64), 0, s->stream, L, sy, base, ep,
-                       (const Scalars *)nullptr, K.y.p, K.sh.p);
-  }
+  iluApplyDev<float>(s, s->dFac32.p, r, K.y.p, K.sh.p);
   gnnFeatures(s, r, K.sh.p);
-  hipLaunchKernelGGL(kGnnTarget, dim3(g1(n)), dim3(256), 0, s->stream, n, e, G.sd.p, G.scal.p, G.dOut.p);
-  gnnForward(s, G.dOut.p);
-  gnnBackward(s, G.dOut.p);
+  hipLaunchKernelGGL(kGnnTarget, dim3(g1(n)), dim3(256), 0, s->stream, n, e, G.sd.p, G.scal.p, K.s.p);
+  gnnForward(s);
+  gnnLoss(s, K.s.p);
+  gnnBackward(s);
   ++G.step;
   const float b1 = 0.9f, b2 = 0.999f;
   const float c1 = 1.f - std::pow(b1, (float)G.step), c2 = 1.f - std::pow(b2, (float)G.step);
@@ -704,25 +727,12 @@ void applyNet(phase_hip_solver *s, const double *r64, double *e64) {
   GnnState &G = *s->gnn;
   const int n = s->rank;
   KrylovVecs<float> &K = s->k32;
+  if (K.t.n < (size_t)n || K.y.n < (size_t)n || K.sh.n < (size_t)n || !s->dFac32.p || G.n != n)
+    fail(PHASE_HIP_ESTATE, "network not prepared for this system");
   hipLaunchKernelGGL((kConvert<double, float>), dim3(g1(n)), dim3(256), 0, s->stream, n, r64, K.t.p);
-  unsigned long long base = s->ticketBase;
-  s->ticketBase += (unsigned long long)s->ord.nBlocks;
-  int ep = ++s->epoch;
-  IluDev<float> L;
-  L.blocks = s->dBlocks.p; L.rows = s->dRows.p; L.pred = s->dPred.p; L.succ = s->dSucc.p;
-  L.lCol = s->dLCol.p; L.uCol = s->dUCol.p; L.fac = s->dFac32.p; L.nb = s->ord.nBlocks;
-  L.nLSlots = s->ord.nLSlots; L.nUSlots = s->ord.nUSlots; L.n = n;
-  SyncDev sy{s->dFlags.p, s->dTicket.p, s->dErr.p};
-  hipLaunchKernelGGL((kLSolve<float, kProNone>), dim3(s->ord.nBlocks), dim3(64), 0, s->stream, L, sy, base, ep,
-                     (const Scalars *)nullptr, (const float *)K.t.p, K.y.p, (float *)nullptr, (const float *)nullptr,
-                     (const float *)nullptr);
-  base = s->ticketBase;
-  s->ticketBase += (unsigned long long)s->ord.nBlocks;
-  ep = ++s->epoch;
-  hipLaunchKernelGGL((kUSolve<float>), dim3(s->ord.nBlocks), dim3(64), 0, s->stream, L, sy, base, ep,
-                     (const Scalars *)nullptr, K.y.p, K.sh.p);
+  iluApplyDev<float>(s, s->dFac32.p, K.t.p, K.y.p, K.sh.p);
   gnnFeatures(s, K.t.p, K.sh.p);
-  gnnForward(s, nullptr);
+  gnnForward(s);
   hipLaunchKernelGGL(kGnnScaleOut, dim3(g1(n)), dim3(256), 0, s->stream, n, G.out.p, G.sd.p, G.scal.p, e64);
   PH_CHECK(hipGetLastError());
 }
@@ -757,6 +767,9 @@ void gnnFree(phase_hip_solver *s) {
 
 void gnnPrepare(phase_hip_solver *s) {
   solverEnsureSingle(s);
+  // the FP32 work vectors (ILU apply of the features, inference staging) exist
+  // even when only FP64 solves have run
+  s->k32.alloc(s->rank);
   ensureNet(s);
   prepareGraph(s);
   castWeights(s);
@@ -840,13 +853,11 @@ int phase_hip_learned_train(phase_hip_solver *s, int steps, double lr, int seed,
     GnnState &G = *s->gnn;
     std::mt19937 rng((unsigned)seed);
     double loss = 0.;
-    for (int t = 0; t < steps; ++t) {
-      gnnTrainStep(s, (int)(rng() % (unsigned)G.nPairs), (float)lr);
-      if (t + 1 == steps) {
-        PH_CHECK(hipMemcpyAsync(&loss, G.scal.p + 1, sizeof(double), hipMemcpyDeviceToHost, s->stream));
-      }
-    }
+    PH_CHECK(hipMemsetAsync(G.scal.p + 5, 0, sizeof(double), s->stream));
+    for (int t = 0; t < steps; ++t) gnnTrainStep(s, (int)(rng() % (unsigned)G.nPairs), (float)lr);
+    PH_CHECK(hipMemcpyAsync(&loss, G.scal.p + 5, sizeof(double), hipMemcpyDeviceToHost, s->stream));
     PH_CHECK(hipStreamSynchronize(s->stream));
+    if (steps > 0) loss /= steps;
     G.trained = true;
     if (lastLoss) *lastLoss = loss;
   });
@@ -908,8 +919,9 @@ int phase_hip_learned_eval(phase_hip_solver *s, const double *r, const double *z
     hipLaunchKernelGGL((kPermuteIn<double, float>), dim3(g1(n)), dim3(256), 0, s->stream, n, s->dPerm.p, tmp.p + 2 * n,
                        K.s.p, 1.0);
     gnnFeatures(s, K.t.p, K.sh.p);
-    gnnForward(s, K.s.p);
-    gnnBackward(s, K.s.p);
+    gnnForward(s);
+    gnnLoss(s, K.s.p);
+    gnnBackward(s);
     hipLaunchKernelGGL((kConvert<float, double>), dim3(g1(n)), dim3(256), 0, s->stream, n, G.out.p, tmp.p);
     hipLaunchKernelGGL((kPermuteOut<double>), dim3(g1(n)), dim3(256), 0, s->stream, n, s->dPerm.p, tmp.p, tmp.p + n);
     PH_CHECK(hipMemcpyAsync(out, tmp.p + n, n * sizeof(double), hipMemcpyDeviceToHost, s->stream));

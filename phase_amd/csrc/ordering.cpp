@@ -22,6 +22,7 @@
 #include <cstring>
 #include <numeric>
 #include <queue>
+#include <tuple>
 
 #include "phase_internal.h"
 
@@ -182,25 +183,44 @@ void buildOrdering(Ordering &o, int n, const int *rowPtr, const int *colInd, int
   }
   int nb = (int)members.size();
 
-  // 4. greedy colouring of the block graph
-  std::vector<int> colour(nb, -1), seen(nb + 64, -1);
-  int nColours = 0;
-  std::vector<int> used;
+  // 4. colouring of the block graph, DSATUR (Brelaz 1979): repeatedly colour
+  //    the block whose neighbours already use the most distinct colours (ties:
+  //    most neighbours, then lowest index) with the smallest free colour.
+  //    Every colour costs one kernel launch per sweep, so fewer is faster.
+  std::vector<std::vector<int>> badj(nb);
   for (int b = 0; b < nb; ++b) {
-    used.clear();
+    auto &a = badj[b];
     for (int u : members[b])
       for (int k = g.ptr[u]; k < g.ptr[u + 1]; ++k) {
         int ob = blk[g.adj[k]];
-        if (ob != b && colour[ob] >= 0) used.push_back(colour[ob]);
+        if (ob != b) a.push_back(ob);
       }
-    std::sort(used.begin(), used.end());
+    std::sort(a.begin(), a.end());
+    a.erase(std::unique(a.begin(), a.end()), a.end());
+  }
+  std::vector<int> colour(nb, -1), satur(nb, 0);
+  std::vector<uint64_t> usedMask(nb, 0);  // colours seen among neighbours (< 64)
+  // priority: (saturation, degree, -index); max-heap with lazy entries
+  using Key = std::tuple<int, int, int>;
+  std::priority_queue<Key> heap;
+  for (int b = 0; b < nb; ++b) heap.push(Key(0, (int)badj[b].size(), -b));
+  int nColours = 0;
+  while (!heap.empty()) {
+    auto [sat, deg, nbIdx] = heap.top();
+    heap.pop();
+    const int b = -nbIdx;
+    if (colour[b] >= 0 || sat != satur[b]) continue;  // stale entry
     int c = 0;
-    for (int x : used) {
-      if (x == c) ++c;
-      else if (x > c) break;
-    }
+    while (c < 64 && (usedMask[b] >> c & 1ull)) ++c;
+    if (c >= 64) fail(PHASE_HIP_EINVAL, "block graph needs more than 64 colours");
     colour[b] = c;
     nColours = std::max(nColours, c + 1);
+    for (int ob : badj[b]) {
+      if (colour[ob] >= 0 || (usedMask[ob] >> c & 1ull)) continue;
+      usedMask[ob] |= 1ull << c;
+      ++satur[ob];
+      heap.push(Key(satur[ob], (int)badj[ob].size(), -ob));
+    }
   }
   o.nColours = nColours;
 
@@ -224,6 +244,9 @@ void buildOrdering(Ordering &o, int n, const int *rowPtr, const int *colInd, int
   }
   o.iperm.assign(n, 0);
   for (int k = 0; k < n; ++k) o.iperm[o.perm[k]] = k;
+  o.colourPtr.assign(nColours + 1, 0);
+  for (int b = 0; b < nb; ++b) ++o.colourPtr[colour[border[b]] + 1];
+  for (int c = 0; c < nColours; ++c) o.colourPtr[c + 1] += o.colourPtr[c];
 
   // 6. SELL-64 layout of A over solver rows, original in-row order
   o.nSlices = (n + kLanes - 1) / kLanes;
@@ -285,6 +308,8 @@ void buildOrdering(Ordering &o, int n, const int *rowPtr, const int *colInd, int
       if (nL > 255 || nU > 255) fail(PHASE_HIP_EINVAL, "row too long for the ILU layout");
       if (nU - nUin > 8)
         fail(PHASE_HIP_EINVAL, "row couples to more than 8 rows of later ILU blocks");
+      if (nL - nLoff > 8 || nUin > 8)
+        fail(PHASE_HIP_EINVAL, "row couples to more than 8 rows of its own ILU block");
       o.rows[k].nL = (uint8_t)nL;
       o.rows[k].nLoff = (uint8_t)nLoff;
       o.rows[k].nU = (uint8_t)nU;

@@ -89,6 +89,7 @@ struct Ordering {
   std::vector<int> compactSrc;       // compact entry -> caller slot (padded index)
   std::vector<int> perm, iperm;      // solver row k <-> original row perm[k]
   int nBlocks = 0, nColours = 0;
+  std::vector<int> colourPtr;         // blocks of colour c: [colourPtr[c], colourPtr[c+1])
   std::vector<BlockInfo> blocks;
   std::vector<int> pred, succ;
   std::vector<RowInfo> rows;          // per solver row

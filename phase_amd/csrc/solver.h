@@ -33,11 +33,12 @@ struct phase_hip_solver {
   double tol = 1e-8;
   int maxIters = 500;
   int mode = PHASE_HIP_MODE_FP64;
-  double innerTol = 1e-3;
+  double innerTol = 0.1;
   int blockRows = 64;
   double fallbackRatio = 0.5;
   int maxOuter = 100;
   int chunk = 8;  // iterations enqueued between host checks
+  int syncFree = 0;  // ILU sweeps: 0 = one launch per block colour, 1 = one sync-free launch
 
   // pattern and ordering
   int rank = 0;
@@ -99,4 +100,8 @@ void solverFreeBinding(phase_hip_solver *s);
 void checkAssembly(phase_hip_solver *s);
 // FP32 copies of A and the FP32 ILU(0) factor are current.
 void solverEnsureSingle(phase_hip_solver *s);
+// out = U^{-1} L^{-1} in with the ILU(0) factor fac (solver order; tmp holds
+// the forward-sweep result).  Instantiated for float and double.
+template <class T>
+void iluApplyDev(phase_hip_solver *s, T *fac, const T *in, T *tmp, T *out);
 }  // namespace phase

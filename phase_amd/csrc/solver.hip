@@ -66,11 +66,94 @@ IluDev<T> iluDev(phase_hip_solver *s, T *fac) {
 
 SyncDev syncDev(phase_hip_solver *s) { return SyncDev{s->dFlags.p, s->dTicket.p, s->dErr.p}; }
 
-// Every sweep launch takes nb tickets and a fresh epoch.
-void nextSweep(phase_hip_solver *s, unsigned long long &base, int &epoch) {
-  base = s->ticketBase;
+// Sweep launchers: one launch per block colour (forward colours ascending,
+// backward descending), or one sync-free launch taking nb tickets and a
+// fresh epoch.
+SweepCtl syncFreeCtl(phase_hip_solver *s) {
+  SweepCtl c;
+  c.syncFree = 1;
+  c.blockBase = 0;
+  c.ticketBase = s->ticketBase;
   s->ticketBase += (unsigned long long)s->ord.nBlocks;
-  epoch = ++s->epoch;
+  c.epoch = ++s->epoch;
+  return c;
+}
+
+SweepCtl colourCtl(int base) {
+  SweepCtl c;
+  c.syncFree = 0;
+  c.blockBase = base;
+  c.ticketBase = 0;
+  c.epoch = 0;
+  return c;
+}
+
+template <class T, int PRO>
+void launchL(phase_hip_solver *s, const IluDev<T> &L, const Scalars *sc, const T *in, T *y, T *pv, const T *r,
+             const T *v) {
+  if (s->syncFree) {
+    hipLaunchKernelGGL((kLSolve<T, PRO>), dim3(s->ord.nBlocks), dim3(64), 0, s->stream, L, syncDev(s), syncFreeCtl(s),
+                       sc, in, y, pv, r, v);
+    return;
+  }
+  const std::vector<int> &cp = s->ord.colourPtr;
+  for (int c = 0; c + 1 < (int)cp.size(); ++c)
+    if (cp[c + 1] > cp[c])
+      hipLaunchKernelGGL((kLSolve<T, PRO>), dim3(cp[c + 1] - cp[c]), dim3(64), 0, s->stream, L, syncDev(s),
+                         colourCtl(cp[c]), sc, in, y, pv, r, v);
+}
+
+template <class T>
+void launchU(phase_hip_solver *s, const IluDev<T> &L, const Scalars *sc, const T *y, T *x) {
+  if (s->syncFree) {
+    hipLaunchKernelGGL((kUSolve<T>), dim3(s->ord.nBlocks), dim3(64), 0, s->stream, L, syncDev(s), syncFreeCtl(s), sc,
+                       y, x);
+    return;
+  }
+  const std::vector<int> &cp = s->ord.colourPtr;
+  for (int c = (int)cp.size() - 2; c >= 0; --c)
+    if (cp[c + 1] > cp[c])
+      hipLaunchKernelGGL((kUSolve<T>), dim3(cp[c + 1] - cp[c]), dim3(64), 0, s->stream, L, syncDev(s),
+                         colourCtl(cp[c]), sc, y, x);
+}
+
+// out = U^{-1} L^{-1} (right-hand side formed by PRO); per-colour mode fuses
+// the last forward colour with the first backward colour.
+template <class T, int PRO>
+void launchLU(phase_hip_solver *s, const IluDev<T> &L, const Scalars *sc, const T *in, T *y, T *pv, const T *r,
+              const T *v, T *out) {
+  const std::vector<int> &cp = s->ord.colourPtr;
+  const int C = (int)cp.size() - 1;
+  if (s->syncFree || C < 1) {
+    launchL<T, PRO>(s, L, sc, in, y, pv, r, v);
+    launchU<T>(s, L, sc, y, out);
+    return;
+  }
+  for (int c = 0; c + 1 < C; ++c)
+    if (cp[c + 1] > cp[c])
+      hipLaunchKernelGGL((kLSolve<T, PRO>), dim3(cp[c + 1] - cp[c]), dim3(64), 0, s->stream, L, syncDev(s),
+                         colourCtl(cp[c]), sc, in, y, pv, r, v);
+  if (cp[C] > cp[C - 1])
+    hipLaunchKernelGGL((kLUSolve<T, PRO>), dim3(cp[C] - cp[C - 1]), dim3(64), 0, s->stream, L, syncDev(s),
+                       colourCtl(cp[C - 1]), sc, in, (const T *)y, pv, r, v, out);
+  for (int c = C - 2; c >= 0; --c)
+    if (cp[c + 1] > cp[c])
+      hipLaunchKernelGGL((kUSolve<T>), dim3(cp[c + 1] - cp[c]), dim3(64), 0, s->stream, L, syncDev(s),
+                         colourCtl(cp[c]), sc, (const T *)y, out);
+}
+
+template <class T>
+void launchFactor(phase_hip_solver *s, const IluDev<T> &L) {
+  if (s->syncFree) {
+    hipLaunchKernelGGL((kIluFactor<T>), dim3(s->ord.nBlocks), dim3(64), 0, s->stream, L, s->dUpdPtr.p, s->dUpdSrc.p,
+                       s->dUpdDst.p, syncDev(s), syncFreeCtl(s));
+    return;
+  }
+  const std::vector<int> &cp = s->ord.colourPtr;
+  for (int c = 0; c + 1 < (int)cp.size(); ++c)
+    if (cp[c + 1] > cp[c])
+      hipLaunchKernelGGL((kIluFactor<T>), dim3(cp[c + 1] - cp[c]), dim3(64), 0, s->stream, L, s->dUpdPtr.p,
+                         s->dUpdSrc.p, s->dUpdDst.p, syncDev(s), colourCtl(cp[c]));
 }
 
 int spmvGrid(phase_hip_solver *s) { return std::max(1, (s->ord.nSlices + 3) / 4); }
@@ -94,11 +177,7 @@ void factorize(phase_hip_solver *s, DBuf<T> &fac) {
   const int m = (int)fac.n;
   hipLaunchKernelGGL((kGather<T>), dim3(gridFor(m, 256, kElemCap)), dim3(256), 0, s->stream, m,
                      s->dFacSrc.p, s->dVals.p, fac.p);
-  unsigned long long base;
-  int ep;
-  nextSweep(s, base, ep);
-  hipLaunchKernelGGL((kIluFactor<T>), dim3(s->ord.nBlocks), dim3(64), 0, s->stream, iluDev<T>(s, fac.p),
-                     s->dUpdPtr.p, s->dUpdSrc.p, s->dUpdDst.p, syncDev(s), base, ep);
+  launchFactor<T>(s, iluDev<T>(s, fac.p));
   PH_CHECK(hipGetLastError());
 }
 
@@ -136,50 +215,33 @@ void ensureSingle(phase_hip_solver *s) {
 
 template <class T>
 void iluApply(phase_hip_solver *s, DBuf<T> &fac, const T *in, T *tmp, T *out, const Scalars *sc) {
-  unsigned long long base;
-  int ep;
-  nextSweep(s, base, ep);
-  hipLaunchKernelGGL((kLSolve<T, kProNone>), dim3(s->ord.nBlocks), dim3(64), 0, s->stream,
-                     iluDev<T>(s, fac.p), syncDev(s), base, ep, sc, in, tmp, (T *)nullptr,
-                     (const T *)nullptr, (const T *)nullptr);
-  nextSweep(s, base, ep);
-  hipLaunchKernelGGL((kUSolve<T>), dim3(s->ord.nBlocks), dim3(64), 0, s->stream, iluDev<T>(s, fac.p),
-                     syncDev(s), base, ep, sc, tmp, out);
+  IluDev<T> L = iluDev<T>(s, fac.p);
+  launchLU<T, kProNone>(s, L, sc, in, tmp, (T *)nullptr, (const T *)nullptr, (const T *)nullptr, out);
 }
 
 // One right-preconditioned BiCGStab iteration, all scalars on the device.
-template <class T>
-void iterate(phase_hip_solver *s, KrylovVecs<T> &K, DBuf<T> &fac, const T *sell, T *x, Scalars *sc) {
+template <class T, class TX>
+void iterate(phase_hip_solver *s, KrylovVecs<T> &K, DBuf<T> &fac, const T *sell, TX *x, Scalars *sc) {
   const int n = s->rank;
-  unsigned long long base;
-  int ep;
   IluDev<T> L = iluDev<T>(s, fac.p);
-  SyncDev sy = syncDev(s);
   // p = r + beta (p - omega v) ; ph = M^{-1} p
-  nextSweep(s, base, ep);
-  hipLaunchKernelGGL((kLSolve<T, kProP>), dim3(s->ord.nBlocks), dim3(64), 0, s->stream, L, sy, base, ep,
-                     sc, (const T *)nullptr, K.y.p, K.p.p, K.r.p, K.v.p);
-  nextSweep(s, base, ep);
-  hipLaunchKernelGGL((kUSolve<T>), dim3(s->ord.nBlocks), dim3(64), 0, s->stream, L, sy, base, ep, sc, K.y.p,
-                     K.ph.p);
+  launchLU<T, kProP>(s, L, sc, (const T *)nullptr, K.y.p, K.p.p, K.r.p, K.v.p, K.ph.p);
   // v = A ph ; alpha
-  hipLaunchKernelGGL((kSpmvDotV<T>), dim3(spmvGrid(s)), dim3(kRedBlock), 0, s->stream, n, s->ord.nSlices,
-                     s->dSliceOff.p, s->dSellCol.p, sell, K.ph.p, K.v.p, K.r0.p, sc, s->dPartials.p,
-                     s->dCounter.p);
+  const int gs = spmvGrid(s);
+  hipLaunchKernelGGL((kSpmvDotV<T>), dim3(gs), dim3(kRedBlock), 0, s->stream, n, s->ord.nSlices, s->dSliceOff.p,
+                     s->dSellCol.p, sell, K.ph.p, K.v.p, K.r0.p, sc, s->dPartials.p);
+  hipLaunchKernelGGL(kFinishV, dim3(1), dim3(kRedBlock), 0, s->stream, sc, s->dPartials.p, gs);
   // s = r - alpha v ; sh = M^{-1} s
-  nextSweep(s, base, ep);
-  hipLaunchKernelGGL((kLSolve<T, kProS>), dim3(s->ord.nBlocks), dim3(64), 0, s->stream, L, sy, base, ep,
-                     sc, (const T *)nullptr, K.y.p, K.s.p, K.r.p, K.v.p);
-  nextSweep(s, base, ep);
-  hipLaunchKernelGGL((kUSolve<T>), dim3(s->ord.nBlocks), dim3(64), 0, s->stream, L, sy, base, ep, sc, K.y.p,
-                     K.sh.p);
+  launchLU<T, kProS>(s, L, sc, (const T *)nullptr, K.y.p, K.s.p, K.r.p, K.v.p, K.sh.p);
   // t = A sh ; omega
-  hipLaunchKernelGGL((kSpmvDotT<T>), dim3(spmvGrid(s)), dim3(kRedBlock), 0, s->stream, n, s->ord.nSlices,
-                     s->dSliceOff.p, s->dSellCol.p, sell, K.sh.p, K.t.p, K.s.p, sc, s->dPartials.p,
-                     s->dCounter.p);
+  hipLaunchKernelGGL((kSpmvDotT<T>), dim3(gs), dim3(kRedBlock), 0, s->stream, n, s->ord.nSlices, s->dSliceOff.p,
+                     s->dSellCol.p, sell, K.sh.p, K.t.p, K.s.p, sc, s->dPartials.p);
+  hipLaunchKernelGGL(kFinishT, dim3(1), dim3(kRedBlock), 0, s->stream, sc, s->dPartials.p, gs);
   // x, r, rho, beta, convergence
-  hipLaunchKernelGGL((kUpdate<T>), dim3(gridFor(n, kRedBlock, kRedCap)), dim3(kRedBlock), 0, s->stream, n,
-                     sc, x, K.ph.p, K.sh.p, K.r.p, K.s.p, K.t.p, K.r0.p, s->dPartials.p, s->dCounter.p);
+  const int gu = gridFor(n, kRedBlock, kRedCap);
+  hipLaunchKernelGGL((kUpdate<T, TX>), dim3(gu), dim3(kRedBlock), 0, s->stream, n, sc, x, K.ph.p, K.sh.p, K.r.p,
+                     K.s.p, K.t.p, K.r0.p, s->dPartials.p);
+  hipLaunchKernelGGL(kFinishUpdate, dim3(1), dim3(kRedBlock), 0, s->stream, sc, s->dPartials.p, gu);
 }
 
 void pullScalars(phase_hip_solver *s) {
@@ -199,13 +261,13 @@ double normB2(phase_hip_solver *s) {
 
 // Run BiCGStab chunks until the device says done; returns iterations taken.
 // onChunk(iterationsSoFar) runs on the host after every chunk.
-template <class T, class OnChunk>
-int runKrylov(phase_hip_solver *s, KrylovVecs<T> &K, DBuf<T> &fac, const T *sell, T *x, int budget,
+template <class T, class TX, class OnChunk>
+int runKrylov(phase_hip_solver *s, KrylovVecs<T> &K, DBuf<T> &fac, const T *sell, TX *x, int budget,
               OnChunk &&onChunk) {
   Scalars *sc = s->dSc.p;  // slot 0
   int it = 0;
   while (true) {
-    for (int c = 0; c < s->chunk; ++c) iterate<T>(s, K, fac, sell, x, sc);
+    for (int c = 0; c < s->chunk; ++c) iterate<T, TX>(s, K, fac, sell, x, sc);
     PH_CHECK(hipGetLastError());
     pullScalars(s);
     it = s->hSc[0].iters;
@@ -250,7 +312,7 @@ void solveFp64(phase_hip_solver *s) {
       s->lastCaptureIter = 0;
     }
     const int base = total;
-    int it = runKrylov<double>(s, K, s->dFac64, s->dSell64.p, s->dX.p, s->maxIters - total, [&](int sofar) {
+    int it = runKrylov<double, double>(s, K, s->dFac64, s->dSell64.p, s->dX.p, s->maxIters - total, [&](int sofar) {
       if (capturing(s) && base + sofar - s->lastCaptureIter >= s->gnn->captureEvery) {
         gnnCaptureIter(s, K.r.p, s->dX.p, base + sofar);
         s->lastCaptureIter = base + sofar;
@@ -289,6 +351,12 @@ bool learnedStep(phase_hip_solver *s, double r2) {
   return true;
 }
 
+// Mixed precision: BiCGStab iterates in FP32 (SpMV, ILU(0), Krylov vectors)
+// while the solution accumulates in FP64; whenever the recursive residual has
+// dropped by `innerTolerance` since the last refinement point, r is replaced
+// by the FP64 true residual b - A x (a reliable update: FP64 iterative
+// refinement without restarting the Krylov recurrence).  Convergence is judged
+// on the FP64 true residual only.
 void solveMixed(phase_hip_solver *s) {
   const int n = s->rank;
   ensureSingle(s);
@@ -306,14 +374,17 @@ void solveMixed(phase_hip_solver *s) {
   bool useNet = s->mode == PHASE_HIP_MODE_LEARNED && s->gnn && s->gnn->trained;
   if (useNet) gnnPrepare(s);
   Scalars *inner = s->dSc.p, *outer = s->dSc.p + 1;
+  const double tol2 = s->tol * s->tol * b2;
+  const double delta2 = s->innerTol * s->innerTol;
   int total = 0;
   for (int pass = 0; pass <= s->maxOuter; ++pass) {
+    // FP64 true residual
     hipLaunchKernelGGL((kResidual<double>), dim3(spmvGrid(s)), dim3(kRedBlock), 0, s->stream, n,
                        s->ord.nSlices, s->dSliceOff.p, s->dSellCol.p, s->dSell64.p, s->dX.p, s->dB.p,
                        s->dR64.p, (double *)nullptr, outer, s->dPartials.p, s->dCounter.p);
     pullScalars(s);
     s->error = std::sqrt(s->hSc[1].rr / b2);
-    if (s->error <= s->tol || total >= s->maxIters || pass == s->maxOuter) break;
+    if (!(s->error > s->tol) || total >= s->maxIters || pass == s->maxOuter) break;
     ++s->outerPasses;
     if (useNet) {
       if (learnedStep(s, s->hSc[1].rr)) {
@@ -323,18 +394,30 @@ void solveMixed(phase_hip_solver *s) {
         ++s->fallbacks;
       }
     }
-    hipLaunchKernelGGL((kNormalise<float>), dim3(gridFor(n, 256, kElemCap)), dim3(256), 0, s->stream, n,
-                       s->dR64.p, K.r.p, K.r0.p, outer);
+    // restart the FP32 recurrence from the FP64 residual
+    hipLaunchKernelGGL((kConvert2<double, float>), dim3(gridFor(n, 256, kElemCap)), dim3(256), 0, s->stream, n,
+                       s->dR64.p, K.r.p, K.r0.p);
     hipLaunchKernelGGL((kNorm2<float>), dim3(gridFor(n, kRedBlock, kRedCap)), dim3(kRedBlock), 0, s->stream,
                        n, K.r.p, &inner->rr, s->dPartials.p, s->dCounter.p);
-    // stop the inner run at innerTol relative reduction, or at the FP64 goal
-    const double want = std::max(s->innerTol, 0.5 * s->tol / s->error);
     hipLaunchKernelGGL((kBicgInit<float>), dim3(gridFor(n, 256, kElemCap)), dim3(256), 0, s->stream, n, inner,
-                       K.p.p, K.v.p, K.x.p, want * want, s->maxIters - total, 0.0);
-    int it = runKrylov<float>(s, K, s->dFac32, s->dSell32.p, K.x.p, s->maxIters - total, [](int) {});
+                       K.p.p, K.v.p, (float *)nullptr, tol2, s->maxIters - total, 0.0);
+    pullScalars(s);
+    double lastRU = s->hSc[0].rr;
+    int it = runKrylov<float, double>(s, K, s->dFac32, s->dSell32.p, s->dX.p, s->maxIters - total, [&](int) {
+      const double rr = s->hSc[0].rr;
+      if (rr < delta2 * lastRU) {
+        // reliable update: r = b - A x in FP64, handed to the FP32 recurrence
+        hipLaunchKernelGGL((kResidual<double>), dim3(spmvGrid(s)), dim3(kRedBlock), 0, s->stream, n,
+                           s->ord.nSlices, s->dSliceOff.p, s->dSellCol.p, s->dSell64.p, s->dX.p, s->dB.p,
+                           s->dR64.p, (double *)nullptr, outer, s->dPartials.p, s->dCounter.p);
+        hipLaunchKernelGGL((kReplaceResidual<float>), dim3(gridFor(n, 256, kElemCap)), dim3(256), 0, s->stream,
+                           n, s->dR64.p, K.r.p, inner, outer);
+        pullScalars(s);
+        lastRU = s->hSc[1].rr;
+        ++s->outerPasses;
+      }
+    });
     total += it;
-    hipLaunchKernelGGL((kAccumulate<float>), dim3(gridFor(n, 256, kElemCap)), dim3(256), 0, s->stream, n,
-                       s->dX.p, K.x.p, outer);
     if (it == 0) break;
   }
   s->innerIters = total;
@@ -354,7 +437,7 @@ void allocCommon(phase_hip_solver *s) {
   s->dErr.alloc(1);
   s->dErr.zero(s->stream);
   const int maxGrid = std::max(spmvGrid(s), kRedCap);
-  s->dPartials.alloc(2 * (size_t)maxGrid);
+  s->dPartials.alloc(4 * (size_t)maxGrid);
   s->dCounter.alloc(1);
   s->dCounter.zero(s->stream);
   s->dSc.alloc(3);
@@ -400,6 +483,15 @@ void solverSetPattern(phase_hip_solver *s, int n, const int *rowPtr, const int *
 }
 
 void solverEnsureSingle(phase_hip_solver *s) { ensureSingle(s); }
+
+template <class T>
+void iluApplyDev(phase_hip_solver *s, T *fac, const T *in, T *tmp, T *out) {
+  IluDev<T> L = iluDev<T>(s, fac);
+  launchLU<T, kProNone>(s, L, (const Scalars *)nullptr, in, tmp, (T *)nullptr, (const T *)nullptr, (const T *)nullptr,
+                        out);
+}
+template void iluApplyDev<float>(phase_hip_solver *, float *, const float *, float *, float *);
+template void iluApplyDev<double>(phase_hip_solver *, double *, const double *, double *, double *);
 
 void solverValuesChanged(phase_hip_solver *s) {
   const int m = (int)s->dSell64.n;
@@ -494,6 +586,11 @@ int phase_hip_solver_setup(phase_hip_solver *s, const char *key, const char *val
     } else if (k == "fallbackRatio") s->fallbackRatio = std::stod(v);
     else if (k == "maxOuter") s->maxOuter = std::stoi(v);
     else if (k == "chunk") s->chunk = std::max(1, std::stoi(v));
+    else if (k == "sweep") {
+      if (lv == "colour" || lv == "color" || lv == "0") s->syncFree = 0;
+      else if (lv == "syncfree" || lv == "1") s->syncFree = 1;
+      else fail(PHASE_HIP_EINVAL, "unknown sweep '" + v + "'");
+    }
   });
 }
 
@@ -765,20 +862,12 @@ int phase_hip_solver_time_kernels(phase_hip_solver *s, int reps, double *ms, int
     const Scalars *noSc = nullptr;
     auto lsweep = [&](auto *fac, auto *in, auto *out) {
       using T = std::remove_pointer_t<decltype(in)>;
-      unsigned long long base;
-      int ep;
-      nextSweep(s, base, ep);
-      hipLaunchKernelGGL((kLSolve<T, kProNone>), dim3(s->ord.nBlocks), dim3(64), 0, s->stream, iluDev<T>(s, fac),
-                         syncDev(s), base, ep, noSc, (const T *)in, out, (T *)nullptr, (const T *)nullptr,
-                         (const T *)nullptr);
+      launchL<T, kProNone>(s, iluDev<T>(s, fac), noSc, (const T *)in, out, (T *)nullptr, (const T *)nullptr,
+                           (const T *)nullptr);
     };
     auto usweep = [&](auto *fac, auto *in, auto *out) {
       using T = std::remove_pointer_t<decltype(in)>;
-      unsigned long long base;
-      int ep;
-      nextSweep(s, base, ep);
-      hipLaunchKernelGGL((kUSolve<T>), dim3(s->ord.nBlocks), dim3(64), 0, s->stream, iluDev<T>(s, fac), syncDev(s),
-                         base, ep, noSc, (const T *)in, out);
+      launchU<T>(s, iluDev<T>(s, fac), noSc, (const T *)in, out);
     };
     double out[8];
     out[0] = timeIt([&] {

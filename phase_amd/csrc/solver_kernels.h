@@ -18,6 +18,7 @@
 namespace phase {
 
 constexpr int kMaxOffU = 8;  // off-block U entries of one row held in registers
+constexpr int kMaxIn = 8;    // in-block L / U entries of one row held in registers
 constexpr int kRedBlock = 256;
 
 enum { kProNone = 0, kProP = 1, kProS = 2 };
@@ -120,6 +121,55 @@ __device__ bool reduceLast(double (&v)[NV], double *partials, unsigned *counter)
   return true;
 }
 
+// Fence-free deterministic reduction, part 1: each block writes its
+// fixed-order sums to partials[q * gridDim.x + blockIdx.x].  A one-block
+// finisher kernel (launched next on the stream, so the kernel boundary orders
+// the writes) adds them up in index order.  Unlike reduceLast this needs no
+// device-scope fence per block.
+template <int NV>
+__device__ void blockPartials(double (&v)[NV], double *partials) {
+  __shared__ double sred[NV][kRedBlock / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    double s = waveSum(v[q]);
+    if (lane == 0) sred[q][wave] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      double s = 0.;
+      for (int w = 0; w < kRedBlock / 64; ++w) s += sred[q][w];
+      partials[q * gridDim.x + blockIdx.x] = s;
+    }
+  }
+}
+
+// Part 2 (one block of kRedBlock threads): totals of nb partial blocks in
+// index order; thread 0 holds them in v.
+template <int NV>
+__device__ void finishPartials(const double *partials, int nb, double (&v)[NV]) {
+  __shared__ double sred[NV][kRedBlock / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    double s = 0.;
+    for (int i = threadIdx.x; i < nb; i += kRedBlock) s += partials[q * nb + i];
+    s = waveSum(s);
+    if (lane == 0) sred[q][wave] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      double t = 0.;
+      for (int w = 0; w < kRedBlock / 64; ++w) t += sred[q][w];
+      v[q] = t;
+    }
+  }
+}
+
 // ---------------------------------------------------------------- data movement
 template <class T>
 __global__ void kGather(int m, const int *__restrict__ map, const double *__restrict__ src,
@@ -181,13 +231,13 @@ __global__ __launch_bounds__(kRedBlock) void kSpmv(int n, int nSlices, const int
   y[row] = sellRow(slice, lane, sliceOff, col, val, x);
 }
 
-// v = A ph ; alpha = rho / (r0, v)
+// v = A ph ; partials of (r0, v)   [kFinishV: alpha = rho / (r0, v)]
 template <class T>
 __global__ __launch_bounds__(kRedBlock) void kSpmvDotV(int n, int nSlices, const int *__restrict__ sliceOff,
                                                        const int *__restrict__ col, const T *__restrict__ val,
                                                        const T *__restrict__ ph, T *__restrict__ v,
-                                                       const T *__restrict__ r0, Scalars *sc,
-                                                       double *partials, unsigned *counter) {
+                                                       const T *__restrict__ r0, const Scalars *sc,
+                                                       double *partials) {
   if (sc->done) return;
   const int slice = blockIdx.x * (kRedBlock / 64) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
@@ -198,20 +248,27 @@ __global__ __launch_bounds__(kRedBlock) void kSpmvDotV(int n, int nSlices, const
     v[row] = y;
     d[0] = (double)r0[row] * (double)y;
   }
-  if (reduceLast<1>(d, partials, counter) && threadIdx.x == 0) {
+  blockPartials<1>(d, partials);
+}
+
+static __global__ __launch_bounds__(kRedBlock) void kFinishV(Scalars *sc, const double *partials, int nb) {
+  if (sc->done) return;
+  double d[1];
+  finishPartials<1>(partials, nb, d);
+  if (threadIdx.x == 0) {
     sc->rv = d[0];
-    if (d[0] == 0.) sc->done = kDoneBreakdown;
+    if (d[0] == 0. || !isfinite(d[0])) sc->done = kDoneBreakdown;
     else sc->alpha = sc->rho / d[0];
   }
 }
 
-// t = A sh ; omega = (t, s) / (t, t)
+// t = A sh ; partials of (t, s), (t, t)   [kFinishT: omega = (t, s) / (t, t)]
 template <class T>
 __global__ __launch_bounds__(kRedBlock) void kSpmvDotT(int n, int nSlices, const int *__restrict__ sliceOff,
                                                        const int *__restrict__ col, const T *__restrict__ val,
                                                        const T *__restrict__ sh, T *__restrict__ t,
-                                                       const T *__restrict__ s, Scalars *sc,
-                                                       double *partials, unsigned *counter) {
+                                                       const T *__restrict__ s, const Scalars *sc,
+                                                       double *partials) {
   if (sc->done) return;
   const int slice = blockIdx.x * (kRedBlock / 64) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
@@ -223,7 +280,14 @@ __global__ __launch_bounds__(kRedBlock) void kSpmvDotT(int n, int nSlices, const
     d[0] = (double)y * (double)s[row];
     d[1] = (double)y * (double)y;
   }
-  if (reduceLast<2>(d, partials, counter) && threadIdx.x == 0) {
+  blockPartials<2>(d, partials);
+}
+
+static __global__ __launch_bounds__(kRedBlock) void kFinishT(Scalars *sc, const double *partials, int nb) {
+  if (sc->done) return;
+  double d[2];
+  finishPartials<2>(partials, nb, d);
+  if (threadIdx.x == 0) {
     sc->ts = d[0];
     sc->tt = d[1];
     sc->omega = d[1] > 0. ? d[0] / d[1] : 0.;
@@ -284,34 +348,69 @@ __global__ void kBicgInit(int n, Scalars *sc, T *p, T *v, T *x, double tol2, int
   }
 }
 
-// x += alpha ph + omega sh ; r = s - omega t ; rho' = (r0, r) ; rr = (r, r)
-template <class T>
-__global__ __launch_bounds__(kRedBlock) void kUpdate(int n, Scalars *sc, T *__restrict__ x,
+// x += alpha ph + omega sh ; r = s - omega t ; partials of (r0, r), (r, r).
+// The solution x may be kept in a wider type (TX) than the Krylov vectors.
+template <class T, class TX>
+__global__ __launch_bounds__(kRedBlock) void kUpdate(int n, const Scalars *sc, TX *__restrict__ x,
                                                      const T *__restrict__ ph, const T *__restrict__ sh,
                                                      T *__restrict__ r, const T *__restrict__ s,
                                                      const T *__restrict__ t, const T *__restrict__ r0,
-                                                     double *partials, unsigned *counter) {
+                                                     double *partials) {
   if (sc->done) return;
-  const T al = (T)sc->alpha, om = (T)sc->omega;
+  const double alpha = sc->alpha, omega = sc->omega;
+  if (!isfinite(alpha) || !isfinite(omega)) return;  // kFinishUpdate flags the breakdown
+  const T om = (T)omega;
+  const TX alx = (TX)alpha, omx = (TX)omega;
   double d[2] = {0., 0.};
   for (int i = blockIdx.x * kRedBlock + threadIdx.x; i < n; i += gridDim.x * kRedBlock) {
-    x[i] = x[i] + (al * ph[i] + om * sh[i]);
+    const TX inc = alx * (TX)ph[i] + omx * (TX)sh[i];
+    if (isfinite((double)inc)) x[i] = x[i] + inc;
+    else d[1] = NAN;  // poisoned direction: reported as a breakdown, x untouched
     const T ri = s[i] - om * t[i];
     r[i] = ri;
     d[0] += (double)r0[i] * (double)ri;
     d[1] += (double)ri * (double)ri;
   }
-  if (reduceLast<2>(d, partials, counter) && threadIdx.x == 0) {
-    const double rhoNew = d[0], rr = d[1];
-    sc->rrPrev = sc->rr;
-    sc->rr = rr;
-    sc->beta = (rhoNew / sc->rho) * (sc->alpha / sc->omega);
-    sc->rho = rhoNew;
-    sc->iters += 1;
-    if (rr <= sc->tol2) sc->done = kDoneConverged;
-    else if (rhoNew == 0. || sc->omega == 0.) sc->done = kDoneBreakdown;
-    else if (sc->minRatio > 0. && rr > sc->minRatio * sc->minRatio * sc->rrPrev) sc->done = kDoneStagnation;
-    else if (sc->iters >= sc->maxIters) sc->done = kDoneMaxIters;
+  blockPartials<2>(d, partials);
+}
+
+// rho' = (r0, r), rr = (r, r), beta, stopping tests
+static __global__ __launch_bounds__(kRedBlock) void kFinishUpdate(Scalars *sc, const double *partials, int nb) {
+  if (sc->done) return;
+  double d[2];
+  finishPartials<2>(partials, nb, d);
+  if (threadIdx.x != 0) return;
+  if (!isfinite(sc->alpha) || !isfinite(sc->omega)) {
+    sc->done = kDoneBreakdown;
+    return;
+  }
+  const double rhoNew = d[0], rr = d[1];
+  sc->rrPrev = sc->rr;
+  sc->rr = rr;
+  sc->beta = (rhoNew / sc->rho) * (sc->alpha / sc->omega);
+  sc->rho = rhoNew;
+  sc->iters += 1;
+  if (!isfinite(rr) || !isfinite(rhoNew)) sc->done = kDoneBreakdown;
+  else if (rr <= sc->tol2) sc->done = kDoneConverged;
+  else if (rhoNew == 0. || sc->omega == 0.) sc->done = kDoneBreakdown;
+  else if (sc->minRatio > 0. && rr > sc->minRatio * sc->minRatio * sc->rrPrev) sc->done = kDoneStagnation;
+  else if (sc->iters >= sc->maxIters) sc->done = kDoneMaxIters;
+}
+
+// r32 = r64 (reliable update of the recursive residual); inner->rr = outer->rr
+template <class T>
+__global__ void kReplaceResidual(int n, const double *__restrict__ r64, T *__restrict__ r, Scalars *inner,
+                                 const Scalars *outer) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) r[i] = (T)r64[i];
+  if (blockIdx.x == 0 && threadIdx.x == 0) inner->rr = outer->rr;
+}
+
+template <class TI, class TO>
+__global__ void kConvert2(int n, const TI *__restrict__ in, TO *__restrict__ a, TO *__restrict__ b) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const TO v = (TO)in[i];
+    a[i] = v;
+    b[i] = v;
   }
 }
 
@@ -376,24 +475,56 @@ __global__ void kNormalise(int n, const double *__restrict__ r64, T *__restrict_
 }
 
 // ---------------------------------------------------------------- ILU(0)
+// How a sweep launch maps workgroups to ILU blocks.
+//  * per colour (syncFree == 0): one launch per block colour; blocks of one
+//    colour share no coupling, so block = blockBase + blockIdx.x needs no
+//    waiting -- kernel boundaries order the colours.
+//  * sync-free (syncFree == 1): one launch for the whole sweep; blocks take
+//    tickets in order and wait on completion flags of their predecessor
+//    blocks (device-scope release/acquire).
+struct SweepCtl {
+  int syncFree;
+  int blockBase;
+  unsigned long long ticketBase;
+  int epoch;
+};
+
+// Block of this workgroup, or -1 (sync-free tickets beyond the sweep).
+__device__ __forceinline__ int sweepBlock(const SweepCtl &c, const SyncDev &sy, int nb, bool reverse) {
+  if (!c.syncFree) return c.blockBase + (int)blockIdx.x;
+  const long long bi = takeTicket(sy.ticket, c.ticketBase);
+  if (bi < 0 || bi >= nb) return -1;
+  return reverse ? nb - 1 - (int)bi : (int)bi;
+}
+
+__device__ __forceinline__ void sweepWait(const SweepCtl &c, const SyncDev &sy, const int *list, int beg, int end) {
+  if (c.syncFree) {
+    if ((int)threadIdx.x < end - beg) waitFlag(&sy.flags[list[beg + threadIdx.x]], c.epoch, sy.err);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ void sweepPublish(const SweepCtl &c, const SyncDev &sy, int b) {
+  if (c.syncFree) {
+    __syncthreads();
+    if (threadIdx.x == 0) stRelease(&sy.flags[b], c.epoch);
+  }
+}
+
 // Factorisation, IKJ order per row (the oracle's orc_ilu0): for each L entry
 // (i,k) in ascending k: l = a_ik / u_kk, then a_ij -= l * u_kj over U(k) in
-// ascending j.  Rows of earlier blocks are final once their flag is set; rows
-// of the same block are final after their level.
+// ascending j.  Rows of earlier blocks are final (earlier colour launch, or
+// flag set); rows of the same block are final after their level.
 template <class T>
 __global__ __launch_bounds__(64) void kIluFactor(IluDev<T> L, const int *__restrict__ updPtr,
                                                  const int *__restrict__ updSrc,
-                                                 const int *__restrict__ updDst, SyncDev sy,
-                                                 unsigned long long base, int epoch) {
-  const long long bi = takeTicket(sy.ticket, base);
-  if (bi < 0 || bi >= L.nb) return;
-  const int b = (int)bi;
+                                                 const int *__restrict__ updDst, SyncDev sy, SweepCtl ctl) {
+  const int b = sweepBlock(ctl, sy, L.nb, false);
+  if (b < 0) return;
   const BlockInfo B = L.blocks[b];
   const int lane = threadIdx.x;
-  const int np = B.predEnd - B.predBeg;
-  if (lane < np) waitFlag(&sy.flags[L.pred[B.predBeg + lane]], epoch, sy.err);
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  __syncthreads();
+  sweepWait(ctl, sy, L.pred, B.predBeg, B.predEnd);
   const bool act = lane < B.nRows;
   const int row = B.rowStart + lane;
   const RowInfo ri = act ? L.rows[row] : RowInfo{};
@@ -415,28 +546,17 @@ __global__ __launch_bounds__(64) void kIluFactor(IluDev<T> L, const int *__restr
     }
     __syncthreads();
   }
-  __syncthreads();
-  if (lane == 0) stRelease(&sy.flags[b], epoch);
+  sweepPublish(ctl, sy, b);
 }
 
-// Forward sweep y = L^{-1} b (unit L).  PRO selects the fused BiCGStab
-// prologue that forms b on the fly: kProP: p = r + beta (p - omega v);
-// kProS: s = r - alpha v (both stored).
+// Forward sweep body for one block: returns y of the lane's row.  PRO selects
+// the fused BiCGStab prologue that forms the right-hand side on the fly:
+// kProP: p = r + beta (p - omega v);  kProS: s = r - alpha v (both stored).
 template <class T, int PRO>
-__global__ __launch_bounds__(64) void kLSolve(IluDev<T> L, SyncDev sy, unsigned long long base, int epoch,
-                                              const Scalars *sc, const T *__restrict__ in, T *__restrict__ y,
-                                              T *__restrict__ pv, const T *__restrict__ r,
-                                              const T *__restrict__ v) {
-  __shared__ T ylds[64];
-  const long long bi = takeTicket(sy.ticket, base);
-  if (bi < 0 || bi >= L.nb) return;
-  if (sc && sc->done) return;
-  const int b = (int)bi;
-  const BlockInfo B = L.blocks[b];
-  const int lane = threadIdx.x;
-  const bool act = lane < B.nRows;
-  const int row = B.rowStart + lane;
-  const RowInfo ri = act ? L.rows[row] : RowInfo{};
+__device__ __forceinline__ T lBody(const IluDev<T> &L, const SyncDev &sy, const SweepCtl &ctl, int b,
+                                   const BlockInfo &B, const RowInfo &ri, bool act, int row, int lane,
+                                   const Scalars *sc, const T *__restrict__ in, const T *__restrict__ y,
+                                   T *__restrict__ pv, const T *__restrict__ r, const T *__restrict__ v, T *ylds) {
   T acc = 0;
   if (act) {
     if (PRO == kProP) {
@@ -453,10 +573,21 @@ __global__ __launch_bounds__(64) void kLSolve(IluDev<T> L, SyncDev sy, unsigned 
       acc = in[row];
     }
   }
-  const int np = B.predEnd - B.predBeg;
-  if (lane < np) waitFlag(&sy.flags[L.pred[B.predBeg + lane]], epoch, sy.err);
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  __syncthreads();
+  // in-block entries into registers first: the level loop then touches LDS only
+  const int nIn = act ? ri.nL - ri.nLoff : 0;
+  T lv[kMaxIn];
+  int lc[kMaxIn];
+#pragma unroll
+  for (int q = 0; q < kMaxIn; ++q) {
+    lv[q] = 0;
+    lc[q] = 0;
+    if (q < nIn) {
+      const int s = B.lOff + ((ri.nLoff + q) << 6) + lane;
+      lv[q] = L.fac[s];
+      lc[q] = L.lCol[s] - B.rowStart;
+    }
+  }
+  sweepWait(ctl, sy, L.pred, B.predBeg, B.predEnd);
   if (act) {
     for (int e = 0; e < ri.nLoff; ++e) {
       const int s = B.lOff + (e << 6) + lane;
@@ -465,40 +596,38 @@ __global__ __launch_bounds__(64) void kLSolve(IluDev<T> L, SyncDev sy, unsigned 
   }
   for (int lev = 0; lev < B.nLev; ++lev) {
     if (act && ri.lev == lev) {
-      for (int e = ri.nLoff; e < ri.nL; ++e) {
-        const int s = B.lOff + (e << 6) + lane;
-        acc = fma(-L.fac[s], ylds[L.lCol[s] - B.rowStart], acc);
-      }
+#pragma unroll
+      for (int q = 0; q < kMaxIn; ++q)
+        if (q < nIn) acc = fma(-lv[q], ylds[lc[q]], acc);
       ylds[lane] = acc;
     }
     __syncthreads();
   }
-  if (act) y[row] = acc;
-  __syncthreads();
-  if (lane == 0) stRelease(&sy.flags[b], epoch);
+  return acc;
 }
 
-// Backward sweep x = U^{-1} y, blocks in reverse order.
+// Backward sweep body for one block, starting from acc = y of the lane's row;
+// returns x of the row.
 template <class T>
-__global__ __launch_bounds__(64) void kUSolve(IluDev<T> L, SyncDev sy, unsigned long long base, int epoch,
-                                              const Scalars *sc, const T *__restrict__ y, T *__restrict__ x) {
-  __shared__ T xlds[64];
-  const long long bi = takeTicket(sy.ticket, base);
-  if (bi < 0 || bi >= L.nb) return;
-  if (sc && sc->done) return;
-  const int b = L.nb - 1 - (int)bi;
-  const BlockInfo B = L.blocks[b];
-  const int lane = threadIdx.x;
-  const bool act = lane < B.nRows;
-  const int row = B.rowStart + lane;
-  const RowInfo ri = act ? L.rows[row] : RowInfo{};
+__device__ __forceinline__ T uBody(const IluDev<T> &L, const SyncDev &sy, const SweepCtl &ctl, const BlockInfo &B,
+                                   const RowInfo &ri, bool act, int row, int lane, T acc,
+                                   const T *__restrict__ x, T *xlds) {
   const T *facU = L.fac + L.nLSlots;
-  T acc = act ? y[row] : (T)0;
   const T d = act ? L.fac[L.nLSlots + L.nUSlots + row] : (T)1;
-  const int ns = B.succEnd - B.succBeg;
-  if (lane < ns) waitFlag(&sy.flags[L.succ[B.succBeg + lane]], epoch, sy.err);
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  __syncthreads();
+  const int nIn = act ? ri.nUin : 0;
+  T uv[kMaxIn];
+  int uc[kMaxIn];
+#pragma unroll
+  for (int q = 0; q < kMaxIn; ++q) {
+    uv[q] = 0;
+    uc[q] = 0;
+    if (q < nIn) {
+      const int s = B.uOff + (q << 6) + lane;
+      uv[q] = facU[s];
+      uc[q] = L.uCol[s] - B.rowStart;
+    }
+  }
+  sweepWait(ctl, sy, L.succ, B.succBeg, B.succEnd);
   const int nOff = act ? ri.nU - ri.nUin : 0;
   T ou[kMaxOffU], ox[kMaxOffU];
 #pragma unroll
@@ -513,10 +642,9 @@ __global__ __launch_bounds__(64) void kUSolve(IluDev<T> L, SyncDev sy, unsigned 
   }
   for (int lev = 0; lev < B.nULev; ++lev) {
     if (act && ri.ulev == lev) {
-      for (int e = 0; e < ri.nUin; ++e) {
-        const int s = B.uOff + (e << 6) + lane;
-        acc = fma(-facU[s], xlds[L.uCol[s] - B.rowStart], acc);
-      }
+#pragma unroll
+      for (int q = 0; q < kMaxIn; ++q)
+        if (q < nIn) acc = fma(-uv[q], xlds[uc[q]], acc);
 #pragma unroll
       for (int q = 0; q < kMaxOffU; ++q)
         if (q < nOff) acc = fma(-ou[q], ox[q], acc);
@@ -525,9 +653,66 @@ __global__ __launch_bounds__(64) void kUSolve(IluDev<T> L, SyncDev sy, unsigned 
     }
     __syncthreads();
   }
+  return acc;
+}
+
+// Forward sweep y = L^{-1} b (unit L).
+template <class T, int PRO>
+__global__ __launch_bounds__(64) void kLSolve(IluDev<T> L, SyncDev sy, SweepCtl ctl, const Scalars *sc,
+                                              const T *__restrict__ in, T *__restrict__ y, T *__restrict__ pv,
+                                              const T *__restrict__ r, const T *__restrict__ v) {
+  __shared__ T ylds[64];
+  const int b = sweepBlock(ctl, sy, L.nb, false);
+  if (b < 0) return;
+  if (sc && sc->done) return;
+  const BlockInfo B = L.blocks[b];
+  const int lane = threadIdx.x;
+  const bool act = lane < B.nRows;
+  const int row = B.rowStart + lane;
+  const RowInfo ri = act ? L.rows[row] : RowInfo{};
+  const T acc = lBody<T, PRO>(L, sy, ctl, b, B, ri, act, row, lane, sc, in, y, pv, r, v, ylds);
+  if (act) y[row] = acc;
+  sweepPublish(ctl, sy, b);
+}
+
+// Backward sweep x = U^{-1} y (colours in reverse order).
+template <class T>
+__global__ __launch_bounds__(64) void kUSolve(IluDev<T> L, SyncDev sy, SweepCtl ctl, const Scalars *sc,
+                                              const T *__restrict__ y, T *__restrict__ x) {
+  __shared__ T xlds[64];
+  const int b = sweepBlock(ctl, sy, L.nb, true);
+  if (b < 0) return;
+  if (sc && sc->done) return;
+  const BlockInfo B = L.blocks[b];
+  const int lane = threadIdx.x;
+  const bool act = lane < B.nRows;
+  const int row = B.rowStart + lane;
+  const RowInfo ri = act ? L.rows[row] : RowInfo{};
+  const T acc = uBody<T>(L, sy, ctl, B, ri, act, row, lane, act ? y[row] : (T)0, x, xlds);
   if (act) x[row] = acc;
-  __syncthreads();
-  if (lane == 0) stRelease(&sy.flags[b], epoch);
+  sweepPublish(ctl, sy, b);
+}
+
+// Last colour of a preconditioner apply (per-colour launches only): blocks of
+// the last colour have no later-coloured neighbours, so the forward and the
+// backward sweep of such a block run back to back in one launch.
+template <class T, int PRO>
+__global__ __launch_bounds__(64) void kLUSolve(IluDev<T> L, SyncDev sy, SweepCtl ctl, const Scalars *sc,
+                                               const T *__restrict__ in, const T *__restrict__ y,
+                                               T *__restrict__ pv, const T *__restrict__ r,
+                                               const T *__restrict__ v, T *__restrict__ x) {
+  __shared__ T ylds[64];
+  __shared__ T xlds[64];
+  const int b = ctl.blockBase + (int)blockIdx.x;
+  if (sc && sc->done) return;
+  const BlockInfo B = L.blocks[b];
+  const int lane = threadIdx.x;
+  const bool act = lane < B.nRows;
+  const int row = B.rowStart + lane;
+  const RowInfo ri = act ? L.rows[row] : RowInfo{};
+  const T yv = lBody<T, PRO>(L, sy, ctl, b, B, ri, act, row, lane, sc, in, y, pv, r, v, ylds);
+  const T acc = uBody<T>(L, sy, ctl, B, ri, act, row, lane, yv, x, xlds);
+  if (act) x[row] = acc;
 }
 
 }  // namespace phase

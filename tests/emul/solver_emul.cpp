@@ -94,6 +94,18 @@ extern "C" int emul_run(int n, const int *rowPtr, const int *colInd, const doubl
       REQ(s >= 0 && s < o.nUSlots, "U slot out of range");
       return s;
     };
+    // per-colour launches: every predecessor block of a block lies in an
+    // earlier colour, every successor in a later one
+    REQ((int)o.colourPtr.size() == o.nColours + 1 && o.colourPtr[0] == 0 && o.colourPtr[o.nColours] == nb,
+        "colourPtr");
+    std::vector<int> colourOf(nb, -1);
+    for (int c = 0; c < o.nColours; ++c)
+      for (int b = o.colourPtr[c]; b < o.colourPtr[c + 1]; ++b) colourOf[b] = c;
+    for (int b = 0; b < nb; ++b) {
+      const BlockInfo &B = o.blocks[b];
+      for (int q = B.predBeg; q < B.predEnd; ++q) REQ(colourOf[o.pred[q]] < colourOf[b], "pred not in an earlier colour");
+      for (int q = B.succBeg; q < B.succEnd; ++q) REQ(colourOf[o.succ[q]] > colourOf[b], "succ not in a later colour");
+    }
     std::vector<char> doneBlk(nb, 0), rowDone(n, 0);
     // --- factorisation (kIluFactor), blocks in ticket order
     for (int b = 0; b < nb; ++b) {

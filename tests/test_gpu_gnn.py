@@ -77,7 +77,8 @@ def test_forward_backward_match_fp64_oracle(hip, system):
     target = rng.standard_normal(n)
     Ah, sd, rowsum = NO.scaled_graph(A)
     X, sc = NO.features(r, z, sd, rowsum)
-    loss_ref, g_ref, out_ref = NO.loss_and_grad(Ah, X, p, target * sd / sc)
+    # the target is given in network units (out space)
+    loss_ref, g_ref, out_ref = NO.loss_and_grad(Ah, X, p, target)
     out, loss, grad = s.learned_eval(r, z, target)
     assert rel(out, out_ref) < 3e-2
     assert abs(loss - loss_ref) <= 5e-2 * loss_ref
@@ -100,9 +101,12 @@ def test_training_and_learned_mode(hip, system):
         s.solve()
     npairs = s.learned_n_pairs()
     assert npairs >= 30
-    first = s.learned_train(1, lr=1e-3, seed=1)
-    last = np.mean([s.learned_train(50, lr=1e-3, seed=t) for t in range(8)])
-    assert np.isfinite(last) and last < first
+    first = s.learned_train(40, lr=1e-3, seed=1)      # mean 1 - cos^2 over 40 steps
+    for t in range(8):
+        s.learned_train(50, lr=1e-3, seed=10 + t)
+    last = s.learned_train(40, lr=1e-3, seed=99)
+    assert 0.0 <= last <= 1.0 and np.isfinite(first)
+    assert last < first
     # learned mode reaches the same FP64 goal; the FP64 true residual is the judge
     A = H.compact_sorted(rp, ci, v)
     for mode in ("mixed", "learned"):
