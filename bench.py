@@ -78,6 +78,34 @@ def cpu_baseline(xy, cptr, cind, bc, g, b, iters_target):
                        % (t_asm, setup, 1e3 * per_iter, iters_target))
 
 
+def train_network(s, mesh, d_g, d_bc, d_pf, args):
+    """The learned part's setup, outside the timed steps: FP64 solves of earlier
+    'time steps' (right-hand sides from seeds 100, 101, ...) emit (residual,
+    correction) pairs; the network trains on them with the device kernels."""
+    from phase_amd import meshgen as G
+
+    t0 = time.perf_counter()
+    s.setup("mode", "fp64")
+    for t in range(args.capture_solves):
+        s.assemble_laplacian_device(mesh, 3, gamma_face=d_g, bc_type=d_bc, phi_face=d_pf)
+        if t == 0:
+            s.learned_capture(args.capture_every, 256)
+        s.sub_rhs(G.zero_mean_normal(mesh.n_cells, seed=100 + t))
+        s.zero_guess()
+        s.solve(raise_on_noconv=False)
+    s.learned_capture(0, 256)
+    pairs = s.learned_n_pairs()
+    t1 = time.perf_counter()
+    first = s.learned_train(50, lr=args.lr, seed=1)
+    loss = s.learned_train(args.train_steps, lr=args.lr, seed=2)
+    t2 = time.perf_counter()
+    s.setup("mode", "learned")
+    log("learned: %d pairs from %d FP64 solves (%.1fs), %d Adam steps (%.1fs), loss %.3f -> %.3f"
+        % (pairs, args.capture_solves, t1 - t0, args.train_steps + 50, t2 - t1, first, loss))
+    return dict(train_pairs=pairs, train_steps=args.train_steps + 50, capture_s=round(t1 - t0, 2),
+                train_s=round(t2 - t1, 2), loss_first=round(first, 4), loss_last=round(loss, 4))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,7 +117,11 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--inner-tol", type=float, default=0.1)
     ap.add_argument("--sweep", default="colour")
-    ap.add_argument("--modes", default="fp64,mixed")
+    ap.add_argument("--modes", default="fp64,mixed,learned")
+    ap.add_argument("--capture-solves", type=int, default=4)
+    ap.add_argument("--capture-every", type=int, default=100)
+    ap.add_argument("--train-steps", type=int, default=600)
+    ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=20)
     args = ap.parse_args()
@@ -126,6 +158,9 @@ def main():
             s.zero_guess()
             return s.solve(raise_on_noconv=False)
 
+        train_info = None
+        if mode == "learned":
+            train_info = train_network(s, mesh, d_g, d_bc, d_pf, args)
         for _ in range(args.warmup):
             step()
         t0 = time.perf_counter()
@@ -136,6 +171,9 @@ def main():
         st = s.stats()
         results[mode] = dict(ms_per_step=1e3 * dt, relres=max(rel), iters=st["inner_iters"],
                              passes=st["outer_passes"], solve_ms=st["solve_ms"])
+        if train_info is not None:
+            results[mode].update(train_info)
+            results[mode]["fallbacks"] = st["fallbacks"]
         log("%s: %.2f ms/step, relres %.2e, %d iterations, %d passes" % (mode, 1e3 * dt, max(rel),
                                                                           st["inner_iters"], st["outer_passes"]))
         if kernels is None:
@@ -154,10 +192,12 @@ def main():
     log("kernels (ms): " + ", ".join("%s %.4f" % (k, v) for k, v in kernels.items()))
     log("kernels (GB/s): " + ", ".join("%s %.0f" % (k, v) for k, v in gbs.items()))
 
-    head = "mixed" if "mixed" in results else list(results)[0]
+    # headline: the fastest method that reached the FP64 goal
+    ok = {k: v for k, v in results.items() if v["relres"] <= args.tol}
+    head = min(ok, key=lambda k: ok[k]["ms_per_step"]) if ok else list(results)[0]
     # dominant kernel of the headline solve: per BiCGStab iteration 2 forward
     # sweeps, 2 backward sweeps and 2 SpMVs of the inner precision
-    suf = "32" if head == "mixed" else "64"
+    suf = "64" if head == "fp64" else "32"
     per_iter = {k: 2 * kernels[k + suf] for k in ("lsweep", "usweep", "spmv")}
     dom = max(per_iter, key=per_iter.get) + suf
     roof = dict(kernel=dom, bound="hbm", achieved=round(gbs[dom], 1), peak=PEAK_HBM_GBS, unit="GB/s",
@@ -175,11 +215,12 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32-inner/f64-refine" if head == "mixed" else "f64",
+        "dtype": "f64" if head == "fp64" else "f32-inner/f64-refine",
         "data": "synthetic",
         "config": {"workload": "pcorr-poisson-delaunay-1M", "cells": n, "faces": mesh.n_faces, "nnz": nnz,
                    "sigma": args.sigma, "tol": args.tol, "method": head},
-        "modes": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+        "modes": {k: {kk: (None if isinstance(vv, float) and not np.isfinite(vv)
+                           else (round(vv, 4) if isinstance(vv, float) else vv)) for kk, vv in v.items()}
                   for k, v in results.items()},
         "speedup_vs_plain_fp64": (round(results["fp64"]["ms_per_step"] / results[head]["ms_per_step"], 3)
                                   if "fp64" in results else None),

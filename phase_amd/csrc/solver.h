@@ -58,7 +58,7 @@ struct phase_hip_solver {
   bool haveValues = false, sell32Valid = false, fac64Valid = false, fac32Valid = false;
 
   // right-hand side / solution, solver order
-  phase::DBuf<double> dB, dX, dR64;
+  phase::DBuf<double> dB, dX, dR64, dXBest;
   bool haveRhs = false, haveGuess = false, haveSolution = false;
   phase::KrylovVecs<double> k64;
   phase::KrylovVecs<float> k32;

@@ -220,21 +220,21 @@ void iluApply(phase_hip_solver *s, DBuf<T> &fac, const T *in, T *tmp, T *out, co
 }
 
 // One right-preconditioned BiCGStab iteration, all scalars on the device.
-template <class T, class TX>
-void iterate(phase_hip_solver *s, KrylovVecs<T> &K, DBuf<T> &fac, const T *sell, TX *x, Scalars *sc) {
+template <class T, class TX, class TV>
+void iterate(phase_hip_solver *s, KrylovVecs<T> &K, DBuf<T> &fac, const TV *sell, TX *x, Scalars *sc) {
   const int n = s->rank;
   IluDev<T> L = iluDev<T>(s, fac.p);
   // p = r + beta (p - omega v) ; ph = M^{-1} p
   launchLU<T, kProP>(s, L, sc, (const T *)nullptr, K.y.p, K.p.p, K.r.p, K.v.p, K.ph.p);
   // v = A ph ; alpha
   const int gs = spmvGrid(s);
-  hipLaunchKernelGGL((kSpmvDotV<T>), dim3(gs), dim3(kRedBlock), 0, s->stream, n, s->ord.nSlices, s->dSliceOff.p,
+  hipLaunchKernelGGL((kSpmvDotV<T, TV>), dim3(gs), dim3(kRedBlock), 0, s->stream, n, s->ord.nSlices, s->dSliceOff.p,
                      s->dSellCol.p, sell, K.ph.p, K.v.p, K.r0.p, sc, s->dPartials.p);
   hipLaunchKernelGGL(kFinishV, dim3(1), dim3(kRedBlock), 0, s->stream, sc, s->dPartials.p, gs);
   // s = r - alpha v ; sh = M^{-1} s
   launchLU<T, kProS>(s, L, sc, (const T *)nullptr, K.y.p, K.s.p, K.r.p, K.v.p, K.sh.p);
   // t = A sh ; omega
-  hipLaunchKernelGGL((kSpmvDotT<T>), dim3(gs), dim3(kRedBlock), 0, s->stream, n, s->ord.nSlices, s->dSliceOff.p,
+  hipLaunchKernelGGL((kSpmvDotT<T, TV>), dim3(gs), dim3(kRedBlock), 0, s->stream, n, s->ord.nSlices, s->dSliceOff.p,
                      s->dSellCol.p, sell, K.sh.p, K.t.p, K.s.p, sc, s->dPartials.p);
   hipLaunchKernelGGL(kFinishT, dim3(1), dim3(kRedBlock), 0, s->stream, sc, s->dPartials.p, gs);
   // x, r, rho, beta, convergence
@@ -260,19 +260,20 @@ double normB2(phase_hip_solver *s) {
 }
 
 // Run BiCGStab chunks until the device says done; returns iterations taken.
-// onChunk(iterationsSoFar) runs on the host after every chunk.
-template <class T, class TX, class OnChunk>
-int runKrylov(phase_hip_solver *s, KrylovVecs<T> &K, DBuf<T> &fac, const T *sell, TX *x, int budget,
+// onChunk(iterationsSoFar) runs on the host after every chunk and returns
+// false to stop the recurrence.
+template <class T, class TX, class TV, class OnChunk>
+int runKrylov(phase_hip_solver *s, KrylovVecs<T> &K, DBuf<T> &fac, const TV *sell, TX *x, int budget,
               OnChunk &&onChunk) {
   Scalars *sc = s->dSc.p;  // slot 0
   int it = 0;
   while (true) {
-    for (int c = 0; c < s->chunk; ++c) iterate<T, TX>(s, K, fac, sell, x, sc);
+    for (int c = 0; c < s->chunk; ++c) iterate<T, TX, TV>(s, K, fac, sell, x, sc);
     PH_CHECK(hipGetLastError());
     pullScalars(s);
     it = s->hSc[0].iters;
     if (s->hSc[0].done != kDoneRunning || it >= budget) break;
-    onChunk(it);
+    if (!onChunk(it)) break;
   }
   checkDeviceErr(s);
   return it;
@@ -280,22 +281,16 @@ int runKrylov(phase_hip_solver *s, KrylovVecs<T> &K, DBuf<T> &fac, const T *sell
 
 bool capturing(phase_hip_solver *s) { return s->gnn && s->gnn->captureEvery > 0; }
 
-void solveFp64(phase_hip_solver *s) {
+// FP64 BiCGStab + ILU(0) from the current x.  `total` iterations were already
+// spent by the caller (a mixed solve falling back); the loop restarts the
+// recurrence from the FP64 true residual after a breakdown.
+void fp64Loop(phase_hip_solver *s, double b2, int total) {
   const int n = s->rank;
   ensureFactor64(s);
   KrylovVecs<double> &K = s->k64;
   K.alloc(n);
-  const double b2 = normB2(s);
-  s->innerIters = 0;
-  s->outerPasses = 0;
-  if (b2 == 0.) {
-    PH_CHECK(hipMemsetAsync(s->dX.p, 0, n * sizeof(double), s->stream));
-    s->error = 0.;
-    return;
-  }
   const double tol2 = s->tol * s->tol * b2;
   Scalars *sc = s->dSc.p;
-  int total = 0;
   while (true) {
     // r = b - A x (true residual), r0 = r
     hipLaunchKernelGGL((kResidual<double>), dim3(spmvGrid(s)), dim3(kRedBlock), 0, s->stream, n,
@@ -306,23 +301,38 @@ void solveFp64(phase_hip_solver *s) {
     pullScalars(s);
     s->error = std::sqrt(s->hSc[0].rr / b2);
     ++s->outerPasses;
-    if (s->hSc[0].done == kDoneConverged || total >= s->maxIters) break;
+    if (s->hSc[0].done == kDoneConverged || total >= s->maxIters || !std::isfinite(s->error)) break;
     if (capturing(s) && total == 0) {
       gnnCaptureIter(s, K.r.p, s->dX.p, 0);
       s->lastCaptureIter = 0;
     }
     const int base = total;
-    int it = runKrylov<double, double>(s, K, s->dFac64, s->dSell64.p, s->dX.p, s->maxIters - total, [&](int sofar) {
+    int it = runKrylov<double, double, double>(s, K, s->dFac64, s->dSell64.p, s->dX.p, s->maxIters - total,
+                                               [&](int sofar) {
       if (capturing(s) && base + sofar - s->lastCaptureIter >= s->gnn->captureEvery) {
         gnnCaptureIter(s, K.r.p, s->dX.p, base + sofar);
         s->lastCaptureIter = base + sofar;
       }
+      return true;
     });
     total += it;
     if (s->hSc[0].done == kDoneBreakdown && it == 0) break;
   }
   s->innerIters = total;
   s->nIters = total;
+}
+
+void solveFp64(phase_hip_solver *s) {
+  const int n = s->rank;
+  const double b2 = normB2(s);
+  s->innerIters = 0;
+  s->outerPasses = 0;
+  if (b2 == 0.) {
+    PH_CHECK(hipMemsetAsync(s->dX.p, 0, n * sizeof(double), s->stream));
+    s->error = 0.;
+    return;
+  }
+  fp64Loop(s, b2, 0);
   if (capturing(s)) gnnCaptureFinish(s, s->dX.p, s->error <= s->tol);
 }
 
@@ -351,18 +361,23 @@ bool learnedStep(phase_hip_solver *s, double r2) {
   return true;
 }
 
-// Mixed precision: BiCGStab iterates in FP32 (SpMV, ILU(0), Krylov vectors)
-// while the solution accumulates in FP64; whenever the recursive residual has
-// dropped by `innerTolerance` since the last refinement point, r is replaced
-// by the FP64 true residual b - A x (a reliable update: FP64 iterative
-// refinement without restarting the Krylov recurrence).  Convergence is judged
-// on the FP64 true residual only.
+// Mixed precision: BiCGStab iterates with FP32 Krylov vectors and an FP32
+// ILU(0) factor, while the matrix coefficients stay FP64 (each SpMV row
+// accumulates in FP64 and rounds once), all dot products reduce in FP64 and
+// the solution accumulates in FP64.  Whenever the recursive residual has
+// dropped by `innerTolerance` since the last refinement point, r is replaced by
+// the FP64 true residual b - A x (a reliable update: FP64 iterative refinement
+// that keeps the Krylov recurrence).  Convergence is judged on the FP64 true
+// residual only.  The iterate with the smallest true residual is kept; if the
+// true residual grows a hundredfold past it or stops being finite, the solve
+// falls back to FP64 BiCGStab from that iterate.
 void solveMixed(phase_hip_solver *s) {
   const int n = s->rank;
   ensureSingle(s);
   KrylovVecs<float> &K = s->k32;
   K.alloc(n);
   s->dR64.alloc(n);
+  s->dXBest.alloc(n);
   const double b2 = normB2(s);
   s->innerIters = 0;
   s->outerPasses = 0;
@@ -376,18 +391,33 @@ void solveMixed(phase_hip_solver *s) {
   Scalars *inner = s->dSc.p, *outer = s->dSc.p + 1;
   const double tol2 = s->tol * s->tol * b2;
   const double delta2 = s->innerTol * s->innerTol;
-  int total = 0;
-  for (int pass = 0; pass <= s->maxOuter; ++pass) {
-    // FP64 true residual
+  double bestRR = HUGE_VAL;
+  bool diverged = false;
+  // true residual into dR64 / outer->rr; keeps the best iterate
+  auto trueResidual = [&]() {
     hipLaunchKernelGGL((kResidual<double>), dim3(spmvGrid(s)), dim3(kRedBlock), 0, s->stream, n,
                        s->ord.nSlices, s->dSliceOff.p, s->dSellCol.p, s->dSell64.p, s->dX.p, s->dB.p,
                        s->dR64.p, (double *)nullptr, outer, s->dPartials.p, s->dCounter.p);
     pullScalars(s);
-    s->error = std::sqrt(s->hSc[1].rr / b2);
-    if (!(s->error > s->tol) || total >= s->maxIters || pass == s->maxOuter) break;
+    const double rr = s->hSc[1].rr;
+    if (!(rr <= 1e4 * bestRR)) {  // NaN included
+      diverged = true;
+      return rr;
+    }
+    if (rr < bestRR) {
+      bestRR = rr;
+      PH_CHECK(hipMemcpyAsync(s->dXBest.p, s->dX.p, n * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+    }
+    return rr;
+  };
+  int total = 0;
+  for (int pass = 0; pass <= s->maxOuter; ++pass) {
+    const double rr = trueResidual();
+    s->error = std::sqrt(rr / b2);
+    if (diverged || rr <= tol2 || total >= s->maxIters || pass == s->maxOuter) break;
     ++s->outerPasses;
     if (useNet) {
-      if (learnedStep(s, s->hSc[1].rr)) {
+      if (learnedStep(s, rr)) {
         ++s->learnedSteps;
       } else {
         useNet = false;
@@ -403,22 +433,32 @@ void solveMixed(phase_hip_solver *s) {
                        K.p.p, K.v.p, (float *)nullptr, tol2, s->maxIters - total, 0.0);
     pullScalars(s);
     double lastRU = s->hSc[0].rr;
-    int it = runKrylov<float, double>(s, K, s->dFac32, s->dSell32.p, s->dX.p, s->maxIters - total, [&](int) {
-      const double rr = s->hSc[0].rr;
-      if (rr < delta2 * lastRU) {
-        // reliable update: r = b - A x in FP64, handed to the FP32 recurrence
-        hipLaunchKernelGGL((kResidual<double>), dim3(spmvGrid(s)), dim3(kRedBlock), 0, s->stream, n,
-                           s->ord.nSlices, s->dSliceOff.p, s->dSellCol.p, s->dSell64.p, s->dX.p, s->dB.p,
-                           s->dR64.p, (double *)nullptr, outer, s->dPartials.p, s->dCounter.p);
-        hipLaunchKernelGGL((kReplaceResidual<float>), dim3(gridFor(n, 256, kElemCap)), dim3(256), 0, s->stream,
-                           n, s->dR64.p, K.r.p, inner, outer);
-        pullScalars(s);
-        lastRU = s->hSc[1].rr;
-        ++s->outerPasses;
-      }
+    int it = runKrylov<float, double, double>(s, K, s->dFac32, s->dSell64.p, s->dX.p, s->maxIters - total,
+                                              [&](int) {
+      if (!(s->hSc[0].rr < delta2 * lastRU)) return true;
+      // reliable update: r = b - A x in FP64, handed to the FP32 recurrence
+      trueResidual();
+      if (diverged) return false;
+      hipLaunchKernelGGL((kReplaceResidual<float>), dim3(gridFor(n, 256, kElemCap)), dim3(256), 0, s->stream,
+                         n, s->dR64.p, K.r.p, inner, outer);
+      lastRU = s->hSc[1].rr;
+      ++s->outerPasses;
+      return true;
     });
     total += it;
-    if (it == 0) break;
+    if (diverged || it == 0) break;
+  }
+  if (diverged || !(s->error <= s->tol)) {
+    if ((diverged || total >= s->maxIters) && bestRR < HUGE_VAL) {
+      // restore the best iterate; finish in FP64 if the budget allows
+      PH_CHECK(hipMemcpyAsync(s->dX.p, s->dXBest.p, n * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+      s->error = std::sqrt(bestRR / b2);
+    }
+    if (diverged) {
+      ++s->fallbacks;
+      fp64Loop(s, b2, total);
+      return;
+    }
   }
   s->innerIters = total;
   s->nIters = total;

@@ -12,6 +12,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <type_traits>
 
 #include "phase_internal.h"
 
@@ -203,19 +204,23 @@ __global__ void kConvert(int n, const TI *__restrict__ in, TO *__restrict__ out)
 }
 
 // ---------------------------------------------------------------- SpMV (SELL-64)
-template <class T>
+// Row `slice*64+lane` of A x.  The matrix values may be wider than the vectors
+// (TV = double with T = float: FP64 coefficients, FP32 Krylov vectors); the
+// products then accumulate in the wider type and round once at the end.
+template <class T, class TV>
 __device__ __forceinline__ T sellRow(int slice, int lane, const int *__restrict__ sliceOff,
-                                     const int *__restrict__ col, const T *__restrict__ val,
+                                     const int *__restrict__ col, const TV *__restrict__ val,
                                      const T *__restrict__ x) {
+  using A = typename std::conditional<(sizeof(TV) > sizeof(T)), TV, T>::type;
   const int o = sliceOff[slice];
   const int K = (sliceOff[slice + 1] - o) >> 6;
-  T acc = 0;
+  A acc = 0;
   for (int e = 0; e < K; ++e) {
     const int idx = o + (e << 6) + lane;
     const int c = col[idx];
-    if (c >= 0) acc = fma(val[idx], x[c], acc);
+    if (c >= 0) acc = fma((A)val[idx], (A)x[c], acc);
   }
-  return acc;
+  return (T)acc;
 }
 
 // y = A x
@@ -232,9 +237,9 @@ __global__ __launch_bounds__(kRedBlock) void kSpmv(int n, int nSlices, const int
 }
 
 // v = A ph ; partials of (r0, v)   [kFinishV: alpha = rho / (r0, v)]
-template <class T>
+template <class T, class TV = T>
 __global__ __launch_bounds__(kRedBlock) void kSpmvDotV(int n, int nSlices, const int *__restrict__ sliceOff,
-                                                       const int *__restrict__ col, const T *__restrict__ val,
+                                                       const int *__restrict__ col, const TV *__restrict__ val,
                                                        const T *__restrict__ ph, T *__restrict__ v,
                                                        const T *__restrict__ r0, const Scalars *sc,
                                                        double *partials) {
@@ -263,9 +268,9 @@ static __global__ __launch_bounds__(kRedBlock) void kFinishV(Scalars *sc, const 
 }
 
 // t = A sh ; partials of (t, s), (t, t)   [kFinishT: omega = (t, s) / (t, t)]
-template <class T>
+template <class T, class TV = T>
 __global__ __launch_bounds__(kRedBlock) void kSpmvDotT(int n, int nSlices, const int *__restrict__ sliceOff,
-                                                       const int *__restrict__ col, const T *__restrict__ val,
+                                                       const int *__restrict__ col, const TV *__restrict__ val,
                                                        const T *__restrict__ sh, T *__restrict__ t,
                                                        const T *__restrict__ s, const Scalars *sc,
                                                        double *partials) {
