@@ -33,21 +33,19 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def clean(v):
+    """JSON-safe number: non-finite -> null, floats to 4 significant digits."""
+    if isinstance(v, float):
+        return float("%.4g" % v) if np.isfinite(v) else None
+    return v
+
+
 def build_problem(n_cells, k, sigma):
     from phase_amd import meshgen as G
 
     ni = G.delaunay_cells_for(n_cells, k)
     xy, cptr, cind = G.delaunay_square(k, ni, seed=1)
     return xy, cptr, cind
-
-
-def spmv_bytes(n, nnz, s):
-    return nnz * (s + 4) + (n + 1) * 4 + 2 * n * s
-
-
-def sweep_bytes(n, nnz_tri, s, diag):
-    # triangular part values+columns, row metadata, rhs in, result out (+ diagonal)
-    return nnz_tri * (s + 4) + (n + 1) * 4 + 2 * n * s + (n * s if diag else 0)
 
 
 def cpu_baseline(xy, cptr, cind, bc, g, b, iters_target):
@@ -171,38 +169,39 @@ def main():
         st = s.stats()
         results[mode] = dict(ms_per_step=1e3 * dt, relres=max(rel), iters=st["inner_iters"],
                              passes=st["outer_passes"], solve_ms=st["solve_ms"])
+        results[mode]["fallbacks"] = st["fallbacks"]
         if train_info is not None:
             results[mode].update(train_info)
-            results[mode]["fallbacks"] = st["fallbacks"]
         log("%s: %.2f ms/step, relres %.2e, %d iterations, %d passes" % (mode, 1e3 * dt, max(rel),
                                                                           st["inner_iters"], st["outer_passes"]))
         if kernels is None:
             kernels = s.time_kernels(args.kernel_reps)
+            kbytes = s.kernel_bytes()
+            ordinfo = s.ordering_info()
             nnz = len(s.export()[1])
         s.close()
 
     n = mesh.n_cells
-    nnz_tri = (nnz - n) // 2
-    kb = {
-        "spmv64": spmv_bytes(n, nnz, 8), "spmv32": spmv_bytes(n, nnz, 4),
-        "lsweep64": sweep_bytes(n, nnz_tri, 8, False), "usweep64": sweep_bytes(n, nnz_tri, 8, True),
-        "lsweep32": sweep_bytes(n, nnz_tri, 4, False), "usweep32": sweep_bytes(n, nnz_tri, 4, True),
-    }
-    gbs = {k: kb[k] / (kernels[k] * 1e-3) / 1e9 for k in kb}
+    gbs = {k: kbytes[k] / (kernels[k] * 1e-3) / 1e9 for k in kernels if kernels[k] > 0}
     log("kernels (ms): " + ", ".join("%s %.4f" % (k, v) for k, v in kernels.items()))
     log("kernels (GB/s): " + ", ".join("%s %.0f" % (k, v) for k, v in gbs.items()))
 
     # headline: the fastest method that reached the FP64 goal
     ok = {k: v for k, v in results.items() if v["relres"] <= args.tol}
     head = min(ok, key=lambda k: ok[k]["ms_per_step"]) if ok else list(results)[0]
-    # dominant kernel of the headline solve: per BiCGStab iteration 2 forward
-    # sweeps, 2 backward sweeps and 2 SpMVs of the inner precision
-    suf = "64" if head == "fp64" else "32"
-    per_iter = {k: 2 * kernels[k + suf] for k in ("lsweep", "usweep", "spmv")}
-    dom = max(per_iter, key=per_iter.get) + suf
-    roof = dict(kernel=dom, bound="hbm", achieved=round(gbs[dom], 1), peak=PEAK_HBM_GBS, unit="GB/s",
-                frac=round(gbs[dom] / PEAK_HBM_GBS, 4), traffic=None, algorithmic_bytes=kb[dom],
-                avg_ms=kernels[dom])
+    # dominant kernel of an FP64 BiCGStab iteration: two preconditioner applies
+    # (C-1 kLSolve + 1 kLUSolve + C-1 kUSolve launches each) and two SpMVs
+    C = ordinfo["colours"]
+    share = {"usolve64_launch": 2 * (C - 1) * kernels["usolve64_launch"],
+             "lsolve64_launch": 2 * (C - 1) * kernels["lsolve64_launch"],
+             "lusolve64_launch": 2 * kernels["lusolve64_launch"],
+             "spmv64": 2 * kernels["spmv64"]}
+    dom = max(share, key=share.get)
+    names = {"usolve64_launch": "kUSolve<double>", "lsolve64_launch": "kLSolve<double>",
+             "lusolve64_launch": "kLUSolve<double>", "spmv64": "kSpmvDotV/T<double>"}
+    roof = dict(kernel=names[dom], bound="hbm", achieved=round(gbs[dom], 1), peak=PEAK_HBM_GBS, unit="GB/s",
+                frac=round(gbs[dom] / PEAK_HBM_GBS, 4), traffic=None, algorithmic_bytes=round(kbytes[dom]),
+                avg_launch_ms=round(kernels[dom], 5))
 
     out = {
         "metric": METRIC,
@@ -219,13 +218,13 @@ def main():
         "data": "synthetic",
         "config": {"workload": "pcorr-poisson-delaunay-1M", "cells": n, "faces": mesh.n_faces, "nnz": nnz,
                    "sigma": args.sigma, "tol": args.tol, "method": head},
-        "modes": {k: {kk: (None if isinstance(vv, float) and not np.isfinite(vv)
-                           else (round(vv, 4) if isinstance(vv, float) else vv)) for kk, vv in v.items()}
-                  for k, v in results.items()},
+        "modes": {k: {kk: clean(vv) for kk, vv in v.items()} for k, v in results.items()},
         "speedup_vs_plain_fp64": (round(results["fp64"]["ms_per_step"] / results[head]["ms_per_step"], 3)
                                   if "fp64" in results else None),
         "spmv_gbs": {"fp64": round(gbs["spmv64"], 1), "fp32": round(gbs["spmv32"], 1)},
         "kernel_ms": {k: round(v, 5) for k, v in kernels.items()},
+        "kernel_gbs": {k: round(v, 1) for k, v in gbs.items()},
+        "ordering": ordinfo,
         "roofline": roof,
     }
     if not args.no_cpu_baseline:

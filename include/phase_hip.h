@@ -130,6 +130,11 @@ double phase_hip_solver_error(const phase_hip_solver *s);
 int phase_hip_solver_stats(const phase_hip_solver *s, int *innerIters, int *outerPasses,
                            int *fallbacks, double *solveMs);
 
+/* Shape of the device ordering of the current pattern: ILU blocks (<= 64 rows,
+ * one wavefront each) and block colours (= sweep launches per triangular
+ * solve).  No counterpart in the reference. */
+int phase_hip_solver_ordering_info(const phase_hip_solver *s, int *nBlocks, int *nColours);
+
 /* Export the system held on the device, in the ORIGINAL row/column numbering
  * and compact CSR form (empty slots dropped), each row in the solver's slot
  * order.  nnz is queried with rowPtr == NULL.  Used for parity checks. */
@@ -148,12 +153,21 @@ int phase_hip_solver_ilu_export(phase_hip_solver *s, int *nnz, int *perm, int *r
                                 int *colInd, double *vals);
 
 /* Timing hook for the benchmark: runs each kernel `reps` times on the
- * resident system and writes the mean device time of one launch (ms, HIP
- * events on the handle's stream) to ms[0..nms): 0 SpMV FP64, 1 SpMV FP32,
- * 2 ILU forward sweep FP64, 3 backward sweep FP64, 4 forward FP32,
+ * resident system and writes the mean device time (ms, HIP events on the
+ * handle's stream) to ms[0..nms): 0 SpMV FP64, 1 SpMV FP32, 2 ILU forward
+ * sweep FP64 (all colours), 3 backward sweep FP64, 4 forward FP32,
  * 5 backward FP32, 6 ILU(0) factorisation FP64 (value gather + factor),
- * 7 factorisation FP32. */
+ * 7 factorisation FP32, 8 one backward-sweep launch (kUSolve, FP64; mean over
+ * the colours a solve launches it for), 9 the fused last-colour launch
+ * (kLUSolve, FP64), 10 one forward-sweep launch (kLSolve, FP64, mean as 8).
+ * Benchmark support; no counterpart in the reference. */
 int phase_hip_solver_time_kernels(phase_hip_solver *s, int reps, double *ms, int nms);
+
+/* Algorithmic bytes of the entries of phase_hip_solver_time_kernels: every
+ * matrix or factor entry touched (value + column index), every vector element
+ * read or written once, one gathered value per off-block factor entry, and the
+ * block / lane records the sweeps read.  Benchmark support. */
+int phase_hip_solver_kernel_bytes(phase_hip_solver *s, double *bytes, int nb);
 
 /* ------------------------------------------------------------------ learned part
  * No counterpart in the reference.  A message-passing network on the matrix

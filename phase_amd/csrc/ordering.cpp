@@ -19,6 +19,8 @@
 // its own block, so one wavefront per block can run the in-block recurrence by
 // levels while blocks wait for their (few) predecessor blocks.
 #include <algorithm>
+#include <climits>
+#include <random>
 #include <cstring>
 #include <numeric>
 #include <queue>
@@ -127,6 +129,135 @@ std::vector<int> rcm(const Graph &g, int n) {
 
 }  // namespace
 
+// Remove the top colour k-1 by Kempe-chain interchanges (the recolouring step
+// of Kempe's 1879 argument): a block b of colour k-1 takes colour a < k-1 once
+// the (a, c)-chains through its a-coloured neighbours are swapped, provided no
+// such chain also reaches a c-coloured neighbour of b.  Returns the number of
+// blocks left with colour k-1 (stopping once more than maxLeft are); `col` is a
+// proper colouring either way.
+static int kempeRecolour(const std::vector<std::vector<int>> &adj, int k, std::vector<int> &col, int maxLeft) {
+  const int nb = (int)adj.size();
+  std::vector<int> mark(nb, -1), chain, stack;
+  int stamp = 0;
+  int left = 0;
+  for (int b = 0; b < nb; ++b) {
+    if (col[b] != k - 1) continue;
+    bool done = false;
+    for (int a = 0; a < k - 1 && !done; ++a) {
+      bool freeA = true;
+      for (int w : adj[b]) freeA = freeA && col[w] != a;
+      if (freeA) {
+        col[b] = a;
+        done = true;
+        break;
+      }
+      for (int c = 0; c < k - 1 && !done; ++c) {
+        if (c == a) continue;
+        // union of the (a, c)-chains through b's a-coloured neighbours
+        ++stamp;
+        chain.clear();
+        for (int w : adj[b]) {
+          if (col[w] != a || mark[w] == stamp) continue;
+          mark[w] = stamp;
+          stack.assign(1, w);
+          while (!stack.empty()) {
+            const int u = stack.back();
+            stack.pop_back();
+            chain.push_back(u);
+            for (int x : adj[u])
+              if (x != b && mark[x] != stamp && (col[x] == a || col[x] == c)) {
+                mark[x] = stamp;
+                stack.push_back(x);
+              }
+          }
+        }
+        bool ok = true;
+        for (int w : adj[b]) ok = ok && !(col[w] == c && mark[w] == stamp);
+        if (!ok) continue;
+        for (int u : chain) col[u] = col[u] == a ? c : a;
+        col[b] = a;
+        done = true;
+      }
+    }
+    left += done ? 0 : 1;
+    if (left > maxLeft) break;  // hopeless: stop early
+  }
+  return left;
+}
+
+// Tabucol (Hertz & de Werra 1987): look for a proper colouring with colours
+// 0..k-1 starting from `col` (entries >= k first move to their least-conflicting
+// colour).  A move recolours one conflicting vertex to the colour that lowers
+// the conflict count most; the reverse move is tabu for 0.6 * conflicts +
+// U{0..9} iterations unless it reaches zero conflicts.  Deterministic (fixed
+// seed).  Returns true with the colouring in `col` on success within maxIters.
+static bool tabuRecolour(const std::vector<std::vector<int>> &adj, int k, std::vector<int> &col, long maxIters) {
+  const int nb = (int)adj.size();
+  std::vector<int> gam((size_t)nb * k, 0);  // gam[v*k+c]: neighbours of v with colour c
+  for (int v = 0; v < nb; ++v)
+    if (col[v] >= k) col[v] = -1;
+  for (int v = 0; v < nb; ++v) {
+    if (col[v] >= 0) continue;
+    std::vector<int> cnt(k, 0);
+    for (int w : adj[v])
+      if (col[w] >= 0) ++cnt[col[w]];
+    col[v] = (int)(std::min_element(cnt.begin(), cnt.end()) - cnt.begin());
+  }
+  long f = 0;
+  for (int v = 0; v < nb; ++v)
+    for (int w : adj[v]) {
+      ++gam[(size_t)v * k + col[w]];
+      if (w > v && col[w] == col[v]) ++f;
+    }
+  std::vector<int> conf, pos(nb, -1);  // conflicting vertices, as an indexed set
+  auto upd = [&](int v) {
+    const bool in = gam[(size_t)v * k + col[v]] > 0;
+    if (in && pos[v] < 0) {
+      pos[v] = (int)conf.size();
+      conf.push_back(v);
+    } else if (!in && pos[v] >= 0) {
+      const int last = conf.back();
+      conf[pos[v]] = last;
+      pos[last] = pos[v];
+      conf.pop_back();
+      pos[v] = -1;
+    }
+  };
+  for (int v = 0; v < nb; ++v) upd(v);
+  std::vector<long> tabu((size_t)nb * k, 0);
+  std::mt19937 rng(12345);
+  for (long it = 0; it < maxIters && f > 0; ++it) {
+    int bv = -1, bc = -1;
+    long bd = LONG_MAX;
+    for (int v : conf) {
+      const int cv = col[v];
+      const int own = gam[(size_t)v * k + cv];
+      for (int c = 0; c < k; ++c) {
+        if (c == cv) continue;
+        const long d = gam[(size_t)v * k + c] - own;
+        if (tabu[(size_t)v * k + c] > it && f + d > 0) continue;
+        if (d < bd) {
+          bd = d;
+          bv = v;
+          bc = c;
+        }
+      }
+    }
+    if (bv < 0) continue;  // every move tabu: wait for tenures to expire
+    const int old = col[bv];
+    col[bv] = bc;
+    f += bd;
+    tabu[(size_t)bv * k + old] = it + (long)(0.6 * f) + (long)(rng() % 10);
+    for (int w : adj[bv]) {
+      --gam[(size_t)w * k + old];
+      ++gam[(size_t)w * k + bc];
+      upd(w);
+    }
+    upd(bv);
+  }
+  return f == 0;
+}
+
 void buildOrdering(Ordering &o, int n, const int *rowPtr, const int *colInd, int blockRows) {
   if (n <= 0) fail(PHASE_HIP_EINVAL, "rank must be positive");
   if (blockRows < 1 || blockRows > kLanes) fail(PHASE_HIP_EINVAL, "blockRows must be in [1,64]");
@@ -222,6 +353,17 @@ void buildOrdering(Ordering &o, int n, const int *rowPtr, const int *colInd, int
       heap.push(Key(satur[ob], (int)badj[ob].size(), -ob));
     }
   }
+  // 4b. fewer colours where Kempe-chain interchanges, finished by a bounded
+  //     Tabucol search for the blocks they leave, find them; each colour
+  //     removed saves two launches per preconditioner apply.
+  while (nColours > 2) {
+    std::vector<int> trial = colour;
+    const int maxLeft = std::max(16, nb / 1000);
+    const int left = kempeRecolour(badj, nColours, trial, maxLeft);
+    if (left > 0 && (left > maxLeft || !tabuRecolour(badj, nColours - 1, trial, 100000))) break;
+    colour.swap(trial);
+    --nColours;
+  }
   o.nColours = nColours;
 
   // 5. solver order
@@ -310,6 +452,7 @@ void buildOrdering(Ordering &o, int n, const int *rowPtr, const int *colInd, int
         fail(PHASE_HIP_EINVAL, "row couples to more than 8 rows of later ILU blocks");
       if (nL - nLoff > 8 || nUin > 8)
         fail(PHASE_HIP_EINVAL, "row couples to more than 8 rows of its own ILU block");
+      if (nLoff > 63) fail(PHASE_HIP_EINVAL, "row couples to more than 63 rows of earlier ILU blocks");
       o.rows[k].nL = (uint8_t)nL;
       o.rows[k].nLoff = (uint8_t)nLoff;
       o.rows[k].nU = (uint8_t)nU;
@@ -337,6 +480,16 @@ void buildOrdering(Ordering &o, int n, const int *rowPtr, const int *colInd, int
   }
   o.nLSlots = lRun;
   o.nUSlots = uRun;
+  o.lanes.assign((size_t)nb * kLanes, LaneRec{-1, 0u});
+  for (int b = 0; b < nb; ++b) {
+    const BlockInfo &B = o.blocks[b];
+    for (int l = 0; l < B.nRows; ++l) {
+      const int k = B.rowStart + l;
+      const RowInfo &r = o.rows[k];
+      o.lanes[(size_t)b * kLanes + l] =
+          LaneRec{k, packLane(r.nL - r.nLoff, r.nLoff, r.nUin, r.nU - r.nUin, r.lev, r.ulev)};
+    }
+  }
   o.lCol.assign(lRun, -1);
   o.uCol.assign(uRun, -1);
   o.facSrc.assign((size_t)lRun + uRun + n, -1);

@@ -80,6 +80,20 @@ struct RowInfo {
   uint8_t lev, ulev, pad0, pad1;
 };
 
+// Per-lane sweep record, indexed by block * 64 + lane so that a sweep reads it
+// in parallel with the block record.  row = solver row (-1: idle lane); info
+// packs the counts and in-block levels of the row:
+//   bits 0-3 in-block L, 4-9 off-block L, 10-13 in-block U, 14-17 off-block U,
+//   18-23 forward level, 24-29 backward level.
+struct LaneRec {
+  int row;
+  unsigned info;
+};
+inline __host__ __device__ unsigned packLane(int nLin, int nLoff, int nUin, int nUoff, int lev, int ulev) {
+  return (unsigned)nLin | ((unsigned)nLoff << 4) | ((unsigned)nUin << 10) | ((unsigned)nUoff << 14) |
+         ((unsigned)lev << 18) | ((unsigned)ulev << 24);
+}
+
 // Host-built ordering of one sparsity pattern.  "compact" = the caller's CSR
 // with empty slots dropped, original numbering and in-row order.
 struct Ordering {
@@ -93,6 +107,7 @@ struct Ordering {
   std::vector<BlockInfo> blocks;
   std::vector<int> pred, succ;
   std::vector<RowInfo> rows;          // per solver row
+  std::vector<LaneRec> lanes;         // per block * 64 + lane
   // SELL-64 SpMV layout over solver rows: slice s covers rows [64s, 64s+64)
   int nSlices = 0;
   std::vector<int> sliceOff;          // nSlices+1, in units of slots

@@ -48,6 +48,7 @@ struct phase_hip_solver {
 
   phase::DBuf<phase::BlockInfo> dBlocks;
   phase::DBuf<phase::RowInfo> dRows;
+  phase::DBuf<phase::LaneRec> dLanes;
   phase::DBuf<int> dPred, dSucc, dLCol, dUCol, dUpdPtr, dUpdSrc, dUpdDst;
   phase::DBuf<int> dSliceOff, dSellCol, dSellSrc, dFacSrc, dPerm;
 

@@ -47,11 +47,15 @@ int guarded(phase_hip_solver *s, F &&f) {
 
 void setDevice(phase_hip_solver *s) { PH_CHECK(hipSetDevice(s->device)); }
 
+// entries of phase_hip_solver_time_kernels / phase_hip_solver_kernel_bytes
+constexpr int kTimedKernels = 11;
+
 template <class T>
 IluDev<T> iluDev(phase_hip_solver *s, T *fac) {
   IluDev<T> L;
   L.blocks = s->dBlocks.p;
   L.rows = s->dRows.p;
+  L.lanes = s->dLanes.p;
   L.pred = s->dPred.p;
   L.succ = s->dSucc.p;
   L.lCol = s->dLCol.p;
@@ -63,6 +67,9 @@ IluDev<T> iluDev(phase_hip_solver *s, T *fac) {
   L.n = s->rank;
   return L;
 }
+
+// Scalars slot that is never set done: sweeps run outside a Krylov iteration
+const Scalars *idleScalars(phase_hip_solver *s) { return s->dSc.p + 3; }
 
 SyncDev syncDev(phase_hip_solver *s) { return SyncDev{s->dFlags.p, s->dTicket.p, s->dErr.p}; }
 
@@ -206,9 +213,15 @@ void ensureSingle(phase_hip_solver *s) {
     s->sell32Valid = true;
   }
   if (!s->fac32Valid) {
-    s->dFac32.alloc((size_t)s->ord.nLSlots + s->ord.nUSlots + s->rank);
-    factorize<float>(s, s->dFac32);
-    checkDeviceErr(s);
+    // factor in FP64, store and apply in FP32: rounding the finished factor
+    // keeps the FP64 pivots, where an FP32 factorisation of an ill-conditioned
+    // matrix loses them to cancellation
+    ensureFactor64(s);
+    const int m = (int)s->dFac64.n;
+    s->dFac32.alloc(m);
+    hipLaunchKernelGGL((kConvert<double, float>), dim3(gridFor(m, 256, kElemCap)), dim3(256), 0, s->stream, m,
+                       s->dFac64.p, s->dFac32.p);
+    PH_CHECK(hipGetLastError());
     s->fac32Valid = true;
   }
 }
@@ -480,7 +493,7 @@ void allocCommon(phase_hip_solver *s) {
   s->dPartials.alloc(4 * (size_t)maxGrid);
   s->dCounter.alloc(1);
   s->dCounter.zero(s->stream);
-  s->dSc.alloc(3);
+  s->dSc.alloc(4);  // [3]: idle slot, done == 0, for sweeps outside a Krylov run
   s->dSc.zero(s->stream);
   s->dMisc.alloc(4);
   if (!s->hSc) PH_CHECK(hipHostMalloc((void **)&s->hSc, 3 * sizeof(Scalars), hipHostMallocDefault));
@@ -499,6 +512,7 @@ void solverSetPattern(phase_hip_solver *s, int n, const int *rowPtr, const int *
   hipStream_t st = s->stream;
   s->dBlocks.upload(o.blocks, st);
   s->dRows.upload(o.rows, st);
+  s->dLanes.upload(o.lanes, st);
   s->dPred.upload(o.pred.empty() ? std::vector<int>{0} : o.pred, st);
   s->dSucc.upload(o.succ.empty() ? std::vector<int>{0} : o.succ, st);
   s->dLCol.upload(o.lCol.empty() ? std::vector<int>{-1} : o.lCol, st);
@@ -527,7 +541,7 @@ void solverEnsureSingle(phase_hip_solver *s) { ensureSingle(s); }
 template <class T>
 void iluApplyDev(phase_hip_solver *s, T *fac, const T *in, T *tmp, T *out) {
   IluDev<T> L = iluDev<T>(s, fac);
-  launchLU<T, kProNone>(s, L, (const Scalars *)nullptr, in, tmp, (T *)nullptr, (const T *)nullptr, (const T *)nullptr,
+  launchLU<T, kProNone>(s, L, idleScalars(s), in, tmp, (T *)nullptr, (const T *)nullptr, (const T *)nullptr,
                         out);
 }
 template void iluApplyDev<float>(phase_hip_solver *, float *, const float *, float *, float *);
@@ -773,6 +787,13 @@ int phase_hip_solver_stats(const phase_hip_solver *s, int *innerIters, int *oute
   return PHASE_HIP_OK;
 }
 
+int phase_hip_solver_ordering_info(const phase_hip_solver *s, int *nBlocks, int *nColours) {
+  if (!s || !s->havePattern) return PHASE_HIP_ESTATE;
+  if (nBlocks) *nBlocks = s->ord.nBlocks;
+  if (nColours) *nColours = s->ord.nColours;
+  return PHASE_HIP_OK;
+}
+
 int phase_hip_solver_export(phase_hip_solver *s, int *nnz, int *rowPtr, int *colInd, double *vals,
                             double *rhs) {
   return guarded(s, [&] {
@@ -874,6 +895,69 @@ int phase_hip_solver_ilu_export(phase_hip_solver *s, int *nnz, int *perm, int *r
   });
 }
 
+}  // extern "C"
+
+namespace {
+
+// Algorithmic bytes of the timed launches (entries as phase_hip_solver_time_kernels):
+// every matrix or factor entry touched (value + column index), every vector
+// element read or written once, one gathered value per off-block entry, and
+// the block / lane records the sweeps read.
+void kernelBytes(const Ordering &o, double *out) {
+  const double n = o.n;
+  const int C = (int)o.colourPtr.size() - 1;
+  double nnzL = 0., nnzU = 0.;
+  for (const RowInfo &r : o.rows) {
+    nnzL += r.nL;
+    nnzU += r.nU;
+  }
+  // forward / backward sweep over blocks [b0, b1) with value size sz
+  auto sweep = [&](int b0, int b1, double sz, bool upper) {
+    double rows = 0., ent = 0., off = 0.;
+    for (int b = b0; b < b1; ++b) {
+      const BlockInfo &B = o.blocks[b];
+      rows += B.nRows;
+      for (int k = B.rowStart; k < B.rowStart + B.nRows; ++k) {
+        const RowInfo &r = o.rows[k];
+        ent += upper ? r.nU : r.nL;
+        off += upper ? r.nU - r.nUin : r.nLoff;
+      }
+    }
+    const double recs = (b1 - b0) * (sizeof(BlockInfo) + 64.0 * sizeof(LaneRec));
+    return ent * (sz + 4) + off * sz + rows * sz * (upper ? 3 : 2) + recs;
+  };
+  auto spmv = [&](double sz) { return o.nnz * (sz + 4) + 2 * n * sz + (o.nSlices + 1) * 4.0; };
+  auto factor = [&](double sz) {
+    return (nnzL + nnzU + n) * sz * 2 + nnzL * 4 + (o.updSrc.size() + o.updDst.size()) * 4.0 +
+           (o.updPtr.size()) * 4.0 + o.facSrc.size() * 4.0;
+  };
+  out[0] = spmv(8);
+  out[1] = spmv(4);
+  out[2] = sweep(0, o.nBlocks, 8, false);
+  out[3] = sweep(0, o.nBlocks, 8, true);
+  out[4] = sweep(0, o.nBlocks, 4, false);
+  out[5] = sweep(0, o.nBlocks, 4, true);
+  out[6] = factor(8);
+  out[7] = factor(4);
+  out[8] = out[9] = out[10] = 0.;
+  if (C >= 2) {
+    out[8] = sweep(0, o.colourPtr[C - 1], 8, true) / (C - 1);
+    out[10] = sweep(0, o.colourPtr[C - 1], 8, false) / (C - 1);
+  }
+  if (C >= 1) {
+    // fused: records read once, y kept in registers (neither stored nor reloaded)
+    const int b0 = o.colourPtr[C - 1];
+    double rows = 0.;
+    for (int b = b0; b < o.nBlocks; ++b) rows += o.blocks[b].nRows;
+    out[9] = sweep(b0, o.nBlocks, 8, false) + sweep(b0, o.nBlocks, 8, true) -
+             (o.nBlocks - b0) * (sizeof(BlockInfo) + 64.0 * sizeof(LaneRec)) - 2 * rows * 8;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
 int phase_hip_solver_time_kernels(phase_hip_solver *s, int reps, double *ms, int nms) {
   return guarded(s, [&] {
     if (!s->havePattern || !s->haveValues) fail(PHASE_HIP_ESTATE, "no system");
@@ -899,7 +983,7 @@ int phase_hip_solver_time_kernels(phase_hip_solver *s, int reps, double *ms, int
       PH_CHECK(hipEventElapsedTime(&t, s->ev0, s->ev1));
       return (double)t / reps;
     };
-    const Scalars *noSc = nullptr;
+    const Scalars *noSc = idleScalars(s);
     auto lsweep = [&](auto *fac, auto *in, auto *out) {
       using T = std::remove_pointer_t<decltype(in)>;
       launchL<T, kProNone>(s, iluDev<T>(s, fac), noSc, (const T *)in, out, (T *)nullptr, (const T *)nullptr,
@@ -909,7 +993,7 @@ int phase_hip_solver_time_kernels(phase_hip_solver *s, int reps, double *ms, int
       using T = std::remove_pointer_t<decltype(in)>;
       launchU<T>(s, iluDev<T>(s, fac), noSc, (const T *)in, out);
     };
-    double out[8];
+    double out[kTimedKernels];
     out[0] = timeIt([&] {
       hipLaunchKernelGGL((kSpmv<double>), dim3(spmvGrid(s)), dim3(kRedBlock), 0, s->stream, n, s->ord.nSlices,
                          s->dSliceOff.p, s->dSellCol.p, s->dSell64.p, s->k64.ph.p, s->k64.t.p);
@@ -924,8 +1008,46 @@ int phase_hip_solver_time_kernels(phase_hip_solver *s, int reps, double *ms, int
     out[5] = timeIt([&] { usweep(s->dFac32.p, s->k32.y.p, s->k32.sh.p); });
     out[6] = timeIt([&] { factorize<double>(s, s->dFac64); });
     out[7] = timeIt([&] { factorize<float>(s, s->dFac32); });
+    s->fac32Valid = false;  // solves use the rounded FP64 factor
+    // the launches of one FP64 preconditioner apply inside a solve: kUSolve and
+    // kLSolve over every colour but the last, kLUSolve over the last
+    const std::vector<int> &cp = s->ord.colourPtr;
+    const int C = (int)cp.size() - 1;
+    const IluDev<double> L64 = iluDev<double>(s, s->dFac64.p);
+    const double *in = s->k64.ph.p;
+    double *y = s->k64.y.p, *x = s->k64.sh.p;
+    out[8] = out[9] = out[10] = 0.;
+    if (C >= 2) {
+      out[8] = timeIt([&] {
+                 for (int c = C - 2; c >= 0; --c)
+                   hipLaunchKernelGGL((kUSolve<double>), dim3(cp[c + 1] - cp[c]), dim3(64), 0, s->stream, L64,
+                                      syncDev(s), colourCtl(cp[c]), noSc, (const double *)y, x);
+               }) / (C - 1);
+      out[10] = timeIt([&] {
+                  for (int c = 0; c + 1 < C; ++c)
+                    hipLaunchKernelGGL((kLSolve<double, kProNone>), dim3(cp[c + 1] - cp[c]), dim3(64), 0,
+                                       s->stream, L64, syncDev(s), colourCtl(cp[c]), noSc, in, y, (double *)nullptr,
+                                       (const double *)nullptr, (const double *)nullptr);
+                }) / (C - 1);
+    }
+    if (C >= 1)
+      out[9] = timeIt([&] {
+        hipLaunchKernelGGL((kLUSolve<double, kProNone>), dim3(cp[C] - cp[C - 1]), dim3(64), 0, s->stream, L64,
+                           syncDev(s), colourCtl(cp[C - 1]), noSc, in, (const double *)y, (double *)nullptr,
+                           (const double *)nullptr, (const double *)nullptr, x);
+      });
     checkDeviceErr(s);
-    for (int i = 0; i < std::min(nms, 8); ++i) ms[i] = out[i];
+    for (int i = 0; i < std::min(nms, kTimedKernels); ++i) ms[i] = out[i];
+  });
+}
+
+int phase_hip_solver_kernel_bytes(phase_hip_solver *s, double *bytes, int nb) {
+  return guarded(s, [&] {
+    if (!s->havePattern) fail(PHASE_HIP_ESTATE, "no pattern");
+    if (!bytes || nb < 1) fail(PHASE_HIP_EINVAL, "no output");
+    double out[kTimedKernels];
+    kernelBytes(s->ord, out);
+    for (int i = 0; i < std::min(nb, kTimedKernels); ++i) bytes[i] = out[i];
   });
 }
 

@@ -19,6 +19,7 @@
 namespace phase {
 
 constexpr int kMaxOffU = 8;  // off-block U entries of one row held in registers
+constexpr int kMaxOffL = 8;  // off-block L entries of one row held in registers (more: a tail loop)
 constexpr int kMaxIn = 8;    // in-block L / U entries of one row held in registers
 constexpr int kRedBlock = 256;
 
@@ -28,6 +29,7 @@ template <class T>
 struct IluDev {
   const BlockInfo *blocks;
   const RowInfo *rows;
+  const LaneRec *lanes;
   const int *pred, *succ, *lCol, *uCol;
   T *fac;
   int nb, nLSlots, nUSlots, n;
@@ -153,11 +155,29 @@ template <int NV>
 __device__ void finishPartials(const double *partials, int nb, double (&v)[NV]) {
   __shared__ double sred[NV][kRedBlock / 64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // eight loads per vector in flight per thread: the sum is bound by a few
+  // memory round trips, not by one per partial
+  constexpr int U = 8;
+  double acc[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) acc[q] = 0.;
+  for (int i0 = threadIdx.x; i0 < nb; i0 += U * kRedBlock) {
+    double t[NV][U];
+#pragma unroll
+    for (int q = 0; q < NV; ++q)
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int i = i0 + u * kRedBlock;
+        t[q][u] = i < nb ? partials[q * nb + i] : 0.;
+      }
+#pragma unroll
+    for (int q = 0; q < NV; ++q)
+#pragma unroll
+      for (int u = 0; u < U; ++u) acc[q] += t[q][u];
+  }
 #pragma unroll
   for (int q = 0; q < NV; ++q) {
-    double s = 0.;
-    for (int i = threadIdx.x; i < nb; i += kRedBlock) s += partials[q * nb + i];
-    s = waveSum(s);
+    const double s = waveSum(acc[q]);
     if (lane == 0) sred[q][wave] = s;
   }
   __syncthreads();
@@ -506,8 +526,8 @@ __device__ __forceinline__ void sweepWait(const SweepCtl &c, const SyncDev &sy, 
   if (c.syncFree) {
     if ((int)threadIdx.x < end - beg) waitFlag(&sy.flags[list[beg + threadIdx.x]], c.epoch, sy.err);
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    __syncthreads();
   }
-  __syncthreads();
 }
 
 __device__ __forceinline__ void sweepPublish(const SweepCtl &c, const SyncDev &sy, int b) {
@@ -554,32 +574,43 @@ __global__ __launch_bounds__(64) void kIluFactor(IluDev<T> L, const int *__restr
   sweepPublish(ctl, sy, b);
 }
 
+// A sweep launch is latency-bound: every block runs one wavefront and a
+// colour's blocks are all resident at once, so the launch lasts about as long
+// as one wavefront's chain of dependent memory round trips.  The kernels keep
+// that chain at three: (1) block record, lane record and the run's scalars;
+// (2) the row's own vector entries and every factor value and column index it
+// needs; (3) the solution values of earlier-coloured rows.  The in-block
+// triangular solve then runs level by level through LDS.
+
 // Forward sweep body for one block: returns y of the lane's row.  PRO selects
 // the fused BiCGStab prologue that forms the right-hand side on the fly:
 // kProP: p = r + beta (p - omega v);  kProS: s = r - alpha v (both stored).
+// Accumulation order (earlier columns first) is that of the oracle's forward
+// substitution, so the result is bit-identical to it.
 template <class T, int PRO>
-__device__ __forceinline__ T lBody(const IluDev<T> &L, const SyncDev &sy, const SweepCtl &ctl, int b,
-                                   const BlockInfo &B, const RowInfo &ri, bool act, int row, int lane,
-                                   const Scalars *sc, const T *__restrict__ in, const T *__restrict__ y,
-                                   T *__restrict__ pv, const T *__restrict__ r, const T *__restrict__ v, T *ylds) {
-  T acc = 0;
-  if (act) {
-    if (PRO == kProP) {
-      const T beta = (T)sc->beta, omega = (T)sc->omega;
-      const T p = r[row] + beta * (pv[row] - omega * v[row]);
-      pv[row] = p;
-      acc = p;
-    } else if (PRO == kProS) {
-      const T alpha = (T)sc->alpha;
-      const T s = r[row] - alpha * v[row];
-      pv[row] = s;
-      acc = s;
-    } else {
-      acc = in[row];
-    }
+__device__ __forceinline__ T lBody(const IluDev<T> &L, const SyncDev &sy, const SweepCtl &ctl, const BlockInfo &B,
+                                   bool act, int row, unsigned info, int lane, const Scalars *sc,
+                                   const T *__restrict__ in, const T *__restrict__ y, const T *__restrict__ pv,
+                                   const T *__restrict__ r, const T *__restrict__ v, T *ylds, T &pro) {
+  const int nIn = act ? (int)(info & 15u) : 0;
+  const int nOff = act ? (int)((info >> 4) & 63u) : 0;
+  const int lev = (int)((info >> 18) & 63u);
+  // the row's own entries load unconditionally (idle lanes read row 0 and
+  // discard it): a branch here would make the compiler wait for them before
+  // the factor loads below
+  const int rs = act ? row : 0;
+  T acc;
+  if (PRO == kProP) {
+    const T beta = (T)sc->beta, omega = (T)sc->omega;
+    acc = r[rs] + beta * (pv[rs] - omega * v[rs]);
+  } else if (PRO == kProS) {
+    const T alpha = (T)sc->alpha;
+    acc = r[rs] - alpha * v[rs];
+  } else {
+    acc = in[rs];
   }
-  // in-block entries into registers first: the level loop then touches LDS only
-  const int nIn = act ? ri.nL - ri.nLoff : 0;
+  pro = acc;  // stored by the caller at the end: a store here would wait for the loads
+  if (!act) acc = 0;
   T lv[kMaxIn];
   int lc[kMaxIn];
 #pragma unroll
@@ -587,23 +618,44 @@ __device__ __forceinline__ T lBody(const IluDev<T> &L, const SyncDev &sy, const 
     lv[q] = 0;
     lc[q] = 0;
     if (q < nIn) {
-      const int s = B.lOff + ((ri.nLoff + q) << 6) + lane;
+      const int s = B.lOff + ((nOff + q) << 6) + lane;
       lv[q] = L.fac[s];
-      lc[q] = L.lCol[s] - B.rowStart;
+      lc[q] = L.lCol[s];
     }
   }
+  // column indices stay 32-bit unsigned until the gathers: a sign extension
+  // here would make each predicated load wait for its own result
+  T ov[kMaxOffL];
+  unsigned oc[kMaxOffL];
+#pragma unroll
+  for (int q = 0; q < kMaxOffL; ++q) {
+    ov[q] = 0;
+    oc[q] = 0;
+    if (q < nOff) {
+      const int s = B.lOff + (q << 6) + lane;
+      ov[q] = L.fac[s];
+      oc[q] = (unsigned)L.lCol[s];
+    }
+  }
+  // keep every load above in flight before the gathers below are addressed
+  // (otherwise each column load is waited for in turn)
+  asm volatile("" ::: "memory");
   sweepWait(ctl, sy, L.pred, B.predBeg, B.predEnd);
-  if (act) {
-    for (int e = 0; e < ri.nLoff; ++e) {
-      const int s = B.lOff + (e << 6) + lane;
-      acc = fma(-L.fac[s], y[L.lCol[s]], acc);
-    }
+  T oy[kMaxOffL];
+#pragma unroll
+  for (int q = 0; q < kMaxOffL; ++q) oy[q] = q < nOff ? y[oc[q]] : (T)0;
+#pragma unroll
+  for (int q = 0; q < kMaxOffL; ++q)
+    if (q < nOff) acc = fma(-ov[q], oy[q], acc);
+  for (int e = kMaxOffL; e < nOff; ++e) {
+    const int s = B.lOff + (e << 6) + lane;
+    acc = fma(-L.fac[s], y[L.lCol[s]], acc);
   }
-  for (int lev = 0; lev < B.nLev; ++lev) {
-    if (act && ri.lev == lev) {
+  for (int l = 0; l < B.nLev; ++l) {
+    if (act && lev == l) {
 #pragma unroll
       for (int q = 0; q < kMaxIn; ++q)
-        if (q < nIn) acc = fma(-lv[q], ylds[lc[q]], acc);
+        if (q < nIn) acc = fma(-lv[q], ylds[lc[q] - B.rowStart], acc);
       ylds[lane] = acc;
     }
     __syncthreads();
@@ -611,49 +663,69 @@ __device__ __forceinline__ T lBody(const IluDev<T> &L, const SyncDev &sy, const 
   return acc;
 }
 
-// Backward sweep body for one block, starting from acc = y of the lane's row;
-// returns x of the row.
+// Backward sweep of one block in two parts: uLoad issues every load the row
+// needs from the factor (so a fused launch can overlap them with its forward
+// sweep), uSolve gathers later-coloured solution values and runs the levels
+// from acc = y of the lane's row, returning x of the row.  In-block columns
+// precede later-block columns, as in the oracle's back substitution.
 template <class T>
-__device__ __forceinline__ T uBody(const IluDev<T> &L, const SyncDev &sy, const SweepCtl &ctl, const BlockInfo &B,
-                                   const RowInfo &ri, bool act, int row, int lane, T acc,
-                                   const T *__restrict__ x, T *xlds) {
-  const T *facU = L.fac + L.nLSlots;
-  const T d = act ? L.fac[L.nLSlots + L.nUSlots + row] : (T)1;
-  const int nIn = act ? ri.nUin : 0;
+struct UPre {
+  T d;
+  int nIn, nOff, ulev;
   T uv[kMaxIn];
   int uc[kMaxIn];
+  T ou[kMaxOffU];
+  unsigned oc[kMaxOffU];
+};
+
+template <class T>
+__device__ __forceinline__ void uLoad(const IluDev<T> &L, const BlockInfo &B, bool act, int row, unsigned info,
+                                      int lane, UPre<T> &u) {
+  const T *facU = L.fac + L.nLSlots;
+  u.nIn = act ? (int)((info >> 10) & 15u) : 0;
+  u.nOff = act ? (int)((info >> 14) & 15u) : 0;
+  u.ulev = (int)((info >> 24) & 63u);
+  const T d0 = L.fac[L.nLSlots + L.nUSlots + (act ? row : 0)];  // unconditional, as in lBody
+  u.d = act ? d0 : (T)1;
 #pragma unroll
   for (int q = 0; q < kMaxIn; ++q) {
-    uv[q] = 0;
-    uc[q] = 0;
-    if (q < nIn) {
+    u.uv[q] = 0;
+    u.uc[q] = 0;
+    if (q < u.nIn) {
       const int s = B.uOff + (q << 6) + lane;
-      uv[q] = facU[s];
-      uc[q] = L.uCol[s] - B.rowStart;
+      u.uv[q] = facU[s];
+      u.uc[q] = L.uCol[s];
     }
   }
-  sweepWait(ctl, sy, L.succ, B.succBeg, B.succEnd);
-  const int nOff = act ? ri.nU - ri.nUin : 0;
-  T ou[kMaxOffU], ox[kMaxOffU];
 #pragma unroll
   for (int q = 0; q < kMaxOffU; ++q) {
-    ou[q] = 0;
-    ox[q] = 0;
-    if (q < nOff) {
-      const int s = B.uOff + ((ri.nUin + q) << 6) + lane;
-      ou[q] = facU[s];
-      ox[q] = x[L.uCol[s]];
+    u.ou[q] = 0;
+    u.oc[q] = 0;
+    if (q < u.nOff) {
+      const int s = B.uOff + ((u.nIn + q) << 6) + lane;
+      u.ou[q] = facU[s];
+      u.oc[q] = (unsigned)L.uCol[s];
     }
   }
-  for (int lev = 0; lev < B.nULev; ++lev) {
-    if (act && ri.ulev == lev) {
+}
+
+template <class T>
+__device__ __forceinline__ T uSolve(const IluDev<T> &L, const SyncDev &sy, const SweepCtl &ctl, const BlockInfo &B,
+                                    bool act, int lane, T acc, const T *__restrict__ x, T *xlds, const UPre<T> &u) {
+  asm volatile("" ::: "memory");
+  sweepWait(ctl, sy, L.succ, B.succBeg, B.succEnd);
+  T ox[kMaxOffU];
+#pragma unroll
+  for (int q = 0; q < kMaxOffU; ++q) ox[q] = q < u.nOff ? x[u.oc[q]] : (T)0;
+  for (int l = 0; l < B.nULev; ++l) {
+    if (act && u.ulev == l) {
 #pragma unroll
       for (int q = 0; q < kMaxIn; ++q)
-        if (q < nIn) acc = fma(-uv[q], xlds[uc[q]], acc);
+        if (q < u.nIn) acc = fma(-u.uv[q], xlds[u.uc[q] - B.rowStart], acc);
 #pragma unroll
       for (int q = 0; q < kMaxOffU; ++q)
-        if (q < nOff) acc = fma(-ou[q], ox[q], acc);
-      acc = acc / d;
+        if (q < u.nOff) acc = fma(-u.ou[q], ox[q], acc);
+      acc = acc / u.d;
       xlds[lane] = acc;
     }
     __syncthreads();
@@ -661,7 +733,8 @@ __device__ __forceinline__ T uBody(const IluDev<T> &L, const SyncDev &sy, const 
   return acc;
 }
 
-// Forward sweep y = L^{-1} b (unit L).
+// Forward sweep y = L^{-1} b (unit L).  After convergence (sc->done) every
+// lane is idle and the launch only walks the block's levels.
 template <class T, int PRO>
 __global__ __launch_bounds__(64) void kLSolve(IluDev<T> L, SyncDev sy, SweepCtl ctl, const Scalars *sc,
                                               const T *__restrict__ in, T *__restrict__ y, T *__restrict__ pv,
@@ -669,14 +742,17 @@ __global__ __launch_bounds__(64) void kLSolve(IluDev<T> L, SyncDev sy, SweepCtl 
   __shared__ T ylds[64];
   const int b = sweepBlock(ctl, sy, L.nb, false);
   if (b < 0) return;
-  if (sc && sc->done) return;
-  const BlockInfo B = L.blocks[b];
   const int lane = threadIdx.x;
-  const bool act = lane < B.nRows;
-  const int row = B.rowStart + lane;
-  const RowInfo ri = act ? L.rows[row] : RowInfo{};
-  const T acc = lBody<T, PRO>(L, sy, ctl, b, B, ri, act, row, lane, sc, in, y, pv, r, v, ylds);
-  if (act) y[row] = acc;
+  const int done = sc->done;
+  const BlockInfo B = L.blocks[b];
+  const LaneRec rec = L.lanes[(size_t)b * 64 + lane];
+  const bool act = rec.row >= 0 && !done;
+  T pro;
+  const T acc = lBody<T, PRO>(L, sy, ctl, B, act, rec.row, rec.info, lane, sc, in, y, pv, r, v, ylds, pro);
+  if (act) {
+    if (PRO != kProNone) pv[rec.row] = pro;
+    y[rec.row] = acc;
+  }
   sweepPublish(ctl, sy, b);
 }
 
@@ -687,14 +763,16 @@ __global__ __launch_bounds__(64) void kUSolve(IluDev<T> L, SyncDev sy, SweepCtl 
   __shared__ T xlds[64];
   const int b = sweepBlock(ctl, sy, L.nb, true);
   if (b < 0) return;
-  if (sc && sc->done) return;
-  const BlockInfo B = L.blocks[b];
   const int lane = threadIdx.x;
-  const bool act = lane < B.nRows;
-  const int row = B.rowStart + lane;
-  const RowInfo ri = act ? L.rows[row] : RowInfo{};
-  const T acc = uBody<T>(L, sy, ctl, B, ri, act, row, lane, act ? y[row] : (T)0, x, xlds);
-  if (act) x[row] = acc;
+  const int done = sc->done;
+  const BlockInfo B = L.blocks[b];
+  const LaneRec rec = L.lanes[(size_t)b * 64 + lane];
+  const bool act = rec.row >= 0 && !done;
+  const T y0 = y[act ? rec.row : 0];
+  UPre<T> u;
+  uLoad<T>(L, B, act, rec.row, rec.info, lane, u);
+  const T acc = uSolve<T>(L, sy, ctl, B, act, lane, act ? y0 : (T)0, x, xlds, u);
+  if (act) x[rec.row] = acc;
   sweepPublish(ctl, sy, b);
 }
 
@@ -709,15 +787,20 @@ __global__ __launch_bounds__(64) void kLUSolve(IluDev<T> L, SyncDev sy, SweepCtl
   __shared__ T ylds[64];
   __shared__ T xlds[64];
   const int b = ctl.blockBase + (int)blockIdx.x;
-  if (sc && sc->done) return;
-  const BlockInfo B = L.blocks[b];
   const int lane = threadIdx.x;
-  const bool act = lane < B.nRows;
-  const int row = B.rowStart + lane;
-  const RowInfo ri = act ? L.rows[row] : RowInfo{};
-  const T yv = lBody<T, PRO>(L, sy, ctl, b, B, ri, act, row, lane, sc, in, y, pv, r, v, ylds);
-  const T acc = uBody<T>(L, sy, ctl, B, ri, act, row, lane, yv, x, xlds);
-  if (act) x[row] = acc;
+  const int done = sc->done;
+  const BlockInfo B = L.blocks[b];
+  const LaneRec rec = L.lanes[(size_t)b * 64 + lane];
+  const bool act = rec.row >= 0 && !done;
+  UPre<T> u;
+  uLoad<T>(L, B, act, rec.row, rec.info, lane, u);
+  T pro;
+  const T yv = lBody<T, PRO>(L, sy, ctl, B, act, rec.row, rec.info, lane, sc, in, y, pv, r, v, ylds, pro);
+  const T acc = uSolve<T>(L, sy, ctl, B, act, lane, yv, x, xlds, u);
+  if (act) {
+    if (PRO != kProNone) pv[rec.row] = pro;
+    x[rec.row] = acc;
+  }
 }
 
 }  // namespace phase

@@ -48,6 +48,8 @@ _SIGS = {
     "phase_hip_solver_spmv": (ctypes.c_int, [_vp, ctypes.c_int, _f64p, _f64p]),
     "phase_hip_solver_ilu_export": (ctypes.c_int, [_vp, _ip, _vp, _vp, _vp, _vp]),
     "phase_hip_solver_time_kernels": (ctypes.c_int, [_vp, ctypes.c_int, _f64p, ctypes.c_int]),
+    "phase_hip_solver_kernel_bytes": (ctypes.c_int, [_vp, _f64p, ctypes.c_int]),
+    "phase_hip_solver_ordering_info": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
     "phase_hip_mesh_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, _f64p,
                                              ctypes.c_int, _i32p, _i32p]),
     "phase_hip_mesh_destroy": (ctypes.c_int, [_vp]),
@@ -263,6 +265,13 @@ class HipSparseMatrixSolver:
         self._check(load_library().phase_hip_solver_stats(self._h, a, b, c, d), "stats")
         return dict(inner_iters=a.value, outer_passes=b.value, fallbacks=c.value, solve_ms=d.value)
 
+    def ordering_info(self):
+        """ILU blocks and block colours of the device ordering."""
+        a, b = ctypes.c_int(), ctypes.c_int()
+        self._check(load_library().phase_hip_solver_ordering_info(self._h, ctypes.byref(a), ctypes.byref(b)),
+                    "ordering")
+        return dict(blocks=a.value, colours=b.value)
+
     # --- device-side helpers used by tests and the benchmark
     def export(self):
         nnz = ctypes.c_int()
@@ -291,12 +300,19 @@ class HipSparseMatrixSolver:
                     "ilu")
         return perm, rp, ci, v
 
-    KERNEL_NAMES = ("spmv64", "spmv32", "lsweep64", "usweep64", "lsweep32", "usweep32", "factor64", "factor32")
+    KERNEL_NAMES = ("spmv64", "spmv32", "lsweep64", "usweep64", "lsweep32", "usweep32", "factor64", "factor32",
+                    "usolve64_launch", "lusolve64_launch", "lsolve64_launch")
 
     def time_kernels(self, reps=20):
-        """Mean device ms of one launch of each solver kernel (HIP events)."""
+        """Mean device ms of each timed kernel or sweep (HIP events)."""
         out = np.zeros(len(self.KERNEL_NAMES))
         self._check(load_library().phase_hip_solver_time_kernels(self._h, int(reps), out, len(out)), "time")
+        return dict(zip(self.KERNEL_NAMES, out.tolist()))
+
+    def kernel_bytes(self):
+        """Algorithmic bytes of the entries of time_kernels()."""
+        out = np.zeros(len(self.KERNEL_NAMES))
+        self._check(load_library().phase_hip_solver_kernel_bytes(self._h, out, len(out)), "bytes")
         return dict(zip(self.KERNEL_NAMES, out.tolist()))
 
     # --- learned correction network

@@ -106,6 +106,25 @@ extern "C" int emul_run(int n, const int *rowPtr, const int *colInd, const doubl
       for (int q = B.predBeg; q < B.predEnd; ++q) REQ(colourOf[o.pred[q]] < colourOf[b], "pred not in an earlier colour");
       for (int q = B.succBeg; q < B.succEnd; ++q) REQ(colourOf[o.succ[q]] > colourOf[b], "succ not in a later colour");
     }
+    // per-lane sweep records agree with the row records
+    REQ(o.lanes.size() == (size_t)nb * 64, "lane record count");
+    for (int b = 0; b < nb; ++b) {
+      const BlockInfo &B = o.blocks[b];
+      for (int lane = 0; lane < 64; ++lane) {
+        const LaneRec &L = o.lanes[(size_t)b * 64 + lane];
+        if (lane >= B.nRows) {
+          REQ(L.row == -1, "idle lane with a row");
+          continue;
+        }
+        const int k = B.rowStart + lane;
+        const RowInfo &ri = o.rows[k];
+        REQ(L.row == k, "lane row");
+        REQ((int)(L.info & 15u) == ri.nL - ri.nLoff && (int)((L.info >> 4) & 63u) == ri.nLoff &&
+                (int)((L.info >> 10) & 15u) == ri.nUin && (int)((L.info >> 14) & 15u) == ri.nU - ri.nUin &&
+                (int)((L.info >> 18) & 63u) == ri.lev && (int)((L.info >> 24) & 63u) == ri.ulev,
+            "lane info packing");
+      }
+    }
     std::vector<char> doneBlk(nb, 0), rowDone(n, 0);
     // --- factorisation (kIluFactor), blocks in ticket order
     for (int b = 0; b < nb; ++b) {
