@@ -257,6 +257,74 @@ int phase_hip_equation_sub_rhs_device(phase_hip_solver *s, const double *v);
 int phase_hip_equation_export_padded(phase_hip_solver *s, int *nslots, int *rowPtr,
                                      int *colInd, double *vals, double *rhs);
 
+/* ------------------------------------------------------------------ equations
+ * FiniteVolumeEquation<T> / CrsEquation held on the device (CrsEquation.cpp,
+ * FiniteVolumeEquation.tpp): ncomp * nCells rows in the IndexMap block layout
+ * (row = comp * nCells + cell, IndexMap.cpp:31-36), each row its (column,
+ * value) pairs in first-touch order, and the stored rhs (A x + rhs = 0).  Every
+ * array argument below is a DEVICE pointer (phase_hip_malloc); cell vector
+ * fields are interleaved (x0, y0, x1, y1, ...), face fields likewise.  Work is
+ * queued on the mesh's stream. */
+typedef struct phase_hip_equation phase_hip_equation;
+
+int phase_hip_equation_create(phase_hip_equation **e, phase_hip_mesh *m, int ncomp);
+int phase_hip_equation_destroy(phase_hip_equation *e);
+int phase_hip_equation_rank(const phase_hip_equation *e);
+const char *phase_hip_equation_last_error(const phase_hip_equation *e);
+
+/* Operators: each replaces e's content with a fresh CrsEquation(rank, 5).
+ * fv::laplacian, variants as phase_hip_assemble_laplacian (Laplacian.h:7-167;
+ * ncomp 2: Laplacian.cpp:5-120 with the SYMMETRY tangential term). */
+int phase_hip_eq_laplacian(phase_hip_equation *e, int variant, double gamma, const double *gammaFace,
+                           const double *gamma0Face, double theta, const int *bcType, const double *phiFace,
+                           const double *phi0Cell, const double *phi0Face);
+/* fv::ddt(field, timeStep) -- TimeDerivative.h:37-48 */
+int phase_hip_eq_ddt(phase_hip_equation *e, double timeStep, const double *phi0Cell);
+/* fv::div(u, phi, theta), upwind -- Divergence.h:7-53 (u, u0: face vectors) */
+int phase_hip_eq_div(phase_hip_equation *e, const double *uFace, const double *u0Face, double theta,
+                     const int *bcType, const double *phiFace, const double *phi0Cell, const double *phi0Face);
+
+/* Algebra: operator+= (sign 1) / -= (sign -1) -- CrsEquation.cpp:185-275;
+ * operator*=(Scalar) :287-291; -=(Vector) / ==(Vector) :282-285, 308-311;
+ * ==(Scalar) :299-303; implicit under-relaxation a_P /= alpha,
+ * rhs -= (1 - alpha) a_P phi (ScalarFiniteVolumeEquation.cpp:45-55); copy. */
+int phase_hip_eq_add(phase_hip_equation *e, const phase_hip_equation *o, int sign);
+int phase_hip_eq_scale(phase_hip_equation *e, double s);
+int phase_hip_eq_sub_vector(phase_hip_equation *e, const double *v);
+int phase_hip_eq_eq_scalar(phase_hip_equation *e, double s);
+int phase_hip_eq_relax(phase_hip_equation *e, double alpha, const double *phi);
+int phase_hip_eq_copy(phase_hip_equation *dst, const phase_hip_equation *src);
+
+/* The equation in the reference's layout (host arrays): rows of a fresh
+ * operator have 5 slots (-1 = empty) or more when a row grew; rows rebuilt by
+ * += / -= hold exactly their entries.  nslots is queried with rowPtr == NULL. */
+int phase_hip_eq_export(phase_hip_equation *e, int *nslots, int *rowPtr, int *colInd, double *vals, double *rhs);
+
+/* FiniteVolumeEquation<T>::solve -- FiniteVolumeEquation.tpp:64-86: the
+ * solver's system becomes A = the equation's rows and b = -rhs, device to
+ * device (the ordering is rebuilt only when the pattern changed). */
+int phase_hip_solver_set_equation(phase_hip_solver *s, const phase_hip_equation *e);
+
+/* x(Index) for every row into a DEVICE array (original numbering), and
+ * setGuess from a device array: the field stays in HBM between solves
+ * (TrilinosSparseMatrixSolver.cpp:31-38 keeps x across solves). */
+int phase_hip_solver_get_x_device(phase_hip_solver *s, double *x);
+int phase_hip_solver_set_guess_device(phase_hip_solver *s, const double *x0);
+
+/* Fields (device pointers).  src::div(u) per cell -- Source.cpp:5-21;
+ * src::src(phi) = phi V per row, block layout -- Source.cpp:77-95;
+ * ScalarGradient::computeFaces + compute(FACE_TO_CELL) -- ScalarGradient.cpp:
+ * 34-74; interpolateFaces + setBoundaryFaces -- FiniteVolumeField.tpp:129-182,
+ * VectorFiniteVolumeField.cpp:140-160 (FIXED faces keep their value). */
+int phase_hip_field_src_div(phase_hip_mesh *m, const double *uFace, double *out);
+int phase_hip_field_src_src(phase_hip_mesh *m, int ncomp, const double *phiCell, double *out);
+int phase_hip_field_grad(phase_hip_mesh *m, const double *phiCell, const double *phiFace, double *gradFace,
+                         double *gradCell);
+int phase_hip_field_interpolate_faces(phase_hip_mesh *m, int ncomp, const double *phiCell, const int *bcType,
+                                      double *phiFace);
+/* Wait for the work queued on the mesh's stream. */
+int phase_hip_mesh_synchronize(phase_hip_mesh *m);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,761 @@
+// fv:: operators and CrsEquation algebra on the device.
+//
+// A phase_hip_equation is FiniteVolumeEquation<T> / CrsEquation
+// (src/Math/CrsEquation.cpp, src/2D/Unstructured/FiniteVolume/Equation/
+// FiniteVolumeEquation.tpp) held in HBM: rank = ncomp * nCells rows in the
+// IndexMap block layout (row = comp * nCells + cell, IndexMap.cpp:31-36), each
+// row a list of up to `stride` (column, value) pairs in first-touch order plus
+// the stored rhs (A x + rhs = 0).
+//
+// Operators run one thread per cell and replay the reference's per-cell face
+// loop, so every row has the reference's column order and every sum its
+// summation order (bit-identical with -ffp-contract=off).  A row is built in
+// registers with CrsEquation::addCoeff's first-touch rule (CrsEquation.cpp:
+// 113-131).  operator+= / -= (CrsEquation.cpp:185-275) rebuilds each row the
+// same way: the left rows' non-zero entries, then the right rows' non-zero
+// entries accumulated into a matching column or appended.
+//
+// Field kernels: src::div / src::src (Source.cpp:5-25, 77-95), the face and
+// cell gradients (ScalarGradient.cpp:34-74) and face interpolation with the
+// boundary rules (FiniteVolumeField.tpp:129-182, VectorFiniteVolumeField.cpp:
+// 140-160).  Vector fields are interleaved (x0, y0, x1, y1, ...).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "mesh.h"
+#include "solver.h"
+#include "solver_kernels.h"
+
+using namespace phase;
+
+struct phase_hip_equation {
+  phase_hip_mesh *mesh = nullptr;
+  int ncomp = 1, rank = 0, stride = 0;
+  DBuf<int> cols, cnt;
+  DBuf<double> vals, rhs;
+  DBuf<int> dErr;
+  bool compact = false;  // rows rebuilt by += / -= (no empty slots)
+  std::string err;
+};
+
+namespace {
+
+constexpr int kRowCap = 16;  // register row capacity: diagonal + 14 neighbours + one cross-component entry
+
+// One row under construction, CrsEquation::addCoeff semantics: the first touch
+// of a column claims the next slot, later touches accumulate in call order.
+struct RowB {
+  int col[kRowCap];
+  double val[kRowCap];
+  int n;
+  bool over;
+  __device__ void init() {
+    n = 0;
+    over = false;
+#pragma unroll
+    for (int j = 0; j < kRowCap; ++j) {
+      col[j] = -1;
+      val[j] = 0.;
+    }
+  }
+  __device__ void add(int c, double v) {
+    bool done = false;
+#pragma unroll
+    for (int j = 0; j < kRowCap; ++j)
+      if (!done && j < n && col[j] == c) {
+        val[j] += v;
+        done = true;
+      }
+    if (done) return;
+    if (n >= kRowCap) {
+      over = true;
+      return;
+    }
+#pragma unroll
+    for (int j = 0; j < kRowCap; ++j)
+      if (j == n) {
+        col[j] = c;
+        val[j] = v;
+      }
+    ++n;
+  }
+  __device__ void store(int row, int stride, int *cols, double *vals, int *cnt, int *err) const {
+    if (over || n > stride) {
+      atomicOr(err, 4);
+      return;
+    }
+#pragma unroll
+    for (int j = 0; j < kRowCap; ++j)
+      if (j < stride) {
+        cols[(size_t)row * stride + j] = j < n ? col[j] : -1;
+        vals[(size_t)row * stride + j] = j < n ? val[j] : 0.;
+      }
+    cnt[row] = n;
+  }
+};
+
+struct MeshDev {
+  int nCells;
+  const double *vol, *cc, *fc, *sf;
+  const int *iptr, *iface, *inb, *bptr, *bface;
+  const double *idot, *ims, *bdot, *bms;
+};
+
+MeshDev meshDev(const phase_hip_mesh *m) {
+  return MeshDev{m->nCells, m->dVol.p, m->dCC.p, m->dFC.p, m->dSf.p, m->dIptr.p, m->dIface.p, m->dInb.p,
+                 m->dBptr.p, m->dBface.p, m->dIDot.p, m->dIMs.p, m->dBDot.p, m->dBMs.p};
+}
+
+__device__ __forceinline__ Dv2 ld2(const double *a, int i) { return Dv2{a[2 * i], a[2 * i + 1]}; }
+
+// fv::laplacian, all four forms (Laplacian.h:7-167) and the Vector2D theta
+// forms with the SYMMETRY tangential term (Laplacian.cpp:5-120).  Same
+// coefficient (gamma * dot(r, Sf)) / |r|^2 as the oracle; err bit 1 =
+// unsupported boundary type, bit 2 = missing boundary/old-time input.
+__global__ void kEqLaplacian(MeshDev M, int ncomp, int variant, double gamma, const double *__restrict__ gF,
+                             const double *__restrict__ g0F, double theta, const int *__restrict__ bc,
+                             const double *__restrict__ phiF, const double *__restrict__ phi0C,
+                             const double *__restrict__ phi0F, int stride, int *cols, double *vals, int *cnt,
+                             double *rhs, int *err) {
+  const bool hasTheta = (variant == 0 || variant == 2);
+  const bool field = (variant >= 2);
+  const int N = M.nCells;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < N; c += gridDim.x * blockDim.x) {
+    RowB rb[2];
+    rb[0].init();
+    rb[1].init();
+    double src[2] = {0., 0.};
+    auto addS = [&](int nb, double v) {
+#pragma unroll
+      for (int q = 0; q < 2; ++q)
+        if (q < ncomp) rb[q].add(q * N + nb, v);
+    };
+    for (int k = M.iptr[c]; k < M.iptr[c + 1]; ++k) {
+      const int f = M.iface[k], nb = M.inb[k];
+      const double g = field ? gF[f] : gamma;
+      const double coeff = g * M.idot[k] / M.ims[k];
+      if (!hasTheta) {
+        if (!field) {  // Laplacian.h:57-58
+          addS(nb, coeff);
+          addS(c, -coeff);
+        } else {  // Laplacian.h:141-142
+          addS(c, -coeff);
+          addS(nb, coeff);
+        }
+        continue;
+      }
+      if (!phi0C) {
+        atomicOr(err, 2);
+        continue;
+      }
+      const double coeff0 = field ? g0F[f] * M.idot[k] / M.ims[k] : coeff;
+      if (!field) {  // Laplacian.h:17-21
+        addS(nb, theta * coeff);
+        addS(c, theta * -coeff);
+      } else {  // Laplacian.h:99-101, Laplacian.cpp:75-77
+        addS(c, theta * -coeff);
+        addS(nb, theta * coeff);
+      }
+      for (int q = 0; q < 2; ++q)
+        if (q < ncomp) src[q] += (1. - theta) * coeff0 * (phi0C[ncomp * nb + q] - phi0C[ncomp * c + q]);
+    }
+    const Dv2 pc = ld2(M.cc, c);
+    for (int k = M.bptr[c]; k < M.bptr[c + 1]; ++k) {
+      const int f = M.bface[k];
+      const int bt = bc ? bc[f] : PHASE_HIP_BC_NORMAL_GRADIENT;
+      const double g = field ? gF[f] : gamma;
+      const double coeff = g * M.bdot[k] / M.bms[k];
+      if (bt == PHASE_HIP_BC_FIXED) {
+        if (!phiF || (hasTheta && (!phi0F || !phi0C))) {
+          atomicOr(err, 2);
+          continue;
+        }
+        if (!hasTheta) {  // Laplacian.h:69-72
+          addS(c, -coeff);
+          for (int q = 0; q < 2; ++q) if (q < ncomp) src[q] += coeff * phiF[ncomp * f + q];
+        } else {  // Laplacian.h:31-36
+          const double coeff0 = field ? g0F[f] * M.bdot[k] / M.bms[k] : coeff;
+          addS(c, theta * -coeff);
+          for (int q = 0; q < 2; ++q) if (q < ncomp) src[q] += theta * coeff * phiF[ncomp * f + q];
+          for (int q = 0; q < 2; ++q)
+            if (q < ncomp) src[q] += (1. - theta) * coeff0 * (phi0F[ncomp * f + q] - phi0C[ncomp * c + q]);
+        }
+      } else if (bt == PHASE_HIP_BC_NORMAL_GRADIENT) {
+      } else if (bt == PHASE_HIP_BC_SYMMETRY) {
+        if (ncomp == 2 && hasTheta) {  // Laplacian.cpp:36-43
+          if (!phi0C) {
+            atomicOr(err, 2);
+            continue;
+          }
+          const Dv2 sfo = outward(ld2(M.fc, f), ld2(M.sf, f), pc);
+          const Dv2 tv{-sfo.y, sfo.x};  // Vector2D::tangentVec, Vector2D.h:28
+          const double tm = sqrt(tv.x * tv.x + tv.y * tv.y);
+          const Dv2 tw{tv.x / tm, tv.y / tm};
+          const double coeff0 = field ? g0F[f] * M.bdot[k] / M.bms[k] : coeff;
+          addS(c, theta * -coeff);
+          const double tc = theta * coeff;
+          // add(cell, cell, Tensor2D), VectorFiniteVolumeEquation.cpp:48-66
+          const double xx = tc * (tw.x * tw.x), xy = tc * (tw.x * tw.y), yx = tc * (tw.y * tw.x),
+                       yy = tc * (tw.y * tw.y);
+          rb[0].add(c, xx);
+          if (xy != 0.) rb[0].add(N + c, xy);
+          if (yx != 0.) rb[1].add(c, yx);
+          rb[1].add(N + c, yy);
+          const double p0x = phi0C[2 * c], p0y = phi0C[2 * c + 1];
+          const double d = p0x * tw.x + p0y * tw.y;
+          const double w = (1. - theta) * coeff0;
+          src[0] += w * (tw.x * d - p0x);
+          src[1] += w * (tw.y * d - p0y);
+        }
+      } else {
+        atomicOr(err, 1);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+      if (q < ncomp) {
+        rb[q].store(q * N + c, stride, cols, vals, cnt, err);
+        rhs[q * N + c] = src[q];
+      }
+  }
+}
+
+// fv::ddt(field, timeStep), TimeDerivative.h:37-48
+__global__ void kEqDdt(MeshDev M, int ncomp, double dt, const double *__restrict__ phi0C, int stride, int *cols,
+                       double *vals, int *cnt, double *rhs) {
+  const int N = M.nCells;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < N; c += gridDim.x * blockDim.x) {
+    for (int q = 0; q < ncomp; ++q) {
+      const size_t row = (size_t)q * N + c;
+      cols[row * stride] = (int)row;
+      vals[row * stride] = M.vol[c] / dt;
+      for (int j = 1; j < stride; ++j) {
+        cols[row * stride + j] = -1;
+        vals[row * stride + j] = 0.;
+      }
+      cnt[row] = 1;
+      rhs[row] = -M.vol[c] * phi0C[ncomp * c + q] / dt;
+    }
+  }
+}
+
+__device__ __forceinline__ double upMax(double a) { return a < 0. ? 0. : a; }  // std::max(a, 0.)
+__device__ __forceinline__ double upMin(double a) { return 0. < a ? 0. : a; }  // std::min(a, 0.)
+
+// fv::div(u, phi, theta), first-order upwind, Divergence.h:7-53
+__global__ void kEqDiv(MeshDev M, int ncomp, const double *__restrict__ uF, const double *__restrict__ u0F,
+                       double theta, const int *__restrict__ bc, const double *__restrict__ phiF,
+                       const double *__restrict__ phi0C, const double *__restrict__ phi0F, int stride, int *cols,
+                       double *vals, int *cnt, double *rhs, int *err) {
+  const int N = M.nCells;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < N; c += gridDim.x * blockDim.x) {
+    RowB rb[2];
+    rb[0].init();
+    rb[1].init();
+    double src[2] = {0., 0.};
+    auto addS = [&](int nb, double v) {
+#pragma unroll
+      for (int q = 0; q < 2; ++q)
+        if (q < ncomp) rb[q].add(q * N + nb, v);
+    };
+    const Dv2 pc = ld2(M.cc, c);
+    for (int k = M.iptr[c]; k < M.iptr[c + 1]; ++k) {
+      const int f = M.iface[k], nb = M.inb[k];
+      const Dv2 s = outward(ld2(M.fc, f), ld2(M.sf, f), pc);
+      const double flux = uF[2 * f] * s.x + uF[2 * f + 1] * s.y;
+      const double flux0 = u0F[2 * f] * s.x + u0F[2 * f + 1] * s.y;
+      addS(c, theta * upMax(flux));
+      addS(nb, theta * upMin(flux));
+      for (int q = 0; q < 2; ++q) if (q < ncomp) src[q] += (1. - theta) * upMax(flux0) * phi0C[ncomp * c + q];
+      for (int q = 0; q < 2; ++q) if (q < ncomp) src[q] += (1. - theta) * upMin(flux0) * phi0C[ncomp * nb + q];
+    }
+    for (int k = M.bptr[c]; k < M.bptr[c + 1]; ++k) {
+      const int f = M.bface[k];
+      const Dv2 s = outward(ld2(M.fc, f), ld2(M.sf, f), pc);
+      const double flux = uF[2 * f] * s.x + uF[2 * f + 1] * s.y;
+      const double flux0 = u0F[2 * f] * s.x + u0F[2 * f + 1] * s.y;
+      const int bt = bc ? bc[f] : PHASE_HIP_BC_NORMAL_GRADIENT;
+      if (bt == PHASE_HIP_BC_FIXED) {
+        if (!phiF || !phi0F) {
+          atomicOr(err, 2);
+          continue;
+        }
+        for (int q = 0; q < 2; ++q) if (q < ncomp) src[q] += theta * flux * phiF[ncomp * f + q];
+        for (int q = 0; q < 2; ++q) if (q < ncomp) src[q] += (1. - theta) * flux0 * phi0F[ncomp * f + q];
+      } else if (bt == PHASE_HIP_BC_NORMAL_GRADIENT) {
+        addS(c, theta * flux);
+        for (int q = 0; q < 2; ++q) if (q < ncomp) src[q] += (1. - theta) * flux0 * phi0C[ncomp * c + q];
+      } else if (bt == PHASE_HIP_BC_SYMMETRY) {
+      } else {
+        atomicOr(err, 1);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+      if (q < ncomp) {
+        rb[q].store(q * N + c, stride, cols, vals, cnt, err);
+        rhs[q * N + c] = src[q];
+      }
+  }
+}
+
+// lhs (+|-)= rhs, CrsEquation.cpp:185-275
+__global__ void kEqMerge(int rank, int stride, int sign, int *lc, double *lv, int *ln, double *lr,
+                         const int *__restrict__ rc, const double *__restrict__ rv, const int *__restrict__ rn,
+                         const double *__restrict__ rr, int *err) {
+  for (int row = blockIdx.x * blockDim.x + threadIdx.x; row < rank; row += gridDim.x * blockDim.x) {
+    RowB rb;
+    rb.init();
+    const size_t o = (size_t)row * stride;
+    // the left row's non-zeros keep their order: distinct columns, so add() appends
+    for (int j = 0; j < ln[row]; ++j)
+      if (lv[o + j] != 0. && lc[o + j] >= 0) rb.add(lc[o + j], lv[o + j]);
+    for (int j = 0; j < rn[row]; ++j) {
+      const double v = rv[o + j];
+      if (v == 0. || rc[o + j] < 0) continue;
+      rb.add(rc[o + j], sign > 0 ? v : -v);
+    }
+    rb.store(row, stride, lc, lv, ln, err);
+    lr[row] = sign > 0 ? lr[row] + rr[row] : lr[row] - rr[row];
+  }
+}
+
+// CrsEquation::operator*=(Scalar), CrsEquation.cpp:287-291
+__global__ void kEqScale(int rank, int stride, double s, double *vals, double *rhs) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < (size_t)rank * stride;
+       i += (size_t)gridDim.x * blockDim.x)
+    vals[i] *= s;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < rank; i += gridDim.x * blockDim.x) rhs[i] *= s;
+}
+
+// rhs -= v (operator-=(Vector) / ==(Vector), CrsEquation.cpp:282-285, 308-311);
+// v == nullptr: rhs -= s (operator==(Scalar), :299-303)
+__global__ void kEqSubRhs(int rank, const double *__restrict__ v, double s, double *rhs) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < rank; i += gridDim.x * blockDim.x)
+    rhs[i] -= v ? v[i] : s;
+}
+
+// implicit under-relaxation: a_P /= alpha; rhs -= (1 - alpha) * a_P * phi
+// (ScalarFiniteVolumeEquation.cpp:45-55, VectorFiniteVolumeEquation.cpp:129-146)
+__global__ void kEqRelax(int rank, int stride, double alpha, const double *__restrict__ phi, const int *cols,
+                         double *vals, const int *cnt, double *rhs) {
+  for (int row = blockIdx.x * blockDim.x + threadIdx.x; row < rank; row += gridDim.x * blockDim.x) {
+    const size_t o = (size_t)row * stride;
+    for (int j = 0; j < cnt[row]; ++j)
+      if (cols[o + j] == row) {
+        vals[o + j] /= alpha;
+        rhs[row] -= (1. - alpha) * vals[o + j] * phi[row];
+        break;
+      }
+  }
+}
+
+// src::div(u), Source.cpp:5-21: sum over the cell's faces of dot(u_f, Sf_out)
+__global__ void kSrcDiv(MeshDev M, const double *__restrict__ uF, double *out) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < M.nCells; c += gridDim.x * blockDim.x) {
+    const Dv2 pc = ld2(M.cc, c);
+    double d = 0.;
+    for (int k = M.iptr[c]; k < M.iptr[c + 1]; ++k) {
+      const int f = M.iface[k];
+      const Dv2 s = outward(ld2(M.fc, f), ld2(M.sf, f), pc);
+      d += uF[2 * f] * s.x + uF[2 * f + 1] * s.y;
+    }
+    for (int k = M.bptr[c]; k < M.bptr[c + 1]; ++k) {
+      const int f = M.bface[k];
+      const Dv2 s = outward(ld2(M.fc, f), ld2(M.sf, f), pc);
+      d += uF[2 * f] * s.x + uF[2 * f + 1] * s.y;
+    }
+    out[c] = d;
+  }
+}
+
+// src::src(field), Source.cpp:77-95: phi * V per row (block layout)
+__global__ void kSrcSrc(MeshDev M, int ncomp, const double *__restrict__ phiC, double *out) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < M.nCells; c += gridDim.x * blockDim.x)
+    for (int q = 0; q < ncomp; ++q) out[(size_t)q * M.nCells + c] = phiC[ncomp * c + q] * M.vol[c];
+}
+
+// ScalarGradient::computeFaces, ScalarGradient.cpp:34-52
+__global__ void kGradFaces(int nFaces, const int *__restrict__ fl, const int *__restrict__ fr,
+                           const double *__restrict__ cc, const double *__restrict__ fc,
+                           const double *__restrict__ phiC, const double *__restrict__ phiF, double *gF) {
+  for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < nFaces; f += gridDim.x * blockDim.x) {
+    const int l = fl[f], r = fr[f];
+    double dx, dy, dphi;
+    if (r >= 0) {
+      dx = cc[2 * r] - cc[2 * l];
+      dy = cc[2 * r + 1] - cc[2 * l + 1];
+      dphi = phiC[r] - phiC[l];
+    } else {
+      dx = fc[2 * f] - cc[2 * l];
+      dy = fc[2 * f + 1] - cc[2 * l + 1];
+      dphi = phiF[f] - phiC[l];
+    }
+    const double ms = dx * dx + dy * dy;
+    gF[2 * f] = dx * dphi / ms;
+    gF[2 * f + 1] = dy * dphi / ms;
+  }
+}
+
+// ScalarGradient::compute(FACE_TO_CELL), ScalarGradient.cpp:54-74:
+// componentwise sum_f g_f |Sf_out| / sum_f |Sf_out| over the cell's links
+__global__ void kGradCells(MeshDev M, const double *__restrict__ gF, double *gC) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < M.nCells; c += gridDim.x * blockDim.x) {
+    const Dv2 pc = ld2(M.cc, c);
+    double sx = 0., sy = 0., tx = 0., ty = 0.;
+    auto acc = [&](int f) {
+      const Dv2 n = outward(ld2(M.fc, f), ld2(M.sf, f), pc);
+      const double ax = fabs(n.x), ay = fabs(n.y);
+      tx += gF[2 * f] * ax;
+      ty += gF[2 * f + 1] * ay;
+      sx += ax;
+      sy += ay;
+    };
+    for (int k = M.iptr[c]; k < M.iptr[c + 1]; ++k) acc(M.iface[k]);
+    for (int k = M.bptr[c]; k < M.bptr[c + 1]; ++k) acc(M.bface[k]);
+    gC[2 * c] = tx / sx;
+    gC[2 * c + 1] = ty / sy;
+  }
+}
+
+// interpolateFaces(DISTANCE) + setBoundaryFaces, FiniteVolumeField.tpp:129-182
+// and VectorFiniteVolumeField.cpp:140-160; FIXED faces keep their value.
+__global__ void kInterpFaces(int nFaces, int ncomp, const int *__restrict__ fl, const int *__restrict__ fr,
+                             const double *__restrict__ cc, const double *__restrict__ fc,
+                             const double *__restrict__ sf, const int *__restrict__ bc,
+                             const double *__restrict__ phiC, double *phiF) {
+  for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < nFaces; f += gridDim.x * blockDim.x) {
+    const int l = fl[f], r = fr[f];
+    if (r >= 0) {
+      const double ax = fc[2 * f] - cc[2 * r], ay = fc[2 * f + 1] - cc[2 * r + 1];
+      const double bx = fc[2 * f] - cc[2 * l], by = fc[2 * f + 1] - cc[2 * l + 1];
+      const double l1 = sqrt(ax * ax + ay * ay), l2 = sqrt(bx * bx + by * by);
+      const double g = l1 / (l1 + l2);  // Face::distanceWeight, Face.cpp:60-64
+      for (int q = 0; q < ncomp; ++q)
+        phiF[ncomp * f + q] = g * phiC[ncomp * l + q] + (1. - g) * phiC[ncomp * r + q];
+      continue;
+    }
+    const int bt = bc ? bc[f] : PHASE_HIP_BC_NORMAL_GRADIENT;
+    if (bt == PHASE_HIP_BC_FIXED) continue;
+    if (bt == PHASE_HIP_BC_SYMMETRY && ncomp == 2) {
+      const double nx = sf[2 * f], ny = sf[2 * f + 1];
+      const double px = phiC[2 * l], py = phiC[2 * l + 1];
+      const double d = px * nx + py * ny;
+      const double ms = nx * nx + ny * ny;
+      phiF[2 * f] = px - nx * d / ms;
+      phiF[2 * f + 1] = py - ny * d / ms;
+    } else {
+      for (int q = 0; q < ncomp; ++q) phiF[ncomp * f + q] = phiC[ncomp * l + q];
+    }
+  }
+}
+
+inline int g256(long n) { return (int)std::max<long>(1, std::min<long>((n + 255) / 256, 8192)); }
+
+template <class F>
+int guardedEq(phase_hip_equation *e, F &&f) {
+  if (!e) return PHASE_HIP_EINVAL;
+  try {
+    f();
+    return PHASE_HIP_OK;
+  } catch (const HipError &x) {
+    e->err = x.msg;
+    return x.code;
+  } catch (const std::exception &x) {
+    e->err = x.what();
+    return PHASE_HIP_EINVAL;
+  }
+}
+
+template <class F>
+int guardedM(phase_hip_mesh *m, F &&f) {
+  if (!m) return PHASE_HIP_EINVAL;
+  try {
+    f();
+    return PHASE_HIP_OK;
+  } catch (const HipError &x) {
+    m->err = x.msg;
+    return x.code;
+  } catch (const std::exception &x) {
+    m->err = x.what();
+    return PHASE_HIP_EINVAL;
+  }
+}
+
+// Device error bits of the last operator: raised as the reference's exceptions.
+void checkEq(phase_hip_equation *e, const char *op) {
+  int v = 0;
+  PH_CHECK(hipMemcpyAsync(&v, e->dErr.p, sizeof(int), hipMemcpyDeviceToHost, e->mesh->stream));
+  PH_CHECK(hipStreamSynchronize(e->mesh->stream));
+  if (!v) return;
+  PH_CHECK(hipMemsetAsync(e->dErr.p, 0, sizeof(int), e->mesh->stream));
+  if (v & 1) fail(PHASE_HIP_EBOUNDARY, std::string(op) + ": unrecognized or unspecified boundary type.");
+  if (v & 2) fail(PHASE_HIP_EINVAL, std::string(op) + ": a boundary or old-time input is missing");
+  fail(PHASE_HIP_EINVAL, std::string(op) + ": a row needs more slots than the equation holds");
+}
+
+void fresh(phase_hip_equation *e) { e->compact = false; }
+
+}  // namespace
+
+namespace phase {
+
+// Solver side of FiniteVolumeEquation<T>::solve (FiniteVolumeEquation.tpp:64-86):
+// A from the equation's rows, b = -rhs.  Defined in solver.hip.
+void solverSetFromRows(phase_hip_solver *s, int rank, int stride, const int *dCols, const double *dVals,
+                       const double *dRhs, hipStream_t producer);
+
+}  // namespace phase
+
+extern "C" {
+
+const char *phase_hip_equation_last_error(const phase_hip_equation *e) { return e ? e->err.c_str() : "null equation"; }
+
+int phase_hip_equation_create(phase_hip_equation **out, phase_hip_mesh *m, int ncomp) {
+  if (!out || !m || (ncomp != 1 && ncomp != 2)) return PHASE_HIP_EINVAL;
+  *out = nullptr;
+  phase_hip_equation *e = new phase_hip_equation();
+  e->mesh = m;
+  e->ncomp = ncomp;
+  e->rank = ncomp * m->nCells;
+  // diagonal + neighbours (+ one cross-component column for Vector2D)
+  e->stride = std::max(5, 1 + m->maxDeg + (ncomp == 2 ? 1 : 0));
+  const int rc = guardedEq(e, [&] {
+    if (e->stride > kRowCap) fail(PHASE_HIP_EINVAL, "a cell has more faces than an equation row holds");
+    PH_CHECK(hipSetDevice(m->device));
+    const size_t slots = (size_t)e->rank * e->stride;
+    e->cols.alloc(slots);
+    e->vals.alloc(slots);
+    e->cnt.alloc(e->rank);
+    e->rhs.alloc(e->rank);
+    e->dErr.alloc(1);
+    e->dErr.zero(m->stream);
+    // CrsEquation(N, 5): every slot empty
+    PH_CHECK(hipMemsetAsync(e->cols.p, 0xff, slots * sizeof(int), m->stream));
+    e->vals.zero(m->stream);
+    e->cnt.zero(m->stream);
+    e->rhs.zero(m->stream);
+    PH_CHECK(hipStreamSynchronize(m->stream));
+  });
+  if (rc != PHASE_HIP_OK) {
+    delete e;
+    return rc;
+  }
+  *out = e;
+  return PHASE_HIP_OK;
+}
+
+int phase_hip_equation_destroy(phase_hip_equation *e) {
+  delete e;
+  return PHASE_HIP_OK;
+}
+
+int phase_hip_equation_rank(const phase_hip_equation *e) { return e ? e->rank : 0; }
+
+int phase_hip_eq_laplacian(phase_hip_equation *e, int variant, double gamma, const double *gammaFace,
+                           const double *gamma0Face, double theta, const int *bcType, const double *phiFace,
+                           const double *phi0Cell, const double *phi0Face) {
+  return guardedEq(e, [&] {
+    if (variant < 0 || variant > 3) fail(PHASE_HIP_EINVAL, "laplacian variant must be 0..3");
+    if (variant >= 2 && (!gammaFace || (variant == 2 && !gamma0Face)))
+      fail(PHASE_HIP_EINVAL, "face-field laplacian needs gammaFace (and gamma0Face with theta)");
+    phase_hip_mesh *m = e->mesh;
+    PH_CHECK(hipSetDevice(m->device));
+    hipLaunchKernelGGL(kEqLaplacian, dim3(g256(m->nCells)), dim3(256), 0, m->stream, meshDev(m), e->ncomp,
+                       variant, gamma, gammaFace, gamma0Face, theta, bcType, phiFace, phi0Cell, phi0Face, e->stride,
+                       e->cols.p, e->vals.p, e->cnt.p, e->rhs.p, e->dErr.p);
+    PH_CHECK(hipGetLastError());
+    fresh(e);
+    checkEq(e, "fv::laplacian");
+  });
+}
+
+int phase_hip_eq_ddt(phase_hip_equation *e, double timeStep, const double *phi0Cell) {
+  return guardedEq(e, [&] {
+    if (!phi0Cell) fail(PHASE_HIP_EINVAL, "fv::ddt needs the old-time field");
+    phase_hip_mesh *m = e->mesh;
+    PH_CHECK(hipSetDevice(m->device));
+    hipLaunchKernelGGL(kEqDdt, dim3(g256(m->nCells)), dim3(256), 0, m->stream, meshDev(m), e->ncomp, timeStep,
+                       phi0Cell, e->stride, e->cols.p, e->vals.p, e->cnt.p, e->rhs.p);
+    PH_CHECK(hipGetLastError());
+    fresh(e);
+  });
+}
+
+int phase_hip_eq_div(phase_hip_equation *e, const double *uFace, const double *u0Face, double theta,
+                     const int *bcType, const double *phiFace, const double *phi0Cell, const double *phi0Face) {
+  return guardedEq(e, [&] {
+    if (!uFace || !u0Face || !phi0Cell) fail(PHASE_HIP_EINVAL, "fv::div needs u, u0 (faces) and phi0 (cells)");
+    phase_hip_mesh *m = e->mesh;
+    PH_CHECK(hipSetDevice(m->device));
+    hipLaunchKernelGGL(kEqDiv, dim3(g256(m->nCells)), dim3(256), 0, m->stream, meshDev(m), e->ncomp, uFace, u0Face,
+                       theta, bcType, phiFace, phi0Cell, phi0Face, e->stride, e->cols.p, e->vals.p, e->cnt.p,
+                       e->rhs.p, e->dErr.p);
+    PH_CHECK(hipGetLastError());
+    fresh(e);
+    checkEq(e, "fv::div");
+  });
+}
+
+int phase_hip_eq_add(phase_hip_equation *e, const phase_hip_equation *o, int sign) {
+  return guardedEq(e, [&] {
+    if (!o || o->rank != e->rank || o->stride != e->stride || o->mesh != e->mesh)
+      fail(PHASE_HIP_EINVAL, "operands of += / -= must be equations of the same mesh and rank");
+    phase_hip_mesh *m = e->mesh;
+    PH_CHECK(hipSetDevice(m->device));
+    hipLaunchKernelGGL(kEqMerge, dim3(g256(e->rank)), dim3(256), 0, m->stream, e->rank, e->stride, sign >= 0 ? 1 : -1,
+                       e->cols.p, e->vals.p, e->cnt.p, e->rhs.p, o->cols.p, o->vals.p, o->cnt.p, o->rhs.p, e->dErr.p);
+    PH_CHECK(hipGetLastError());
+    e->compact = true;
+    checkEq(e, "CrsEquation::operator+=");
+  });
+}
+
+int phase_hip_eq_copy(phase_hip_equation *dst, const phase_hip_equation *src) {
+  return guardedEq(dst, [&] {
+    if (!src || src->rank != dst->rank || src->stride != dst->stride)
+      fail(PHASE_HIP_EINVAL, "copy needs equations of the same rank");
+    hipStream_t st = dst->mesh->stream;
+    const size_t slots = (size_t)dst->rank * dst->stride;
+    PH_CHECK(hipMemcpyAsync(dst->cols.p, src->cols.p, slots * sizeof(int), hipMemcpyDeviceToDevice, st));
+    PH_CHECK(hipMemcpyAsync(dst->vals.p, src->vals.p, slots * sizeof(double), hipMemcpyDeviceToDevice, st));
+    PH_CHECK(hipMemcpyAsync(dst->cnt.p, src->cnt.p, dst->rank * sizeof(int), hipMemcpyDeviceToDevice, st));
+    PH_CHECK(hipMemcpyAsync(dst->rhs.p, src->rhs.p, dst->rank * sizeof(double), hipMemcpyDeviceToDevice, st));
+    dst->compact = src->compact;
+  });
+}
+
+int phase_hip_eq_scale(phase_hip_equation *e, double s) {
+  return guardedEq(e, [&] {
+    hipLaunchKernelGGL(kEqScale, dim3(g256((long)e->rank * e->stride)), dim3(256), 0, e->mesh->stream, e->rank,
+                       e->stride, s, e->vals.p, e->rhs.p);
+    PH_CHECK(hipGetLastError());
+  });
+}
+
+int phase_hip_eq_sub_vector(phase_hip_equation *e, const double *v) {
+  return guardedEq(e, [&] {
+    if (!v) fail(PHASE_HIP_EINVAL, "null vector");
+    hipLaunchKernelGGL(kEqSubRhs, dim3(g256(e->rank)), dim3(256), 0, e->mesh->stream, e->rank, v, 0., e->rhs.p);
+    PH_CHECK(hipGetLastError());
+  });
+}
+
+int phase_hip_eq_eq_scalar(phase_hip_equation *e, double s) {
+  return guardedEq(e, [&] {
+    if (s == 0.) return;  // CrsEquation.cpp:299-303
+    hipLaunchKernelGGL(kEqSubRhs, dim3(g256(e->rank)), dim3(256), 0, e->mesh->stream, e->rank, (const double *)nullptr,
+                       s, e->rhs.p);
+    PH_CHECK(hipGetLastError());
+  });
+}
+
+int phase_hip_eq_relax(phase_hip_equation *e, double alpha, const double *phi) {
+  return guardedEq(e, [&] {
+    if (!phi || !(alpha > 0.)) fail(PHASE_HIP_EINVAL, "relax needs alpha > 0 and the current field");
+    hipLaunchKernelGGL(kEqRelax, dim3(g256(e->rank)), dim3(256), 0, e->mesh->stream, e->rank, e->stride, alpha, phi,
+                       e->cols.p, e->vals.p, e->cnt.p, e->rhs.p);
+    PH_CHECK(hipGetLastError());
+  });
+}
+
+int phase_hip_eq_export(phase_hip_equation *e, int *nslots, int *rowPtr, int *colInd, double *vals, double *rhs) {
+  return guardedEq(e, [&] {
+    if (!nslots) fail(PHASE_HIP_EINVAL, "nslots is required");
+    hipStream_t st = e->mesh->stream;
+    const size_t slots = (size_t)e->rank * e->stride;
+    std::vector<int> c(slots), n(e->rank);
+    std::vector<double> v(slots), r(e->rank);
+    PH_CHECK(hipMemcpyAsync(c.data(), e->cols.p, slots * sizeof(int), hipMemcpyDeviceToHost, st));
+    PH_CHECK(hipMemcpyAsync(v.data(), e->vals.p, slots * sizeof(double), hipMemcpyDeviceToHost, st));
+    PH_CHECK(hipMemcpyAsync(n.data(), e->cnt.p, e->rank * sizeof(int), hipMemcpyDeviceToHost, st));
+    PH_CHECK(hipMemcpyAsync(r.data(), e->rhs.p, e->rank * sizeof(double), hipMemcpyDeviceToHost, st));
+    PH_CHECK(hipStreamSynchronize(st));
+    // fresh rows: the reference's 5 slots (-1 = empty) or more when a row
+    // grew; rebuilt rows: exactly their entries
+    int total = 0;
+    for (int i = 0; i < e->rank; ++i) total += e->compact ? n[i] : std::max(5, n[i]);
+    if (!rowPtr) {
+      *nslots = total;
+      return;
+    }
+    if (*nslots < total) fail(PHASE_HIP_EINVAL, "output arrays too small");
+    int w = 0;
+    rowPtr[0] = 0;
+    for (int i = 0; i < e->rank; ++i) {
+      const int width = e->compact ? n[i] : std::max(5, n[i]);
+      for (int j = 0; j < width; ++j, ++w) {
+        const bool used = j < n[i];
+        colInd[w] = used ? c[(size_t)i * e->stride + j] : -1;
+        vals[w] = used ? v[(size_t)i * e->stride + j] : 0.;
+      }
+      rowPtr[i + 1] = w;
+      if (rhs) rhs[i] = r[i];
+    }
+    *nslots = total;
+  });
+}
+
+int phase_hip_solver_set_equation(phase_hip_solver *s, const phase_hip_equation *e) {
+  if (!s || !e) return PHASE_HIP_EINVAL;
+  try {
+    solverSetFromRows(s, e->rank, e->stride, e->cols.p, e->vals.p, e->rhs.p, e->mesh->stream);
+    return PHASE_HIP_OK;
+  } catch (const HipError &x) {
+    s->err = x.msg;
+    return x.code;
+  } catch (const std::exception &x) {
+    s->err = x.what();
+    return PHASE_HIP_EINVAL;
+  }
+}
+
+int phase_hip_field_src_div(phase_hip_mesh *m, const double *uFace, double *out) {
+  return guardedM(m, [&] {
+    if (!uFace || !out) fail(PHASE_HIP_EINVAL, "null field");
+    PH_CHECK(hipSetDevice(m->device));
+    hipLaunchKernelGGL(kSrcDiv, dim3(g256(m->nCells)), dim3(256), 0, m->stream, meshDev(m), uFace, out);
+    PH_CHECK(hipGetLastError());
+  });
+}
+
+int phase_hip_field_src_src(phase_hip_mesh *m, int ncomp, const double *phiCell, double *out) {
+  return guardedM(m, [&] {
+    if (!phiCell || !out || (ncomp != 1 && ncomp != 2)) fail(PHASE_HIP_EINVAL, "bad field");
+    PH_CHECK(hipSetDevice(m->device));
+    hipLaunchKernelGGL(kSrcSrc, dim3(g256(m->nCells)), dim3(256), 0, m->stream, meshDev(m), ncomp, phiCell, out);
+    PH_CHECK(hipGetLastError());
+  });
+}
+
+int phase_hip_field_grad(phase_hip_mesh *m, const double *phiCell, const double *phiFace, double *gradFace,
+                         double *gradCell) {
+  return guardedM(m, [&] {
+    if (!phiCell || !phiFace || !gradFace || !gradCell) fail(PHASE_HIP_EINVAL, "null field");
+    PH_CHECK(hipSetDevice(m->device));
+    hipLaunchKernelGGL(kGradFaces, dim3(g256(m->nFaces)), dim3(256), 0, m->stream, m->nFaces, m->dFL.p, m->dFR.p,
+                       m->dCC.p, m->dFC.p, phiCell, phiFace, gradFace);
+    hipLaunchKernelGGL(kGradCells, dim3(g256(m->nCells)), dim3(256), 0, m->stream, meshDev(m), gradFace, gradCell);
+    PH_CHECK(hipGetLastError());
+  });
+}
+
+int phase_hip_field_interpolate_faces(phase_hip_mesh *m, int ncomp, const double *phiCell, const int *bcType,
+                                      double *phiFace) {
+  return guardedM(m, [&] {
+    if (!phiCell || !phiFace || (ncomp != 1 && ncomp != 2)) fail(PHASE_HIP_EINVAL, "bad field");
+    PH_CHECK(hipSetDevice(m->device));
+    hipLaunchKernelGGL(kInterpFaces, dim3(g256(m->nFaces)), dim3(256), 0, m->stream, m->nFaces, ncomp, m->dFL.p,
+                       m->dFR.p, m->dCC.p, m->dFC.p, m->dSf.p, bcType, phiCell, phiFace);
+    PH_CHECK(hipGetLastError());
+  });
+}
+
+int phase_hip_mesh_synchronize(phase_hip_mesh *m) {
+  return guardedM(m, [&] { PH_CHECK(hipStreamSynchronize(m->stream)); });
+}
+
+}  // extern "C"

@@ -22,23 +22,12 @@
 #include <unordered_map>
 #include <vector>
 
+#include "mesh.h"
 #include "solver.h"
 #include "solver_kernels.h"
 
 using namespace phase;
 
-struct phase_hip_mesh {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  std::string err;
-  int nNodes = 0, nCells = 0, nFaces = 0, nInterior = 0, nBoundary = 0;
-  std::vector<int> fNode1, fNode2, fL, fR;
-  std::vector<int> iptr, iface, inb, bptr, bface;
-  DBuf<double> dXY, dVol, dCC, dFC, dSf;
-  DBuf<int> dCptr, dCind, dFNode1, dFNode2, dFL;
-  DBuf<int> dIptr, dIface, dInb, dBptr, dBface;
-  DBuf<double> dIDot, dIMs, dBDot, dBMs;  // per link: dot(r, Sf_out), |r|^2
-};
 
 namespace phase {
 
@@ -69,15 +58,6 @@ void solverFreeBinding(phase_hip_solver *s) {
 
 namespace {
 
-struct Dv2 {
-  double x, y;
-};
-
-__device__ __forceinline__ Dv2 outward(Dv2 fc, Dv2 sf, Dv2 p) {
-  // Face::outwardNorm(point): dot(centroid - point, normal) > 0 ? normal : -normal
-  const double dx = fc.x - p.x, dy = fc.y - p.y;
-  return (dx * sf.x + dy * sf.y) > 0. ? sf : Dv2{-sf.x, -sf.y};
-}
 
 __global__ void kCellGeom(int nCells, const int *__restrict__ cptr, const int *__restrict__ cind,
                           const double *__restrict__ xy, double *__restrict__ vol, double *__restrict__ cc) {
@@ -390,6 +370,9 @@ int phase_hip_mesh_create(phase_hip_mesh **out, int device, int nNodes, const do
     m->dCind.upload(std::vector<int>(cind, cind + cptr[nCells]), st);
     m->dFNode1.upload(m->fNode1, st);
     m->dFNode2.upload(m->fNode2, st);
+    m->dFL.upload(m->fL, st);
+    m->dFR.upload(m->fR, st);
+    for (int c = 0; c < nCells; ++c) m->maxDeg = std::max(m->maxDeg, m->iptr[c + 1] - m->iptr[c]);
     m->dIptr.upload(m->iptr, st);
     m->dIface.upload(m->iface.empty() ? std::vector<int>{0} : m->iface, st);
     m->dInb.upload(m->inb.empty() ? std::vector<int>{0} : m->inb, st);

@@ -84,6 +84,10 @@ struct phase_hip_solver {
   phase::DBuf<double> dE64, dQ64;
   int lastCaptureIter = 0;
 
+  // pattern of the last equation handed over by phase_hip_solver_set_equation
+  phase::DBuf<int> dEqCols, dEqFlag;
+  int eqStride = 0;
+
   // mesh-assembled operator (mesh.hip)
   phase::MeshBinding *binding = nullptr;
   phase_hip_mesh *boundMesh = nullptr;

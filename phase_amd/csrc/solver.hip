@@ -526,6 +526,7 @@ void solverSetPattern(phase_hip_solver *s, int n, const int *rowPtr, const int *
   s->dFacSrc.upload(o.facSrc, st);
   s->dPerm.upload(o.perm, st);
   s->dSell64.alloc(o.sellCol.size());
+  s->eqStride = 0;  // the pattern no longer comes from an equation (set_equation re-marks it)
   s->patRowPtr.assign(rowPtr, rowPtr + n + 1);
   s->patColInd.assign(colInd, colInd + rowPtr[n]);
   s->dVals.alloc(std::max<size_t>(1, (size_t)rowPtr[n]));
@@ -537,6 +538,70 @@ void solverSetPattern(phase_hip_solver *s, int n, const int *rowPtr, const int *
 }
 
 void solverEnsureSingle(phase_hip_solver *s) { ensureSingle(s); }
+
+}  // namespace phase
+
+namespace {
+__global__ void kCmpInt(size_t n, const int *__restrict__ a, const int *__restrict__ b, int *flag) {
+  bool diff = false;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    diff |= a[i] != b[i];
+  if (diff) atomicOr(flag, 1);
+}
+}  // namespace
+
+namespace phase {
+
+void solverSetFromRows(phase_hip_solver *s, int rank, int stride, const int *dCols, const double *dVals,
+                       const double *dRhs, hipStream_t producer) {
+  setDevice(s);
+  if (rank <= 0) fail(PHASE_HIP_EINVAL, "empty equation");
+  PH_CHECK(hipStreamSynchronize(producer));
+  const size_t slots = (size_t)rank * stride;
+  bool same = s->havePattern && !s->binding && s->rank == rank && s->eqStride == stride && s->dEqCols.n == slots;
+  if (same) {
+    // the pattern follows the values (CrsEquation drops zeros): compare on the device
+    s->dEqFlag.alloc(1);
+    s->dEqFlag.zero(s->stream);
+    hipLaunchKernelGGL(kCmpInt, dim3(gridFor((long)slots, 256, kElemCap)), dim3(256), 0, s->stream, slots, dCols,
+                       s->dEqCols.p, s->dEqFlag.p);
+    int flag = 0;
+    PH_CHECK(hipMemcpyAsync(&flag, s->dEqFlag.p, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    PH_CHECK(hipStreamSynchronize(s->stream));
+    same = flag == 0;
+  }
+  if (!same) {
+    std::vector<int> cols(slots), rowPtr(rank + 1);
+    PH_CHECK(hipMemcpy(cols.data(), dCols, slots * sizeof(int), hipMemcpyDeviceToHost));
+    for (int i = 0; i <= rank; ++i) rowPtr[i] = i * stride;
+    // the last solution survives a new ordering of the same rows (warm start)
+    DBuf<double> keep;
+    const bool warm = s->havePattern && s->rank == rank && (s->haveSolution || s->haveGuess);
+    if (warm) {
+      keep.alloc(rank);
+      hipLaunchKernelGGL((kPermuteOut<double>), dim3(gridFor(rank, 256, kElemCap)), dim3(256), 0, s->stream, rank,
+                         s->dPerm.p, s->dX.p, keep.p);
+      PH_CHECK(hipStreamSynchronize(s->stream));
+    }
+    solverFreeBinding(s);
+    s->rank = 0;
+    solverSetPattern(s, rank, rowPtr.data(), cols.data());
+    if (warm) {
+      hipLaunchKernelGGL((kPermuteIn<double, double>), dim3(gridFor(rank, 256, kElemCap)), dim3(256), 0, s->stream,
+                         rank, s->dPerm.p, keep.p, s->dX.p, 1.0);
+      PH_CHECK(hipStreamSynchronize(s->stream));  // before `keep` is freed
+    }
+    s->dEqCols.alloc(slots);  // after solverSetPattern, which clears eqStride
+    PH_CHECK(hipMemcpyAsync(s->dEqCols.p, dCols, slots * sizeof(int), hipMemcpyDeviceToDevice, s->stream));
+    s->eqStride = stride;
+  }
+  PH_CHECK(hipMemcpyAsync(s->dVals.p, dVals, slots * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+  solverValuesChanged(s);
+  hipLaunchKernelGGL((kPermuteIn<double, double>), dim3(gridFor(rank, 256, kElemCap)), dim3(256), 0, s->stream,
+                     rank, s->dPerm.p, dRhs, s->dB.p, -1.0);
+  PH_CHECK(hipGetLastError());
+  s->haveRhs = true;
+}
 
 template <class T>
 void iluApplyDev(phase_hip_solver *s, T *fac, const T *in, T *tmp, T *out) {
@@ -771,6 +836,32 @@ int phase_hip_solver_get_x(phase_hip_solver *s, double *x) {
                        s->dPerm.p, s->dX.p, s->dR64.p);
     PH_CHECK(hipMemcpyAsync(x, s->dR64.p, n * sizeof(double), hipMemcpyDeviceToHost, s->stream));
     PH_CHECK(hipStreamSynchronize(s->stream));
+  });
+}
+
+int phase_hip_solver_get_x_device(phase_hip_solver *s, double *x) {
+  return guarded(s, [&] {
+    if (!s->havePattern) fail(PHASE_HIP_ESTATE, "no system");
+    if (!x) fail(PHASE_HIP_EINVAL, "null output");
+    setDevice(s);
+    const int n = s->rank;
+    hipLaunchKernelGGL((kPermuteOut<double>), dim3(gridFor(n, 256, kElemCap)), dim3(256), 0, s->stream, n,
+                       s->dPerm.p, s->dX.p, x);
+    PH_CHECK(hipGetLastError());
+    PH_CHECK(hipStreamSynchronize(s->stream));
+  });
+}
+
+int phase_hip_solver_set_guess_device(phase_hip_solver *s, const double *x0) {
+  return guarded(s, [&] {
+    if (!s->havePattern) fail(PHASE_HIP_ESTATE, "setGuess before set()");
+    if (!x0) fail(PHASE_HIP_EINVAL, "null guess");
+    setDevice(s);
+    const int n = s->rank;
+    hipLaunchKernelGGL((kPermuteIn<double, double>), dim3(gridFor(n, 256, kElemCap)), dim3(256), 0, s->stream, n,
+                       s->dPerm.p, x0, s->dX.p, 1.0);
+    PH_CHECK(hipGetLastError());
+    s->haveGuess = true;
   });
 }
 

@@ -75,6 +75,29 @@ _SIGS = {
     "phase_hip_learned_set_params": (ctypes.c_int, [_vp, _vp, ctypes.c_int]),
     "phase_hip_learned_eval": (ctypes.c_int, [_vp, _f64p, _f64p, _f64p, _f64p, _dp, _vp]),
     "phase_hip_learned_apply": (ctypes.c_int, [_vp, _f64p, _f64p]),
+    "phase_hip_equation_create": (ctypes.c_int, [ctypes.POINTER(_vp), _vp, ctypes.c_int]),
+    "phase_hip_equation_destroy": (ctypes.c_int, [_vp]),
+    "phase_hip_equation_rank": (ctypes.c_int, [_vp]),
+    "phase_hip_equation_last_error": (ctypes.c_char_p, [_vp]),
+    "phase_hip_eq_laplacian": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, _vp, _vp, ctypes.c_double, _vp,
+                                              _vp, _vp, _vp]),
+    "phase_hip_eq_ddt": (ctypes.c_int, [_vp, ctypes.c_double, _vp]),
+    "phase_hip_eq_div": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_double, _vp, _vp, _vp, _vp]),
+    "phase_hip_eq_add": (ctypes.c_int, [_vp, _vp, ctypes.c_int]),
+    "phase_hip_eq_scale": (ctypes.c_int, [_vp, ctypes.c_double]),
+    "phase_hip_eq_sub_vector": (ctypes.c_int, [_vp, _vp]),
+    "phase_hip_eq_eq_scalar": (ctypes.c_int, [_vp, ctypes.c_double]),
+    "phase_hip_eq_relax": (ctypes.c_int, [_vp, ctypes.c_double, _vp]),
+    "phase_hip_eq_copy": (ctypes.c_int, [_vp, _vp]),
+    "phase_hip_eq_export": (ctypes.c_int, [_vp, _ip, _vp, _vp, _vp, _vp]),
+    "phase_hip_solver_set_equation": (ctypes.c_int, [_vp, _vp]),
+    "phase_hip_solver_get_x_device": (ctypes.c_int, [_vp, _vp]),
+    "phase_hip_solver_set_guess_device": (ctypes.c_int, [_vp, _vp]),
+    "phase_hip_field_src_div": (ctypes.c_int, [_vp, _vp, _vp]),
+    "phase_hip_field_src_src": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp]),
+    "phase_hip_field_grad": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "phase_hip_field_interpolate_faces": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp]),
+    "phase_hip_mesh_synchronize": (ctypes.c_int, [_vp]),
 }
 
 _lib = None
@@ -131,6 +154,28 @@ class DeviceArray:
         if rc != 0:
             raise PhaseHipError(rc, "malloc", "device allocation failed")
         rc = lib.phase_hip_memcpy(self.ptr, self.host.ctypes.data_as(_vp), self.nbytes, 1)
+        if rc != 0:
+            raise PhaseHipError(rc, "memcpy", "host to device copy failed")
+
+    @classmethod
+    def zeros(cls, n, dtype=np.float64, device=0):
+        """A zero-filled device array of n elements."""
+        return cls(np.zeros(n, dtype), device)
+
+    def get(self):
+        """Copy the device contents back into a new host array."""
+        out = np.empty_like(self.host)
+        rc = load_library().phase_hip_memcpy(out.ctypes.data_as(_vp), self.ptr, self.nbytes, 0)
+        if rc != 0:
+            raise PhaseHipError(rc, "memcpy", "device to host copy failed")
+        return out
+
+    def set(self, host):
+        """Overwrite the device contents from a host array of the same size."""
+        h = np.ascontiguousarray(host, dtype=self.host.dtype)
+        if h.nbytes != self.nbytes:
+            raise ValueError("size mismatch")
+        rc = load_library().phase_hip_memcpy(self.ptr, h.ctypes.data_as(_vp), self.nbytes, 1)
         if rc != 0:
             raise PhaseHipError(rc, "memcpy", "host to device copy failed")
 
@@ -191,6 +236,120 @@ class HipMesh:
         if rc != 0:
             raise PhaseHipError(rc, "geometry", load_library().phase_hip_mesh_last_error(self._h).decode())
         return vol, cc.reshape(N, 2), fc.reshape(F, 2), sf.reshape(F, 2)
+
+
+    def _mcheck(self, rc, method):
+        if rc != 0:
+            raise PhaseHipError(rc, method, load_library().phase_hip_mesh_last_error(self._h).decode())
+
+    def synchronize(self):
+        self._mcheck(load_library().phase_hip_mesh_synchronize(self._h), "synchronize")
+
+    # --- field kernels (device arrays in and out; Source.cpp, ScalarGradient.cpp,
+    # FiniteVolumeField.tpp)
+    def src_div(self, u_face, out):
+        self._mcheck(load_library().phase_hip_field_src_div(self._h, u_face.ptr, out.ptr), "src::div")
+        self.synchronize()
+
+    def src_src(self, phi_cell, out, ncomp=1):
+        self._mcheck(load_library().phase_hip_field_src_src(self._h, ncomp, phi_cell.ptr, out.ptr), "src::src")
+        self.synchronize()
+
+    def grad(self, phi_cell, phi_face, grad_face, grad_cell):
+        self._mcheck(load_library().phase_hip_field_grad(self._h, phi_cell.ptr, phi_face.ptr, grad_face.ptr,
+                                                         grad_cell.ptr), "grad")
+        self.synchronize()
+
+    def interpolate_faces(self, phi_cell, phi_face, bc_type=None, ncomp=1):
+        self._mcheck(load_library().phase_hip_field_interpolate_faces(self._h, ncomp, phi_cell.ptr,
+                                                                      _dp_or_none(bc_type), phi_face.ptr),
+                     "interpolateFaces")
+        self.synchronize()
+
+
+class HipEquation:
+    """FiniteVolumeEquation<T> / CrsEquation held on the device; operator
+    inputs are DeviceArray objects (HBM-resident)."""
+
+    def __init__(self, mesh, ncomp=1):
+        h = _vp()
+        rc = load_library().phase_hip_equation_create(ctypes.byref(h), mesh._h, ncomp)
+        if rc != 0:
+            raise PhaseHipError(rc, "equationCreate", "equation creation failed")
+        self._h = h
+        self.mesh = mesh
+        self.ncomp = ncomp
+        self.rank = load_library().phase_hip_equation_rank(h)
+
+    def _check(self, rc, method):
+        if rc != 0:
+            raise PhaseHipError(rc, method, load_library().phase_hip_equation_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().phase_hip_equation_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def laplacian(self, variant, gamma=1.0, gamma_face=None, gamma0_face=None, theta=1.0, bc_type=None,
+                  phi_face=None, phi0_cell=None, phi0_face=None):
+        d = _dp_or_none
+        self._check(load_library().phase_hip_eq_laplacian(self._h, variant, float(gamma), d(gamma_face),
+                                                          d(gamma0_face), float(theta), d(bc_type), d(phi_face),
+                                                          d(phi0_cell), d(phi0_face)), "fv::laplacian")
+        return self
+
+    def ddt(self, time_step, phi0_cell):
+        self._check(load_library().phase_hip_eq_ddt(self._h, float(time_step), phi0_cell.ptr), "fv::ddt")
+        return self
+
+    def div(self, u_face, u0_face, theta=1.0, bc_type=None, phi_face=None, phi0_cell=None, phi0_face=None):
+        d = _dp_or_none
+        self._check(load_library().phase_hip_eq_div(self._h, u_face.ptr, u0_face.ptr, float(theta), d(bc_type),
+                                                    d(phi_face), d(phi0_cell), d(phi0_face)), "fv::div")
+        return self
+
+    def __iadd__(self, other):
+        self._check(load_library().phase_hip_eq_add(self._h, other._h, 1), "operator+=")
+        return self
+
+    def __isub__(self, other):
+        self._check(load_library().phase_hip_eq_add(self._h, other._h, -1), "operator-=")
+        return self
+
+    def scale(self, s):
+        self._check(load_library().phase_hip_eq_scale(self._h, float(s)), "operator*=")
+        return self
+
+    def sub_vector(self, v):
+        self._check(load_library().phase_hip_eq_sub_vector(self._h, v.ptr), "operator-=")
+        return self
+
+    def eq_scalar(self, s):
+        self._check(load_library().phase_hip_eq_eq_scalar(self._h, float(s)), "operator==")
+        return self
+
+    def relax(self, alpha, phi_rows):
+        self._check(load_library().phase_hip_eq_relax(self._h, float(alpha), phi_rows.ptr), "relax")
+        return self
+
+    def copy_from(self, other):
+        self._check(load_library().phase_hip_eq_copy(self._h, other._h), "copy")
+        return self
+
+    def export(self):
+        """(rowPtr, colInd, vals, rhs) in the reference's CrsEquation layout."""
+        n = ctypes.c_int()
+        self._check(load_library().phase_hip_eq_export(self._h, n, None, None, None, None), "export")
+        rp = np.empty(self.rank + 1, np.int32)
+        ci, v, rhs = np.empty(n.value, np.int32), np.empty(n.value), np.empty(self.rank)
+        self._check(load_library().phase_hip_eq_export(self._h, n, _p(rp), _p(ci), _p(v), _p(rhs)), "export")
+        return rp, ci, v, rhs
 
 
 class HipSparseMatrixSolver:
@@ -264,6 +423,18 @@ class HipSparseMatrixSolver:
         a, b, c, d = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
         self._check(load_library().phase_hip_solver_stats(self._h, a, b, c, d), "stats")
         return dict(inner_iters=a.value, outer_passes=b.value, fallbacks=c.value, solve_ms=d.value)
+
+    def set_equation(self, eq):
+        """FiniteVolumeEquation<T>::solve's hand-over: A and b = -rhs from a
+        device equation, device to device."""
+        self._check(load_library().phase_hip_solver_set_equation(self._h, eq._h), "setEquation")
+        self.rank = eq.rank
+
+    def x_device(self, out):
+        self._check(load_library().phase_hip_solver_get_x_device(self._h, out.ptr), "x")
+
+    def set_guess_device(self, x0):
+        self._check(load_library().phase_hip_solver_set_guess_device(self._h, x0.ptr), "setGuess")
 
     def ordering_info(self):
         """ILU blocks and block colours of the device ordering."""
