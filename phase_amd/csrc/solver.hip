@@ -179,18 +179,23 @@ void checkDeviceErr(phase_hip_solver *s) {
   }
 }
 
+// fac = [L slots | U slots | diagonal (n) | inverse diagonal (n)]
+size_t facSize(const phase_hip_solver *s) { return (size_t)s->ord.nLSlots + s->ord.nUSlots + 2 * (size_t)s->rank; }
+
 template <class T>
 void factorize(phase_hip_solver *s, DBuf<T> &fac) {
-  const int m = (int)fac.n;
+  const int m = (int)(fac.n - s->rank);  // gathered entries: L, U, diagonal
   hipLaunchKernelGGL((kGather<T>), dim3(gridFor(m, 256, kElemCap)), dim3(256), 0, s->stream, m,
                      s->dFacSrc.p, s->dVals.p, fac.p);
   launchFactor<T>(s, iluDev<T>(s, fac.p));
+  hipLaunchKernelGGL((kInvDiag<T>), dim3(gridFor(s->rank, 256, kElemCap)), dim3(256), 0, s->stream, s->rank, fac.p,
+                     s->ord.nLSlots + s->ord.nUSlots);
   PH_CHECK(hipGetLastError());
 }
 
 void ensureFactor64(phase_hip_solver *s) {
   if (s->fac64Valid) return;
-  s->dFac64.alloc((size_t)s->ord.nLSlots + s->ord.nUSlots + s->rank);
+  s->dFac64.alloc(facSize(s));
   factorize<double>(s, s->dFac64);
   checkDeviceErr(s);
   s->fac64Valid = true;

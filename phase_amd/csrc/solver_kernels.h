@@ -582,6 +582,13 @@ __global__ __launch_bounds__(64) void kIluFactor(IluDev<T> L, const int *__restr
 // needs; (3) the solution values of earlier-coloured rows.  The in-block
 // triangular solve then runs level by level through LDS.
 
+// fac[dbase + n + i] = 1 / u_ii (read by the backward sweep)
+template <class T>
+__global__ void kInvDiag(int n, T *fac, int dbase) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    fac[dbase + n + i] = (T)1 / fac[dbase + i];
+}
+
 // Forward sweep body for one block: returns y of the lane's row.  PRO selects
 // the fused BiCGStab prologue that forms the right-hand side on the fly:
 // kProP: p = r + beta (p - omega v);  kProS: s = r - alpha v (both stored).
@@ -670,7 +677,7 @@ __device__ __forceinline__ T lBody(const IluDev<T> &L, const SyncDev &sy, const 
 // precede later-block columns, as in the oracle's back substitution.
 template <class T>
 struct UPre {
-  T d;
+  T d;  // 1 / u_ii
   int nIn, nOff, ulev;
   T uv[kMaxIn];
   int uc[kMaxIn];
@@ -685,7 +692,8 @@ __device__ __forceinline__ void uLoad(const IluDev<T> &L, const BlockInfo &B, bo
   u.nIn = act ? (int)((info >> 10) & 15u) : 0;
   u.nOff = act ? (int)((info >> 14) & 15u) : 0;
   u.ulev = (int)((info >> 24) & 63u);
-  const T d0 = L.fac[L.nLSlots + L.nUSlots + (act ? row : 0)];  // unconditional, as in lBody
+  // inverse diagonal, stored after the diagonal: unconditional load, as in lBody
+  const T d0 = L.fac[L.nLSlots + L.nUSlots + L.n + (act ? row : 0)];
   u.d = act ? d0 : (T)1;
 #pragma unroll
   for (int q = 0; q < kMaxIn; ++q) {
@@ -717,15 +725,19 @@ __device__ __forceinline__ T uSolve(const IluDev<T> &L, const SyncDev &sy, const
   T ox[kMaxOffU];
 #pragma unroll
   for (int q = 0; q < kMaxOffU; ++q) ox[q] = q < u.nOff ? x[u.oc[q]] : (T)0;
+  // later-block terms are final before the levels start: their sum leaves the
+  // level loop, whose body (issued for the whole wave at every level) keeps
+  // only the in-block terms and one multiply by the inverse diagonal
+  T soff = 0;
+#pragma unroll
+  for (int q = 0; q < kMaxOffU; ++q)
+    if (q < u.nOff) soff = fma(u.ou[q], ox[q], soff);
   for (int l = 0; l < B.nULev; ++l) {
     if (act && u.ulev == l) {
 #pragma unroll
       for (int q = 0; q < kMaxIn; ++q)
         if (q < u.nIn) acc = fma(-u.uv[q], xlds[u.uc[q] - B.rowStart], acc);
-#pragma unroll
-      for (int q = 0; q < kMaxOffU; ++q)
-        if (q < u.nOff) acc = fma(-u.ou[q], ox[q], acc);
-      acc = acc / u.d;
+      acc = (acc - soff) * u.d;
       xlds[lane] = acc;
     }
     __syncthreads();

@@ -228,26 +228,31 @@ extern "C" int emul_run(int n, const int *rowPtr, const int *colInd, const doubl
         REQ(o.succ[q] > b, "succ block not later");
         REQ(doneBlk[o.succ[q]], "bwd succ not finished");
       }
-      std::vector<double> acc(64, 0.), lds(64, 0.);
-      for (int lane = 0; lane < B.nRows; ++lane) acc[lane] = y[B.rowStart + lane];
+      std::vector<double> acc(64, 0.), lds(64, 0.), soff(64, 0.);
+      for (int lane = 0; lane < B.nRows; ++lane) {
+        const int row = B.rowStart + lane;
+        const RowInfo &ri = o.rows[row];
+        acc[lane] = y[row];
+        // later-block terms: one sum, formed before the levels (kUSolve)
+        REQ(ri.nU - ri.nUin <= 8, "too many off-block U entries");
+        for (int e = ri.nUin; e < ri.nU; ++e) {
+          const int s = uslot(B, e, lane);
+          REQ(o.uCol[s] >= B.rowStart + B.nRows && rowDone[o.uCol[s]], "bwd off-block value not ready");
+          soff[lane] = std::fma(fac[o.nLSlots + s], x[o.uCol[s]], soff[lane]);
+        }
+      }
       for (int lev = 0; lev < B.nULev; ++lev) {
         for (int lane = 0; lane < B.nRows; ++lane) {
           const int row = B.rowStart + lane;
           const RowInfo &ri = o.rows[row];
           if (ri.ulev != lev) continue;
-          REQ(ri.nU - ri.nUin <= 8, "too many off-block U entries");
           for (int e = 0; e < ri.nUin; ++e) {
             const int s = uslot(B, e, lane);
             const int li = o.uCol[s] - B.rowStart;
             REQ(li > lane && li < B.nRows && rowDone[o.uCol[s]], "bwd in-block value not ready");
             acc[lane] = std::fma(-fac[o.nLSlots + s], lds[li], acc[lane]);
           }
-          for (int e = ri.nUin; e < ri.nU; ++e) {
-            const int s = uslot(B, e, lane);
-            REQ(o.uCol[s] >= B.rowStart + B.nRows && rowDone[o.uCol[s]], "bwd off-block value not ready");
-            acc[lane] = std::fma(-fac[o.nLSlots + s], x[o.uCol[s]], acc[lane]);
-          }
-          acc[lane] = acc[lane] / fac[dbase + row];
+          acc[lane] = (acc[lane] - soff[lane]) * (1.0 / fac[dbase + row]);
           lds[lane] = acc[lane];
           x[row] = acc[lane];
         }
