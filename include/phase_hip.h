@@ -325,6 +325,31 @@ int phase_hip_field_interpolate_faces(phase_hip_mesh *m, int ncomp, const double
 /* Wait for the work queued on the mesh's stream. */
 int phase_hip_mesh_synchronize(phase_hip_mesh *m);
 
+/* ------------------------------------------------------------------ SIMPLE driver
+ * No counterpart module in the reference snapshot (its solvers are projection
+ * methods, src/2D/Unstructured/Solvers/FractionalStep.cpp:79-117); built from
+ * the README's SIMPLE equations (README.md:27-37) and Patankar's algorithm,
+ * with the implicit under-relaxation of ScalarFiniteVolumeEquation.cpp:45-55:
+ *   uEqn = rho div(u_f, u) == laplacian(mu, u) - src(grad p), relax(alphaU)
+ *   pCorrEqn = laplacian(rho d_f, p') == src::div(u_f)   (Rhie-Chow u_f)
+ *   u -= d grad p', u_f -= d_f grad_f p', p += alphaP p'
+ * Velocity FIXED on every boundary face (uBcType per face, uBoundary 2F host
+ * values); pressure zero-gradient, so p' is pinned to 0 at cell pinCell. */
+typedef struct phase_hip_simple phase_hip_simple;
+
+int phase_hip_simple_create(phase_hip_simple **s, phase_hip_mesh *m, double rho, double mu, double alphaU,
+                            double alphaP, const int *uBcType, const double *uBoundary, int pinCell);
+int phase_hip_simple_destroy(phase_hip_simple *s);
+const char *phase_hip_simple_last_error(const phase_hip_simple *s);
+/* Inner-solver settings: "u.<key>" / "p.<key>" with phase_hip_solver_setup's keys. */
+int phase_hip_simple_setup(phase_hip_simple *s, const char *key, const char *value);
+/* nIters outer iterations; history (optional, 4 per iteration): momentum and
+ * pressure-correction BiCGStab iterations, max |mass imbalance| before the
+ * correction, max |p'|. */
+int phase_hip_simple_iterate(phase_hip_simple *s, int nIters, double *history);
+/* Host copies of u (cells, interleaved), p (cells), u (faces); NULL skips. */
+int phase_hip_simple_fields(phase_hip_simple *s, double *uCell, double *pCell, double *uFace);
+
 #ifdef __cplusplus
 }
 #endif

@@ -25,21 +25,12 @@
 #include <string>
 #include <vector>
 
+#include "equation.h"
 #include "mesh.h"
 #include "solver.h"
 #include "solver_kernels.h"
 
 using namespace phase;
-
-struct phase_hip_equation {
-  phase_hip_mesh *mesh = nullptr;
-  int ncomp = 1, rank = 0, stride = 0;
-  DBuf<int> cols, cnt;
-  DBuf<double> vals, rhs;
-  DBuf<int> dErr;
-  bool compact = false;  // rows rebuilt by += / -= (no empty slots)
-  std::string err;
-};
 
 namespace {
 

@@ -1,0 +1,364 @@
+// SIMPLE driver (Patankar & Spalding) for steady incompressible flow on the
+// device: the momentum and pressure-correction equations of the reference's
+// README (README.md:27-37), assembled with the device fv:: operators and solved
+// through the device solver plugin, with collocated Rhie-Chow face velocities.
+// The reference snapshot ships no SIMPLE module (its solvers are projection
+// methods, src/2D/Unstructured/Solvers/FractionalStep.cpp); this follows the
+// README equations and the textbook algorithm, with the implicit
+// under-relaxation that survives in the comment of
+// ScalarFiniteVolumeEquation.cpp:45-55.
+//
+// One outer iteration, all fields in HBM:
+//   p_f = setBoundaryFaces/interpolateFaces(p);  grad p (ScalarGradient)
+//   uEqn = rho div(u_f, u) == laplacian(mu, u) - src(grad p);  relax(alpha_u)
+//   d_P = V_P / a_P (relaxed diagonal);  solve uEqn -> u*
+//   u_f = avg(u*) + avg(d) (avg(grad p) - grad_f p)       (Rhie-Chow)
+//   pCorrEqn = laplacian(rho avg(d), p') == src::div(u_f), one cell pinned to 0
+//   u -= d grad p';  u_f -= avg(d) grad_f p';  p += alpha_p p'
+// Boundaries: velocity FIXED faces take their prescribed value; pressure and
+// pressure correction are zero-gradient (NORMAL_GRADIENT) everywhere, so the
+// pressure-correction matrix is singular and the `pin` cell's row is replaced
+// by p'_pin = 0.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "equation.h"
+#include "mesh.h"
+#include "solver.h"
+#include "solver_kernels.h"
+
+using namespace phase;
+
+struct phase_hip_simple {
+  phase_hip_mesh *mesh = nullptr;
+  int N = 0, F = 0, pin = 0;
+  double rho = 1., mu = 1., alphaU = 0.7, alphaP = 0.3;
+  phase_hip_equation *uEq = nullptr, *tmp = nullptr, *pEq = nullptr;
+  phase_hip_solver *su = nullptr, *sp = nullptr;
+  DBuf<double> uC, uB, uF, uWall, pC, pF, gpF, gpC, gsrc, d, df, rdf, m, pp, ppF, gppF, gppC, zeroF, maxBuf;
+  DBuf<int> uBc, pBc;
+  std::string err;
+};
+
+namespace {
+
+inline int g256(long n) { return (int)std::max<long>(1, std::min<long>((n + 255) / 256, 8192)); }
+
+// interleaved (x0, y0, x1, y1, ...) <-> block (x0..xN-1, y0..yN-1)
+__global__ void kToBlocks(int n, const double *__restrict__ in, double *__restrict__ out) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+    out[c] = in[2 * c];
+    out[n + c] = in[2 * c + 1];
+  }
+}
+__global__ void kFromBlocks(int n, const double *__restrict__ in, double *__restrict__ out) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+    out[2 * c] = in[c];
+    out[2 * c + 1] = in[n + c];
+  }
+}
+
+// d_P = V_P / a_P from the diagonal of the x-momentum row of cell P
+__global__ void kMomentumD(int n, int stride, const int *__restrict__ cols, const double *__restrict__ vals,
+                           const int *__restrict__ cnt, const double *__restrict__ vol, double *d) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+    double ap = 0.;
+    for (int j = 0; j < cnt[c]; ++j)
+      if (cols[(size_t)c * stride + j] == c) ap = vals[(size_t)c * stride + j];
+    d[c] = vol[c] / ap;
+  }
+}
+
+// Rhie-Chow face velocity; rdf = rho * avg(d) (pressure-correction
+// coefficient), df = avg(d).  Face weights are Face::distanceWeight
+// (Face.cpp:60-64), as in interpolateFaces.
+__global__ void kRhieChow(int nFaces, const int *__restrict__ fl, const int *__restrict__ fr,
+                          const double *__restrict__ cc, const double *__restrict__ fc,
+                          const double *__restrict__ uC, const double *__restrict__ d,
+                          const double *__restrict__ gpC, const double *__restrict__ gpF,
+                          const double *__restrict__ uWall, double rho, double *uF, double *df, double *rdf) {
+  for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < nFaces; f += gridDim.x * blockDim.x) {
+    const int l = fl[f], r = fr[f];
+    if (r < 0) {
+      uF[2 * f] = uWall[2 * f];
+      uF[2 * f + 1] = uWall[2 * f + 1];
+      df[f] = d[l];
+      rdf[f] = rho * d[l];
+      continue;
+    }
+    const double ax = fc[2 * f] - cc[2 * r], ay = fc[2 * f + 1] - cc[2 * r + 1];
+    const double bx = fc[2 * f] - cc[2 * l], by = fc[2 * f + 1] - cc[2 * l + 1];
+    const double l1 = sqrt(ax * ax + ay * ay), l2 = sqrt(bx * bx + by * by);
+    const double g = l1 / (l1 + l2);
+    const double dd = g * d[l] + (1. - g) * d[r];
+    for (int q = 0; q < 2; ++q) {
+      const double ub = g * uC[2 * l + q] + (1. - g) * uC[2 * r + q];
+      const double gb = g * gpC[2 * l + q] + (1. - g) * gpC[2 * r + q];
+      uF[2 * f + q] = ub + dd * (gb - gpF[2 * f + q]);
+    }
+    df[f] = dd;
+    rdf[f] = rho * dd;
+  }
+}
+
+// row `pin` of a scalar equation becomes x_pin = 0
+__global__ void kPinRow(int pin, int stride, int *cols, double *vals, int *cnt, double *rhs) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    cols[(size_t)pin * stride] = pin;
+    vals[(size_t)pin * stride] = 1.;
+    for (int j = 1; j < stride; ++j) {
+      cols[(size_t)pin * stride + j] = -1;
+      vals[(size_t)pin * stride + j] = 0.;
+    }
+    cnt[pin] = 1;
+    rhs[pin] = 0.;
+  }
+}
+
+// u_P -= d_P grad p'_P ; p += alpha_p p'
+__global__ void kCorrectCells(int n, const double *__restrict__ d, const double *__restrict__ gppC,
+                              const double *__restrict__ pp, double alphaP, double *uC, double *pC) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+    uC[2 * c] -= d[c] * gppC[2 * c];
+    uC[2 * c + 1] -= d[c] * gppC[2 * c + 1];
+    pC[c] += alphaP * pp[c];
+  }
+}
+
+// interior faces: u_f -= avg(d) grad_f p'
+__global__ void kCorrectFaces(int nFaces, const int *__restrict__ fr, const double *__restrict__ df,
+                              const double *__restrict__ gppF, double *uF) {
+  for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < nFaces; f += gridDim.x * blockDim.x) {
+    if (fr[f] < 0) continue;
+    uF[2 * f] -= df[f] * gppF[2 * f];
+    uF[2 * f + 1] -= df[f] * gppF[2 * f + 1];
+  }
+}
+
+// out[0] = max |a_i| (one block, fixed order)
+__global__ void kMaxAbs(int n, const double *__restrict__ a, double *out) {
+  __shared__ double s[256];
+  double m = 0.;
+  for (int i = threadIdx.x; i < n; i += 256) m = fmax(m, fabs(a[i]));
+  s[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = s[0];
+}
+
+void ck(int rc, const char *what, const char *detail) {
+  if (rc != PHASE_HIP_OK) fail(rc, std::string(what) + ": " + (detail ? detail : ""));
+}
+
+template <class F>
+int guardedS(phase_hip_simple *s, F &&f) {
+  if (!s) return PHASE_HIP_EINVAL;
+  try {
+    f();
+    return PHASE_HIP_OK;
+  } catch (const HipError &x) {
+    s->err = x.msg;
+    return x.code;
+  } catch (const std::exception &x) {
+    s->err = x.what();
+    return PHASE_HIP_EINVAL;
+  }
+}
+
+double maxAbs(phase_hip_simple *s, const double *a, int n) {
+  hipLaunchKernelGGL(kMaxAbs, dim3(1), dim3(256), 0, s->mesh->stream, n, a, s->maxBuf.p);
+  double v = 0.;
+  PH_CHECK(hipMemcpyAsync(&v, s->maxBuf.p, sizeof(double), hipMemcpyDeviceToHost, s->mesh->stream));
+  PH_CHECK(hipStreamSynchronize(s->mesh->stream));
+  return v;
+}
+
+void outerIteration(phase_hip_simple *s, double *hist) {
+  phase_hip_mesh *m = s->mesh;
+  hipStream_t st = m->stream;
+  const int N = s->N, F = s->F;
+  // pressure faces (zero gradient) and gradient
+  ck(phase_hip_field_interpolate_faces(m, 1, s->pC.p, s->pBc.p, s->pF.p), "p faces", m->err.c_str());
+  ck(phase_hip_field_grad(m, s->pC.p, s->pF.p, s->gpF.p, s->gpC.p), "grad p", m->err.c_str());
+  ck(phase_hip_field_src_src(m, 2, s->gpC.p, s->gsrc.p), "src(grad p)", m->err.c_str());
+  // uEqn = rho div(u_f, u) == laplacian(mu, u) - src(grad p)
+  ck(phase_hip_eq_div(s->uEq, s->uF.p, s->uF.p, 1.0, s->uBc.p, s->uWall.p, s->uC.p, s->uWall.p), "fv::div",
+     s->uEq->err.c_str());
+  if (s->rho != 1.) ck(phase_hip_eq_scale(s->uEq, s->rho), "scale", s->uEq->err.c_str());
+  ck(phase_hip_eq_laplacian(s->tmp, 1, s->mu, nullptr, nullptr, 1.0, s->uBc.p, s->uWall.p, nullptr, nullptr),
+     "fv::laplacian", s->tmp->err.c_str());
+  ck(phase_hip_eq_sub_vector(s->tmp, s->gsrc.p), "operator-", s->tmp->err.c_str());
+  ck(phase_hip_eq_add(s->uEq, s->tmp, -1), "operator==", s->uEq->err.c_str());
+  hipLaunchKernelGGL(kToBlocks, dim3(g256(N)), dim3(256), 0, st, N, s->uC.p, s->uB.p);
+  ck(phase_hip_eq_relax(s->uEq, s->alphaU, s->uB.p), "relax", s->uEq->err.c_str());
+  hipLaunchKernelGGL(kMomentumD, dim3(g256(N)), dim3(256), 0, st, N, s->uEq->stride, s->uEq->cols.p,
+                     s->uEq->vals.p, s->uEq->cnt.p, m->dVol.p, s->d.p);
+  // momentum solve from the current velocity
+  ck(phase_hip_solver_set_equation(s->su, s->uEq), "uEqn set", s->su->err.c_str());
+  ck(phase_hip_solver_set_guess_device(s->su, s->uB.p), "uEqn guess", s->su->err.c_str());
+  // like the reference's solve(), an unconverged inner solve returns its best
+  // iterate and the outer loop goes on
+  double uErr = 0.;
+  int rc = phase_hip_solver_solve(s->su, &uErr);
+  if (rc != PHASE_HIP_ENOCONV) ck(rc, "uEqn solve", s->su->err.c_str());
+  ck(phase_hip_solver_get_x_device(s->su, s->uB.p), "uEqn x", s->su->err.c_str());
+  hipLaunchKernelGGL(kFromBlocks, dim3(g256(N)), dim3(256), 0, st, N, s->uB.p, s->uC.p);
+  // Rhie-Chow face velocities and mass imbalance
+  hipLaunchKernelGGL(kRhieChow, dim3(g256(F)), dim3(256), 0, st, F, m->dFL.p, m->dFR.p, m->dCC.p, m->dFC.p,
+                     s->uC.p, s->d.p, s->gpC.p, s->gpF.p, s->uWall.p, s->rho, s->uF.p, s->df.p, s->rdf.p);
+  ck(phase_hip_field_src_div(m, s->uF.p, s->m.p), "src::div", m->err.c_str());
+  const double mMax = maxAbs(s, s->m.p, N);
+  // pCorrEqn = laplacian(rho d, p') == m
+  ck(phase_hip_eq_laplacian(s->pEq, 3, 0., s->rdf.p, nullptr, 1.0, s->pBc.p, s->zeroF.p, nullptr, nullptr),
+     "fv::laplacian", s->pEq->err.c_str());
+  ck(phase_hip_eq_sub_vector(s->pEq, s->m.p), "operator==", s->pEq->err.c_str());
+  hipLaunchKernelGGL(kPinRow, dim3(1), dim3(64), 0, st, s->pin, s->pEq->stride, s->pEq->cols.p, s->pEq->vals.p,
+                     s->pEq->cnt.p, s->pEq->rhs.p);
+  ck(phase_hip_solver_set_equation(s->sp, s->pEq), "pCorrEqn set", s->sp->err.c_str());
+  ck(phase_hip_solver_zero_guess(s->sp), "pCorrEqn guess", s->sp->err.c_str());
+  double pErr = 0.;
+  rc = phase_hip_solver_solve(s->sp, &pErr);
+  if (rc != PHASE_HIP_ENOCONV) ck(rc, "pCorrEqn solve", s->sp->err.c_str());
+  ck(phase_hip_solver_get_x_device(s->sp, s->pp.p), "pCorrEqn x", s->sp->err.c_str());
+  // corrections
+  ck(phase_hip_field_interpolate_faces(m, 1, s->pp.p, s->pBc.p, s->ppF.p), "p' faces", m->err.c_str());
+  ck(phase_hip_field_grad(m, s->pp.p, s->ppF.p, s->gppF.p, s->gppC.p), "grad p'", m->err.c_str());
+  hipLaunchKernelGGL(kCorrectCells, dim3(g256(N)), dim3(256), 0, st, N, s->d.p, s->gppC.p, s->pp.p, s->alphaP,
+                     s->uC.p, s->pC.p);
+  hipLaunchKernelGGL(kCorrectFaces, dim3(g256(F)), dim3(256), 0, st, F, m->dFR.p, s->df.p, s->gppF.p, s->uF.p);
+  PH_CHECK(hipGetLastError());
+  const double ppMax = maxAbs(s, s->pp.p, N);
+  if (hist) {
+    hist[0] = phase_hip_solver_n_iters(s->su);
+    hist[1] = phase_hip_solver_n_iters(s->sp);
+    hist[2] = mMax;
+    hist[3] = ppMax;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *phase_hip_simple_last_error(const phase_hip_simple *s) { return s ? s->err.c_str() : "null driver"; }
+
+int phase_hip_simple_destroy(phase_hip_simple *s) {
+  if (!s) return PHASE_HIP_OK;
+  phase_hip_equation_destroy(s->uEq);
+  phase_hip_equation_destroy(s->tmp);
+  phase_hip_equation_destroy(s->pEq);
+  phase_hip_solver_destroy(s->su);
+  phase_hip_solver_destroy(s->sp);
+  delete s;
+  return PHASE_HIP_OK;
+}
+
+int phase_hip_simple_create(phase_hip_simple **out, phase_hip_mesh *m, double rho, double mu, double alphaU,
+                            double alphaP, const int *uBcType, const double *uBoundary, int pinCell) {
+  if (!out || !m || !uBcType || !uBoundary) return PHASE_HIP_EINVAL;
+  *out = nullptr;
+  phase_hip_simple *s = new phase_hip_simple();
+  s->mesh = m;
+  s->N = m->nCells;
+  s->F = m->nFaces;
+  s->rho = rho;
+  s->mu = mu;
+  s->alphaU = alphaU;
+  s->alphaP = alphaP;
+  s->pin = pinCell;
+  const int rc = guardedS(s, [&] {
+    if (!(rho > 0.) || !(mu > 0.) || !(alphaU > 0. && alphaU <= 1.) || !(alphaP > 0. && alphaP <= 1.))
+      fail(PHASE_HIP_EINVAL, "SIMPLE needs rho, mu > 0 and relaxation factors in (0, 1]");
+    if (pinCell < 0 || pinCell >= m->nCells) fail(PHASE_HIP_EINVAL, "pin cell out of range");
+    for (int f = 0; f < m->nFaces; ++f)
+      if (m->fR[f] < 0 && uBcType[f] != PHASE_HIP_BC_FIXED)
+        fail(PHASE_HIP_EBOUNDARY, "this SIMPLE driver takes FIXED velocity on every boundary face");
+    PH_CHECK(hipSetDevice(m->device));
+    hipStream_t st = m->stream;
+    const int N = s->N, F = s->F;
+    ck(phase_hip_equation_create(&s->uEq, m, 2), "uEqn", "");
+    ck(phase_hip_equation_create(&s->tmp, m, 2), "uEqn", "");
+    ck(phase_hip_equation_create(&s->pEq, m, 1), "pCorrEqn", "");
+    ck(phase_hip_solver_create(&s->su, m->device), "solver", "");
+    ck(phase_hip_solver_create(&s->sp, m->device), "solver", "");
+    for (phase_hip_solver *v : {s->su, s->sp}) {
+      ck(phase_hip_solver_setup(v, "tolerance", "1e-8"), "setup", v->err.c_str());
+      ck(phase_hip_solver_setup(v, "maxIters", "5000"), "setup", v->err.c_str());
+    }
+    s->uC.alloc(2 * (size_t)N);
+    s->uB.alloc(2 * (size_t)N);
+    s->gpC.alloc(2 * (size_t)N);
+    s->gsrc.alloc(2 * (size_t)N);
+    s->gppC.alloc(2 * (size_t)N);
+    s->pC.alloc(N);
+    s->d.alloc(N);
+    s->m.alloc(N);
+    s->pp.alloc(N);
+    s->uF.alloc(2 * (size_t)F);
+    s->uWall.alloc(2 * (size_t)F);
+    s->gpF.alloc(2 * (size_t)F);
+    s->gppF.alloc(2 * (size_t)F);
+    s->pF.alloc(F);
+    s->ppF.alloc(F);
+    s->df.alloc(F);
+    s->rdf.alloc(F);
+    s->zeroF.alloc(F);
+    s->maxBuf.alloc(1);
+    for (DBuf<double> *b : {&s->uC, &s->pC, &s->pF, &s->zeroF, &s->pp}) b->zero(st);
+    std::vector<int> ubc(uBcType, uBcType + F), pbc(F, PHASE_HIP_BC_NORMAL_GRADIENT);
+    s->uBc.upload(ubc, st);
+    s->pBc.upload(pbc, st);
+    std::vector<double> wall(uBoundary, uBoundary + 2 * (size_t)F);
+    for (int f = 0; f < F; ++f)
+      if (m->fR[f] >= 0) wall[2 * f] = wall[2 * f + 1] = 0.;
+    PH_CHECK(hipMemcpyAsync(s->uWall.p, wall.data(), wall.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    // initial face velocities: the boundary values, zero inside (u = 0 at rest)
+    PH_CHECK(hipMemcpyAsync(s->uF.p, s->uWall.p, 2 * (size_t)F * sizeof(double), hipMemcpyDeviceToDevice, st));
+    PH_CHECK(hipStreamSynchronize(st));
+  });
+  if (rc != PHASE_HIP_OK) {
+    phase_hip_simple_destroy(s);
+    return rc;
+  }
+  *out = s;
+  return PHASE_HIP_OK;
+}
+
+int phase_hip_simple_setup(phase_hip_simple *s, const char *key, const char *value) {
+  return guardedS(s, [&] {
+    if (!key || !value) fail(PHASE_HIP_EINVAL, "null key/value");
+    std::string k(key);
+    phase_hip_solver *v = nullptr;
+    if (k.rfind("u.", 0) == 0) v = s->su;
+    else if (k.rfind("p.", 0) == 0) v = s->sp;
+    else fail(PHASE_HIP_EINVAL, "keys are u.<solver key> or p.<solver key>");
+    ck(phase_hip_solver_setup(v, k.c_str() + 2, value), "setup", v->err.c_str());
+  });
+}
+
+int phase_hip_simple_iterate(phase_hip_simple *s, int nIters, double *history) {
+  return guardedS(s, [&] {
+    if (nIters < 0) fail(PHASE_HIP_EINVAL, "negative iteration count");
+    PH_CHECK(hipSetDevice(s->mesh->device));
+    for (int it = 0; it < nIters; ++it) outerIteration(s, history ? history + 4 * (size_t)it : nullptr);
+  });
+}
+
+int phase_hip_simple_fields(phase_hip_simple *s, double *uCell, double *pCell, double *uFace) {
+  return guardedS(s, [&] {
+    hipStream_t st = s->mesh->stream;
+    if (uCell) PH_CHECK(hipMemcpyAsync(uCell, s->uC.p, 2 * (size_t)s->N * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (pCell) PH_CHECK(hipMemcpyAsync(pCell, s->pC.p, (size_t)s->N * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (uFace) PH_CHECK(hipMemcpyAsync(uFace, s->uF.p, 2 * (size_t)s->F * sizeof(double), hipMemcpyDeviceToHost, st));
+    PH_CHECK(hipStreamSynchronize(st));
+  });
+}
+
+}  // extern "C"

@@ -98,6 +98,13 @@ _SIGS = {
     "phase_hip_field_grad": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "phase_hip_field_interpolate_faces": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp]),
     "phase_hip_mesh_synchronize": (ctypes.c_int, [_vp]),
+    "phase_hip_simple_create": (ctypes.c_int, [ctypes.POINTER(_vp), _vp, ctypes.c_double, ctypes.c_double,
+                                               ctypes.c_double, ctypes.c_double, _i32p, _f64p, ctypes.c_int]),
+    "phase_hip_simple_destroy": (ctypes.c_int, [_vp]),
+    "phase_hip_simple_last_error": (ctypes.c_char_p, [_vp]),
+    "phase_hip_simple_setup": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_char_p]),
+    "phase_hip_simple_iterate": (ctypes.c_int, [_vp, ctypes.c_int, _vp]),
+    "phase_hip_simple_fields": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
 }
 
 _lib = None
@@ -350,6 +357,52 @@ class HipEquation:
         ci, v, rhs = np.empty(n.value, np.int32), np.empty(n.value), np.empty(self.rank)
         self._check(load_library().phase_hip_eq_export(self._h, n, _p(rp), _p(ci), _p(v), _p(rhs)), "export")
         return rp, ci, v, rhs
+
+
+class HipSimple:
+    """SIMPLE driver on the device (steady incompressible flow): momentum and
+    pressure-correction equations of the README, Rhie-Chow face velocities."""
+
+    def __init__(self, mesh, rho, mu, alpha_u, alpha_p, u_bc, u_boundary, pin_cell=0):
+        h = _vp()
+        rc = load_library().phase_hip_simple_create(ctypes.byref(h), mesh._h, float(rho), float(mu), float(alpha_u),
+                                                    float(alpha_p), _i32(u_bc), _f64(np.asarray(u_boundary).reshape(-1)),
+                                                    int(pin_cell))
+        if rc != 0:
+            raise PhaseHipError(rc, "simpleCreate", load_library().phase_hip_last_error(None).decode())
+        self._h = h
+        self.mesh = mesh
+
+    def _check(self, rc, method):
+        if rc != 0:
+            raise PhaseHipError(rc, method, load_library().phase_hip_simple_last_error(self._h).decode())
+
+    def setup(self, key, value):
+        self._check(load_library().phase_hip_simple_setup(self._h, key.encode(), str(value).encode()), "setup")
+
+    def iterate(self, n):
+        """n outer iterations; returns an (n, 4) history: momentum and
+        pressure-correction iterations, max |mass imbalance|, max |p'|."""
+        hist = np.zeros((n, 4))
+        self._check(load_library().phase_hip_simple_iterate(self._h, int(n), _p(hist)), "iterate")
+        return hist
+
+    def fields(self):
+        N, F = self.mesh.n_cells, self.mesh.n_faces
+        u, p, uf = np.empty(2 * N), np.empty(N), np.empty(2 * F)
+        self._check(load_library().phase_hip_simple_fields(self._h, _p(u), _p(p), _p(uf)), "fields")
+        return u.reshape(N, 2), p, uf.reshape(F, 2)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().phase_hip_simple_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class HipSparseMatrixSolver:
