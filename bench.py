@@ -104,6 +104,51 @@ def train_network(s, mesh, d_g, d_bc, d_pf, args):
                 train_s=round(t2 - t1, 2), loss_first=round(first, 4), loss_last=round(loss, 4))
 
 
+def simple_cavity(args):
+    """The C1 case on the device SIMPLE driver: lid-driven cavity on the
+    128x128 unit square, Re = 100 (rho 1, mu 0.01), alpha_u 0.7, alpha_p 0.3,
+    `args.simple_iters` outer iterations from rest (FP64 inner solves to 1e-8).
+    Reported beside the headline; the CPU figure is the oracle's restatement
+    of the same iterations (direct inner solves, 1 thread) on a 10-iteration
+    sample."""
+    from phase_amd import hip as P
+    from phase_amd import meshgen as G
+
+    xy, cptr, cind = G.rectilinear(128, 128)
+    mesh = P.HipMesh(xy, cptr, cind)
+    f = mesh.faces()
+    pid = G.boundary_patch(f["lnode"], f["rnode"], f["rcell"], xy)
+    ubc = np.full(mesh.n_faces, P.BC_FIXED, np.int32)
+    ub = np.zeros((mesh.n_faces, 2))
+    ub[pid == 3, 0] = 1.0
+    warm = P.HipSimple(mesh, 1.0, 0.01, 0.7, 0.3, ubc, ub, 0)
+    warm.iterate(3)
+    warm.close()
+    s = P.HipSimple(mesh, 1.0, 0.01, 0.7, 0.3, ubc, ub, 0)
+    t0 = time.perf_counter()
+    hist = s.iterate(args.simple_iters)
+    dt = time.perf_counter() - t0
+    out = dict(workload="lid-driven-cavity-128x128-re100", cells=mesh.n_cells, outer_iters=args.simple_iters,
+               ms_per_outer=round(1e3 * dt / args.simple_iters, 3), total_s=round(dt, 3),
+               mean_momentum_iters=round(float(hist[:, 0].mean()), 2),
+               mean_pcorr_iters=round(float(hist[:, 1].mean()), 2),
+               final_mass_imbalance=float("%.3g" % hist[-1, 2]), final_max_pcorr=float("%.3g" % hist[-1, 3]))
+    s.close()
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        from oracle.simple_oracle import SimpleOracle
+
+        O.build()
+        om = O.Mesh(xy, cptr, cind)
+        ref = SimpleOracle(om, 1.0, 0.01, 0.7, 0.3, ubc, ub, pin=0)
+        t0 = time.perf_counter()
+        ref.iterate(10)
+        out["cpu_port_ms_per_outer"] = round(1e3 * (time.perf_counter() - t0) / 10, 2)
+    log("simple C1: %.3f ms per outer iteration (%d iterations), momentum %.1f / pCorr %.1f inner iterations"
+        % (out["ms_per_outer"], args.simple_iters, out["mean_momentum_iters"], out["mean_pcorr_iters"]))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,6 +160,7 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--inner-tol", type=float, default=0.1)
     ap.add_argument("--sweep", default="colour")
+    ap.add_argument("--block-rows", type=int, default=32, help="rows per ILU aggregate (blocks pack aggregates to 64)")
     ap.add_argument("--modes", default="fp64,mixed,learned")
     ap.add_argument("--capture-solves", type=int, default=4)
     ap.add_argument("--capture-every", type=int, default=100)
@@ -122,6 +168,7 @@ def main():
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--simple-iters", type=int, default=200, help="C1 SIMPLE outer iterations (0 skips)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported (the path would shard by "
@@ -148,7 +195,7 @@ def main():
     nnz = None
     for mode in args.modes.split(","):
         s = P.HipSparseMatrixSolver(0, tolerance=args.tol, maxIters=50000, mode=mode,
-                                    innerTolerance=args.inner_tol, sweep=args.sweep)
+                                    innerTolerance=args.inner_tol, sweep=args.sweep, blockRows=args.block_rows)
 
         def step():
             s.assemble_laplacian_device(mesh, 3, gamma_face=d_g, bc_type=d_bc, phi_face=d_pf)
@@ -227,6 +274,8 @@ def main():
         "ordering": ordinfo,
         "roofline": roof,
     }
+    if args.simple_iters > 0:
+        out["simple_c1"] = simple_cavity(args)
     if not args.no_cpu_baseline:
         it = results.get("fp64", results[head])["iters"]
         out["cpu_baseline"] = cpu_baseline(xy, cptr, cind, bc, g, b, it)

@@ -91,7 +91,8 @@ int phase_hip_solver_destroy(phase_hip_solver *s);
  * "innerTolerance" (mixed: residual reduction between FP64 refinement points, 0.1),
  * "sweep" (ILU sweeps: "colour" = one launch per block colour, "syncfree" =
  * one launch with device-scope completion flags),
- * "blockRows" (rows per ILU block, <= 64), "fallbackRatio" (learned: minimum
+ * "blockRows" (rows per ILU aggregate, <= 64, default 32; aggregates of one
+ * colour are packed into blocks of up to 64 rows), "fallbackRatio" (learned: minimum
  * per-iteration residual reduction before falling back, 0.5).  Unknown keys
  * are ignored, as the reference ignores ptree keys it does not read. */
 int phase_hip_solver_setup(phase_hip_solver *s, const char *key, const char *value);

@@ -366,17 +366,33 @@ void buildOrdering(Ordering &o, int n, const int *rowPtr, const int *colInd, int
   }
   o.nColours = nColours;
 
-  // 5. solver order
+  // 5. ILU blocks: aggregates of one colour share no coupling, so consecutive
+  //    aggregates of a colour are packed into one block of <= 64 rows (one
+  //    wavefront).  The block's in-block dependency graph then has one
+  //    component per aggregate and its level count is the largest of theirs:
+  //    smaller aggregates cut the levels a wavefront walks, at the cost of
+  //    more coupling between blocks.  Solver order = blocks by colour.
   std::vector<int> border(nb);
   std::iota(border.begin(), border.end(), 0);
   std::stable_sort(border.begin(), border.end(), [&](int a, int c) { return colour[a] < colour[c]; });
+  std::vector<std::vector<int>> packs;
+  std::vector<int> packColour;
+  for (int ci = 0; ci < nb; ++ci) {
+    const int a = border[ci];
+    if (packs.empty() || packColour.back() != colour[a] || packs.back().size() + members[a].size() > (size_t)kLanes) {
+      packs.emplace_back();
+      packColour.push_back(colour[a]);
+    }
+    packs.back().insert(packs.back().end(), members[a].begin(), members[a].end());
+  }
+  nb = (int)packs.size();
   o.perm.clear();
   o.perm.reserve(n);
   o.nBlocks = nb;
   o.blocks.assign(nb, BlockInfo{});
   std::vector<int> blockOfRow(n);
   for (int bi = 0; bi < nb; ++bi) {
-    auto &mem = members[border[bi]];
+    auto &mem = packs[bi];
     o.blocks[bi].rowStart = (int)o.perm.size();
     o.blocks[bi].nRows = (int)mem.size();
     for (int r : mem) {
@@ -387,7 +403,7 @@ void buildOrdering(Ordering &o, int n, const int *rowPtr, const int *colInd, int
   o.iperm.assign(n, 0);
   for (int k = 0; k < n; ++k) o.iperm[o.perm[k]] = k;
   o.colourPtr.assign(nColours + 1, 0);
-  for (int b = 0; b < nb; ++b) ++o.colourPtr[colour[border[b]] + 1];
+  for (int b = 0; b < nb; ++b) ++o.colourPtr[packColour[b] + 1];
   for (int c = 0; c < nColours; ++c) o.colourPtr[c + 1] += o.colourPtr[c];
 
   // 6. SELL-64 layout of A over solver rows, original in-row order

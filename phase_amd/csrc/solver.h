@@ -34,7 +34,7 @@ struct phase_hip_solver {
   int maxIters = 500;
   int mode = PHASE_HIP_MODE_FP64;
   double innerTol = 0.1;
-  int blockRows = 64;
+  int blockRows = 32;  // rows per ILU aggregate; blocks pack same-colour aggregates up to 64 rows
   double fallbackRatio = 0.5;
   int maxOuter = 100;
   int chunk = 8;  // iterations enqueued between host checks
