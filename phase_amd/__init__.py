@@ -12,15 +12,22 @@ interfaces:
 * HipSparseMatrixSolver.assemble_laplacian -- fv::laplacian
   (src/2D/Unstructured/FiniteVolume/Discretization/Laplacian.h:7-167) assembled
   on the device into the solver's system.
+* HipEquation -- FiniteVolumeEquation<T> / CrsEquation on the device: the fv::
+  operators (laplacian, ddt, div), += / -= / == / *= / relax, export in the
+  reference's layout, and HipSparseMatrixSolver.set_equation for the solve.
+* HipMesh.src_div / src_src / grad / interpolate_faces -- src::div, src::src,
+  ScalarGradient, interpolateFaces on device fields.
+* HipSimple -- a SIMPLE outer loop over the above (the README's uEqn/pCorrEqn).
 
 There is no CPU fallback: loading fails loudly when the library is absent.
 """
 from .hip import (BC_FIXED, BC_NORMAL_GRADIENT, BC_OUTFLOW, BC_PARTIAL_SLIP, BC_SYMMETRY,
-                  MODE_FP64, MODE_LEARNED, MODE_MIXED, DeviceArray, HipMesh, HipSparseMatrixSolver,
-                  PhaseHipError, device_count, library_path, load_library)
+                  MODE_FP64, MODE_LEARNED, MODE_MIXED, DeviceArray, HipEquation, HipMesh, HipSimple,
+                  HipSparseMatrixSolver, PhaseHipError, device_count, library_path, load_library)
 
 __all__ = [
     "BC_FIXED", "BC_NORMAL_GRADIENT", "BC_OUTFLOW", "BC_PARTIAL_SLIP", "BC_SYMMETRY",
-    "MODE_FP64", "MODE_LEARNED", "MODE_MIXED", "DeviceArray", "HipMesh", "HipSparseMatrixSolver",
+    "MODE_FP64", "MODE_LEARNED", "MODE_MIXED", "DeviceArray", "HipEquation", "HipMesh", "HipSimple",
+    "HipSparseMatrixSolver",
     "PhaseHipError", "device_count", "library_path", "load_library",
 ]
