@@ -326,6 +326,28 @@ int phase_hip_field_interpolate_faces(phase_hip_mesh *m, int ncomp, const double
 /* Wait for the work queued on the mesh's stream. */
 int phase_hip_mesh_synchronize(phase_hip_mesh *m);
 
+/* ------------------------------------------------------------------ fractional step
+ * FractionalStep::solve(timeStep) -- src/2D/Unstructured/Solvers/FractionalStep.cpp:36-117,
+ * one time step on the device (all fields in HBM):
+ *   uEqn = ddt(u, dt) + div(u, u, 0) == laplacian(mu/rho, u, 0.5) - src(grad p)
+ *   u += dt grad p; interpolateFaces(u)
+ *   pEqn = laplacian(dt, p) == src::div(u); setBoundaryFaces(p); grad p
+ *   u -= dt grad p (cells), u_f -= dt grad_f p (faces)
+ * uBcType / pBcType per face (pBcType NULL: zero gradient everywhere);
+ * uFace0 / pFace0: initial face values (boundary values of FIXED faces);
+ * pinCell >= 0 fixes p at that cell (needed when no pressure face is FIXED),
+ * -1 for none.  history (optional, 3 per step): momentum and pressure
+ * iterations, FractionalStep::maxDivergenceError of the corrected faces. */
+typedef struct phase_hip_fstep phase_hip_fstep;
+
+int phase_hip_fstep_create(phase_hip_fstep **s, phase_hip_mesh *m, double rho, double mu, const int *uBcType,
+                           const double *uFace0, const int *pBcType, const double *pFace0, int pinCell);
+int phase_hip_fstep_destroy(phase_hip_fstep *s);
+const char *phase_hip_fstep_last_error(const phase_hip_fstep *s);
+int phase_hip_fstep_setup(phase_hip_fstep *s, const char *key, const char *value);
+int phase_hip_fstep_advance(phase_hip_fstep *s, double dt, int nSteps, double *history);
+int phase_hip_fstep_fields(phase_hip_fstep *s, double *uCell, double *pCell, double *uFace);
+
 /* ------------------------------------------------------------------ SIMPLE driver
  * No counterpart module in the reference snapshot (its solvers are projection
  * methods, src/2D/Unstructured/Solvers/FractionalStep.cpp:79-117); built from

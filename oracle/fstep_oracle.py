@@ -1,0 +1,78 @@
+"""TEST INFRASTRUCTURE ONLY.  FractionalStep::solve restated on the CPU with the
+FP64 oracle's operators and direct solves, the checker for the device
+fractional-step driver (phase_amd/csrc/simple.hip).  Only tests/ may use it.
+
+One time step (src/2D/Unstructured/Solvers/FractionalStep.cpp:36-117):
+  u0 = u                                               (savePreviousTimeStep)
+  uEqn = ddt(u, dt) + div(u, u, 0) == laplacian(mu/rho, u, 0.5) - src(grad p)
+  u = solve(uEqn); u += dt grad p; u_f = interpolateFaces(u)
+  pEqn = laplacian(dt, p) == src::div(u_f); p = solve(pEqn)
+  p_f = setBoundaryFaces(p); grad p (ScalarGradient)
+  u -= dt grad p (cells); u_f -= dt grad_f p (every face)
+With no FIXED pressure face the pressure matrix is singular and one cell is
+pinned to 0, as on the device.
+"""
+import numpy as np
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from oracle import oracle as O
+
+
+def _csr(rp, ci, v):
+    n = len(rp) - 1
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    keep = ci >= 0
+    return sp.csr_matrix((v[keep], (rows[keep], ci[keep])), shape=(n, n))
+
+
+class FractionalStepOracle:
+    def __init__(self, mesh, rho, mu, u_bc, u_face0, p_bc=None, p_face0=None, pin=0):
+        self.m, self.rho, self.mu, self.pin = mesh, rho, mu, pin
+        N, F = mesh.n_cells, mesh.n_faces
+        self.ubc = np.asarray(u_bc, np.int32)
+        self.pbc = (np.full(F, O.BC_NORMAL_GRADIENT, np.int32) if p_bc is None else np.asarray(p_bc, np.int32))
+        self.uC = np.zeros((N, 2))
+        self.pC = np.zeros(N)
+        self.gpC = np.zeros((N, 2))
+        self.gpF = np.zeros((F, 2))
+        self.pF = np.zeros(F) if p_face0 is None else np.asarray(p_face0, float).copy()
+        uf = np.asarray(u_face0, float).reshape(F, 2)
+        self.uF = O.interpolate_faces(mesh, self.uC.reshape(-1), uf.reshape(-1), bc_type=self.ubc,
+                                      ncomp=2).reshape(F, 2)
+
+    def _solve(self, eqn, pin=-1):
+        rp, ci, v, rhs = eqn.get()
+        A = _csr(rp, ci, v)
+        b = -rhs
+        if pin >= 0:
+            A = A.tolil()
+            A[pin, :] = 0.0
+            A[pin, pin] = 1.0
+            b = b.copy()
+            b[pin] = 0.0
+        return spla.spsolve(A.tocsc(), b)
+
+    def step(self, dt):
+        m, N, F = self.m, self.m.n_cells, self.m.n_faces
+        u0C, u0F = self.uC.copy(), self.uF.copy()
+        E = O.ddt(m, dt, u0C.reshape(-1), ncomp=2)
+        E += O.div(m, self.uF.reshape(-1), u0F.reshape(-1), theta=0.0, bc_type=self.ubc,
+                   phi_face=self.uF.reshape(-1), phi0_cell=u0C.reshape(-1), phi0_face=u0F.reshape(-1), ncomp=2)
+        L = O.laplacian(m, 0, ncomp=2, gamma=self.mu / self.rho, theta=0.5, bc_type=self.ubc,
+                        phi_face=self.uF.reshape(-1), phi0_cell=u0C.reshape(-1), phi0_face=u0F.reshape(-1))
+        L -= O.src_src(m, self.gpC.reshape(-1), ncomp=2)
+        E.eq(L)
+        self.uC = self._solve(E).reshape(2, N).T.copy()
+        self.uC = self.uC + dt * self.gpC
+        self.uF = O.interpolate_faces(m, self.uC.reshape(-1), self.uF.reshape(-1), bc_type=self.ubc,
+                                      ncomp=2).reshape(F, 2)
+        P = O.laplacian(m, 1, gamma=dt, bc_type=self.pbc, phi_face=self.pF)
+        P -= O.src_div(m, self.uF.reshape(-1))
+        self.pC = self._solve(P, self.pin)
+        self.pF = O.interpolate_faces(m, self.pC, self.pF, bc_type=self.pbc)
+        gF, gC = O.grad(m, self.pC, self.pF)
+        self.gpF, self.gpC = gF, gC
+        self.uC = self.uC + (-dt) * self.gpC
+        self.uF = self.uF + (-dt) * self.gpF
+        return np.abs(O.src_div(m, self.uF.reshape(-1))).max()

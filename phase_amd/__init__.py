@@ -17,17 +17,19 @@ interfaces:
   reference's layout, and HipSparseMatrixSolver.set_equation for the solve.
 * HipMesh.src_div / src_src / grad / interpolate_faces -- src::div, src::src,
   ScalarGradient, interpolateFaces on device fields.
+* HipFractionalStep -- FractionalStep::solve, the reference's projection solver
+  (src/2D/Unstructured/Solvers/FractionalStep.cpp:36-117), over the above.
 * HipSimple -- a SIMPLE outer loop over the above (the README's uEqn/pCorrEqn).
 
 There is no CPU fallback: loading fails loudly when the library is absent.
 """
 from .hip import (BC_FIXED, BC_NORMAL_GRADIENT, BC_OUTFLOW, BC_PARTIAL_SLIP, BC_SYMMETRY,
-                  MODE_FP64, MODE_LEARNED, MODE_MIXED, DeviceArray, HipEquation, HipMesh, HipSimple,
-                  HipSparseMatrixSolver, PhaseHipError, device_count, library_path, load_library)
+                  MODE_FP64, MODE_LEARNED, MODE_MIXED, DeviceArray, HipEquation, HipFractionalStep, HipMesh,
+                  HipSimple, HipSparseMatrixSolver, PhaseHipError, device_count, library_path, load_library)
 
 __all__ = [
     "BC_FIXED", "BC_NORMAL_GRADIENT", "BC_OUTFLOW", "BC_PARTIAL_SLIP", "BC_SYMMETRY",
-    "MODE_FP64", "MODE_LEARNED", "MODE_MIXED", "DeviceArray", "HipEquation", "HipMesh", "HipSimple",
+    "MODE_FP64", "MODE_LEARNED", "MODE_MIXED", "DeviceArray", "HipEquation", "HipFractionalStep", "HipMesh", "HipSimple",
     "HipSparseMatrixSolver",
     "PhaseHipError", "device_count", "library_path", "load_library",
 ]

@@ -1,5 +1,6 @@
-// SIMPLE driver (Patankar & Spalding) for steady incompressible flow on the
-// device: the momentum and pressure-correction equations of the reference's
+// Outer-loop drivers on the device: the reference's fractional-step
+// projection solver (FractionalStep, below) and a SIMPLE driver (Patankar &
+// Spalding) for steady incompressible flow: the momentum and pressure-correction equations of the reference's
 // README (README.md:27-37), assembled with the device fv:: operators and solved
 // through the device solver plugin, with collocated Rhie-Chow face velocities.
 // The reference snapshot ships no SIMPLE module (its solvers are projection
@@ -243,9 +244,210 @@ void outerIteration(phase_hip_simple *s, double *hist) {
   }
 }
 
+// u += s * g (both interleaved vectors of n entries)
+__global__ void kAxpy(long n, double sc, const double *__restrict__ g, double *u) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) u[i] += sc * g[i];
+}
+
+}  // namespace
+
+// FractionalStep::solve (src/2D/Unstructured/Solvers/FractionalStep.cpp:36-117)
+// on the device, one time step:
+//   u0 = u (cells and faces)                                   savePreviousTimeStep
+//   uEqn = ddt(u, dt) + div(u, u, 0) == laplacian(mu/rho, u, 0.5) - src(grad p)
+//   u = solve(uEqn);  u += dt grad p;  u_f = interpolateFaces(u)
+//   pEqn = laplacian(dt, p) == src::div(u_f);  p = solve(pEqn)  (pin cell fixed at 0)
+//   p_f = setBoundaryFaces(p);  grad p (faces, FACE_TO_CELL)
+//   u -= dt grad p (cells);  u_f -= dt grad_f p (every face)
+struct phase_hip_fstep {
+  phase_hip_mesh *mesh = nullptr;
+  int N = 0, F = 0, pin = 0;
+  double rho = 1., mu = 1.;
+  phase_hip_equation *uEq = nullptr, *tmp = nullptr, *pEq = nullptr;
+  phase_hip_solver *su = nullptr, *sp = nullptr;
+  DBuf<double> uC, u0C, uB, uF, u0F, pC, pB, pF, gpF, gpC, gsrc, m, maxBuf;
+  DBuf<int> uBc, pBc;
+  std::string err;
+};
+
+namespace {
+
+template <class F>
+int guardedF(phase_hip_fstep *s, F &&f) {
+  if (!s) return PHASE_HIP_EINVAL;
+  try {
+    f();
+    return PHASE_HIP_OK;
+  } catch (const HipError &x) {
+    s->err = x.msg;
+    return x.code;
+  } catch (const std::exception &x) {
+    s->err = x.what();
+    return PHASE_HIP_EINVAL;
+  }
+}
+
+double maxAbsF(phase_hip_fstep *s, const double *a, int n) {
+  hipLaunchKernelGGL(kMaxAbs, dim3(1), dim3(256), 0, s->mesh->stream, n, a, s->maxBuf.p);
+  double v = 0.;
+  PH_CHECK(hipMemcpyAsync(&v, s->maxBuf.p, sizeof(double), hipMemcpyDeviceToHost, s->mesh->stream));
+  PH_CHECK(hipStreamSynchronize(s->mesh->stream));
+  return v;
+}
+
+void timeStep(phase_hip_fstep *s, double dt, double *hist) {
+  phase_hip_mesh *m = s->mesh;
+  hipStream_t st = m->stream;
+  const int N = s->N, F = s->F;
+  // savePreviousTimeStep
+  PH_CHECK(hipMemcpyAsync(s->u0C.p, s->uC.p, 2 * (size_t)N * sizeof(double), hipMemcpyDeviceToDevice, st));
+  PH_CHECK(hipMemcpyAsync(s->u0F.p, s->uF.p, 2 * (size_t)F * sizeof(double), hipMemcpyDeviceToDevice, st));
+  // solveUEqn (FractionalStep.cpp:79-92)
+  ck(phase_hip_eq_ddt(s->uEq, dt, s->u0C.p), "fv::ddt", s->uEq->err.c_str());
+  ck(phase_hip_eq_div(s->tmp, s->uF.p, s->u0F.p, 0.0, s->uBc.p, s->uF.p, s->u0C.p, s->u0F.p), "fv::div",
+     s->tmp->err.c_str());
+  ck(phase_hip_eq_add(s->uEq, s->tmp, 1), "operator+", s->uEq->err.c_str());
+  ck(phase_hip_eq_laplacian(s->tmp, 0, s->mu / s->rho, nullptr, nullptr, 0.5, s->uBc.p, s->uF.p, s->u0C.p, s->u0F.p),
+     "fv::laplacian", s->tmp->err.c_str());
+  ck(phase_hip_field_src_src(m, 2, s->gpC.p, s->gsrc.p), "src::src", m->err.c_str());
+  ck(phase_hip_eq_sub_vector(s->tmp, s->gsrc.p), "operator-", s->tmp->err.c_str());
+  ck(phase_hip_eq_add(s->uEq, s->tmp, -1), "operator==", s->uEq->err.c_str());
+  hipLaunchKernelGGL(kToBlocks, dim3(g256(N)), dim3(256), 0, st, N, s->uC.p, s->uB.p);
+  ck(phase_hip_solver_set_equation(s->su, s->uEq), "uEqn set", s->su->err.c_str());
+  ck(phase_hip_solver_set_guess_device(s->su, s->uB.p), "uEqn guess", s->su->err.c_str());
+  double err = 0.;
+  int rc = phase_hip_solver_solve(s->su, &err);
+  if (rc != PHASE_HIP_ENOCONV) ck(rc, "uEqn solve", s->su->err.c_str());
+  ck(phase_hip_solver_get_x_device(s->su, s->uB.p), "uEqn x", s->su->err.c_str());
+  hipLaunchKernelGGL(kFromBlocks, dim3(g256(N)), dim3(256), 0, st, N, s->uB.p, s->uC.p);
+  hipLaunchKernelGGL(kAxpy, dim3(g256(2L * N)), dim3(256), 0, st, 2L * N, dt, s->gpC.p, s->uC.p);
+  ck(phase_hip_field_interpolate_faces(m, 2, s->uC.p, s->uBc.p, s->uF.p), "interpolateFaces", m->err.c_str());
+  // solvePEqn (FractionalStep.cpp:94-106)
+  ck(phase_hip_eq_laplacian(s->pEq, 1, dt, nullptr, nullptr, 1.0, s->pBc.p, s->pF.p, nullptr, nullptr),
+     "fv::laplacian", s->pEq->err.c_str());
+  ck(phase_hip_field_src_div(m, s->uF.p, s->m.p), "src::div", m->err.c_str());
+  ck(phase_hip_eq_sub_vector(s->pEq, s->m.p), "operator==", s->pEq->err.c_str());
+  if (s->pin >= 0)  // all-Neumann pressure: one cell fixed at 0
+    hipLaunchKernelGGL(kPinRow, dim3(1), dim3(64), 0, st, s->pin, s->pEq->stride, s->pEq->cols.p, s->pEq->vals.p,
+                       s->pEq->cnt.p, s->pEq->rhs.p);
+  ck(phase_hip_solver_set_equation(s->sp, s->pEq), "pEqn set", s->sp->err.c_str());
+  ck(phase_hip_solver_set_guess_device(s->sp, s->pC.p), "pEqn guess", s->sp->err.c_str());
+  rc = phase_hip_solver_solve(s->sp, &err);
+  if (rc != PHASE_HIP_ENOCONV) ck(rc, "pEqn solve", s->sp->err.c_str());
+  ck(phase_hip_solver_get_x_device(s->sp, s->pC.p), "pEqn x", s->sp->err.c_str());
+  ck(phase_hip_field_interpolate_faces(m, 1, s->pC.p, s->pBc.p, s->pF.p), "setBoundaryFaces", m->err.c_str());
+  ck(phase_hip_field_grad(m, s->pC.p, s->pF.p, s->gpF.p, s->gpC.p), "grad p", m->err.c_str());
+  // correctVelocity (FractionalStep.cpp:108-117)
+  hipLaunchKernelGGL(kAxpy, dim3(g256(2L * N)), dim3(256), 0, st, 2L * N, -dt, s->gpC.p, s->uC.p);
+  hipLaunchKernelGGL(kAxpy, dim3(g256(2L * F)), dim3(256), 0, st, 2L * F, -dt, s->gpF.p, s->uF.p);
+  PH_CHECK(hipGetLastError());
+  if (hist) {
+    // FractionalStep::maxDivergenceError (FractionalStep.cpp:119-135) on the corrected faces
+    ck(phase_hip_field_src_div(m, s->uF.p, s->m.p), "src::div", m->err.c_str());
+    hist[0] = phase_hip_solver_n_iters(s->su);
+    hist[1] = phase_hip_solver_n_iters(s->sp);
+    hist[2] = maxAbsF(s, s->m.p, N);
+  }
+}
+
 }  // namespace
 
 extern "C" {
+
+int phase_hip_fstep_destroy(phase_hip_fstep *s) {
+  if (!s) return PHASE_HIP_OK;
+  phase_hip_equation_destroy(s->uEq);
+  phase_hip_equation_destroy(s->tmp);
+  phase_hip_equation_destroy(s->pEq);
+  phase_hip_solver_destroy(s->su);
+  phase_hip_solver_destroy(s->sp);
+  delete s;
+  return PHASE_HIP_OK;
+}
+
+const char *phase_hip_fstep_last_error(const phase_hip_fstep *s) { return s ? s->err.c_str() : "null driver"; }
+
+int phase_hip_fstep_create(phase_hip_fstep **out, phase_hip_mesh *m, double rho, double mu, const int *uBcType,
+                           const double *uFace0, const int *pBcType, const double *pFace0, int pinCell) {
+  if (!out || !m || !uBcType || !uFace0) return PHASE_HIP_EINVAL;
+  *out = nullptr;
+  phase_hip_fstep *s = new phase_hip_fstep();
+  s->mesh = m;
+  s->N = m->nCells;
+  s->F = m->nFaces;
+  s->rho = rho;
+  s->mu = mu;
+  s->pin = pinCell;
+  const int rc = guardedF(s, [&] {
+    if (!(rho > 0.) || !(mu > 0.)) fail(PHASE_HIP_EINVAL, "rho and mu must be positive");
+    if (pinCell < -1 || pinCell >= m->nCells) fail(PHASE_HIP_EINVAL, "pin cell out of range (-1: none)");
+    PH_CHECK(hipSetDevice(m->device));
+    hipStream_t st = m->stream;
+    const int N = s->N, F = s->F;
+    ck(phase_hip_equation_create(&s->uEq, m, 2), "uEqn", "");
+    ck(phase_hip_equation_create(&s->tmp, m, 2), "uEqn", "");
+    ck(phase_hip_equation_create(&s->pEq, m, 1), "pEqn", "");
+    ck(phase_hip_solver_create(&s->su, m->device), "solver", "");
+    ck(phase_hip_solver_create(&s->sp, m->device), "solver", "");
+    for (phase_hip_solver *v : {s->su, s->sp}) {
+      ck(phase_hip_solver_setup(v, "tolerance", "1e-8"), "setup", v->err.c_str());
+      ck(phase_hip_solver_setup(v, "maxIters", "5000"), "setup", v->err.c_str());
+    }
+    for (DBuf<double> *b : {&s->uC, &s->u0C, &s->uB, &s->gpC, &s->gsrc}) b->alloc(2 * (size_t)N);
+    for (DBuf<double> *b : {&s->pC, &s->m}) b->alloc(N);
+    for (DBuf<double> *b : {&s->uF, &s->u0F, &s->gpF}) b->alloc(2 * (size_t)F);
+    s->pF.alloc(F);
+    s->maxBuf.alloc(1);
+    for (DBuf<double> *b : {&s->uC, &s->pC, &s->gpC, &s->gpF}) b->zero(st);
+    std::vector<int> ubc(uBcType, uBcType + F), pbc(F, PHASE_HIP_BC_NORMAL_GRADIENT);
+    if (pBcType) pbc.assign(pBcType, pBcType + F);
+    s->uBc.upload(ubc, st);
+    s->pBc.upload(pbc, st);
+    // u_f at rest: the boundary values (FIXED faces keep them; interpolateFaces writes the rest)
+    PH_CHECK(hipMemcpyAsync(s->uF.p, uFace0, 2 * (size_t)F * sizeof(double), hipMemcpyHostToDevice, st));
+    if (pFace0) PH_CHECK(hipMemcpyAsync(s->pF.p, pFace0, (size_t)F * sizeof(double), hipMemcpyHostToDevice, st));
+    else s->pF.zero(st);
+    // FractionalStep::initialize (FractionalStep.cpp:24-27)
+    ck(phase_hip_field_interpolate_faces(m, 2, s->uC.p, s->uBc.p, s->uF.p), "interpolateFaces", m->err.c_str());
+    PH_CHECK(hipStreamSynchronize(st));
+  });
+  if (rc != PHASE_HIP_OK) {
+    phase_hip_fstep_destroy(s);
+    return rc;
+  }
+  *out = s;
+  return PHASE_HIP_OK;
+}
+
+int phase_hip_fstep_setup(phase_hip_fstep *s, const char *key, const char *value) {
+  return guardedF(s, [&] {
+    if (!key || !value) fail(PHASE_HIP_EINVAL, "null key/value");
+    std::string k(key);
+    phase_hip_solver *v = nullptr;
+    if (k.rfind("u.", 0) == 0) v = s->su;
+    else if (k.rfind("p.", 0) == 0) v = s->sp;
+    else fail(PHASE_HIP_EINVAL, "keys are u.<solver key> or p.<solver key>");
+    ck(phase_hip_solver_setup(v, k.c_str() + 2, value), "setup", v->err.c_str());
+  });
+}
+
+int phase_hip_fstep_advance(phase_hip_fstep *s, double dt, int nSteps, double *history) {
+  return guardedF(s, [&] {
+    if (nSteps < 0 || !(dt > 0.)) fail(PHASE_HIP_EINVAL, "need dt > 0 and nSteps >= 0");
+    PH_CHECK(hipSetDevice(s->mesh->device));
+    for (int it = 0; it < nSteps; ++it) timeStep(s, dt, history ? history + 3 * (size_t)it : nullptr);
+  });
+}
+
+int phase_hip_fstep_fields(phase_hip_fstep *s, double *uCell, double *pCell, double *uFace) {
+  return guardedF(s, [&] {
+    hipStream_t st = s->mesh->stream;
+    if (uCell) PH_CHECK(hipMemcpyAsync(uCell, s->uC.p, 2 * (size_t)s->N * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (pCell) PH_CHECK(hipMemcpyAsync(pCell, s->pC.p, (size_t)s->N * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (uFace) PH_CHECK(hipMemcpyAsync(uFace, s->uF.p, 2 * (size_t)s->F * sizeof(double), hipMemcpyDeviceToHost, st));
+    PH_CHECK(hipStreamSynchronize(st));
+  });
+}
 
 const char *phase_hip_simple_last_error(const phase_hip_simple *s) { return s ? s->err.c_str() : "null driver"; }
 

@@ -98,6 +98,13 @@ _SIGS = {
     "phase_hip_field_grad": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "phase_hip_field_interpolate_faces": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp]),
     "phase_hip_mesh_synchronize": (ctypes.c_int, [_vp]),
+    "phase_hip_fstep_create": (ctypes.c_int, [ctypes.POINTER(_vp), _vp, ctypes.c_double, ctypes.c_double, _i32p,
+                                              _f64p, _vp, _vp, ctypes.c_int]),
+    "phase_hip_fstep_destroy": (ctypes.c_int, [_vp]),
+    "phase_hip_fstep_last_error": (ctypes.c_char_p, [_vp]),
+    "phase_hip_fstep_setup": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_char_p]),
+    "phase_hip_fstep_advance": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, _vp]),
+    "phase_hip_fstep_fields": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "phase_hip_simple_create": (ctypes.c_int, [ctypes.POINTER(_vp), _vp, ctypes.c_double, ctypes.c_double,
                                                ctypes.c_double, ctypes.c_double, _i32p, _f64p, ctypes.c_int]),
     "phase_hip_simple_destroy": (ctypes.c_int, [_vp]),
@@ -357,6 +364,54 @@ class HipEquation:
         ci, v, rhs = np.empty(n.value, np.int32), np.empty(n.value), np.empty(self.rank)
         self._check(load_library().phase_hip_eq_export(self._h, n, _p(rp), _p(ci), _p(v), _p(rhs)), "export")
         return rp, ci, v, rhs
+
+
+class HipFractionalStep:
+    """FractionalStep::solve (src/2D/Unstructured/Solvers/FractionalStep.cpp:36-117)
+    on the device: one projection time step per call of advance()."""
+
+    def __init__(self, mesh, rho, mu, u_bc, u_face0, p_bc=None, p_face0=None, pin_cell=0):
+        h = _vp()
+        self._pbc = None if p_bc is None else _i32(p_bc)
+        self._pf0 = None if p_face0 is None else _f64(p_face0)
+        rc = load_library().phase_hip_fstep_create(ctypes.byref(h), mesh._h, float(rho), float(mu), _i32(u_bc),
+                                                   _f64(np.asarray(u_face0).reshape(-1)), _p(self._pbc),
+                                                   _p(self._pf0), int(pin_cell))
+        if rc != 0:
+            raise PhaseHipError(rc, "fractionalStepCreate", "creation failed")
+        self._h = h
+        self.mesh = mesh
+
+    def _check(self, rc, method):
+        if rc != 0:
+            raise PhaseHipError(rc, method, load_library().phase_hip_fstep_last_error(self._h).decode())
+
+    def setup(self, key, value):
+        self._check(load_library().phase_hip_fstep_setup(self._h, key.encode(), str(value).encode()), "setup")
+
+    def advance(self, dt, n=1):
+        """n time steps; returns an (n, 3) history: momentum and pressure
+        iterations, max divergence of the corrected face velocities."""
+        hist = np.zeros((n, 3))
+        self._check(load_library().phase_hip_fstep_advance(self._h, float(dt), int(n), _p(hist)), "advance")
+        return hist
+
+    def fields(self):
+        N, F = self.mesh.n_cells, self.mesh.n_faces
+        u, p, uf = np.empty(2 * N), np.empty(N), np.empty(2 * F)
+        self._check(load_library().phase_hip_fstep_fields(self._h, _p(u), _p(p), _p(uf)), "fields")
+        return u.reshape(N, 2), p, uf.reshape(F, 2)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().phase_hip_fstep_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class HipSimple:
