@@ -370,6 +370,13 @@ int phase_hip_simple_setup(phase_hip_simple *s, const char *key, const char *val
  * pressure-correction BiCGStab iterations, max |mass imbalance| before the
  * correction, max |p'|. */
 int phase_hip_simple_iterate(phase_hip_simple *s, int nIters, double *history);
+/* PISO (Issa 1986) on the same driver, transient: per time step a momentum
+ * predictor with fv::ddt(rho, u, dt) and no relaxation, then nCorrectors
+ * pressure corrections (Rhie-Chow from the latest u and grad p, p += p').
+ * The relaxation factors given at creation are not used.  history as for
+ * phase_hip_simple_iterate (pressure iterations summed over the correctors,
+ * imbalance and |p'| of the last corrector). */
+int phase_hip_simple_piso_advance(phase_hip_simple *s, double dt, int nSteps, int nCorrectors, double *history);
 /* Host copies of u (cells, interleaved), p (cells), u (faces); NULL skips. */
 int phase_hip_simple_fields(phase_hip_simple *s, double *uCell, double *pCell, double *uFace);
 

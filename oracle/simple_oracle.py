@@ -15,6 +15,10 @@ ScalarFiniteVolumeEquation.cpp:45-55 and collocated Rhie-Chow face velocities:
   p'    = solve(laplacian(rho avg(d), p') == src::div(u_f)), p'_pin = 0
   u    -= d grad p';  u_f -= avg(d) grad_f p' (interior);  p += alpha_p p'
 
+PISO (Issa 1986) uses the same pieces per time step: the momentum predictor
+with fv::ddt(rho, u, dt) and no relaxation, then several pressure corrections
+(alpha_p = 1), each from the latest velocity and pressure gradient.
+
 Face averages use Face::distanceWeight (Face.cpp:60-64).  Elementwise
 arithmetic is written in the device kernels' order, so only the linear solves
 (direct here, BiCGStab there) differ.
@@ -67,31 +71,59 @@ class SimpleOracle:
         self.g = l1 / (l1 + l2)
 
     def iterate(self, n):
+        """n SIMPLE outer iterations; rows of (max |imbalance|, max |p'|)."""
         hist = []
         for _ in range(n):
-            hist.append(self._one())
+            self._pressure_gradient()
+            self._momentum(0.0, self.au)
+            hist.append(self._correct(self.ap))
         return np.array(hist)
 
-    def _one(self):
-        m, N, F = self.m, self.m.n_cells, self.m.n_faces
+    def piso(self, dt, n=1, correctors=2):
+        """n PISO time steps: momentum predictor with ddt, no relaxation, then
+        `correctors` pressure corrections from the latest u and grad p."""
+        hist = []
+        for _ in range(n):
+            self.u0C = self.uC.copy()
+            self._pressure_gradient()
+            self._momentum(dt, 1.0)
+            for k in range(correctors):
+                if k > 0:
+                    self._pressure_gradient()
+                last = self._correct(1.0)
+            hist.append(last)
+        return np.array(hist)
+
+    def _pressure_gradient(self):
+        m, F = self.m, self.m.n_faces
         pF = O.interpolate_faces(m, self.pC, np.zeros(F), bc_type=self.pbc)
-        gF, gC = O.grad(m, self.pC, pF)
-        gsrc = O.src_src(m, gC.reshape(-1), ncomp=2)
+        self.gF, self.gC = O.grad(m, self.pC, pF)
+
+    def _momentum(self, dt, alpha_u):
+        m, N = self.m, self.m.n_cells
+        gsrc = O.src_src(m, self.gC.reshape(-1), ncomp=2)
         E = O.div(m, self.uF.reshape(-1), self.uF.reshape(-1), theta=1.0, bc_type=self.ubc,
                   phi_face=self.uwall.reshape(-1), phi0_cell=self.uC.reshape(-1), phi0_face=self.uwall.reshape(-1),
                   ncomp=2)
+        if dt > 0.0:
+            T = O.ddt(m, dt, self.u0C.reshape(-1), ncomp=2)
+            T += E
+            E = T
         if self.rho != 1.0:
             E.scale(self.rho)
         L = O.laplacian(m, 1, ncomp=2, gamma=self.mu, bc_type=self.ubc, phi_face=self.uwall.reshape(-1))
         L -= gsrc
         E.eq(L)
-        uB = self.uC.T.reshape(-1).copy()
-        E.relax(self.au, uB)
+        if alpha_u < 1.0:
+            E.relax(alpha_u, self.uC.T.reshape(-1).copy())
         rp, ci, v, rhs = E.get()
-        d = np.array([self.vol[c] / _diag_of(rp, ci, v, c) for c in range(N)])
+        self.d = np.array([self.vol[c] / _diag_of(rp, ci, v, c) for c in range(N)])
         x = spla.spsolve(_csr(rp, ci, v).tocsc(), -rhs)
         self.uC = x.reshape(2, N).T.copy()
-        # Rhie-Chow
+
+    def _correct(self, alpha_p):
+        m, F = self.m, self.m.n_faces
+        d, gF, gC = self.d, self.gF, self.gC
         g, l, r = self.g, self.il, self.ir
         dd = g * d[l] + (1.0 - g) * d[r]
         uF = self.uwall.copy()
@@ -115,6 +147,6 @@ class SimpleOracle:
         ppF = O.interpolate_faces(m, pp, np.zeros(F), bc_type=self.pbc)
         gppF, gppC = O.grad(m, pp, ppF)
         self.uC = self.uC - d[:, None] * gppC
-        self.pC = self.pC + self.ap * pp
+        self.pC = self.pC + alpha_p * pp
         self.uF[self.inner] = self.uF[self.inner] - dd[:, None] * gppF[self.inner]
         return np.abs(mdot).max(), np.abs(pp).max()

@@ -40,7 +40,7 @@ struct phase_hip_simple {
   double rho = 1., mu = 1., alphaU = 0.7, alphaP = 0.3;
   phase_hip_equation *uEq = nullptr, *tmp = nullptr, *pEq = nullptr;
   phase_hip_solver *su = nullptr, *sp = nullptr;
-  DBuf<double> uC, uB, uF, uWall, pC, pF, gpF, gpC, gsrc, d, df, rdf, m, pp, ppF, gppF, gppC, zeroF, maxBuf;
+  DBuf<double> uC, u0C, uB, uF, uWall, pC, pF, gpF, gpC, gsrc, d, df, rdf, m, pp, ppF, gppF, gppC, zeroF, maxBuf;
   DBuf<int> uBc, pBc;
   std::string err;
 };
@@ -140,6 +140,11 @@ __global__ void kCorrectFaces(int nFaces, const int *__restrict__ fr, const doub
   }
 }
 
+// u += s * g (both interleaved vectors of n entries)
+__global__ void kAxpy(long n, double sc, const double *__restrict__ g, double *u) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) u[i] += sc * g[i];
+}
+
 // out[0] = max |a_i| (one block, fixed order)
 __global__ void kMaxAbs(int n, const double *__restrict__ a, double *out) {
   __shared__ double s[256];
@@ -181,42 +186,60 @@ double maxAbs(phase_hip_simple *s, const double *a, int n) {
   return v;
 }
 
-void outerIteration(phase_hip_simple *s, double *hist) {
+// p_f (zero gradient on the boundary) and grad p, faces and cells
+void pressureGradient(phase_hip_simple *s) {
   phase_hip_mesh *m = s->mesh;
-  hipStream_t st = m->stream;
-  const int N = s->N, F = s->F;
-  // pressure faces (zero gradient) and gradient
   ck(phase_hip_field_interpolate_faces(m, 1, s->pC.p, s->pBc.p, s->pF.p), "p faces", m->err.c_str());
   ck(phase_hip_field_grad(m, s->pC.p, s->pF.p, s->gpF.p, s->gpC.p), "grad p", m->err.c_str());
+}
+
+// uEqn = rho (ddt(u) +) rho div(u_f, u) == laplacian(mu, u) - src(grad p);
+// dt <= 0: steady (no ddt); alphaU < 1: implicit relaxation.  Leaves d = V/a_P
+// and the solved velocity in uC; returns the momentum iterations.
+int momentum(phase_hip_simple *s, double dt, double alphaU) {
+  phase_hip_mesh *m = s->mesh;
+  hipStream_t st = m->stream;
+  const int N = s->N;
   ck(phase_hip_field_src_src(m, 2, s->gpC.p, s->gsrc.p), "src(grad p)", m->err.c_str());
-  // uEqn = rho div(u_f, u) == laplacian(mu, u) - src(grad p)
   ck(phase_hip_eq_div(s->uEq, s->uF.p, s->uF.p, 1.0, s->uBc.p, s->uWall.p, s->uC.p, s->uWall.p), "fv::div",
      s->uEq->err.c_str());
+  if (dt > 0.) {  // fv::ddt(rho, u, dt) + fv::div(rho u, u), TimeDerivative.h
+    ck(phase_hip_eq_ddt(s->tmp, dt, s->u0C.p), "fv::ddt", s->tmp->err.c_str());
+    ck(phase_hip_eq_add(s->tmp, s->uEq, 1), "operator+", s->tmp->err.c_str());
+    ck(phase_hip_eq_copy(s->uEq, s->tmp), "copy", s->uEq->err.c_str());
+  }
   if (s->rho != 1.) ck(phase_hip_eq_scale(s->uEq, s->rho), "scale", s->uEq->err.c_str());
   ck(phase_hip_eq_laplacian(s->tmp, 1, s->mu, nullptr, nullptr, 1.0, s->uBc.p, s->uWall.p, nullptr, nullptr),
      "fv::laplacian", s->tmp->err.c_str());
   ck(phase_hip_eq_sub_vector(s->tmp, s->gsrc.p), "operator-", s->tmp->err.c_str());
   ck(phase_hip_eq_add(s->uEq, s->tmp, -1), "operator==", s->uEq->err.c_str());
   hipLaunchKernelGGL(kToBlocks, dim3(g256(N)), dim3(256), 0, st, N, s->uC.p, s->uB.p);
-  ck(phase_hip_eq_relax(s->uEq, s->alphaU, s->uB.p), "relax", s->uEq->err.c_str());
+  if (alphaU < 1.) ck(phase_hip_eq_relax(s->uEq, alphaU, s->uB.p), "relax", s->uEq->err.c_str());
   hipLaunchKernelGGL(kMomentumD, dim3(g256(N)), dim3(256), 0, st, N, s->uEq->stride, s->uEq->cols.p,
                      s->uEq->vals.p, s->uEq->cnt.p, m->dVol.p, s->d.p);
-  // momentum solve from the current velocity
+  // momentum solve from the current velocity; like the reference's solve(),
+  // an unconverged inner solve returns its best iterate and the loop goes on
   ck(phase_hip_solver_set_equation(s->su, s->uEq), "uEqn set", s->su->err.c_str());
   ck(phase_hip_solver_set_guess_device(s->su, s->uB.p), "uEqn guess", s->su->err.c_str());
-  // like the reference's solve(), an unconverged inner solve returns its best
-  // iterate and the outer loop goes on
-  double uErr = 0.;
-  int rc = phase_hip_solver_solve(s->su, &uErr);
+  double err = 0.;
+  const int rc = phase_hip_solver_solve(s->su, &err);
   if (rc != PHASE_HIP_ENOCONV) ck(rc, "uEqn solve", s->su->err.c_str());
   ck(phase_hip_solver_get_x_device(s->su, s->uB.p), "uEqn x", s->su->err.c_str());
   hipLaunchKernelGGL(kFromBlocks, dim3(g256(N)), dim3(256), 0, st, N, s->uB.p, s->uC.p);
-  // Rhie-Chow face velocities and mass imbalance
+  return phase_hip_solver_n_iters(s->su);
+}
+
+// Rhie-Chow face velocities from uC and grad p, pCorrEqn = laplacian(rho d_f,
+// p') == src::div(u_f) with p'_pin = 0, then u -= d grad p', u_f -= d_f
+// grad_f p', p += alphaP p'.  Returns the pressure iterations.
+int pressureCorrection(phase_hip_simple *s, double alphaP, double *mMax, double *ppMax) {
+  phase_hip_mesh *m = s->mesh;
+  hipStream_t st = m->stream;
+  const int N = s->N, F = s->F;
   hipLaunchKernelGGL(kRhieChow, dim3(g256(F)), dim3(256), 0, st, F, m->dFL.p, m->dFR.p, m->dCC.p, m->dFC.p,
                      s->uC.p, s->d.p, s->gpC.p, s->gpF.p, s->uWall.p, s->rho, s->uF.p, s->df.p, s->rdf.p);
   ck(phase_hip_field_src_div(m, s->uF.p, s->m.p), "src::div", m->err.c_str());
-  const double mMax = maxAbs(s, s->m.p, N);
-  // pCorrEqn = laplacian(rho d, p') == m
+  if (mMax) *mMax = maxAbs(s, s->m.p, N);
   ck(phase_hip_eq_laplacian(s->pEq, 3, 0., s->rdf.p, nullptr, 1.0, s->pBc.p, s->zeroF.p, nullptr, nullptr),
      "fv::laplacian", s->pEq->err.c_str());
   ck(phase_hip_eq_sub_vector(s->pEq, s->m.p), "operator==", s->pEq->err.c_str());
@@ -224,29 +247,54 @@ void outerIteration(phase_hip_simple *s, double *hist) {
                      s->pEq->cnt.p, s->pEq->rhs.p);
   ck(phase_hip_solver_set_equation(s->sp, s->pEq), "pCorrEqn set", s->sp->err.c_str());
   ck(phase_hip_solver_zero_guess(s->sp), "pCorrEqn guess", s->sp->err.c_str());
-  double pErr = 0.;
-  rc = phase_hip_solver_solve(s->sp, &pErr);
+  double err = 0.;
+  const int rc = phase_hip_solver_solve(s->sp, &err);
   if (rc != PHASE_HIP_ENOCONV) ck(rc, "pCorrEqn solve", s->sp->err.c_str());
   ck(phase_hip_solver_get_x_device(s->sp, s->pp.p), "pCorrEqn x", s->sp->err.c_str());
-  // corrections
   ck(phase_hip_field_interpolate_faces(m, 1, s->pp.p, s->pBc.p, s->ppF.p), "p' faces", m->err.c_str());
   ck(phase_hip_field_grad(m, s->pp.p, s->ppF.p, s->gppF.p, s->gppC.p), "grad p'", m->err.c_str());
-  hipLaunchKernelGGL(kCorrectCells, dim3(g256(N)), dim3(256), 0, st, N, s->d.p, s->gppC.p, s->pp.p, s->alphaP,
+  hipLaunchKernelGGL(kCorrectCells, dim3(g256(N)), dim3(256), 0, st, N, s->d.p, s->gppC.p, s->pp.p, alphaP,
                      s->uC.p, s->pC.p);
   hipLaunchKernelGGL(kCorrectFaces, dim3(g256(F)), dim3(256), 0, st, F, m->dFR.p, s->df.p, s->gppF.p, s->uF.p);
   PH_CHECK(hipGetLastError());
-  const double ppMax = maxAbs(s, s->pp.p, N);
+  if (ppMax) *ppMax = maxAbs(s, s->pp.p, N);
+  return phase_hip_solver_n_iters(s->sp);
+}
+
+// One SIMPLE outer iteration (steady, relaxed).
+void outerIteration(phase_hip_simple *s, double *hist) {
+  pressureGradient(s);
+  const int uIt = momentum(s, 0., s->alphaU);
+  double mMax = 0., ppMax = 0.;
+  const int pIt = pressureCorrection(s, s->alphaP, &mMax, &ppMax);
   if (hist) {
-    hist[0] = phase_hip_solver_n_iters(s->su);
-    hist[1] = phase_hip_solver_n_iters(s->sp);
+    hist[0] = uIt;
+    hist[1] = pIt;
     hist[2] = mMax;
     hist[3] = ppMax;
   }
 }
 
-// u += s * g (both interleaved vectors of n entries)
-__global__ void kAxpy(long n, double sc, const double *__restrict__ g, double *u) {
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) u[i] += sc * g[i];
+// One PISO time step (Issa 1986): momentum predictor with fv::ddt and no
+// relaxation, then nCorr pressure corrections, each from the latest velocity
+// and pressure gradient.
+void pisoStep(phase_hip_simple *s, double dt, int nCorr, double *hist) {
+  hipStream_t st = s->mesh->stream;
+  PH_CHECK(hipMemcpyAsync(s->u0C.p, s->uC.p, 2 * (size_t)s->N * sizeof(double), hipMemcpyDeviceToDevice, st));
+  pressureGradient(s);
+  const int uIt = momentum(s, dt, 1.0);
+  int pIt = 0;
+  double mMax = 0., ppMax = 0.;
+  for (int k = 0; k < nCorr; ++k) {
+    if (k > 0) pressureGradient(s);
+    pIt += pressureCorrection(s, 1.0, &mMax, &ppMax);
+  }
+  if (hist) {
+    hist[0] = uIt;
+    hist[1] = pIt;
+    hist[2] = mMax;
+    hist[3] = ppMax;
+  }
 }
 
 }  // namespace
@@ -495,6 +543,7 @@ int phase_hip_simple_create(phase_hip_simple **out, phase_hip_mesh *m, double rh
       ck(phase_hip_solver_setup(v, "maxIters", "5000"), "setup", v->err.c_str());
     }
     s->uC.alloc(2 * (size_t)N);
+    s->u0C.alloc(2 * (size_t)N);
     s->uB.alloc(2 * (size_t)N);
     s->gpC.alloc(2 * (size_t)N);
     s->gsrc.alloc(2 * (size_t)N);
@@ -550,6 +599,14 @@ int phase_hip_simple_iterate(phase_hip_simple *s, int nIters, double *history) {
     if (nIters < 0) fail(PHASE_HIP_EINVAL, "negative iteration count");
     PH_CHECK(hipSetDevice(s->mesh->device));
     for (int it = 0; it < nIters; ++it) outerIteration(s, history ? history + 4 * (size_t)it : nullptr);
+  });
+}
+
+int phase_hip_simple_piso_advance(phase_hip_simple *s, double dt, int nSteps, int nCorrectors, double *history) {
+  return guardedS(s, [&] {
+    if (nSteps < 0 || !(dt > 0.) || nCorrectors < 1) fail(PHASE_HIP_EINVAL, "need dt > 0, nSteps >= 0, nCorrectors >= 1");
+    PH_CHECK(hipSetDevice(s->mesh->device));
+    for (int it = 0; it < nSteps; ++it) pisoStep(s, dt, nCorrectors, history ? history + 4 * (size_t)it : nullptr);
   });
 }
 

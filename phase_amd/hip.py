@@ -112,6 +112,7 @@ _SIGS = {
     "phase_hip_simple_setup": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_char_p]),
     "phase_hip_simple_iterate": (ctypes.c_int, [_vp, ctypes.c_int, _vp]),
     "phase_hip_simple_fields": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    "phase_hip_simple_piso_advance": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, _vp]),
 }
 
 _lib = None
@@ -440,6 +441,13 @@ class HipSimple:
         pressure-correction iterations, max |mass imbalance|, max |p'|."""
         hist = np.zeros((n, 4))
         self._check(load_library().phase_hip_simple_iterate(self._h, int(n), _p(hist)), "iterate")
+        return hist
+
+    def piso(self, dt, n=1, correctors=2):
+        """n PISO time steps of size dt with `correctors` pressure corrections."""
+        hist = np.zeros((n, 4))
+        self._check(load_library().phase_hip_simple_piso_advance(self._h, float(dt), int(n), int(correctors),
+                                                                 _p(hist)), "piso")
         return hist
 
     def fields(self):
