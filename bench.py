@@ -104,6 +104,54 @@ def train_network(s, mesh, d_g, d_bc, d_pf, args):
                 train_s=round(t2 - t1, 2), loss_first=round(first, 4), loss_last=round(loss, 4))
 
 
+def bubble_case(args):
+    """The C3 rising-bubble pressure system on a 1x2 quad mesh stored
+    unstructured: a circular bubble of radius 0.25 centred at (0.5, 0.5), face
+    density rho = 1000 outside, 1 inside (sharp indicator from the face
+    centroids; FractionalStepMultiphase::updateProperties,
+    FractionalStepMultiphase.cpp:175-185), pressure equation
+    laplacian(dt / rho_f, p) == src::div(u) with dt = 1e-4
+    (FractionalStepMultiphase.cpp:152-154), p = 0 on the top patch and zero
+    gradient elsewhere; src::div(u) stands in as a zero-mean N(0,1) field."""
+    from phase_amd import hip as P
+    from phase_amd import meshgen as G
+
+    nx = int(round(np.sqrt(args.bubble_cells / 2)))
+    xy, cptr, cind = G.rectilinear(nx, 2 * nx, width=1.0, height=2.0)
+    mesh = P.HipMesh(xy, cptr, cind)
+    f = mesh.faces()
+    _, _, fc, _ = mesh.geometry()
+    a, bn = xy[f["lnode"]], xy[f["rnode"]]
+    top = (f["rcell"] < 0) & (a[:, 1] == 2.0) & (bn[:, 1] == 2.0)
+    bc = np.where(top, P.BC_FIXED, P.BC_NORMAL_GRADIENT).astype(np.int32)
+    inside = (fc[:, 0] - 0.5) ** 2 + (fc[:, 1] - 0.5) ** 2 < 0.25 ** 2
+    rho_f = np.where(inside, 1.0, 1000.0)
+    gamma = 1e-4 / rho_f
+    rhs = G.zero_mean_normal(mesh.n_cells, seed=7)
+    d_g, d_bc, d_pf, d_b = (P.DeviceArray(gamma), P.DeviceArray(bc), P.DeviceArray(np.zeros(mesh.n_faces)),
+                            P.DeviceArray(rhs))
+    out = dict(workload="rising-bubble-pressure-1x2-quads", cells=mesh.n_cells, density_ratio=1000.0)
+    for mode in ("fp64", "mixed"):
+        s = P.HipSparseMatrixSolver(0, tolerance=args.tol, maxIters=100000, mode=mode, blockRows=args.block_rows)
+
+        def step():
+            s.assemble_laplacian_device(mesh, 3, gamma_face=d_g, bc_type=d_bc, phi_face=d_pf)
+            s.sub_rhs_device(d_b)
+            s.zero_guess()
+            return s.solve(raise_on_noconv=False)
+
+        step()
+        t0 = time.perf_counter()
+        rel = step()
+        dt = time.perf_counter() - t0
+        st = s.stats()
+        out[mode] = dict(ms_per_step=round(1e3 * dt, 2), relres=float("%.4g" % rel), iters=st["inner_iters"],
+                         fallbacks=st["fallbacks"])
+        log("bubble %s: %.2f ms/step, relres %.2e, %d iterations" % (mode, 1e3 * dt, rel, st["inner_iters"]))
+        s.close()
+    return out
+
+
 def simple_cavity(args):
     """The C1 case on the device SIMPLE driver: lid-driven cavity on the
     128x128 unit square, Re = 100 (rho 1, mu 0.01), alpha_u 0.7, alpha_p 0.3,
@@ -169,6 +217,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--simple-iters", type=int, default=200, help="C1 SIMPLE outer iterations (0 skips)")
+    ap.add_argument("--bubble-cells", type=int, default=2_000_000, help="C3 bubble pressure solve (0 skips)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported (the path would shard by "
@@ -276,6 +325,8 @@ def main():
     }
     if args.simple_iters > 0:
         out["simple_c1"] = simple_cavity(args)
+    if args.bubble_cells > 0:
+        out["bubble_c3"] = bubble_case(args)
     if not args.no_cpu_baseline:
         it = results.get("fp64", results[head])["iters"]
         out["cpu_baseline"] = cpu_baseline(xy, cptr, cind, bc, g, b, it)
