@@ -227,18 +227,37 @@ __global__ void kConvert(int n, const TI *__restrict__ in, TO *__restrict__ out)
 // Row `slice*64+lane` of A x.  The matrix values may be wider than the vectors
 // (TV = double with T = float: FP64 coefficients, FP32 Krylov vectors); the
 // products then accumulate in the wider type and round once at the end.
+// The slice is uniform over the wavefront, so its width K is a scalar and the
+// loads are issued as three batches -- columns and values, then the gathered
+// x, then the fma chain in slot order -- rather than one dependent pair of
+// round trips per slot (padding slots, column -1, read x[0] and add nothing).
+constexpr int kSellBatch = 8;
 template <class T, class TV>
 __device__ __forceinline__ T sellRow(int slice, int lane, const int *__restrict__ sliceOff,
                                      const int *__restrict__ col, const TV *__restrict__ val,
                                      const T *__restrict__ x) {
   using A = typename std::conditional<(sizeof(TV) > sizeof(T)), TV, T>::type;
+  slice = __builtin_amdgcn_readfirstlane(slice);
   const int o = sliceOff[slice];
   const int K = (sliceOff[slice + 1] - o) >> 6;
   A acc = 0;
-  for (int e = 0; e < K; ++e) {
-    const int idx = o + (e << 6) + lane;
-    const int c = col[idx];
-    if (c >= 0) acc = fma((A)val[idx], (A)x[c], acc);
+  for (int e0 = 0; e0 < K; e0 += kSellBatch) {
+    int c[kSellBatch];
+    TV a[kSellBatch];
+#pragma unroll
+    for (int q = 0; q < kSellBatch; ++q)
+      if (e0 + q < K) {
+        const int idx = o + ((e0 + q) << 6) + lane;
+        c[q] = col[idx];
+        a[q] = val[idx];
+      }
+    T xv[kSellBatch];
+#pragma unroll
+    for (int q = 0; q < kSellBatch; ++q)
+      if (e0 + q < K) xv[q] = x[c[q] >= 0 ? c[q] : 0];
+#pragma unroll
+    for (int q = 0; q < kSellBatch; ++q)
+      if (e0 + q < K && c[q] >= 0) acc = fma((A)a[q], (A)xv[q], acc);
   }
   return (T)acc;
 }
