@@ -10,9 +10,14 @@ edge, zero-gradient elsewhere.  Each step assembles fv::laplacian(gamma_f, p)
 == b on the device from HBM-resident inputs, factors ILU(0) and solves to an
 FP64 true relative residual of 1e-8 from a zero initial guess.
 
-Prints ONE JSON line.  `value` is ms per step of the fastest method
-("mixed": FP32 BiCGStab+ILU(0) in FP64 iterative refinement); the plain FP64
-BiCGStab+ILU(0) time is reported beside it, as is the FP64 SpMV bandwidth.
+Prints ONE JSON line.  `value` is ms per step of the fastest method that
+reaches the FP64 goal (plain FP64 BiCGStab+ILU(0), or "mixed": FP32
+BiCGStab+ILU(0) in FP64 iterative refinement); every method's time is reported
+beside it, as are the SpMV bandwidths and the per-launch kernel timings.
+`roofline.traffic` is the HBM bytes per launch of the dominant kernel read from
+the committed rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, each scaled by
+the same counter on a copy kernel of known size; `--copy-probe-mb` runs that
+copy kernel for the counter pass).
 """
 import argparse
 import json
@@ -31,6 +36,34 @@ METRIC = "pCorr solve ms/step at 1e-8 rel. residual, learned+mixed vs plain; SpM
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_sweeps_pcorr_poisson_1M.md")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` in the committed counter summary's
+    table (column 5 of the row that names it), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("| `%s` |" % kernel):
+                    return int(line.split("|")[5].strip().replace(",", ""))
+    except (OSError, ValueError, IndexError):
+        return None
+    return None
+
+
+def copy_probe(mib):
+    """One plain copy of `mib` MiB (read once, written once; kCopy16) for the
+    counter scale factors of the --pmc passes."""
+    import phase_amd as P
+    n = mib * (1 << 20) // 8
+    a, b = P.DeviceArray.zeros(n), P.DeviceArray.zeros(n)
+    b.copy_from(a)
+    log("copy probe: %d bytes read, %d bytes written" % (8 * n, 8 * n))
+    a.free()
+    b.free()
 
 
 def clean(v):
@@ -218,6 +251,7 @@ def main():
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--simple-iters", type=int, default=200, help="C1 SIMPLE outer iterations (0 skips)")
     ap.add_argument("--bubble-cells", type=int, default=2_000_000, help="C3 bubble pressure solve (0 skips)")
+    ap.add_argument("--copy-probe-mb", type=int, default=0, help="run one copy kernel of this size (counter scale)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported (the path would shard by "
@@ -276,6 +310,8 @@ def main():
             ordinfo = s.ordering_info()
             nnz = len(s.export()[1])
         s.close()
+    if args.copy_probe_mb > 0:
+        copy_probe(args.copy_probe_mb)
 
     n = mesh.n_cells
     gbs = {k: kbytes[k] / (kernels[k] * 1e-3) / 1e9 for k in kernels if kernels[k] > 0}
@@ -296,7 +332,8 @@ def main():
     names = {"usolve64_launch": "kUSolve<double>", "lsolve64_launch": "kLSolve<double>",
              "lusolve64_launch": "kLUSolve<double>", "spmv64": "kSpmvDotV/T<double>"}
     roof = dict(kernel=names[dom], bound="hbm", achieved=round(gbs[dom], 1), peak=PEAK_HBM_GBS, unit="GB/s",
-                frac=round(gbs[dom] / PEAK_HBM_GBS, 4), traffic=None, algorithmic_bytes=round(kbytes[dom]),
+                frac=round(gbs[dom] / PEAK_HBM_GBS, 4), traffic=measured_traffic(names[dom]),
+                algorithmic_bytes=round(kbytes[dom]),
                 avg_launch_ms=round(kernels[dom], 5))
 
     out = {

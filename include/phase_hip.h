@@ -79,6 +79,10 @@ int phase_hip_device_count(int *count);
 int phase_hip_malloc(int device, size_t bytes, void **ptr);
 int phase_hip_free(void *ptr);
 int phase_hip_memcpy(void *dst, const void *src, size_t bytes, int toDevice);
+/* Device-to-device copy through a plain copy kernel (each byte read once and
+ * written once; pointers and size 16-byte aligned, else EINVAL).  Also the
+ * known-size copy against which HBM traffic counters are scaled. */
+int phase_hip_copy_device(void *dst, const void *src, size_t bytes);
 
 /* ------------------------------------------------------------------ solver */
 

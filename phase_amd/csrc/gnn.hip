@@ -94,7 +94,7 @@ __global__ __launch_bounds__(kRedBlock) void kGnnRms(int n, const float *__restr
     const double v = (double)r[i] / (double)sd[i];
     d[0] += v * v;
   }
-  if (reduceLast<1>(d, partials, counter) && threadIdx.x == 0) {
+  if (reduceLast<1>(d, partials, counter, blockIdx.x) && threadIdx.x == 0) {
     const double rms = sqrt(d[0] / (double)n);
     scal[0] = rms > 0. ? rms : 1.;
   }
@@ -275,7 +275,7 @@ __global__ __launch_bounds__(kRedBlock) void kGnnLossCos(int n, const float *__r
     d[1] += o * o;
     d[2] += ti * ti;
   }
-  if (reduceLast<3>(d, partials, counter) && threadIdx.x == 0) {
+  if (reduceLast<3>(d, partials, counter, blockIdx.x) && threadIdx.x == 0) {
     const double pq = d[1] * d[2];
     const double loss = pq > 0. ? 1.0 - d[0] * d[0] / pq : 1.0;
     scal[1] = loss;

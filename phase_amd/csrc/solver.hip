@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -547,6 +548,12 @@ void solverEnsureSingle(phase_hip_solver *s) { ensureSingle(s); }
 }  // namespace phase
 
 namespace {
+// Plain copy, 16 bytes per lane per step: one read and one write of every byte.
+__global__ void kCopy16(size_t n16, const uint4 *__restrict__ src, uint4 *__restrict__ dst) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+
 __global__ void kCmpInt(size_t n, const int *__restrict__ a, const int *__restrict__ b, int *flag) {
   bool diff = false;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -801,6 +808,17 @@ int phase_hip_free(void *ptr) { return hipFree(ptr) == hipSuccess ? PHASE_HIP_OK
 int phase_hip_memcpy(void *dst, const void *src, size_t bytes, int toDevice) {
   hipError_t e = hipMemcpy(dst, src, bytes, toDevice ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost);
   return e == hipSuccess ? PHASE_HIP_OK : PHASE_HIP_EDEVICE;
+}
+
+int phase_hip_copy_device(void *dst, const void *src, size_t bytes) {
+  if ((!dst || !src) && bytes) return PHASE_HIP_EINVAL;
+  if (((uintptr_t)dst | (uintptr_t)src | bytes) & 15) return PHASE_HIP_EINVAL;
+  if (!bytes) return PHASE_HIP_OK;
+  const size_t n16 = bytes / 16;
+  const unsigned grid = (unsigned)std::min<size_t>((n16 + 255) / 256, 8192);
+  hipLaunchKernelGGL(kCopy16, dim3(grid), dim3(256), 0, 0, n16, (const uint4 *)src, (uint4 *)dst);
+  if (hipGetLastError() != hipSuccess) return PHASE_HIP_EDEVICE;
+  return hipDeviceSynchronize() == hipSuccess ? PHASE_HIP_OK : PHASE_HIP_EDEVICE;
 }
 
 int phase_hip_solver_solve(phase_hip_solver *s, double *relres) {

@@ -23,6 +23,17 @@ constexpr int kMaxOffL = 8;  // off-block L entries of one row held in registers
 constexpr int kMaxIn = 8;    // in-block L / U entries of one row held in registers
 constexpr int kRedBlock = 256;
 
+// The dispatcher hands workgroups to the 8 XCDs round robin (workgroup w runs
+// on XCD w % 8), and each XCD has its own L2.  xcdTile(w, G) gives XCD k one
+// contiguous range of the grid's G tiles instead of every eighth tile: under
+// the bandwidth-reducing row order neighbouring tiles gather the same x lines,
+// which then come from one L2 instead of up to eight.  A bijection of [0, G).
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcdTile(int w, int G) {
+  const int per = G / kXcds, rem = G % kXcds, x = w % kXcds;
+  return x * per + min(x, rem) + w / kXcds;
+}
+
 enum { kProNone = 0, kProP = 1, kProS = 2 };
 
 template <class T>
@@ -82,7 +93,7 @@ __device__ __forceinline__ double waveSum(double v) {
 // last block to finish (atomic ticket) sums the partials in index order.
 // Returns true in the last block; thread 0 then holds the totals in v.
 template <int NV>
-__device__ bool reduceLast(double (&v)[NV], double *partials, unsigned *counter) {
+__device__ bool reduceLast(double (&v)[NV], double *partials, unsigned *counter, int slot) {
   __shared__ double sred[NV][kRedBlock / 64];
   __shared__ bool sLast;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -97,7 +108,7 @@ __device__ bool reduceLast(double (&v)[NV], double *partials, unsigned *counter)
     for (int q = 0; q < NV; ++q) {
       double s = 0.;
       for (int w = 0; w < kRedBlock / 64; ++w) s += sred[q][w];
-      partials[q * gridDim.x + blockIdx.x] = s;
+      partials[q * gridDim.x + slot] = s;
     }
     __threadfence();
     unsigned t = atomicAdd(counter, 1u);
@@ -125,12 +136,12 @@ __device__ bool reduceLast(double (&v)[NV], double *partials, unsigned *counter)
 }
 
 // Fence-free deterministic reduction, part 1: each block writes its
-// fixed-order sums to partials[q * gridDim.x + blockIdx.x].  A one-block
+// fixed-order sums to partials[q * gridDim.x + slot] (slot: the block's tile).  A one-block
 // finisher kernel (launched next on the stream, so the kernel boundary orders
 // the writes) adds them up in index order.  Unlike reduceLast this needs no
 // device-scope fence per block.
 template <int NV>
-__device__ void blockPartials(double (&v)[NV], double *partials) {
+__device__ void blockPartials(double (&v)[NV], double *partials, int slot) {
   __shared__ double sred[NV][kRedBlock / 64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -144,7 +155,7 @@ __device__ void blockPartials(double (&v)[NV], double *partials) {
     for (int q = 0; q < NV; ++q) {
       double s = 0.;
       for (int w = 0; w < kRedBlock / 64; ++w) s += sred[q][w];
-      partials[q * gridDim.x + blockIdx.x] = s;
+      partials[q * gridDim.x + slot] = s;
     }
   }
 }
@@ -267,7 +278,8 @@ template <class T>
 __global__ __launch_bounds__(kRedBlock) void kSpmv(int n, int nSlices, const int *__restrict__ sliceOff,
                                                    const int *__restrict__ col, const T *__restrict__ val,
                                                    const T *__restrict__ x, T *__restrict__ y) {
-  const int slice = blockIdx.x * (kRedBlock / 64) + (threadIdx.x >> 6);
+  const int tile = xcdTile(blockIdx.x, gridDim.x);
+  const int slice = tile * (kRedBlock / 64) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (slice >= nSlices) return;
   const int row = slice * 64 + lane;
@@ -283,7 +295,8 @@ __global__ __launch_bounds__(kRedBlock) void kSpmvDotV(int n, int nSlices, const
                                                        const T *__restrict__ r0, const Scalars *sc,
                                                        double *partials) {
   if (sc->done) return;
-  const int slice = blockIdx.x * (kRedBlock / 64) + (threadIdx.x >> 6);
+  const int tile = xcdTile(blockIdx.x, gridDim.x);
+  const int slice = tile * (kRedBlock / 64) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int row = slice * 64 + lane;
   double d[1] = {0.};
@@ -292,7 +305,7 @@ __global__ __launch_bounds__(kRedBlock) void kSpmvDotV(int n, int nSlices, const
     v[row] = y;
     d[0] = (double)r0[row] * (double)y;
   }
-  blockPartials<1>(d, partials);
+  blockPartials<1>(d, partials, tile);
 }
 
 static __global__ __launch_bounds__(kRedBlock) void kFinishV(Scalars *sc, const double *partials, int nb) {
@@ -314,7 +327,8 @@ __global__ __launch_bounds__(kRedBlock) void kSpmvDotT(int n, int nSlices, const
                                                        const T *__restrict__ s, const Scalars *sc,
                                                        double *partials) {
   if (sc->done) return;
-  const int slice = blockIdx.x * (kRedBlock / 64) + (threadIdx.x >> 6);
+  const int tile = xcdTile(blockIdx.x, gridDim.x);
+  const int slice = tile * (kRedBlock / 64) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int row = slice * 64 + lane;
   double d[2] = {0., 0.};
@@ -324,7 +338,7 @@ __global__ __launch_bounds__(kRedBlock) void kSpmvDotT(int n, int nSlices, const
     d[0] = (double)y * (double)s[row];
     d[1] = (double)y * (double)y;
   }
-  blockPartials<2>(d, partials);
+  blockPartials<2>(d, partials, tile);
 }
 
 static __global__ __launch_bounds__(kRedBlock) void kFinishT(Scalars *sc, const double *partials, int nb) {
@@ -345,7 +359,8 @@ __global__ __launch_bounds__(kRedBlock) void kResidual(int n, int nSlices, const
                                                        const T *__restrict__ x, const T *__restrict__ b,
                                                        T *__restrict__ r, T *__restrict__ r0, Scalars *sc,
                                                        double *partials, unsigned *counter) {
-  const int slice = blockIdx.x * (kRedBlock / 64) + (threadIdx.x >> 6);
+  const int tile = xcdTile(blockIdx.x, gridDim.x);
+  const int slice = tile * (kRedBlock / 64) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int row = slice * 64 + lane;
   double d[1] = {0.};
@@ -356,7 +371,7 @@ __global__ __launch_bounds__(kRedBlock) void kResidual(int n, int nSlices, const
     if (r0) r0[row] = ri;
     d[0] = (double)ri * (double)ri;
   }
-  if (reduceLast<1>(d, partials, counter) && threadIdx.x == 0) sc->rr = d[0];
+  if (reduceLast<1>(d, partials, counter, tile) && threadIdx.x == 0) sc->rr = d[0];
 }
 
 // *out = (v, v)
@@ -366,7 +381,7 @@ __global__ __launch_bounds__(kRedBlock) void kNorm2(int n, const T *__restrict__
   double d[1] = {0.};
   for (int i = blockIdx.x * kRedBlock + threadIdx.x; i < n; i += gridDim.x * kRedBlock)
     d[0] += (double)v[i] * (double)v[i];
-  if (reduceLast<1>(d, partials, counter) && threadIdx.x == 0) *out = d[0];
+  if (reduceLast<1>(d, partials, counter, blockIdx.x) && threadIdx.x == 0) *out = d[0];
 }
 
 // ---------------------------------------------------------------- BiCGStab vector steps
@@ -415,7 +430,7 @@ __global__ __launch_bounds__(kRedBlock) void kUpdate(int n, const Scalars *sc, T
     d[0] += (double)r0[i] * (double)ri;
     d[1] += (double)ri * (double)ri;
   }
-  blockPartials<2>(d, partials);
+  blockPartials<2>(d, partials, blockIdx.x);
 }
 
 // rho' = (r0, r), rr = (r, r), beta, stopping tests
@@ -465,7 +480,8 @@ __global__ __launch_bounds__(kRedBlock) void kSpmvDot2(int n, int nSlices, const
                                                        const T *__restrict__ e, T *__restrict__ q,
                                                        const T *__restrict__ r, Scalars *sc, double *partials,
                                                        unsigned *counter) {
-  const int slice = blockIdx.x * (kRedBlock / 64) + (threadIdx.x >> 6);
+  const int tile = xcdTile(blockIdx.x, gridDim.x);
+  const int slice = tile * (kRedBlock / 64) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int row = slice * 64 + lane;
   double d[2] = {0., 0.};
@@ -475,7 +491,7 @@ __global__ __launch_bounds__(kRedBlock) void kSpmvDot2(int n, int nSlices, const
     d[0] = (double)r[row] * (double)y;
     d[1] = (double)y * (double)y;
   }
-  if (reduceLast<2>(d, partials, counter) && threadIdx.x == 0) {
+  if (reduceLast<2>(d, partials, counter, tile) && threadIdx.x == 0) {
     sc->ts = d[0];
     sc->tt = d[1];
   }
@@ -495,7 +511,7 @@ __global__ __launch_bounds__(kRedBlock) void kMinResUpdate(int n, const Scalars 
     r[i] = ri;
     d[0] += (double)ri * (double)ri;
   }
-  if (reduceLast<1>(d, partials, counter) && threadIdx.x == 0) outer->rr = d[0];
+  if (reduceLast<1>(d, partials, counter, blockIdx.x) && threadIdx.x == 0) outer->rr = d[0];
 }
 
 // x64 += scale * x32 with scale = sqrt(*rrOuter) (undoing the inner normalisation)
@@ -521,8 +537,8 @@ __global__ void kNormalise(int n, const double *__restrict__ r64, T *__restrict_
 // ---------------------------------------------------------------- ILU(0)
 // How a sweep launch maps workgroups to ILU blocks.
 //  * per colour (syncFree == 0): one launch per block colour; blocks of one
-//    colour share no coupling, so block = blockBase + blockIdx.x needs no
-//    waiting -- kernel boundaries order the colours.
+//    colour share no coupling, so block = blockBase + xcdTile(blockIdx.x)
+//    needs no waiting -- kernel boundaries order the colours.
 //  * sync-free (syncFree == 1): one launch for the whole sweep; blocks take
 //    tickets in order and wait on completion flags of their predecessor
 //    blocks (device-scope release/acquire).
@@ -535,7 +551,7 @@ struct SweepCtl {
 
 // Block of this workgroup, or -1 (sync-free tickets beyond the sweep).
 __device__ __forceinline__ int sweepBlock(const SweepCtl &c, const SyncDev &sy, int nb, bool reverse) {
-  if (!c.syncFree) return c.blockBase + (int)blockIdx.x;
+  if (!c.syncFree) return c.blockBase + xcdTile(blockIdx.x, gridDim.x);
   const long long bi = takeTicket(sy.ticket, c.ticketBase);
   if (bi < 0 || bi >= nb) return -1;
   return reverse ? nb - 1 - (int)bi : (int)bi;
@@ -817,7 +833,7 @@ __global__ __launch_bounds__(64) void kLUSolve(IluDev<T> L, SyncDev sy, SweepCtl
                                                const T *__restrict__ v, T *__restrict__ x) {
   __shared__ T ylds[64];
   __shared__ T xlds[64];
-  const int b = ctl.blockBase + (int)blockIdx.x;
+  const int b = ctl.blockBase + xcdTile(blockIdx.x, gridDim.x);
   const int lane = threadIdx.x;
   const int done = sc->done;
   const BlockInfo B = L.blocks[b];

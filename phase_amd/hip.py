@@ -67,6 +67,7 @@ _SIGS = {
     "phase_hip_malloc": (ctypes.c_int, [ctypes.c_int, ctypes.c_size_t, ctypes.POINTER(_vp)]),
     "phase_hip_free": (ctypes.c_int, [_vp]),
     "phase_hip_memcpy": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t, ctypes.c_int]),
+    "phase_hip_copy_device": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
     "phase_hip_equation_export_padded": (ctypes.c_int, [_vp, _ip, _vp, _vp, _vp, _vp]),
     "phase_hip_learned_capture": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "phase_hip_learned_n_pairs": (ctypes.c_int, [_vp, _ip]),
@@ -193,6 +194,14 @@ class DeviceArray:
         rc = load_library().phase_hip_memcpy(self.ptr, h.ctypes.data_as(_vp), self.nbytes, 1)
         if rc != 0:
             raise PhaseHipError(rc, "memcpy", "host to device copy failed")
+
+    def copy_from(self, other):
+        """Device-to-device copy of another DeviceArray of the same size."""
+        if other.nbytes != self.nbytes:
+            raise ValueError("size mismatch")
+        rc = load_library().phase_hip_copy_device(self.ptr, other.ptr, self.nbytes)
+        if rc != 0:
+            raise PhaseHipError(rc, "copy_device", "device to device copy failed")
 
     def free(self):
         if getattr(self, "ptr", None) and self.ptr.value:

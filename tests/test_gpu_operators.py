@@ -201,3 +201,15 @@ def test_equation_solve(hip, oracle, meshes):
     s.set_equation(he)
     s.solve()
     assert np.linalg.norm(s.x() - xs) <= 1e-8 * np.linalg.norm(xs)
+
+
+def test_copy_device(hip):
+    """phase_hip_copy_device: a byte-exact copy (with a ragged tail of the
+    grid-stride loop) and EINVAL for sizes that are not 16-byte multiples."""
+    src = np.random.default_rng(5).standard_normal(1_000_002)
+    a, b = hip.DeviceArray(src), hip.DeviceArray.zeros(len(src))
+    b.copy_from(a)
+    assert same(b.get(), src)
+    c = hip.DeviceArray(np.zeros(3))
+    with pytest.raises(hip.PhaseHipError):
+        hip.DeviceArray(np.zeros(3)).copy_from(c)  # 24 bytes
