@@ -109,22 +109,23 @@ def cpu_baseline(xy, cptr, cind, bc, g, b, iters_target):
                        % (t_asm, setup, 1e3 * per_iter, iters_target))
 
 
-def train_network(s, mesh, d_g, d_bc, d_pf, args):
+def train_network(s, mesh, d_g, d_bc, d_pf, args, max_pairs=256, capture_solves=None):
     """The learned part's setup, outside the timed steps: FP64 solves of earlier
     'time steps' (right-hand sides from seeds 100, 101, ...) emit (residual,
     correction) pairs; the network trains on them with the device kernels."""
     from phase_amd import meshgen as G
 
     t0 = time.perf_counter()
+    nsolves = args.capture_solves if capture_solves is None else capture_solves
     s.setup("mode", "fp64")
-    for t in range(args.capture_solves):
+    for t in range(nsolves):
         s.assemble_laplacian_device(mesh, 3, gamma_face=d_g, bc_type=d_bc, phi_face=d_pf)
         if t == 0:
-            s.learned_capture(args.capture_every, 256)
+            s.learned_capture(args.capture_every, max_pairs)
         s.sub_rhs(G.zero_mean_normal(mesh.n_cells, seed=100 + t))
         s.zero_guess()
         s.solve(raise_on_noconv=False)
-    s.learned_capture(0, 256)
+    s.learned_capture(0, max_pairs)
     pairs = s.learned_n_pairs()
     t1 = time.perf_counter()
     first = s.learned_train(50, lr=args.lr, seed=1)
@@ -132,9 +133,62 @@ def train_network(s, mesh, d_g, d_bc, d_pf, args):
     t2 = time.perf_counter()
     s.setup("mode", "learned")
     log("learned: %d pairs from %d FP64 solves (%.1fs), %d Adam steps (%.1fs), loss %.3f -> %.3f"
-        % (pairs, args.capture_solves, t1 - t0, args.train_steps + 50, t2 - t1, first, loss))
+        % (pairs, nsolves, t1 - t0, args.train_steps + 50, t2 - t1, first, loss))
     return dict(train_pairs=pairs, train_steps=args.train_steps + 50, capture_s=round(t1 - t0, 2),
                 train_s=round(t2 - t1, 2), loss_first=round(first, 4), loss_last=round(loss, 4))
+
+
+def learned_case(args):
+    """The learned-preconditioner case: the standalone pressure Poisson on the
+    same Delaunay generator at `args.c4_cells` cells with lognormal face
+    coefficients of sigma = 2.  Plain FP64, mixed, and learned + mixed with a
+    network trained on up to `args.c4_pairs` (residual, correction) pairs that
+    FP64 solves of other right-hand sides emit (training outside the timed
+    steps)."""
+    from phase_amd import hip as P
+    from phase_amd import meshgen as G
+
+    k = int(round(512 * np.sqrt(args.c4_cells / 1e6)))
+    xy, cptr, cind = build_problem(args.c4_cells, k, 2.0)
+    mesh = P.HipMesh(xy, cptr, cind)
+    f = mesh.faces()
+    pid = G.boundary_patch(f["lnode"], f["rnode"], f["rcell"], xy)
+    bc = np.where(pid == 2, P.BC_FIXED, P.BC_NORMAL_GRADIENT).astype(np.int32)
+    g = G.lognormal_face_coeffs(mesh.n_faces, 2.0, seed=2)
+    b = G.zero_mean_normal(mesh.n_cells, seed=3)
+    d_g, d_bc, d_pf, d_b = (P.DeviceArray(g), P.DeviceArray(bc), P.DeviceArray(np.zeros(mesh.n_faces)),
+                            P.DeviceArray(b))
+    out = dict(workload="pcorr-poisson-delaunay-sigma2", cells=mesh.n_cells, edge_points=k, sigma=2.0)
+    for mode in ("fp64", "mixed", "learned"):
+        s = P.HipSparseMatrixSolver(0, tolerance=args.tol, maxIters=100000, mode=mode,
+                                    innerTolerance=args.inner_tol, blockRows=args.block_rows)
+
+        def step():
+            s.assemble_laplacian_device(mesh, 3, gamma_face=d_g, bc_type=d_bc, phi_face=d_pf)
+            s.sub_rhs_device(d_b)
+            s.zero_guess()
+            return s.solve(raise_on_noconv=False)
+
+        info = None
+        if mode == "learned":
+            every = max(1, args.capture_every // 4)
+            per_solve = max(1, out["fp64"]["iters"] // every)
+            a2 = argparse.Namespace(**vars(args))
+            a2.capture_every = every
+            info = train_network(s, mesh, d_g, d_bc, d_pf, a2, max_pairs=args.c4_pairs,
+                                 capture_solves=max(1, -(-args.c4_pairs // per_solve)))
+        step()
+        t0 = time.perf_counter()
+        rel = step()
+        dt = time.perf_counter() - t0
+        st = s.stats()
+        out[mode] = dict(ms_per_step=round(1e3 * dt, 2), relres=float("%.4g" % rel), iters=st["inner_iters"],
+                         fallbacks=st["fallbacks"])
+        if info is not None:
+            out[mode].update(info)
+        log("sigma2 %s: %.2f ms/step, relres %.2e, %d iterations" % (mode, 1e3 * dt, rel, st["inner_iters"]))
+        s.close()
+    return out
 
 
 def bubble_case(args):
@@ -252,6 +306,8 @@ def main():
     ap.add_argument("--simple-iters", type=int, default=200, help="C1 SIMPLE outer iterations (0 skips)")
     ap.add_argument("--bubble-cells", type=int, default=2_000_000, help="C3 bubble pressure solve (0 skips)")
     ap.add_argument("--copy-probe-mb", type=int, default=0, help="run one copy kernel of this size (counter scale)")
+    ap.add_argument("--c4-cells", type=int, default=0, help="learned case, sigma 2, at this many cells (0 skips)")
+    ap.add_argument("--c4-pairs", type=int, default=2000, help="training pairs for the learned case")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported (the path would shard by "
@@ -354,7 +410,8 @@ def main():
         "modes": {k: {kk: clean(vv) for kk, vv in v.items()} for k, v in results.items()},
         "speedup_vs_plain_fp64": (round(results["fp64"]["ms_per_step"] / results[head]["ms_per_step"], 3)
                                   if "fp64" in results else None),
-        "spmv_gbs": {"fp64": round(gbs["spmv64"], 1), "fp32": round(gbs["spmv32"], 1)},
+        "spmv_gbs": {"fp64": round(gbs["spmv64"], 1), "fp32": round(gbs["spmv32"], 1),
+                     "fp64_cold": round(gbs["spmv64_cold"], 1), "fp32_cold": round(gbs["spmv32_cold"], 1)},
         "kernel_ms": {k: round(v, 5) for k, v in kernels.items()},
         "kernel_gbs": {k: round(v, 1) for k, v in gbs.items()},
         "ordering": ordinfo,
@@ -364,6 +421,8 @@ def main():
         out["simple_c1"] = simple_cavity(args)
     if args.bubble_cells > 0:
         out["bubble_c3"] = bubble_case(args)
+    if args.c4_cells > 0:
+        out["learned_c4"] = learned_case(args)
     if not args.no_cpu_baseline:
         it = results.get("fp64", results[head])["iters"]
         out["cpu_baseline"] = cpu_baseline(xy, cptr, cind, bc, g, b, it)

@@ -164,7 +164,9 @@ int phase_hip_solver_ilu_export(phase_hip_solver *s, int *nnz, int *perm, int *r
  * 5 backward FP32, 6 ILU(0) factorisation FP64 (value gather + factor),
  * 7 factorisation FP32, 8 one backward-sweep launch (kUSolve, FP64; mean over
  * the colours a solve launches it for), 9 the fused last-colour launch
- * (kLUSolve, FP64), 10 one forward-sweep launch (kLSolve, FP64, mean as 8).
+ * (kLUSolve, FP64), 10 one forward-sweep launch (kLSolve, FP64, mean as 8),
+ * 11 / 12 the FP64 / FP32 SpMV cache-cold (a 1 GiB copy before each launch
+ * evicts L2 and the last-level cache; the events bracket the one launch).
  * Benchmark support; no counterpart in the reference. */
 int phase_hip_solver_time_kernels(phase_hip_solver *s, int reps, double *ms, int nms);
 

@@ -49,7 +49,7 @@ int guarded(phase_hip_solver *s, F &&f) {
 void setDevice(phase_hip_solver *s) { PH_CHECK(hipSetDevice(s->device)); }
 
 // entries of phase_hip_solver_time_kernels / phase_hip_solver_kernel_bytes
-constexpr int kTimedKernels = 11;
+constexpr int kTimedKernels = 13;
 
 template <class T>
 IluDev<T> iluDev(phase_hip_solver *s, T *fac) {
@@ -165,6 +165,7 @@ void launchFactor(phase_hip_solver *s, const IluDev<T> &L) {
 }
 
 int spmvGrid(phase_hip_solver *s) { return std::max(1, (s->ord.nSlices + 3) / 4); }
+
 
 void checkDeviceErr(phase_hip_solver *s) {
   int e = 0;
@@ -1053,6 +1054,8 @@ void kernelBytes(const Ordering &o, double *out) {
   out[5] = sweep(0, o.nBlocks, 4, true);
   out[6] = factor(8);
   out[7] = factor(4);
+  out[11] = out[0];
+  out[12] = out[1];
   out[8] = out[9] = out[10] = 0.;
   if (C >= 2) {
     out[8] = sweep(0, o.colourPtr[C - 1], 8, true) / (C - 1);
@@ -1150,6 +1153,39 @@ int phase_hip_solver_time_kernels(phase_hip_solver *s, int reps, double *ms, int
                            syncDev(s), colourCtl(cp[C - 1]), noSc, in, (const double *)y, (double *)nullptr,
                            (const double *)nullptr, (const double *)nullptr, x);
       });
+    // cache-cold SpMV: before each timed launch a 1 GiB copy evicts the matrix
+    // and vectors from L2 and from the 256 MB last-level cache, and the events
+    // bracket the one launch
+    {
+      const size_t flushBytes = (size_t)1 << 30;
+      void *flush = nullptr;
+      PH_CHECK(hipMalloc(&flush, flushBytes));
+      const size_t n16 = flushBytes / 32;
+      auto cold = [&](auto &&launch) {
+        double tot = 0.;
+        for (int r = 0; r < reps; ++r) {
+          hipLaunchKernelGGL(kCopy16, dim3(8192), dim3(256), 0, s->stream, n16, (const uint4 *)flush,
+                             (uint4 *)flush + n16);
+          PH_CHECK(hipEventRecord(s->ev0, s->stream));
+          launch();
+          PH_CHECK(hipEventRecord(s->ev1, s->stream));
+          PH_CHECK(hipEventSynchronize(s->ev1));
+          float t = 0.f;
+          PH_CHECK(hipEventElapsedTime(&t, s->ev0, s->ev1));
+          tot += t;
+        }
+        return tot / reps;
+      };
+      out[11] = cold([&] {
+        hipLaunchKernelGGL((kSpmv<double>), dim3(spmvGrid(s)), dim3(kRedBlock), 0, s->stream, n, s->ord.nSlices,
+                         s->dSliceOff.p, s->dSellCol.p, s->dSell64.p, s->k64.ph.p, s->k64.t.p);
+      });
+      out[12] = cold([&] {
+        hipLaunchKernelGGL((kSpmv<float>), dim3(spmvGrid(s)), dim3(kRedBlock), 0, s->stream, n, s->ord.nSlices,
+                         s->dSliceOff.p, s->dSellCol.p, s->dSell32.p, s->k32.ph.p, s->k32.t.p);
+      });
+      PH_CHECK(hipFree(flush));
+    }
     checkDeviceErr(s);
     for (int i = 0; i < std::min(nms, kTimedKernels); ++i) ms[i] = out[i];
   });
