@@ -333,8 +333,11 @@ def main():
     kernels = None
     nnz = None
     for mode in args.modes.split(","):
-        s = P.HipSparseMatrixSolver(0, tolerance=args.tol, maxIters=50000, mode=mode,
-                                    innerTolerance=args.inner_tol, sweep=args.sweep, blockRows=args.block_rows)
+        # "<precision>" runs BiCGStab, "<precision>-cg" preconditioned CG
+        prec, _, kry = mode.partition("-")
+        s = P.HipSparseMatrixSolver(0, tolerance=args.tol, maxIters=50000, mode=prec,
+                                    innerTolerance=args.inner_tol, sweep=args.sweep, blockRows=args.block_rows,
+                                    solver="cg" if kry == "cg" else "bicgstab")
 
         def step():
             s.assemble_laplacian_device(mesh, 3, gamma_face=d_g, bc_type=d_bc, phi_face=d_pf)
@@ -343,7 +346,7 @@ def main():
             return s.solve(raise_on_noconv=False)
 
         train_info = None
-        if mode == "learned":
+        if prec == "learned":
             train_info = train_network(s, mesh, d_g, d_bc, d_pf, args)
         for _ in range(args.warmup):
             step()
@@ -403,7 +406,7 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if head == "fp64" else "f32-inner/f64-refine",
+        "dtype": "f64" if head.partition("-")[0] == "fp64" else "f32-inner/f64-refine",
         "data": "synthetic",
         "config": {"workload": "pcorr-poisson-delaunay-1M", "cells": n, "faces": mesh.n_faces, "nnz": nnz,
                    "sigma": args.sigma, "tol": args.tol, "method": head},

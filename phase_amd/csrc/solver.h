@@ -39,6 +39,7 @@ struct phase_hip_solver {
   int maxOuter = 100;
   int chunk = 8;  // iterations enqueued between host checks
   int syncFree = 0;  // ILU sweeps: 0 = one launch per block colour, 1 = one sync-free launch
+  int cg = 0;        // Krylov method: 0 = BiCGStab, 1 = preconditioned CG (symmetric systems)
 
   // pattern and ordering
   int rank = 0;

@@ -239,6 +239,20 @@ void iluApply(phase_hip_solver *s, DBuf<T> &fac, const T *in, T *tmp, T *out, co
   launchLU<T, kProNone>(s, L, sc, in, tmp, (T *)nullptr, (const T *)nullptr, (const T *)nullptr, out);
 }
 
+// z = M^{-1} r ; rho' = (r, z), beta = rho' / rho ; p = z + beta p  (the CG
+// direction update; with p = 0 it starts the recurrence)
+template <class T>
+void cgDirection(phase_hip_solver *s, const IluDev<T> &L, KrylovVecs<T> &K, Scalars *sc) {
+  const int n = s->rank;
+  launchLU<T, kProNone>(s, L, sc, (const T *)K.r.p, K.y.p, (T *)nullptr, (const T *)nullptr, (const T *)nullptr,
+                        K.ph.p);
+  const int gu = gridFor(n, kRedBlock, kRedCap);
+  hipLaunchKernelGGL((kDotRZ<T>), dim3(gu), dim3(kRedBlock), 0, s->stream, n, sc, K.r.p, K.ph.p, s->dPartials.p);
+  hipLaunchKernelGGL(kFinishCgZ, dim3(1), dim3(kRedBlock), 0, s->stream, sc, s->dPartials.p, gu);
+  hipLaunchKernelGGL((kCgDirection<T>), dim3(gridFor(n, 256, kElemCap)), dim3(256), 0, s->stream, n, sc, K.ph.p,
+                     K.p.p);
+}
+
 // One right-preconditioned BiCGStab iteration, all scalars on the device.
 template <class T, class TX, class TV>
 void iterate(phase_hip_solver *s, KrylovVecs<T> &K, DBuf<T> &fac, const TV *sell, TX *x, Scalars *sc) {
@@ -262,6 +276,22 @@ void iterate(phase_hip_solver *s, KrylovVecs<T> &K, DBuf<T> &fac, const TV *sell
   hipLaunchKernelGGL((kUpdate<T, TX>), dim3(gu), dim3(kRedBlock), 0, s->stream, n, sc, x, K.ph.p, K.sh.p, K.r.p,
                      K.s.p, K.t.p, K.r0.p, s->dPartials.p);
   hipLaunchKernelGGL(kFinishUpdate, dim3(1), dim3(kRedBlock), 0, s->stream, sc, s->dPartials.p, gu);
+}
+
+// One preconditioned CG iteration: q = A p, alpha; x, r; z = M^{-1} r; beta; p.
+template <class T, class TX, class TV>
+void iterateCg(phase_hip_solver *s, KrylovVecs<T> &K, DBuf<T> &fac, const TV *sell, TX *x, Scalars *sc) {
+  const int n = s->rank;
+  IluDev<T> L = iluDev<T>(s, fac.p);
+  const int gs = spmvGrid(s);
+  hipLaunchKernelGGL((kSpmvDotV<T, TV>), dim3(gs), dim3(kRedBlock), 0, s->stream, n, s->ord.nSlices, s->dSliceOff.p,
+                     s->dSellCol.p, sell, K.p.p, K.v.p, K.p.p, sc, s->dPartials.p);
+  hipLaunchKernelGGL(kFinishV, dim3(1), dim3(kRedBlock), 0, s->stream, sc, s->dPartials.p, gs);
+  const int gu = gridFor(n, kRedBlock, kRedCap);
+  hipLaunchKernelGGL((kCgUpdate<T, TX>), dim3(gu), dim3(kRedBlock), 0, s->stream, n, sc, x, K.p.p, K.r.p, K.v.p,
+                     s->dPartials.p);
+  hipLaunchKernelGGL(kFinishCgR, dim3(1), dim3(kRedBlock), 0, s->stream, sc, s->dPartials.p, gu);
+  cgDirection<T>(s, L, K, sc);
 }
 
 void pullScalars(phase_hip_solver *s) {
@@ -288,7 +318,10 @@ int runKrylov(phase_hip_solver *s, KrylovVecs<T> &K, DBuf<T> &fac, const TV *sel
   Scalars *sc = s->dSc.p;  // slot 0
   int it = 0;
   while (true) {
-    for (int c = 0; c < s->chunk; ++c) iterate<T, TX, TV>(s, K, fac, sell, x, sc);
+    for (int c = 0; c < s->chunk; ++c) {
+      if (s->cg) iterateCg<T, TX, TV>(s, K, fac, sell, x, sc);
+      else iterate<T, TX, TV>(s, K, fac, sell, x, sc);
+    }
     PH_CHECK(hipGetLastError());
     pullScalars(s);
     it = s->hSc[0].iters;
@@ -318,6 +351,7 @@ void fp64Loop(phase_hip_solver *s, double b2, int total) {
                        K.r0.p, sc, s->dPartials.p, s->dCounter.p);
     hipLaunchKernelGGL((kBicgInit<double>), dim3(gridFor(n, 256, kElemCap)), dim3(256), 0, s->stream, n, sc,
                        K.p.p, K.v.p, (double *)nullptr, tol2, s->maxIters - total, 0.0);
+    if (s->cg) cgDirection<double>(s, iluDev<double>(s, s->dFac64.p), K, sc);
     pullScalars(s);
     s->error = std::sqrt(s->hSc[0].rr / b2);
     ++s->outerPasses;
@@ -451,6 +485,7 @@ void solveMixed(phase_hip_solver *s) {
                        n, K.r.p, &inner->rr, s->dPartials.p, s->dCounter.p);
     hipLaunchKernelGGL((kBicgInit<float>), dim3(gridFor(n, 256, kElemCap)), dim3(256), 0, s->stream, n, inner,
                        K.p.p, K.v.p, (float *)nullptr, tol2, s->maxIters - total, 0.0);
+    if (s->cg) cgDirection<float>(s, iluDev<float>(s, s->dFac32.p), K, inner);
     pullScalars(s);
     double lastRU = s->hSc[0].rr;
     int it = runKrylov<float, double, double>(s, K, s->dFac32, s->dSell64.p, s->dX.p, s->maxIters - total,
@@ -718,7 +753,11 @@ int phase_hip_solver_setup(phase_hip_solver *s, const char *key, const char *val
     } else if (k == "fallbackRatio") s->fallbackRatio = std::stod(v);
     else if (k == "maxOuter") s->maxOuter = std::stoi(v);
     else if (k == "chunk") s->chunk = std::max(1, std::stoi(v));
-    else if (k == "sweep") {
+    else if (k == "solver") {
+      if (lv == "bicgstab") s->cg = 0;
+      else if (lv == "cg") s->cg = 1;
+      else fail(PHASE_HIP_EINVAL, "unknown solver '" + v + "' (BICGSTAB or CG)");
+    } else if (k == "sweep") {
       if (lv == "colour" || lv == "color" || lv == "0") s->syncFree = 0;
       else if (lv == "syncfree" || lv == "1") s->syncFree = 1;
       else fail(PHASE_HIP_EINVAL, "unknown sweep '" + v + "'");

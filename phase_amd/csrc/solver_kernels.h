@@ -456,6 +456,85 @@ static __global__ __launch_bounds__(kRedBlock) void kFinishUpdate(Scalars *sc, c
   else if (sc->iters >= sc->maxIters) sc->done = kDoneMaxIters;
 }
 
+// ---------------------------------------------------------------- CG vector steps
+// Preconditioned conjugate gradients (Hestenes-Stiefel, with M = ILU(0)) for
+// symmetric systems, on the same sweeps and SpMV-dot kernel: q = A p and
+// (p, q) come from kSpmvDotV(p, q, p) and kFinishV sets alpha = rho / (p, q).
+
+// x += alpha p ; r -= alpha q ; partials of (r, r)
+template <class T, class TX>
+__global__ __launch_bounds__(kRedBlock) void kCgUpdate(int n, const Scalars *sc, TX *__restrict__ x,
+                                                       const T *__restrict__ p, T *__restrict__ r,
+                                                       const T *__restrict__ q, double *partials) {
+  if (sc->done) return;
+  const double alpha = sc->alpha;
+  if (!isfinite(alpha)) return;  // kFinishCgR flags the breakdown
+  const T al = (T)alpha;
+  const TX alx = (TX)alpha;
+  double d[1] = {0.};
+  for (int i = blockIdx.x * kRedBlock + threadIdx.x; i < n; i += gridDim.x * kRedBlock) {
+    x[i] = x[i] + alx * (TX)p[i];
+    const T ri = r[i] - al * q[i];
+    r[i] = ri;
+    d[0] += (double)ri * (double)ri;
+  }
+  blockPartials<1>(d, partials, blockIdx.x);
+}
+
+// rr = (r, r) and the stopping tests
+static __global__ __launch_bounds__(kRedBlock) void kFinishCgR(Scalars *sc, const double *partials, int nb) {
+  if (sc->done) return;
+  double d[1];
+  finishPartials<1>(partials, nb, d);
+  if (threadIdx.x != 0) return;
+  if (!isfinite(sc->alpha)) {
+    sc->done = kDoneBreakdown;
+    return;
+  }
+  const double rr = d[0];
+  sc->rrPrev = sc->rr;
+  sc->rr = rr;
+  sc->iters += 1;
+  if (!isfinite(rr)) sc->done = kDoneBreakdown;
+  else if (rr <= sc->tol2) sc->done = kDoneConverged;
+  else if (sc->minRatio > 0. && rr > sc->minRatio * sc->minRatio * sc->rrPrev) sc->done = kDoneStagnation;
+  else if (sc->iters >= sc->maxIters) sc->done = kDoneMaxIters;
+}
+
+// partials of (r, z)
+template <class T>
+__global__ __launch_bounds__(kRedBlock) void kDotRZ(int n, const Scalars *sc, const T *__restrict__ r,
+                                                    const T *__restrict__ z, double *partials) {
+  if (sc->done) return;
+  double d[1] = {0.};
+  for (int i = blockIdx.x * kRedBlock + threadIdx.x; i < n; i += gridDim.x * kRedBlock)
+    d[0] += (double)r[i] * (double)z[i];
+  blockPartials<1>(d, partials, blockIdx.x);
+}
+
+// rho' = (r, z), beta = rho' / rho
+static __global__ __launch_bounds__(kRedBlock) void kFinishCgZ(Scalars *sc, const double *partials, int nb) {
+  if (sc->done) return;
+  double d[1];
+  finishPartials<1>(partials, nb, d);
+  if (threadIdx.x != 0) return;
+  const double rhoNew = d[0];
+  if (!isfinite(rhoNew) || rhoNew == 0. || sc->rho == 0.) {
+    sc->done = kDoneBreakdown;
+    return;
+  }
+  sc->beta = rhoNew / sc->rho;
+  sc->rho = rhoNew;
+}
+
+// p = z + beta p
+template <class T>
+__global__ void kCgDirection(int n, const Scalars *sc, const T *__restrict__ z, T *__restrict__ p) {
+  if (sc->done) return;
+  const T beta = (T)sc->beta;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = z[i] + beta * p[i];
+}
+
 // r32 = r64 (reliable update of the recursive residual); inner->rr = outer->rr
 template <class T>
 __global__ void kReplaceResidual(int n, const double *__restrict__ r64, T *__restrict__ r, Scalars *inner,

@@ -64,12 +64,15 @@ def test_export_roundtrip(hip, tri_case):
     assert np.array_equal(erhs, -b)
 
 
+@pytest.mark.parametrize("solver", ["bicgstab", "cg"])
 @pytest.mark.parametrize("mode", ["fp64", "mixed"])
-def test_solve_matches_direct(hip, tri_case, mode):
+def test_solve_matches_direct(hip, tri_case, mode, solver):
+    """BiCGStab (the reference's default Belos solver) and preconditioned CG
+    (Belos "CG", for the symmetric pressure matrix) against a direct solve."""
     rp, ci, v, b = tri_case
     A = H.compact_sorted(rp, ci, v).tocsc()
     x_ref = spla.splu(A).solve(b)
-    s = _solver(hip, mode=mode, tolerance=1e-8, maxIters=5000)
+    s = _solver(hip, mode=mode, tolerance=1e-8, maxIters=5000, solver=solver)
     s.set(rp, ci, v)
     s.set_rhs(b)
     rel = s.solve()
@@ -126,6 +129,11 @@ def test_noconvergence_reported(hip, tri_case):
     assert s.n_iters() <= 3 + 8
 
 
+def test_unknown_solver_rejected(hip):
+    with pytest.raises(hip.PhaseHipError):
+        _solver(hip, solver="gmres")
+
+
 def test_bad_pattern_rejected(hip):
     s = _solver(hip)
     with pytest.raises(hip.PhaseHipError):
@@ -155,8 +163,8 @@ def test_variable_density_pressure(hip, oracle):
     rp, ci, v, r = e.get()
     A = H.compact_sorted(rp, ci, v)
     xs = spla.spsolve(A.tocsc(), -r)
-    for mode in ("fp64", "mixed"):
-        s = hip.HipSparseMatrixSolver(0, tolerance=1e-8, maxIters=20000, mode=mode)
+    for mode, solver in (("fp64", "bicgstab"), ("mixed", "bicgstab"), ("fp64", "cg"), ("mixed", "cg")):
+        s = hip.HipSparseMatrixSolver(0, tolerance=1e-8, maxIters=20000, mode=mode, solver=solver)
         s.assemble_laplacian_device(hm, 3, gamma_face=hip.DeviceArray(gamma), bc_type=hip.DeviceArray(bc),
                                     phi_face=hip.DeviceArray(np.zeros(om.n_faces)))
         s.sub_rhs_device(hip.DeviceArray(rhs))
