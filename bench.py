@@ -11,9 +11,12 @@ edge, zero-gradient elsewhere.  Each step assembles fv::laplacian(gamma_f, p)
 FP64 true relative residual of 1e-8 from a zero initial guess.
 
 Prints ONE JSON line.  `value` is ms per step of the fastest method that
-reaches the FP64 goal (plain FP64 BiCGStab+ILU(0), or "mixed": FP32
-BiCGStab+ILU(0) in FP64 iterative refinement); every method's time is reported
-beside it, as are the SpMV bandwidths and the per-launch kernel timings.
+reaches the FP64 goal: plain FP64 BiCGStab+ILU(0) ("fp64"), FP32 BiCGStab+ILU(0)
+in FP64 iterative refinement ("mixed"), either with the learned correction
+("learned"), or the same with preconditioned CG in place of BiCGStab ("-cg":
+Belos "CG", valid because the pressure matrix is symmetric).  Every method's
+time is reported beside it, as are the SpMV bandwidths and the per-launch
+kernel timings.
 `roofline.traffic` is the HBM bytes per launch of the dominant kernel read from
 the committed rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, each scaled by
 the same counter on a copy kernel of known size; `--copy-probe-mb` runs that
@@ -296,7 +299,8 @@ def main():
     ap.add_argument("--inner-tol", type=float, default=0.1)
     ap.add_argument("--sweep", default="colour")
     ap.add_argument("--block-rows", type=int, default=32, help="rows per ILU aggregate (blocks pack aggregates to 64)")
-    ap.add_argument("--modes", default="fp64,mixed,learned")
+    ap.add_argument("--modes", default="fp64,fp64-cg,mixed,mixed-cg,learned",
+                    help="precision fp64|mixed|learned, with -cg for preconditioned CG instead of BiCGStab")
     ap.add_argument("--capture-solves", type=int, default=4)
     ap.add_argument("--capture-every", type=int, default=100)
     ap.add_argument("--train-steps", type=int, default=600)
@@ -337,7 +341,7 @@ def main():
         prec, _, kry = mode.partition("-")
         s = P.HipSparseMatrixSolver(0, tolerance=args.tol, maxIters=50000, mode=prec,
                                     innerTolerance=args.inner_tol, sweep=args.sweep, blockRows=args.block_rows,
-                                    solver="cg" if kry == "cg" else "bicgstab")
+                                    solver=kry or "bicgstab")
 
         def step():
             s.assemble_laplacian_device(mesh, 3, gamma_face=d_g, bc_type=d_bc, phi_face=d_pf)

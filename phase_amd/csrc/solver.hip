@@ -240,15 +240,18 @@ void iluApply(phase_hip_solver *s, DBuf<T> &fac, const T *in, T *tmp, T *out, co
 }
 
 // z = M^{-1} r ; rho' = (r, z), beta = rho' / rho ; p = z + beta p  (the CG
-// direction update; with p = 0 it starts the recurrence)
+// direction update).  start: the first direction (p is zero); otherwise the
+// (r, z) partials follow kCgUpdate's (r, r) partials and kFinishCg judges both.
 template <class T>
-void cgDirection(phase_hip_solver *s, const IluDev<T> &L, KrylovVecs<T> &K, Scalars *sc) {
+void cgDirection(phase_hip_solver *s, const IluDev<T> &L, KrylovVecs<T> &K, Scalars *sc, bool start) {
   const int n = s->rank;
   launchLU<T, kProNone>(s, L, sc, (const T *)K.r.p, K.y.p, (T *)nullptr, (const T *)nullptr, (const T *)nullptr,
                         K.ph.p);
   const int gu = gridFor(n, kRedBlock, kRedCap);
-  hipLaunchKernelGGL((kDotRZ<T>), dim3(gu), dim3(kRedBlock), 0, s->stream, n, sc, K.r.p, K.ph.p, s->dPartials.p);
-  hipLaunchKernelGGL(kFinishCgZ, dim3(1), dim3(kRedBlock), 0, s->stream, sc, s->dPartials.p, gu);
+  hipLaunchKernelGGL((kDotRZ<T>), dim3(gu), dim3(kRedBlock), 0, s->stream, n, sc, K.r.p, K.ph.p,
+                     s->dPartials.p + (start ? 0 : gu));
+  if (start) hipLaunchKernelGGL(kFinishCgZ, dim3(1), dim3(kRedBlock), 0, s->stream, sc, s->dPartials.p, gu);
+  else hipLaunchKernelGGL(kFinishCg, dim3(1), dim3(kRedBlock), 0, s->stream, sc, s->dPartials.p, gu);
   hipLaunchKernelGGL((kCgDirection<T>), dim3(gridFor(n, 256, kElemCap)), dim3(256), 0, s->stream, n, sc, K.ph.p,
                      K.p.p);
 }
@@ -278,7 +281,8 @@ void iterate(phase_hip_solver *s, KrylovVecs<T> &K, DBuf<T> &fac, const TV *sell
   hipLaunchKernelGGL(kFinishUpdate, dim3(1), dim3(kRedBlock), 0, s->stream, sc, s->dPartials.p, gu);
 }
 
-// One preconditioned CG iteration: q = A p, alpha; x, r; z = M^{-1} r; beta; p.
+// One preconditioned CG iteration: q = A p, alpha; x, r; z = M^{-1} r; the
+// stopping tests and beta; p.
 template <class T, class TX, class TV>
 void iterateCg(phase_hip_solver *s, KrylovVecs<T> &K, DBuf<T> &fac, const TV *sell, TX *x, Scalars *sc) {
   const int n = s->rank;
@@ -290,8 +294,7 @@ void iterateCg(phase_hip_solver *s, KrylovVecs<T> &K, DBuf<T> &fac, const TV *se
   const int gu = gridFor(n, kRedBlock, kRedCap);
   hipLaunchKernelGGL((kCgUpdate<T, TX>), dim3(gu), dim3(kRedBlock), 0, s->stream, n, sc, x, K.p.p, K.r.p, K.v.p,
                      s->dPartials.p);
-  hipLaunchKernelGGL(kFinishCgR, dim3(1), dim3(kRedBlock), 0, s->stream, sc, s->dPartials.p, gu);
-  cgDirection<T>(s, L, K, sc);
+  cgDirection<T>(s, L, K, sc, false);
 }
 
 void pullScalars(phase_hip_solver *s) {
@@ -351,7 +354,7 @@ void fp64Loop(phase_hip_solver *s, double b2, int total) {
                        K.r0.p, sc, s->dPartials.p, s->dCounter.p);
     hipLaunchKernelGGL((kBicgInit<double>), dim3(gridFor(n, 256, kElemCap)), dim3(256), 0, s->stream, n, sc,
                        K.p.p, K.v.p, (double *)nullptr, tol2, s->maxIters - total, 0.0);
-    if (s->cg) cgDirection<double>(s, iluDev<double>(s, s->dFac64.p), K, sc);
+    if (s->cg) cgDirection<double>(s, iluDev<double>(s, s->dFac64.p), K, sc, true);
     pullScalars(s);
     s->error = std::sqrt(s->hSc[0].rr / b2);
     ++s->outerPasses;
@@ -485,7 +488,7 @@ void solveMixed(phase_hip_solver *s) {
                        n, K.r.p, &inner->rr, s->dPartials.p, s->dCounter.p);
     hipLaunchKernelGGL((kBicgInit<float>), dim3(gridFor(n, 256, kElemCap)), dim3(256), 0, s->stream, n, inner,
                        K.p.p, K.v.p, (float *)nullptr, tol2, s->maxIters - total, 0.0);
-    if (s->cg) cgDirection<float>(s, iluDev<float>(s, s->dFac32.p), K, inner);
+    if (s->cg) cgDirection<float>(s, iluDev<float>(s, s->dFac32.p), K, inner, true);
     pullScalars(s);
     double lastRU = s->hSc[0].rr;
     int it = runKrylov<float, double, double>(s, K, s->dFac32, s->dSell64.p, s->dX.p, s->maxIters - total,

@@ -462,13 +462,14 @@ static __global__ __launch_bounds__(kRedBlock) void kFinishUpdate(Scalars *sc, c
 // (p, q) come from kSpmvDotV(p, q, p) and kFinishV sets alpha = rho / (p, q).
 
 // x += alpha p ; r -= alpha q ; partials of (r, r)
+// (kFinishCg tests them together with those of the next (r, z))
 template <class T, class TX>
 __global__ __launch_bounds__(kRedBlock) void kCgUpdate(int n, const Scalars *sc, TX *__restrict__ x,
                                                        const T *__restrict__ p, T *__restrict__ r,
                                                        const T *__restrict__ q, double *partials) {
   if (sc->done) return;
   const double alpha = sc->alpha;
-  if (!isfinite(alpha)) return;  // kFinishCgR flags the breakdown
+  if (!isfinite(alpha)) return;  // kFinishCg flags the breakdown
   const T al = (T)alpha;
   const TX alx = (TX)alpha;
   double d[1] = {0.};
@@ -481,24 +482,32 @@ __global__ __launch_bounds__(kRedBlock) void kCgUpdate(int n, const Scalars *sc,
   blockPartials<1>(d, partials, blockIdx.x);
 }
 
-// rr = (r, r) and the stopping tests
-static __global__ __launch_bounds__(kRedBlock) void kFinishCgR(Scalars *sc, const double *partials, int nb) {
+// rr = (r, r) (partials of kCgUpdate, first nb) and rho' = (r, z) (partials of
+// kDotRZ, next nb): stopping tests, then beta = rho' / rho.  The test on rr
+// comes one preconditioner apply after the update that produced r; x is final
+// at that point and the extra apply is discarded.
+static __global__ __launch_bounds__(kRedBlock) void kFinishCg(Scalars *sc, const double *partials, int nb) {
   if (sc->done) return;
-  double d[1];
-  finishPartials<1>(partials, nb, d);
+  double d[2];
+  finishPartials<2>(partials, nb, d);
   if (threadIdx.x != 0) return;
   if (!isfinite(sc->alpha)) {
     sc->done = kDoneBreakdown;
     return;
   }
-  const double rr = d[0];
+  const double rr = d[0], rhoNew = d[1];
   sc->rrPrev = sc->rr;
   sc->rr = rr;
   sc->iters += 1;
   if (!isfinite(rr)) sc->done = kDoneBreakdown;
   else if (rr <= sc->tol2) sc->done = kDoneConverged;
-  else if (sc->minRatio > 0. && rr > sc->minRatio * sc->minRatio * sc->rrPrev) sc->done = kDoneStagnation;
-  else if (sc->iters >= sc->maxIters) sc->done = kDoneMaxIters;
+  else if (!isfinite(rhoNew) || rhoNew == 0. || sc->rho == 0.) sc->done = kDoneBreakdown;
+  else {
+    sc->beta = rhoNew / sc->rho;
+    sc->rho = rhoNew;
+    if (sc->minRatio > 0. && rr > sc->minRatio * sc->minRatio * sc->rrPrev) sc->done = kDoneStagnation;
+    else if (sc->iters >= sc->maxIters) sc->done = kDoneMaxIters;
+  }
 }
 
 // partials of (r, z)
@@ -512,7 +521,7 @@ __global__ __launch_bounds__(kRedBlock) void kDotRZ(int n, const Scalars *sc, co
   blockPartials<1>(d, partials, blockIdx.x);
 }
 
-// rho' = (r, z), beta = rho' / rho
+// start of the recurrence: rho' = (r, z), beta = rho' / rho (p is zero)
 static __global__ __launch_bounds__(kRedBlock) void kFinishCgZ(Scalars *sc, const double *partials, int nb) {
   if (sc->done) return;
   double d[1];
