@@ -299,6 +299,7 @@ def main():
     ap.add_argument("--inner-tol", type=float, default=0.1)
     ap.add_argument("--sweep", default="colour")
     ap.add_argument("--block-rows", type=int, default=32, help="rows per ILU aggregate (blocks pack aggregates to 64)")
+    ap.add_argument("--ordering", default="geometric", help="ILU aggregates: geometric (bricks) or graph")
     ap.add_argument("--modes", default="fp64,fp64-cg,mixed,mixed-cg,learned",
                     help="precision fp64|mixed|learned, with -cg for preconditioned CG instead of BiCGStab")
     ap.add_argument("--capture-solves", type=int, default=4)
@@ -341,7 +342,7 @@ def main():
         prec, _, kry = mode.partition("-")
         s = P.HipSparseMatrixSolver(0, tolerance=args.tol, maxIters=50000, mode=prec,
                                     innerTolerance=args.inner_tol, sweep=args.sweep, blockRows=args.block_rows,
-                                    solver=kry or "bicgstab")
+                                    solver=kry or "bicgstab", ordering=args.ordering)
 
         def step():
             s.assemble_laplacian_device(mesh, 3, gamma_face=d_g, bc_type=d_bc, phi_face=d_pf)

@@ -497,7 +497,7 @@ namespace phase {
 // Solver side of FiniteVolumeEquation<T>::solve (FiniteVolumeEquation.tpp:64-86):
 // A from the equation's rows, b = -rhs.  Defined in solver.hip.
 void solverSetFromRows(phase_hip_solver *s, int rank, int stride, const int *dCols, const double *dVals,
-                       const double *dRhs, hipStream_t producer);
+                       const double *dRhs, hipStream_t producer, const double *dCellXY);
 
 }  // namespace phase
 
@@ -693,7 +693,9 @@ int phase_hip_eq_export(phase_hip_equation *e, int *nslots, int *rowPtr, int *co
 int phase_hip_solver_set_equation(phase_hip_solver *s, const phase_hip_equation *e) {
   if (!s || !e) return PHASE_HIP_EINVAL;
   try {
-    solverSetFromRows(s, e->rank, e->stride, e->cols.p, e->vals.p, e->rhs.p, e->mesh->stream);
+    // scalar equations carry one row per cell: their centroids order the rows
+    solverSetFromRows(s, e->rank, e->stride, e->cols.p, e->vals.p, e->rhs.p, e->mesh->stream,
+                      e->ncomp == 1 ? e->mesh->dCC.p : nullptr);
     return PHASE_HIP_OK;
   } catch (const HipError &x) {
     s->err = x.msg;

@@ -459,7 +459,12 @@ void bindMesh(phase_hip_solver *s, phase_hip_mesh *m) {
   auto *b = new MeshBinding();
   buildBinding(m, *b);
   s->rank = 0;
-  solverSetPattern(s, m->nCells, b->rowPtr.data(), b->colInd.data());
+  std::vector<double> xy;
+  if (s->geometric) {
+    xy.resize(2 * (size_t)m->nCells);
+    PH_CHECK(hipMemcpy(xy.data(), m->dCC.p, xy.size() * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  solverSetPattern(s, m->nCells, b->rowPtr.data(), b->colInd.data(), xy.empty() ? nullptr : xy.data());
   b->dIslot.upload(b->islot.empty() ? std::vector<int>{0} : b->islot, s->stream);
   b->dIfirst.upload(b->ifirst.empty() ? std::vector<uint8_t>{0} : b->ifirst, s->stream);
   b->dDslot.upload(b->dslot, s->stream);

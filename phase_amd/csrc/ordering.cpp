@@ -20,6 +20,7 @@
 // levels while blocks wait for their (few) predecessor blocks.
 #include <algorithm>
 #include <climits>
+#include <cmath>
 #include <random>
 #include <cstring>
 #include <numeric>
@@ -258,7 +259,83 @@ static bool tabuRecolour(const std::vector<std::vector<int>> &adj, int k, std::v
   return f == 0;
 }
 
-void buildOrdering(Ordering &o, int n, const int *rowPtr, const int *colInd, int blockRows) {
+// Brick aggregates for a 2-D mesh whose cell centroids are known: rows fall
+// into rectangular bricks of about `target` rows laid as a running-bond wall
+// (every other row of bricks shifted by half a brick).  A brick then touches
+// two bricks of its own row and two of each neighbouring row, and colouring
+// brick (i, j) with (i + 2 (j mod 2)) mod 3 separates it from all six: three
+// colours, where a general colouring of breadth-first aggregates needs four or
+// more.  Fails (the caller falls back) when a brick would exceed one
+// wavefront of rows even after shrinking the bricks, or when a matrix entry
+// couples two bricks of one colour (cells not small against a brick).
+static bool brickAggregates(int n, const Graph &g, const double *xy, int target, const std::vector<int> &rpos,
+                            std::vector<int> &blk, std::vector<std::vector<int>> &members,
+                            std::vector<int> &colour) {
+  double x0 = xy[0], x1 = xy[0], y0 = xy[1], y1 = xy[1];
+  for (int i = 0; i < n; ++i) {
+    x0 = std::min(x0, xy[2 * i]);
+    x1 = std::max(x1, xy[2 * i]);
+    y0 = std::min(y0, xy[2 * i + 1]);
+    y1 = std::max(y1, xy[2 * i + 1]);
+  }
+  const double area = (x1 - x0) * (y1 - y0);
+  if (!(area > 0.) || n < 4 * target) return false;
+  std::vector<long long> key(n);
+  std::vector<int> bi(n), bj(n), bk(n);  // bk: brick of each row (blk is written only on success)
+  double side = std::sqrt(target * area / n);
+  for (int attempt = 0;; ++attempt) {
+    if (attempt == 6) return false;
+    long long ni = 0;
+    for (int r = 0; r < n; ++r) {
+      const int j = (int)std::floor((xy[2 * r + 1] - y0) / side);
+      const int i = (int)std::floor((xy[2 * r] - x0) / side - 0.5 * (j & 1)) + 1;
+      bi[r] = i;
+      bj[r] = j;
+      ni = std::max<long long>(ni, i + 1);
+    }
+    for (int r = 0; r < n; ++r) key[r] = (long long)bj[r] * (ni + 1) + bi[r];
+    std::vector<long long> uk(key);
+    std::sort(uk.begin(), uk.end());
+    uk.erase(std::unique(uk.begin(), uk.end()), uk.end());
+    const int nb = (int)uk.size();
+    std::vector<int> size(nb, 0);
+    for (int r = 0; r < n; ++r) {
+      bk[r] = (int)(std::lower_bound(uk.begin(), uk.end(), key[r]) - uk.begin());
+      ++size[bk[r]];
+    }
+    if (*std::max_element(size.begin(), size.end()) > kLanes) {
+      side *= 0.9;
+      continue;
+    }
+    std::vector<int> bcol(nb);
+    for (int r = 0; r < n; ++r) bcol[bk[r]] = (bi[r] + 2 * (bj[r] & 1)) % 3;
+    for (int u = 0; u < n; ++u)
+      for (int k = g.ptr[u]; k < g.ptr[u + 1]; ++k) {
+        const int w = g.adj[k];
+        if (bk[w] != bk[u] && bcol[bk[w]] == bcol[bk[u]]) return false;
+      }
+    // number the bricks by their first row in RCM order, as breadth-first
+    // aggregation would have created them
+    std::vector<int> first(nb, INT_MAX);
+    for (int r = 0; r < n; ++r) first[bk[r]] = std::min(first[bk[r]], rpos[r]);
+    std::vector<int> order(nb), renum(nb);
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(), [&](int a, int c) { return first[a] < first[c]; });
+    for (int k = 0; k < nb; ++k) renum[order[k]] = k;
+    members.assign(nb, {});
+    colour.assign(nb, 0);
+    for (int k = 0; k < nb; ++k) colour[k] = bcol[order[k]];
+    for (int r = 0; r < n; ++r) {
+      blk[r] = renum[bk[r]];
+      members[blk[r]].push_back(r);
+    }
+    for (auto &mem : members)
+      std::sort(mem.begin(), mem.end(), [&](int a, int c) { return rpos[a] < rpos[c]; });
+    return true;
+  }
+}
+
+void buildOrdering(Ordering &o, int n, const int *rowPtr, const int *colInd, int blockRows, const double *cellXY) {
   if (n <= 0) fail(PHASE_HIP_EINVAL, "rank must be positive");
   if (blockRows < 1 || blockRows > kLanes) fail(PHASE_HIP_EINVAL, "blockRows must be in [1,64]");
   o = Ordering();
@@ -284,86 +361,93 @@ void buildOrdering(Ordering &o, int n, const int *rowPtr, const int *colInd, int
   std::vector<int> rpos(n);
   for (int k = 0; k < n; ++k) rpos[rorder[k]] = k;
 
-  // 3. breadth-first aggregation into blocks of <= blockRows rows
   std::vector<int> blk(n, -1);
   std::vector<std::vector<int>> members;
-  std::vector<int> queue, nbs;
-  for (int s : rorder) {
-    if (blk[s] >= 0) continue;
-    int b = (int)members.size();
-    members.emplace_back();
-    auto &mem = members.back();
-    mem.push_back(s);
-    blk[s] = b;
-    queue.assign(1, s);
-    size_t head = 0;
-    while (head < queue.size() && (int)mem.size() < blockRows) {
-      int u = queue[head++];
-      nbs.clear();
-      for (int k = g.ptr[u]; k < g.ptr[u + 1]; ++k)
-        if (blk[g.adj[k]] < 0) nbs.push_back(g.adj[k]);
-      std::sort(nbs.begin(), nbs.end(), [&](int a, int c) { return rpos[a] < rpos[c]; });
-      for (int w : nbs) {
-        if ((int)mem.size() >= blockRows) break;
-        blk[w] = b;
-        mem.push_back(w);
-        queue.push_back(w);
+  std::vector<int> colour;
+  int nColours = 0;
+  if (cellXY && brickAggregates(n, g, cellXY, blockRows, rpos, blk, members, colour)) {
+    nColours = 3;
+  } else {
+    // 3. breadth-first aggregation into blocks of <= blockRows rows
+    std::vector<int> queue, nbs;
+    for (int s : rorder) {
+      if (blk[s] >= 0) continue;
+      int b = (int)members.size();
+      members.emplace_back();
+      auto &mem = members.back();
+      mem.push_back(s);
+      blk[s] = b;
+      queue.assign(1, s);
+      size_t head = 0;
+      while (head < queue.size() && (int)mem.size() < blockRows) {
+        int u = queue[head++];
+        nbs.clear();
+        for (int k = g.ptr[u]; k < g.ptr[u + 1]; ++k)
+          if (blk[g.adj[k]] < 0) nbs.push_back(g.adj[k]);
+        std::sort(nbs.begin(), nbs.end(), [&](int a, int c) { return rpos[a] < rpos[c]; });
+        for (int w : nbs) {
+          if ((int)mem.size() >= blockRows) break;
+          blk[w] = b;
+          mem.push_back(w);
+          queue.push_back(w);
+        }
+      }
+      std::sort(mem.begin(), mem.end(), [&](int a, int c) { return rpos[a] < rpos[c]; });
+    }
+    int nb = (int)members.size();
+
+    // 4. colouring of the block graph, DSATUR (Brelaz 1979): repeatedly colour
+    //    the block whose neighbours already use the most distinct colours (ties:
+    //    most neighbours, then lowest index) with the smallest free colour.
+    //    Every colour costs one kernel launch per sweep, so fewer is faster.
+    std::vector<std::vector<int>> badj(nb);
+    for (int b = 0; b < nb; ++b) {
+      auto &a = badj[b];
+      for (int u : members[b])
+        for (int k = g.ptr[u]; k < g.ptr[u + 1]; ++k) {
+          int ob = blk[g.adj[k]];
+          if (ob != b) a.push_back(ob);
+        }
+      std::sort(a.begin(), a.end());
+      a.erase(std::unique(a.begin(), a.end()), a.end());
+    }
+    colour.assign(nb, -1);
+    std::vector<int> satur(nb, 0);
+    std::vector<uint64_t> usedMask(nb, 0);  // colours seen among neighbours (< 64)
+    // priority: (saturation, degree, -index); max-heap with lazy entries
+    using Key = std::tuple<int, int, int>;
+    std::priority_queue<Key> heap;
+    for (int b = 0; b < nb; ++b) heap.push(Key(0, (int)badj[b].size(), -b));
+    while (!heap.empty()) {
+      auto [sat, deg, nbIdx] = heap.top();
+      heap.pop();
+      const int b = -nbIdx;
+      if (colour[b] >= 0 || sat != satur[b]) continue;  // stale entry
+      int c = 0;
+      while (c < 64 && (usedMask[b] >> c & 1ull)) ++c;
+      if (c >= 64) fail(PHASE_HIP_EINVAL, "block graph needs more than 64 colours");
+      colour[b] = c;
+      nColours = std::max(nColours, c + 1);
+      for (int ob : badj[b]) {
+        if (colour[ob] >= 0 || (usedMask[ob] >> c & 1ull)) continue;
+        usedMask[ob] |= 1ull << c;
+        ++satur[ob];
+        heap.push(Key(satur[ob], (int)badj[ob].size(), -ob));
       }
     }
-    std::sort(mem.begin(), mem.end(), [&](int a, int c) { return rpos[a] < rpos[c]; });
+    // 4b. fewer colours where Kempe-chain interchanges, finished by a bounded
+    //     Tabucol search for the blocks they leave, find them; each colour
+    //     removed saves two launches per preconditioner apply.
+    while (nColours > 2) {
+      std::vector<int> trial = colour;
+      const int maxLeft = std::max(16, nb / 1000);
+      const int left = kempeRecolour(badj, nColours, trial, maxLeft);
+      if (left > 0 && (left > maxLeft || !tabuRecolour(badj, nColours - 1, trial, 100000))) break;
+      colour.swap(trial);
+      --nColours;
+    }
   }
   int nb = (int)members.size();
-
-  // 4. colouring of the block graph, DSATUR (Brelaz 1979): repeatedly colour
-  //    the block whose neighbours already use the most distinct colours (ties:
-  //    most neighbours, then lowest index) with the smallest free colour.
-  //    Every colour costs one kernel launch per sweep, so fewer is faster.
-  std::vector<std::vector<int>> badj(nb);
-  for (int b = 0; b < nb; ++b) {
-    auto &a = badj[b];
-    for (int u : members[b])
-      for (int k = g.ptr[u]; k < g.ptr[u + 1]; ++k) {
-        int ob = blk[g.adj[k]];
-        if (ob != b) a.push_back(ob);
-      }
-    std::sort(a.begin(), a.end());
-    a.erase(std::unique(a.begin(), a.end()), a.end());
-  }
-  std::vector<int> colour(nb, -1), satur(nb, 0);
-  std::vector<uint64_t> usedMask(nb, 0);  // colours seen among neighbours (< 64)
-  // priority: (saturation, degree, -index); max-heap with lazy entries
-  using Key = std::tuple<int, int, int>;
-  std::priority_queue<Key> heap;
-  for (int b = 0; b < nb; ++b) heap.push(Key(0, (int)badj[b].size(), -b));
-  int nColours = 0;
-  while (!heap.empty()) {
-    auto [sat, deg, nbIdx] = heap.top();
-    heap.pop();
-    const int b = -nbIdx;
-    if (colour[b] >= 0 || sat != satur[b]) continue;  // stale entry
-    int c = 0;
-    while (c < 64 && (usedMask[b] >> c & 1ull)) ++c;
-    if (c >= 64) fail(PHASE_HIP_EINVAL, "block graph needs more than 64 colours");
-    colour[b] = c;
-    nColours = std::max(nColours, c + 1);
-    for (int ob : badj[b]) {
-      if (colour[ob] >= 0 || (usedMask[ob] >> c & 1ull)) continue;
-      usedMask[ob] |= 1ull << c;
-      ++satur[ob];
-      heap.push(Key(satur[ob], (int)badj[ob].size(), -ob));
-    }
-  }
-  // 4b. fewer colours where Kempe-chain interchanges, finished by a bounded
-  //     Tabucol search for the blocks they leave, find them; each colour
-  //     removed saves two launches per preconditioner apply.
-  while (nColours > 2) {
-    std::vector<int> trial = colour;
-    const int maxLeft = std::max(16, nb / 1000);
-    const int left = kempeRecolour(badj, nColours, trial, maxLeft);
-    if (left > 0 && (left > maxLeft || !tabuRecolour(badj, nColours - 1, trial, 100000))) break;
-    colour.swap(trial);
-    --nColours;
-  }
   o.nColours = nColours;
 
   // 5. ILU blocks: aggregates of one colour share no coupling, so consecutive

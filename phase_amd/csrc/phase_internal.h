@@ -121,7 +121,10 @@ struct Ordering {
   std::vector<int> updSrc, updDst;    // fac indices
 };
 
-void buildOrdering(Ordering &o, int n, const int *rowPtr, const int *colInd, int blockRows);
+// cellXY (optional): centroid (x, y) of each row's cell, interleaved; with it
+// a scalar equation on a 2-D mesh gets three-colour brick aggregates.
+void buildOrdering(Ordering &o, int n, const int *rowPtr, const int *colInd, int blockRows,
+                   const double *cellXY = nullptr);
 
 }  // namespace phase
 

@@ -39,6 +39,7 @@ struct phase_hip_solver {
   int maxOuter = 100;
   int chunk = 8;  // iterations enqueued between host checks
   int syncFree = 0;  // ILU sweeps: 0 = one launch per block colour, 1 = one sync-free launch
+  int geometric = 1; // cell centroids known: brick aggregates in three colours (else graph aggregates)
   int cg = 0;        // Krylov method: 0 = BiCGStab, 1 = preconditioned CG (symmetric systems)
 
   // pattern and ordering
@@ -98,7 +99,8 @@ struct phase_hip_solver {
 namespace phase {
 // Upload a new pattern (rebuilding the ordering) -- used by set() and by the
 // mesh assembly path.
-void solverSetPattern(phase_hip_solver *s, int n, const int *rowPtr, const int *colInd);
+void solverSetPattern(phase_hip_solver *s, int n, const int *rowPtr, const int *colInd,
+                      const double *cellXY = nullptr);
 // dVals changed on the device: refresh SpMV copies, invalidate factors.
 void solverValuesChanged(phase_hip_solver *s);
 void solverFreeBinding(phase_hip_solver *s);

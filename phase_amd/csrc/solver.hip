@@ -548,10 +548,10 @@ void allocCommon(phase_hip_solver *s) {
 
 namespace phase {
 
-void solverSetPattern(phase_hip_solver *s, int n, const int *rowPtr, const int *colInd) {
+void solverSetPattern(phase_hip_solver *s, int n, const int *rowPtr, const int *colInd, const double *cellXY) {
   setDevice(s);
   if (s->rank != 0 && n != s->rank) fail(PHASE_HIP_EINVAL, "pattern size differs from setRank");
-  buildOrdering(s->ord, n, rowPtr, colInd, s->blockRows);
+  buildOrdering(s->ord, n, rowPtr, colInd, s->blockRows, s->geometric ? cellXY : nullptr);
   s->rank = n;
   const Ordering &o = s->ord;
   hipStream_t st = s->stream;
@@ -604,7 +604,7 @@ __global__ void kCmpInt(size_t n, const int *__restrict__ a, const int *__restri
 namespace phase {
 
 void solverSetFromRows(phase_hip_solver *s, int rank, int stride, const int *dCols, const double *dVals,
-                       const double *dRhs, hipStream_t producer) {
+                       const double *dRhs, hipStream_t producer, const double *dCellXY) {
   setDevice(s);
   if (rank <= 0) fail(PHASE_HIP_EINVAL, "empty equation");
   PH_CHECK(hipStreamSynchronize(producer));
@@ -624,6 +624,11 @@ void solverSetFromRows(phase_hip_solver *s, int rank, int stride, const int *dCo
   if (!same) {
     std::vector<int> cols(slots), rowPtr(rank + 1);
     PH_CHECK(hipMemcpy(cols.data(), dCols, slots * sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<double> xy;
+    if (dCellXY && s->geometric) {
+      xy.resize(2 * (size_t)rank);
+      PH_CHECK(hipMemcpy(xy.data(), dCellXY, xy.size() * sizeof(double), hipMemcpyDeviceToHost));
+    }
     for (int i = 0; i <= rank; ++i) rowPtr[i] = i * stride;
     // the last solution survives a new ordering of the same rows (warm start)
     DBuf<double> keep;
@@ -636,7 +641,7 @@ void solverSetFromRows(phase_hip_solver *s, int rank, int stride, const int *dCo
     }
     solverFreeBinding(s);
     s->rank = 0;
-    solverSetPattern(s, rank, rowPtr.data(), cols.data());
+    solverSetPattern(s, rank, rowPtr.data(), cols.data(), xy.empty() ? nullptr : xy.data());
     if (warm) {
       hipLaunchKernelGGL((kPermuteIn<double, double>), dim3(gridFor(rank, 256, kElemCap)), dim3(256), 0, s->stream,
                          rank, s->dPerm.p, keep.p, s->dX.p, 1.0);
@@ -760,6 +765,11 @@ int phase_hip_solver_setup(phase_hip_solver *s, const char *key, const char *val
       if (lv == "bicgstab") s->cg = 0;
       else if (lv == "cg") s->cg = 1;
       else fail(PHASE_HIP_EINVAL, "unknown solver '" + v + "' (BICGSTAB or CG)");
+    } else if (k == "ordering") {
+      const int g2 = lv == "geometric" ? 1 : lv == "graph" ? 0 : -1;
+      if (g2 < 0) fail(PHASE_HIP_EINVAL, "unknown ordering '" + v + "' (geometric or graph)");
+      if (g2 != s->geometric && s->havePattern) s->havePattern = false;  // rebuild on next set
+      s->geometric = g2;
     } else if (k == "sweep") {
       if (lv == "colour" || lv == "color" || lv == "0") s->syncFree = 0;
       else if (lv == "syncfree" || lv == "1") s->syncFree = 1;

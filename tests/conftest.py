@@ -43,7 +43,7 @@ def emul():
     f64 = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
     lib.emul_run.restype = ctypes.c_int
     lib.emul_run.argtypes = [ctypes.c_int, i32, i32, f64, ctypes.c_int, i32, i32, i32, f64, ctypes.c_int, f64,
-                             f64, f64, f64, f64, i32, ctypes.c_char_p, ctypes.c_int]
+                             f64, f64, f64, f64, i32, ctypes.c_char_p, ctypes.c_int, ctypes.c_void_p]
     return lib
 
 

@@ -30,11 +30,11 @@ struct Bad {
 extern "C" int emul_run(int n, const int *rowPtr, const int *colInd, const double *vals, int blockRows,
                         int *perm, int *facRowPtr, int *facCol, double *facVal, int cap, const double *rhsIn,
                         double *y, double *x, const double *xin, double *spmvOut, int *info, char *err,
-                        int errLen) {
+                        int errLen, const double *cellXY) {
   try {
     Ordering o;
     try {
-      buildOrdering(o, n, rowPtr, colInd, blockRows);
+      buildOrdering(o, n, rowPtr, colInd, blockRows, cellXY);
     } catch (const HipError &e) {
       throw Bad{e.msg};
     }

@@ -13,7 +13,7 @@ import pytest
 from tests import helpers as H
 
 
-def _run(emul, rp, ci, v, rhs_solver, x_solver, block_rows=64):
+def _run(emul, rp, ci, v, rhs_solver, x_solver, block_rows=64, xy=None):
     n = len(rp) - 1
     cap = 2 * len(v) + n + 16
     perm = np.empty(n, np.int32)
@@ -23,23 +23,24 @@ def _run(emul, rp, ci, v, rhs_solver, x_solver, block_rows=64):
     y, x, sp = np.empty(n), np.empty(n), np.empty(n)
     info = np.zeros(8, np.int32)
     err = ctypes.create_string_buffer(256)
+    xyp = None if xy is None else np.ascontiguousarray(xy, np.float64)
     rc = emul.emul_run(n, rp, ci, v, block_rows, perm, frp, fci, fv, cap, rhs_solver, y, x, x_solver, sp, info,
-                       err, 256)
+                       err, 256, None if xyp is None else xyp.ctypes.data)
     assert rc == 0, err.value.decode()
     return dict(perm=perm, frp=frp, fci=fci[:frp[-1]], fv=fv[:frp[-1]], y=y, x=x, spmv=sp, info=info)
 
 
-def _check(oracle, emul, rp, ci, v, block_rows=64, seed=0):
+def _check(oracle, emul, rp, ci, v, block_rows=64, seed=0, xy=None):
     n = len(rp) - 1
     rng = np.random.default_rng(seed)
     # first pass only to learn the permutation
     dummy = np.zeros(n)
-    r0 = _run(emul, rp, ci, v, dummy, dummy, block_rows)
+    r0 = _run(emul, rp, ci, v, dummy, dummy, block_rows, xy)
     perm = r0["perm"]
     assert sorted(perm.tolist()) == list(range(n))
     x_orig = rng.standard_normal(n)
     b_solver = rng.standard_normal(n)
-    r = _run(emul, rp, ci, v, b_solver, x_orig[perm], block_rows)
+    r = _run(emul, rp, ci, v, b_solver, x_orig[perm], block_rows, xy)
     # SpMV: same rows, same in-row order -> bitwise equal
     y_ref = oracle.spmv(rp, ci, v, x_orig)
     assert np.array_equal(r["spmv"], y_ref[perm])
@@ -77,6 +78,31 @@ def test_emul_delaunay_block_sizes(oracle, emul, block_rows):
     assert ncol >= 2
 
 
+@pytest.mark.parametrize("case", ["delaunay", "rectilinear"])
+def test_emul_brick_ordering(oracle, emul, case):
+    """With cell centroids the rows fall into running-bond brick aggregates in
+    exactly three colours; layout, factor and sweeps keep the bit-exact checks."""
+    if case == "delaunay":
+        xy, cptr, cind = H.delaunay_case(6000, 32, seed=9)
+    else:
+        xy, cptr, cind = H.rect_case(60, 50)
+    m, (rp, ci, v), b, g, bc = H.pressure_system(oracle, xy, cptr, cind, sigma=1.0)
+    _, cc = m.cells()
+    r = _check(oracle, emul, rp, ci, v, block_rows=32, seed=11, xy=cc.reshape(-1))
+    assert r["info"][1] == 3
+
+
+def test_emul_brick_fallback(oracle, emul):
+    """Centroids that do not belong to the rows (shuffled) couple same-colour
+    bricks: the ordering falls back to graph aggregates and stays correct."""
+    xy, cptr, cind = H.delaunay_case(3000, 24, seed=5)
+    m, (rp, ci, v), b, g, bc = H.pressure_system(oracle, xy, cptr, cind, sigma=1.0)
+    _, cc = m.cells()
+    cc = cc[np.random.default_rng(1).permutation(len(cc))]
+    r = _check(oracle, emul, rp, ci, v, block_rows=32, seed=12, xy=cc.reshape(-1))
+    assert r["info"][1] >= 2
+
+
 def test_emul_nonsymmetric_pattern(oracle, emul):
     """A random nonsymmetric, diagonally dominant pattern with empty slots."""
     rng = np.random.default_rng(11)
@@ -108,5 +134,6 @@ def test_emul_rejects_out_of_range(emul):
     z = np.zeros(n)
     info = np.zeros(8, np.int32)
     err = ctypes.create_string_buffer(256)
-    rc = emul.emul_run(n, rp, ci, v, 64, perm, frp, fci, fv, 16, z, z.copy(), z.copy(), z, z.copy(), info, err, 256)
+    rc = emul.emul_run(n, rp, ci, v, 64, perm, frp, fci, fv, 16, z, z.copy(), z.copy(), z, z.copy(), info, err, 256,
+                       None)
     assert rc == 1 and b"out of range" in err.value
