@@ -528,7 +528,7 @@ void buildOrdering(Ordering &o, int n, const int *rowPtr, const int *colInd, int
       if (u.empty() || u.back().col != e.col) u.push_back(e);
     r.swap(u);
   }
-  int lRun = 0, uRun = 0;
+  int lRun = 0, uRun = 0, maxRegs = 0;
   for (int b = 0; b < nb; ++b) {
     BlockInfo &B = o.blocks[b];
     int bs = B.rowStart, be = bs + B.nRows;
@@ -553,6 +553,7 @@ void buildOrdering(Ordering &o, int n, const int *rowPtr, const int *colInd, int
       if (nL - nLoff > 8 || nUin > 8)
         fail(PHASE_HIP_EINVAL, "row couples to more than 8 rows of its own ILU block");
       if (nLoff > 63) fail(PHASE_HIP_EINVAL, "row couples to more than 63 rows of earlier ILU blocks");
+      maxRegs = std::max({maxRegs, nL - nLoff, nLoff, nUin, nU - nUin});
       o.rows[k].nL = (uint8_t)nL;
       o.rows[k].nLoff = (uint8_t)nLoff;
       o.rows[k].nU = (uint8_t)nU;
@@ -579,6 +580,7 @@ void buildOrdering(Ordering &o, int n, const int *rowPtr, const int *colInd, int
     uRun += kU * kLanes;
   }
   o.nLSlots = lRun;
+  o.regWidth = maxRegs <= 4 ? 4 : 8;
   o.nUSlots = uRun;
   o.lanes.assign((size_t)nb * kLanes, LaneRec{-1, 0u});
   for (int b = 0; b < nb; ++b) {

@@ -103,6 +103,7 @@ struct Ordering {
   std::vector<int> compactSrc;       // compact entry -> caller slot (padded index)
   std::vector<int> perm, iperm;      // solver row k <-> original row perm[k]
   int nBlocks = 0, nColours = 0;
+  int regWidth = 8;                  // sweep kernels' per-row register width (4 when every count fits)
   std::vector<int> colourPtr;         // blocks of colour c: [colourPtr[c], colourPtr[c+1])
   std::vector<BlockInfo> blocks;
   std::vector<int> pred, succ;

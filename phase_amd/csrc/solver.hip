@@ -96,33 +96,46 @@ SweepCtl colourCtl(int base) {
   return c;
 }
 
+// Calls f with the ordering's sweep register width as a compile-time constant.
+template <class F>
+void byWidth(const phase_hip_solver *s, F &&f) {
+  if (s->ord.regWidth == 4) f(std::integral_constant<int, 4>());
+  else f(std::integral_constant<int, 8>());
+}
+
 template <class T, int PRO>
 void launchL(phase_hip_solver *s, const IluDev<T> &L, const Scalars *sc, const T *in, T *y, T *pv, const T *r,
              const T *v) {
-  if (s->syncFree) {
-    hipLaunchKernelGGL((kLSolve<T, PRO>), dim3(s->ord.nBlocks), dim3(64), 0, s->stream, L, syncDev(s), syncFreeCtl(s),
-                       sc, in, y, pv, r, v);
-    return;
-  }
-  const std::vector<int> &cp = s->ord.colourPtr;
-  for (int c = 0; c + 1 < (int)cp.size(); ++c)
-    if (cp[c + 1] > cp[c])
-      hipLaunchKernelGGL((kLSolve<T, PRO>), dim3(cp[c + 1] - cp[c]), dim3(64), 0, s->stream, L, syncDev(s),
-                         colourCtl(cp[c]), sc, in, y, pv, r, v);
+  byWidth(s, [&](auto Wc) {
+    constexpr int W = decltype(Wc)::value;
+    if (s->syncFree) {
+      hipLaunchKernelGGL((kLSolve<T, PRO, W>), dim3(s->ord.nBlocks), dim3(64), 0, s->stream, L, syncDev(s),
+                         syncFreeCtl(s), sc, in, y, pv, r, v);
+      return;
+    }
+    const std::vector<int> &cp = s->ord.colourPtr;
+    for (int c = 0; c + 1 < (int)cp.size(); ++c)
+      if (cp[c + 1] > cp[c])
+        hipLaunchKernelGGL((kLSolve<T, PRO, W>), dim3(cp[c + 1] - cp[c]), dim3(64), 0, s->stream, L, syncDev(s),
+                           colourCtl(cp[c]), sc, in, y, pv, r, v);
+  });
 }
 
 template <class T>
 void launchU(phase_hip_solver *s, const IluDev<T> &L, const Scalars *sc, const T *y, T *x) {
-  if (s->syncFree) {
-    hipLaunchKernelGGL((kUSolve<T>), dim3(s->ord.nBlocks), dim3(64), 0, s->stream, L, syncDev(s), syncFreeCtl(s), sc,
-                       y, x);
-    return;
-  }
-  const std::vector<int> &cp = s->ord.colourPtr;
-  for (int c = (int)cp.size() - 2; c >= 0; --c)
-    if (cp[c + 1] > cp[c])
-      hipLaunchKernelGGL((kUSolve<T>), dim3(cp[c + 1] - cp[c]), dim3(64), 0, s->stream, L, syncDev(s),
-                         colourCtl(cp[c]), sc, y, x);
+  byWidth(s, [&](auto Wc) {
+    constexpr int W = decltype(Wc)::value;
+    if (s->syncFree) {
+      hipLaunchKernelGGL((kUSolve<T, W>), dim3(s->ord.nBlocks), dim3(64), 0, s->stream, L, syncDev(s),
+                         syncFreeCtl(s), sc, y, x);
+      return;
+    }
+    const std::vector<int> &cp = s->ord.colourPtr;
+    for (int c = (int)cp.size() - 2; c >= 0; --c)
+      if (cp[c + 1] > cp[c])
+        hipLaunchKernelGGL((kUSolve<T, W>), dim3(cp[c + 1] - cp[c]), dim3(64), 0, s->stream, L, syncDev(s),
+                           colourCtl(cp[c]), sc, y, x);
+  });
 }
 
 // out = U^{-1} L^{-1} (right-hand side formed by PRO); per-colour mode fuses
@@ -137,17 +150,20 @@ void launchLU(phase_hip_solver *s, const IluDev<T> &L, const Scalars *sc, const 
     launchU<T>(s, L, sc, y, out);
     return;
   }
-  for (int c = 0; c + 1 < C; ++c)
-    if (cp[c + 1] > cp[c])
-      hipLaunchKernelGGL((kLSolve<T, PRO>), dim3(cp[c + 1] - cp[c]), dim3(64), 0, s->stream, L, syncDev(s),
-                         colourCtl(cp[c]), sc, in, y, pv, r, v);
-  if (cp[C] > cp[C - 1])
-    hipLaunchKernelGGL((kLUSolve<T, PRO>), dim3(cp[C] - cp[C - 1]), dim3(64), 0, s->stream, L, syncDev(s),
-                       colourCtl(cp[C - 1]), sc, in, (const T *)y, pv, r, v, out);
-  for (int c = C - 2; c >= 0; --c)
-    if (cp[c + 1] > cp[c])
-      hipLaunchKernelGGL((kUSolve<T>), dim3(cp[c + 1] - cp[c]), dim3(64), 0, s->stream, L, syncDev(s),
-                         colourCtl(cp[c]), sc, (const T *)y, out);
+  byWidth(s, [&](auto Wc) {
+    constexpr int W = decltype(Wc)::value;
+    for (int c = 0; c + 1 < C; ++c)
+      if (cp[c + 1] > cp[c])
+        hipLaunchKernelGGL((kLSolve<T, PRO, W>), dim3(cp[c + 1] - cp[c]), dim3(64), 0, s->stream, L, syncDev(s),
+                           colourCtl(cp[c]), sc, in, y, pv, r, v);
+    if (cp[C] > cp[C - 1])
+      hipLaunchKernelGGL((kLUSolve<T, PRO, W>), dim3(cp[C] - cp[C - 1]), dim3(64), 0, s->stream, L, syncDev(s),
+                         colourCtl(cp[C - 1]), sc, in, (const T *)y, pv, r, v, out);
+    for (int c = C - 2; c >= 0; --c)
+      if (cp[c + 1] > cp[c])
+        hipLaunchKernelGGL((kUSolve<T, W>), dim3(cp[c + 1] - cp[c]), dim3(64), 0, s->stream, L, syncDev(s),
+                           colourCtl(cp[c]), sc, (const T *)y, out);
+  });
 }
 
 template <class T>
@@ -1186,25 +1202,28 @@ int phase_hip_solver_time_kernels(phase_hip_solver *s, int reps, double *ms, int
     const double *in = s->k64.ph.p;
     double *y = s->k64.y.p, *x = s->k64.sh.p;
     out[8] = out[9] = out[10] = 0.;
-    if (C >= 2) {
-      out[8] = timeIt([&] {
-                 for (int c = C - 2; c >= 0; --c)
-                   hipLaunchKernelGGL((kUSolve<double>), dim3(cp[c + 1] - cp[c]), dim3(64), 0, s->stream, L64,
-                                      syncDev(s), colourCtl(cp[c]), noSc, (const double *)y, x);
-               }) / (C - 1);
-      out[10] = timeIt([&] {
-                  for (int c = 0; c + 1 < C; ++c)
-                    hipLaunchKernelGGL((kLSolve<double, kProNone>), dim3(cp[c + 1] - cp[c]), dim3(64), 0,
-                                       s->stream, L64, syncDev(s), colourCtl(cp[c]), noSc, in, y, (double *)nullptr,
-                                       (const double *)nullptr, (const double *)nullptr);
-                }) / (C - 1);
-    }
-    if (C >= 1)
-      out[9] = timeIt([&] {
-        hipLaunchKernelGGL((kLUSolve<double, kProNone>), dim3(cp[C] - cp[C - 1]), dim3(64), 0, s->stream, L64,
-                           syncDev(s), colourCtl(cp[C - 1]), noSc, in, (const double *)y, (double *)nullptr,
-                           (const double *)nullptr, (const double *)nullptr, x);
-      });
+    byWidth(s, [&](auto Wc) {
+      constexpr int W = decltype(Wc)::value;
+      if (C >= 2) {
+        out[8] = timeIt([&] {
+                   for (int c = C - 2; c >= 0; --c)
+                     hipLaunchKernelGGL((kUSolve<double, W>), dim3(cp[c + 1] - cp[c]), dim3(64), 0, s->stream, L64,
+                                        syncDev(s), colourCtl(cp[c]), noSc, (const double *)y, x);
+                 }) / (C - 1);
+        out[10] = timeIt([&] {
+                    for (int c = 0; c + 1 < C; ++c)
+                      hipLaunchKernelGGL((kLSolve<double, kProNone, W>), dim3(cp[c + 1] - cp[c]), dim3(64), 0,
+                                         s->stream, L64, syncDev(s), colourCtl(cp[c]), noSc, in, y, (double *)nullptr,
+                                         (const double *)nullptr, (const double *)nullptr);
+                  }) / (C - 1);
+      }
+      if (C >= 1)
+        out[9] = timeIt([&] {
+          hipLaunchKernelGGL((kLUSolve<double, kProNone, W>), dim3(cp[C] - cp[C - 1]), dim3(64), 0, s->stream, L64,
+                             syncDev(s), colourCtl(cp[C - 1]), noSc, in, (const double *)y, (double *)nullptr,
+                             (const double *)nullptr, (const double *)nullptr, x);
+        });
+    });
     // cache-cold SpMV: before each timed launch a 1 GiB copy evicts the matrix
     // and vectors from L2 and from the 256 MB last-level cache, and the events
     // bracket the one launch

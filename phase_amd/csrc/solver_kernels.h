@@ -18,9 +18,10 @@
 
 namespace phase {
 
-constexpr int kMaxOffU = 8;  // off-block U entries of one row held in registers
-constexpr int kMaxOffL = 8;  // off-block L entries of one row held in registers (more: a tail loop)
-constexpr int kMaxIn = 8;    // in-block L / U entries of one row held in registers
+// The sweep kernels hold a row's factor entries in registers, W of each kind
+// (in-block L, in-block U, off-block U; off-block L beyond W take a tail
+// loop).  W = 4 covers triangle and quad meshes and halves the registers, so
+// more blocks are resident per launch; W = 8 is the general case.
 constexpr int kRedBlock = 256;
 
 // The dispatcher hands workgroups to the 8 XCDs round robin (workgroup w runs
@@ -717,7 +718,7 @@ __global__ void kInvDiag(int n, T *fac, int dbase) {
 // kProP: p = r + beta (p - omega v);  kProS: s = r - alpha v (both stored).
 // Accumulation order (earlier columns first) is that of the oracle's forward
 // substitution, so the result is bit-identical to it.
-template <class T, int PRO>
+template <class T, int PRO, int W>
 __device__ __forceinline__ T lBody(const IluDev<T> &L, const SyncDev &sy, const SweepCtl &ctl, const BlockInfo &B,
                                    bool act, int row, unsigned info, int lane, const Scalars *sc,
                                    const T *__restrict__ in, const T *__restrict__ y, const T *__restrict__ pv,
@@ -741,10 +742,10 @@ __device__ __forceinline__ T lBody(const IluDev<T> &L, const SyncDev &sy, const 
   }
   pro = acc;  // stored by the caller at the end: a store here would wait for the loads
   if (!act) acc = 0;
-  T lv[kMaxIn];
-  int lc[kMaxIn];
+  T lv[W];
+  int lc[W];
 #pragma unroll
-  for (int q = 0; q < kMaxIn; ++q) {
+  for (int q = 0; q < W; ++q) {
     lv[q] = 0;
     lc[q] = 0;
     if (q < nIn) {
@@ -755,10 +756,10 @@ __device__ __forceinline__ T lBody(const IluDev<T> &L, const SyncDev &sy, const 
   }
   // column indices stay 32-bit unsigned until the gathers: a sign extension
   // here would make each predicated load wait for its own result
-  T ov[kMaxOffL];
-  unsigned oc[kMaxOffL];
+  T ov[W];
+  unsigned oc[W];
 #pragma unroll
-  for (int q = 0; q < kMaxOffL; ++q) {
+  for (int q = 0; q < W; ++q) {
     ov[q] = 0;
     oc[q] = 0;
     if (q < nOff) {
@@ -771,20 +772,20 @@ __device__ __forceinline__ T lBody(const IluDev<T> &L, const SyncDev &sy, const 
   // (otherwise each column load is waited for in turn)
   asm volatile("" ::: "memory");
   sweepWait(ctl, sy, L.pred, B.predBeg, B.predEnd);
-  T oy[kMaxOffL];
+  T oy[W];
 #pragma unroll
-  for (int q = 0; q < kMaxOffL; ++q) oy[q] = q < nOff ? y[oc[q]] : (T)0;
+  for (int q = 0; q < W; ++q) oy[q] = q < nOff ? y[oc[q]] : (T)0;
 #pragma unroll
-  for (int q = 0; q < kMaxOffL; ++q)
+  for (int q = 0; q < W; ++q)
     if (q < nOff) acc = fma(-ov[q], oy[q], acc);
-  for (int e = kMaxOffL; e < nOff; ++e) {
+  for (int e = W; e < nOff; ++e) {
     const int s = B.lOff + (e << 6) + lane;
     acc = fma(-L.fac[s], y[L.lCol[s]], acc);
   }
   for (int l = 0; l < B.nLev; ++l) {
     if (act && lev == l) {
 #pragma unroll
-      for (int q = 0; q < kMaxIn; ++q)
+      for (int q = 0; q < W; ++q)
         if (q < nIn) acc = fma(-lv[q], ylds[lc[q] - B.rowStart], acc);
       ylds[lane] = acc;
     }
@@ -798,19 +799,19 @@ __device__ __forceinline__ T lBody(const IluDev<T> &L, const SyncDev &sy, const 
 // sweep), uSolve gathers later-coloured solution values and runs the levels
 // from acc = y of the lane's row, returning x of the row.  In-block columns
 // precede later-block columns, as in the oracle's back substitution.
-template <class T>
+template <class T, int W>
 struct UPre {
   T d;  // 1 / u_ii
   int nIn, nOff, ulev;
-  T uv[kMaxIn];
-  int uc[kMaxIn];
-  T ou[kMaxOffU];
-  unsigned oc[kMaxOffU];
+  T uv[W];
+  int uc[W];
+  T ou[W];
+  unsigned oc[W];
 };
 
-template <class T>
+template <class T, int W>
 __device__ __forceinline__ void uLoad(const IluDev<T> &L, const BlockInfo &B, bool act, int row, unsigned info,
-                                      int lane, UPre<T> &u) {
+                                      int lane, UPre<T, W> &u) {
   const T *facU = L.fac + L.nLSlots;
   u.nIn = act ? (int)((info >> 10) & 15u) : 0;
   u.nOff = act ? (int)((info >> 14) & 15u) : 0;
@@ -819,7 +820,7 @@ __device__ __forceinline__ void uLoad(const IluDev<T> &L, const BlockInfo &B, bo
   const T d0 = L.fac[L.nLSlots + L.nUSlots + L.n + (act ? row : 0)];
   u.d = act ? d0 : (T)1;
 #pragma unroll
-  for (int q = 0; q < kMaxIn; ++q) {
+  for (int q = 0; q < W; ++q) {
     u.uv[q] = 0;
     u.uc[q] = 0;
     if (q < u.nIn) {
@@ -829,7 +830,7 @@ __device__ __forceinline__ void uLoad(const IluDev<T> &L, const BlockInfo &B, bo
     }
   }
 #pragma unroll
-  for (int q = 0; q < kMaxOffU; ++q) {
+  for (int q = 0; q < W; ++q) {
     u.ou[q] = 0;
     u.oc[q] = 0;
     if (q < u.nOff) {
@@ -840,25 +841,25 @@ __device__ __forceinline__ void uLoad(const IluDev<T> &L, const BlockInfo &B, bo
   }
 }
 
-template <class T>
+template <class T, int W>
 __device__ __forceinline__ T uSolve(const IluDev<T> &L, const SyncDev &sy, const SweepCtl &ctl, const BlockInfo &B,
-                                    bool act, int lane, T acc, const T *__restrict__ x, T *xlds, const UPre<T> &u) {
+                                    bool act, int lane, T acc, const T *__restrict__ x, T *xlds, const UPre<T, W> &u) {
   asm volatile("" ::: "memory");
   sweepWait(ctl, sy, L.succ, B.succBeg, B.succEnd);
-  T ox[kMaxOffU];
+  T ox[W];
 #pragma unroll
-  for (int q = 0; q < kMaxOffU; ++q) ox[q] = q < u.nOff ? x[u.oc[q]] : (T)0;
+  for (int q = 0; q < W; ++q) ox[q] = q < u.nOff ? x[u.oc[q]] : (T)0;
   // later-block terms are final before the levels start: their sum leaves the
   // level loop, whose body (issued for the whole wave at every level) keeps
   // only the in-block terms and one multiply by the inverse diagonal
   T soff = 0;
 #pragma unroll
-  for (int q = 0; q < kMaxOffU; ++q)
+  for (int q = 0; q < W; ++q)
     if (q < u.nOff) soff = fma(u.ou[q], ox[q], soff);
   for (int l = 0; l < B.nULev; ++l) {
     if (act && u.ulev == l) {
 #pragma unroll
-      for (int q = 0; q < kMaxIn; ++q)
+      for (int q = 0; q < W; ++q)
         if (q < u.nIn) acc = fma(-u.uv[q], xlds[u.uc[q] - B.rowStart], acc);
       acc = (acc - soff) * u.d;
       xlds[lane] = acc;
@@ -870,7 +871,7 @@ __device__ __forceinline__ T uSolve(const IluDev<T> &L, const SyncDev &sy, const
 
 // Forward sweep y = L^{-1} b (unit L).  After convergence (sc->done) every
 // lane is idle and the launch only walks the block's levels.
-template <class T, int PRO>
+template <class T, int PRO, int W = 8>
 __global__ __launch_bounds__(64) void kLSolve(IluDev<T> L, SyncDev sy, SweepCtl ctl, const Scalars *sc,
                                               const T *__restrict__ in, T *__restrict__ y, T *__restrict__ pv,
                                               const T *__restrict__ r, const T *__restrict__ v) {
@@ -883,7 +884,7 @@ __global__ __launch_bounds__(64) void kLSolve(IluDev<T> L, SyncDev sy, SweepCtl 
   const LaneRec rec = L.lanes[(size_t)b * 64 + lane];
   const bool act = rec.row >= 0 && !done;
   T pro;
-  const T acc = lBody<T, PRO>(L, sy, ctl, B, act, rec.row, rec.info, lane, sc, in, y, pv, r, v, ylds, pro);
+  const T acc = lBody<T, PRO, W>(L, sy, ctl, B, act, rec.row, rec.info, lane, sc, in, y, pv, r, v, ylds, pro);
   if (act) {
     if (PRO != kProNone) pv[rec.row] = pro;
     y[rec.row] = acc;
@@ -892,7 +893,7 @@ __global__ __launch_bounds__(64) void kLSolve(IluDev<T> L, SyncDev sy, SweepCtl 
 }
 
 // Backward sweep x = U^{-1} y (colours in reverse order).
-template <class T>
+template <class T, int W = 8>
 __global__ __launch_bounds__(64) void kUSolve(IluDev<T> L, SyncDev sy, SweepCtl ctl, const Scalars *sc,
                                               const T *__restrict__ y, T *__restrict__ x) {
   __shared__ T xlds[64];
@@ -904,9 +905,9 @@ __global__ __launch_bounds__(64) void kUSolve(IluDev<T> L, SyncDev sy, SweepCtl 
   const LaneRec rec = L.lanes[(size_t)b * 64 + lane];
   const bool act = rec.row >= 0 && !done;
   const T y0 = y[act ? rec.row : 0];
-  UPre<T> u;
-  uLoad<T>(L, B, act, rec.row, rec.info, lane, u);
-  const T acc = uSolve<T>(L, sy, ctl, B, act, lane, act ? y0 : (T)0, x, xlds, u);
+  UPre<T, W> u;
+  uLoad<T, W>(L, B, act, rec.row, rec.info, lane, u);
+  const T acc = uSolve<T, W>(L, sy, ctl, B, act, lane, act ? y0 : (T)0, x, xlds, u);
   if (act) x[rec.row] = acc;
   sweepPublish(ctl, sy, b);
 }
@@ -914,7 +915,7 @@ __global__ __launch_bounds__(64) void kUSolve(IluDev<T> L, SyncDev sy, SweepCtl 
 // Last colour of a preconditioner apply (per-colour launches only): blocks of
 // the last colour have no later-coloured neighbours, so the forward and the
 // backward sweep of such a block run back to back in one launch.
-template <class T, int PRO>
+template <class T, int PRO, int W = 8>
 __global__ __launch_bounds__(64) void kLUSolve(IluDev<T> L, SyncDev sy, SweepCtl ctl, const Scalars *sc,
                                                const T *__restrict__ in, const T *__restrict__ y,
                                                T *__restrict__ pv, const T *__restrict__ r,
@@ -927,11 +928,11 @@ __global__ __launch_bounds__(64) void kLUSolve(IluDev<T> L, SyncDev sy, SweepCtl
   const BlockInfo B = L.blocks[b];
   const LaneRec rec = L.lanes[(size_t)b * 64 + lane];
   const bool act = rec.row >= 0 && !done;
-  UPre<T> u;
-  uLoad<T>(L, B, act, rec.row, rec.info, lane, u);
+  UPre<T, W> u;
+  uLoad<T, W>(L, B, act, rec.row, rec.info, lane, u);
   T pro;
-  const T yv = lBody<T, PRO>(L, sy, ctl, B, act, rec.row, rec.info, lane, sc, in, y, pv, r, v, ylds, pro);
-  const T acc = uSolve<T>(L, sy, ctl, B, act, lane, yv, x, xlds, u);
+  const T yv = lBody<T, PRO, W>(L, sy, ctl, B, act, rec.row, rec.info, lane, sc, in, y, pv, r, v, ylds, pro);
+  const T acc = uSolve<T, W>(L, sy, ctl, B, act, lane, yv, x, xlds, u);
   if (act) {
     if (PRO != kProNone) pv[rec.row] = pro;
     x[rec.row] = acc;
