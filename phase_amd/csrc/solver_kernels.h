@@ -795,10 +795,10 @@ __device__ __forceinline__ T lBody(const IluDev<T> &L, const SyncDev &sy, const 
 }
 
 // Backward sweep of one block in two parts: uLoad issues every load the row
-// needs from the factor (so a fused launch can overlap them with its forward
-// sweep), uSolve gathers later-coloured solution values and runs the levels
-// from acc = y of the lane's row, returning x of the row.  In-block columns
-// precede later-block columns, as in the oracle's back substitution.
+// needs from the factor, uSolve gathers later-coloured solution values and
+// runs the levels from acc = y of the lane's row, returning x of the row.
+// In-block columns precede later-block columns, as in the oracle's back
+// substitution.
 template <class T, int W>
 struct UPre {
   T d;  // 1 / u_ii
@@ -914,7 +914,10 @@ __global__ __launch_bounds__(64) void kUSolve(IluDev<T> L, SyncDev sy, SweepCtl 
 
 // Last colour of a preconditioner apply (per-colour launches only): blocks of
 // the last colour have no later-coloured neighbours, so the forward and the
-// backward sweep of such a block run back to back in one launch.
+// backward sweep of such a block run back to back in one launch.  The U
+// entries load after the forward levels: issued up front they overlap the
+// forward sweep's loads but double the registers, and the lower occupancy
+// costs more (a second round of blocks per launch) than the round trip.
 template <class T, int PRO, int W = 8>
 __global__ __launch_bounds__(64) void kLUSolve(IluDev<T> L, SyncDev sy, SweepCtl ctl, const Scalars *sc,
                                                const T *__restrict__ in, const T *__restrict__ y,
@@ -928,10 +931,10 @@ __global__ __launch_bounds__(64) void kLUSolve(IluDev<T> L, SyncDev sy, SweepCtl
   const BlockInfo B = L.blocks[b];
   const LaneRec rec = L.lanes[(size_t)b * 64 + lane];
   const bool act = rec.row >= 0 && !done;
-  UPre<T, W> u;
-  uLoad<T, W>(L, B, act, rec.row, rec.info, lane, u);
   T pro;
   const T yv = lBody<T, PRO, W>(L, sy, ctl, B, act, rec.row, rec.info, lane, sc, in, y, pv, r, v, ylds, pro);
+  UPre<T, W> u;
+  uLoad<T, W>(L, B, act, rec.row, rec.info, lane, u);
   const T acc = uSolve<T, W>(L, sy, ctl, B, act, lane, yv, x, xlds, u);
   if (act) {
     if (PRO != kProNone) pv[rec.row] = pro;
