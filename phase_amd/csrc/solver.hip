@@ -105,19 +105,19 @@ void byWidth(const phase_hip_solver *s, F &&f) {
 
 template <class T, int PRO>
 void launchL(phase_hip_solver *s, const IluDev<T> &L, const Scalars *sc, const T *in, T *y, T *pv, const T *r,
-             const T *v) {
+             const T *v, double *xs = nullptr) {
   byWidth(s, [&](auto Wc) {
     constexpr int W = decltype(Wc)::value;
     if (s->syncFree) {
       hipLaunchKernelGGL((kLSolve<T, PRO, W>), dim3(s->ord.nBlocks), dim3(64), 0, s->stream, L, syncDev(s),
-                         syncFreeCtl(s), sc, in, y, pv, r, v);
+                         syncFreeCtl(s), sc, in, y, pv, r, v, xs);
       return;
     }
     const std::vector<int> &cp = s->ord.colourPtr;
     for (int c = 0; c + 1 < (int)cp.size(); ++c)
       if (cp[c + 1] > cp[c])
         hipLaunchKernelGGL((kLSolve<T, PRO, W>), dim3(cp[c + 1] - cp[c]), dim3(64), 0, s->stream, L, syncDev(s),
-                           colourCtl(cp[c]), sc, in, y, pv, r, v);
+                           colourCtl(cp[c]), sc, in, y, pv, r, v, xs);
   });
 }
 
@@ -142,11 +142,11 @@ void launchU(phase_hip_solver *s, const IluDev<T> &L, const Scalars *sc, const T
 // the last forward colour with the first backward colour.
 template <class T, int PRO>
 void launchLU(phase_hip_solver *s, const IluDev<T> &L, const Scalars *sc, const T *in, T *y, T *pv, const T *r,
-              const T *v, T *out) {
+              const T *v, T *out, double *xs = nullptr) {
   const std::vector<int> &cp = s->ord.colourPtr;
   const int C = (int)cp.size() - 1;
   if (s->syncFree || C < 1) {
-    launchL<T, PRO>(s, L, sc, in, y, pv, r, v);
+    launchL<T, PRO>(s, L, sc, in, y, pv, r, v, xs);
     launchU<T>(s, L, sc, y, out);
     return;
   }
@@ -155,10 +155,10 @@ void launchLU(phase_hip_solver *s, const IluDev<T> &L, const Scalars *sc, const 
     for (int c = 0; c + 1 < C; ++c)
       if (cp[c + 1] > cp[c])
         hipLaunchKernelGGL((kLSolve<T, PRO, W>), dim3(cp[c + 1] - cp[c]), dim3(64), 0, s->stream, L, syncDev(s),
-                           colourCtl(cp[c]), sc, in, y, pv, r, v);
+                           colourCtl(cp[c]), sc, in, y, pv, r, v, xs);
     if (cp[C] > cp[C - 1])
       hipLaunchKernelGGL((kLUSolve<T, PRO, W>), dim3(cp[C] - cp[C - 1]), dim3(64), 0, s->stream, L, syncDev(s),
-                         colourCtl(cp[C - 1]), sc, in, (const T *)y, pv, r, v, out);
+                         colourCtl(cp[C - 1]), sc, in, (const T *)y, pv, r, v, out, xs);
     for (int c = C - 2; c >= 0; --c)
       if (cp[c + 1] > cp[c])
         hipLaunchKernelGGL((kUSolve<T, W>), dim3(cp[c + 1] - cp[c]), dim3(64), 0, s->stream, L, syncDev(s),
@@ -255,21 +255,23 @@ void iluApply(phase_hip_solver *s, DBuf<T> &fac, const T *in, T *tmp, T *out, co
   launchLU<T, kProNone>(s, L, sc, in, tmp, (T *)nullptr, (const T *)nullptr, (const T *)nullptr, out);
 }
 
-// z = M^{-1} r ; rho' = (r, z), beta = rho' / rho ; p = z + beta p  (the CG
-// direction update).  start: the first direction (p is zero); otherwise the
-// (r, z) partials follow kCgUpdate's (r, r) partials and kFinishCg judges both.
+// rho' = (r, z), beta = rho' / rho ; p = z + beta p  (z in ph)
 template <class T>
-void cgDirection(phase_hip_solver *s, const IluDev<T> &L, KrylovVecs<T> &K, Scalars *sc, bool start) {
+void cgBeta(phase_hip_solver *s, KrylovVecs<T> &K, Scalars *sc) {
   const int n = s->rank;
-  launchLU<T, kProNone>(s, L, sc, (const T *)K.r.p, K.y.p, (T *)nullptr, (const T *)nullptr, (const T *)nullptr,
-                        K.ph.p);
   const int gu = gridFor(n, kRedBlock, kRedCap);
-  hipLaunchKernelGGL((kDotRZ<T>), dim3(gu), dim3(kRedBlock), 0, s->stream, n, sc, K.r.p, K.ph.p,
-                     s->dPartials.p + (start ? 0 : gu));
-  if (start) hipLaunchKernelGGL(kFinishCgZ, dim3(1), dim3(kRedBlock), 0, s->stream, sc, s->dPartials.p, gu);
-  else hipLaunchKernelGGL(kFinishCg, dim3(1), dim3(kRedBlock), 0, s->stream, sc, s->dPartials.p, gu);
+  hipLaunchKernelGGL((kDotRZ<T>), dim3(gu), dim3(kRedBlock), 0, s->stream, n, sc, K.r.p, K.ph.p, s->dPartials.p);
+  hipLaunchKernelGGL(kFinishCgZ, dim3(1), dim3(kRedBlock), 0, s->stream, sc, s->dPartials.p, gu);
   hipLaunchKernelGGL((kCgDirection<T>), dim3(gridFor(n, 256, kElemCap)), dim3(256), 0, s->stream, n, sc, K.ph.p,
                      K.p.p);
+}
+
+// Start of the CG recurrence from r (p is zero): z = M^{-1} r, p = z.
+template <class T>
+void cgStart(phase_hip_solver *s, const IluDev<T> &L, KrylovVecs<T> &K, Scalars *sc) {
+  launchLU<T, kProNone>(s, L, sc, (const T *)K.r.p, K.y.p, (T *)nullptr, (const T *)nullptr, (const T *)nullptr,
+                        K.ph.p);
+  cgBeta<T>(s, K, sc);
 }
 
 // One right-preconditioned BiCGStab iteration, all scalars on the device.
@@ -297,20 +299,19 @@ void iterate(phase_hip_solver *s, KrylovVecs<T> &K, DBuf<T> &fac, const TV *sell
   hipLaunchKernelGGL(kFinishUpdate, dim3(1), dim3(kRedBlock), 0, s->stream, sc, s->dPartials.p, gu);
 }
 
-// One preconditioned CG iteration: q = A p, alpha; x, r; z = M^{-1} r; the
-// stopping tests and beta; p.
+// One preconditioned CG iteration: q = A p, the stopping test and alpha; x and
+// r moved inside the sweeps that form z = M^{-1} r; beta; p.
 template <class T, class TX, class TV>
 void iterateCg(phase_hip_solver *s, KrylovVecs<T> &K, DBuf<T> &fac, const TV *sell, TX *x, Scalars *sc) {
+  static_assert(std::is_same<TX, double>::value, "the CG prologue updates an FP64 solution");
   const int n = s->rank;
   IluDev<T> L = iluDev<T>(s, fac.p);
   const int gs = spmvGrid(s);
-  hipLaunchKernelGGL((kSpmvDotV<T, TV>), dim3(gs), dim3(kRedBlock), 0, s->stream, n, s->ord.nSlices, s->dSliceOff.p,
-                     s->dSellCol.p, sell, K.p.p, K.v.p, K.p.p, sc, s->dPartials.p);
-  hipLaunchKernelGGL(kFinishV, dim3(1), dim3(kRedBlock), 0, s->stream, sc, s->dPartials.p, gs);
-  const int gu = gridFor(n, kRedBlock, kRedCap);
-  hipLaunchKernelGGL((kCgUpdate<T, TX>), dim3(gu), dim3(kRedBlock), 0, s->stream, n, sc, x, K.p.p, K.r.p, K.v.p,
-                     s->dPartials.p);
-  cgDirection<T>(s, L, K, sc, false);
+  hipLaunchKernelGGL((kSpmvDotCg<T, TV>), dim3(gs), dim3(kRedBlock), 0, s->stream, n, s->ord.nSlices,
+                     s->dSliceOff.p, s->dSellCol.p, sell, K.p.p, K.v.p, K.r.p, sc, s->dPartials.p);
+  hipLaunchKernelGGL(kFinishCgQ, dim3(1), dim3(kRedBlock), 0, s->stream, sc, s->dPartials.p, gs);
+  launchLU<T, kProCgR>(s, L, sc, (const T *)K.p.p, K.y.p, K.r.p, (const T *)nullptr, (const T *)K.v.p, K.ph.p, x);
+  cgBeta<T>(s, K, sc);
 }
 
 void pullScalars(phase_hip_solver *s) {
@@ -370,7 +371,7 @@ void fp64Loop(phase_hip_solver *s, double b2, int total) {
                        K.r0.p, sc, s->dPartials.p, s->dCounter.p);
     hipLaunchKernelGGL((kBicgInit<double>), dim3(gridFor(n, 256, kElemCap)), dim3(256), 0, s->stream, n, sc,
                        K.p.p, K.v.p, (double *)nullptr, tol2, s->maxIters - total, 0.0);
-    if (s->cg) cgDirection<double>(s, iluDev<double>(s, s->dFac64.p), K, sc, true);
+    if (s->cg) cgStart<double>(s, iluDev<double>(s, s->dFac64.p), K, sc);
     pullScalars(s);
     s->error = std::sqrt(s->hSc[0].rr / b2);
     ++s->outerPasses;
@@ -504,7 +505,7 @@ void solveMixed(phase_hip_solver *s) {
                        n, K.r.p, &inner->rr, s->dPartials.p, s->dCounter.p);
     hipLaunchKernelGGL((kBicgInit<float>), dim3(gridFor(n, 256, kElemCap)), dim3(256), 0, s->stream, n, inner,
                        K.p.p, K.v.p, (float *)nullptr, tol2, s->maxIters - total, 0.0);
-    if (s->cg) cgDirection<float>(s, iluDev<float>(s, s->dFac32.p), K, inner, true);
+    if (s->cg) cgStart<float>(s, iluDev<float>(s, s->dFac32.p), K, inner);
     pullScalars(s);
     double lastRU = s->hSc[0].rr;
     int it = runKrylov<float, double, double>(s, K, s->dFac32, s->dSell64.p, s->dX.p, s->maxIters - total,
@@ -1214,14 +1215,14 @@ int phase_hip_solver_time_kernels(phase_hip_solver *s, int reps, double *ms, int
                     for (int c = 0; c + 1 < C; ++c)
                       hipLaunchKernelGGL((kLSolve<double, kProNone, W>), dim3(cp[c + 1] - cp[c]), dim3(64), 0,
                                          s->stream, L64, syncDev(s), colourCtl(cp[c]), noSc, in, y, (double *)nullptr,
-                                         (const double *)nullptr, (const double *)nullptr);
+                                         (const double *)nullptr, (const double *)nullptr, (double *)nullptr);
                   }) / (C - 1);
       }
       if (C >= 1)
         out[9] = timeIt([&] {
           hipLaunchKernelGGL((kLUSolve<double, kProNone, W>), dim3(cp[C] - cp[C - 1]), dim3(64), 0, s->stream, L64,
                              syncDev(s), colourCtl(cp[C - 1]), noSc, in, (const double *)y, (double *)nullptr,
-                             (const double *)nullptr, (const double *)nullptr, x);
+                             (const double *)nullptr, (const double *)nullptr, x, (double *)nullptr);
         });
     });
     // cache-cold SpMV: before each timed launch a 1 GiB copy evicts the matrix

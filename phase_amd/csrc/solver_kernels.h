@@ -35,7 +35,9 @@ __device__ __forceinline__ int xcdTile(int w, int G) {
   return x * per + min(x, rem) + w / kXcds;
 }
 
-enum { kProNone = 0, kProP = 1, kProS = 2 };
+// Prologues a forward sweep can form its right-hand side with (and store):
+// BiCGStab's p and s, or CG's x and r update.
+enum { kProNone = 0, kProP = 1, kProS = 2, kProCgR = 3 };
 
 template <class T>
 struct IluDev {
@@ -459,55 +461,55 @@ static __global__ __launch_bounds__(kRedBlock) void kFinishUpdate(Scalars *sc, c
 
 // ---------------------------------------------------------------- CG vector steps
 // Preconditioned conjugate gradients (Hestenes-Stiefel, with M = ILU(0)) for
-// symmetric systems, on the same sweeps and SpMV-dot kernel: q = A p and
-// (p, q) come from kSpmvDotV(p, q, p) and kFinishV sets alpha = rho / (p, q).
+// symmetric systems.  One iteration: q = A p with (p, q) and (r, r)
+// (kSpmvDotCg, kFinishCgQ: stopping test, alpha); x += alpha p and
+// r -= alpha q inside the forward sweeps that apply M^{-1} to the new r
+// (kProCgR); (r, z) and beta (kDotRZ, kFinishCgZ); p = z + beta p.
 
-// x += alpha p ; r -= alpha q ; partials of (r, r)
-// (kFinishCg tests them together with those of the next (r, z))
-template <class T, class TX>
-__global__ __launch_bounds__(kRedBlock) void kCgUpdate(int n, const Scalars *sc, TX *__restrict__ x,
-                                                       const T *__restrict__ p, T *__restrict__ r,
-                                                       const T *__restrict__ q, double *partials) {
+// q = A p ; partials of (p, q) and of (r, r) for the stopping test (r as the
+// previous iteration's sweeps left it)
+template <class T, class TV = T>
+__global__ __launch_bounds__(kRedBlock) void kSpmvDotCg(int n, int nSlices, const int *__restrict__ sliceOff,
+                                                        const int *__restrict__ col, const TV *__restrict__ val,
+                                                        const T *__restrict__ p, T *__restrict__ q,
+                                                        const T *__restrict__ r, const Scalars *sc,
+                                                        double *partials) {
   if (sc->done) return;
-  const double alpha = sc->alpha;
-  if (!isfinite(alpha)) return;  // kFinishCg flags the breakdown
-  const T al = (T)alpha;
-  const TX alx = (TX)alpha;
-  double d[1] = {0.};
-  for (int i = blockIdx.x * kRedBlock + threadIdx.x; i < n; i += gridDim.x * kRedBlock) {
-    x[i] = x[i] + alx * (TX)p[i];
-    const T ri = r[i] - al * q[i];
-    r[i] = ri;
-    d[0] += (double)ri * (double)ri;
+  const int tile = xcdTile(blockIdx.x, gridDim.x);
+  const int slice = tile * (kRedBlock / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int row = slice * 64 + lane;
+  double d[2] = {0., 0.};
+  if (slice < nSlices && row < n) {
+    const T y = sellRow(slice, lane, sliceOff, col, val, p);
+    q[row] = y;
+    const double ri = (double)r[row];
+    d[0] = (double)p[row] * (double)y;
+    d[1] = ri * ri;
   }
-  blockPartials<1>(d, partials, blockIdx.x);
+  blockPartials<2>(d, partials, tile);
 }
 
-// rr = (r, r) (partials of kCgUpdate, first nb) and rho' = (r, z) (partials of
-// kDotRZ, next nb): stopping tests, then beta = rho' / rho.  The test on rr
-// comes one preconditioner apply after the update that produced r; x is final
-// at that point and the extra apply is discarded.
-static __global__ __launch_bounds__(kRedBlock) void kFinishCg(Scalars *sc, const double *partials, int nb) {
+// Stopping tests on rr = (r, r) (the residual of the current x), then
+// alpha = rho / (p, q); the sweeps that follow move x and r by alpha.
+static __global__ __launch_bounds__(kRedBlock) void kFinishCgQ(Scalars *sc, const double *partials, int nb) {
   if (sc->done) return;
   double d[2];
   finishPartials<2>(partials, nb, d);
   if (threadIdx.x != 0) return;
-  if (!isfinite(sc->alpha)) {
-    sc->done = kDoneBreakdown;
-    return;
-  }
-  const double rr = d[0], rhoNew = d[1];
+  const double pq = d[0], rr = d[1];
   sc->rrPrev = sc->rr;
   sc->rr = rr;
-  sc->iters += 1;
+  sc->rv = pq;
   if (!isfinite(rr)) sc->done = kDoneBreakdown;
   else if (rr <= sc->tol2) sc->done = kDoneConverged;
-  else if (!isfinite(rhoNew) || rhoNew == 0. || sc->rho == 0.) sc->done = kDoneBreakdown;
+  else if (sc->iters > 0 && sc->minRatio > 0. && rr > sc->minRatio * sc->minRatio * sc->rrPrev)
+    sc->done = kDoneStagnation;
+  else if (sc->iters >= sc->maxIters) sc->done = kDoneMaxIters;
+  else if (pq == 0. || !isfinite(pq)) sc->done = kDoneBreakdown;
   else {
-    sc->beta = rhoNew / sc->rho;
-    sc->rho = rhoNew;
-    if (sc->minRatio > 0. && rr > sc->minRatio * sc->minRatio * sc->rrPrev) sc->done = kDoneStagnation;
-    else if (sc->iters >= sc->maxIters) sc->done = kDoneMaxIters;
+    sc->alpha = sc->rho / pq;
+    sc->iters += 1;
   }
 }
 
@@ -522,7 +524,7 @@ __global__ __launch_bounds__(kRedBlock) void kDotRZ(int n, const Scalars *sc, co
   blockPartials<1>(d, partials, blockIdx.x);
 }
 
-// start of the recurrence: rho' = (r, z), beta = rho' / rho (p is zero)
+// rho' = (r, z), beta = rho' / rho  (at the start p is zero)
 static __global__ __launch_bounds__(kRedBlock) void kFinishCgZ(Scalars *sc, const double *partials, int nb) {
   if (sc->done) return;
   double d[1];
@@ -714,15 +716,18 @@ __global__ void kInvDiag(int n, T *fac, int dbase) {
 }
 
 // Forward sweep body for one block: returns y of the lane's row.  PRO selects
-// the fused BiCGStab prologue that forms the right-hand side on the fly:
-// kProP: p = r + beta (p - omega v);  kProS: s = r - alpha v (both stored).
+// the fused Krylov prologue that forms the right-hand side on the fly:
+// kProP: p = r + beta (p - omega v);  kProS: s = r - alpha v (both stored);
+// kProCgR: r = r - alpha q with r and q in pv and v, and x += alpha p with p
+// in `in` (r and x stored).
 // Accumulation order (earlier columns first) is that of the oracle's forward
 // substitution, so the result is bit-identical to it.
 template <class T, int PRO, int W>
 __device__ __forceinline__ T lBody(const IluDev<T> &L, const SyncDev &sy, const SweepCtl &ctl, const BlockInfo &B,
                                    bool act, int row, unsigned info, int lane, const Scalars *sc,
                                    const T *__restrict__ in, const T *__restrict__ y, const T *__restrict__ pv,
-                                   const T *__restrict__ r, const T *__restrict__ v, T *ylds, T &pro) {
+                                   const T *__restrict__ r, const T *__restrict__ v, T *ylds, T &pro,
+                                   const double *__restrict__ xs, double &xo) {
   const int nIn = act ? (int)(info & 15u) : 0;
   const int nOff = act ? (int)((info >> 4) & 63u) : 0;
   const int lev = (int)((info >> 18) & 63u);
@@ -737,6 +742,10 @@ __device__ __forceinline__ T lBody(const IluDev<T> &L, const SyncDev &sy, const 
   } else if (PRO == kProS) {
     const T alpha = (T)sc->alpha;
     acc = r[rs] - alpha * v[rs];
+  } else if (PRO == kProCgR) {
+    const double alpha = sc->alpha;
+    acc = pv[rs] - (T)alpha * v[rs];
+    xo = xs[rs] + alpha * (double)in[rs];
   } else {
     acc = in[rs];
   }
@@ -874,7 +883,8 @@ __device__ __forceinline__ T uSolve(const IluDev<T> &L, const SyncDev &sy, const
 template <class T, int PRO, int W = 8>
 __global__ __launch_bounds__(64) void kLSolve(IluDev<T> L, SyncDev sy, SweepCtl ctl, const Scalars *sc,
                                               const T *__restrict__ in, T *__restrict__ y, T *__restrict__ pv,
-                                              const T *__restrict__ r, const T *__restrict__ v) {
+                                              const T *__restrict__ r, const T *__restrict__ v,
+                                              double *__restrict__ xs) {
   __shared__ T ylds[64];
   const int b = sweepBlock(ctl, sy, L.nb, false);
   if (b < 0) return;
@@ -884,9 +894,12 @@ __global__ __launch_bounds__(64) void kLSolve(IluDev<T> L, SyncDev sy, SweepCtl 
   const LaneRec rec = L.lanes[(size_t)b * 64 + lane];
   const bool act = rec.row >= 0 && !done;
   T pro;
-  const T acc = lBody<T, PRO, W>(L, sy, ctl, B, act, rec.row, rec.info, lane, sc, in, y, pv, r, v, ylds, pro);
+  double xo = 0.;
+  const T acc = lBody<T, PRO, W>(L, sy, ctl, B, act, rec.row, rec.info, lane, sc, in, y, pv, r, v, ylds, pro,
+                                 xs, xo);
   if (act) {
     if (PRO != kProNone) pv[rec.row] = pro;
+    if (PRO == kProCgR) xs[rec.row] = xo;
     y[rec.row] = acc;
   }
   sweepPublish(ctl, sy, b);
@@ -922,7 +935,7 @@ template <class T, int PRO, int W = 8>
 __global__ __launch_bounds__(64) void kLUSolve(IluDev<T> L, SyncDev sy, SweepCtl ctl, const Scalars *sc,
                                                const T *__restrict__ in, const T *__restrict__ y,
                                                T *__restrict__ pv, const T *__restrict__ r,
-                                               const T *__restrict__ v, T *__restrict__ x) {
+                                               const T *__restrict__ v, T *__restrict__ x, double *__restrict__ xs) {
   __shared__ T ylds[64];
   __shared__ T xlds[64];
   const int b = ctl.blockBase + xcdTile(blockIdx.x, gridDim.x);
@@ -932,12 +945,15 @@ __global__ __launch_bounds__(64) void kLUSolve(IluDev<T> L, SyncDev sy, SweepCtl
   const LaneRec rec = L.lanes[(size_t)b * 64 + lane];
   const bool act = rec.row >= 0 && !done;
   T pro;
-  const T yv = lBody<T, PRO, W>(L, sy, ctl, B, act, rec.row, rec.info, lane, sc, in, y, pv, r, v, ylds, pro);
+  double xo = 0.;
+  const T yv = lBody<T, PRO, W>(L, sy, ctl, B, act, rec.row, rec.info, lane, sc, in, y, pv, r, v, ylds, pro, xs,
+                                xo);
   UPre<T, W> u;
   uLoad<T, W>(L, B, act, rec.row, rec.info, lane, u);
   const T acc = uSolve<T, W>(L, sy, ctl, B, act, lane, yv, x, xlds, u);
   if (act) {
     if (PRO != kProNone) pv[rec.row] = pro;
+    if (PRO == kProCgR) xs[rec.row] = xo;
     x[rec.row] = acc;
   }
 }
