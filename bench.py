@@ -46,13 +46,18 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_sweeps_pcorr_poisson_1M.m
 
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` in the committed counter summary's
-    table (column 5 of the row that names it), or None."""
+    table (the "HBM bytes per launch" column of the row that names it), or
+    None."""
     try:
+        col = None
         with open(TRAFFIC_FILE) as f:
             for line in f:
-                if line.startswith("| `%s` |" % kernel):
-                    return int(line.split("|")[5].strip().replace(",", ""))
-    except (OSError, ValueError, IndexError):
+                cells = [c.strip() for c in line.split("|")]
+                if "HBM bytes per launch" in cells:
+                    col = cells.index("HBM bytes per launch")
+                elif col is not None and len(cells) > col and cells[1] == "`%s`" % kernel:
+                    return int(cells[col].replace(",", ""))
+    except (OSError, ValueError):
         return None
     return None
 
@@ -162,9 +167,11 @@ def learned_case(args):
     d_g, d_bc, d_pf, d_b = (P.DeviceArray(g), P.DeviceArray(bc), P.DeviceArray(np.zeros(mesh.n_faces)),
                             P.DeviceArray(b))
     out = dict(workload="pcorr-poisson-delaunay-sigma2", cells=mesh.n_cells, edge_points=k, sigma=2.0)
-    for mode in ("fp64", "mixed", "learned"):
-        s = P.HipSparseMatrixSolver(0, tolerance=args.tol, maxIters=100000, mode=mode,
-                                    innerTolerance=args.inner_tol, blockRows=args.block_rows)
+    for mode in ("fp64", "mixed", "learned", "fp64-cg", "mixed-cg"):
+        prec, _, kry = mode.partition("-")
+        s = P.HipSparseMatrixSolver(0, tolerance=args.tol, maxIters=100000, mode=prec,
+                                    innerTolerance=args.inner_tol, blockRows=args.block_rows,
+                                    solver=kry or "bicgstab", ordering=args.ordering)
 
         def step():
             s.assemble_laplacian_device(mesh, 3, gamma_face=d_g, bc_type=d_bc, phi_face=d_pf)
@@ -173,7 +180,7 @@ def learned_case(args):
             return s.solve(raise_on_noconv=False)
 
         info = None
-        if mode == "learned":
+        if prec == "learned":
             every = max(1, args.capture_every // 4)
             per_solve = max(1, out["fp64"]["iters"] // every)
             a2 = argparse.Namespace(**vars(args))
@@ -221,8 +228,10 @@ def bubble_case(args):
     d_g, d_bc, d_pf, d_b = (P.DeviceArray(gamma), P.DeviceArray(bc), P.DeviceArray(np.zeros(mesh.n_faces)),
                             P.DeviceArray(rhs))
     out = dict(workload="rising-bubble-pressure-1x2-quads", cells=mesh.n_cells, density_ratio=1000.0)
-    for mode in ("fp64", "mixed"):
-        s = P.HipSparseMatrixSolver(0, tolerance=args.tol, maxIters=100000, mode=mode, blockRows=args.block_rows)
+    for mode in ("fp64", "mixed", "fp64-cg", "mixed-cg"):
+        prec, _, kry = mode.partition("-")
+        s = P.HipSparseMatrixSolver(0, tolerance=args.tol, maxIters=100000, mode=prec, blockRows=args.block_rows,
+                                    solver=kry or "bicgstab", ordering=args.ordering)
 
         def step():
             s.assemble_laplacian_device(mesh, 3, gamma_face=d_g, bc_type=d_bc, phi_face=d_pf)
@@ -385,20 +394,24 @@ def main():
     # headline: the fastest method that reached the FP64 goal
     ok = {k: v for k, v in results.items() if v["relres"] <= args.tol}
     head = min(ok, key=lambda k: ok[k]["ms_per_step"]) if ok else list(results)[0]
-    # dominant kernel of an FP64 BiCGStab iteration: two preconditioner applies
-    # (C-1 kLSolve + 1 kLUSolve + C-1 kUSolve launches each) and two SpMVs
+    # dominant kernel of one iteration of the headline method: per apply C-1
+    # kLSolve, one kLUSolve and C-1 kUSolve launches (FP32 sweeps for mixed /
+    # learned), per SpMV one SpMV launch with FP64 coefficients; BiCGStab
+    # applies and multiplies twice per iteration, CG once, so the shares are
+    # the same
     C = ordinfo["colours"]
-    share = {"usolve64_launch": 2 * (C - 1) * kernels["usolve64_launch"],
-             "lsolve64_launch": 2 * (C - 1) * kernels["lsolve64_launch"],
-             "lusolve64_launch": 2 * kernels["lusolve64_launch"],
-             "spmv64": 2 * kernels["spmv64"]}
+    w = "32" if head.partition("-")[0] in ("mixed", "learned") else "64"
+    share = {"usolve%s_launch" % w: (C - 1) * kernels["usolve%s_launch" % w],
+             "lsolve%s_launch" % w: (C - 1) * kernels["lsolve%s_launch" % w],
+             "lusolve%s_launch" % w: kernels["lusolve%s_launch" % w],
+             "spmv64": kernels["spmv64"]}
     dom = max(share, key=share.get)
-    names = {"usolve64_launch": "kUSolve<double>", "lsolve64_launch": "kLSolve<double>",
-             "lusolve64_launch": "kLUSolve<double>", "spmv64": "kSpmvDotV/T<double>"}
+    T = "double" if w == "64" else "float"
+    names = {"usolve%s_launch" % w: "kUSolve<%s>" % T, "lsolve%s_launch" % w: "kLSolve<%s>" % T,
+             "lusolve%s_launch" % w: "kLUSolve<%s>" % T, "spmv64": "kSpmv<double>"}
     roof = dict(kernel=names[dom], bound="hbm", achieved=round(gbs[dom], 1), peak=PEAK_HBM_GBS, unit="GB/s",
                 frac=round(gbs[dom] / PEAK_HBM_GBS, 4), traffic=measured_traffic(names[dom]),
-                algorithmic_bytes=round(kbytes[dom]),
-                avg_launch_ms=round(kernels[dom], 5))
+                algorithmic_bytes=round(kbytes[dom]), avg_launch_ms=round(kernels[dom], 5))
 
     out = {
         "metric": METRIC,

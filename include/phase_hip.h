@@ -166,7 +166,8 @@ int phase_hip_solver_ilu_export(phase_hip_solver *s, int *nnz, int *perm, int *r
  * the colours a solve launches it for), 9 the fused last-colour launch
  * (kLUSolve, FP64), 10 one forward-sweep launch (kLSolve, FP64, mean as 8),
  * 11 / 12 the FP64 / FP32 SpMV cache-cold (a 1 GiB copy before each launch
- * evicts L2 and the last-level cache; the events bracket the one launch).
+ * evicts L2 and the last-level cache; the events bracket the one launch),
+ * 13..15 as 8..10 with the FP32 factor (mixed mode's sweeps).
  * Benchmark support; no counterpart in the reference. */
 int phase_hip_solver_time_kernels(phase_hip_solver *s, int reps, double *ms, int nms);
 

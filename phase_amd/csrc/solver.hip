@@ -49,7 +49,7 @@ int guarded(phase_hip_solver *s, F &&f) {
 void setDevice(phase_hip_solver *s) { PH_CHECK(hipSetDevice(s->device)); }
 
 // entries of phase_hip_solver_time_kernels / phase_hip_solver_kernel_bytes
-constexpr int kTimedKernels = 13;
+constexpr int kTimedKernels = 16;
 
 template <class T>
 IluDev<T> iluDev(phase_hip_solver *s, T *fac) {
@@ -1125,18 +1125,22 @@ void kernelBytes(const Ordering &o, double *out) {
   out[7] = factor(4);
   out[11] = out[0];
   out[12] = out[1];
-  out[8] = out[9] = out[10] = 0.;
-  if (C >= 2) {
-    out[8] = sweep(0, o.colourPtr[C - 1], 8, true) / (C - 1);
-    out[10] = sweep(0, o.colourPtr[C - 1], 8, false) / (C - 1);
-  }
-  if (C >= 1) {
-    // fused: records read once, y kept in registers (neither stored nor reloaded)
-    const int b0 = o.colourPtr[C - 1];
-    double rows = 0.;
-    for (int b = b0; b < o.nBlocks; ++b) rows += o.blocks[b].nRows;
-    out[9] = sweep(b0, o.nBlocks, 8, false) + sweep(b0, o.nBlocks, 8, true) -
-             (o.nBlocks - b0) * (sizeof(BlockInfo) + 64.0 * sizeof(LaneRec)) - 2 * rows * 8;
+  // per-launch sweeps, FP64 in 8..10 and FP32 in 13..15
+  for (const int sz : {8, 4}) {
+    double *o3 = sz == 8 ? out + 8 : out + 13;  // [U launch, fused LU launch, L launch]
+    o3[0] = o3[1] = o3[2] = 0.;
+    if (C >= 2) {
+      o3[0] = sweep(0, o.colourPtr[C - 1], sz, true) / (C - 1);
+      o3[2] = sweep(0, o.colourPtr[C - 1], sz, false) / (C - 1);
+    }
+    if (C >= 1) {
+      // fused: records read once, y kept in registers (neither stored nor reloaded)
+      const int b0 = o.colourPtr[C - 1];
+      double rows = 0.;
+      for (int b = b0; b < o.nBlocks; ++b) rows += o.blocks[b].nRows;
+      o3[1] = sweep(b0, o.nBlocks, sz, false) + sweep(b0, o.nBlocks, sz, true) -
+              (o.nBlocks - b0) * (sizeof(BlockInfo) + 64.0 * sizeof(LaneRec)) - 2 * rows * sz;
+    }
   }
 }
 
@@ -1195,36 +1199,43 @@ int phase_hip_solver_time_kernels(phase_hip_solver *s, int reps, double *ms, int
     out[6] = timeIt([&] { factorize<double>(s, s->dFac64); });
     out[7] = timeIt([&] { factorize<float>(s, s->dFac32); });
     s->fac32Valid = false;  // solves use the rounded FP64 factor
-    // the launches of one FP64 preconditioner apply inside a solve: kUSolve and
-    // kLSolve over every colour but the last, kLUSolve over the last
+    // the launches of one preconditioner apply inside a solve: kUSolve and
+    // kLSolve over every colour but the last, kLUSolve over the last; FP64
+    // in 8..10, FP32 in 13..15
     const std::vector<int> &cp = s->ord.colourPtr;
     const int C = (int)cp.size() - 1;
-    const IluDev<double> L64 = iluDev<double>(s, s->dFac64.p);
-    const double *in = s->k64.ph.p;
-    double *y = s->k64.y.p, *x = s->k64.sh.p;
-    out[8] = out[9] = out[10] = 0.;
-    byWidth(s, [&](auto Wc) {
-      constexpr int W = decltype(Wc)::value;
-      if (C >= 2) {
-        out[8] = timeIt([&] {
-                   for (int c = C - 2; c >= 0; --c)
-                     hipLaunchKernelGGL((kUSolve<double, W>), dim3(cp[c + 1] - cp[c]), dim3(64), 0, s->stream, L64,
-                                        syncDev(s), colourCtl(cp[c]), noSc, (const double *)y, x);
-                 }) / (C - 1);
-        out[10] = timeIt([&] {
-                    for (int c = 0; c + 1 < C; ++c)
-                      hipLaunchKernelGGL((kLSolve<double, kProNone, W>), dim3(cp[c + 1] - cp[c]), dim3(64), 0,
-                                         s->stream, L64, syncDev(s), colourCtl(cp[c]), noSc, in, y, (double *)nullptr,
-                                         (const double *)nullptr, (const double *)nullptr, (double *)nullptr);
+    auto perLaunch = [&](auto *fac, auto &K, double *o3) {
+      using T = std::remove_pointer_t<decltype(fac)>;
+      const IluDev<T> L = iluDev<T>(s, fac);
+      const T *in = K.ph.p;
+      T *y = K.y.p, *x = K.sh.p;
+      o3[0] = o3[1] = o3[2] = 0.;
+      byWidth(s, [&](auto Wc) {
+        constexpr int W = decltype(Wc)::value;
+        if (C >= 2) {
+          o3[0] = timeIt([&] {
+                    for (int c = C - 2; c >= 0; --c)
+                      hipLaunchKernelGGL((kUSolve<T, W>), dim3(cp[c + 1] - cp[c]), dim3(64), 0, s->stream, L,
+                                         syncDev(s), colourCtl(cp[c]), noSc, (const T *)y, x);
                   }) / (C - 1);
-      }
-      if (C >= 1)
-        out[9] = timeIt([&] {
-          hipLaunchKernelGGL((kLUSolve<double, kProNone, W>), dim3(cp[C] - cp[C - 1]), dim3(64), 0, s->stream, L64,
-                             syncDev(s), colourCtl(cp[C - 1]), noSc, in, (const double *)y, (double *)nullptr,
-                             (const double *)nullptr, (const double *)nullptr, x, (double *)nullptr);
-        });
-    });
+          o3[2] = timeIt([&] {
+                    for (int c = 0; c + 1 < C; ++c)
+                      hipLaunchKernelGGL((kLSolve<T, kProNone, W>), dim3(cp[c + 1] - cp[c]), dim3(64), 0,
+                                         s->stream, L, syncDev(s), colourCtl(cp[c]), noSc, in, y, (T *)nullptr,
+                                         (const T *)nullptr, (const T *)nullptr, (double *)nullptr);
+                  }) / (C - 1);
+        }
+        if (C >= 1)
+          o3[1] = timeIt([&] {
+            hipLaunchKernelGGL((kLUSolve<T, kProNone, W>), dim3(cp[C] - cp[C - 1]), dim3(64), 0, s->stream, L,
+                               syncDev(s), colourCtl(cp[C - 1]), noSc, in, (const T *)y, (T *)nullptr,
+                               (const T *)nullptr, (const T *)nullptr, x, (double *)nullptr);
+          });
+      });
+    };
+    perLaunch(s->dFac64.p, s->k64, out + 8);
+    ensureSingle(s);  // the FP32 copy of the FP64 factor, as mixed solves use it
+    perLaunch(s->dFac32.p, s->k32, out + 13);
     // cache-cold SpMV: before each timed launch a 1 GiB copy evicts the matrix
     // and vectors from L2 and from the 256 MB last-level cache, and the events
     // bracket the one launch

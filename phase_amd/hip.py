@@ -597,7 +597,8 @@ class HipSparseMatrixSolver:
         return perm, rp, ci, v
 
     KERNEL_NAMES = ("spmv64", "spmv32", "lsweep64", "usweep64", "lsweep32", "usweep32", "factor64", "factor32",
-                    "usolve64_launch", "lusolve64_launch", "lsolve64_launch", "spmv64_cold", "spmv32_cold")
+                    "usolve64_launch", "lusolve64_launch", "lsolve64_launch", "spmv64_cold", "spmv32_cold",
+                    "usolve32_launch", "lusolve32_launch", "lsolve32_launch")
 
     def time_kernels(self, reps=20):
         """Mean device ms of each timed kernel or sweep (HIP events)."""
