@@ -146,6 +146,39 @@ def train_network(s, mesh, d_g, d_bc, d_pf, args, max_pairs=256, capture_solves=
                 train_s=round(t2 - t1, 2), loss_first=round(first, 4), loss_last=round(loss, 4))
 
 
+def host_path(args, csr, method):
+    """The same solve through the host-buffer boundary, as the reference's
+    FiniteVolumeEquation::solve drives SparseMatrixSolver: set(rowPtr, colInd,
+    vals) from host vectors (pattern once, values every step), setRhs, solve,
+    x() back to the host.  Without cell centroids the rows get graph
+    aggregates.  Timed per step including both PCIe copies."""
+    from phase_amd import hip as P
+
+    rp, ci, v, rhs = csr
+    b = -rhs  # setRhs(-rhs_), FiniteVolumeEquation.tpp:73
+    prec, _, kry = method.partition("-")
+    s = P.HipSparseMatrixSolver(0, tolerance=args.tol, maxIters=50000, mode=prec, innerTolerance=args.inner_tol,
+                                blockRows=args.block_rows, solver=kry or "bicgstab")
+    s.set(rp, ci, v)
+    s.set_rhs(b)
+    s.solve(raise_on_noconv=False)
+    times, iters = [], 0
+    for _ in range(2):
+        t0 = time.perf_counter()
+        s.set(rp, ci, v)
+        s.set_rhs(b)
+        s.zero_guess()
+        rel = s.solve(raise_on_noconv=False)
+        x = s.x()
+        times.append(time.perf_counter() - t0)
+        iters = s.stats()["inner_iters"]
+    s.close()
+    ms = 1e3 * min(times)
+    log("host path %s: %.2f ms/step (PCIe copies included), relres %.2e, %d iterations" % (method, ms, rel, iters))
+    return dict(method=method, ms_per_step=round(ms, 2), relres=float("%.4g" % rel), iters=iters,
+                ordering="graph aggregates (no centroids at this boundary)", x_norm=float("%.6g" % np.linalg.norm(x)))
+
+
 def learned_case(args):
     """The learned-preconditioner case: the standalone pressure Poisson on the
     same Delaunay generator at `args.c4_cells` cells with lognormal face
@@ -320,6 +353,8 @@ def main():
     ap.add_argument("--simple-iters", type=int, default=200, help="C1 SIMPLE outer iterations (0 skips)")
     ap.add_argument("--bubble-cells", type=int, default=2_000_000, help="C3 bubble pressure solve (0 skips)")
     ap.add_argument("--copy-probe-mb", type=int, default=0, help="run one copy kernel of this size (counter scale)")
+    ap.add_argument("--no-host-path", dest="host_path", action="store_false",
+                    help="skip the host-buffer (PCIe-inclusive) solve")
     ap.add_argument("--c4-cells", type=int, default=0, help="learned case, sigma 2, at this many cells (0 skips)")
     ap.add_argument("--c4-pairs", type=int, default=2000, help="training pairs for the learned case")
     args = ap.parse_args()
@@ -381,7 +416,8 @@ def main():
             kernels = s.time_kernels(args.kernel_reps)
             kbytes = s.kernel_bytes()
             ordinfo = s.ordering_info()
-            nnz = len(s.export()[1])
+            host_csr = s.export()
+            nnz = len(host_csr[1])
         s.close()
     if args.copy_probe_mb > 0:
         copy_probe(args.copy_probe_mb)
@@ -438,6 +474,8 @@ def main():
         "ordering": ordinfo,
         "roofline": roof,
     }
+    if args.host_path:
+        out["host_path"] = host_path(args, host_csr, head)
     if args.simple_iters > 0:
         out["simple_c1"] = simple_cavity(args)
     if args.bubble_cells > 0:
