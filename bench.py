@@ -230,6 +230,11 @@ def learned_case(args):
         if info is not None:
             out[mode].update(info)
         log("sigma2 %s: %.2f ms/step, relres %.2e, %d iterations" % (mode, 1e3 * dt, rel, st["inner_iters"]))
+        if mode == "fp64":
+            km, kb = s.time_kernels(5), s.kernel_bytes()
+            out["kernel_ms"] = {k: round(v, 5) for k, v in km.items()}
+            out["spmv_gbs"] = {k: round(kb[k] / (km[k] * 1e-3) / 1e9, 1) for k in ("spmv64", "spmv64_cold")}
+            out["ordering"] = s.ordering_info()
         s.close()
     return out
 
