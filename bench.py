@@ -179,6 +179,38 @@ def host_path(args, csr, method):
                 ordering="graph aggregates (no centroids at this boundary)", x_norm=float("%.6g" % np.linalg.norm(x)))
 
 
+def spmv_size_point(args):
+    """SpMV bandwidth far beyond the 256 MB last-level cache: the pressure
+    matrix of an n x n quad mesh (`args.spmv_cells` cells, lognormal face
+    coefficients), assembled on the device; FP64 and FP32 SpMV timed back to
+    back and cache-cold.  No solve."""
+    from phase_amd import hip as P
+    from phase_amd import meshgen as G
+
+    n = int(round(np.sqrt(args.spmv_cells)))
+    t0 = time.perf_counter()
+    xy, cptr, cind = G.rectilinear(n, n)
+    mesh = P.HipMesh(xy, cptr, cind)
+    log("spmv point: %d cells meshed (%.1fs)" % (mesh.n_cells, time.perf_counter() - t0))
+    f = mesh.faces()
+    pid = G.boundary_patch(f["lnode"], f["rnode"], f["rcell"], xy)
+    bc = np.where(pid == 2, P.BC_FIXED, P.BC_NORMAL_GRADIENT).astype(np.int32)
+    g = G.lognormal_face_coeffs(mesh.n_faces, 1.0, seed=2)
+    s = P.HipSparseMatrixSolver(0, blockRows=args.block_rows)
+    s.assemble_laplacian_device(mesh, 3, gamma_face=P.DeviceArray(g), bc_type=P.DeviceArray(bc),
+                                phi_face=P.DeviceArray(np.zeros(mesh.n_faces)))
+    log("spmv point: assembled and ordered (%.1fs)" % (time.perf_counter() - t0))
+    km, kb = s.time_kernels(5), s.kernel_bytes()
+    s.close()
+    keys = ("spmv64", "spmv64_cold", "spmv32", "spmv32_cold")
+    out = dict(workload="pcorr-poisson-quads-%d" % mesh.n_cells, cells=mesh.n_cells,
+               bytes={k: round(kb[k]) for k in ("spmv64", "spmv32")},
+               ms={k: round(km[k], 5) for k in keys},
+               gbs={k: round(kb[k] / (km[k] * 1e-3) / 1e9, 1) for k in keys})
+    log("spmv point: %s" % out["gbs"])
+    return out
+
+
 def learned_case(args):
     """The learned-preconditioner case: the standalone pressure Poisson on the
     same Delaunay generator at `args.c4_cells` cells with lognormal face
@@ -361,6 +393,7 @@ def main():
     ap.add_argument("--no-host-path", dest="host_path", action="store_false",
                     help="skip the host-buffer (PCIe-inclusive) solve")
     ap.add_argument("--c4-cells", type=int, default=0, help="learned case, sigma 2, at this many cells (0 skips)")
+    ap.add_argument("--spmv-cells", type=int, default=0, help="SpMV bandwidth point at this many cells (0 skips)")
     ap.add_argument("--c4-pairs", type=int, default=2000, help="training pairs for the learned case")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -487,6 +520,8 @@ def main():
         out["bubble_c3"] = bubble_case(args)
     if args.c4_cells > 0:
         out["learned_c4"] = learned_case(args)
+    if args.spmv_cells > 0:
+        out["spmv_size_point"] = spmv_size_point(args)
     if not args.no_cpu_baseline:
         it = results.get("fp64", results[head])["iters"]
         out["cpu_baseline"] = cpu_baseline(xy, cptr, cind, bc, g, b, it)

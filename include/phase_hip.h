@@ -291,6 +291,11 @@ int phase_hip_eq_ddt(phase_hip_equation *e, double timeStep, const double *phi0C
 /* fv::div(u, phi, theta), upwind -- Divergence.h:7-53 (u, u0: face vectors) */
 int phase_hip_eq_div(phase_hip_equation *e, const double *uFace, const double *u0Face, double theta,
                      const int *bcType, const double *phiFace, const double *phi0Cell, const double *phi0Face);
+/* cicsam::div(u, gamma, beta, theta) -- Cicsam.cpp:87-138 (scalar equation;
+ * beta from phase_hip_cicsam_weights; gammaFace, gamma0Face: FIXED faces). */
+int phase_hip_eq_cicsam_div(phase_hip_equation *e, const double *uFace, const double *beta, double theta,
+                            const int *bcType, const double *gammaFace, const double *gamma0Cell,
+                            const double *gamma0Face);
 
 /* Algebra: operator+= (sign 1) / -= (sign -1) -- CrsEquation.cpp:185-275;
  * operator*=(Scalar) :287-291; -=(Vector) / ==(Vector) :282-285, 308-311;
@@ -330,6 +335,21 @@ int phase_hip_field_grad(phase_hip_mesh *m, const double *phiCell, const double 
                          double *gradCell);
 int phase_hip_field_interpolate_faces(phase_hip_mesh *m, int ncomp, const double *phiCell, const int *bcType,
                                       double *phiFace);
+/* Volume of fluid, device fields:
+ * cicsam::faceInterpolationWeights(u, gamma, gradGamma, timeStep) --
+ *   Cicsam.cpp:20-66 (beta per face, 0 on boundary faces);
+ * cicsam::computeMomentumFlux(rho1, rho2, u, gamma, beta, rhoU) --
+ *   Cicsam.cpp:68-85 (rhoU interleaved per face);
+ * the density and viscosity part of FractionalStepMultiphase::updateProperties
+ *   -- FractionalStepMultiphase.cpp:175-214 (cells and faces). */
+int phase_hip_cicsam_weights(phase_hip_mesh *m, const double *uFace, const double *gammaCell,
+                             const double *gradGammaCell, double timeStep, double *beta);
+int phase_hip_cicsam_momentum_flux(phase_hip_mesh *m, double rho1, double rho2, const double *uFace,
+                                   const double *gammaCell, const double *gammaFace, const double *beta,
+                                   double *rhoU);
+int phase_hip_vof_properties(phase_hip_mesh *m, double rho1, double rho2, double mu1, double mu2,
+                             const double *gammaCell, const double *gammaFace, double *rhoCell, double *rhoFace,
+                             double *muCell, double *muFace);
 /* Wait for the work queued on the mesh's stream. */
 int phase_hip_mesh_synchronize(phase_hip_mesh *m);
 

@@ -57,6 +57,15 @@ def _load():
         "orc_div": (_vp, [_vp, ctypes.c_int, _f64p, _f64p, ctypes.c_double, _vp, _vp, _vp, _vp,
                           ctypes.c_char_p, ctypes.c_int]),
         "orc_src_div": (None, [_vp, _f64p, _f64p]),
+        "orc_cicsam_hc": (ctypes.c_double, [ctypes.c_double, ctypes.c_double]),
+        "orc_cicsam_uq": (ctypes.c_double, [ctypes.c_double, ctypes.c_double]),
+        "orc_cicsam_weights": (None, [_vp, _f64p, _f64p, _f64p, ctypes.c_double, _f64p]),
+        "orc_cicsam_div": (_vp, [_vp, _f64p, _f64p, ctypes.c_double, _vp, _vp, _vp, _vp, ctypes.c_char_p,
+                                 ctypes.c_int]),
+        "orc_cicsam_momentum_flux": (None, [_vp, ctypes.c_double, ctypes.c_double, _f64p, _f64p, _f64p, _f64p,
+                                            _f64p]),
+        "orc_vof_properties": (None, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                      _f64p, _f64p, _f64p, _f64p, _f64p, _f64p]),
         "orc_src_src": (None, [_vp, ctypes.c_int, _f64p, _f64p]),
         "orc_grad": (None, [_vp, _f64p, _f64p, _f64p, _f64p]),
         "orc_interpolate_faces": (None, [_vp, ctypes.c_int, _f64p, _vp, _f64p]),
@@ -269,6 +278,55 @@ def div(mesh, u_face, u0_face=None, theta=1.0, bc_type=None, phi_face=None, phi0
     if not h:
         raise RuntimeError(err.value.decode())
     return Eqn(h)
+
+
+def cicsam_hc(g, co):
+    return lib().orc_cicsam_hc(float(g), float(co))
+
+
+def cicsam_uq(g, co):
+    return lib().orc_cicsam_uq(float(g), float(co))
+
+
+def cicsam_weights(mesh, u_face, gamma_cell, grad_cell, time_step):
+    """cicsam::faceInterpolationWeights: beta per face (0 on boundary faces)."""
+    beta = np.empty(mesh.n_faces)
+    lib().orc_cicsam_weights(mesh._h, _f64(u_face), _f64(gamma_cell), _f64(grad_cell), float(time_step), beta)
+    return beta
+
+
+def cicsam_div(mesh, u_face, beta, theta, bc_type=None, gamma_face=None, gamma0_cell=None, gamma0_face=None):
+    """cicsam::div(u, gamma, beta, theta) as an equation."""
+    if gamma_face is None:
+        gamma_face = np.zeros(mesh.n_faces)
+    if gamma0_cell is None:
+        gamma0_cell = np.zeros(mesh.n_cells)
+    if gamma0_face is None:
+        gamma0_face = np.zeros(mesh.n_faces)
+    bc_type, gamma_face, gamma0_cell, gamma0_face = (_i32(bc_type), _f64(gamma_face), _f64(gamma0_cell),
+                                                     _f64(gamma0_face))
+    err = ctypes.create_string_buffer(256)
+    h = lib().orc_cicsam_div(mesh._h, _f64(u_face), _f64(beta), float(theta), _ptr(bc_type), _ptr(gamma_face),
+                             _ptr(gamma0_cell), _ptr(gamma0_face), err, 256)
+    if not h:
+        raise RuntimeError(err.value.decode())
+    return Eqn(h)
+
+
+def cicsam_momentum_flux(mesh, rho1, rho2, u_face, gamma_cell, gamma_face, beta):
+    out = np.empty(2 * mesh.n_faces)
+    lib().orc_cicsam_momentum_flux(mesh._h, float(rho1), float(rho2), _f64(u_face), _f64(gamma_cell),
+                                   _f64(gamma_face), _f64(beta), out)
+    return out
+
+
+def vof_properties(mesh, rho1, rho2, mu1, mu2, gamma_cell, gamma_face):
+    """(rho_cell, rho_face, mu_cell, mu_face) of updateProperties."""
+    rc, rf = np.empty(mesh.n_cells), np.empty(mesh.n_faces)
+    mc, mf = np.empty(mesh.n_cells), np.empty(mesh.n_faces)
+    lib().orc_vof_properties(mesh._h, float(rho1), float(rho2), float(mu1), float(mu2), _f64(gamma_cell),
+                             _f64(gamma_face), rc, rf, mc, mf)
+    return rc, rf, mc, mf
 
 
 def src_div(mesh, u_face):

@@ -506,6 +506,144 @@ void *orc_div(const void *mesh, int ncomp, const double *uFace, const double *u0
   return e;
 }
 
+// ---------------------------------------------------------------- VOF (CICSAM)
+namespace {
+// std::min / std::max as the reference calls them -- min(a, b) = (b < a) ? b : a,
+// max(a, b) = (a < b) ? b : a -- so a NaN first argument passes through
+inline double smin(double a, double b) { return (b < a) ? b : a; }
+inline double smax(double a, double b) { return (a < b) ? b : a; }
+// clamp(v, lo, hi) = max(min(v, hi), lo), Math/Algorithm.h:11-13
+inline double sclamp(double v, double lo, double hi) { return smax(smin(v, hi), lo); }
+// cicsam::hc, Cicsam.cpp:5-8
+inline double cicsamHc(double g, double co) { return g >= 0 && g <= 1 ? smin(1., g / co) : g; }
+// cicsam::uq, Cicsam.cpp:10-18
+inline double cicsamUq(double g, double co) {
+  return g >= 0 && g <= 1 ? smin((8. * co * g + (1. - co) * (6. * g + 3.)) / 8., cicsamHc(g, co)) : g;
+}
+}  // namespace
+
+double orc_cicsam_hc(double g, double co) { return cicsamHc(g, co); }
+double orc_cicsam_uq(double g, double co) { return cicsamUq(g, co); }
+
+// cicsam::faceInterpolationWeights, Cicsam.cpp:20-66 (k = 1; boundary faces 0).
+// cos(2 theta_F) with theta_F = acos(c) is evaluated as 2 c^2 - 1, which is
+// the same quantity without the two transcendental roundings; c > 1 from
+// rounding (where acos is NaN) stays NaN, so the weight falls to 0 as there.
+void orc_cicsam_weights(const void *mesh, const double *uFace, const double *gammaCell, const double *gradCell,
+                        double timeStep, double *beta) {
+  const Mesh &m = *static_cast<const Mesh *>(mesh);
+  for (int f = 0; f < m.nFaces; ++f) beta[f] = 0.;
+  for (int f = 0; f < m.nFaces; ++f) {
+    const int l = m.fL[f], r = m.fR[f];
+    if (r < 0) continue;
+    const V2 uf{uFace[2 * f], uFace[2 * f + 1]};
+    const double flux = dot(uf, outwardNorm(m, f, m.cC[l]));
+    const int d = flux > 0. ? l : r, a = flux <= 0. ? l : r;
+    const V2 rc = m.cC[a] - m.cC[d];
+    const V2 gd{gradCell[2 * d], gradCell[2 * d + 1]};
+    const double gD = sclamp(gammaCell[d], 0., 1.), gA = sclamp(gammaCell[a], 0., 1.);
+    const double gU = sclamp(gA - 2. * dot(rc, gd), 0., 1.);
+    double gDT = (gD - gU) / (gA - gU);
+    if (!std::isfinite(gDT)) gDT = 0.;
+    // donor Courant number over all of the donor's faces
+    double coD = 0.;
+    for (int k = m.iptr[d]; k < m.iptr[d + 1]; ++k) {
+      const int g = m.iface[k];
+      coD += smax(dot(V2{uFace[2 * g], uFace[2 * g + 1]}, outwardNorm(m, g, m.cC[d])) / m.vol[d] * timeStep, 0.);
+    }
+    for (int k = m.bptr[d]; k < m.bptr[d + 1]; ++k) {
+      const int g = m.bface[k];
+      coD += smax(dot(V2{uFace[2 * g], uFace[2 * g + 1]}, outwardNorm(m, g, m.cC[d])) / m.vol[d] * timeStep, 0.);
+    }
+    const double mg = std::sqrt(gd.x * gd.x + gd.y * gd.y), mr = std::sqrt(rc.x * rc.x + rc.y * rc.y);
+    const double c = std::abs(dot(V2{gd.x / mg, gd.y / mg}, V2{rc.x / mr, rc.y / mr}));
+    const double cos2 = c > 1. ? NAN : 2. * c * c - 1.;
+    const double psiF = smin(1. * (cos2 + 1.) / 2., 1.);
+    const double gFT = psiF * cicsamHc(gDT, coD) + (1. - psiF) * cicsamUq(gDT, coD);
+    const double b = (gFT - gDT) / (1. - gDT);
+    beta[f] = std::isfinite(b) ? smax(smin(1., b), 0.) : 0.;
+  }
+}
+
+// cicsam::div(u, gamma, beta, theta), Cicsam.cpp:87-138: per link the donor and
+// acceptor coefficients theta (1 - b) F and theta b F in that order, the
+// explicit part (1 - theta) F gamma0_f as a source; boundary links by type.
+void *orc_cicsam_div(const void *mesh, const double *uFace, const double *beta, double theta, const int *bcType,
+                     const double *gammaFace, const double *gamma0Cell, const double *gamma0Face, char *err,
+                     int errLen) {
+  const Mesh &m = *static_cast<const Mesh *>(mesh);
+  Eqn *e = new Eqn(m.nCells, 5);
+  for (int c = 0; c < m.nCells; ++c) {
+    const V2 cc = m.cC[c];
+    for (int k = m.iptr[c]; k < m.iptr[c + 1]; ++k) {
+      const int f = m.iface[k], nb = m.inb[k];
+      const double flux = dot(V2{uFace[2 * f], uFace[2 * f + 1]}, outwardNorm(m, f, cc));
+      const int d = flux > 0. ? c : nb, a = flux <= 0. ? c : nb;
+      const double b = beta[f];
+      e->addCoeff(c, d, theta * (1. - b) * flux);
+      e->addCoeff(c, a, theta * b * flux);
+      const double gF = (1. - b) * gamma0Cell[d] + b * gamma0Cell[a];
+      e->rhs[c] += (1. - theta) * flux * gF;
+    }
+    for (int k = m.bptr[c]; k < m.bptr[c + 1]; ++k) {
+      const int f = m.bface[k];
+      const double flux = dot(V2{uFace[2 * f], uFace[2 * f + 1]}, outwardNorm(m, f, cc));
+      const int bt = bcType ? bcType[f] : ORC_BC_NORMAL_GRADIENT;
+      if (bt == ORC_BC_FIXED) {
+        e->rhs[c] += theta * flux * gammaFace[f];
+        e->rhs[c] += (1. - theta) * flux * gamma0Face[f];
+      } else if (bt == ORC_BC_NORMAL_GRADIENT) {
+        e->addCoeff(c, c, theta * flux);
+        e->rhs[c] += (1. - theta) * flux * gamma0Cell[c];
+      } else if (bt == ORC_BC_SYMMETRY) {
+      } else {
+        setErr(err, errLen, "cicsam::div: unrecognized or unspecified boundary type.");
+        delete e;
+        return nullptr;
+      }
+    }
+  }
+  return e;
+}
+
+// cicsam::computeMomentumFlux, Cicsam.cpp:68-85: rho(g) u_f with the same face
+// value g the gamma equation used (interior) or the face value (boundary)
+void orc_cicsam_momentum_flux(const void *mesh, double rho1, double rho2, const double *uFace,
+                              const double *gammaCell, const double *gammaFace, const double *beta, double *rhoU) {
+  const Mesh &m = *static_cast<const Mesh *>(mesh);
+  for (int f = 0; f < m.nFaces; ++f) {
+    const int l = m.fL[f], r = m.fR[f];
+    double g;
+    if (r >= 0) {
+      const double flux = dot(V2{uFace[2 * f], uFace[2 * f + 1]}, outwardNorm(m, f, m.cC[l]));
+      const int d = flux > 0. ? l : r, a = flux <= 0. ? l : r;
+      const double b = beta[f];
+      g = (1. - b) * gammaCell[d] + b * gammaCell[a];
+    } else {
+      g = gammaFace[f];
+    }
+    const double rho = rho1 + sclamp(g, 0., 1.) * (rho2 - rho1);
+    rhoU[2 * f] = rho * uFace[2 * f];
+    rhoU[2 * f + 1] = rho * uFace[2 * f + 1];
+  }
+}
+
+// FractionalStepMultiphase::updateProperties, FractionalStepMultiphase.cpp:175-214:
+// density and (from the kinematic viscosities) dynamic viscosity on cells and faces
+void orc_vof_properties(const void *mesh, double rho1, double rho2, double mu1, double mu2,
+                        const double *gammaCell, const double *gammaFace, double *rhoCell, double *rhoFace,
+                        double *muCell, double *muFace) {
+  const Mesh &m = *static_cast<const Mesh *>(mesh);
+  for (int c = 0; c < m.nCells; ++c) {
+    rhoCell[c] = rho1 + sclamp(gammaCell[c], 0., 1.) * (rho2 - rho1);
+    muCell[c] = rhoCell[c] / (rho1 / mu1 + sclamp(gammaCell[c], 0., 1.) * (rho2 / mu2 - rho1 / mu1));
+  }
+  for (int f = 0; f < m.nFaces; ++f) {
+    rhoFace[f] = rho1 + sclamp(gammaFace[f], 0., 1.) * (rho2 - rho1);
+    muFace[f] = rhoFace[f] / (rho1 / mu1 + sclamp(gammaFace[f], 0., 1.) * (rho2 / mu2 - rho1 / mu1));
+  }
+}
+
 // src::div(u), Source.cpp:5-21
 void orc_src_div(const void *mesh, const double *uFace, double *out) {
   const Mesh &m = *static_cast<const Mesh *>(mesh);

@@ -75,6 +75,18 @@ void *orc_div(const void *mesh, int ncomp, const double *uFace,
               const double *phiFace, const double *phi0Cell,
               const double *phi0Face, char *err, int errLen);
 void orc_src_div(const void *mesh, const double *uFace, double *out);
+double orc_cicsam_hc(double g, double co);
+double orc_cicsam_uq(double g, double co);
+void orc_cicsam_weights(const void *mesh, const double *uFace, const double *gammaCell, const double *gradCell,
+                        double timeStep, double *beta);
+void *orc_cicsam_div(const void *mesh, const double *uFace, const double *beta, double theta, const int *bcType,
+                     const double *gammaFace, const double *gamma0Cell, const double *gamma0Face, char *err,
+                     int errLen);
+void orc_cicsam_momentum_flux(const void *mesh, double rho1, double rho2, const double *uFace,
+                              const double *gammaCell, const double *gammaFace, const double *beta, double *rhoU);
+void orc_vof_properties(const void *mesh, double rho1, double rho2, double mu1, double mu2,
+                        const double *gammaCell, const double *gammaFace, double *rhoCell, double *rhoFace,
+                        double *muCell, double *muFace);
 void orc_src_src(const void *mesh, int ncomp, const double *phiCell,
                  double *out);
 void orc_grad(const void *mesh, const double *phiCell, const double *phiFace,

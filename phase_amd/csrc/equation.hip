@@ -293,6 +293,152 @@ __global__ void kEqDiv(MeshDev M, int ncomp, const double *__restrict__ uF, cons
   }
 }
 
+// ---------------------------------------------------------------- VOF (CICSAM)
+// std::min / std::max as the reference calls them -- min(a, b) = (b < a) ? b : a,
+// max(a, b) = (a < b) ? b : a -- so a NaN first argument passes through
+__device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }
+__device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
+// clamp(v, lo, hi) = max(min(v, hi), lo), Math/Algorithm.h:11-13
+__device__ __forceinline__ double sclamp(double v, double lo, double hi) { return smax(smin(v, hi), lo); }
+// cicsam::hc and cicsam::uq, Cicsam.cpp:5-18
+__device__ __forceinline__ double cicsamHc(double g, double co) { return g >= 0 && g <= 1 ? smin(1., g / co) : g; }
+__device__ __forceinline__ double cicsamUq(double g, double co) {
+  return g >= 0 && g <= 1 ? smin((8. * co * g + (1. - co) * (6. * g + 3.)) / 8., cicsamHc(g, co)) : g;
+}
+
+// cicsam::faceInterpolationWeights, Cicsam.cpp:20-66 (k = 1), one face per
+// thread; boundary faces 0.  cos(2 theta_F) with theta_F = acos(c) is
+// evaluated as 2 c^2 - 1 (c > 1 from rounding stays NaN, as acos makes it).
+__global__ void kCicsamWeights(MeshDev M, int nFaces, const int *__restrict__ fl, const int *__restrict__ fr,
+                               const double *__restrict__ uF, const double *__restrict__ gC,
+                               const double *__restrict__ gradC, double dt, double *__restrict__ beta) {
+  for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < nFaces; f += gridDim.x * blockDim.x) {
+    const int l = fl[f], r = fr[f];
+    if (r < 0) {
+      beta[f] = 0.;
+      continue;
+    }
+    const Dv2 uf = ld2(uF, f);
+    const Dv2 s = outward(ld2(M.fc, f), ld2(M.sf, f), ld2(M.cc, l));
+    const double flux = uf.x * s.x + uf.y * s.y;
+    const int d = flux > 0. ? l : r, a = flux <= 0. ? l : r;
+    const Dv2 cd = ld2(M.cc, d), ca = ld2(M.cc, a);
+    const Dv2 rc{ca.x - cd.x, ca.y - cd.y};
+    const Dv2 gd = ld2(gradC, d);
+    const double gD = sclamp(gC[d], 0., 1.), gA = sclamp(gC[a], 0., 1.);
+    const double gU = sclamp(gA - 2. * (rc.x * gd.x + rc.y * gd.y), 0., 1.);
+    double gDT = (gD - gU) / (gA - gU);
+    if (!isfinite(gDT)) gDT = 0.;
+    // donor Courant number over all of the donor's faces
+    const double vd = M.vol[d];
+    double coD = 0.;
+    for (int k = M.iptr[d]; k < M.iptr[d + 1]; ++k) {
+      const int g = M.iface[k];
+      const Dv2 n = outward(ld2(M.fc, g), ld2(M.sf, g), cd);
+      const Dv2 ug = ld2(uF, g);
+      coD += smax((ug.x * n.x + ug.y * n.y) / vd * dt, 0.);
+    }
+    for (int k = M.bptr[d]; k < M.bptr[d + 1]; ++k) {
+      const int g = M.bface[k];
+      const Dv2 n = outward(ld2(M.fc, g), ld2(M.sf, g), cd);
+      const Dv2 ug = ld2(uF, g);
+      coD += smax((ug.x * n.x + ug.y * n.y) / vd * dt, 0.);
+    }
+    const double mg = sqrt(gd.x * gd.x + gd.y * gd.y), mr = sqrt(rc.x * rc.x + rc.y * rc.y);
+    const double c = fabs((gd.x / mg) * (rc.x / mr) + (gd.y / mg) * (rc.y / mr));
+    const double cos2 = c > 1. ? __builtin_nan("") : 2. * c * c - 1.;
+    const double psiF = smin(1. * (cos2 + 1.) / 2., 1.);
+    const double gFT = psiF * cicsamHc(gDT, coD) + (1. - psiF) * cicsamUq(gDT, coD);
+    const double b = (gFT - gDT) / (1. - gDT);
+    beta[f] = isfinite(b) ? smax(smin(1., b), 0.) : 0.;
+  }
+}
+
+// cicsam::div(u, gamma, beta, theta), Cicsam.cpp:87-138: per link the donor and
+// acceptor coefficients theta (1 - b) F and theta b F in that order, the
+// explicit part as a source, boundary links by type (err bit 1: other type)
+__global__ void kEqCicsamDiv(MeshDev M, const double *__restrict__ uF, const double *__restrict__ beta, double theta,
+                             const int *__restrict__ bc, const double *__restrict__ gF,
+                             const double *__restrict__ g0C, const double *__restrict__ g0F, int stride, int *cols,
+                             double *vals, int *cnt, double *rhs, int *err) {
+  const int N = M.nCells;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < N; c += gridDim.x * blockDim.x) {
+    RowB rb;
+    rb.init();
+    double src = 0.;
+    const Dv2 pc = ld2(M.cc, c);
+    for (int k = M.iptr[c]; k < M.iptr[c + 1]; ++k) {
+      const int f = M.iface[k], nb = M.inb[k];
+      const Dv2 s = outward(ld2(M.fc, f), ld2(M.sf, f), pc);
+      const double flux = uF[2 * f] * s.x + uF[2 * f + 1] * s.y;
+      const int d = flux > 0. ? c : nb, a = flux <= 0. ? c : nb;
+      const double b = beta[f];
+      rb.add(d, theta * (1. - b) * flux);
+      rb.add(a, theta * b * flux);
+      const double gFace = (1. - b) * g0C[d] + b * g0C[a];
+      src += (1. - theta) * flux * gFace;
+    }
+    for (int k = M.bptr[c]; k < M.bptr[c + 1]; ++k) {
+      const int f = M.bface[k];
+      const Dv2 s = outward(ld2(M.fc, f), ld2(M.sf, f), pc);
+      const double flux = uF[2 * f] * s.x + uF[2 * f + 1] * s.y;
+      const int bt = bc ? bc[f] : PHASE_HIP_BC_NORMAL_GRADIENT;
+      if (bt == PHASE_HIP_BC_FIXED) {
+        if (!gF || !g0F) {
+          atomicOr(err, 2);
+          continue;
+        }
+        src += theta * flux * gF[f];
+        src += (1. - theta) * flux * g0F[f];
+      } else if (bt == PHASE_HIP_BC_NORMAL_GRADIENT) {
+        rb.add(c, theta * flux);
+        src += (1. - theta) * flux * g0C[c];
+      } else if (bt == PHASE_HIP_BC_SYMMETRY) {
+      } else {
+        atomicOr(err, 1);
+      }
+    }
+    rb.store(c, stride, cols, vals, cnt, err);
+    rhs[c] = src;
+  }
+}
+
+// cicsam::computeMomentumFlux, Cicsam.cpp:68-85 (rhoU per face, interleaved)
+__global__ void kCicsamMomentumFlux(MeshDev M, int nFaces, const int *__restrict__ fl, const int *__restrict__ fr,
+                                    double rho1, double rho2, const double *__restrict__ uF,
+                                    const double *__restrict__ gC, const double *__restrict__ gF,
+                                    const double *__restrict__ beta, double *__restrict__ rhoU) {
+  for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < nFaces; f += gridDim.x * blockDim.x) {
+    const int l = fl[f], r = fr[f];
+    const Dv2 uf = ld2(uF, f);
+    double g;
+    if (r >= 0) {
+      const Dv2 s = outward(ld2(M.fc, f), ld2(M.sf, f), ld2(M.cc, l));
+      const double flux = uf.x * s.x + uf.y * s.y;
+      const int d = flux > 0. ? l : r, a = flux <= 0. ? l : r;
+      const double b = beta[f];
+      g = (1. - b) * gC[d] + b * gC[a];
+    } else {
+      g = gF[f];
+    }
+    const double rho = rho1 + sclamp(g, 0., 1.) * (rho2 - rho1);
+    rhoU[2 * f] = rho * uf.x;
+    rhoU[2 * f + 1] = rho * uf.y;
+  }
+}
+
+// FractionalStepMultiphase::updateProperties, FractionalStepMultiphase.cpp:175-214:
+// density, and dynamic viscosity from the kinematic viscosities, for n values
+__global__ void kVofProperties(int n, double rho1, double rho2, double mu1, double mu2, const double *__restrict__ g,
+                               double *__restrict__ rho, double *__restrict__ mu) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double gi = sclamp(g[i], 0., 1.);
+    const double r = rho1 + gi * (rho2 - rho1);
+    rho[i] = r;
+    mu[i] = r / (rho1 / mu1 + gi * (rho2 / mu2 - rho1 / mu1));
+  }
+}
+
 // lhs (+|-)= rhs, CrsEquation.cpp:185-275
 __global__ void kEqMerge(int rank, int stride, int sign, int *lc, double *lv, int *ln, double *lr,
                          const int *__restrict__ rc, const double *__restrict__ rv, const int *__restrict__ rn,
@@ -591,6 +737,23 @@ int phase_hip_eq_div(phase_hip_equation *e, const double *uFace, const double *u
   });
 }
 
+int phase_hip_eq_cicsam_div(phase_hip_equation *e, const double *uFace, const double *beta, double theta,
+                            const int *bcType, const double *gammaFace, const double *gamma0Cell,
+                            const double *gamma0Face) {
+  return guardedEq(e, [&] {
+    if (e->ncomp != 1) fail(PHASE_HIP_EINVAL, "cicsam::div builds a scalar equation");
+    if (!uFace || !beta || !gamma0Cell) fail(PHASE_HIP_EINVAL, "cicsam::div needs u, beta (faces) and gamma0 (cells)");
+    phase_hip_mesh *m = e->mesh;
+    PH_CHECK(hipSetDevice(m->device));
+    hipLaunchKernelGGL(kEqCicsamDiv, dim3(g256(m->nCells)), dim3(256), 0, m->stream, meshDev(m), uFace, beta, theta,
+                       bcType, gammaFace, gamma0Cell, gamma0Face, e->stride, e->cols.p, e->vals.p, e->cnt.p,
+                       e->rhs.p, e->dErr.p);
+    PH_CHECK(hipGetLastError());
+    fresh(e);
+    checkEq(e, "cicsam::div");
+  });
+}
+
 int phase_hip_eq_add(phase_hip_equation *e, const phase_hip_equation *o, int sign) {
   return guardedEq(e, [&] {
     if (!o || o->rank != e->rank || o->stride != e->stride || o->mesh != e->mesh)
@@ -743,6 +906,44 @@ int phase_hip_field_interpolate_faces(phase_hip_mesh *m, int ncomp, const double
     PH_CHECK(hipSetDevice(m->device));
     hipLaunchKernelGGL(kInterpFaces, dim3(g256(m->nFaces)), dim3(256), 0, m->stream, m->nFaces, ncomp, m->dFL.p,
                        m->dFR.p, m->dCC.p, m->dFC.p, m->dSf.p, bcType, phiCell, phiFace);
+    PH_CHECK(hipGetLastError());
+  });
+}
+
+int phase_hip_cicsam_weights(phase_hip_mesh *m, const double *uFace, const double *gammaCell,
+                             const double *gradGammaCell, double timeStep, double *beta) {
+  return guardedM(m, [&] {
+    if (!uFace || !gammaCell || !gradGammaCell || !beta) fail(PHASE_HIP_EINVAL, "null field");
+    PH_CHECK(hipSetDevice(m->device));
+    hipLaunchKernelGGL(kCicsamWeights, dim3(g256(m->nFaces)), dim3(256), 0, m->stream, meshDev(m), m->nFaces,
+                       m->dFL.p, m->dFR.p, uFace, gammaCell, gradGammaCell, timeStep, beta);
+    PH_CHECK(hipGetLastError());
+  });
+}
+
+int phase_hip_cicsam_momentum_flux(phase_hip_mesh *m, double rho1, double rho2, const double *uFace,
+                                   const double *gammaCell, const double *gammaFace, const double *beta,
+                                   double *rhoU) {
+  return guardedM(m, [&] {
+    if (!uFace || !gammaCell || !gammaFace || !beta || !rhoU) fail(PHASE_HIP_EINVAL, "null field");
+    PH_CHECK(hipSetDevice(m->device));
+    hipLaunchKernelGGL(kCicsamMomentumFlux, dim3(g256(m->nFaces)), dim3(256), 0, m->stream, meshDev(m), m->nFaces,
+                       m->dFL.p, m->dFR.p, rho1, rho2, uFace, gammaCell, gammaFace, beta, rhoU);
+    PH_CHECK(hipGetLastError());
+  });
+}
+
+int phase_hip_vof_properties(phase_hip_mesh *m, double rho1, double rho2, double mu1, double mu2,
+                             const double *gammaCell, const double *gammaFace, double *rhoCell, double *rhoFace,
+                             double *muCell, double *muFace) {
+  return guardedM(m, [&] {
+    if (!gammaCell || !gammaFace || !rhoCell || !rhoFace || !muCell || !muFace) fail(PHASE_HIP_EINVAL, "null field");
+    if (!(mu1 > 0.) || !(mu2 > 0.)) fail(PHASE_HIP_EINVAL, "viscosities must be positive");
+    PH_CHECK(hipSetDevice(m->device));
+    hipLaunchKernelGGL(kVofProperties, dim3(g256(m->nCells)), dim3(256), 0, m->stream, m->nCells, rho1, rho2, mu1,
+                       mu2, gammaCell, rhoCell, muCell);
+    hipLaunchKernelGGL(kVofProperties, dim3(g256(m->nFaces)), dim3(256), 0, m->stream, m->nFaces, rho1, rho2, mu1,
+                       mu2, gammaFace, rhoFace, muFace);
     PH_CHECK(hipGetLastError());
   });
 }

@@ -84,6 +84,11 @@ _SIGS = {
                                               _vp, _vp, _vp]),
     "phase_hip_eq_ddt": (ctypes.c_int, [_vp, ctypes.c_double, _vp]),
     "phase_hip_eq_div": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_double, _vp, _vp, _vp, _vp]),
+    "phase_hip_eq_cicsam_div": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_double, _vp, _vp, _vp, _vp]),
+    "phase_hip_cicsam_weights": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_double, _vp]),
+    "phase_hip_cicsam_momentum_flux": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, _vp, _vp, _vp, _vp,
+                                                      _vp]),
+    "phase_hip_vof_properties": (ctypes.c_int, [_vp] + [ctypes.c_double] * 4 + [_vp] * 6),
     "phase_hip_eq_add": (ctypes.c_int, [_vp, _vp, ctypes.c_int]),
     "phase_hip_eq_scale": (ctypes.c_int, [_vp, ctypes.c_double]),
     "phase_hip_eq_sub_vector": (ctypes.c_int, [_vp, _vp]),
@@ -284,6 +289,26 @@ class HipMesh:
                                                          grad_cell.ptr), "grad")
         self.synchronize()
 
+    # --- volume of fluid (Cicsam.cpp, FractionalStepMultiphase.cpp)
+    def cicsam_weights(self, u_face, gamma_cell, grad_gamma_cell, time_step, beta):
+        self._mcheck(load_library().phase_hip_cicsam_weights(self._h, u_face.ptr, gamma_cell.ptr,
+                                                             grad_gamma_cell.ptr, float(time_step), beta.ptr),
+                     "cicsam::faceInterpolationWeights")
+        self.synchronize()
+
+    def cicsam_momentum_flux(self, rho1, rho2, u_face, gamma_cell, gamma_face, beta, rho_u):
+        self._mcheck(load_library().phase_hip_cicsam_momentum_flux(self._h, float(rho1), float(rho2), u_face.ptr,
+                                                                   gamma_cell.ptr, gamma_face.ptr, beta.ptr,
+                                                                   rho_u.ptr), "cicsam::computeMomentumFlux")
+        self.synchronize()
+
+    def vof_properties(self, rho1, rho2, mu1, mu2, gamma_cell, gamma_face, rho_cell, rho_face, mu_cell, mu_face):
+        self._mcheck(load_library().phase_hip_vof_properties(self._h, float(rho1), float(rho2), float(mu1),
+                                                             float(mu2), gamma_cell.ptr, gamma_face.ptr,
+                                                             rho_cell.ptr, rho_face.ptr, mu_cell.ptr, mu_face.ptr),
+                     "updateProperties")
+        self.synchronize()
+
     def interpolate_faces(self, phi_cell, phi_face, bc_type=None, ncomp=1):
         self._mcheck(load_library().phase_hip_field_interpolate_faces(self._h, ncomp, phi_cell.ptr,
                                                                       _dp_or_none(bc_type), phi_face.ptr),
@@ -336,6 +361,13 @@ class HipEquation:
         d = _dp_or_none
         self._check(load_library().phase_hip_eq_div(self._h, u_face.ptr, u0_face.ptr, float(theta), d(bc_type),
                                                     d(phi_face), d(phi0_cell), d(phi0_face)), "fv::div")
+        return self
+
+    def cicsam_div(self, u_face, beta, theta, bc_type=None, gamma_face=None, gamma0_cell=None, gamma0_face=None):
+        d = _dp_or_none
+        self._check(load_library().phase_hip_eq_cicsam_div(self._h, u_face.ptr, beta.ptr, float(theta), d(bc_type),
+                                                           d(gamma_face), d(gamma0_cell), d(gamma0_face)),
+                    "cicsam::div")
         return self
 
     def __iadd__(self, other):
