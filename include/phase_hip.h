@@ -291,6 +291,18 @@ int phase_hip_eq_ddt(phase_hip_equation *e, double timeStep, const double *phi0C
 /* fv::div(u, phi, theta), upwind -- Divergence.h:7-53 (u, u0: face vectors) */
 int phase_hip_eq_div(phase_hip_equation *e, const double *uFace, const double *u0Face, double theta,
                      const int *bcType, const double *phiFace, const double *phi0Cell, const double *phi0Face);
+/* fv::divc(u, phi, theta), central -- Divergence.h:55-103 */
+int phase_hip_eq_divc(phase_hip_equation *e, const double *uFace, const double *u0Face, double theta,
+                      const int *bcType, const double *phiFace, const double *phi0Cell, const double *phi0Face);
+/* fv::dive(u, phi, theta), explicit (sources only) -- ExplicitDivergence.h:7-51;
+ * u0/u1 and phi0/phi1: the two older time levels */
+int phase_hip_eq_dive(phase_hip_equation *e, const double *u0Face, const double *u1Face, double theta,
+                      const int *bcType, const double *phi0Cell, const double *phi1Cell, const double *phi0Face,
+                      const double *phi1Face);
+/* fv::div(phiU, gradU, u) -- Divergence.cpp:3-47 (Vector2D equation; gradU a
+ * Jacobian per cell, xx xy yx yy) */
+int phase_hip_eq_div_grad(phase_hip_equation *e, const double *phiUFace, const double *gradUCell, const int *bcType,
+                          const double *uFace);
 /* cicsam::div(u, gamma, beta, theta) -- Cicsam.cpp:87-138 (scalar equation;
  * beta from phase_hip_cicsam_weights; gammaFace, gamma0Face: FIXED faces). */
 int phase_hip_eq_cicsam_div(phase_hip_equation *e, const double *uFace, const double *beta, double theta,

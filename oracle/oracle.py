@@ -57,6 +57,11 @@ def _load():
         "orc_div": (_vp, [_vp, ctypes.c_int, _f64p, _f64p, ctypes.c_double, _vp, _vp, _vp, _vp,
                           ctypes.c_char_p, ctypes.c_int]),
         "orc_src_div": (None, [_vp, _f64p, _f64p]),
+        "orc_divc": (_vp, [_vp, ctypes.c_int, _f64p, _f64p, ctypes.c_double, _vp, _vp, _vp, _vp,
+                           ctypes.c_char_p, ctypes.c_int]),
+        "orc_dive": (_vp, [_vp, ctypes.c_int, _f64p, _f64p, ctypes.c_double, _vp, _vp, _vp, _vp, _vp,
+                           ctypes.c_char_p, ctypes.c_int]),
+        "orc_div_grad": (_vp, [_vp, _f64p, _f64p, _vp, _vp, ctypes.c_char_p, ctypes.c_int]),
         "orc_cicsam_hc": (ctypes.c_double, [ctypes.c_double, ctypes.c_double]),
         "orc_cicsam_uq": (ctypes.c_double, [ctypes.c_double, ctypes.c_double]),
         "orc_cicsam_weights": (None, [_vp, _f64p, _f64p, _f64p, ctypes.c_double, _f64p]),
@@ -278,6 +283,45 @@ def div(mesh, u_face, u0_face=None, theta=1.0, bc_type=None, phi_face=None, phi0
     if not h:
         raise RuntimeError(err.value.decode())
     return Eqn(h)
+
+
+def _eqn_or_raise(h, err):
+    if not h:
+        raise RuntimeError(err.value.decode())
+    return Eqn(h)
+
+
+def divc(mesh, u_face, u0_face=None, theta=1.0, bc_type=None, phi_face=None, phi0_cell=None, phi0_face=None,
+         ncomp=1):
+    """fv::divc (central)."""
+    u_face = _f64(u_face)
+    u0_face = u_face if u0_face is None else _f64(u0_face)
+    z = lambda n: np.zeros(ncomp * n)
+    phi_face = z(mesh.n_faces) if phi_face is None else phi_face
+    phi0_cell = z(mesh.n_cells) if phi0_cell is None else phi0_cell
+    phi0_face = z(mesh.n_faces) if phi0_face is None else phi0_face
+    bc_type, phi_face, phi0_cell, phi0_face = (_i32(bc_type), _f64(phi_face), _f64(phi0_cell), _f64(phi0_face))
+    err = ctypes.create_string_buffer(256)
+    return _eqn_or_raise(lib().orc_divc(mesh._h, ncomp, u_face, u0_face, float(theta), _ptr(bc_type),
+                                        _ptr(phi_face), _ptr(phi0_cell), _ptr(phi0_face), err, 256), err)
+
+
+def dive(mesh, u0_face, u1_face, theta, phi0_cell, phi1_cell, phi0_face, phi1_face, bc_type=None, ncomp=1):
+    """fv::dive (explicit: sources only)."""
+    a = [_f64(x) for x in (phi0_cell, phi1_cell, phi0_face, phi1_face)]
+    bc_type = _i32(bc_type)
+    err = ctypes.create_string_buffer(256)
+    return _eqn_or_raise(lib().orc_dive(mesh._h, ncomp, _f64(u0_face), _f64(u1_face), float(theta),
+                                        _ptr(bc_type), *[_ptr(x) for x in a], err, 256), err)
+
+
+def div_grad(mesh, phiu_face, grad_u, bc_type=None, u_face=None):
+    """fv::div(phiU, gradU, u) (Vector2D; grad_u: xx, xy, yx, yy per cell)."""
+    u_face = np.zeros(2 * mesh.n_faces) if u_face is None else u_face
+    bc_type, u_face = _i32(bc_type), _f64(u_face)
+    err = ctypes.create_string_buffer(256)
+    return _eqn_or_raise(lib().orc_div_grad(mesh._h, _f64(phiu_face), _f64(grad_u), _ptr(bc_type),
+                                            _ptr(u_face), err, 256), err)
 
 
 def cicsam_hc(g, co):

@@ -506,6 +506,152 @@ void *orc_div(const void *mesh, int ncomp, const double *uFace, const double *u0
   return e;
 }
 
+// fv::divc(u, phi, theta), central, Divergence.h:55-103.  The boundary
+// flux0 uses u, not u0, as the reference does.
+void *orc_divc(const void *mesh, int ncomp, const double *uFace, const double *u0Face, double theta,
+               const int *bcType, const double *phiFace, const double *phi0Cell, const double *phi0Face, char *err,
+               int errLen) {
+  const Mesh &m = *static_cast<const Mesh *>(mesh);
+  Eqn *e = new Eqn(ncomp * m.nCells, 5);
+  for (int c = 0; c < m.nCells; ++c) {
+    const V2 cc = m.cC[c];
+    double s[2];
+    for (int k = m.iptr[c]; k < m.iptr[c + 1]; ++k) {
+      const int f = m.iface[k], nb = m.inb[k];
+      const V2 sf = outwardNorm(m, f, cc);
+      const double flux = theta * dot(V2{uFace[2 * f], uFace[2 * f + 1]}, sf);
+      const double flux0 = (1. - theta) * dot(V2{u0Face[2 * f], u0Face[2 * f + 1]}, sf);
+      const double lc = mag(m.fC[f] - cc), ln = mag(m.fC[f] - m.cC[nb]);
+      const double g = ln / (lc + ln);
+      addS(*e, m, ncomp, c, c, g * flux);
+      addS(*e, m, ncomp, c, nb, (1. - g) * flux);
+      for (int q = 0; q < ncomp; ++q)
+        s[q] = flux0 * (g * phi0Cell[ncomp * c + q] + (1. - g) * phi0Cell[ncomp * nb + q]);
+      addSrc(*e, m, ncomp, c, s);
+    }
+    for (int k = m.bptr[c]; k < m.bptr[c + 1]; ++k) {
+      const int f = m.bface[k];
+      const V2 sf = outwardNorm(m, f, cc);
+      const double flux = theta * dot(V2{uFace[2 * f], uFace[2 * f + 1]}, sf);
+      const double flux0 = (1. - theta) * dot(V2{uFace[2 * f], uFace[2 * f + 1]}, sf);
+      const int bt = bcType ? bcType[f] : ORC_BC_NORMAL_GRADIENT;
+      if (bt == ORC_BC_FIXED) {
+        for (int q = 0; q < ncomp; ++q) s[q] = flux * phiFace[ncomp * f + q];
+        addSrc(*e, m, ncomp, c, s);
+        for (int q = 0; q < ncomp; ++q) s[q] = flux0 * phi0Face[ncomp * f + q];
+        addSrc(*e, m, ncomp, c, s);
+      } else if (bt == ORC_BC_NORMAL_GRADIENT) {
+        addS(*e, m, ncomp, c, c, flux);
+        for (int q = 0; q < ncomp; ++q) s[q] = flux0 * phi0Cell[ncomp * c + q];
+        addSrc(*e, m, ncomp, c, s);
+      } else if (bt == ORC_BC_SYMMETRY) {
+      } else {
+        setErr(err, errLen, "fv::divc: unrecognized or unspecified boundary type.");
+        delete e;
+        return nullptr;
+      }
+    }
+  }
+  return e;
+}
+
+// fv::dive(u, phi, theta), explicit, ExplicitDivergence.h:7-51: sources only,
+// from the two older levels (u0, phi0) and (u1, phi1)
+void *orc_dive(const void *mesh, int ncomp, const double *u0Face, const double *u1Face, double theta,
+               const int *bcType, const double *phi0Cell, const double *phi1Cell, const double *phi0Face,
+               const double *phi1Face, char *err, int errLen) {
+  const Mesh &m = *static_cast<const Mesh *>(mesh);
+  Eqn *e = new Eqn(ncomp * m.nCells, 5);
+  for (int c = 0; c < m.nCells; ++c) {
+    const V2 cc = m.cC[c];
+    double s[2];
+    for (int k = m.iptr[c]; k < m.iptr[c + 1]; ++k) {
+      const int f = m.iface[k], nb = m.inb[k];
+      const V2 sf = outwardNorm(m, f, cc);
+      const double flux0 = dot(V2{u0Face[2 * f], u0Face[2 * f + 1]}, sf);
+      const double flux1 = dot(V2{u1Face[2 * f], u1Face[2 * f + 1]}, sf);
+      for (int q = 0; q < ncomp; ++q) s[q] = theta * std::max(flux0, 0.) * phi0Cell[ncomp * c + q];
+      addSrc(*e, m, ncomp, c, s);
+      for (int q = 0; q < ncomp; ++q) s[q] = theta * std::min(flux0, 0.) * phi0Cell[ncomp * nb + q];
+      addSrc(*e, m, ncomp, c, s);
+      for (int q = 0; q < ncomp; ++q) s[q] = (1. - theta) * std::max(flux1, 0.) * phi1Cell[ncomp * c + q];
+      addSrc(*e, m, ncomp, c, s);
+      for (int q = 0; q < ncomp; ++q) s[q] = (1. - theta) * std::min(flux1, 0.) * phi1Cell[ncomp * nb + q];
+      addSrc(*e, m, ncomp, c, s);
+    }
+    for (int k = m.bptr[c]; k < m.bptr[c + 1]; ++k) {
+      const int f = m.bface[k];
+      const V2 sf = outwardNorm(m, f, cc);
+      const double flux0 = dot(V2{u0Face[2 * f], u0Face[2 * f + 1]}, sf);
+      const double flux1 = dot(V2{u1Face[2 * f], u1Face[2 * f + 1]}, sf);
+      const int bt = bcType ? bcType[f] : ORC_BC_NORMAL_GRADIENT;
+      if (bt == ORC_BC_FIXED || bt == ORC_BC_NORMAL_GRADIENT) {
+        for (int q = 0; q < ncomp; ++q) s[q] = theta * flux0 * phi0Face[ncomp * f + q];
+        addSrc(*e, m, ncomp, c, s);
+        for (int q = 0; q < ncomp; ++q) s[q] = (1. - theta) * flux1 * phi1Face[ncomp * f + q];
+        addSrc(*e, m, ncomp, c, s);
+      } else if (bt == ORC_BC_SYMMETRY) {
+      } else {
+        setErr(err, errLen, "fv::dive: unrecognized or unspecified boundary type.");
+        delete e;
+        return nullptr;
+      }
+    }
+  }
+  return e;
+}
+
+// fv::div(phiU, gradU, u), Divergence.cpp:3-47: upwind with the upwind cell's
+// gradient (a Jacobian per cell, xx xy yx yy; dot(J, r) = (xx r.x + xy r.y,
+// yx r.x + yy r.y), Tensor2D.cpp:48-50) carried to the face; Vector2D only
+void *orc_div_grad(const void *mesh, const double *phiUFace, const double *gradU, const int *bcType,
+                   const double *uFace, char *err, int errLen) {
+  const Mesh &m = *static_cast<const Mesh *>(mesh);
+  Eqn *e = new Eqn(2 * m.nCells, 5);
+  auto jdot = [&](int c, V2 r, double flux, double *s) {
+    const double *J = gradU + 4 * c;
+    const double vx = J[0] * r.x + J[1] * r.y, vy = J[2] * r.x + J[3] * r.y;
+    s[0] = flux * vx;
+    s[1] = flux * vy;
+  };
+  for (int c = 0; c < m.nCells; ++c) {
+    const V2 cc = m.cC[c];
+    double s[2];
+    for (int k = m.iptr[c]; k < m.iptr[c + 1]; ++k) {
+      const int f = m.iface[k], nb = m.inb[k];
+      const double flux = dot(V2{phiUFace[2 * f], phiUFace[2 * f + 1]}, outwardNorm(m, f, cc));
+      if (flux > 0.) {
+        addS(*e, m, 2, c, c, flux);
+        jdot(c, m.fC[f] - cc, flux, s);
+      } else {
+        addS(*e, m, 2, c, nb, flux);
+        jdot(nb, m.fC[f] - m.cC[nb], flux, s);
+      }
+      addSrc(*e, m, 2, c, s);
+    }
+    for (int k = m.bptr[c]; k < m.bptr[c + 1]; ++k) {
+      const int f = m.bface[k];
+      const double flux = dot(V2{phiUFace[2 * f], phiUFace[2 * f + 1]}, outwardNorm(m, f, cc));
+      const int bt = bcType ? bcType[f] : ORC_BC_NORMAL_GRADIENT;
+      if (bt == ORC_BC_FIXED) {
+        s[0] = flux * uFace[2 * f];
+        s[1] = flux * uFace[2 * f + 1];
+        addSrc(*e, m, 2, c, s);
+      } else if (bt == ORC_BC_NORMAL_GRADIENT) {
+        addS(*e, m, 2, c, c, flux);
+        jdot(c, m.fC[f] - cc, flux, s);
+        addSrc(*e, m, 2, c, s);
+      } else if (bt == ORC_BC_SYMMETRY) {
+      } else {
+        setErr(err, errLen, "fv::div: unrecognized or unspecified boundary type.");
+        delete e;
+        return nullptr;
+      }
+    }
+  }
+  return e;
+}
+
 // ---------------------------------------------------------------- VOF (CICSAM)
 namespace {
 // std::min / std::max as the reference calls them -- min(a, b) = (b < a) ? b : a,

@@ -75,6 +75,14 @@ void *orc_div(const void *mesh, int ncomp, const double *uFace,
               const double *phiFace, const double *phi0Cell,
               const double *phi0Face, char *err, int errLen);
 void orc_src_div(const void *mesh, const double *uFace, double *out);
+void *orc_divc(const void *mesh, int ncomp, const double *uFace, const double *u0Face, double theta,
+               const int *bcType, const double *phiFace, const double *phi0Cell, const double *phi0Face, char *err,
+               int errLen);
+void *orc_dive(const void *mesh, int ncomp, const double *u0Face, const double *u1Face, double theta,
+               const int *bcType, const double *phi0Cell, const double *phi1Cell, const double *phi0Face,
+               const double *phi1Face, char *err, int errLen);
+void *orc_div_grad(const void *mesh, const double *phiUFace, const double *gradU, const int *bcType,
+                   const double *uFace, char *err, int errLen);
 double orc_cicsam_hc(double g, double co);
 double orc_cicsam_uq(double g, double co);
 void orc_cicsam_weights(const void *mesh, const double *uFace, const double *gammaCell, const double *gradCell,

@@ -293,6 +293,182 @@ __global__ void kEqDiv(MeshDev M, int ncomp, const double *__restrict__ uF, cons
   }
 }
 
+// fv::divc(u, phi, theta), central, Divergence.h:55-103 (boundary flux0 from u,
+// as the reference computes it)
+__global__ void kEqDivc(MeshDev M, int ncomp, const double *__restrict__ uF, const double *__restrict__ u0F,
+                        double theta, const int *__restrict__ bc, const double *__restrict__ phiF,
+                        const double *__restrict__ phi0C, const double *__restrict__ phi0F, int stride, int *cols,
+                        double *vals, int *cnt, double *rhs, int *err) {
+  const int N = M.nCells;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < N; c += gridDim.x * blockDim.x) {
+    RowB rb[2];
+    rb[0].init();
+    rb[1].init();
+    double src[2] = {0., 0.};
+    auto addS = [&](int nb, double v) {
+#pragma unroll
+      for (int q = 0; q < 2; ++q)
+        if (q < ncomp) rb[q].add(q * N + nb, v);
+    };
+    const Dv2 pc = ld2(M.cc, c);
+    for (int k = M.iptr[c]; k < M.iptr[c + 1]; ++k) {
+      const int f = M.iface[k], nb = M.inb[k];
+      const Dv2 fc = ld2(M.fc, f), pn = ld2(M.cc, nb);
+      const Dv2 s = outward(fc, ld2(M.sf, f), pc);
+      const double flux = theta * (uF[2 * f] * s.x + uF[2 * f + 1] * s.y);
+      const double flux0 = (1. - theta) * (u0F[2 * f] * s.x + u0F[2 * f + 1] * s.y);
+      const double ax = fc.x - pc.x, ay = fc.y - pc.y, bx = fc.x - pn.x, by = fc.y - pn.y;
+      const double lc = sqrt(ax * ax + ay * ay), ln = sqrt(bx * bx + by * by);
+      const double g = ln / (lc + ln);
+      addS(c, g * flux);
+      addS(nb, (1. - g) * flux);
+      for (int q = 0; q < 2; ++q)
+        if (q < ncomp) src[q] += flux0 * (g * phi0C[ncomp * c + q] + (1. - g) * phi0C[ncomp * nb + q]);
+    }
+    for (int k = M.bptr[c]; k < M.bptr[c + 1]; ++k) {
+      const int f = M.bface[k];
+      const Dv2 s = outward(ld2(M.fc, f), ld2(M.sf, f), pc);
+      const double fl = uF[2 * f] * s.x + uF[2 * f + 1] * s.y;
+      const double flux = theta * fl, flux0 = (1. - theta) * fl;
+      const int bt = bc ? bc[f] : PHASE_HIP_BC_NORMAL_GRADIENT;
+      if (bt == PHASE_HIP_BC_FIXED) {
+        if (!phiF || !phi0F) {
+          atomicOr(err, 2);
+          continue;
+        }
+        for (int q = 0; q < 2; ++q) if (q < ncomp) src[q] += flux * phiF[ncomp * f + q];
+        for (int q = 0; q < 2; ++q) if (q < ncomp) src[q] += flux0 * phi0F[ncomp * f + q];
+      } else if (bt == PHASE_HIP_BC_NORMAL_GRADIENT) {
+        addS(c, flux);
+        for (int q = 0; q < 2; ++q) if (q < ncomp) src[q] += flux0 * phi0C[ncomp * c + q];
+      } else if (bt == PHASE_HIP_BC_SYMMETRY) {
+      } else {
+        atomicOr(err, 1);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+      if (q < ncomp) {
+        rb[q].store(q * N + c, stride, cols, vals, cnt, err);
+        rhs[q * N + c] = src[q];
+      }
+  }
+}
+
+// fv::dive(u, phi, theta), ExplicitDivergence.h:7-51: sources only, from
+// (u0, phi0) and (u1, phi1); the rows stay empty
+__global__ void kEqDive(MeshDev M, int ncomp, const double *__restrict__ u0F, const double *__restrict__ u1F,
+                        double theta, const int *__restrict__ bc, const double *__restrict__ phi0C,
+                        const double *__restrict__ phi1C, const double *__restrict__ phi0F,
+                        const double *__restrict__ phi1F, int stride, int *cols, double *vals, int *cnt,
+                        double *rhs, int *err) {
+  const int N = M.nCells;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < N; c += gridDim.x * blockDim.x) {
+    double src[2] = {0., 0.};
+    const Dv2 pc = ld2(M.cc, c);
+    for (int k = M.iptr[c]; k < M.iptr[c + 1]; ++k) {
+      const int f = M.iface[k], nb = M.inb[k];
+      const Dv2 s = outward(ld2(M.fc, f), ld2(M.sf, f), pc);
+      const double flux0 = u0F[2 * f] * s.x + u0F[2 * f + 1] * s.y;
+      const double flux1 = u1F[2 * f] * s.x + u1F[2 * f + 1] * s.y;
+      for (int q = 0; q < 2; ++q) if (q < ncomp) src[q] += theta * upMax(flux0) * phi0C[ncomp * c + q];
+      for (int q = 0; q < 2; ++q) if (q < ncomp) src[q] += theta * upMin(flux0) * phi0C[ncomp * nb + q];
+      for (int q = 0; q < 2; ++q) if (q < ncomp) src[q] += (1. - theta) * upMax(flux1) * phi1C[ncomp * c + q];
+      for (int q = 0; q < 2; ++q) if (q < ncomp) src[q] += (1. - theta) * upMin(flux1) * phi1C[ncomp * nb + q];
+    }
+    for (int k = M.bptr[c]; k < M.bptr[c + 1]; ++k) {
+      const int f = M.bface[k];
+      const Dv2 s = outward(ld2(M.fc, f), ld2(M.sf, f), pc);
+      const double flux0 = u0F[2 * f] * s.x + u0F[2 * f + 1] * s.y;
+      const double flux1 = u1F[2 * f] * s.x + u1F[2 * f + 1] * s.y;
+      const int bt = bc ? bc[f] : PHASE_HIP_BC_NORMAL_GRADIENT;
+      if (bt == PHASE_HIP_BC_FIXED || bt == PHASE_HIP_BC_NORMAL_GRADIENT) {
+        for (int q = 0; q < 2; ++q) if (q < ncomp) src[q] += theta * flux0 * phi0F[ncomp * f + q];
+        for (int q = 0; q < 2; ++q) if (q < ncomp) src[q] += (1. - theta) * flux1 * phi1F[ncomp * f + q];
+      } else if (bt == PHASE_HIP_BC_SYMMETRY) {
+      } else {
+        atomicOr(err, 1);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+      if (q < ncomp) {
+        const int row = q * N + c;
+        for (int j = 0; j < stride; ++j) {
+          cols[(size_t)row * stride + j] = -1;
+          vals[(size_t)row * stride + j] = 0.;
+        }
+        cnt[row] = 0;
+        rhs[row] = src[q];
+      }
+  }
+}
+
+// fv::div(phiU, gradU, u), Divergence.cpp:3-47 (Vector2D): upwind with the
+// upwind cell's Jacobian (xx xy yx yy) carried to the face centroid
+__global__ void kEqDivGrad(MeshDev M, const double *__restrict__ phiUF, const double *__restrict__ J,
+                           const int *__restrict__ bc, const double *__restrict__ uF, int stride, int *cols,
+                           double *vals, int *cnt, double *rhs, int *err) {
+  const int N = M.nCells;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < N; c += gridDim.x * blockDim.x) {
+    RowB rb[2];
+    rb[0].init();
+    rb[1].init();
+    double src[2] = {0., 0.};
+    auto addS = [&](int nb, double v) {
+      rb[0].add(nb, v);
+      rb[1].add(N + nb, v);
+    };
+    auto jsrc = [&](int cell, double rx, double ry, double flux) {
+      const double *j = J + 4 * cell;
+      const double vx = j[0] * rx + j[1] * ry, vy = j[2] * rx + j[3] * ry;
+      src[0] += flux * vx;
+      src[1] += flux * vy;
+    };
+    const Dv2 pc = ld2(M.cc, c);
+    for (int k = M.iptr[c]; k < M.iptr[c + 1]; ++k) {
+      const int f = M.iface[k], nb = M.inb[k];
+      const Dv2 fc = ld2(M.fc, f);
+      const Dv2 s = outward(fc, ld2(M.sf, f), pc);
+      const double flux = phiUF[2 * f] * s.x + phiUF[2 * f + 1] * s.y;
+      if (flux > 0.) {
+        addS(c, flux);
+        jsrc(c, fc.x - pc.x, fc.y - pc.y, flux);
+      } else {
+        const Dv2 pn = ld2(M.cc, nb);
+        addS(nb, flux);
+        jsrc(nb, fc.x - pn.x, fc.y - pn.y, flux);
+      }
+    }
+    for (int k = M.bptr[c]; k < M.bptr[c + 1]; ++k) {
+      const int f = M.bface[k];
+      const Dv2 fc = ld2(M.fc, f);
+      const Dv2 s = outward(fc, ld2(M.sf, f), pc);
+      const double flux = phiUF[2 * f] * s.x + phiUF[2 * f + 1] * s.y;
+      const int bt = bc ? bc[f] : PHASE_HIP_BC_NORMAL_GRADIENT;
+      if (bt == PHASE_HIP_BC_FIXED) {
+        if (!uF) {
+          atomicOr(err, 2);
+          continue;
+        }
+        src[0] += flux * uF[2 * f];
+        src[1] += flux * uF[2 * f + 1];
+      } else if (bt == PHASE_HIP_BC_NORMAL_GRADIENT) {
+        addS(c, flux);
+        jsrc(c, fc.x - pc.x, fc.y - pc.y, flux);
+      } else if (bt == PHASE_HIP_BC_SYMMETRY) {
+      } else {
+        atomicOr(err, 1);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      rb[q].store(q * N + c, stride, cols, vals, cnt, err);
+      rhs[q * N + c] = src[q];
+    }
+  }
+}
+
 // ---------------------------------------------------------------- VOF (CICSAM)
 // std::min / std::max as the reference calls them -- min(a, b) = (b < a) ? b : a,
 // max(a, b) = (a < b) ? b : a -- so a NaN first argument passes through
@@ -731,6 +907,53 @@ int phase_hip_eq_div(phase_hip_equation *e, const double *uFace, const double *u
     hipLaunchKernelGGL(kEqDiv, dim3(g256(m->nCells)), dim3(256), 0, m->stream, meshDev(m), e->ncomp, uFace, u0Face,
                        theta, bcType, phiFace, phi0Cell, phi0Face, e->stride, e->cols.p, e->vals.p, e->cnt.p,
                        e->rhs.p, e->dErr.p);
+    PH_CHECK(hipGetLastError());
+    fresh(e);
+    checkEq(e, "fv::div");
+  });
+}
+
+int phase_hip_eq_divc(phase_hip_equation *e, const double *uFace, const double *u0Face, double theta,
+                      const int *bcType, const double *phiFace, const double *phi0Cell, const double *phi0Face) {
+  return guardedEq(e, [&] {
+    if (!uFace || !u0Face || !phi0Cell) fail(PHASE_HIP_EINVAL, "fv::divc needs u, u0 (faces) and phi0 (cells)");
+    phase_hip_mesh *m = e->mesh;
+    PH_CHECK(hipSetDevice(m->device));
+    hipLaunchKernelGGL(kEqDivc, dim3(g256(m->nCells)), dim3(256), 0, m->stream, meshDev(m), e->ncomp, uFace, u0Face,
+                       theta, bcType, phiFace, phi0Cell, phi0Face, e->stride, e->cols.p, e->vals.p, e->cnt.p,
+                       e->rhs.p, e->dErr.p);
+    PH_CHECK(hipGetLastError());
+    fresh(e);
+    checkEq(e, "fv::divc");
+  });
+}
+
+int phase_hip_eq_dive(phase_hip_equation *e, const double *u0Face, const double *u1Face, double theta,
+                      const int *bcType, const double *phi0Cell, const double *phi1Cell, const double *phi0Face,
+                      const double *phi1Face) {
+  return guardedEq(e, [&] {
+    if (!u0Face || !u1Face || !phi0Cell || !phi1Cell || !phi0Face || !phi1Face)
+      fail(PHASE_HIP_EINVAL, "fv::dive needs u0, u1, phi0, phi1 (cells and faces)");
+    phase_hip_mesh *m = e->mesh;
+    PH_CHECK(hipSetDevice(m->device));
+    hipLaunchKernelGGL(kEqDive, dim3(g256(m->nCells)), dim3(256), 0, m->stream, meshDev(m), e->ncomp, u0Face, u1Face,
+                       theta, bcType, phi0Cell, phi1Cell, phi0Face, phi1Face, e->stride, e->cols.p, e->vals.p,
+                       e->cnt.p, e->rhs.p, e->dErr.p);
+    PH_CHECK(hipGetLastError());
+    fresh(e);
+    checkEq(e, "fv::dive");
+  });
+}
+
+int phase_hip_eq_div_grad(phase_hip_equation *e, const double *phiUFace, const double *gradUCell, const int *bcType,
+                          const double *uFace) {
+  return guardedEq(e, [&] {
+    if (e->ncomp != 2) fail(PHASE_HIP_EINVAL, "fv::div(phiU, gradU, u) builds a Vector2D equation");
+    if (!phiUFace || !gradUCell) fail(PHASE_HIP_EINVAL, "fv::div needs phiU (faces) and gradU (cells)");
+    phase_hip_mesh *m = e->mesh;
+    PH_CHECK(hipSetDevice(m->device));
+    hipLaunchKernelGGL(kEqDivGrad, dim3(g256(m->nCells)), dim3(256), 0, m->stream, meshDev(m), phiUFace, gradUCell,
+                       bcType, uFace, e->stride, e->cols.p, e->vals.p, e->cnt.p, e->rhs.p, e->dErr.p);
     PH_CHECK(hipGetLastError());
     fresh(e);
     checkEq(e, "fv::div");

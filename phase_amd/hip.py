@@ -85,6 +85,9 @@ _SIGS = {
     "phase_hip_eq_ddt": (ctypes.c_int, [_vp, ctypes.c_double, _vp]),
     "phase_hip_eq_div": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_double, _vp, _vp, _vp, _vp]),
     "phase_hip_eq_cicsam_div": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_double, _vp, _vp, _vp, _vp]),
+    "phase_hip_eq_divc": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_double, _vp, _vp, _vp, _vp]),
+    "phase_hip_eq_dive": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_double, _vp, _vp, _vp, _vp, _vp]),
+    "phase_hip_eq_div_grad": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "phase_hip_cicsam_weights": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_double, _vp]),
     "phase_hip_cicsam_momentum_flux": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, _vp, _vp, _vp, _vp,
                                                       _vp]),
@@ -361,6 +364,23 @@ class HipEquation:
         d = _dp_or_none
         self._check(load_library().phase_hip_eq_div(self._h, u_face.ptr, u0_face.ptr, float(theta), d(bc_type),
                                                     d(phi_face), d(phi0_cell), d(phi0_face)), "fv::div")
+        return self
+
+    def divc(self, u_face, u0_face, theta=1.0, bc_type=None, phi_face=None, phi0_cell=None, phi0_face=None):
+        d = _dp_or_none
+        self._check(load_library().phase_hip_eq_divc(self._h, u_face.ptr, u0_face.ptr, float(theta), d(bc_type),
+                                                     d(phi_face), d(phi0_cell), d(phi0_face)), "fv::divc")
+        return self
+
+    def dive(self, u0_face, u1_face, theta, phi0_cell, phi1_cell, phi0_face, phi1_face, bc_type=None):
+        self._check(load_library().phase_hip_eq_dive(self._h, u0_face.ptr, u1_face.ptr, float(theta),
+                                                     _dp_or_none(bc_type), phi0_cell.ptr, phi1_cell.ptr,
+                                                     phi0_face.ptr, phi1_face.ptr), "fv::dive")
+        return self
+
+    def div_grad(self, phiu_face, grad_u, bc_type=None, u_face=None):
+        self._check(load_library().phase_hip_eq_div_grad(self._h, phiu_face.ptr, grad_u.ptr, _dp_or_none(bc_type),
+                                                         _dp_or_none(u_face)), "fv::div")
         return self
 
     def cicsam_div(self, u_face, beta, theta, bc_type=None, gamma_face=None, gamma0_cell=None, gamma0_face=None):

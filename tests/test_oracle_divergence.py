@@ -1,0 +1,93 @@
+"""The remaining convection operators of the oracle pinned by closed forms:
+fv::divc (central) is conservative and reproduces src::div on a uniform field;
+fv::dive (explicit) is theta c0 div(u0) + (1 - theta) c1 div(u1) on uniform
+old fields; fv::div(phiU, gradU, u) with the upwind cell's gradient is exact
+for a linear field (each row equals sum_f F_f u(x_f))."""
+import numpy as np
+import pytest
+
+from phase_amd import meshgen as G
+
+
+@pytest.fixture(scope="module")
+def tri(oracle):
+    xy, cptr, cind = G.delaunay_square(12, G.delaunay_cells_for(500, 12), seed=8)
+    return oracle.Mesh(xy, cptr, cind)
+
+
+def _boundary_flux(m, u):
+    """Outward flux per boundary face and its lCell."""
+    f = m.faces()
+    _, cc = m.cells()
+    bnd = np.where(f["rcell"] < 0)[0]
+    l = f["lcell"][bnd]
+    sf = f["sf"][bnd]
+    sign = np.where(np.sum((f["centroid"][bnd] - cc[l]) * sf, axis=1) > 0, 1.0, -1.0)
+    return bnd, l, np.sum(u.reshape(-1, 2)[bnd] * sf, axis=1) * sign
+
+
+def _apply(e, x):
+    rp, ci, v, rhs = e.get()
+    rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    keep = ci >= 0
+    y = np.zeros(len(rp) - 1)
+    np.add.at(y, rows[keep], v[keep] * x[ci[keep]])
+    return y, rhs
+
+
+def test_divc_conserves_and_uniform(oracle, tri):
+    m = tri
+    rng = np.random.default_rng(1)
+    u = rng.standard_normal(2 * m.n_faces)
+    phi = rng.standard_normal(m.n_cells)
+    y, rhs = _apply(oracle.divc(m, u, theta=1.0), phi)
+    bnd, l, fb = _boundary_flux(m, u)
+    scale = np.abs(y).sum()
+    assert abs(y.sum() + rhs.sum() - np.sum(fb * phi[l])) <= 1e-13 * scale
+    y1, _ = _apply(oracle.divc(m, u, theta=1.0), np.full(m.n_cells, 2.5))
+    assert np.allclose(y1, 2.5 * oracle.src_div(m, u), rtol=0, atol=1e-13 * np.abs(u).max())
+
+
+def test_dive_uniform_old_fields(oracle, tri):
+    m = tri
+    rng = np.random.default_rng(2)
+    u0, u1 = rng.standard_normal(2 * m.n_faces), rng.standard_normal(2 * m.n_faces)
+    N, F = m.n_cells, m.n_faces
+    theta = 0.3
+    f = m.faces()
+    bc = np.where(f["rcell"] < 0, oracle.BC_FIXED, oracle.BC_NORMAL_GRADIENT).astype(np.int32)
+    e = oracle.dive(m, u0, u1, theta, np.full(N, 2.0), np.full(N, -3.0), np.full(F, 2.0), np.full(F, -3.0),
+                    bc_type=bc)
+    rp, ci, v, rhs = e.get()
+    assert np.all(ci == -1)  # sources only
+    expect = theta * 2.0 * oracle.src_div(m, u0) + (1 - theta) * -3.0 * oracle.src_div(m, u1)
+    assert np.allclose(rhs, expect, rtol=0, atol=1e-13 * (np.abs(u0).max() + np.abs(u1).max()))
+
+
+@pytest.mark.parametrize("fixed", [True, False])
+def test_div_grad_exact_for_linear_fields(oracle, tri, fixed):
+    m = tri
+    _, cc = m.cells()
+    f = m.faces()
+    a = np.array([0.3, -1.1])
+    B = np.array([[1.7, -0.4], [2.3, 0.9]])  # not symmetric: a transposed dot would show
+    lin = lambda x: a + x @ B.T
+    rng = np.random.default_rng(3)
+    phiu = rng.standard_normal(2 * m.n_faces)
+    grad = np.tile(B.reshape(-1), m.n_cells)
+    bc = np.full(m.n_faces, oracle.BC_FIXED if fixed else oracle.BC_NORMAL_GRADIENT, np.int32)
+    uf = lin(f["centroid"]).reshape(-1)
+    e = oracle.div_grad(m, phiu, grad, bc_type=bc, u_face=uf)
+    N = m.n_cells
+    ucell = lin(cc)
+    x = np.r_[ucell[:, 0], ucell[:, 1]]  # IndexMap block layout
+    y, rhs = _apply(e, x)
+    # sum over every face of the cell of F_f u(x_f)
+    expect = np.zeros((N, 2))
+    n_out = lambda k, c: f["sf"][k] * (1.0 if np.dot(f["centroid"][k] - cc[c], f["sf"][k]) > 0 else -1.0)
+    for k in range(m.n_faces):
+        for c in (f["lcell"][k], f["rcell"][k]):
+            if c >= 0:
+                expect[c] += np.dot(phiu.reshape(-1, 2)[k], n_out(k, c)) * lin(f["centroid"][k])
+    got = (y + rhs).reshape(2, N).T
+    assert np.abs(got - expect).max() <= 1e-12 * np.abs(expect).max()
