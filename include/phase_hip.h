@@ -288,6 +288,12 @@ int phase_hip_eq_laplacian(phase_hip_equation *e, int variant, double gamma, con
                            const double *phi0Cell, const double *phi0Face);
 /* fv::ddt(field, timeStep) -- TimeDerivative.h:37-48 */
 int phase_hip_eq_ddt(phase_hip_equation *e, double timeStep, const double *phi0Cell);
+/* fv::ddt(Scalar rho, field, timeStep) -- TimeDerivative.h:7-19 (rhoCell NULL)
+ * and fv::ddt(rho field, field, timeStep) -- :21-35 (rhoCell, rho0Cell: the
+ * density now and at the old time); cellMask (nonzero = in the group, may be
+ * NULL) is ddt(field, timeStep, cells) -- :50-62 */
+int phase_hip_eq_ddt_rho(phase_hip_equation *e, double timeStep, double rho, const double *rhoCell,
+                         const double *rho0Cell, const double *phi0Cell, const int *cellMask);
 /* fv::div(u, phi, theta), upwind -- Divergence.h:7-53 (u, u0: face vectors) */
 int phase_hip_eq_div(phase_hip_equation *e, const double *uFace, const double *u0Face, double theta,
                      const int *bcType, const double *phiFace, const double *phi0Cell, const double *phi0Face);

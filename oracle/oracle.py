@@ -62,6 +62,7 @@ def _load():
         "orc_dive": (_vp, [_vp, ctypes.c_int, _f64p, _f64p, ctypes.c_double, _vp, _vp, _vp, _vp, _vp,
                            ctypes.c_char_p, ctypes.c_int]),
         "orc_div_grad": (_vp, [_vp, _f64p, _f64p, _vp, _vp, ctypes.c_char_p, ctypes.c_int]),
+        "orc_ddt_rho": (_vp, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _vp, _vp, _f64p, _vp]),
         "orc_cicsam_hc": (ctypes.c_double, [ctypes.c_double, ctypes.c_double]),
         "orc_cicsam_uq": (ctypes.c_double, [ctypes.c_double, ctypes.c_double]),
         "orc_cicsam_weights": (None, [_vp, _f64p, _f64p, _f64p, ctypes.c_double, _f64p]),
@@ -283,6 +284,13 @@ def div(mesh, u_face, u0_face=None, theta=1.0, bc_type=None, phi_face=None, phi0
     if not h:
         raise RuntimeError(err.value.decode())
     return Eqn(h)
+
+
+def ddt_rho(mesh, time_step, phi0_cell, rho=1.0, rho_cell=None, rho0_cell=None, cell_mask=None, ncomp=1):
+    """fv::ddt with a scalar or a field density, optionally over a cell group."""
+    rho_cell, rho0_cell, cell_mask = _f64(rho_cell), _f64(rho0_cell), _i32(cell_mask)
+    return Eqn(lib().orc_ddt_rho(mesh._h, ncomp, float(time_step), float(rho), _ptr(rho_cell), _ptr(rho0_cell),
+                                 _f64(phi0_cell), _ptr(cell_mask)))
 
 
 def _eqn_or_raise(h, err):

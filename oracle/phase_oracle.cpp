@@ -458,6 +458,29 @@ void *orc_ddt(const void *mesh, int ncomp, double timeStep, const double *phi0Ce
   return e;
 }
 
+// fv::ddt(Scalar rho, field, dt) (TimeDerivative.h:7-19; rhoCell == null) and
+// fv::ddt(rho field, field, dt) (:21-35: rho for the coefficient, rho0 for
+// the source); cellMask (optional) restricts the rows to a cell group as
+// ddt(field, dt, cells) does (:50-62)
+void *orc_ddt_rho(const void *mesh, int ncomp, double timeStep, double rho, const double *rhoCell,
+                  const double *rho0Cell, const double *phi0Cell, const int *cellMask) {
+  const Mesh &m = *static_cast<const Mesh *>(mesh);
+  Eqn *e = new Eqn(ncomp * m.nCells, 5);
+  for (int c = 0; c < m.nCells; ++c) {
+    if (cellMask && !cellMask[c]) continue;
+    double s[2];
+    if (rhoCell) {
+      addS(*e, m, ncomp, c, c, rhoCell[c] * m.vol[c] / timeStep);
+      for (int q = 0; q < ncomp; ++q) s[q] = -rho0Cell[c] * m.vol[c] * phi0Cell[ncomp * c + q] / timeStep;
+    } else {
+      addS(*e, m, ncomp, c, c, rho * m.vol[c] / timeStep);
+      for (int q = 0; q < ncomp; ++q) s[q] = -rho * m.vol[c] * phi0Cell[ncomp * c + q] / timeStep;
+    }
+    addSrc(*e, m, ncomp, c, s);
+  }
+  return e;
+}
+
 // fv::div(u, phi, theta) upwind, Divergence.h:7-53
 void *orc_div(const void *mesh, int ncomp, const double *uFace, const double *u0Face,
               double theta, const int *bcType, const double *phiFace, const double *phi0Cell,

@@ -83,6 +83,8 @@ void *orc_dive(const void *mesh, int ncomp, const double *u0Face, const double *
                const double *phi1Face, char *err, int errLen);
 void *orc_div_grad(const void *mesh, const double *phiUFace, const double *gradU, const int *bcType,
                    const double *uFace, char *err, int errLen);
+void *orc_ddt_rho(const void *mesh, int ncomp, double timeStep, double rho, const double *rhoCell,
+                  const double *rho0Cell, const double *phi0Cell, const int *cellMask);
 double orc_cicsam_hc(double g, double co);
 double orc_cicsam_uq(double g, double co);
 void orc_cicsam_weights(const void *mesh, const double *uFace, const double *gammaCell, const double *gradCell,

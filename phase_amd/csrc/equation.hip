@@ -233,6 +233,41 @@ __global__ void kEqDdt(MeshDev M, int ncomp, double dt, const double *__restrict
   }
 }
 
+// fv::ddt with a density: scalar rho (TimeDerivative.h:7-19) or a field rho with
+// rho0 for the source (:21-35); mask (optional) keeps the rows of a cell
+// group only (:50-62), the others stay empty
+__global__ void kEqDdtRho(MeshDev M, int ncomp, double dt, double rho, const double *__restrict__ rhoC,
+                          const double *__restrict__ rho0C, const double *__restrict__ phi0C,
+                          const int *__restrict__ mask, int stride, int *cols, double *vals, int *cnt, double *rhs) {
+  const int N = M.nCells;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < N; c += gridDim.x * blockDim.x) {
+    const bool in = !mask || mask[c];
+    for (int q = 0; q < ncomp; ++q) {
+      const size_t row = (size_t)q * N + c;
+      for (int j = 1; j < stride; ++j) {
+        cols[row * stride + j] = -1;
+        vals[row * stride + j] = 0.;
+      }
+      if (!in) {
+        cols[row * stride] = -1;
+        vals[row * stride] = 0.;
+        cnt[row] = 0;
+        rhs[row] = 0.;
+        continue;
+      }
+      cols[row * stride] = (int)row;
+      cnt[row] = 1;
+      if (rhoC) {
+        vals[row * stride] = rhoC[c] * M.vol[c] / dt;
+        rhs[row] = -rho0C[c] * M.vol[c] * phi0C[ncomp * c + q] / dt;
+      } else {
+        vals[row * stride] = rho * M.vol[c] / dt;
+        rhs[row] = -rho * M.vol[c] * phi0C[ncomp * c + q] / dt;
+      }
+    }
+  }
+}
+
 __device__ __forceinline__ double upMax(double a) { return a < 0. ? 0. : a; }  // std::max(a, 0.)
 __device__ __forceinline__ double upMin(double a) { return 0. < a ? 0. : a; }  // std::min(a, 0.)
 
@@ -893,6 +928,21 @@ int phase_hip_eq_ddt(phase_hip_equation *e, double timeStep, const double *phi0C
     PH_CHECK(hipSetDevice(m->device));
     hipLaunchKernelGGL(kEqDdt, dim3(g256(m->nCells)), dim3(256), 0, m->stream, meshDev(m), e->ncomp, timeStep,
                        phi0Cell, e->stride, e->cols.p, e->vals.p, e->cnt.p, e->rhs.p);
+    PH_CHECK(hipGetLastError());
+    fresh(e);
+  });
+}
+
+int phase_hip_eq_ddt_rho(phase_hip_equation *e, double timeStep, double rho, const double *rhoCell,
+                         const double *rho0Cell, const double *phi0Cell, const int *cellMask) {
+  return guardedEq(e, [&] {
+    if (!phi0Cell) fail(PHASE_HIP_EINVAL, "fv::ddt needs the old-time field");
+    if (!rhoCell != !rho0Cell) fail(PHASE_HIP_EINVAL, "a density field needs its old-time values too");
+    phase_hip_mesh *m = e->mesh;
+    PH_CHECK(hipSetDevice(m->device));
+    hipLaunchKernelGGL(kEqDdtRho, dim3(g256(m->nCells)), dim3(256), 0, m->stream, meshDev(m), e->ncomp, timeStep,
+                       rho, rhoCell, rho0Cell, phi0Cell, cellMask, e->stride, e->cols.p, e->vals.p, e->cnt.p,
+                       e->rhs.p);
     PH_CHECK(hipGetLastError());
     fresh(e);
   });

@@ -85,6 +85,7 @@ _SIGS = {
     "phase_hip_eq_ddt": (ctypes.c_int, [_vp, ctypes.c_double, _vp]),
     "phase_hip_eq_div": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_double, _vp, _vp, _vp, _vp]),
     "phase_hip_eq_cicsam_div": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_double, _vp, _vp, _vp, _vp]),
+    "phase_hip_eq_ddt_rho": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, _vp, _vp, _vp, _vp]),
     "phase_hip_eq_divc": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_double, _vp, _vp, _vp, _vp]),
     "phase_hip_eq_dive": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_double, _vp, _vp, _vp, _vp, _vp]),
     "phase_hip_eq_div_grad": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
@@ -364,6 +365,12 @@ class HipEquation:
         d = _dp_or_none
         self._check(load_library().phase_hip_eq_div(self._h, u_face.ptr, u0_face.ptr, float(theta), d(bc_type),
                                                     d(phi_face), d(phi0_cell), d(phi0_face)), "fv::div")
+        return self
+
+    def ddt_rho(self, time_step, phi0_cell, rho=1.0, rho_cell=None, rho0_cell=None, cell_mask=None):
+        d = _dp_or_none
+        self._check(load_library().phase_hip_eq_ddt_rho(self._h, float(time_step), float(rho), d(rho_cell),
+                                                        d(rho0_cell), phi0_cell.ptr, d(cell_mask)), "fv::ddt")
         return self
 
     def divc(self, u_face, u0_face, theta=1.0, bc_type=None, phi_face=None, phi0_cell=None, phi0_face=None):

@@ -90,6 +90,22 @@ def test_ddt_matches_oracle(hip, oracle, meshes, ncomp):
     check_eq(he, oe)
 
 
+@pytest.mark.parametrize("form", ["scalar", "field", "group"])
+@pytest.mark.parametrize("ncomp", [1, 2])
+def test_ddt_density_matches_oracle(hip, oracle, meshes, form, ncomp):
+    om, hm, bc = meshes
+    fl = fields(om, 4)
+    phi0 = fl["s0c"] if ncomp == 1 else fl["v0c"]
+    rho, rho0 = np.exp(fl["s1c"]), np.exp(fl["s0c"])
+    mask = (np.arange(om.n_cells) % 3 != 1).astype(np.int32)
+    kw = dict(scalar=dict(rho=1.5), field=dict(rho_cell=rho, rho0_cell=rho0),
+              group=dict(rho=0.75, cell_mask=mask))[form]
+    oe = oracle.ddt_rho(om, 0.0125, phi0, ncomp=ncomp, **kw)
+    dkw = {k: (hip.DeviceArray(v) if isinstance(v, np.ndarray) else v) for k, v in kw.items()}
+    he = hip.HipEquation(hm, ncomp).ddt_rho(0.0125, hip.DeviceArray(phi0), **dkw)
+    check_eq(he, oe)
+
+
 @pytest.mark.parametrize("theta", [1.0, 0.5, 0.0])
 @pytest.mark.parametrize("ncomp", [1, 2])
 def test_div_matches_oracle(hip, oracle, meshes, theta, ncomp):

@@ -91,3 +91,28 @@ def test_div_grad_exact_for_linear_fields(oracle, tri, fixed):
                 expect[c] += np.dot(phiu.reshape(-1, 2)[k], n_out(k, c)) * lin(f["centroid"][k])
     got = (y + rhs).reshape(2, N).T
     assert np.abs(got - expect).max() <= 1e-12 * np.abs(expect).max()
+
+
+def test_ddt_density_forms(oracle):
+    """fv::ddt with rho = 1 is the plain form bit for bit; a field density puts
+    rho on the diagonal and rho0 in the source (dyadic values: exact); a cell
+    group leaves the other rows empty."""
+    xy, cptr, cind = G.rectilinear(8, 8)
+    m = oracle.Mesh(xy, cptr, cind)
+    rng = np.random.default_rng(4)
+    phi0 = rng.standard_normal(2 * m.n_cells)
+    a = oracle.ddt(m, 0.5, phi0, ncomp=2).get()
+    b = oracle.ddt_rho(m, 0.5, phi0, rho=1.0, ncomp=2).get()
+    assert all(np.array_equal(x, y) for x, y in zip(a, b))
+    rho = np.full(m.n_cells, 2.0)
+    rho0 = np.full(m.n_cells, 4.0)
+    phi = np.full(m.n_cells, 0.25)
+    rp, ci, v, rhs = oracle.ddt_rho(m, 0.5, phi, rho_cell=rho, rho0_cell=rho0).get()
+    diag = v[rp[:-1]]
+    assert np.all(ci[rp[:-1]] == np.arange(m.n_cells))
+    assert np.all(diag == 2.0 * (1 / 64) / 0.5) and np.all(rhs == -4.0 * (1 / 64) * 0.25 / 0.5)
+    mask = (np.arange(m.n_cells) % 3 == 0).astype(np.int32)
+    rp, ci, v, rhs = oracle.ddt_rho(m, 0.5, phi, rho=3.0, cell_mask=mask).get()
+    rows = np.repeat(np.arange(m.n_cells), np.diff(rp))
+    assert set(rows[ci >= 0]) == set(np.where(mask == 1)[0])
+    assert np.all(rhs[mask == 0] == 0.0)
