@@ -320,6 +320,13 @@ int phase_hip_eq_cicsam_div(phase_hip_equation *e, const double *uFace, const do
  * ==(Scalar) :299-303; implicit under-relaxation a_P /= alpha,
  * rhs -= (1 - alpha) a_P phi (ScalarFiniteVolumeEquation.cpp:45-55); copy. */
 int phase_hip_eq_add(phase_hip_equation *e, const phase_hip_equation *o, int sign);
+/* operator*(ScalarFiniteVolumeField, FiniteVolumeEquation<T>) -- FiniteVolumeEquation.h:100-102,
+ * VectorFiniteVolumeEquation.cpp:164-170: scale(cell, field(cell)) of every
+ * cell (CrsEquation::scaleRow, CrsEquation.cpp:153-159; values and rhs);
+ * remove(cell) -- ScalarFiniteVolumeEquation.cpp:38-43, VectorFiniteVolumeEquation.cpp:78-91:
+ * the masked cells' rows lose their columns and rhs. */
+int phase_hip_eq_scale_cells(phase_hip_equation *e, const double *field);
+int phase_hip_eq_remove_cells(phase_hip_equation *e, const int *cellMask);
 int phase_hip_eq_scale(phase_hip_equation *e, double s);
 int phase_hip_eq_sub_vector(phase_hip_equation *e, const double *v);
 int phase_hip_eq_eq_scalar(phase_hip_equation *e, double s);

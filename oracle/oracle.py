@@ -51,6 +51,8 @@ def _load():
         "orc_eqn_eq_scalar": (None, [_vp, ctypes.c_double]),
         "orc_eqn_scale": (None, [_vp, ctypes.c_double]),
         "orc_eqn_relax": (None, [_vp, ctypes.c_double, _f64p]),
+        "orc_eqn_scale_cells": (None, [_vp, ctypes.c_int, _f64p]),
+        "orc_eqn_remove_cells": (None, [_vp, ctypes.c_int, _i32p]),
         "orc_laplacian": (_vp, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _vp, _vp,
                                 ctypes.c_double, _vp, _vp, _vp, _vp, ctypes.c_char_p, ctypes.c_int]),
         "orc_ddt": (_vp, [_vp, ctypes.c_int, ctypes.c_double, _f64p]),
@@ -230,6 +232,15 @@ class Eqn:
 
     def scale(self, s):
         lib().orc_eqn_scale(self._h, float(s))
+        return self
+
+    def scale_cells(self, n_cells, field):
+        """operator*(ScalarFiniteVolumeField, equation): each cell's rows times field(cell)."""
+        lib().orc_eqn_scale_cells(self._h, int(n_cells), _f64(field))
+        return self
+
+    def remove_cells(self, n_cells, mask):
+        lib().orc_eqn_remove_cells(self._h, int(n_cells), _i32(mask))
         return self
 
     def relax(self, alpha, phi_rows):

@@ -334,6 +334,29 @@ void orc_eqn_scale(void *eqn, double s) {
   for (double &v : e.vals) v *= s;
   for (double &v : e.rhs) v *= s;
 }
+// operator*(ScalarFiniteVolumeField, FiniteVolumeEquation<T>)
+// (FiniteVolumeEquation.h:100-102, VectorFiniteVolumeEquation.cpp:164-170):
+// scale(cell, field(cell)) = CrsEquation::scaleRow (CrsEquation.cpp:153-159)
+// of every row of the cell, values and rhs; rows are comp * nCells + cell.
+void orc_eqn_scale_cells(void *eqn, int nCells, const double *field) {
+  Eqn &e = *static_cast<Eqn *>(eqn);
+  for (int row = 0; row < e.rank(); ++row) {
+    const double f = field[row % nCells];
+    for (int j = e.rowPtr[row]; j < e.rowPtr[row + 1]; ++j) e.vals[j] *= f;
+    e.rhs[row] *= f;
+  }
+}
+// remove(cell) (ScalarFiniteVolumeEquation.cpp:38-43,
+// VectorFiniteVolumeEquation.cpp:78-91): the cell's rows lose their columns
+// (-1, values kept) and their rhs
+void orc_eqn_remove_cells(void *eqn, int nCells, const int *mask) {
+  Eqn &e = *static_cast<Eqn *>(eqn);
+  for (int row = 0; row < e.rank(); ++row) {
+    if (!mask[row % nCells]) continue;
+    for (int j = e.rowPtr[row]; j < e.rowPtr[row + 1]; ++j) e.colInd[j] = -1;
+    e.rhs[row] = 0.;
+  }
+}
 // Implicit under-relaxation as preserved in the comment of
 // ScalarFiniteVolumeEquation.cpp:45-55 / VectorFiniteVolumeEquation.cpp:129-146:
 //   a_P /= alpha;  rhs -= (1 - alpha) * a_P * phi_old   (new a_P)

@@ -679,6 +679,28 @@ __global__ void kEqScale(int rank, int stride, double s, double *vals, double *r
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < rank; i += gridDim.x * blockDim.x) rhs[i] *= s;
 }
 
+// operator*(ScalarFiniteVolumeField, equation): every slot and the rhs of row
+// r times field[r mod nCells] (scaleRow, CrsEquation.cpp:153-159)
+__global__ void kEqScaleCells(int rank, int nCells, int stride, const double *__restrict__ field, double *vals,
+                              double *rhs) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < (size_t)rank * stride;
+       i += (size_t)gridDim.x * blockDim.x)
+    vals[i] *= field[(i / stride) % nCells];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < rank; i += gridDim.x * blockDim.x)
+    rhs[i] *= field[i % nCells];
+}
+
+// remove(cell): the masked cells' rows keep their values but lose their
+// columns (-1) and rhs
+__global__ void kEqRemoveCells(int rank, int nCells, int stride, const int *__restrict__ mask, int *cols,
+                               double *rhs) {
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < rank; r += gridDim.x * blockDim.x) {
+    if (!mask[r % nCells]) continue;
+    for (int j = 0; j < stride; ++j) cols[(size_t)r * stride + j] = -1;
+    rhs[r] = 0.;
+  }
+}
+
 // rhs -= v (operator-=(Vector) / ==(Vector), CrsEquation.cpp:282-285, 308-311);
 // v == nullptr: rhs -= s (operator==(Scalar), :299-303)
 __global__ void kEqSubRhs(int rank, const double *__restrict__ v, double s, double *rhs) {
@@ -1059,6 +1081,24 @@ int phase_hip_eq_scale(phase_hip_equation *e, double s) {
   return guardedEq(e, [&] {
     hipLaunchKernelGGL(kEqScale, dim3(g256((long)e->rank * e->stride)), dim3(256), 0, e->mesh->stream, e->rank,
                        e->stride, s, e->vals.p, e->rhs.p);
+    PH_CHECK(hipGetLastError());
+  });
+}
+
+int phase_hip_eq_scale_cells(phase_hip_equation *e, const double *field) {
+  return guardedEq(e, [&] {
+    if (!field) fail(PHASE_HIP_EINVAL, "null field");
+    hipLaunchKernelGGL(kEqScaleCells, dim3(g256((long)e->rank * e->stride)), dim3(256), 0, e->mesh->stream, e->rank,
+                       e->mesh->nCells, e->stride, field, e->vals.p, e->rhs.p);
+    PH_CHECK(hipGetLastError());
+  });
+}
+
+int phase_hip_eq_remove_cells(phase_hip_equation *e, const int *cellMask) {
+  return guardedEq(e, [&] {
+    if (!cellMask) fail(PHASE_HIP_EINVAL, "null mask");
+    hipLaunchKernelGGL(kEqRemoveCells, dim3(g256(e->rank)), dim3(256), 0, e->mesh->stream, e->rank,
+                       e->mesh->nCells, e->stride, cellMask, e->cols.p, e->rhs.p);
     PH_CHECK(hipGetLastError());
   });
 }

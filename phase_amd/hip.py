@@ -86,6 +86,8 @@ _SIGS = {
     "phase_hip_eq_div": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_double, _vp, _vp, _vp, _vp]),
     "phase_hip_eq_cicsam_div": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_double, _vp, _vp, _vp, _vp]),
     "phase_hip_eq_ddt_rho": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, _vp, _vp, _vp, _vp]),
+    "phase_hip_eq_scale_cells": (ctypes.c_int, [_vp, _vp]),
+    "phase_hip_eq_remove_cells": (ctypes.c_int, [_vp, _vp]),
     "phase_hip_eq_divc": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_double, _vp, _vp, _vp, _vp]),
     "phase_hip_eq_dive": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_double, _vp, _vp, _vp, _vp, _vp]),
     "phase_hip_eq_div_grad": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
@@ -403,6 +405,15 @@ class HipEquation:
 
     def __isub__(self, other):
         self._check(load_library().phase_hip_eq_add(self._h, other._h, -1), "operator-=")
+        return self
+
+    def scale_cells(self, field):
+        """operator*(ScalarFiniteVolumeField, equation)."""
+        self._check(load_library().phase_hip_eq_scale_cells(self._h, field.ptr), "operator*")
+        return self
+
+    def remove_cells(self, mask):
+        self._check(load_library().phase_hip_eq_remove_cells(self._h, mask.ptr), "remove")
         return self
 
     def scale(self, s):

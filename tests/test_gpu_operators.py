@@ -192,6 +192,28 @@ def test_algebra_matches_oracle(hip, oracle, meshes):
     check_eq(he, oe)
 
 
+def test_scale_cells_and_remove_match_oracle(hip, oracle, meshes):
+    """rho * (ddt + laplacian) with a cell field, then remove(cell) on a
+    cell group and a merge that drops the removed columns."""
+    om, hm, bc = meshes
+    fl = fields(om, 12)
+    N = om.n_cells
+    rho = np.exp(fl["s1c"])
+    mask = (np.arange(N) % 5 == 2).astype(np.int32)
+    oe = oracle.ddt(om, 0.02, fl["v0c"], ncomp=2)
+    oe += oracle.laplacian(om, 1, ncomp=2, gamma=0.4, bc_type=bc, phi_face=fl["v1f"])
+    oe.scale_cells(N, rho)
+    oe.remove_cells(N, mask)
+    d = dev(hip, v0c=fl["v0c"], bc=bc, v1f=fl["v1f"], rho=rho, mask=mask)
+    he = hip.HipEquation(hm, 2).ddt(0.02, d["v0c"])
+    he += hip.HipEquation(hm, 2).laplacian(1, gamma=0.4, bc_type=d["bc"], phi_face=d["v1f"])
+    he.scale_cells(d["rho"]).remove_cells(d["mask"])
+    check_eq(he, oe)
+    oe += oracle.ddt(om, 0.02, fl["v0c"], ncomp=2)
+    he += hip.HipEquation(hm, 2).ddt(0.02, d["v0c"])
+    check_eq(he, oe)
+
+
 def test_unsupported_boundary_raises(hip, oracle, meshes):
     om, hm, bc = meshes
     bad = bc.copy()

@@ -85,3 +85,36 @@ def test_relax_restored_formula(oracle):
     rp, ci, v, rhs = e.get()
     assert list(v) == [8.0, -1.0, 4.0, 0.0]
     assert list(rhs) == [-4.0, -6.0]
+
+
+def test_scale_cells_and_remove(oracle):
+    """operator*(field, equation) with a constant field is operator*=(Scalar)
+    bit for bit, and scales each cell's rows (both components) by its own
+    value otherwise; remove(cell) empties the columns and rhs of the cell's
+    rows and nothing else."""
+    from phase_amd import meshgen as G
+
+    xy, cptr, cind = G.rectilinear(6, 5)
+    m = oracle.Mesh(xy, cptr, cind)
+    N = m.n_cells
+    rng = np.random.default_rng(2)
+    mk = lambda: oracle.laplacian(m, 1, ncomp=2, gamma=0.3, bc_type=np.zeros(m.n_faces, np.int32),
+                                  phi_face=rng.standard_normal(2 * m.n_faces))
+    a, b = mk(), None
+    b = a.copy()
+    a.scale(1.75)
+    b.scale_cells(N, np.full(N, 1.75))
+    assert all(np.array_equal(x, y) for x, y in zip(a.get(), b.get()))
+    f = np.arange(1, N + 1, dtype=float)
+    c = mk()
+    rp0, ci0, v0, r0 = c.get()
+    c.scale_cells(N, f)
+    rp, ci, v, r = c.get()
+    rows = np.repeat(np.arange(2 * N), np.diff(rp))
+    assert np.array_equal(v, v0 * f[rows % N]) and np.array_equal(r, r0 * f[np.arange(2 * N) % N])
+    mask = (np.arange(N) % 4 == 1).astype(np.int32)
+    c.remove_cells(N, mask)
+    rp2, ci2, v2, r2 = c.get()
+    gone = mask[rows % N] == 1
+    assert np.all(ci2[gone] == -1) and np.array_equal(ci2[~gone], ci[~gone]) and np.array_equal(v2, v)
+    assert np.all(r2[mask[np.arange(2 * N) % N] == 1] == 0.0)
