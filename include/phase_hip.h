@@ -360,6 +360,11 @@ int phase_hip_field_grad(phase_hip_mesh *m, const double *phiCell, const double 
                          double *gradCell);
 int phase_hip_field_interpolate_faces(phase_hip_mesh *m, int ncomp, const double *phiCell, const int *bcType,
                                       double *phiFace);
+/* VectorFiniteVolumeField::faceToCell -- VectorFiniteVolumeField.cpp:5-26
+ * (weights NULL) and its weighted form faceToCell(cellWeight, faceWeight) --
+ * :28-51 (both weights given; one without the other is PHASE_HIP_EINVAL). */
+int phase_hip_field_face_to_cell(phase_hip_mesh *m, const double *faceVec, const double *cellWeight,
+                                 const double *faceWeight, double *cellVec);
 /* Volume of fluid, device fields:
  * cicsam::faceInterpolationWeights(u, gamma, gradGamma, timeStep) --
  *   Cicsam.cpp:20-66 (beta per face, 0 on boundary faces);
@@ -399,6 +404,37 @@ const char *phase_hip_fstep_last_error(const phase_hip_fstep *s);
 int phase_hip_fstep_setup(phase_hip_fstep *s, const char *key, const char *value);
 int phase_hip_fstep_advance(phase_hip_fstep *s, double dt, int nSteps, double *history);
 int phase_hip_fstep_fields(phase_hip_fstep *s, double *uCell, double *pCell, double *uFace);
+
+/* ------------------------------------------------------------------ multiphase fractional step
+ * FractionalStepMultiphase::solve(timeStep) -- src/2D/Unstructured/Solvers/
+ * FractionalStepMultiphase.cpp:58-218, one time step on the device without
+ * the surface-tension force (fst = 0; the reference's SurfaceTensionForce
+ * models are not part of this library):
+ *   gamma: ddt(gamma) + cicsam::div(u, gamma, beta, 0.5) == 0 (:76-100)
+ *   rho, mu from gamma; sg_f = dot(g, -x_f) grad rho_f; sg = faceToCell (:175-210)
+ *   rho ddt(u) + rho dive(u, u, 0.5) == laplacian(mu, u, 0.5) + src(sg - gradP),
+ *     u += dt / rho gradP, momentum-interpolated faces (:102-150)
+ *   laplacian(dt / rho, p) == src::div(u); gradP = faceToCell(rho, rho) (:152-163)
+ *   u -= dt / rho gradP on cells and every face (:165-173)
+ * Phase 1 (gamma = 0) has rho1, mu1; phase 2 (gamma = 1) rho2, mu2; (gx, gy)
+ * the gravity vector.  Boundary types per face: uBcType (required), pBcType
+ * and gammaBcType (NULL: zero gradient everywhere); uFace0, pFace0 boundary
+ * values; gammaCell0 the initial volume fraction; pinCell as for
+ * phase_hip_fstep_create.  Setup keys gamma.<k>, u.<k>, p.<k> go to the three
+ * solvers.  history (optional, 4 per step): gamma, momentum and pressure
+ * iterations, max |src::div(u_f)| after the correction. */
+typedef struct phase_hip_multiphase phase_hip_multiphase;
+
+int phase_hip_multiphase_create(phase_hip_multiphase **s, phase_hip_mesh *m, double rho1, double rho2, double mu1,
+                                double mu2, double gx, double gy, const int *uBcType, const double *uFace0,
+                                const int *pBcType, const double *pFace0, const int *gammaBcType,
+                                const double *gammaCell0, int pinCell);
+int phase_hip_multiphase_destroy(phase_hip_multiphase *s);
+const char *phase_hip_multiphase_last_error(const phase_hip_multiphase *s);
+int phase_hip_multiphase_setup(phase_hip_multiphase *s, const char *key, const char *value);
+int phase_hip_multiphase_advance(phase_hip_multiphase *s, double dt, int nSteps, double *history);
+int phase_hip_multiphase_fields(phase_hip_multiphase *s, double *uCell, double *pCell, double *uFace,
+                                double *gammaCell);
 
 /* ------------------------------------------------------------------ SIMPLE driver
  * No counterpart module in the reference snapshot (its solvers are projection

@@ -65,6 +65,7 @@ def _load():
                            ctypes.c_char_p, ctypes.c_int]),
         "orc_div_grad": (_vp, [_vp, _f64p, _f64p, _vp, _vp, ctypes.c_char_p, ctypes.c_int]),
         "orc_ddt_rho": (_vp, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _vp, _vp, _f64p, _vp]),
+        "orc_face_to_cell": (None, [_vp, _f64p, _vp, _vp, _f64p]),
         "orc_cicsam_hc": (ctypes.c_double, [ctypes.c_double, ctypes.c_double]),
         "orc_cicsam_uq": (ctypes.c_double, [ctypes.c_double, ctypes.c_double]),
         "orc_cicsam_weights": (None, [_vp, _f64p, _f64p, _f64p, ctypes.c_double, _f64p]),
@@ -341,6 +342,16 @@ def div_grad(mesh, phiu_face, grad_u, bc_type=None, u_face=None):
     err = ctypes.create_string_buffer(256)
     return _eqn_or_raise(lib().orc_div_grad(mesh._h, _f64(phiu_face), _f64(grad_u), _ptr(bc_type),
                                             _ptr(u_face), err, 256), err)
+
+
+def face_to_cell(mesh, face_vec, cell_weight=None, face_weight=None):
+    """VectorFiniteVolumeField::faceToCell, plain or weighted; (N, 2)."""
+    if (cell_weight is None) != (face_weight is None):
+        raise ValueError("faceToCell takes both weights or neither")
+    out = np.empty(2 * mesh.n_cells)
+    cw, fw = _f64(cell_weight), _f64(face_weight)
+    lib().orc_face_to_cell(mesh._h, _f64(face_vec), _ptr(cw), _ptr(fw), out)
+    return out.reshape(-1, 2)
 
 
 def cicsam_hc(g, co):

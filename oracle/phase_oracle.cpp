@@ -698,6 +698,40 @@ void *orc_div_grad(const void *mesh, const double *phiUFace, const double *gradU
   return e;
 }
 
+// VectorFiniteVolumeField::faceToCell (VectorFiniteVolumeField.cpp:5-26) and
+// its weighted form (:28-51): per cell, sum over its faces of
+// pointwise(v_f, |S_f|) [/ faceWeight_f], divided componentwise by sum |S_f|
+// [times cellWeight]; null weights give the unweighted form
+void orc_face_to_cell(const void *mesh, const double *faceVec, const double *cellWeight, const double *faceWeight,
+                      double *out) {
+  const Mesh &m = *static_cast<const Mesh *>(mesh);
+  for (int c = 0; c < m.nCells; ++c) {
+    double sx = 0., sy = 0., tx = 0., ty = 0.;
+    auto add = [&](int f) {
+      const V2 n = m.fSf[f];
+      const double ax = std::abs(n.x), ay = std::abs(n.y);
+      double px = faceVec[2 * f] * ax, py = faceVec[2 * f + 1] * ay;
+      if (faceWeight) {
+        px /= faceWeight[f];
+        py /= faceWeight[f];
+      }
+      tx += px;
+      ty += py;
+      sx += ax;
+      sy += ay;
+    };
+    for (int k = m.iptr[c]; k < m.iptr[c + 1]; ++k) add(m.iface[k]);
+    for (int k = m.bptr[c]; k < m.bptr[c + 1]; ++k) add(m.bface[k]);
+    double vx = tx / sx, vy = ty / sy;
+    if (cellWeight) {
+      vx = cellWeight[c] * vx;
+      vy = cellWeight[c] * vy;
+    }
+    out[2 * c] = vx;
+    out[2 * c + 1] = vy;
+  }
+}
+
 // ---------------------------------------------------------------- VOF (CICSAM)
 namespace {
 // std::min / std::max as the reference calls them -- min(a, b) = (b < a) ? b : a,

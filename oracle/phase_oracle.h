@@ -87,6 +87,8 @@ void *orc_ddt_rho(const void *mesh, int ncomp, double timeStep, double rho, cons
                   const double *rho0Cell, const double *phi0Cell, const int *cellMask);
 void orc_eqn_scale_cells(void *eqn, int nCells, const double *field);
 void orc_eqn_remove_cells(void *eqn, int nCells, const int *mask);
+void orc_face_to_cell(const void *mesh, const double *faceVec, const double *cellWeight, const double *faceWeight,
+                      double *out);
 double orc_cicsam_hc(double g, double co);
 double orc_cicsam_uq(double g, double co);
 void orc_cicsam_weights(const void *mesh, const double *uFace, const double *gammaCell, const double *gradCell,

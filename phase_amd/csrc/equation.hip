@@ -504,6 +504,38 @@ __global__ void kEqDivGrad(MeshDev M, const double *__restrict__ phiUF, const do
   }
 }
 
+// VectorFiniteVolumeField::faceToCell, plain (VectorFiniteVolumeField.cpp:5-26)
+// or weighted (:28-51): sum of pointwise(v_f, |S_f|) [/ faceWeight_f] over the
+// cell's faces, componentwise over sum |S_f| [times cellWeight]
+__global__ void kFaceToCell(MeshDev M, const double *__restrict__ fv, const double *__restrict__ cw,
+                            const double *__restrict__ fw, double *__restrict__ out) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < M.nCells; c += gridDim.x * blockDim.x) {
+    double sx = 0., sy = 0., tx = 0., ty = 0.;
+    auto add = [&](int f) {
+      const Dv2 n = ld2(M.sf, f);
+      const double ax = fabs(n.x), ay = fabs(n.y);
+      double px = fv[2 * f] * ax, py = fv[2 * f + 1] * ay;
+      if (fw) {
+        px /= fw[f];
+        py /= fw[f];
+      }
+      tx += px;
+      ty += py;
+      sx += ax;
+      sy += ay;
+    };
+    for (int k = M.iptr[c]; k < M.iptr[c + 1]; ++k) add(M.iface[k]);
+    for (int k = M.bptr[c]; k < M.bptr[c + 1]; ++k) add(M.bface[k]);
+    double vx = tx / sx, vy = ty / sy;
+    if (cw) {
+      vx = cw[c] * vx;
+      vy = cw[c] * vy;
+    }
+    out[2 * c] = vx;
+    out[2 * c + 1] = vy;
+  }
+}
+
 // ---------------------------------------------------------------- VOF (CICSAM)
 // std::min / std::max as the reference calls them -- min(a, b) = (b < a) ? b : a,
 // max(a, b) = (a < b) ? b : a -- so a NaN first argument passes through
@@ -1219,6 +1251,18 @@ int phase_hip_field_interpolate_faces(phase_hip_mesh *m, int ncomp, const double
     PH_CHECK(hipSetDevice(m->device));
     hipLaunchKernelGGL(kInterpFaces, dim3(g256(m->nFaces)), dim3(256), 0, m->stream, m->nFaces, ncomp, m->dFL.p,
                        m->dFR.p, m->dCC.p, m->dFC.p, m->dSf.p, bcType, phiCell, phiFace);
+    PH_CHECK(hipGetLastError());
+  });
+}
+
+int phase_hip_field_face_to_cell(phase_hip_mesh *m, const double *faceVec, const double *cellWeight,
+                                 const double *faceWeight, double *cellVec) {
+  return guardedM(m, [&] {
+    if (!faceVec || !cellVec) fail(PHASE_HIP_EINVAL, "null field");
+    if (!cellWeight != !faceWeight) fail(PHASE_HIP_EINVAL, "faceToCell takes both weights or neither");
+    PH_CHECK(hipSetDevice(m->device));
+    hipLaunchKernelGGL(kFaceToCell, dim3(g256(m->nCells)), dim3(256), 0, m->stream, meshDev(m), faceVec, cellWeight,
+                       faceWeight, cellVec);
     PH_CHECK(hipGetLastError());
   });
 }

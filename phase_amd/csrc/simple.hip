@@ -621,3 +621,343 @@ int phase_hip_simple_fields(phase_hip_simple *s, double *uCell, double *pCell, d
 }
 
 }  // extern "C"
+
+// FractionalStepMultiphase::solve (src/2D/Unstructured/Solvers/FractionalStepMultiphase.cpp:58-72)
+// on the device, without the surface-tension force (fst = 0), one time step:
+//   solveGammaEqn (:76-100): beta = cicsam weights; ddt(gamma) + cicsam::div(u,
+//     gamma, beta, 0.5) == 0; interpolateFaces(gamma); grad gamma
+//   updateProperties (:175-218): rho0, mu0 = rho, mu; rho, mu from gamma;
+//     grad rho (faces); sg_f = dot(g, -x_f) grad rho_f; sg = faceToCell(rho, rho)
+//   solveUEqn (:102-150): u1, u0 = u0, u; gradP = faceToCell(rho, rho0);
+//     rho ddt(u) + rho dive(u, u, 0.5) == laplacian(mu, u, 0.5) + src(sg - gradP);
+//     u += dt / rho gradP; momentum-interpolated face velocities
+//   solvePEqn (:152-163): laplacian(dt / rho, p) == src::div(u); boundary
+//     faces; grad_f p; gradP = faceToCell(rho, rho)
+//   correctVelocity (:165-173): u -= dt / rho gradP (cells and every face)
+struct phase_hip_multiphase {
+  phase_hip_mesh *mesh = nullptr;
+  int N = 0, F = 0, pin = -1;
+  double rho1 = 1., rho2 = 1., mu1 = 1., mu2 = 1., gx = 0., gy = 0.;
+  phase_hip_equation *gEq = nullptr, *uEq = nullptr, *tmp = nullptr, *gTmp = nullptr, *pEq = nullptr;
+  phase_hip_solver *sg = nullptr, *su = nullptr, *sp = nullptr;
+  DBuf<double> gC, gF, g0C, g0F, ggF, ggC, beta;                      // volume fraction
+  DBuf<double> rhoC, rhoF, muC, muF, rho0F, mu0F, grF, grC, sgF, sgC;  // properties, gravity
+  DBuf<double> uC, u0C, u1C, uF, u0F, u1F, uB, pC, pF, gpF, gpC, mob, gsrc, srcT, m, maxBuf;
+  DBuf<int> uBc, pBc, gBc;
+  std::string err;
+};
+
+namespace {
+
+// sg_f = dot(g, -x_f) grad rho_f (FractionalStepMultiphase.cpp:190-192)
+__global__ void kGravityFaces(int nFaces, double gx, double gy, const double *__restrict__ fc,
+                              const double *__restrict__ grF, double *__restrict__ sgF) {
+  for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < nFaces; f += gridDim.x * blockDim.x) {
+    const double h = gx * -fc[2 * f] + gy * -fc[2 * f + 1];
+    sgF[2 * f] = h * grF[2 * f];
+    sgF[2 * f + 1] = h * grF[2 * f + 1];
+  }
+}
+
+// u_i += (dt / rho_c) g_i (add) or u_i -= (dt / rho_c) g_i, c = i / 2
+__global__ void kDtOverRhoUpdate(long n2, double dt, int add, const double *__restrict__ rho,
+                                 const double *__restrict__ g, double *u) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n2; i += (long)gridDim.x * blockDim.x) {
+    const double v = (dt / rho[i >> 1]) * g[i];
+    u[i] = add ? u[i] + v : u[i] - v;
+  }
+}
+
+// out = dt / rho (the field timeStep / rho_ of solvePEqn)
+__global__ void kScalarOver(int n, double a, const double *__restrict__ x, double *__restrict__ out) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) out[i] = a / x[i];
+}
+
+// out = a - b
+__global__ void kSub(long n, const double *__restrict__ a, const double *__restrict__ b, double *__restrict__ out) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    out[i] = a[i] - b[i];
+}
+
+// Face velocities after the momentum solve (FractionalStepMultiphase.cpp:118-147):
+// interior faces volume-weighted (Face::volumeWeight = V_r / (V_r + V_l),
+// Face.cpp:54-58) u - dt / rho sg plus dt / rho_f sg_f; NORMAL_GRADIENT
+// patches from the lCell; SYMMETRY patches the lCell's tangential component
+// (Vector2D.cpp:30-36); FIXED (and OUTFLOW) faces unchanged.
+__global__ void kMultiphaseFaceVelocity(int nFaces, double dt, const int *__restrict__ fl, const int *__restrict__ fr,
+                                        const double *__restrict__ vol, const double *__restrict__ sf,
+                                        const int *__restrict__ bc, const double *__restrict__ uC,
+                                        const double *__restrict__ rhoC, const double *__restrict__ rhoF,
+                                        const double *__restrict__ sgC, const double *__restrict__ sgF, double *uF) {
+  for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < nFaces; f += gridDim.x * blockDim.x) {
+    const int a = fl[f], b = fr[f];
+    const double kf = dt / rhoF[f];
+    if (b >= 0) {
+      const double v1 = vol[b], v2 = vol[a];
+      const double g = v1 / (v1 + v2);
+      const double ka = dt / rhoC[a], kb = dt / rhoC[b];
+      for (int q = 0; q < 2; ++q)
+        uF[2 * f + q] = g * (uC[2 * a + q] - ka * sgC[2 * a + q]) + (1. - g) * (uC[2 * b + q] - kb * sgC[2 * b + q]) +
+                        kf * sgF[2 * f + q];
+    } else if (bc[f] == PHASE_HIP_BC_NORMAL_GRADIENT) {
+      const double ka = dt / rhoC[a];
+      for (int q = 0; q < 2; ++q) uF[2 * f + q] = uC[2 * a + q] - ka * sgC[2 * a + q] + kf * sgF[2 * f + q];
+    } else if (bc[f] == PHASE_HIP_BC_SYMMETRY) {
+      const double nx = sf[2 * f], ny = sf[2 * f + 1];
+      const double t = (uC[2 * a] * nx + uC[2 * a + 1] * ny) / (nx * nx + ny * ny);
+      uF[2 * f] = uC[2 * a] - t * nx;
+      uF[2 * f + 1] = uC[2 * a + 1] - t * ny;
+    }
+  }
+}
+
+template <class Fn>
+int guardedMP(phase_hip_multiphase *s, Fn &&f) {
+  if (!s) return PHASE_HIP_EINVAL;
+  try {
+    f();
+    return PHASE_HIP_OK;
+  } catch (const HipError &x) {
+    s->err = x.msg;
+    return x.code;
+  } catch (const std::exception &x) {
+    s->err = x.what();
+    return PHASE_HIP_EINVAL;
+  }
+}
+
+double mpMaxAbs(phase_hip_multiphase *s, const double *a, int n) {
+  hipLaunchKernelGGL(kMaxAbs, dim3(1), dim3(256), 0, s->mesh->stream, n, a, s->maxBuf.p);
+  double v = 0.;
+  PH_CHECK(hipMemcpyAsync(&v, s->maxBuf.p, sizeof(double), hipMemcpyDeviceToHost, s->mesh->stream));
+  PH_CHECK(hipStreamSynchronize(s->mesh->stream));
+  return v;
+}
+
+void copyD(double *dst, const double *src, size_t n, hipStream_t st) {
+  PH_CHECK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+}
+
+// solve e into x (device, original numbering), x also the initial guess;
+// returns the iterations
+int solveInto(phase_hip_solver *sv, phase_hip_equation *e, double *x, const char *what) {
+  ck(phase_hip_solver_set_equation(sv, e), what, sv->err.c_str());
+  ck(phase_hip_solver_set_guess_device(sv, x), what, sv->err.c_str());
+  double err = 0.;
+  const int rc = phase_hip_solver_solve(sv, &err);
+  if (rc != PHASE_HIP_ENOCONV) ck(rc, what, sv->err.c_str());
+  ck(phase_hip_solver_get_x_device(sv, x), what, sv->err.c_str());
+  return phase_hip_solver_n_iters(sv);
+}
+
+// updateProperties without the surface tension (FractionalStepMultiphase.cpp:175-210)
+void mpProperties(phase_hip_multiphase *s) {
+  phase_hip_mesh *m = s->mesh;
+  hipStream_t st = m->stream;
+  copyD(s->rho0F.p, s->rhoF.p, s->F, st);
+  copyD(s->mu0F.p, s->muF.p, s->F, st);
+  ck(phase_hip_vof_properties(m, s->rho1, s->rho2, s->mu1, s->mu2, s->gC.p, s->gF.p, s->rhoC.p, s->rhoF.p, s->muC.p,
+                              s->muF.p),
+     "properties", m->err.c_str());
+  ck(phase_hip_field_grad(m, s->rhoC.p, s->rhoF.p, s->grF.p, s->grC.p), "grad rho", m->err.c_str());
+  hipLaunchKernelGGL(kGravityFaces, dim3(g256(s->F)), dim3(256), 0, st, s->F, s->gx, s->gy, m->dFC.p, s->grF.p,
+                     s->sgF.p);
+  ck(phase_hip_field_face_to_cell(m, s->sgF.p, s->rhoC.p, s->rhoF.p, s->sgC.p), "sg faceToCell", m->err.c_str());
+}
+
+void mpStep(phase_hip_multiphase *s, double dt, double *hist) {
+  phase_hip_mesh *m = s->mesh;
+  hipStream_t st = m->stream;
+  const int N = s->N, F = s->F;
+  // solveGammaEqn
+  ck(phase_hip_cicsam_weights(m, s->uF.p, s->gC.p, s->ggC.p, dt, s->beta.p), "cicsam weights", m->err.c_str());
+  copyD(s->g0C.p, s->gC.p, N, st);
+  copyD(s->g0F.p, s->gF.p, F, st);
+  ck(phase_hip_eq_ddt(s->gEq, dt, s->g0C.p), "fv::ddt", s->gEq->err.c_str());
+  ck(phase_hip_eq_cicsam_div(s->gTmp, s->uF.p, s->beta.p, 0.5, s->gBc.p, s->g0F.p, s->g0C.p, s->g0F.p),
+     "cicsam::div", s->gTmp->err.c_str());
+  ck(phase_hip_eq_add(s->gEq, s->gTmp, 1), "operator+", s->gEq->err.c_str());
+  const int gIt = solveInto(s->sg, s->gEq, s->gC.p, "gammaEqn");
+  ck(phase_hip_field_interpolate_faces(m, 1, s->gC.p, s->gBc.p, s->gF.p), "gamma faces", m->err.c_str());
+  ck(phase_hip_field_grad(m, s->gC.p, s->gF.p, s->ggF.p, s->ggC.p), "grad gamma", m->err.c_str());
+  mpProperties(s);
+  // solveUEqn
+  copyD(s->u1C.p, s->u0C.p, 2 * (size_t)N, st);
+  copyD(s->u0C.p, s->uC.p, 2 * (size_t)N, st);
+  copyD(s->u1F.p, s->u0F.p, 2 * (size_t)F, st);
+  copyD(s->u0F.p, s->uF.p, 2 * (size_t)F, st);
+  ck(phase_hip_field_face_to_cell(m, s->gpF.p, s->rhoC.p, s->rho0F.p, s->gpC.p), "gradP faceToCell", m->err.c_str());
+  ck(phase_hip_eq_ddt(s->uEq, dt, s->u0C.p), "fv::ddt", s->uEq->err.c_str());
+  ck(phase_hip_eq_scale_cells(s->uEq, s->rhoC.p), "rho *", s->uEq->err.c_str());
+  ck(phase_hip_eq_dive(s->tmp, s->u0F.p, s->u1F.p, 0.5, s->uBc.p, s->u0C.p, s->u1C.p, s->u0F.p, s->u1F.p), "fv::dive",
+     s->tmp->err.c_str());
+  ck(phase_hip_eq_scale_cells(s->tmp, s->rhoC.p), "rho *", s->tmp->err.c_str());
+  ck(phase_hip_eq_add(s->uEq, s->tmp, 1), "operator+", s->uEq->err.c_str());
+  ck(phase_hip_eq_laplacian(s->tmp, 2, 0., s->muF.p, s->mu0F.p, 0.5, s->uBc.p, s->uF.p, s->u0C.p, s->u0F.p),
+     "fv::laplacian", s->tmp->err.c_str());
+  // + src::src(sg - gradP) is -= src::src(gradP - sg)
+  hipLaunchKernelGGL(kSub, dim3(g256(2L * N)), dim3(256), 0, st, 2L * N, s->gpC.p, s->sgC.p, s->srcT.p);
+  ck(phase_hip_field_src_src(m, 2, s->srcT.p, s->gsrc.p), "src::src", m->err.c_str());
+  ck(phase_hip_eq_sub_vector(s->tmp, s->gsrc.p), "operator+", s->tmp->err.c_str());
+  ck(phase_hip_eq_add(s->uEq, s->tmp, -1), "operator==", s->uEq->err.c_str());
+  hipLaunchKernelGGL(kToBlocks, dim3(g256(N)), dim3(256), 0, st, N, s->uC.p, s->uB.p);
+  const int uIt = solveInto(s->su, s->uEq, s->uB.p, "uEqn");
+  hipLaunchKernelGGL(kFromBlocks, dim3(g256(N)), dim3(256), 0, st, N, s->uB.p, s->uC.p);
+  hipLaunchKernelGGL(kDtOverRhoUpdate, dim3(g256(2L * N)), dim3(256), 0, st, 2L * N, dt, 1, s->rhoC.p, s->gpC.p,
+                     s->uC.p);
+  hipLaunchKernelGGL(kMultiphaseFaceVelocity, dim3(g256(F)), dim3(256), 0, st, F, dt, m->dFL.p, m->dFR.p, m->dVol.p,
+                     m->dSf.p, s->uBc.p, s->uC.p, s->rhoC.p, s->rhoF.p, s->sgC.p, s->sgF.p, s->uF.p);
+  PH_CHECK(hipGetLastError());
+  // solvePEqn
+  hipLaunchKernelGGL(kScalarOver, dim3(g256(F)), dim3(256), 0, st, F, dt, s->rhoF.p, s->mob.p);
+  ck(phase_hip_eq_laplacian(s->pEq, 3, 0., s->mob.p, nullptr, 1.0, s->pBc.p, s->pF.p, nullptr, nullptr),
+     "fv::laplacian", s->pEq->err.c_str());
+  ck(phase_hip_field_src_div(m, s->uF.p, s->m.p), "src::div", m->err.c_str());
+  ck(phase_hip_eq_sub_vector(s->pEq, s->m.p), "operator==", s->pEq->err.c_str());
+  if (s->pin >= 0)
+    hipLaunchKernelGGL(kPinRow, dim3(1), dim3(64), 0, st, s->pin, s->pEq->stride, s->pEq->cols.p, s->pEq->vals.p,
+                       s->pEq->cnt.p, s->pEq->rhs.p);
+  const int pIt = solveInto(s->sp, s->pEq, s->pC.p, "pEqn");
+  ck(phase_hip_field_interpolate_faces(m, 1, s->pC.p, s->pBc.p, s->pF.p), "setBoundaryFaces", m->err.c_str());
+  ck(phase_hip_field_grad(m, s->pC.p, s->pF.p, s->gpF.p, s->gpC.p), "grad p", m->err.c_str());
+  ck(phase_hip_field_face_to_cell(m, s->gpF.p, s->rhoC.p, s->rhoF.p, s->gpC.p), "gradP faceToCell", m->err.c_str());
+  // correctVelocity
+  hipLaunchKernelGGL(kDtOverRhoUpdate, dim3(g256(2L * N)), dim3(256), 0, st, 2L * N, dt, 0, s->rhoC.p, s->gpC.p,
+                     s->uC.p);
+  hipLaunchKernelGGL(kDtOverRhoUpdate, dim3(g256(2L * F)), dim3(256), 0, st, 2L * F, dt, 0, s->rhoF.p, s->gpF.p,
+                     s->uF.p);
+  PH_CHECK(hipGetLastError());
+  if (hist) {
+    // FractionalStep::maxDivergenceError on the corrected faces
+    ck(phase_hip_field_src_div(m, s->uF.p, s->m.p), "src::div", m->err.c_str());
+    hist[0] = gIt;
+    hist[1] = uIt;
+    hist[2] = pIt;
+    hist[3] = mpMaxAbs(s, s->m.p, N);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int phase_hip_multiphase_destroy(phase_hip_multiphase *s) {
+  if (!s) return PHASE_HIP_OK;
+  for (phase_hip_equation *e : {s->gEq, s->uEq, s->tmp, s->gTmp, s->pEq}) phase_hip_equation_destroy(e);
+  for (phase_hip_solver *v : {s->sg, s->su, s->sp}) phase_hip_solver_destroy(v);
+  delete s;
+  return PHASE_HIP_OK;
+}
+
+const char *phase_hip_multiphase_last_error(const phase_hip_multiphase *s) {
+  return s ? s->err.c_str() : "null driver";
+}
+
+int phase_hip_multiphase_create(phase_hip_multiphase **out, phase_hip_mesh *m, double rho1, double rho2, double mu1,
+                                double mu2, double gx, double gy, const int *uBcType, const double *uFace0,
+                                const int *pBcType, const double *pFace0, const int *gammaBcType,
+                                const double *gammaCell0, int pinCell) {
+  if (!out || !m || !uBcType || !uFace0 || !gammaCell0) return PHASE_HIP_EINVAL;
+  *out = nullptr;
+  phase_hip_multiphase *s = new phase_hip_multiphase();
+  s->mesh = m;
+  s->N = m->nCells;
+  s->F = m->nFaces;
+  s->rho1 = rho1;
+  s->rho2 = rho2;
+  s->mu1 = mu1;
+  s->mu2 = mu2;
+  s->gx = gx;
+  s->gy = gy;
+  s->pin = pinCell;
+  const int rc = guardedMP(s, [&] {
+    if (!(rho1 > 0.) || !(rho2 > 0.) || !(mu1 > 0.) || !(mu2 > 0.))
+      fail(PHASE_HIP_EINVAL, "densities and viscosities must be positive");
+    if (pinCell < -1 || pinCell >= m->nCells) fail(PHASE_HIP_EINVAL, "pin cell out of range (-1: none)");
+    PH_CHECK(hipSetDevice(m->device));
+    hipStream_t st = m->stream;
+    const int N = s->N, F = s->F;
+    ck(phase_hip_equation_create(&s->gEq, m, 1), "gammaEqn", "");
+    ck(phase_hip_equation_create(&s->gTmp, m, 1), "gammaEqn", "");
+    ck(phase_hip_equation_create(&s->uEq, m, 2), "uEqn", "");
+    ck(phase_hip_equation_create(&s->tmp, m, 2), "uEqn", "");
+    ck(phase_hip_equation_create(&s->pEq, m, 1), "pEqn", "");
+    for (phase_hip_solver **v : {&s->sg, &s->su, &s->sp}) {
+      ck(phase_hip_solver_create(v, m->device), "solver", "");
+      ck(phase_hip_solver_setup(*v, "tolerance", "1e-8"), "setup", (*v)->err.c_str());
+      ck(phase_hip_solver_setup(*v, "maxIters", "5000"), "setup", (*v)->err.c_str());
+    }
+    for (DBuf<double> *b : {&s->gC, &s->g0C, &s->rhoC, &s->muC, &s->pC, &s->m}) b->alloc(N);
+    for (DBuf<double> *b : {&s->gF, &s->g0F, &s->beta, &s->rhoF, &s->muF, &s->rho0F, &s->mu0F, &s->pF, &s->mob})
+      b->alloc(F);
+    for (DBuf<double> *b : {&s->ggC, &s->grC, &s->sgC, &s->uC, &s->u0C, &s->u1C, &s->uB, &s->gpC, &s->gsrc, &s->srcT})
+      b->alloc(2 * (size_t)N);
+    for (DBuf<double> *b : {&s->ggF, &s->grF, &s->sgF, &s->uF, &s->u0F, &s->u1F, &s->gpF}) b->alloc(2 * (size_t)F);
+    s->maxBuf.alloc(1);
+    for (DBuf<double> *b : {&s->uC, &s->pC, &s->gpC, &s->gpF, &s->gF, &s->rhoF, &s->muF}) b->zero(st);
+    std::vector<int> ubc(uBcType, uBcType + F), pbc(F, PHASE_HIP_BC_NORMAL_GRADIENT),
+        gbc(F, PHASE_HIP_BC_NORMAL_GRADIENT);
+    if (pBcType) pbc.assign(pBcType, pBcType + F);
+    if (gammaBcType) gbc.assign(gammaBcType, gammaBcType + F);
+    s->uBc.upload(ubc, st);
+    s->pBc.upload(pbc, st);
+    s->gBc.upload(gbc, st);
+    PH_CHECK(hipMemcpyAsync(s->uF.p, uFace0, 2 * (size_t)F * sizeof(double), hipMemcpyHostToDevice, st));
+    if (pFace0) PH_CHECK(hipMemcpyAsync(s->pF.p, pFace0, (size_t)F * sizeof(double), hipMemcpyHostToDevice, st));
+    else s->pF.zero(st);
+    PH_CHECK(hipMemcpyAsync(s->gC.p, gammaCell0, (size_t)N * sizeof(double), hipMemcpyHostToDevice, st));
+    ck(phase_hip_field_interpolate_faces(m, 2, s->uC.p, s->uBc.p, s->uF.p), "u faces", m->err.c_str());
+    copyD(s->u0C.p, s->uC.p, 2 * (size_t)N, st);
+    copyD(s->u1C.p, s->uC.p, 2 * (size_t)N, st);
+    copyD(s->u0F.p, s->uF.p, 2 * (size_t)F, st);
+    copyD(s->u1F.p, s->uF.p, 2 * (size_t)F, st);
+    ck(phase_hip_field_interpolate_faces(m, 1, s->pC.p, s->pBc.p, s->pF.p), "p faces", m->err.c_str());
+    ck(phase_hip_field_interpolate_faces(m, 1, s->gC.p, s->gBc.p, s->gF.p), "gamma faces", m->err.c_str());
+    ck(phase_hip_field_grad(m, s->gC.p, s->gF.p, s->ggF.p, s->ggC.p), "grad gamma", m->err.c_str());
+    mpProperties(s);
+    PH_CHECK(hipStreamSynchronize(st));
+  });
+  if (rc != PHASE_HIP_OK) {
+    phase_hip_multiphase_destroy(s);
+    return rc;
+  }
+  *out = s;
+  return PHASE_HIP_OK;
+}
+
+int phase_hip_multiphase_setup(phase_hip_multiphase *s, const char *key, const char *value) {
+  return guardedMP(s, [&] {
+    if (!key || !value) fail(PHASE_HIP_EINVAL, "null key/value");
+    std::string k(key);
+    phase_hip_solver *v = nullptr;
+    size_t skip = 2;
+    if (k.rfind("u.", 0) == 0) v = s->su;
+    else if (k.rfind("p.", 0) == 0) v = s->sp;
+    else if (k.rfind("gamma.", 0) == 0) v = s->sg, skip = 6;
+    else fail(PHASE_HIP_EINVAL, "keys are gamma.<solver key>, u.<solver key> or p.<solver key>");
+    ck(phase_hip_solver_setup(v, k.c_str() + skip, value), "setup", v->err.c_str());
+  });
+}
+
+int phase_hip_multiphase_advance(phase_hip_multiphase *s, double dt, int nSteps, double *history) {
+  return guardedMP(s, [&] {
+    if (nSteps < 0 || !(dt > 0.)) fail(PHASE_HIP_EINVAL, "need dt > 0 and nSteps >= 0");
+    PH_CHECK(hipSetDevice(s->mesh->device));
+    for (int it = 0; it < nSteps; ++it) mpStep(s, dt, history ? history + 4 * (size_t)it : nullptr);
+  });
+}
+
+int phase_hip_multiphase_fields(phase_hip_multiphase *s, double *uCell, double *pCell, double *uFace,
+                                double *gammaCell) {
+  return guardedMP(s, [&] {
+    hipStream_t st = s->mesh->stream;
+    const size_t N = s->N, F = s->F;
+    if (uCell) PH_CHECK(hipMemcpyAsync(uCell, s->uC.p, 2 * N * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (pCell) PH_CHECK(hipMemcpyAsync(pCell, s->pC.p, N * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (uFace) PH_CHECK(hipMemcpyAsync(uFace, s->uF.p, 2 * F * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (gammaCell) PH_CHECK(hipMemcpyAsync(gammaCell, s->gC.p, N * sizeof(double), hipMemcpyDeviceToHost, st));
+    PH_CHECK(hipStreamSynchronize(st));
+  });
+}
+
+}  // extern "C"

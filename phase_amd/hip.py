@@ -92,6 +92,7 @@ _SIGS = {
     "phase_hip_eq_dive": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_double, _vp, _vp, _vp, _vp, _vp]),
     "phase_hip_eq_div_grad": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "phase_hip_cicsam_weights": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_double, _vp]),
+    "phase_hip_field_face_to_cell": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "phase_hip_cicsam_momentum_flux": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, _vp, _vp, _vp, _vp,
                                                       _vp]),
     "phase_hip_vof_properties": (ctypes.c_int, [_vp] + [ctypes.c_double] * 4 + [_vp] * 6),
@@ -117,6 +118,13 @@ _SIGS = {
     "phase_hip_fstep_setup": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_char_p]),
     "phase_hip_fstep_advance": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, _vp]),
     "phase_hip_fstep_fields": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    "phase_hip_multiphase_create": (ctypes.c_int, [ctypes.POINTER(_vp), _vp] + [ctypes.c_double] * 6 +
+                                    [_i32p, _f64p, _vp, _vp, _vp, _f64p, ctypes.c_int]),
+    "phase_hip_multiphase_destroy": (ctypes.c_int, [_vp]),
+    "phase_hip_multiphase_last_error": (ctypes.c_char_p, [_vp]),
+    "phase_hip_multiphase_setup": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_char_p]),
+    "phase_hip_multiphase_advance": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, _vp]),
+    "phase_hip_multiphase_fields": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "phase_hip_simple_create": (ctypes.c_int, [ctypes.POINTER(_vp), _vp, ctypes.c_double, ctypes.c_double,
                                                ctypes.c_double, ctypes.c_double, _i32p, _f64p, ctypes.c_int]),
     "phase_hip_simple_destroy": (ctypes.c_int, [_vp]),
@@ -293,6 +301,12 @@ class HipMesh:
     def grad(self, phi_cell, phi_face, grad_face, grad_cell):
         self._mcheck(load_library().phase_hip_field_grad(self._h, phi_cell.ptr, phi_face.ptr, grad_face.ptr,
                                                          grad_cell.ptr), "grad")
+        self.synchronize()
+
+    def face_to_cell(self, face_vec, cell_vec, cell_weight=None, face_weight=None):
+        self._mcheck(load_library().phase_hip_field_face_to_cell(self._h, face_vec.ptr, _dp_or_none(cell_weight),
+                                                                 _dp_or_none(face_weight), cell_vec.ptr),
+                     "faceToCell")
         self.synchronize()
 
     # --- volume of fluid (Cicsam.cpp, FractionalStepMultiphase.cpp)
@@ -493,6 +507,61 @@ class HipFractionalStep:
         except Exception:
             pass
 
+
+
+class HipMultiphase:
+    """FractionalStepMultiphase::solve (src/2D/Unstructured/Solvers/
+    FractionalStepMultiphase.cpp:58-218) on the device, without the surface
+    tension force: volume fraction (CICSAM), properties, momentum and pressure
+    per call of advance().  gamma = 0 is phase 1 (rho1, mu1), 1 phase 2."""
+
+    def __init__(self, mesh, rho1, rho2, mu1, mu2, gravity, u_bc, u_face0, gamma_cell, p_bc=None, p_face0=None,
+                 gamma_bc=None, pin_cell=-1):
+        h = _vp()
+        self._pbc = None if p_bc is None else _i32(p_bc)
+        self._pf0 = None if p_face0 is None else _f64(p_face0)
+        self._gbc = None if gamma_bc is None else _i32(gamma_bc)
+        gx, gy = (float(v) for v in gravity)
+        rc = load_library().phase_hip_multiphase_create(
+            ctypes.byref(h), mesh._h, float(rho1), float(rho2), float(mu1), float(mu2), gx, gy, _i32(u_bc),
+            _f64(np.asarray(u_face0).reshape(-1)), _p(self._pbc), _p(self._pf0), _p(self._gbc), _f64(gamma_cell),
+            int(pin_cell))
+        if rc != 0:
+            raise PhaseHipError(rc, "multiphaseCreate", "creation failed")
+        self._h = h
+        self.mesh = mesh
+
+    def _check(self, rc, method):
+        if rc != 0:
+            raise PhaseHipError(rc, method, load_library().phase_hip_multiphase_last_error(self._h).decode())
+
+    def setup(self, key, value):
+        self._check(load_library().phase_hip_multiphase_setup(self._h, key.encode(), str(value).encode()), "setup")
+
+    def advance(self, dt, n=1):
+        """n time steps; returns an (n, 4) history: gamma, momentum and
+        pressure iterations, max divergence of the corrected face velocities."""
+        hist = np.zeros((n, 4))
+        self._check(load_library().phase_hip_multiphase_advance(self._h, float(dt), int(n), _p(hist)), "advance")
+        return hist
+
+    def fields(self):
+        """(u cells (N, 2), p cells, u faces (F, 2), gamma cells)."""
+        N, F = self.mesh.n_cells, self.mesh.n_faces
+        u, p, uf, g = np.empty(2 * N), np.empty(N), np.empty(2 * F), np.empty(N)
+        self._check(load_library().phase_hip_multiphase_fields(self._h, _p(u), _p(p), _p(uf), _p(g)), "fields")
+        return u.reshape(N, 2), p, uf.reshape(F, 2), g
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().phase_hip_multiphase_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 class HipSimple:
     """SIMPLE driver on the device (steady incompressible flow): momentum and

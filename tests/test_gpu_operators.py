@@ -279,6 +279,23 @@ def test_equation_solve(hip, oracle, meshes):
     assert np.linalg.norm(s.x() - xs) <= 1e-8 * np.linalg.norm(xs)
 
 
+def test_face_to_cell_matches_oracle(hip, oracle, meshes):
+    """VectorFiniteVolumeField::faceToCell, plain and weighted, bit-identical;
+    one weight without the other is rejected like the reference's overloads."""
+    om, hm, _ = meshes
+    N, F = om.n_cells, om.n_faces
+    rng = np.random.default_rng(12)
+    fv = rng.standard_normal(2 * F)
+    cw, fw = np.exp(rng.standard_normal(N)), np.exp(rng.standard_normal(F))
+    d_fv, d_out = hip.DeviceArray(fv), hip.DeviceArray.zeros(2 * N)
+    hm.face_to_cell(d_fv, d_out)
+    assert same(d_out.get(), oracle.face_to_cell(om, fv).reshape(-1))
+    hm.face_to_cell(d_fv, d_out, hip.DeviceArray(cw), hip.DeviceArray(fw))
+    assert same(d_out.get(), oracle.face_to_cell(om, fv, cw, fw).reshape(-1))
+    with pytest.raises(hip.PhaseHipError):
+        hm.face_to_cell(d_fv, d_out, hip.DeviceArray(cw), None)
+
+
 def test_copy_device(hip):
     """phase_hip_copy_device: a byte-exact copy (with a ragged tail of the
     grid-stride loop) and EINVAL for sizes that are not 16-byte multiples."""

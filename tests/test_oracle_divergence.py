@@ -116,3 +116,47 @@ def test_ddt_density_forms(oracle):
     rows = np.repeat(np.arange(m.n_cells), np.diff(rp))
     assert set(rows[ci >= 0]) == set(np.where(mask == 1)[0])
     assert np.all(rhs[mask == 0] == 0.0)
+
+
+def test_face_to_cell_linear_field(oracle):
+    """faceToCell on a uniform rectilinear mesh with dx != dy: the x component
+    sees only the east and west faces (|S_f|.x), the y component only north and
+    south, so a linear vector field's face values average to its centroid value
+    (different components, so a swapped |S| component shows); the weighted form
+    with constant weights is cw / fw times the plain one (dyadic: exact)."""
+    xy, cptr, cind = G.rectilinear(8, 4, width=1.0, height=2.0)
+    m = oracle.Mesh(xy, cptr, cind)
+    f = m.faces()
+    _, cc = m.cells()
+    B = np.array([[1.5, -0.75], [0.25, 2.0]])
+    lin = lambda x: np.array([0.5, -1.0]) + x @ B.T
+    fv = lin(f["centroid"]).reshape(-1)
+    got = oracle.face_to_cell(m, fv)
+    assert np.abs(got - lin(cc)).max() <= 1e-14
+    w = oracle.face_to_cell(m, fv, np.full(m.n_cells, 4.0), np.full(m.n_faces, 2.0))
+    assert np.array_equal(w, 2.0 * got)
+    with pytest.raises(ValueError):
+        oracle.face_to_cell(m, fv, np.ones(m.n_cells), None)
+
+
+def test_face_to_cell_weighted_brute_force(oracle):
+    """On an unstructured mesh: a face loop that scatters each face to both of
+    its cells (the oracle gathers per cell) gives the same result."""
+    xy, cptr, cind = G.delaunay_square(10, G.delaunay_cells_for(300, 10), seed=5)
+    m = oracle.Mesh(xy, cptr, cind)
+    f = m.faces()
+    rng = np.random.default_rng(6)
+    N, F = m.n_cells, m.n_faces
+    fv = rng.standard_normal((F, 2))
+    cw, fw = np.exp(rng.standard_normal(N)), np.exp(rng.standard_normal(F))
+    t = np.zeros((N, 2))
+    s = np.zeros((N, 2))
+    for k in range(F):
+        a = np.abs(f["sf"][k])
+        for c in (f["lcell"][k], f["rcell"][k]):
+            if c >= 0:
+                t[c] += fv[k] * a / fw[k]
+                s[c] += a
+    expect = cw[:, None] * t / s
+    got = oracle.face_to_cell(m, fv.reshape(-1), cw, fw)
+    assert np.abs(got - expect).max() <= 1e-13 * np.abs(expect).max()
