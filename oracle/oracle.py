@@ -85,6 +85,10 @@ def _load():
                                              ctypes.c_double, ctypes.c_int,
                                              ctypes.POINTER(ctypes.c_int),
                                              ctypes.POINTER(ctypes.c_double)]),
+        "orc_bicgstab_ilu0_omp": (ctypes.c_int, [ctypes.c_int, _i32p, _i32p, _f64p, _f64p, _f64p,
+                                                 ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                                 ctypes.POINTER(ctypes.c_int),
+                                                 ctypes.POINTER(ctypes.c_double)]),
         "orc_bicgstab_ilut": (ctypes.c_int, [ctypes.c_int, _i32p, _i32p, _f64p, _f64p, _f64p,
                                              ctypes.c_double, ctypes.c_int, ctypes.c_double,
                                              ctypes.c_int, ctypes.POINTER(ctypes.c_int),
@@ -458,6 +462,16 @@ def bicgstab_ilu0(row_ptr, col_ind, vals, b, x0=None, tol=1e-8, max_iters=5000):
     it, rr = ctypes.c_int(), ctypes.c_double()
     rc = lib().orc_bicgstab_ilu0(n, _i32(row_ptr), _i32(col_ind), _f64(vals), _f64(b), x,
                                  float(tol), int(max_iters), ctypes.byref(it), ctypes.byref(rr))
+    return x, it.value, rr.value, rc
+
+
+def bicgstab_ilu0_omp(row_ptr, col_ind, vals, b, x0=None, tol=1e-8, max_iters=5000, threads=1):
+    """bicgstab_ilu0 with OpenMP SpMV / dots / updates (CPU timing baseline)."""
+    n = len(row_ptr) - 1
+    x = np.zeros(n) if x0 is None else _f64(x0).copy()
+    it, rr = ctypes.c_int(), ctypes.c_double()
+    rc = lib().orc_bicgstab_ilu0_omp(n, _i32(row_ptr), _i32(col_ind), _f64(vals), _f64(b), x, float(tol),
+                                     int(max_iters), int(threads), ctypes.byref(it), ctypes.byref(rr))
     return x, it.value, rr.value, rc
 
 

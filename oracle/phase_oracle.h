@@ -115,6 +115,10 @@ int orc_ilu0(int n, const int *rowPtr, const int *colInd, double *vals);
 int orc_bicgstab_ilu0(int n, const int *rowPtr, const int *colInd,
                       const double *vals, const double *b, double *x,
                       double tol, int maxIters, int *iters, double *relres);
+/* OpenMP variant of orc_bicgstab_ilu0 (SpMV, dots and vector updates over
+ * nThreads threads; ILU(0) sweeps sequential): CPU timing baseline only. */
+int orc_bicgstab_ilu0_omp(int n, const int *rowPtr, const int *colInd, const double *vals, const double *b,
+                          double *x, double tol, int maxIters, int nThreads, int *iters, double *relres);
 int orc_ilut(int n, const int *rowPtr, const int *colInd, const double *vals,
              double dropTol, int fillFactor, int **outRowPtr, int **outColInd,
              double **outVals);

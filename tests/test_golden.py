@@ -40,6 +40,20 @@ def test_oracle_reproduces_direct_solution(oracle, name):
 
 
 @pytest.mark.parametrize("name", NAMES)
+def test_oracle_omp_solver_reproduces_direct_solution(oracle, name):
+    """The multi-threaded CPU baseline solves the same systems."""
+    fx = load(name)
+    m = oracle.Mesh(fx["xy"], fx["cptr"], fx["cind"])
+    gamma, bc, phi = face_inputs(oracle, fx, m.faces())
+    e = oracle.laplacian(m, 3, gamma_face=gamma, bc_type=bc, phi_face=phi)
+    e.eq(fx["src"])
+    rp, ci, v, rhs = e.get()
+    x, it, rr, rc = oracle.bicgstab_ilu0_omp(rp, ci, v, -rhs, tol=1e-12, max_iters=20000, threads=4)
+    assert rc == 0 and rr <= 1e-12
+    assert np.linalg.norm(x - fx["x"]) / np.linalg.norm(fx["x"]) < 1e-8
+
+
+@pytest.mark.parametrize("name", NAMES)
 def test_oracle_ilut_reproduces_direct_solution(oracle, name):
     fx = load(name)
     m = oracle.Mesh(fx["xy"], fx["cptr"], fx["cind"])
