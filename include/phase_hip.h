@@ -64,6 +64,10 @@ enum {
   PHASE_HIP_BC_PARTIAL_SLIP = 4
 };
 
+/* Smoothing kernels of the surface-tension model
+ * (SurfaceTensionForce::SmoothingKernel::Type, SurfaceTensionForce.h). */
+enum { PHASE_HIP_KERNEL_PESKIN = 0, PHASE_HIP_KERNEL_POW6 = 1, PHASE_HIP_KERNEL_POW8 = 2 };
+
 /* Solve modes (phase_hip_setup key "mode"). */
 enum {
   PHASE_HIP_MODE_FP64 = 0,   /* BiCGStab + ILU(0), all FP64 */
@@ -407,12 +411,12 @@ int phase_hip_fstep_fields(phase_hip_fstep *s, double *uCell, double *pCell, dou
 
 /* ------------------------------------------------------------------ multiphase fractional step
  * FractionalStepMultiphase::solve(timeStep) -- src/2D/Unstructured/Solvers/
- * FractionalStepMultiphase.cpp:58-218, one time step on the device without
- * the surface-tension force (fst = 0; the reference's SurfaceTensionForce
- * models are not part of this library):
+ * FractionalStepMultiphase.cpp:58-218, one time step on the device; the
+ * surface-tension force fst is 0 unless phase_hip_multiphase_surface_tension
+ * turned it on:
  *   gamma: ddt(gamma) + cicsam::div(u, gamma, beta, 0.5) == 0 (:76-100)
  *   rho, mu from gamma; sg_f = dot(g, -x_f) grad rho_f; sg = faceToCell (:175-210)
- *   rho ddt(u) + rho dive(u, u, 0.5) == laplacian(mu, u, 0.5) + src(sg - gradP),
+ *   rho ddt(u) + rho dive(u, u, 0.5) == laplacian(mu, u, 0.5) + src(fst + sg - gradP),
  *     u += dt / rho gradP, momentum-interpolated faces (:102-150)
  *   laplacian(dt / rho, p) == src::div(u); gradP = faceToCell(rho, rho) (:152-163)
  *   u -= dt / rho gradP on cells and every face (:165-173)
@@ -435,6 +439,23 @@ int phase_hip_multiphase_setup(phase_hip_multiphase *s, const char *key, const c
 int phase_hip_multiphase_advance(phase_hip_multiphase *s, double dt, int nSteps, double *history);
 int phase_hip_multiphase_fields(phase_hip_multiphase *s, double *uCell, double *pCell, double *uFace,
                                 double *gammaCell);
+/* Turns on the CELESTE surface-tension force of FractionalStepMultiphase
+ * (Celeste.cpp:9-109 with CelesteStencil.cpp and SurfaceTensionForce.cpp;
+ * input keys Properties.sigma, Solver.smoothingKernelRadius,
+ * Solver.kernelType and Boundaries.gamma.<patch>.contactAngle):
+ * fst_f = sigma kappa_f gradGamma_f, fst = faceToCell(rho, rho), added to
+ * the momentum source and the face-velocity interpolation.  kernelType is a
+ * PHASE_HIP_KERNEL_* value; contactAngle per face in radians (NULL: pi/2
+ * on every face).  Builds the per-mesh stencils and smoothing kernels and
+ * evaluates the force from the current volume fraction, as the reference's
+ * initialize() does.  Without this call sigma is 0. */
+int phase_hip_multiphase_surface_tension(phase_hip_multiphase *s, double sigma, double kernelRadius,
+                                         int kernelType, const double *contactAngle);
+/* The model's fields after the last evaluation (host arrays, each may be
+ * NULL): smoothed gamma (N), interface normal (2N cells, 2F faces),
+ * curvature (N cells, F faces), force (2F faces, 2N cells). */
+int phase_hip_multiphase_surface_fields(phase_hip_multiphase *s, double *gammaTilde, double *nCell, double *nFace,
+                                        double *kappaCell, double *kappaFace, double *fstFace, double *fstCell);
 
 /* ------------------------------------------------------------------ SIMPLE driver
  * No counterpart module in the reference snapshot (its solvers are projection

@@ -3,19 +3,22 @@ CPU with the FP64 oracle's operators (phase_oracle.cpp) and direct solves
 (scipy), the checker for the device multiphase driver (phase_amd/csrc/simple.hip,
 phase_hip_vof_*).  Only tests/ may use it.
 
-One time step (FractionalStepMultiphase.cpp:58-72), without the surface
-tension force (fst = 0):
+One time step (FractionalStepMultiphase.cpp:58-72); the surface-tension
+force fst is the CELESTE model of celeste_oracle.py when `surface_tension`
+is given, else 0:
 
   solveGammaEqn (:76-100): beta = cicsam weights(u_f, gamma, grad gamma, dt);
       gamma0 = gamma; fv::ddt(gamma, dt) + cicsam::div(u, gamma, beta, 0.5) == 0;
       interpolateFaces(gamma); grad gamma
   updateProperties (:175-218): rho0 = rho, mu0 = mu; rho, mu from gamma (cells,
       faces); grad rho on the faces; sg_f = dot(g, -x_f) grad rho_f;
-      sg = faceToCell(rho, rho) of sg_f
+      sg = faceToCell(rho, rho) of sg_f; fst_f = sigma kappa_f grad gamma_f,
+      fst = faceToCell(rho, rho) of fst_f
   solveUEqn (:102-150): u1 = u0, u0 = u; gradP = faceToCell(rho, rho0) of
       gradP_f; rho * ddt(u, dt) + rho * dive(u, u, 0.5) ==
-      laplacian(mu, u, 0.5) + src(sg - gradP); u += dt / rho gradP; faces:
-      volume-weighted u - dt / rho sg plus dt / rho_f sg_f (interior),
+      laplacian(mu, u, 0.5) + src(fst + sg - gradP); u += dt / rho gradP;
+      faces: volume-weighted u - dt / rho (fst + sg) plus dt / rho_f
+      (fst_f + sg_f) (interior),
       by boundary type on the patches
   solvePEqn (:152-163): laplacian(dt / rho, p) == src::div(u); boundary faces;
       gradP_f; gradP = faceToCell(rho, rho) of gradP_f
@@ -42,7 +45,9 @@ def _solve(e):
 
 class MultiphaseOracle:
     def __init__(self, mesh, rho1, rho2, mu1, mu2, gravity, u_bc, u_boundary, p_bc, p_boundary, gamma_cell,
-                 gamma_bc=None):
+                 gamma_bc=None, surface_tension=None):
+        """surface_tension: None or (celeste_oracle.CelesteOracle, sigma,
+        contact angle per face in radians or None)."""
         m = self.m = mesh
         N, F = mesh.n_cells, mesh.n_faces
         self.rho1, self.rho2, self.mu1, self.mu2 = rho1, rho2, mu1, mu2
@@ -73,6 +78,14 @@ class MultiphaseOracle:
         self.rhoF = np.zeros(F)
         self.muC = np.zeros(N)
         self.muF = np.zeros(F)
+        self.st = surface_tension
+        self.fstF, self.fstC = np.zeros((F, 2)), np.zeros((N, 2))
+        self.kappaF = np.zeros(F)
+        self.st_fields = None
+        if surface_tension is not None:
+            ca = surface_tension[2]
+            ca = np.full(F, np.pi / 2) if ca is None else np.asarray(ca, float)
+            self.cos_t, self.sin_t = np.cos(ca), np.sin(ca)
         self._properties()
 
     def _properties(self):
@@ -85,6 +98,14 @@ class MultiphaseOracle:
         h = self.g[0] * -self.fc[:, 0] + self.g[1] * -self.fc[:, 1]
         self.sgF = h[:, None] * grF
         self.sgC = O.face_to_cell(m, self.sgF.reshape(-1), self.rhoC, self.rhoF)
+        if self.st is not None:
+            model, sigma = self.st[0], self.st[1]
+            ggF, _ = O.grad(m, self.gC, self.gF)
+            r = model.face_forces(self.gC, ggF, sigma, self.cos_t, self.sin_t, self.kappaF)
+            self.kappaF = r["kappa_face"]
+            self.fstF = r["fst_face"]
+            self.fstC = O.face_to_cell(m, self.fstF.reshape(-1), self.rhoC, self.rhoF)
+            self.st_fields = r
 
     def step(self, dt):
         m = self.m
@@ -112,7 +133,7 @@ class MultiphaseOracle:
         lap = O.laplacian(m, 2, ncomp=2, gamma_face=self.muF, gamma0_face=self.mu0F, theta=0.5, bc_type=self.ubc,
                           phi_face=self.uF.reshape(-1), phi0_cell=self.u0C.reshape(-1),
                           phi0_face=self.u0F.reshape(-1))
-        s = self.sgC - self.gpC  # fst + sg - gradP with fst = 0
+        s = (self.fstC + self.sgC) - self.gpC
         src = O.src_src(m, s.reshape(-1), ncomp=2)
         lap -= -src  # + src::src(...): rhs += src
         a.eq(lap)
@@ -136,7 +157,8 @@ class MultiphaseOracle:
     def _face_velocity(self, dt):
         l, r = self.fl, self.fr
         uF = self.uF.copy()
-        sg, rc, rf, u = self.sgC, self.rhoC, self.rhoF, self.uC
+        sg, rc, rf, u = self.fstC + self.sgC, self.rhoC, self.rhoF, self.uC
+        sgF = self.fstF + self.sgF
         for f in range(len(l)):
             a, b = l[f], r[f]
             kf = dt / rf[f]
@@ -145,11 +167,11 @@ class MultiphaseOracle:
                 ka, kb = dt / rc[a], dt / rc[b]
                 for q in range(2):
                     uF[f, q] = (g * (u[a, q] - ka * sg[a, q]) + (1. - g) * (u[b, q] - kb * sg[b, q])
-                                + kf * self.sgF[f, q])
+                                + kf * sgF[f, q])
             elif self.ubc[f] == O.BC_NORMAL_GRADIENT:
                 ka = dt / rc[a]
                 for q in range(2):
-                    uF[f, q] = u[a, q] - ka * sg[a, q] + kf * self.sgF[f, q]
+                    uF[f, q] = u[a, q] - ka * sg[a, q] + kf * sgF[f, q]
             elif self.ubc[f] == O.BC_SYMMETRY:
                 nx, ny = self.sf[f]
                 t = (u[a, 0] * nx + u[a, 1] * ny) / (nx * nx + ny * ny)

@@ -27,6 +27,7 @@
 #include <string>
 #include <vector>
 
+#include "celeste.h"
 #include "equation.h"
 #include "mesh.h"
 #include "solver.h"
@@ -623,13 +624,15 @@ int phase_hip_simple_fields(phase_hip_simple *s, double *uCell, double *pCell, d
 }  // extern "C"
 
 // FractionalStepMultiphase::solve (src/2D/Unstructured/Solvers/FractionalStepMultiphase.cpp:58-72)
-// on the device, without the surface-tension force (fst = 0), one time step:
+// on the device, one time step (fst: the CELESTE surface-tension force when
+// turned on, else 0):
 //   solveGammaEqn (:76-100): beta = cicsam weights; ddt(gamma) + cicsam::div(u,
 //     gamma, beta, 0.5) == 0; interpolateFaces(gamma); grad gamma
 //   updateProperties (:175-218): rho0, mu0 = rho, mu; rho, mu from gamma;
-//     grad rho (faces); sg_f = dot(g, -x_f) grad rho_f; sg = faceToCell(rho, rho)
+//     grad rho (faces); sg_f = dot(g, -x_f) grad rho_f; sg = faceToCell(rho, rho);
+//     fst_f = sigma kappa_f grad gamma_f; fst = faceToCell(rho, rho)
 //   solveUEqn (:102-150): u1, u0 = u0, u; gradP = faceToCell(rho, rho0);
-//     rho ddt(u) + rho dive(u, u, 0.5) == laplacian(mu, u, 0.5) + src(sg - gradP);
+//     rho ddt(u) + rho dive(u, u, 0.5) == laplacian(mu, u, 0.5) + src(fst + sg - gradP);
 //     u += dt / rho gradP; momentum-interpolated face velocities
 //   solvePEqn (:152-163): laplacian(dt / rho, p) == src::div(u); boundary
 //     faces; grad_f p; gradP = faceToCell(rho, rho)
@@ -643,7 +646,9 @@ struct phase_hip_multiphase {
   DBuf<double> gC, gF, g0C, g0F, ggF, ggC, beta;                      // volume fraction
   DBuf<double> rhoC, rhoF, muC, muF, rho0F, mu0F, grF, grC, sgF, sgC;  // properties, gravity
   DBuf<double> uC, u0C, u1C, uF, u0F, u1F, uB, pC, pF, gpF, gpC, mob, gsrc, srcT, m, maxBuf;
+  DBuf<double> fstF, fstC;  // surface tension (0 unless turned on)
   DBuf<int> uBc, pBc, gBc;
+  phase::CelesteModel *st = nullptr;
   std::string err;
 };
 
@@ -673,21 +678,23 @@ __global__ void kScalarOver(int n, double a, const double *__restrict__ x, doubl
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) out[i] = a / x[i];
 }
 
-// out = a - b
-__global__ void kSub(long n, const double *__restrict__ a, const double *__restrict__ b, double *__restrict__ out) {
+// out = gradP - (fst + sg): the negated cell source of the momentum equation
+__global__ void kMomentumSource(long n, const double *__restrict__ gp, const double *__restrict__ fst,
+                                const double *__restrict__ sg, double *__restrict__ out) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-    out[i] = a[i] - b[i];
+    out[i] = gp[i] - (fst[i] + sg[i]);
 }
 
 // Face velocities after the momentum solve (FractionalStepMultiphase.cpp:118-147):
 // interior faces volume-weighted (Face::volumeWeight = V_r / (V_r + V_l),
-// Face.cpp:54-58) u - dt / rho sg plus dt / rho_f sg_f; NORMAL_GRADIENT
+// Face.cpp:54-58) u - dt / rho (fst + sg) plus dt / rho_f (fst_f + sg_f); NORMAL_GRADIENT
 // patches from the lCell; SYMMETRY patches the lCell's tangential component
 // (Vector2D.cpp:30-36); FIXED (and OUTFLOW) faces unchanged.
 __global__ void kMultiphaseFaceVelocity(int nFaces, double dt, const int *__restrict__ fl, const int *__restrict__ fr,
                                         const double *__restrict__ vol, const double *__restrict__ sf,
                                         const int *__restrict__ bc, const double *__restrict__ uC,
                                         const double *__restrict__ rhoC, const double *__restrict__ rhoF,
+                                        const double *__restrict__ fstC, const double *__restrict__ fstF,
                                         const double *__restrict__ sgC, const double *__restrict__ sgF, double *uF) {
   for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < nFaces; f += gridDim.x * blockDim.x) {
     const int a = fl[f], b = fr[f];
@@ -697,11 +704,13 @@ __global__ void kMultiphaseFaceVelocity(int nFaces, double dt, const int *__rest
       const double g = v1 / (v1 + v2);
       const double ka = dt / rhoC[a], kb = dt / rhoC[b];
       for (int q = 0; q < 2; ++q)
-        uF[2 * f + q] = g * (uC[2 * a + q] - ka * sgC[2 * a + q]) + (1. - g) * (uC[2 * b + q] - kb * sgC[2 * b + q]) +
-                        kf * sgF[2 * f + q];
+        uF[2 * f + q] = g * (uC[2 * a + q] - ka * (fstC[2 * a + q] + sgC[2 * a + q])) +
+                        (1. - g) * (uC[2 * b + q] - kb * (fstC[2 * b + q] + sgC[2 * b + q])) +
+                        kf * (fstF[2 * f + q] + sgF[2 * f + q]);
     } else if (bc[f] == PHASE_HIP_BC_NORMAL_GRADIENT) {
       const double ka = dt / rhoC[a];
-      for (int q = 0; q < 2; ++q) uF[2 * f + q] = uC[2 * a + q] - ka * sgC[2 * a + q] + kf * sgF[2 * f + q];
+      for (int q = 0; q < 2; ++q)
+        uF[2 * f + q] = uC[2 * a + q] - ka * (fstC[2 * a + q] + sgC[2 * a + q]) + kf * (fstF[2 * f + q] + sgF[2 * f + q]);
     } else if (bc[f] == PHASE_HIP_BC_SYMMETRY) {
       const double nx = sf[2 * f], ny = sf[2 * f + 1];
       const double t = (uC[2 * a] * nx + uC[2 * a + 1] * ny) / (nx * nx + ny * ny);
@@ -750,7 +759,15 @@ int solveInto(phase_hip_solver *sv, phase_hip_equation *e, double *x, const char
   return phase_hip_solver_n_iters(sv);
 }
 
-// updateProperties without the surface tension (FractionalStepMultiphase.cpp:175-210)
+// CELESTE force from the current gamma (Celeste::computeFaceInterfaceForces)
+// and its cell values faceToCell(rho, rho) (FractionalStepMultiphase.cpp:211-214)
+void mpSurfaceForce(phase_hip_multiphase *s) {
+  phase_hip_mesh *m = s->mesh;
+  phase::celesteFaceForces(m, *s->st, s->gC.p, s->ggF.p, s->fstF.p);
+  ck(phase_hip_field_face_to_cell(m, s->fstF.p, s->rhoC.p, s->rhoF.p, s->fstC.p), "fst faceToCell", m->err.c_str());
+}
+
+// updateProperties (FractionalStepMultiphase.cpp:175-218)
 void mpProperties(phase_hip_multiphase *s) {
   phase_hip_mesh *m = s->mesh;
   hipStream_t st = m->stream;
@@ -763,6 +780,7 @@ void mpProperties(phase_hip_multiphase *s) {
   hipLaunchKernelGGL(kGravityFaces, dim3(g256(s->F)), dim3(256), 0, st, s->F, s->gx, s->gy, m->dFC.p, s->grF.p,
                      s->sgF.p);
   ck(phase_hip_field_face_to_cell(m, s->sgF.p, s->rhoC.p, s->rhoF.p, s->sgC.p), "sg faceToCell", m->err.c_str());
+  if (s->st) mpSurfaceForce(s);
 }
 
 void mpStep(phase_hip_multiphase *s, double dt, double *hist) {
@@ -795,8 +813,9 @@ void mpStep(phase_hip_multiphase *s, double dt, double *hist) {
   ck(phase_hip_eq_add(s->uEq, s->tmp, 1), "operator+", s->uEq->err.c_str());
   ck(phase_hip_eq_laplacian(s->tmp, 2, 0., s->muF.p, s->mu0F.p, 0.5, s->uBc.p, s->uF.p, s->u0C.p, s->u0F.p),
      "fv::laplacian", s->tmp->err.c_str());
-  // + src::src(sg - gradP) is -= src::src(gradP - sg)
-  hipLaunchKernelGGL(kSub, dim3(g256(2L * N)), dim3(256), 0, st, 2L * N, s->gpC.p, s->sgC.p, s->srcT.p);
+  // + src::src(fst + sg - gradP) is -= src::src(gradP - (fst + sg))
+  hipLaunchKernelGGL(kMomentumSource, dim3(g256(2L * N)), dim3(256), 0, st, 2L * N, s->gpC.p, s->fstC.p, s->sgC.p,
+                     s->srcT.p);
   ck(phase_hip_field_src_src(m, 2, s->srcT.p, s->gsrc.p), "src::src", m->err.c_str());
   ck(phase_hip_eq_sub_vector(s->tmp, s->gsrc.p), "operator+", s->tmp->err.c_str());
   ck(phase_hip_eq_add(s->uEq, s->tmp, -1), "operator==", s->uEq->err.c_str());
@@ -806,7 +825,8 @@ void mpStep(phase_hip_multiphase *s, double dt, double *hist) {
   hipLaunchKernelGGL(kDtOverRhoUpdate, dim3(g256(2L * N)), dim3(256), 0, st, 2L * N, dt, 1, s->rhoC.p, s->gpC.p,
                      s->uC.p);
   hipLaunchKernelGGL(kMultiphaseFaceVelocity, dim3(g256(F)), dim3(256), 0, st, F, dt, m->dFL.p, m->dFR.p, m->dVol.p,
-                     m->dSf.p, s->uBc.p, s->uC.p, s->rhoC.p, s->rhoF.p, s->sgC.p, s->sgF.p, s->uF.p);
+                     m->dSf.p, s->uBc.p, s->uC.p, s->rhoC.p, s->rhoF.p, s->fstC.p, s->fstF.p, s->sgC.p, s->sgF.p,
+                     s->uF.p);
   PH_CHECK(hipGetLastError());
   // solvePEqn
   hipLaunchKernelGGL(kScalarOver, dim3(g256(F)), dim3(256), 0, st, F, dt, s->rhoF.p, s->mob.p);
@@ -845,6 +865,7 @@ int phase_hip_multiphase_destroy(phase_hip_multiphase *s) {
   if (!s) return PHASE_HIP_OK;
   for (phase_hip_equation *e : {s->gEq, s->uEq, s->tmp, s->gTmp, s->pEq}) phase_hip_equation_destroy(e);
   for (phase_hip_solver *v : {s->sg, s->su, s->sp}) phase_hip_solver_destroy(v);
+  delete s->st;
   delete s;
   return PHASE_HIP_OK;
 }
@@ -890,11 +911,14 @@ int phase_hip_multiphase_create(phase_hip_multiphase **out, phase_hip_mesh *m, d
     for (DBuf<double> *b : {&s->gC, &s->g0C, &s->rhoC, &s->muC, &s->pC, &s->m}) b->alloc(N);
     for (DBuf<double> *b : {&s->gF, &s->g0F, &s->beta, &s->rhoF, &s->muF, &s->rho0F, &s->mu0F, &s->pF, &s->mob})
       b->alloc(F);
-    for (DBuf<double> *b : {&s->ggC, &s->grC, &s->sgC, &s->uC, &s->u0C, &s->u1C, &s->uB, &s->gpC, &s->gsrc, &s->srcT})
+    for (DBuf<double> *b : {&s->ggC, &s->grC, &s->sgC, &s->uC, &s->u0C, &s->u1C, &s->uB, &s->gpC, &s->gsrc, &s->srcT,
+                            &s->fstC})
       b->alloc(2 * (size_t)N);
-    for (DBuf<double> *b : {&s->ggF, &s->grF, &s->sgF, &s->uF, &s->u0F, &s->u1F, &s->gpF}) b->alloc(2 * (size_t)F);
+    for (DBuf<double> *b : {&s->ggF, &s->grF, &s->sgF, &s->uF, &s->u0F, &s->u1F, &s->gpF, &s->fstF})
+      b->alloc(2 * (size_t)F);
     s->maxBuf.alloc(1);
-    for (DBuf<double> *b : {&s->uC, &s->pC, &s->gpC, &s->gpF, &s->gF, &s->rhoF, &s->muF}) b->zero(st);
+    for (DBuf<double> *b : {&s->uC, &s->pC, &s->gpC, &s->gpF, &s->gF, &s->rhoF, &s->muF, &s->fstC, &s->fstF})
+      b->zero(st);
     std::vector<int> ubc(uBcType, uBcType + F), pbc(F, PHASE_HIP_BC_NORMAL_GRADIENT),
         gbc(F, PHASE_HIP_BC_NORMAL_GRADIENT);
     if (pBcType) pbc.assign(pBcType, pBcType + F);
@@ -944,6 +968,48 @@ int phase_hip_multiphase_advance(phase_hip_multiphase *s, double dt, int nSteps,
     if (nSteps < 0 || !(dt > 0.)) fail(PHASE_HIP_EINVAL, "need dt > 0 and nSteps >= 0");
     PH_CHECK(hipSetDevice(s->mesh->device));
     for (int it = 0; it < nSteps; ++it) mpStep(s, dt, history ? history + 4 * (size_t)it : nullptr);
+  });
+}
+
+int phase_hip_multiphase_surface_tension(phase_hip_multiphase *s, double sigma, double kernelRadius,
+                                         int kernelType, const double *contactAngle) {
+  return guardedMP(s, [&] {
+    if (!(sigma >= 0.)) fail(PHASE_HIP_EINVAL, "sigma must be >= 0");
+    PH_CHECK(hipSetDevice(s->mesh->device));
+    auto *cm = new phase::CelesteModel();
+    try {
+      phase::celesteSetup(s->mesh, sigma, kernelRadius, kernelType, contactAngle, *cm);
+    } catch (...) {
+      delete cm;
+      throw;
+    }
+    delete s->st;
+    s->st = cm;
+    mpSurfaceForce(s);
+    PH_CHECK(hipStreamSynchronize(s->mesh->stream));
+  });
+}
+
+int phase_hip_multiphase_surface_fields(phase_hip_multiphase *s, double *gammaTilde, double *nCell, double *nFace,
+                                        double *kappaCell, double *kappaFace, double *fstFace, double *fstCell) {
+  return guardedMP(s, [&] {
+    if (!s->st && (gammaTilde || nCell || nFace || kappaCell || kappaFace))
+      fail(PHASE_HIP_EINVAL, "surface tension is not turned on");
+    hipStream_t st = s->mesh->stream;
+    const size_t N = s->N, F = s->F, d = sizeof(double);
+    auto get = [&](double *h, const double *dv, size_t n) {
+      if (h) PH_CHECK(hipMemcpyAsync(h, dv, n * d, hipMemcpyDeviceToHost, st));
+    };
+    if (s->st) {
+      get(gammaTilde, s->st->gt.p, N);
+      get(nCell, s->st->nC.p, 2 * N);
+      get(nFace, s->st->nF.p, 2 * F);
+      get(kappaCell, s->st->kC.p, N);
+      get(kappaFace, s->st->kF.p, F);
+    }
+    get(fstFace, s->fstF.p, 2 * F);
+    get(fstCell, s->fstC.p, 2 * N);
+    PH_CHECK(hipStreamSynchronize(st));
   });
 }
 

@@ -125,6 +125,9 @@ _SIGS = {
     "phase_hip_multiphase_setup": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_char_p]),
     "phase_hip_multiphase_advance": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, _vp]),
     "phase_hip_multiphase_fields": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "phase_hip_multiphase_surface_tension": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                                             _vp]),
+    "phase_hip_multiphase_surface_fields": (ctypes.c_int, [_vp] + [_vp] * 7),
     "phase_hip_simple_create": (ctypes.c_int, [ctypes.POINTER(_vp), _vp, ctypes.c_double, ctypes.c_double,
                                                ctypes.c_double, ctypes.c_double, _i32p, _f64p, ctypes.c_int]),
     "phase_hip_simple_destroy": (ctypes.c_int, [_vp]),
@@ -544,6 +547,28 @@ class HipMultiphase:
         hist = np.zeros((n, 4))
         self._check(load_library().phase_hip_multiphase_advance(self._h, float(dt), int(n), _p(hist)), "advance")
         return hist
+
+    KERNELS = {"peskin": 0, "pow6": 1, "pow8": 2}
+
+    def surface_tension(self, sigma, kernel_radius, kernel_type="pow8", contact_angle=None):
+        """Turn on the CELESTE surface-tension force (Celeste.cpp); contact
+        angle per face in radians (None: pi/2)."""
+        ca = None if contact_angle is None else _f64(contact_angle)
+        self._check(load_library().phase_hip_multiphase_surface_tension(
+            self._h, float(sigma), float(kernel_radius), self.KERNELS[kernel_type.lower()], _p(ca)),
+            "surfaceTension")
+
+    def surface_fields(self):
+        """dict of the surface-tension model's last fields."""
+        N, F = self.mesh.n_cells, self.mesh.n_faces
+        out = dict(gamma_tilde=np.empty(N), n_cell=np.empty(2 * N), n_face=np.empty(2 * F), kappa_cell=np.empty(N),
+                   kappa_face=np.empty(F), fst_face=np.empty(2 * F), fst_cell=np.empty(2 * N))
+        keys = ("gamma_tilde", "n_cell", "n_face", "kappa_cell", "kappa_face", "fst_face", "fst_cell")
+        self._check(load_library().phase_hip_multiphase_surface_fields(self._h, *[_p(out[k]) for k in keys]),
+                    "surfaceFields")
+        for k in ("n_cell", "n_face", "fst_face", "fst_cell"):
+            out[k] = out[k].reshape(-1, 2)
+        return out
 
     def fields(self):
         """(u cells (N, 2), p cells, u faces (F, 2), gamma cells)."""

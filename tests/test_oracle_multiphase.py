@@ -51,3 +51,31 @@ def test_single_fluid_stays_at_rest(oracle, mp):
     for _ in range(2):
         s.step(1e-2)
     assert np.all(s.uC == 0.0) and np.all(s.uF == 0.0) and np.all(s.pC == 0.0)
+
+
+def test_static_drop_with_surface_tension(oracle, mp):
+    """A drop at rest without gravity: the CELESTE force is balanced by the
+    pressure jump; the projection still leaves divergence-free faces and the
+    spurious currents stay small against the capillary velocity sigma / mu."""
+    from oracle.celeste_oracle import CelesteOracle
+    n = 24
+    xy, cptr, cind = G.rectilinear(n, n)
+    m = oracle.Mesh(xy, cptr, cind)
+    F = m.n_faces
+    _, cc = m.cells()
+    r = np.hypot(cc[:, 0] - 0.5, cc[:, 1] - 0.5)
+    gam = 0.5 * (1.0 - np.tanh((r - 0.25) / (1.5 / n)))
+    ubc = np.full(F, oracle.BC_FIXED, np.int32)
+    pbc = np.full(F, oracle.BC_NORMAL_GRADIENT, np.int32)
+    pbc[0] = oracle.BC_FIXED  # one wall face fixes the pressure level
+    model = CelesteOracle(m, cptr, cind, kernel_width=2.5 / n)
+    s = mp(m, 1.0, 1.0, 0.1, 0.1, (0.0, 0.0), ubc, np.zeros(2 * F), pbc, np.zeros(F), gam,
+           surface_tension=(model, 0.05, None))
+    assert np.abs(s.fstF).max() > 0
+    for _ in range(3):
+        div = s.step(1e-3)
+        assert div <= 1e-10 * max(np.abs(s.uF).max(), 1e-30)
+    assert np.abs(s.uC).max() < 1e-3
+    # the pressure inside the drop exceeds the outside by about sigma / R
+    dp = s.pC[r < 0.1].mean() - s.pC[r > 0.4].mean()
+    assert 0.85 * 0.05 / 0.25 < dp < 1.15 * 0.05 / 0.25
