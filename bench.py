@@ -335,6 +335,56 @@ def bubble_case(args):
     return out
 
 
+def multiphase_case(args):
+    """Whole FractionalStepMultiphase time steps on the device driver, with
+    the CELESTE surface tension, on the C3 rising bubble: 1x2 quad mesh
+    stored unstructured, bubble (gamma = 1, rho2 = 1, mu2 = 0.1) of radius
+    0.25 at (0.5, 0.5) in rho1 = 1000, mu1 = 10; sigma 24.5, g = (0, -0.98),
+    dt = 1e-4; smoothing kernel radius 2.1 cells (the reference's
+    RisingBubble example: 0.021 at h = 0.01), pow8; walls u = 0, p = 0 on the
+    top.  Each step: volume-fraction solve, properties and surface force,
+    momentum solve, pressure solve, corrections."""
+    from phase_amd import hip as P
+    from phase_amd import meshgen as G
+
+    nx = int(round(np.sqrt(args.multiphase_cells / 2)))
+    h = 1.0 / nx
+    xy, cptr, cind = G.rectilinear(nx, 2 * nx, width=1.0, height=2.0)
+    mesh = P.HipMesh(xy, cptr, cind)
+    f = mesh.faces()
+    _, cc, _, _ = mesh.geometry()
+    a, bn = xy[f["lnode"]], xy[f["rnode"]]
+    top = (f["rcell"] < 0) & (a[:, 1] == 2.0) & (bn[:, 1] == 2.0)
+    pbc = np.where(top, P.BC_FIXED, P.BC_NORMAL_GRADIENT).astype(np.int32)
+    ubc = np.full(mesh.n_faces, P.BC_FIXED, np.int32)
+    r = np.hypot(cc[:, 0] - 0.5, cc[:, 1] - 0.5)
+    gamma = np.where(r < 0.25, 1.0, 0.0)
+    t0 = time.perf_counter()
+    d = P.HipMultiphase(mesh, 1000.0, 1.0, 10.0, 0.1, (0.0, -0.98), ubc, np.zeros(2 * mesh.n_faces), gamma,
+                        p_bc=pbc, p_face0=np.zeros(mesh.n_faces))
+    for k in ("gamma", "u", "p"):
+        d.setup(k + ".tolerance", args.tol)
+        d.setup(k + ".maxIters", 20000)
+        d.setup(k + ".mode", args.multiphase_mode)
+    d.setup("p.solver", "cg")
+    d.surface_tension(24.5, 2.1 * h, "pow8")
+    setup_s = time.perf_counter() - t0
+    dt = 1e-4
+    d.advance(dt, 2)
+    t0 = time.perf_counter()
+    hist = d.advance(dt, args.multiphase_steps)
+    ms = 1e3 * (time.perf_counter() - t0) / args.multiphase_steps
+    u, p, uf, g = d.fields()
+    out = dict(workload="rising-bubble-multiphase-step-1x2-quads", cells=mesh.n_cells, ms_per_step=round(ms, 2),
+               steps=args.multiphase_steps, setup_s=round(setup_s, 2), mode=args.multiphase_mode,
+               iters_gamma_u_p=[float(x) for x in hist.mean(0)[:3]], max_div=float("%.3g" % hist[:, 3].max()),
+               max_u=float("%.4g" % np.abs(u).max()))
+    log("multiphase: %.2f ms/step (setup %.1f s), iterations gamma/u/p %s, max div %.2e" %
+        (ms, setup_s, hist.mean(0)[:3], hist[:, 3].max()))
+    d.close()
+    return out
+
+
 def simple_cavity(args):
     """The C1 case on the device SIMPLE driver: lid-driven cavity on the
     128x128 unit square, Re = 100 (rho 1, mu 0.01), alpha_u 0.7, alpha_p 0.3,
@@ -403,6 +453,10 @@ def main():
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--simple-iters", type=int, default=200, help="C1 SIMPLE outer iterations (0 skips)")
     ap.add_argument("--bubble-cells", type=int, default=2_000_000, help="C3 bubble pressure solve (0 skips)")
+    ap.add_argument("--multiphase-cells", type=int, default=2_000_000,
+                    help="C3 whole multiphase time steps with surface tension (0 skips)")
+    ap.add_argument("--multiphase-steps", type=int, default=5)
+    ap.add_argument("--multiphase-mode", default="fp64", help="solver mode of the multiphase driver's solves")
     ap.add_argument("--copy-probe-mb", type=int, default=0, help="run one copy kernel of this size (counter scale)")
     ap.add_argument("--no-host-path", dest="host_path", action="store_false",
                     help="skip the host-buffer (PCIe-inclusive) solve")
@@ -532,6 +586,8 @@ def main():
         out["simple_c1"] = simple_cavity(args)
     if args.bubble_cells > 0:
         out["bubble_c3"] = bubble_case(args)
+    if args.multiphase_cells > 0:
+        out["multiphase_c3"] = multiphase_case(args)
     if args.c4_cells > 0:
         out["learned_c4"] = learned_case(args)
     if args.spmv_cells > 0:

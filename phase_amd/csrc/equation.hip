@@ -906,9 +906,12 @@ void fresh(phase_hip_equation *e) { e->compact = false; }
 namespace phase {
 
 // Solver side of FiniteVolumeEquation<T>::solve (FiniteVolumeEquation.tpp:64-86):
-// A from the equation's rows, b = -rhs.  Defined in solver.hip.
+// A from the equation's rows, b = -rhs.  Defined in solver.hip.  adjPtr /
+// adjNb (host, may be NULL): the rows' structural neighbours (for scalar mesh
+// equations the face neighbours), the largest pattern the rows can take.
 void solverSetFromRows(phase_hip_solver *s, int rank, int stride, const int *dCols, const double *dVals,
-                       const double *dRhs, hipStream_t producer, const double *dCellXY);
+                       const double *dRhs, hipStream_t producer, const double *dCellXY, const int *adjPtr,
+                       const int *adjNb);
 
 }  // namespace phase
 
@@ -1202,8 +1205,10 @@ int phase_hip_solver_set_equation(phase_hip_solver *s, const phase_hip_equation 
   if (!s || !e) return PHASE_HIP_EINVAL;
   try {
     // scalar equations carry one row per cell: their centroids order the rows
+    const bool scalar = e->ncomp == 1;
     solverSetFromRows(s, e->rank, e->stride, e->cols.p, e->vals.p, e->rhs.p, e->mesh->stream,
-                      e->ncomp == 1 ? e->mesh->dCC.p : nullptr);
+                      scalar ? e->mesh->dCC.p : nullptr, scalar ? e->mesh->iptr.data() : nullptr,
+                      scalar ? e->mesh->inb.data() : nullptr);
     return PHASE_HIP_OK;
   } catch (const HipError &x) {
     s->err = x.msg;

@@ -907,6 +907,9 @@ int phase_hip_multiphase_create(phase_hip_multiphase **out, phase_hip_mesh *m, d
       ck(phase_hip_solver_create(v, m->device), "solver", "");
       ck(phase_hip_solver_setup(*v, "tolerance", "1e-8"), "setup", (*v)->err.c_str());
       ck(phase_hip_solver_setup(*v, "maxIters", "5000"), "setup", (*v)->err.c_str());
+      // the volume-fraction rows drop the zero upwind coefficients, so their
+      // pattern follows the flow: keep one ordering for the union
+      ck(phase_hip_solver_setup(*v, "pattern", "union"), "setup", (*v)->err.c_str());
     }
     for (DBuf<double> *b : {&s->gC, &s->g0C, &s->rhoC, &s->muC, &s->pC, &s->m}) b->alloc(N);
     for (DBuf<double> *b : {&s->gF, &s->g0F, &s->beta, &s->rhoF, &s->muF, &s->rho0F, &s->mu0F, &s->pF, &s->mob})

@@ -41,6 +41,7 @@ struct phase_hip_solver {
   int syncFree = 0;  // ILU sweeps: 0 = one launch per block colour, 1 = one sync-free launch
   int geometric = 1; // cell centroids known: brick aggregates in three colours (else graph aggregates)
   int cg = 0;        // Krylov method: 0 = BiCGStab, 1 = preconditioned CG (symmetric systems)
+  int unionPattern = 0;  // keep the stored pattern when a new one lies inside it (its other entries 0)
 
   // pattern and ordering
   int rank = 0;

@@ -616,12 +616,45 @@ __global__ void kCmpInt(size_t n, const int *__restrict__ a, const int *__restri
     diff |= a[i] != b[i];
   if (diff) atomicOr(flag, 1);
 }
+// flag = 1 when a column of a new row (stride ns) is missing from the stored
+// row (stride os)
+__global__ void kSubsetCols(int rank, int ns, const int *__restrict__ nc, int os, const int *__restrict__ oc,
+                            int *flag) {
+  bool missing = false;
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < rank; r += gridDim.x * blockDim.x)
+    for (int j = 0; j < ns; ++j) {
+      const int c = nc[(size_t)r * ns + j];
+      if (c < 0) continue;
+      bool found = false;
+      for (int q = 0; q < os && !found; ++q) found = oc[(size_t)r * os + q] == c;
+      missing |= !found;
+    }
+  if (missing) atomicOr(flag, 1);
+}
+
+// the new rows' values in the stored layout, 0 where a new row has no entry
+__global__ void kRemapVals(int rank, int ns, const int *__restrict__ nc, const double *__restrict__ nv, int os,
+                           const int *__restrict__ oc, double *__restrict__ ov) {
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < rank; r += gridDim.x * blockDim.x)
+    for (int q = 0; q < os; ++q) {
+      const int c = oc[(size_t)r * os + q];
+      double v = 0.;
+      if (c >= 0)
+        for (int j = 0; j < ns; ++j)
+          if (nc[(size_t)r * ns + j] == c) {
+            v = nv[(size_t)r * ns + j];
+            break;
+          }
+      ov[(size_t)r * os + q] = v;
+    }
+}
 }  // namespace
 
 namespace phase {
 
 void solverSetFromRows(phase_hip_solver *s, int rank, int stride, const int *dCols, const double *dVals,
-                       const double *dRhs, hipStream_t producer, const double *dCellXY) {
+                       const double *dRhs, hipStream_t producer, const double *dCellXY, const int *adjPtr,
+                       const int *adjNb) {
   setDevice(s);
   if (rank <= 0) fail(PHASE_HIP_EINVAL, "empty equation");
   PH_CHECK(hipStreamSynchronize(producer));
@@ -638,15 +671,66 @@ void solverSetFromRows(phase_hip_solver *s, int rank, int stride, const int *dCo
     PH_CHECK(hipStreamSynchronize(s->stream));
     same = flag == 0;
   }
-  if (!same) {
+  // union patterns: a pattern inside the stored one keeps the ordering and
+  // factor structure, with explicit zeros for the entries it lacks
+  const bool canUnion = s->unionPattern && s->havePattern && !s->binding && s->rank == rank && s->eqStride > 0;
+  bool remap = false;
+  if (!same && canUnion) {
+    s->dEqFlag.alloc(1);
+    s->dEqFlag.zero(s->stream);
+    hipLaunchKernelGGL(kSubsetCols, dim3(gridFor(rank, 256, kElemCap)), dim3(256), 0, s->stream, rank, stride, dCols,
+                       s->eqStride, s->dEqCols.p, s->dEqFlag.p);
+    int flag = 0;
+    PH_CHECK(hipMemcpyAsync(&flag, s->dEqFlag.p, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    PH_CHECK(hipStreamSynchronize(s->stream));
+    remap = flag == 0;
+  }
+  if (!same && !remap) {
     std::vector<int> cols(slots), rowPtr(rank + 1);
     PH_CHECK(hipMemcpy(cols.data(), dCols, slots * sizeof(int), hipMemcpyDeviceToHost));
+    int useStride = stride;
+    if (s->unionPattern && (canUnion || adjPtr)) {
+      // the new entries, then the stored ones and the structural neighbours
+      // they lack: later patterns inside this union only remap values
+      std::vector<int> old;
+      if (canUnion) {
+        old.resize((size_t)rank * s->eqStride);
+        PH_CHECK(hipMemcpy(old.data(), s->dEqCols.p, old.size() * sizeof(int), hipMemcpyDeviceToHost));
+      }
+      std::vector<int> width(rank), merged;
+      merged.reserve((size_t)rank * (stride + 8));
+      std::vector<int> start(rank + 1, 0);
+      auto add = [&](size_t from, int c) {
+        if (c < 0) return;
+        for (size_t q = from; q < merged.size(); ++q)
+          if (merged[q] == c) return;
+        merged.push_back(c);
+      };
+      int wmax = 1;
+      for (int r = 0; r < rank; ++r) {
+        const size_t from = merged.size();
+        for (int j = 0; j < stride; ++j) add(from, cols[(size_t)r * stride + j]);
+        if (canUnion)
+          for (int q = 0; q < s->eqStride; ++q) add(from, old[(size_t)r * s->eqStride + q]);
+        if (adjPtr) {
+          add(from, r);
+          for (int q = adjPtr[r]; q < adjPtr[r + 1]; ++q) add(from, adjNb[q]);
+        }
+        start[r + 1] = (int)merged.size();
+        wmax = std::max(wmax, start[r + 1] - start[r]);
+      }
+      cols.assign((size_t)rank * wmax, -1);
+      for (int r = 0; r < rank; ++r)
+        std::copy(merged.begin() + start[r], merged.begin() + start[r + 1], cols.begin() + (size_t)r * wmax);
+      useStride = wmax;
+      remap = true;
+    }
     std::vector<double> xy;
     if (dCellXY && s->geometric) {
       xy.resize(2 * (size_t)rank);
       PH_CHECK(hipMemcpy(xy.data(), dCellXY, xy.size() * sizeof(double), hipMemcpyDeviceToHost));
     }
-    for (int i = 0; i <= rank; ++i) rowPtr[i] = i * stride;
+    for (int i = 0; i <= rank; ++i) rowPtr[i] = i * useStride;
     // the last solution survives a new ordering of the same rows (warm start)
     DBuf<double> keep;
     const bool warm = s->havePattern && s->rank == rank && (s->haveSolution || s->haveGuess);
@@ -664,11 +748,18 @@ void solverSetFromRows(phase_hip_solver *s, int rank, int stride, const int *dCo
                          rank, s->dPerm.p, keep.p, s->dX.p, 1.0);
       PH_CHECK(hipStreamSynchronize(s->stream));  // before `keep` is freed
     }
-    s->dEqCols.alloc(slots);  // after solverSetPattern, which clears eqStride
-    PH_CHECK(hipMemcpyAsync(s->dEqCols.p, dCols, slots * sizeof(int), hipMemcpyDeviceToDevice, s->stream));
-    s->eqStride = stride;
+    s->dEqCols.alloc((size_t)rank * useStride);  // after solverSetPattern, which clears eqStride
+    if (remap)
+      PH_CHECK(hipMemcpy(s->dEqCols.p, cols.data(), cols.size() * sizeof(int), hipMemcpyHostToDevice));
+    else
+      PH_CHECK(hipMemcpyAsync(s->dEqCols.p, dCols, slots * sizeof(int), hipMemcpyDeviceToDevice, s->stream));
+    s->eqStride = useStride;
   }
-  PH_CHECK(hipMemcpyAsync(s->dVals.p, dVals, slots * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+  if (remap)
+    hipLaunchKernelGGL(kRemapVals, dim3(gridFor(rank, 256, kElemCap)), dim3(256), 0, s->stream, rank, stride, dCols,
+                       dVals, s->eqStride, s->dEqCols.p, s->dVals.p);
+  else
+    PH_CHECK(hipMemcpyAsync(s->dVals.p, dVals, slots * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
   solverValuesChanged(s);
   hipLaunchKernelGGL((kPermuteIn<double, double>), dim3(gridFor(rank, 256, kElemCap)), dim3(256), 0, s->stream,
                      rank, s->dPerm.p, dRhs, s->dB.p, -1.0);
@@ -787,6 +878,10 @@ int phase_hip_solver_setup(phase_hip_solver *s, const char *key, const char *val
       if (g2 < 0) fail(PHASE_HIP_EINVAL, "unknown ordering '" + v + "' (geometric or graph)");
       if (g2 != s->geometric && s->havePattern) s->havePattern = false;  // rebuild on next set
       s->geometric = g2;
+    } else if (k == "pattern") {
+      if (lv == "exact") s->unionPattern = 0;
+      else if (lv == "union") s->unionPattern = 1;
+      else fail(PHASE_HIP_EINVAL, "unknown pattern policy '" + v + "' (exact or union)");
     } else if (k == "sweep") {
       if (lv == "colour" || lv == "color" || lv == "0") s->syncFree = 0;
       else if (lv == "syncfree" || lv == "1") s->syncFree = 1;
