@@ -387,6 +387,33 @@ int phase_hip_vof_properties(phase_hip_mesh *m, double rho1, double rho2, double
 /* Wait for the work queued on the mesh's stream. */
 int phase_hip_mesh_synchronize(phase_hip_mesh *m);
 
+/* ------------------------------------------------------------------ ghost-cell immersed boundary
+ * GhostCellImmersedBoundary (src/2D/Unstructured/FiniteVolume/ImmersedBoundary/
+ * GhostCellImmersedBoundary.cpp:12-90 with its BcStencil, FixedBcStencil and
+ * NormalGradientBcStencil) for circular objects (cx, cy, r per object, in
+ * order): updateCells classifies the cells (0 fluid, 1 ghost: inside with a
+ * face or node neighbour outside, 2 solid) and the ghost stencils are built
+ * once (image point, the four cells of the nearest mesh node, their bilinear
+ * interpolation; the node needs four cells, as on quad meshes).
+ * phase_hip_eq_gcib_bcs is bcs(phi) (:33-70): a fresh scalar equation with
+ * the FIXED or NORMAL_GRADIENT stencil rows (source -refValue), unit solid
+ * rows (source -refValue for FIXED, 0 for NORMAL_GRADIENT) and empty fluid
+ * rows.  phase_hip_eq_gcib_velocity_bcs is velocityBcs(u) (:72-90) for
+ * translating objects: FIXED stencils on both components, source minus the
+ * object's velocity (2 per object).  A caller assembling over the fluid cells
+ * removes the other rows of its fv:: equations (phase_hip_eq_remove_cells
+ * with the cell types as mask) and adds these with phase_hip_eq_add.
+ * phase_hip_gcib_cell_types copies the types (and owners, host only) to host
+ * memory, or the types to device memory when onDevice is nonzero. */
+typedef struct phase_hip_gcib phase_hip_gcib;
+
+int phase_hip_gcib_create(phase_hip_gcib **ib, phase_hip_mesh *m, int nCircles, const double *circles);
+int phase_hip_gcib_destroy(phase_hip_gcib *ib);
+const char *phase_hip_gcib_last_error(const phase_hip_gcib *ib);
+int phase_hip_gcib_cell_types(const phase_hip_gcib *ib, int *types, int *owner, int onDevice);
+int phase_hip_eq_gcib_bcs(phase_hip_equation *e, const phase_hip_gcib *ib, int bcType, double refValue);
+int phase_hip_eq_gcib_velocity_bcs(phase_hip_equation *e, const phase_hip_gcib *ib, const double *objectVelocity);
+
 /* ------------------------------------------------------------------ fractional step
  * FractionalStep::solve(timeStep) -- src/2D/Unstructured/Solvers/FractionalStep.cpp:36-117,
  * one time step on the device (all fields in HBM):

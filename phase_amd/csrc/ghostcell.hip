@@ -1,0 +1,373 @@
+// Ghost-cell immersed boundary rows (src/2D/Unstructured/FiniteVolume/ImmersedBoundary/
+// GhostCellImmersedBoundary.cpp:12-90, GhostCellImmersedBoundaryBcStencil.cpp:3-70,
+// GhostCellImmersedBoundaryFixedBcStencil.cpp:17-22,
+// GhostCellImmersedBoundaryNormalGradientBcStencil.cpp:22-42) for circular objects.
+//
+// Set-up, host, once per mesh and object set (GhostCellImmersedBoundary::updateCells):
+//   per object in order, the remaining domain cells whose centroid lies
+//   strictly inside the circle are solid; those with a face or node neighbour
+//   outside it (internalPerimeterCells with diagonals) are ghost cells;
+//   ghost-cell stencil: bp = centre + R (x_c - centre) / |x_c - centre|,
+//   ip = 2 bp - x_c, nw = (centre - bp) / |centre - bp|, the four cells of the
+//   mesh node nearest to ip (lowest node id on a tie) and Ainv, the inverse of
+//   their bilinear rows (x y, x, y, 1) (Gauss-Jordan with partial pivoting;
+//   the reference calls LAPACK getrf/getri).
+// Device, per call: a fresh equation whose ghost rows hold the FIXED or
+// NORMAL_GRADIENT stencil, whose solid rows hold a unit diagonal, and whose
+// fluid rows are empty; callers remove the non-fluid rows of their fv::
+// equations (remove(cell)) and add these.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "equation.h"
+#include "mesh.h"
+
+using namespace phase;
+
+struct phase_hip_gcib {
+  phase_hip_mesh *mesh = nullptr;
+  int N = 0, nObj = 0;
+  std::vector<int> types, owner;
+  DBuf<int> dTypes, dStencil, dFixCols, dNgCols;  // per cell: stencil index or -1; 5 columns per stencil
+  DBuf<double> dFixVals, dNgVals;
+  std::string err;
+};
+
+namespace {
+
+inline int g256g(long n) { return (int)std::max<long>(1, std::min<long>((n + 255) / 256, 8192)); }
+
+template <class T>
+std::vector<T> fetch(const DBuf<T> &d, size_t n, hipStream_t st) {
+  std::vector<T> h(n);
+  if (n) PH_CHECK(hipMemcpyAsync(h.data(), d.p, n * sizeof(T), hipMemcpyDeviceToHost, st));
+  PH_CHECK(hipStreamSynchronize(st));
+  return h;
+}
+
+// inverse of a 4x4 row-major matrix, Gauss-Jordan with partial pivoting
+bool inverse4(const double *a, double *inv) {
+  double m[4][8];
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+      m[i][j] = a[4 * i + j];
+      m[i][4 + j] = i == j ? 1. : 0.;
+    }
+  for (int c = 0; c < 4; ++c) {
+    int p = c;
+    for (int i = c + 1; i < 4; ++i)
+      if (std::fabs(m[i][c]) > std::fabs(m[p][c])) p = i;
+    if (!(std::fabs(m[p][c]) > 0.)) return false;
+    if (p != c)
+      for (int j = 0; j < 8; ++j) std::swap(m[p][j], m[c][j]);
+    const double d = m[c][c];
+    for (int j = 0; j < 8; ++j) m[c][j] /= d;
+    for (int i = 0; i < 4; ++i)
+      if (i != c) {
+        const double f = m[i][c];
+        if (f != 0.)
+          for (int j = 0; j < 8; ++j) m[i][j] -= f * m[c][j];
+      }
+  }
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) inv[4 * i + j] = m[i][4 + j];
+  return true;
+}
+
+// one fresh row per (component, cell): stencil, unit diagonal or empty
+__global__ void kEqGhostRows(int n, int ncomp, int stride, const int *__restrict__ types,
+                             const int *__restrict__ stencil, const int *__restrict__ scols,
+                             const double *__restrict__ svals, const double *__restrict__ src, int *cols,
+                             double *vals, int *cnt, double *rhs) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+    for (int q = 0; q < ncomp; ++q) {
+      const long row = (long)q * n + c;
+      const long base = row * stride;
+      for (int j = 0; j < stride; ++j) {
+        cols[base + j] = -1;
+        vals[base + j] = 0.;
+      }
+      int used = 0;
+      auto add = [&](int col, double v) {  // CrsEquation::addCoeff, first touch
+        for (int j = 0; j < used; ++j)
+          if (cols[base + j] == col) {
+            vals[base + j] += v;
+            return;
+          }
+        cols[base + used] = col;
+        vals[base + used] = v;
+        ++used;
+      };
+      const int t = types[c];
+      if (t == 1) {
+        const int s = stencil[c];
+        for (int j = 0; j < 5; ++j) {
+          const int k = scols[5 * s + j];
+          if (k >= 0) add(q * n + k, svals[5 * s + j]);
+        }
+      } else if (t == 2) {
+        add((int)row, 1.);
+      }
+      cnt[row] = used;
+      rhs[row] = t != 0 ? 0. + src[(long)c * ncomp + q] : 0.;
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int phase_hip_gcib_destroy(phase_hip_gcib *ib) {
+  delete ib;
+  return PHASE_HIP_OK;
+}
+
+const char *phase_hip_gcib_last_error(const phase_hip_gcib *ib) { return ib ? ib->err.c_str() : "null object"; }
+
+int phase_hip_gcib_create(phase_hip_gcib **out, phase_hip_mesh *m, int nCircles, const double *circles) {
+  if (!out || !m || nCircles < 0 || (nCircles > 0 && !circles)) return PHASE_HIP_EINVAL;
+  *out = nullptr;
+  auto *ib = new phase_hip_gcib();
+  try {
+    PH_CHECK(hipSetDevice(m->device));
+    hipStream_t st = m->stream;
+    const int N = m->nCells;
+    ib->mesh = m;
+    ib->N = N;
+    ib->nObj = nCircles;
+    const std::vector<double> cc = fetch(m->dCC, 2 * (size_t)N, st), xy = fetch(m->dXY, 2 * (size_t)m->nNodes, st);
+    const std::vector<int> cptr = fetch(m->dCptr, N + 1, st);
+    const std::vector<int> cind = fetch(m->dCind, cptr[N], st);
+    std::vector<std::vector<int>> nodeCells(m->nNodes);
+    for (int c = 0; c < N; ++c)
+      for (int k = cptr[c]; k < cptr[c + 1]; ++k) nodeCells[cind[k]].push_back(c);
+    // Cell::cellLinks: face neighbours and diagonal (node) neighbours
+    auto forLinks = [&](int c, auto &&fn) {
+      for (int e = m->iptr[c]; e < m->iptr[c + 1]; ++e) fn(m->inb[e]);
+      for (int k = cptr[c]; k < cptr[c + 1]; ++k)
+        for (int kc : nodeCells[cind[k]])
+          if (kc != c) fn(kc);
+    };
+    ib->types.assign(N, 0);
+    ib->owner.assign(N, -1);
+    std::vector<char> inside(N);
+    for (int o = 0; o < nCircles; ++o) {
+      const double cx = circles[3 * o], cy = circles[3 * o + 1], r = circles[3 * o + 2];
+      if (!(r > 0.)) fail(PHASE_HIP_EINVAL, "circle radius must be positive");
+      for (int c = 0; c < N; ++c) {
+        const double dx = cc[2 * c] - cx, dy = cc[2 * c + 1] - cy;
+        inside[c] = dx * dx + dy * dy < r * r;  // Circle::isInside
+      }
+      for (int c = 0; c < N; ++c) {
+        if (!inside[c] || ib->types[c] != 0) continue;
+        bool ghost = false;
+        forLinks(c, [&](int k) { ghost |= !inside[k]; });
+        ib->types[c] = ghost ? 1 : 2;
+        ib->owner[c] = o;
+      }
+    }
+    // nearest node: bins over the nodes
+    double x0 = xy[0], x1 = xy[0], y0 = xy[1], y1 = xy[1];
+    for (int i = 0; i < m->nNodes; ++i) {
+      x0 = std::min(x0, xy[2 * i]);
+      x1 = std::max(x1, xy[2 * i]);
+      y0 = std::min(y0, xy[2 * i + 1]);
+      y1 = std::max(y1, xy[2 * i + 1]);
+    }
+    const long nb = std::max<long>(1, std::min<long>(2048, (long)std::sqrt((double)m->nNodes)));
+    const double bw = (x1 - x0) / nb + 1e-300, bh = (y1 - y0) / nb + 1e-300;
+    auto bx = [&](double v) { return std::min<long>(nb - 1, std::max<long>(0, (long)((v - x0) / bw))); };
+    auto by = [&](double v) { return std::min<long>(nb - 1, std::max<long>(0, (long)((v - y0) / bh))); };
+    std::vector<int> bptr(nb * nb + 1, 0), bnode(m->nNodes);
+    for (int i = 0; i < m->nNodes; ++i) ++bptr[by(xy[2 * i + 1]) * nb + bx(xy[2 * i]) + 1];
+    for (long b = 0; b < nb * nb; ++b) bptr[b + 1] += bptr[b];
+    {
+      std::vector<int> fill(bptr.begin(), bptr.end() - 1);
+      for (int i = 0; i < m->nNodes; ++i) bnode[fill[by(xy[2 * i + 1]) * nb + bx(xy[2 * i])]++] = i;
+    }
+    auto nearestNode = [&](double px, double py) {
+      const long ix = bx(px), iy = by(py);
+      int best = -1;
+      double bd = 0.;
+      for (long ring = 0; ring <= nb; ++ring) {
+        for (long j = iy - ring; j <= iy + ring; ++j)
+          for (long i = ix - ring; i <= ix + ring; ++i) {
+            if (j < 0 || i < 0 || j >= nb || i >= nb) continue;
+            if (std::max(std::labs(j - iy), std::labs(i - ix)) != ring) continue;
+            for (int q = bptr[j * nb + i]; q < bptr[j * nb + i + 1]; ++q) {
+              const int k = bnode[q];
+              const double dx = xy[2 * k] - px, dy = xy[2 * k + 1] - py;
+              const double d = dx * dx + dy * dy;
+              if (best < 0 || d < bd || (d == bd && k < best)) {
+                best = k;
+                bd = d;
+              }
+            }
+          }
+        // every node outside the searched rings is at least ring * bin away
+        if (best >= 0 && (double)ring * std::min(bw, bh) > std::sqrt(bd)) break;
+      }
+      return best;
+    };
+    std::vector<int> stencil(N, -1), fixCols, ngCols;
+    std::vector<double> fixVals, ngVals;
+    for (int c = 0; c < N; ++c) {
+      if (ib->types[c] != 1) continue;
+      const int o = ib->owner[c];
+      const double cx = circles[3 * o], cy = circles[3 * o + 1], r = circles[3 * o + 2];
+      const double xc = cc[2 * c], yc = cc[2 * c + 1];
+      // Circle::nearestIntersect, ImmersedBoundaryObject::nearestEdgeUnitNormal
+      const double dx = xc - cx, dy = yc - cy, dm = std::sqrt(dx * dx + dy * dy);
+      const double bpx = dx / dm * r + cx, bpy = dy / dm * r + cy;
+      const double ipx = 2. * bpx - xc, ipy = 2. * bpy - yc;
+      const double wx = cx - bpx, wy = cy - bpy, wm = std::sqrt(wx * wx + wy * wy);
+      const double nwx = wx / wm, nwy = wy / wm;
+      const int node = nearestNode(ipx, ipy);
+      const std::vector<int> &cells = nodeCells[node];
+      if (cells.size() != 4)
+        fail(PHASE_HIP_EINVAL, "ghost-cell stencil of cell " + std::to_string(c) + ": nearest node has " +
+                                   std::to_string(cells.size()) + " cells (four needed)");
+      double A[16], Ai[16];
+      for (int i = 0; i < 4; ++i) {
+        const double x = cc[2 * cells[i]], y = cc[2 * cells[i] + 1];
+        A[4 * i] = x * y;
+        A[4 * i + 1] = x;
+        A[4 * i + 2] = y;
+        A[4 * i + 3] = 1.;
+      }
+      if (!inverse4(A, Ai)) fail(PHASE_HIP_EINVAL, "singular ghost-cell interpolation at cell " + std::to_string(c));
+      const double v[4] = {ipx * ipy, ipx, ipy, 1.};
+      double va[4];
+      for (int j = 0; j < 4; ++j) {
+        double s = 0.;
+        for (int k = 0; k < 4; ++k) s += v[k] * Ai[4 * k + j];
+        va[j] = s;
+      }
+      stencil[c] = (int)(fixCols.size() / 5);
+      for (int j = 0; j < 4; ++j) {
+        fixCols.push_back(cells[j]);
+        fixVals.push_back(va[j] / 2.);
+      }
+      fixCols.push_back(c);
+      fixVals.push_back(0.5);
+      if (std::find(cells.begin(), cells.end(), c) != cells.end()) {
+        const double g[4] = {bpy * nwx + bpx * nwy, nwx, nwy, 0.};
+        for (int j = 0; j < 4; ++j) {
+          double s = 0.;
+          for (int k = 0; k < 4; ++k) s += g[k] * Ai[4 * k + j];
+          ngCols.push_back(cells[j]);
+          ngVals.push_back(s);
+        }
+        ngCols.push_back(-1);
+        ngVals.push_back(0.);
+      } else {
+        const double ex = ipx - xc, ey = ipy - yc, l = std::sqrt(ex * ex + ey * ey);
+        for (int j = 0; j < 4; ++j) {
+          ngCols.push_back(cells[j]);
+          ngVals.push_back(-va[j] / l);
+        }
+        ngCols.push_back(c);
+        ngVals.push_back(1. / l);
+      }
+    }
+    ib->dTypes.upload(ib->types, st);
+    ib->dStencil.upload(stencil, st);
+    ib->dFixCols.upload(fixCols, st);
+    ib->dFixVals.upload(fixVals, st);
+    ib->dNgCols.upload(ngCols, st);
+    ib->dNgVals.upload(ngVals, st);
+    PH_CHECK(hipStreamSynchronize(st));
+  } catch (const HipError &x) {
+    delete ib;
+    return x.code;
+  } catch (const std::exception &) {
+    delete ib;
+    return PHASE_HIP_EINVAL;
+  }
+  *out = ib;
+  return PHASE_HIP_OK;
+}
+
+int phase_hip_gcib_cell_types(const phase_hip_gcib *ib, int *types, int *owner, int onDevice) {
+  if (!ib) return PHASE_HIP_EINVAL;
+  try {
+    hipStream_t st = ib->mesh->stream;
+    if (types) {
+      if (onDevice)
+        PH_CHECK(hipMemcpyAsync(types, ib->dTypes.p, ib->N * sizeof(int), hipMemcpyDeviceToDevice, st));
+      else
+        std::copy(ib->types.begin(), ib->types.end(), types);
+    }
+    if (owner) std::copy(ib->owner.begin(), ib->owner.end(), owner);
+    PH_CHECK(hipStreamSynchronize(st));
+  } catch (const HipError &x) {
+    return x.code;
+  }
+  return PHASE_HIP_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+int ghostRows(phase_hip_equation *e, const phase_hip_gcib *ib, int ncomp, bool fixed, const std::vector<double> &src) {
+  if (!e || !ib) return PHASE_HIP_EINVAL;
+  try {
+    if (e->mesh != ib->mesh) fail(PHASE_HIP_EINVAL, "equation and immersed boundary of different meshes");
+    if (e->ncomp != ncomp) fail(PHASE_HIP_EINVAL, ncomp == 1 ? "bcs needs a scalar equation" :
+                                                              "velocityBcs needs a Vector2D equation");
+    if (e->stride < 5) fail(PHASE_HIP_EINVAL, "equation rows hold fewer than five entries");
+    phase_hip_mesh *m = e->mesh;
+    PH_CHECK(hipSetDevice(m->device));
+    DBuf<double> dSrc;
+    dSrc.upload(src, m->stream);
+    hipLaunchKernelGGL(kEqGhostRows, dim3(g256g(m->nCells)), dim3(256), 0, m->stream, m->nCells, ncomp, e->stride,
+                       ib->dTypes.p, ib->dStencil.p, fixed ? ib->dFixCols.p : ib->dNgCols.p,
+                       fixed ? ib->dFixVals.p : ib->dNgVals.p, dSrc.p, e->cols.p, e->vals.p, e->cnt.p, e->rhs.p);
+    PH_CHECK(hipGetLastError());
+    PH_CHECK(hipStreamSynchronize(m->stream));  // before dSrc is freed
+    e->compact = false;
+  } catch (const HipError &x) {
+    e->err = x.msg;
+    return x.code;
+  } catch (const std::exception &x) {
+    e->err = x.what();
+    return PHASE_HIP_EINVAL;
+  }
+  return PHASE_HIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int phase_hip_eq_gcib_bcs(phase_hip_equation *e, const phase_hip_gcib *ib, int bcType, double refValue) {
+  if (!ib || !e) return PHASE_HIP_EINVAL;
+  if (bcType != PHASE_HIP_BC_FIXED && bcType != PHASE_HIP_BC_NORMAL_GRADIENT) {
+    e->err = "GhostCellImmersedBoundary::bcs: invalid boundary condition type";
+    return PHASE_HIP_EBOUNDARY;
+  }
+  const bool fixed = bcType == PHASE_HIP_BC_FIXED;
+  std::vector<double> src(ib->N, 0.);
+  for (int c = 0; c < ib->N; ++c)
+    if (ib->types[c] == 1 || (ib->types[c] == 2 && fixed)) src[c] = -refValue;
+  return ghostRows(e, ib, 1, fixed, src);
+}
+
+int phase_hip_eq_gcib_velocity_bcs(phase_hip_equation *e, const phase_hip_gcib *ib, const double *objectVelocity) {
+  if (!ib || !e || (ib->nObj > 0 && !objectVelocity)) return PHASE_HIP_EINVAL;
+  std::vector<double> src(2 * (size_t)ib->N, 0.);
+  for (int c = 0; c < ib->N; ++c)
+    if (ib->types[c] != 0) {
+      src[2 * c] = -objectVelocity[2 * ib->owner[c]];
+      src[2 * c + 1] = -objectVelocity[2 * ib->owner[c] + 1];
+    }
+  return ghostRows(e, ib, 2, true, src);
+}
+
+}  // extern "C"

@@ -118,6 +118,12 @@ _SIGS = {
     "phase_hip_fstep_setup": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_char_p]),
     "phase_hip_fstep_advance": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, _vp]),
     "phase_hip_fstep_fields": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    "phase_hip_gcib_create": (ctypes.c_int, [ctypes.POINTER(_vp), _vp, ctypes.c_int, _f64p]),
+    "phase_hip_gcib_destroy": (ctypes.c_int, [_vp]),
+    "phase_hip_gcib_last_error": (ctypes.c_char_p, [_vp]),
+    "phase_hip_gcib_cell_types": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int]),
+    "phase_hip_eq_gcib_bcs": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_double]),
+    "phase_hip_eq_gcib_velocity_bcs": (ctypes.c_int, [_vp, _vp, _f64p]),
     "phase_hip_multiphase_create": (ctypes.c_int, [ctypes.POINTER(_vp), _vp] + [ctypes.c_double] * 6 +
                                     [_i32p, _f64p, _vp, _vp, _vp, _f64p, ctypes.c_int]),
     "phase_hip_multiphase_destroy": (ctypes.c_int, [_vp]),
@@ -416,6 +422,17 @@ class HipEquation:
                     "cicsam::div")
         return self
 
+    def gcib_bcs(self, ib, bc_type, ref_value):
+        """GhostCellImmersedBoundary::bcs(phi) (scalar equation)."""
+        self._check(load_library().phase_hip_eq_gcib_bcs(self._h, ib._h, int(bc_type), float(ref_value)), "bcs")
+        return self
+
+    def gcib_velocity_bcs(self, ib, velocities):
+        """GhostCellImmersedBoundary::velocityBcs(u) for translating objects."""
+        v = _f64(np.asarray(velocities, float).reshape(-1))
+        self._check(load_library().phase_hip_eq_gcib_velocity_bcs(self._h, ib._h, v), "velocityBcs")
+        return self
+
     def __iadd__(self, other):
         self._check(load_library().phase_hip_eq_add(self._h, other._h, 1), "operator+=")
         return self
@@ -510,6 +527,44 @@ class HipFractionalStep:
         except Exception:
             pass
 
+
+
+class HipGhostCellIB:
+    """GhostCellImmersedBoundary for circular objects ((cx, cy, r) each):
+    cell types (0 fluid, 1 ghost, 2 solid) and the boundary rows."""
+
+    def __init__(self, mesh, circles):
+        h = _vp()
+        c = _f64(np.asarray(circles, float).reshape(-1))
+        rc = load_library().phase_hip_gcib_create(ctypes.byref(h), mesh._h, len(c) // 3, c)
+        if rc != 0:
+            raise PhaseHipError(rc, "ghostCellCreate", "creation failed")
+        self._h = h
+        self.mesh = mesh
+
+    def cell_types(self):
+        N = self.mesh.n_cells
+        t, o = np.empty(N, np.int32), np.empty(N, np.int32)
+        rc = load_library().phase_hip_gcib_cell_types(self._h, _p(t), _p(o), 0)
+        if rc != 0:
+            raise PhaseHipError(rc, "cellTypes", "")
+        return t, o
+
+    def cell_types_device(self, out):
+        rc = load_library().phase_hip_gcib_cell_types(self._h, out.ptr, None, 1)
+        if rc != 0:
+            raise PhaseHipError(rc, "cellTypes", "")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().phase_hip_gcib_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class HipMultiphase:
