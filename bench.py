@@ -385,6 +385,62 @@ def multiphase_case(args):
     return out
 
 
+def ib_case(args):
+    """The C5 pressure system with the ghost-cell immersed boundary: 30 x 20
+    channel of quads stored unstructured, cylinder of diameter 1 at (10, 10);
+    FractionalStepGCIB::solvePEqn's laplacian(dt / rho, p) + ib.bcs(p) ==
+    src::div(u) (FractionalStepGCIB.cpp:44-46) with rho = 1, dt = 2e-3, the
+    laplacian rows on the fluid cells only (remove(cell)), NORMAL_GRADIENT
+    ghost rows on the cylinder, p = 0 on the outlet (x = 30) and zero
+    gradient elsewhere; src::div(u) stands in as a zero-mean N(0,1) field on
+    the fluid cells.  The ghost rows are not symmetric: BiCGStab, plain FP64
+    against mixed precision, to 1e-8."""
+    from phase_amd import hip as P
+    from phase_amd import meshgen as G
+
+    ny = int(round(np.sqrt(args.ib_cells / 1.5)))
+    nx = int(round(1.5 * ny))
+    xy, cptr, cind = G.rectilinear(nx, ny, width=30.0, height=20.0)
+    mesh = P.HipMesh(xy, cptr, cind)
+    f = mesh.faces()
+    a, bn = xy[f["lnode"]], xy[f["rnode"]]
+    outlet = (f["rcell"] < 0) & (a[:, 0] == 30.0) & (bn[:, 0] == 30.0)
+    bc = np.where(outlet, P.BC_FIXED, P.BC_NORMAL_GRADIENT).astype(np.int32)
+    ib = P.HipGhostCellIB(mesh, [(10.0, 10.0, 0.5)])
+    types, _ = ib.cell_types()
+    rhs = G.zero_mean_normal(mesh.n_cells, seed=11) * (types == 0)
+    d_bc, d_pf, d_b = P.DeviceArray(bc), P.DeviceArray(np.zeros(mesh.n_faces)), P.DeviceArray(rhs)
+    d_mask = P.DeviceArray.zeros(mesh.n_cells, np.int32)
+    ib.cell_types_device(d_mask)
+    eq = P.HipEquation(mesh, 1)
+    ibq = P.HipEquation(mesh, 1)
+    out = dict(workload="cylinder-ghost-cell-pressure-30x20-quads", cells=mesh.n_cells,
+               ghost_cells=int((types == 1).sum()), solid_cells=int((types == 2).sum()))
+    for mode in ("fp64", "mixed"):
+        s = P.HipSparseMatrixSolver(0, tolerance=args.tol, maxIters=100000, mode=mode, solver="bicgstab")
+
+        def step():
+            nonlocal eq
+            eq.laplacian(1, gamma=2e-3, bc_type=d_bc, phi_face=d_pf)
+            eq.remove_cells(d_mask)
+            eq += ibq.gcib_bcs(ib, P.BC_NORMAL_GRADIENT, 0.0)
+            eq.sub_vector(d_b)
+            s.set_equation(eq)
+            s.zero_guess()
+            return s.solve(raise_on_noconv=False)
+
+        step()
+        t0 = time.perf_counter()
+        rel = step()
+        dt = time.perf_counter() - t0
+        st = s.stats()
+        out[mode] = dict(ms_per_step=round(1e3 * dt, 2), relres=float("%.4g" % rel), iters=st["inner_iters"],
+                         fallbacks=st["fallbacks"])
+        log("ib %s: %.2f ms/step, relres %.2e, %d iterations" % (mode, 1e3 * dt, rel, st["inner_iters"]))
+        s.close()
+    return out
+
+
 def simple_cavity(args):
     """The C1 case on the device SIMPLE driver: lid-driven cavity on the
     128x128 unit square, Re = 100 (rho 1, mu 0.01), alpha_u 0.7, alpha_p 0.3,
@@ -456,6 +512,8 @@ def main():
     ap.add_argument("--multiphase-cells", type=int, default=2_000_000,
                     help="C3 whole multiphase time steps with surface tension (0 skips)")
     ap.add_argument("--multiphase-steps", type=int, default=5)
+    ap.add_argument("--ib-cells", type=int, default=1_000_000,
+                    help="C5 ghost-cell pressure system at this many cells (0 skips)")
     ap.add_argument("--multiphase-mode", default="fp64", help="solver mode of the multiphase driver's solves")
     ap.add_argument("--copy-probe-mb", type=int, default=0, help="run one copy kernel of this size (counter scale)")
     ap.add_argument("--no-host-path", dest="host_path", action="store_false",
@@ -588,6 +646,8 @@ def main():
         out["bubble_c3"] = bubble_case(args)
     if args.multiphase_cells > 0:
         out["multiphase_c3"] = multiphase_case(args)
+    if args.ib_cells > 0:
+        out["ib_c5"] = ib_case(args)
     if args.c4_cells > 0:
         out["learned_c4"] = learned_case(args)
     if args.spmv_cells > 0:
