@@ -514,7 +514,7 @@ def main():
     ap.add_argument("--multiphase-steps", type=int, default=5)
     ap.add_argument("--ib-cells", type=int, default=1_000_000,
                     help="C5 ghost-cell pressure system at this many cells (0 skips)")
-    ap.add_argument("--multiphase-mode", default="fp64", help="solver mode of the multiphase driver's solves")
+    ap.add_argument("--multiphase-mode", default="mixed", help="solver mode of the multiphase driver's solves")
     ap.add_argument("--copy-probe-mb", type=int, default=0, help="run one copy kernel of this size (counter scale)")
     ap.add_argument("--no-host-path", dest="host_path", action="store_false",
                     help="skip the host-buffer (PCIe-inclusive) solve")
