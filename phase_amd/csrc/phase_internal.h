@@ -141,6 +141,7 @@ struct Scalars {
   int done;               // 0 running, 1 converged, 2 breakdown, 3 max iterations, 4 stagnation
   int iters, maxIters, pad;
   double minRatio;        // learned mode: smallest accepted ||r_k||/||r_{k-1}|| ... (<=0 disables)
+  double rhoNext;         // CG: (r, z) of the newest z, taken over as rho by the next alpha
 };
 
 enum { kDoneRunning = 0, kDoneConverged = 1, kDoneBreakdown = 2, kDoneMaxIters = 3,

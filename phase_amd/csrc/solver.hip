@@ -261,9 +261,9 @@ void cgBeta(phase_hip_solver *s, KrylovVecs<T> &K, Scalars *sc) {
   const int n = s->rank;
   const int gu = gridFor(n, kRedBlock, kRedCap);
   hipLaunchKernelGGL((kDotRZ<T>), dim3(gu), dim3(kRedBlock), 0, s->stream, n, sc, K.r.p, K.ph.p, s->dPartials.p);
-  hipLaunchKernelGGL(kFinishCgZ, dim3(1), dim3(kRedBlock), 0, s->stream, sc, s->dPartials.p, gu);
-  hipLaunchKernelGGL((kCgDirection<T>), dim3(gridFor(n, 256, kElemCap)), dim3(256), 0, s->stream, n, sc, K.ph.p,
-                     K.p.p);
+  // each direction block re-reads the gu partials: keep the grid near gu
+  hipLaunchKernelGGL((kCgDirectionBeta<T>), dim3(gridFor(n, 4 * kRedBlock, kRedCap)), dim3(kRedBlock), 0, s->stream,
+                     n, sc, s->dPartials.p, gu, K.ph.p, K.p.p);
 }
 
 // Start of the CG recurrence from r (p is zero): z = M^{-1} r, p = z.

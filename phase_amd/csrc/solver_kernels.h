@@ -464,7 +464,7 @@ static __global__ __launch_bounds__(kRedBlock) void kFinishUpdate(Scalars *sc, c
 // symmetric systems.  One iteration: q = A p with (p, q) and (r, r)
 // (kSpmvDotCg, kFinishCgQ: stopping test, alpha); x += alpha p and
 // r -= alpha q inside the forward sweeps that apply M^{-1} to the new r
-// (kProCgR); (r, z) and beta (kDotRZ, kFinishCgZ); p = z + beta p.
+// (kProCgR); (r, z) partials (kDotRZ); beta and p = z + beta p (kCgDirectionBeta).
 
 // q = A p ; partials of (p, q) and of (r, r) for the stopping test (r as the
 // previous iteration's sweeps left it)
@@ -498,6 +498,7 @@ static __global__ __launch_bounds__(kRedBlock) void kFinishCgQ(Scalars *sc, cons
   finishPartials<2>(partials, nb, d);
   if (threadIdx.x != 0) return;
   const double pq = d[0], rr = d[1];
+  sc->rho = sc->rhoNext;
   sc->rrPrev = sc->rr;
   sc->rr = rr;
   sc->rv = pq;
@@ -524,26 +525,36 @@ __global__ __launch_bounds__(kRedBlock) void kDotRZ(int n, const Scalars *sc, co
   blockPartials<1>(d, partials, blockIdx.x);
 }
 
-// rho' = (r, z), beta = rho' / rho  (at the start p is zero)
-static __global__ __launch_bounds__(kRedBlock) void kFinishCgZ(Scalars *sc, const double *partials, int nb) {
+// rho' = (r, z), beta = rho' / rho and p = z + beta p in one launch: every
+// block adds up the (r, z) partials itself, in the same fixed order, so all
+// blocks hold the same beta; block 0 records beta and rho' (as rhoNext: rho
+// itself is still being read by the other blocks; the next alpha takes it
+// over).  At the start p is zero.
+template <class T>
+__global__ __launch_bounds__(kRedBlock) void kCgDirectionBeta(int n, Scalars *sc, const double *partials, int nb,
+                                                              const T *__restrict__ z, T *__restrict__ p) {
   if (sc->done) return;
   double d[1];
   finishPartials<1>(partials, nb, d);
-  if (threadIdx.x != 0) return;
-  const double rhoNew = d[0];
-  if (!isfinite(rhoNew) || rhoNew == 0. || sc->rho == 0.) {
-    sc->done = kDoneBreakdown;
-    return;
+  __shared__ double sBeta;
+  __shared__ int sBad;
+  if (threadIdx.x == 0) {
+    const double rhoNew = d[0], rho = sc->rho;
+    const bool bad = !isfinite(rhoNew) || rhoNew == 0. || rho == 0.;
+    sBad = bad;
+    sBeta = bad ? 0. : rhoNew / rho;
+    if (blockIdx.x == 0) {
+      if (bad) {
+        sc->done = kDoneBreakdown;
+      } else {
+        sc->beta = sBeta;
+        sc->rhoNext = rhoNew;
+      }
+    }
   }
-  sc->beta = rhoNew / sc->rho;
-  sc->rho = rhoNew;
-}
-
-// p = z + beta p
-template <class T>
-__global__ void kCgDirection(int n, const Scalars *sc, const T *__restrict__ z, T *__restrict__ p) {
-  if (sc->done) return;
-  const T beta = (T)sc->beta;
+  __syncthreads();
+  if (sBad) return;
+  const T beta = (T)sBeta;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = z[i] + beta * p[i];
 }
 
