@@ -285,13 +285,13 @@ void iterate(phase_hip_solver *s, KrylovVecs<T> &K, DBuf<T> &fac, const TV *sell
   const int gs = spmvGrid(s);
   hipLaunchKernelGGL((kSpmvDotV<T, TV>), dim3(gs), dim3(kRedBlock), 0, s->stream, n, s->ord.nSlices, s->dSliceOff.p,
                      s->dSellCol.p, sell, K.ph.p, K.v.p, K.r0.p, sc, s->dPartials.p);
-  hipLaunchKernelGGL(kFinishV, dim3(1), dim3(kRedBlock), 0, s->stream, sc, s->dPartials.p, gs);
+  hipLaunchKernelGGL(kFinishV, dim3(1), dim3(kFinBlock), 0, s->stream, sc, s->dPartials.p, gs);
   // s = r - alpha v ; sh = M^{-1} s
   launchLU<T, kProS>(s, L, sc, (const T *)nullptr, K.y.p, K.s.p, K.r.p, K.v.p, K.sh.p);
   // t = A sh ; omega
   hipLaunchKernelGGL((kSpmvDotT<T, TV>), dim3(gs), dim3(kRedBlock), 0, s->stream, n, s->ord.nSlices, s->dSliceOff.p,
                      s->dSellCol.p, sell, K.sh.p, K.t.p, K.s.p, sc, s->dPartials.p);
-  hipLaunchKernelGGL(kFinishT, dim3(1), dim3(kRedBlock), 0, s->stream, sc, s->dPartials.p, gs);
+  hipLaunchKernelGGL(kFinishT, dim3(1), dim3(kFinBlock), 0, s->stream, sc, s->dPartials.p, gs);
   // x, r, rho, beta, convergence
   const int gu = gridFor(n, kRedBlock, kRedCap);
   hipLaunchKernelGGL((kUpdate<T, TX>), dim3(gu), dim3(kRedBlock), 0, s->stream, n, sc, x, K.ph.p, K.sh.p, K.r.p,
@@ -309,7 +309,7 @@ void iterateCg(phase_hip_solver *s, KrylovVecs<T> &K, DBuf<T> &fac, const TV *se
   const int gs = spmvGrid(s);
   hipLaunchKernelGGL((kSpmvDotCg<T, TV>), dim3(gs), dim3(kRedBlock), 0, s->stream, n, s->ord.nSlices,
                      s->dSliceOff.p, s->dSellCol.p, sell, K.p.p, K.v.p, K.r.p, sc, s->dPartials.p);
-  hipLaunchKernelGGL(kFinishCgQ, dim3(1), dim3(kRedBlock), 0, s->stream, sc, s->dPartials.p, gs);
+  hipLaunchKernelGGL(kFinishCgQ, dim3(1), dim3(kFinBlock), 0, s->stream, sc, s->dPartials.p, gs);
   launchLU<T, kProCgR>(s, L, sc, (const T *)K.p.p, K.y.p, K.r.p, (const T *)nullptr, (const T *)K.v.p, K.ph.p, x);
   cgBeta<T>(s, K, sc);
 }

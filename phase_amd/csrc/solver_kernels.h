@@ -163,11 +163,15 @@ __device__ void blockPartials(double (&v)[NV], double *partials, int slot) {
   }
 }
 
-// Part 2 (one block of kRedBlock threads): totals of nb partial blocks in
-// index order; thread 0 holds them in v.
-template <int NV>
+// One-block finishers run 1024 threads: a few thousand partials then take a
+// single round of loads.
+constexpr int kFinBlock = 1024;
+
+// Part 2 (one block of BS threads): totals of nb partial blocks in index
+// order; thread 0 holds them in v.
+template <int NV, int BS = kRedBlock>
 __device__ void finishPartials(const double *partials, int nb, double (&v)[NV]) {
-  __shared__ double sred[NV][kRedBlock / 64];
+  __shared__ double sred[NV][BS / 64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   // eight loads per vector in flight per thread: the sum is bound by a few
   // memory round trips, not by one per partial
@@ -175,13 +179,13 @@ __device__ void finishPartials(const double *partials, int nb, double (&v)[NV]) 
   double acc[NV];
 #pragma unroll
   for (int q = 0; q < NV; ++q) acc[q] = 0.;
-  for (int i0 = threadIdx.x; i0 < nb; i0 += U * kRedBlock) {
+  for (int i0 = threadIdx.x; i0 < nb; i0 += U * BS) {
     double t[NV][U];
 #pragma unroll
     for (int q = 0; q < NV; ++q)
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int i = i0 + u * kRedBlock;
+        const int i = i0 + u * BS;
         t[q][u] = i < nb ? partials[q * nb + i] : 0.;
       }
 #pragma unroll
@@ -199,7 +203,7 @@ __device__ void finishPartials(const double *partials, int nb, double (&v)[NV]) 
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
       double t = 0.;
-      for (int w = 0; w < kRedBlock / 64; ++w) t += sred[q][w];
+      for (int w = 0; w < BS / 64; ++w) t += sred[q][w];
       v[q] = t;
     }
   }
@@ -311,10 +315,10 @@ __global__ __launch_bounds__(kRedBlock) void kSpmvDotV(int n, int nSlices, const
   blockPartials<1>(d, partials, tile);
 }
 
-static __global__ __launch_bounds__(kRedBlock) void kFinishV(Scalars *sc, const double *partials, int nb) {
+static __global__ __launch_bounds__(kFinBlock) void kFinishV(Scalars *sc, const double *partials, int nb) {
   if (sc->done) return;
   double d[1];
-  finishPartials<1>(partials, nb, d);
+  finishPartials<1, kFinBlock>(partials, nb, d);
   if (threadIdx.x == 0) {
     sc->rv = d[0];
     if (d[0] == 0. || !isfinite(d[0])) sc->done = kDoneBreakdown;
@@ -344,10 +348,10 @@ __global__ __launch_bounds__(kRedBlock) void kSpmvDotT(int n, int nSlices, const
   blockPartials<2>(d, partials, tile);
 }
 
-static __global__ __launch_bounds__(kRedBlock) void kFinishT(Scalars *sc, const double *partials, int nb) {
+static __global__ __launch_bounds__(kFinBlock) void kFinishT(Scalars *sc, const double *partials, int nb) {
   if (sc->done) return;
   double d[2];
-  finishPartials<2>(partials, nb, d);
+  finishPartials<2, kFinBlock>(partials, nb, d);
   if (threadIdx.x == 0) {
     sc->ts = d[0];
     sc->tt = d[1];
@@ -492,10 +496,10 @@ __global__ __launch_bounds__(kRedBlock) void kSpmvDotCg(int n, int nSlices, cons
 
 // Stopping tests on rr = (r, r) (the residual of the current x), then
 // alpha = rho / (p, q); the sweeps that follow move x and r by alpha.
-static __global__ __launch_bounds__(kRedBlock) void kFinishCgQ(Scalars *sc, const double *partials, int nb) {
+static __global__ __launch_bounds__(kFinBlock) void kFinishCgQ(Scalars *sc, const double *partials, int nb) {
   if (sc->done) return;
   double d[2];
-  finishPartials<2>(partials, nb, d);
+  finishPartials<2, kFinBlock>(partials, nb, d);
   if (threadIdx.x != 0) return;
   const double pq = d[0], rr = d[1];
   sc->rho = sc->rhoNext;
