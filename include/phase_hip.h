@@ -435,6 +435,14 @@ const char *phase_hip_fstep_last_error(const phase_hip_fstep *s);
 int phase_hip_fstep_setup(phase_hip_fstep *s, const char *key, const char *value);
 int phase_hip_fstep_advance(phase_hip_fstep *s, double dt, int nSteps, double *history);
 int phase_hip_fstep_fields(phase_hip_fstep *s, double *uCell, double *pCell, double *uFace);
+/* FractionalStepGCIB::solve -- src/2D/Unstructured/Solvers/FractionalStepGCIB.cpp:14-56:
+ * the following steps assemble the fv:: terms on the fluid cells only, add
+ * velocityBcs(u) (objectVelocity: 2 per body) and bcs(p) (pBcType FIXED or
+ * NORMAL_GRADIENT with pRefValue) of the immersed boundary, use the source
+ * src(grad p / rho) and u += dt / rho grad p, and correct and check the
+ * divergence on the fluid cells only. */
+int phase_hip_fstep_immersed_boundary(phase_hip_fstep *s, const phase_hip_gcib *ib, const double *objectVelocity,
+                                      int pBcType, double pRefValue);
 
 /* ------------------------------------------------------------------ multiphase fractional step
  * FractionalStepMultiphase::solve(timeStep) -- src/2D/Unstructured/Solvers/

@@ -11,6 +11,14 @@ One time step (src/2D/Unstructured/Solvers/FractionalStep.cpp:36-117):
   u -= dt grad p (cells); u_f -= dt grad_f p (every face)
 With no FIXED pressure face the pressure matrix is singular and one cell is
 pinned to 0, as on the device.
+
+With an immersed boundary this is FractionalStepGCIB::solve
+(src/2D/Unstructured/Solvers/FractionalStepGCIB.cpp:14-56): the fv:: terms
+cover the fluid cells only (their other rows removed), the ghost-cell rows of
+oracle/ghostcell_oracle.py are added (velocityBcs to the momentum equation,
+bcs(p) to the pressure equation), the momentum source is src(grad p / rho),
+u += dt / rho grad p and u -= dt grad p touch the fluid cells only, and
+src::div and the divergence check cover the fluid cells.
 """
 import numpy as np
 import scipy.sparse as sp
@@ -27,8 +35,11 @@ def _csr(rp, ci, v):
 
 
 class FractionalStepOracle:
-    def __init__(self, mesh, rho, mu, u_bc, u_face0, p_bc=None, p_face0=None, pin=0):
+    def __init__(self, mesh, rho, mu, u_bc, u_face0, p_bc=None, p_face0=None, pin=0, immersed=None):
+        """immersed: None or (GhostCellOracle, body velocities, pressure bc type
+        on the bodies, its reference value)."""
         self.m, self.rho, self.mu, self.pin = mesh, rho, mu, pin
+        self.ib = immersed
         N, F = mesh.n_cells, mesh.n_faces
         self.ubc = np.asarray(u_bc, np.int32)
         self.pbc = (np.full(F, O.BC_NORMAL_GRADIENT, np.int32) if p_bc is None else np.asarray(p_bc, np.int32))
@@ -55,24 +66,51 @@ class FractionalStepOracle:
 
     def step(self, dt):
         m, N, F = self.m, self.m.n_cells, self.m.n_faces
+        ib = self.ib
+        if ib is not None:
+            g, vel, pbc_ib, pref = ib
+            other = (g.types != 0).astype(np.int32)
+            fluid = g.types == 0
         u0C, u0F = self.uC.copy(), self.uF.copy()
         E = O.ddt(m, dt, u0C.reshape(-1), ncomp=2)
         E += O.div(m, self.uF.reshape(-1), u0F.reshape(-1), theta=0.0, bc_type=self.ubc,
                    phi_face=self.uF.reshape(-1), phi0_cell=u0C.reshape(-1), phi0_face=u0F.reshape(-1), ncomp=2)
+        if ib is not None:
+            E.remove_cells(N, other)
+            E += g.velocity_bcs(vel)
         L = O.laplacian(m, 0, ncomp=2, gamma=self.mu / self.rho, theta=0.5, bc_type=self.ubc,
                         phi_face=self.uF.reshape(-1), phi0_cell=u0C.reshape(-1), phi0_face=u0F.reshape(-1))
-        L -= O.src_src(m, self.gpC.reshape(-1), ncomp=2)
+        if ib is None:
+            L -= O.src_src(m, self.gpC.reshape(-1), ncomp=2)
+        else:
+            L.remove_cells(N, other)
+            src = O.src_src(m, (self.gpC / self.rho).reshape(-1), ncomp=2)
+            src[np.r_[~fluid, ~fluid]] = 0.0
+            L -= src
         E.eq(L)
         self.uC = self._solve(E).reshape(2, N).T.copy()
-        self.uC = self.uC + dt * self.gpC
+        if ib is None:
+            self.uC = self.uC + dt * self.gpC
+        else:
+            self.uC[fluid] = self.uC[fluid] + (dt / self.rho) * self.gpC[fluid]
         self.uF = O.interpolate_faces(m, self.uC.reshape(-1), self.uF.reshape(-1), bc_type=self.ubc,
                                       ncomp=2).reshape(F, 2)
-        P = O.laplacian(m, 1, gamma=dt, bc_type=self.pbc, phi_face=self.pF)
-        P -= O.src_div(m, self.uF.reshape(-1))
+        if ib is None:
+            P = O.laplacian(m, 1, gamma=dt, bc_type=self.pbc, phi_face=self.pF)
+            P -= O.src_div(m, self.uF.reshape(-1))
+        else:
+            P = O.laplacian(m, 1, gamma=dt / self.rho, bc_type=self.pbc, phi_face=self.pF)
+            P.remove_cells(N, other)
+            P += g.bcs(pbc_ib, pref)
+            P -= O.src_div(m, self.uF.reshape(-1)) * fluid
         self.pC = self._solve(P, self.pin)
         self.pF = O.interpolate_faces(m, self.pC, self.pF, bc_type=self.pbc)
         gF, gC = O.grad(m, self.pC, self.pF)
         self.gpF, self.gpC = gF, gC
-        self.uC = self.uC + (-dt) * self.gpC
+        if ib is None:
+            self.uC = self.uC + (-dt) * self.gpC
+        else:
+            self.uC[fluid] = self.uC[fluid] + (-dt) * self.gpC[fluid]
         self.uF = self.uF + (-dt) * self.gpF
-        return np.abs(O.src_div(m, self.uF.reshape(-1))).max()
+        div = np.abs(O.src_div(m, self.uF.reshape(-1)))
+        return div.max() if ib is None else div[fluid].max()

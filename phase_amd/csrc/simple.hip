@@ -308,6 +308,13 @@ void pisoStep(phase_hip_simple *s, double dt, int nCorr, double *hist) {
 //   pEqn = laplacian(dt, p) == src::div(u_f);  p = solve(pEqn)  (pin cell fixed at 0)
 //   p_f = setBoundaryFaces(p);  grad p (faces, FACE_TO_CELL)
 //   u -= dt grad p (cells);  u_f -= dt grad_f p (every face)
+//
+// With an immersed boundary (phase_hip_fstep_immersed_boundary) the step is
+// FractionalStepGCIB::solve (src/2D/Unstructured/Solvers/FractionalStepGCIB.cpp:14-56):
+// the fv:: terms keep the fluid cells' rows only, velocityBcs(u) and bcs(p)
+// add the ghost and solid rows, the momentum source is src(grad p / rho),
+// u += dt / rho grad p and u -= dt grad p touch the fluid cells only, and
+// src::div and the divergence check cover the fluid cells.
 struct phase_hip_fstep {
   phase_hip_mesh *mesh = nullptr;
   int N = 0, F = 0, pin = 0;
@@ -316,6 +323,11 @@ struct phase_hip_fstep {
   phase_hip_solver *su = nullptr, *sp = nullptr;
   DBuf<double> uC, u0C, uB, uF, u0F, pC, pB, pF, gpF, gpC, gsrc, m, maxBuf;
   DBuf<int> uBc, pBc;
+  // immersed boundary: cell types (0 fluid), its momentum and pressure rows
+  bool immersed = false;
+  DBuf<int> ibTypes;
+  DBuf<double> gq;
+  phase_hip_equation *ibU = nullptr, *ibP = nullptr;
   std::string err;
 };
 
@@ -334,6 +346,28 @@ int guardedF(phase_hip_fstep *s, F &&f) {
     s->err = x.what();
     return PHASE_HIP_EINVAL;
   }
+}
+
+// u_c += sc g_c for the fluid cells (type 0); u, g interleaved
+__global__ void kFluidAxpy(int n, double sc, const int *__restrict__ type, const double *__restrict__ g, double *u) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x)
+    if (type[c] == 0) {
+      u[2 * c] += sc * g[2 * c];
+      u[2 * c + 1] += sc * g[2 * c + 1];
+    }
+}
+
+// v = 0 on the rows of the non-fluid cells (ncomp blocks of n rows)
+__global__ void kZeroNonFluid(int n, int ncomp, const int *__restrict__ type, double *v) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x)
+    if (type[c] != 0)
+      for (int q = 0; q < ncomp; ++q) v[(long)q * n + c] = 0.;
+}
+
+// out = a / d
+__global__ void kDivide(long n, double d, const double *__restrict__ a, double *__restrict__ out) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    out[i] = a[i] / d;
 }
 
 double maxAbsF(phase_hip_fstep *s, const double *a, int n) {
@@ -356,9 +390,21 @@ void timeStep(phase_hip_fstep *s, double dt, double *hist) {
   ck(phase_hip_eq_div(s->tmp, s->uF.p, s->u0F.p, 0.0, s->uBc.p, s->uF.p, s->u0C.p, s->u0F.p), "fv::div",
      s->tmp->err.c_str());
   ck(phase_hip_eq_add(s->uEq, s->tmp, 1), "operator+", s->uEq->err.c_str());
+  const bool ib = s->immersed;
+  if (ib) {  // + ib_.velocityBcs(u_), the fv:: rows on the fluid cells
+    ck(phase_hip_eq_remove_cells(s->uEq, s->ibTypes.p), "remove", s->uEq->err.c_str());
+    ck(phase_hip_eq_add(s->uEq, s->ibU, 1), "operator+", s->uEq->err.c_str());
+  }
   ck(phase_hip_eq_laplacian(s->tmp, 0, s->mu / s->rho, nullptr, nullptr, 0.5, s->uBc.p, s->uF.p, s->u0C.p, s->u0F.p),
      "fv::laplacian", s->tmp->err.c_str());
-  ck(phase_hip_field_src_src(m, 2, s->gpC.p, s->gsrc.p), "src::src", m->err.c_str());
+  if (ib) {  // - src::src(gradP_ / rho_) over the fluid cells
+    ck(phase_hip_eq_remove_cells(s->tmp, s->ibTypes.p), "remove", s->tmp->err.c_str());
+    hipLaunchKernelGGL(kDivide, dim3(g256(2L * N)), dim3(256), 0, st, 2L * N, s->rho, s->gpC.p, s->gq.p);
+    ck(phase_hip_field_src_src(m, 2, s->gq.p, s->gsrc.p), "src::src", m->err.c_str());
+    hipLaunchKernelGGL(kZeroNonFluid, dim3(g256(N)), dim3(256), 0, st, N, 2, s->ibTypes.p, s->gsrc.p);
+  } else {
+    ck(phase_hip_field_src_src(m, 2, s->gpC.p, s->gsrc.p), "src::src", m->err.c_str());
+  }
   ck(phase_hip_eq_sub_vector(s->tmp, s->gsrc.p), "operator-", s->tmp->err.c_str());
   ck(phase_hip_eq_add(s->uEq, s->tmp, -1), "operator==", s->uEq->err.c_str());
   hipLaunchKernelGGL(kToBlocks, dim3(g256(N)), dim3(256), 0, st, N, s->uC.p, s->uB.p);
@@ -369,12 +415,21 @@ void timeStep(phase_hip_fstep *s, double dt, double *hist) {
   if (rc != PHASE_HIP_ENOCONV) ck(rc, "uEqn solve", s->su->err.c_str());
   ck(phase_hip_solver_get_x_device(s->su, s->uB.p), "uEqn x", s->su->err.c_str());
   hipLaunchKernelGGL(kFromBlocks, dim3(g256(N)), dim3(256), 0, st, N, s->uB.p, s->uC.p);
-  hipLaunchKernelGGL(kAxpy, dim3(g256(2L * N)), dim3(256), 0, st, 2L * N, dt, s->gpC.p, s->uC.p);
+  if (ib)
+    hipLaunchKernelGGL(kFluidAxpy, dim3(g256(N)), dim3(256), 0, st, N, dt / s->rho, s->ibTypes.p, s->gpC.p, s->uC.p);
+  else
+    hipLaunchKernelGGL(kAxpy, dim3(g256(2L * N)), dim3(256), 0, st, 2L * N, dt, s->gpC.p, s->uC.p);
   ck(phase_hip_field_interpolate_faces(m, 2, s->uC.p, s->uBc.p, s->uF.p), "interpolateFaces", m->err.c_str());
-  // solvePEqn (FractionalStep.cpp:94-106)
-  ck(phase_hip_eq_laplacian(s->pEq, 1, dt, nullptr, nullptr, 1.0, s->pBc.p, s->pF.p, nullptr, nullptr),
+  // solvePEqn (FractionalStep.cpp:94-106; FractionalStepGCIB.cpp:44-56)
+  ck(phase_hip_eq_laplacian(s->pEq, 1, ib ? dt / s->rho : dt, nullptr, nullptr, 1.0, s->pBc.p, s->pF.p, nullptr,
+                            nullptr),
      "fv::laplacian", s->pEq->err.c_str());
+  if (ib) {  // + ib_.bcs(p_)
+    ck(phase_hip_eq_remove_cells(s->pEq, s->ibTypes.p), "remove", s->pEq->err.c_str());
+    ck(phase_hip_eq_add(s->pEq, s->ibP, 1), "operator+", s->pEq->err.c_str());
+  }
   ck(phase_hip_field_src_div(m, s->uF.p, s->m.p), "src::div", m->err.c_str());
+  if (ib) hipLaunchKernelGGL(kZeroNonFluid, dim3(g256(N)), dim3(256), 0, st, N, 1, s->ibTypes.p, s->m.p);
   ck(phase_hip_eq_sub_vector(s->pEq, s->m.p), "operator==", s->pEq->err.c_str());
   if (s->pin >= 0)  // all-Neumann pressure: one cell fixed at 0
     hipLaunchKernelGGL(kPinRow, dim3(1), dim3(64), 0, st, s->pin, s->pEq->stride, s->pEq->cols.p, s->pEq->vals.p,
@@ -387,12 +442,16 @@ void timeStep(phase_hip_fstep *s, double dt, double *hist) {
   ck(phase_hip_field_interpolate_faces(m, 1, s->pC.p, s->pBc.p, s->pF.p), "setBoundaryFaces", m->err.c_str());
   ck(phase_hip_field_grad(m, s->pC.p, s->pF.p, s->gpF.p, s->gpC.p), "grad p", m->err.c_str());
   // correctVelocity (FractionalStep.cpp:108-117)
-  hipLaunchKernelGGL(kAxpy, dim3(g256(2L * N)), dim3(256), 0, st, 2L * N, -dt, s->gpC.p, s->uC.p);
+  if (ib)
+    hipLaunchKernelGGL(kFluidAxpy, dim3(g256(N)), dim3(256), 0, st, N, -dt, s->ibTypes.p, s->gpC.p, s->uC.p);
+  else
+    hipLaunchKernelGGL(kAxpy, dim3(g256(2L * N)), dim3(256), 0, st, 2L * N, -dt, s->gpC.p, s->uC.p);
   hipLaunchKernelGGL(kAxpy, dim3(g256(2L * F)), dim3(256), 0, st, 2L * F, -dt, s->gpF.p, s->uF.p);
   PH_CHECK(hipGetLastError());
   if (hist) {
     // FractionalStep::maxDivergenceError (FractionalStep.cpp:119-135) on the corrected faces
     ck(phase_hip_field_src_div(m, s->uF.p, s->m.p), "src::div", m->err.c_str());
+    if (ib) hipLaunchKernelGGL(kZeroNonFluid, dim3(g256(N)), dim3(256), 0, st, N, 1, s->ibTypes.p, s->m.p);
     hist[0] = phase_hip_solver_n_iters(s->su);
     hist[1] = phase_hip_solver_n_iters(s->sp);
     hist[2] = maxAbsF(s, s->m.p, N);
@@ -405,6 +464,8 @@ extern "C" {
 
 int phase_hip_fstep_destroy(phase_hip_fstep *s) {
   if (!s) return PHASE_HIP_OK;
+  phase_hip_equation_destroy(s->ibU);
+  phase_hip_equation_destroy(s->ibP);
   phase_hip_equation_destroy(s->uEq);
   phase_hip_equation_destroy(s->tmp);
   phase_hip_equation_destroy(s->pEq);
@@ -477,6 +538,24 @@ int phase_hip_fstep_setup(phase_hip_fstep *s, const char *key, const char *value
     else if (k.rfind("p.", 0) == 0) v = s->sp;
     else fail(PHASE_HIP_EINVAL, "keys are u.<solver key> or p.<solver key>");
     ck(phase_hip_solver_setup(v, k.c_str() + 2, value), "setup", v->err.c_str());
+  });
+}
+
+int phase_hip_fstep_immersed_boundary(phase_hip_fstep *s, const phase_hip_gcib *ib, const double *objectVelocity,
+                                      int pBcType, double pRefValue) {
+  return guardedF(s, [&] {
+    if (!ib) fail(PHASE_HIP_EINVAL, "null immersed boundary");
+    phase_hip_mesh *m = s->mesh;
+    PH_CHECK(hipSetDevice(m->device));
+    if (!s->ibU) ck(phase_hip_equation_create(&s->ibU, m, 2), "velocityBcs", "");
+    if (!s->ibP) ck(phase_hip_equation_create(&s->ibP, m, 1), "bcs", "");
+    ck(phase_hip_eq_gcib_velocity_bcs(s->ibU, ib, objectVelocity), "velocityBcs", s->ibU->err.c_str());
+    ck(phase_hip_eq_gcib_bcs(s->ibP, ib, pBcType, pRefValue), "bcs", s->ibP->err.c_str());
+    s->ibTypes.alloc(s->N);
+    ck(phase_hip_gcib_cell_types(ib, s->ibTypes.p, nullptr, 1), "cell types", "");
+    s->gq.alloc(2 * (size_t)s->N);
+    PH_CHECK(hipStreamSynchronize(m->stream));
+    s->immersed = true;
   });
 }
 

@@ -118,6 +118,7 @@ _SIGS = {
     "phase_hip_fstep_setup": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_char_p]),
     "phase_hip_fstep_advance": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, _vp]),
     "phase_hip_fstep_fields": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    "phase_hip_fstep_immersed_boundary": (ctypes.c_int, [_vp, _vp, _f64p, ctypes.c_int, ctypes.c_double]),
     "phase_hip_gcib_create": (ctypes.c_int, [ctypes.POINTER(_vp), _vp, ctypes.c_int, _f64p]),
     "phase_hip_gcib_destroy": (ctypes.c_int, [_vp]),
     "phase_hip_gcib_last_error": (ctypes.c_char_p, [_vp]),
@@ -502,6 +503,14 @@ class HipFractionalStep:
 
     def setup(self, key, value):
         self._check(load_library().phase_hip_fstep_setup(self._h, key.encode(), str(value).encode()), "setup")
+
+    def immersed_boundary(self, ib, velocities, p_bc, p_ref=0.0):
+        """FractionalStepGCIB: the ghost-cell rows of `ib` (HipGhostCellIB) in
+        both equations from the next step on."""
+        self._ib = ib  # the driver keeps using its cell types and rows
+        v = _f64(np.asarray(velocities, float).reshape(-1))
+        self._check(load_library().phase_hip_fstep_immersed_boundary(self._h, ib._h, v, int(p_bc), float(p_ref)),
+                    "immersedBoundary")
 
     def advance(self, dt, n=1):
         """n time steps; returns an (n, 3) history: momentum and pressure
