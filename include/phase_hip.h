@@ -413,6 +413,13 @@ const char *phase_hip_gcib_last_error(const phase_hip_gcib *ib);
 int phase_hip_gcib_cell_types(const phase_hip_gcib *ib, int *types, int *owner, int onDevice);
 int phase_hip_eq_gcib_bcs(phase_hip_equation *e, const phase_hip_gcib *ib, int bcType, double refValue);
 int phase_hip_eq_gcib_velocity_bcs(phase_hip_equation *e, const phase_hip_gcib *ib, const double *objectVelocity);
+/* GhostCellImmersedBoundary::applyHydrodynamicForce(rho, mu, u, p, g) --
+ * GhostCellImmersedBoundary.cpp:94-146: per body, pressure (bpValue(p) +
+ * rho g.bp) and wall shear (mu (grad u(bp) . nw) . nw.tangentVec()) at the
+ * boundary points, sorted by angle and integrated edge by edge; uCell
+ * (interleaved) and pCell are device fields, force (2 per body) host. */
+int phase_hip_gcib_hydrodynamic_force(phase_hip_gcib *ib, double rho, double mu, const double *uCell,
+                                      const double *pCell, double gx, double gy, double *force);
 
 /* ------------------------------------------------------------------ fractional step
  * FractionalStep::solve(timeStep) -- src/2D/Unstructured/Solvers/FractionalStep.cpp:36-117,

@@ -31,6 +31,14 @@ source -(object velocity), every object here translating (no rotation).
 
 Ainv is numpy's inverse where the reference calls LAPACK getrf/getri: the
 device rows agree to rounding.
+
+Force on each body (applyHydrodynamicForce, GhostCellImmersedBoundary.cpp:94-146):
+per ghost cell, the boundary pressure bpValue(p) = (p_c + p(ip)) / 2 plus
+rho dot(bp, g), and the wall shear mu dot(dot(grad u(bp), nw), nw.tangentVec())
+with grad u from the bilinear fit over the four cells of the node nearest to
+bp (BcStencil::bpGrad); the points sorted by their angle about the centre;
+force = sum over consecutive pairs of -(p_a + p_b)/2 (x_b - x_a).normalVec()
++ (s_a + s_b)/2 (x_b - x_a).
 """
 import numpy as np
 
@@ -105,6 +113,46 @@ class GhostCellOracle:
             l = np.sqrt(e[0] * e[0] + e[1] * e[1])
             ng = (list(cells) + [c], list(-(v @ Ainv) / l) + [1. / l])
         return dict(bp=bp, ip=ip, nw=nw, cells=cells, fixed=fixed, ng=ng)
+
+    def _node_cells(self, pt):
+        dn = self.xy - pt
+        node = int(np.argmin(dn[:, 0] * dn[:, 0] + dn[:, 1] * dn[:, 1]))
+        return list(self.node_cells[node])
+
+    def hydrodynamic_force(self, rho, mu, u_cell, p_cell, g=(0.0, 0.0)):
+        """Force (fx, fy) per body; also returns the per-point (bp, pressure, shear)."""
+        u = np.asarray(u_cell, float).reshape(-1, 2)
+        p = np.asarray(p_cell, float)
+        out, points = [], []
+        for o, (cx, cy, r) in enumerate(self.circles):
+            pts = []
+            for c in np.where((self.types == GHOST) & (self.owner == o))[0]:
+                st = self.stencils[int(c)]
+                bp, ip, nw = st["bp"], st["ip"], st["nw"]
+                cells = st["cells"]
+                x = self.cc[cells]
+                Ainv = np.linalg.inv(np.c_[x[:, 0] * x[:, 1], x[:, 0], x[:, 1], np.ones(4)])
+                v = np.array([ip[0] * ip[1], ip[0], ip[1], 1.])
+                pb = (p[c] + (v @ Ainv) @ p[cells]) / 2.
+                pr = pb + rho * (bp[0] * g[0] + bp[1] * g[1])
+                bc = self._node_cells(bp)
+                y = self.cc[bc]
+                X = np.linalg.solve(np.c_[y[:, 0] * y[:, 1], y[:, 0], y[:, 1], np.ones(4)], u[bc])
+                G = np.array([[bp[1], 1., 0., 0.], [bp[0], 0., 1., 0.]]) @ X  # rows d/dx, d/dy
+                T = np.array([[G[0, 0], G[0, 1]], [G[1, 0], G[1, 1]]])  # xx, xy / yx, yy
+                tn = T @ nw
+                sh = mu * (tn[0] * -nw[1] + tn[1] * nw[0])
+                pts.append((np.arctan2(bp[1] - cy, bp[0] - cx), bp, pr, sh))
+            pts.sort(key=lambda t: t[0])
+            f = np.zeros(2)
+            for i in range(len(pts)):
+                _, pa, pra, sha = pts[i]
+                _, pb2, prb, shb = pts[(i + 1) % len(pts)]
+                d = pb2 - pa
+                f += -(pra + prb) / 2. * np.array([d[1], -d[0]]) + (sha + shb) / 2. * d
+            out.append(f)
+            points.append(pts)
+        return np.array(out), points
 
     def bcs(self, bc_type, ref_value):
         """GhostCellImmersedBoundary::bcs: a fresh scalar equation holding the

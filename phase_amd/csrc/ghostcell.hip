@@ -28,10 +28,24 @@
 
 using namespace phase;
 
+// per ghost cell, for the force: the cell, its body, bp, ip, nw, the image
+// point's interpolation cells and Ainv, and the boundary point's fit cells
+// with the inverse of their bilinear matrix
+struct GhostPoint {
+  int cell, body;
+  double bp[2], ip[2], nw[2];
+  int ipCells[4], bpCells[4];
+  double ipInv[16], bpInv[16];
+};
+
 struct phase_hip_gcib {
   phase_hip_mesh *mesh = nullptr;
   int N = 0, nObj = 0;
   std::vector<int> types, owner;
+  std::vector<double> circles;
+  std::vector<GhostPoint> points;
+  DBuf<int> dGather;     // 9 cells per ghost point: cell, ipCells, bpCells
+  DBuf<double> dValues;  // p, u.x, u.y at those cells
   DBuf<int> dTypes, dStencil, dFixCols, dNgCols;  // per cell: stencil index or -1; 5 columns per stencil
   DBuf<double> dFixVals, dNgVals;
   std::string err;
@@ -76,6 +90,17 @@ bool inverse4(const double *a, double *inv) {
   for (int i = 0; i < 4; ++i)
     for (int j = 0; j < 4; ++j) inv[4 * i + j] = m[i][4 + j];
   return true;
+}
+
+// p, u.x, u.y at the gathered cells
+__global__ void kGatherPU(int m, const int *__restrict__ idx, const double *__restrict__ p,
+                          const double *__restrict__ u, double *__restrict__ out) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
+    const int c = idx[i];
+    out[3 * i] = p[c];
+    out[3 * i + 1] = u[2 * c];
+    out[3 * i + 2] = u[2 * c + 1];
+  }
 }
 
 // one fresh row per (component, cell): stencil, unit diagonal or empty
@@ -248,6 +273,33 @@ int phase_hip_gcib_create(phase_hip_gcib **out, phase_hip_mesh *m, int nCircles,
         for (int k = 0; k < 4; ++k) s += v[k] * Ai[4 * k + j];
         va[j] = s;
       }
+      {  // the force's data (applyHydrodynamicForce)
+        GhostPoint g{};
+        g.cell = c;
+        g.body = o;
+        g.bp[0] = bpx;
+        g.bp[1] = bpy;
+        g.ip[0] = ipx;
+        g.ip[1] = ipy;
+        g.nw[0] = nwx;
+        g.nw[1] = nwy;
+        std::copy(cells.begin(), cells.end(), g.ipCells);
+        std::copy(Ai, Ai + 16, g.ipInv);
+        const std::vector<int> &bcells = nodeCells[nearestNode(bpx, bpy)];
+        if (bcells.size() != 4)
+          fail(PHASE_HIP_EINVAL, "boundary-point fit of cell " + std::to_string(c) + " needs a node with four cells");
+        double B[16];
+        for (int i = 0; i < 4; ++i) {
+          const double x = cc[2 * bcells[i]], y = cc[2 * bcells[i] + 1];
+          B[4 * i] = x * y;
+          B[4 * i + 1] = x;
+          B[4 * i + 2] = y;
+          B[4 * i + 3] = 1.;
+          g.bpCells[i] = bcells[i];
+        }
+        if (!inverse4(B, g.bpInv)) fail(PHASE_HIP_EINVAL, "singular boundary-point fit at cell " + std::to_string(c));
+        ib->points.push_back(g);
+      }
       stencil[c] = (int)(fixCols.size() / 5);
       for (int j = 0; j < 4; ++j) {
         fixCols.push_back(cells[j]);
@@ -275,6 +327,15 @@ int phase_hip_gcib_create(phase_hip_gcib **out, phase_hip_mesh *m, int nCircles,
         ngVals.push_back(1. / l);
       }
     }
+    ib->circles.assign(circles, circles + 3 * (size_t)nCircles);
+    std::vector<int> gather;
+    for (const GhostPoint &g : ib->points) {
+      gather.push_back(g.cell);
+      gather.insert(gather.end(), g.ipCells, g.ipCells + 4);
+      gather.insert(gather.end(), g.bpCells, g.bpCells + 4);
+    }
+    ib->dGather.upload(gather, st);
+    ib->dValues.alloc(3 * gather.size());
     ib->dTypes.upload(ib->types, st);
     ib->dStencil.upload(stencil, st);
     ib->dFixCols.upload(fixCols, st);
@@ -290,6 +351,81 @@ int phase_hip_gcib_create(phase_hip_gcib **out, phase_hip_mesh *m, int nCircles,
     return PHASE_HIP_EINVAL;
   }
   *out = ib;
+  return PHASE_HIP_OK;
+}
+
+int phase_hip_gcib_hydrodynamic_force(phase_hip_gcib *ib, double rho, double mu, const double *uCell,
+                                      const double *pCell, double gx, double gy, double *force) {
+  if (!ib || !uCell || !pCell || !force) return PHASE_HIP_EINVAL;
+  try {
+    phase_hip_mesh *m = ib->mesh;
+    PH_CHECK(hipSetDevice(m->device));
+    const int ng = (int)ib->dGather.n;
+    std::vector<double> v(3 * (size_t)ng);
+    if (ng) {
+      hipLaunchKernelGGL(kGatherPU, dim3(g256g(ng)), dim3(256), 0, m->stream, ng, ib->dGather.p, pCell, uCell,
+                         ib->dValues.p);
+      PH_CHECK(hipGetLastError());
+      PH_CHECK(hipMemcpyAsync(v.data(), ib->dValues.p, v.size() * sizeof(double), hipMemcpyDeviceToHost, m->stream));
+    }
+    PH_CHECK(hipStreamSynchronize(m->stream));
+    struct Stress {
+      double angle, x, y, pr, sh;
+    };
+    for (int o = 0; o < ib->nObj; ++o) {
+      const double cx = ib->circles[3 * o], cy = ib->circles[3 * o + 1];
+      std::vector<Stress> pts;
+      for (size_t k = 0; k < ib->points.size(); ++k) {
+        const GhostPoint &g = ib->points[k];
+        if (g.body != o) continue;
+        const double *val = &v[27 * k];  // 9 cells x (p, ux, uy)
+        // bpValue(p) = (p_c + ipValue(p)) / 2, ipValue = (ip.x ip.y, ip.x, ip.y, 1) Ainv p_cells
+        const double w[4] = {g.ip[0] * g.ip[1], g.ip[0], g.ip[1], 1.};
+        double ipv = 0.;
+        for (int j = 0; j < 4; ++j) {
+          double cj = 0.;
+          for (int i = 0; i < 4; ++i) cj += w[i] * g.ipInv[4 * i + j];
+          ipv += cj * val[3 * (1 + j)];
+        }
+        const double pr = (val[0] + ipv) / 2. + rho * (g.bp[0] * gx + g.bp[1] * gy);
+        // bpGrad(u): X = A^-1 u(bpCells); rows d/dx = (bp.y, 1, 0, 0) X, d/dy = (bp.x, 0, 1, 0) X
+        double X[4][2];
+        for (int i = 0; i < 4; ++i)
+          for (int q = 0; q < 2; ++q) {
+            double t = 0.;
+            for (int j = 0; j < 4; ++j) t += g.bpInv[4 * i + j] * val[3 * (5 + j) + 1 + q];
+            X[i][q] = t;
+          }
+        double G[2][2];
+        for (int q = 0; q < 2; ++q) {
+          G[0][q] = g.bp[1] * X[0][q] + X[1][q];
+          G[1][q] = g.bp[0] * X[0][q] + X[2][q];
+        }
+        // Tensor2D(xx, xy, yx, yy) = (G00, G01, G10, G11); dot(T, nw); dot(., nw.tangentVec())
+        const double tnx = G[0][0] * g.nw[0] + G[0][1] * g.nw[1];
+        const double tny = G[1][0] * g.nw[0] + G[1][1] * g.nw[1];
+        const double sh = mu * (tnx * -g.nw[1] + tny * g.nw[0]);
+        pts.push_back({std::atan2(g.bp[1] - cy, g.bp[0] - cx), g.bp[0], g.bp[1], pr, sh});
+      }
+      std::sort(pts.begin(), pts.end(), [](const Stress &a, const Stress &b) { return a.angle < b.angle; });
+      double fx = 0., fy = 0.;
+      for (size_t i = 0; i < pts.size(); ++i) {
+        const Stress &a = pts[i], &b = pts[(i + 1) % pts.size()];
+        const double dx = b.x - a.x, dy = b.y - a.y;
+        const double pm = -(a.pr + b.pr) / 2., sm = (a.sh + b.sh) / 2.;
+        fx += pm * dy + sm * dx;  // normalVec() = (dy, -dx)
+        fy += pm * -dx + sm * dy;
+      }
+      force[2 * o] = fx;
+      force[2 * o + 1] = fy;
+    }
+  } catch (const HipError &x) {
+    ib->err = x.msg;
+    return x.code;
+  } catch (const std::exception &x) {
+    ib->err = x.what();
+    return PHASE_HIP_EINVAL;
+  }
   return PHASE_HIP_OK;
 }
 

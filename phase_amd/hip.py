@@ -123,6 +123,8 @@ _SIGS = {
     "phase_hip_gcib_destroy": (ctypes.c_int, [_vp]),
     "phase_hip_gcib_last_error": (ctypes.c_char_p, [_vp]),
     "phase_hip_gcib_cell_types": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int]),
+    "phase_hip_gcib_hydrodynamic_force": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, _vp, _vp,
+                                                          ctypes.c_double, ctypes.c_double, _vp]),
     "phase_hip_eq_gcib_bcs": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_double]),
     "phase_hip_eq_gcib_velocity_bcs": (ctypes.c_int, [_vp, _vp, _f64p]),
     "phase_hip_multiphase_create": (ctypes.c_int, [ctypes.POINTER(_vp), _vp] + [ctypes.c_double] * 6 +
@@ -545,6 +547,7 @@ class HipGhostCellIB:
     def __init__(self, mesh, circles):
         h = _vp()
         c = _f64(np.asarray(circles, float).reshape(-1))
+        self._n_bodies = len(c) // 3
         rc = load_library().phase_hip_gcib_create(ctypes.byref(h), mesh._h, len(c) // 3, c)
         if rc != 0:
             raise PhaseHipError(rc, "ghostCellCreate", "creation failed")
@@ -558,6 +561,16 @@ class HipGhostCellIB:
         if rc != 0:
             raise PhaseHipError(rc, "cellTypes", "")
         return t, o
+
+    def hydrodynamic_force(self, rho, mu, u_cell, p_cell, g=(0.0, 0.0)):
+        """applyHydrodynamicForce: (n_bodies, 2) forces from device fields."""
+        n = load_library()
+        f = np.zeros((self._n_bodies, 2))
+        rc = n.phase_hip_gcib_hydrodynamic_force(self._h, float(rho), float(mu), u_cell.ptr, p_cell.ptr,
+                                                 float(g[0]), float(g[1]), _p(f))
+        if rc != 0:
+            raise PhaseHipError(rc, "hydrodynamicForce", n.phase_hip_gcib_last_error(self._h).decode())
+        return f
 
     def cell_types_device(self, out):
         rc = load_library().phase_hip_gcib_cell_types(self._h, out.ptr, None, 1)

@@ -141,3 +141,19 @@ def test_fractional_step_past_a_cylinder(oracle, hip):
         assert np.abs(p - ref.pC).max() <= 1e-7 * sp_
         assert np.abs(u[solid]).max() <= 1e-9 * su
         assert hist[0, 2] <= 1e-9 * np.abs(uf).max()
+
+
+def test_hydrodynamic_force_matches_oracle(oracle, hip, case):
+    """applyHydrodynamicForce on device fields against the restatement (the
+    4x4 inverses differ in rounding: 1e-10), with gravity."""
+    om, hm, og, dg = case
+    N = om.n_cells
+    _, cc = om.cells()
+    rng = np.random.default_rng(17)
+    u = np.stack([np.sin(3 * cc[:, 0]) * cc[:, 1], np.cos(2 * cc[:, 1]) + cc[:, 0] ** 2], -1) \
+        + 0.01 * rng.standard_normal((N, 2))
+    p = 1.0 + cc[:, 0] - 2.0 * cc[:, 1] ** 2
+    ref, _ = og.hydrodynamic_force(1.3, 0.02, u, p, g=(0.0, -9.81))
+    got = dg.hydrodynamic_force(1.3, 0.02, hip.DeviceArray(u.reshape(-1)), hip.DeviceArray(p), g=(0.0, -9.81))
+    assert got.shape == ref.shape
+    assert np.abs(got - ref).max() <= 1e-10 * max(1.0, np.abs(ref).max())

@@ -90,3 +90,37 @@ def test_velocity_rows_of_a_translating_body(oracle, gc):
         y = A @ field + rhs
         assert np.abs(y[rows]).max() <= 1e-12 and np.abs(y[N + rows]).max() <= 1e-12
     assert set(np.unique(g.owner[g.types > 0])) == {0, 1}
+
+
+def test_hydrodynamic_force_closed_forms(oracle, gc):
+    """A uniform pressure gives no force (the boundary polygon closes); a
+    linear pressure a + b.x gives -b times the polygon's area (divergence
+    theorem; the image-point interpolation and the trapezoid rule are exact
+    for it); a bilinear velocity field's wall shear is mu (grad u . nw) . t
+    with its exact gradient."""
+    n = 64
+    xy, cptr, cind = G.rectilinear(n, n)
+    m = oracle.Mesh(xy, cptr, cind)
+    N = m.n_cells
+    _, cc = m.cells()
+    g = gc(m, xy, cptr, cind, [(0.47, 0.52, 0.2)])
+    zero_u = np.zeros((N, 2))
+    f, _ = g.hydrodynamic_force(1.0, 0.1, zero_u, np.full(N, 3.0))
+    assert np.abs(f).max() <= 1e-12
+    b = np.array([0.7, -1.3])
+    f, pts = g.hydrodynamic_force(1.0, 0.1, zero_u, 0.4 + cc @ b)
+    poly = np.array([t[1] for t in pts[0]])
+    area = 0.5 * np.sum(poly[:, 0] * np.roll(poly[:, 1], -1) - np.roll(poly[:, 0], -1) * poly[:, 1])
+    assert area > 0
+    assert np.abs(f[0] + b * area).max() <= 1e-12
+    # bilinear u: exact gradient at bp
+    ub = lambda x: np.stack([0.2 + 0.5 * x[..., 0] - 0.3 * x[..., 1] + 0.7 * x[..., 0] * x[..., 1],
+                             -0.1 + 0.4 * x[..., 0] + 0.9 * x[..., 1] - 0.6 * x[..., 0] * x[..., 1]], -1)
+    _, pts = g.hydrodynamic_force(1.0, 0.1, ub(cc), np.zeros(N))
+    for _, bp, _, sh in pts[0]:
+        x, y = bp
+        T = np.array([[0.5 + 0.7 * y, 0.4 - 0.6 * y], [-0.3 + 0.7 * x, 0.9 - 0.6 * x]])  # xx xy / yx yy
+        st = [s for s in g.stencils.values() if np.allclose(s["bp"], bp)][0]
+        nw = st["nw"]
+        tn = T @ nw
+        assert abs(sh - 0.1 * (tn[0] * -nw[1] + tn[1] * nw[0])) <= 1e-12
