@@ -441,6 +441,55 @@ def ib_case(args):
     return out
 
 
+def ib_flow_case(args):
+    """Whole FractionalStepGCIB time steps (C5 geometry): 30 x 20 channel of
+    quads stored unstructured, fixed cylinder of diameter 1 at (10, 10) with
+    ghost-cell velocity rows and NORMAL_GRADIENT pressure rows, inflow u = (1, 0)
+    at x = 0, walls at y = 0 and 20, outlet at x = 30 (u zero gradient, p = 0);
+    rho = 1, Re = 200 (mu = 0.005), dt = 2e-3; BiCGStab to 1e-8 for both
+    equations, each warm-started from the previous step.  Reports ms per
+    step and the body force (applyHydrodynamicForce) after the last step."""
+    from phase_amd import hip as P
+    from phase_amd import meshgen as G
+
+    ny = int(round(np.sqrt(args.ib_flow_cells / 1.5)))
+    nx = int(round(1.5 * ny))
+    xy, cptr, cind = G.rectilinear(nx, ny, width=30.0, height=20.0)
+    mesh = P.HipMesh(xy, cptr, cind)
+    f = mesh.faces()
+    a, bn = xy[f["lnode"]], xy[f["rnode"]]
+    bnd = f["rcell"] < 0
+    left = bnd & (a[:, 0] == 0.0) & (bn[:, 0] == 0.0)
+    right = bnd & (a[:, 0] == 30.0) & (bn[:, 0] == 30.0)
+    ubc = np.where(right, P.BC_NORMAL_GRADIENT, P.BC_FIXED).astype(np.int32)
+    ub = np.zeros((mesh.n_faces, 2))
+    ub[left, 0] = 1.0
+    pbc = np.where(right, P.BC_FIXED, P.BC_NORMAL_GRADIENT).astype(np.int32)
+    t0 = time.perf_counter()
+    ib = P.HipGhostCellIB(mesh, [(10.0, 10.0, 0.5)])
+    d = P.HipFractionalStep(mesh, 1.0, 0.005, ubc, ub, p_bc=pbc, pin_cell=-1)
+    d.immersed_boundary(ib, [(0.0, 0.0)], P.BC_NORMAL_GRADIENT, 0.0)
+    for k in ("u", "p"):
+        d.setup(k + ".tolerance", args.tol)
+        d.setup(k + ".maxIters", 100000)
+        d.setup(k + ".mode", args.ib_flow_mode)
+    setup_s = time.perf_counter() - t0
+    d.advance(2e-3, 2)
+    t0 = time.perf_counter()
+    hist = d.advance(2e-3, args.ib_flow_steps)
+    ms = 1e3 * (time.perf_counter() - t0) / args.ib_flow_steps
+    u, p, _ = d.fields()
+    force = ib.hydrodynamic_force(1.0, 0.005, P.DeviceArray(u.reshape(-1)), P.DeviceArray(p))[0]
+    out = dict(workload="cylinder-ghost-cell-flow-step-30x20-quads", cells=mesh.n_cells, ms_per_step=round(ms, 2),
+               steps=args.ib_flow_steps, setup_s=round(setup_s, 2), mode=args.ib_flow_mode,
+               iters_u_p=[float(x) for x in hist.mean(0)[:2]], max_div=float("%.3g" % hist[:, 2].max()),
+               force=[float("%.5g" % x) for x in force])
+    log("ib flow: %.2f ms/step (setup %.1f s), iterations u/p %s, max div %.2e, force %s" %
+        (ms, setup_s, hist.mean(0)[:2], hist[:, 2].max(), force))
+    d.close()
+    return out
+
+
 def simple_cavity(args):
     """The C1 case on the device SIMPLE driver: lid-driven cavity on the
     128x128 unit square, Re = 100 (rho 1, mu 0.01), alpha_u 0.7, alpha_p 0.3,
@@ -512,6 +561,10 @@ def main():
     ap.add_argument("--multiphase-cells", type=int, default=2_000_000,
                     help="C3 whole multiphase time steps with surface tension (0 skips)")
     ap.add_argument("--multiphase-steps", type=int, default=5)
+    ap.add_argument("--ib-flow-cells", type=int, default=1_000_000,
+                    help="C5 FractionalStepGCIB time steps at this many cells (0 skips)")
+    ap.add_argument("--ib-flow-steps", type=int, default=3)
+    ap.add_argument("--ib-flow-mode", default="fp64")
     ap.add_argument("--ib-cells", type=int, default=1_000_000,
                     help="C5 ghost-cell pressure system at this many cells (0 skips)")
     ap.add_argument("--multiphase-mode", default="mixed", help="solver mode of the multiphase driver's solves")
@@ -648,6 +701,8 @@ def main():
         out["multiphase_c3"] = multiphase_case(args)
     if args.ib_cells > 0:
         out["ib_c5"] = ib_case(args)
+    if args.ib_flow_cells > 0:
+        out["ib_flow_c5"] = ib_flow_case(args)
     if args.c4_cells > 0:
         out["learned_c4"] = learned_case(args)
     if args.spmv_cells > 0:
