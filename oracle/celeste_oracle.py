@@ -27,6 +27,10 @@ The least-squares pseudo-inverse is numpy's (SVD), where the reference calls
 LAPACK dgels (QR, Math/Matrix.cpp:231-237): the same matrix, so the device
 results agree to rounding, not bit for bit.  The kernel's cells are summed in
 ascending cell id (the reference takes the order of its R-tree query).
+Bit-level parity with the reference itself: parity unpinned (LAPACK and the
+Boost R-tree are absent); what pins this restatement is in
+tests/test_oracle_celeste.py (exactness for quadratic fields, kernel
+normalisation, the curvature of a disc, contact-angle normals).
 """
 import numpy as np
 

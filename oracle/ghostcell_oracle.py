@@ -30,7 +30,11 @@ Velocity rows (velocityBcs, :73-90): FIXED stencils on both components with
 source -(object velocity), every object here translating (no rotation).
 
 Ainv is numpy's inverse where the reference calls LAPACK getrf/getri: the
-device rows agree to rounding.
+device rows agree to rounding.  Bit-level parity with the reference itself:
+parity unpinned (LAPACK and the Boost R-tree of the nearest-node query are
+absent); tests/test_oracle_ghostcell.py pins the rows by exactness on linear
+fields, the convergence of a Poisson problem around a hole and the closed
+forms of the body force.
 
 Force on each body (applyHydrodynamicForce, GhostCellImmersedBoundary.cpp:94-146):
 per ghost cell, the boundary pressure bpValue(p) = (p_c + p(ip)) / 2 plus

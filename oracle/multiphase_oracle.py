@@ -25,7 +25,10 @@ is given, else 0:
   correctVelocity (:165-173): u -= dt / rho gradP (cells and faces)
 
 Elementwise arithmetic is written in the device kernels' order, so only the
-linear solves (direct here, BiCGStab there) differ.
+linear solves (direct here, BiCGStab there) differ.  Each operator used here
+is pinned on its own (tests/test_oracle_*.py); the composition is pinned by
+the invariants of tests/test_oracle_multiphase.py (divergence-free faces,
+conserved volume, a fluid at rest, the Laplace pressure jump of a drop).
 """
 import numpy as np
 import scipy.sparse as sp
