@@ -577,7 +577,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported (the path would shard by "
-              "cell partition, see DESIGN.md)" % args.gpus, file=sys.stderr)
+              "cell partition with one halo exchange per SpMV)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     import phase_amd as P

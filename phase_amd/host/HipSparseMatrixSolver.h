@@ -7,7 +7,8 @@
 // dependency: setup() takes key/value pairs where the reference takes the
 // LinearAlgebra.<eqn> property tree.  A maintainer dropping the adapter into
 // the reference derives HipSparseMatrixSolver from the reference's own class
-// instead -- the method bodies are unchanged.
+// instead and forwards setup(ptree) keys to setup(key, value) -- the method
+// bodies are unchanged.
 #pragma once
 
 #include <map>
