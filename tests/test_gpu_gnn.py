@@ -107,7 +107,7 @@ def test_training_and_learned_mode(hip, system):
     last = s.learned_train(200, lr=0.0, seed=7)
     assert 0.0 <= last <= 1.0 and np.isfinite(first)
     assert last < first
-    # learned mode reaches the same FP64 goal; the FP64 true residual is the judge
+    # learned mode reaches the same FP64 goal; convergence is decided on the FP64 true residual
     A = H.compact_sorted(rp, ci, v)
     for mode in ("mixed", "learned"):
         s.setup("mode", mode)
