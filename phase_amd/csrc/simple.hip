@@ -146,11 +146,25 @@ __global__ void kAxpy(long n, double sc, const double *__restrict__ g, double *u
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) u[i] += sc * g[i];
 }
 
-// out[0] = max |a_i| (one block, fixed order)
+// max |a_i| in two passes: block maxima into out[1 + b] (fixed order within a
+// block), then one block takes the maximum of those into out[0]
+constexpr int kMaxBlocks = 1024;
 __global__ void kMaxAbs(int n, const double *__restrict__ a, double *out) {
   __shared__ double s[256];
   double m = 0.;
-  for (int i = threadIdx.x; i < n; i += 256) m = fmax(m, fabs(a[i]));
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) m = fmax(m, fabs(a[i]));
+  s[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[1 + blockIdx.x] = s[0];
+}
+__global__ void kMaxAbsFinish(int nb, double *out) {
+  __shared__ double s[256];
+  double m = 0.;
+  for (int i = threadIdx.x; i < nb; i += 256) m = fmax(m, out[1 + i]);
   s[threadIdx.x] = m;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
@@ -159,6 +173,7 @@ __global__ void kMaxAbs(int n, const double *__restrict__ a, double *out) {
   }
   if (threadIdx.x == 0) out[0] = s[0];
 }
+inline int maxAbsBlocks(int n) { return std::max(1, std::min(kMaxBlocks, (n + 255) / 256)); }
 
 void ck(int rc, const char *what, const char *detail) {
   if (rc != PHASE_HIP_OK) fail(rc, std::string(what) + ": " + (detail ? detail : ""));
@@ -180,7 +195,8 @@ int guardedS(phase_hip_simple *s, F &&f) {
 }
 
 double maxAbs(phase_hip_simple *s, const double *a, int n) {
-  hipLaunchKernelGGL(kMaxAbs, dim3(1), dim3(256), 0, s->mesh->stream, n, a, s->maxBuf.p);
+  hipLaunchKernelGGL(kMaxAbs, dim3(maxAbsBlocks(n)), dim3(256), 0, s->mesh->stream, n, a, s->maxBuf.p);
+  hipLaunchKernelGGL(kMaxAbsFinish, dim3(1), dim3(256), 0, s->mesh->stream, maxAbsBlocks(n), s->maxBuf.p);
   double v = 0.;
   PH_CHECK(hipMemcpyAsync(&v, s->maxBuf.p, sizeof(double), hipMemcpyDeviceToHost, s->mesh->stream));
   PH_CHECK(hipStreamSynchronize(s->mesh->stream));
@@ -371,7 +387,8 @@ __global__ void kDivide(long n, double d, const double *__restrict__ a, double *
 }
 
 double maxAbsF(phase_hip_fstep *s, const double *a, int n) {
-  hipLaunchKernelGGL(kMaxAbs, dim3(1), dim3(256), 0, s->mesh->stream, n, a, s->maxBuf.p);
+  hipLaunchKernelGGL(kMaxAbs, dim3(maxAbsBlocks(n)), dim3(256), 0, s->mesh->stream, n, a, s->maxBuf.p);
+  hipLaunchKernelGGL(kMaxAbsFinish, dim3(1), dim3(256), 0, s->mesh->stream, maxAbsBlocks(n), s->maxBuf.p);
   double v = 0.;
   PH_CHECK(hipMemcpyAsync(&v, s->maxBuf.p, sizeof(double), hipMemcpyDeviceToHost, s->mesh->stream));
   PH_CHECK(hipStreamSynchronize(s->mesh->stream));
@@ -507,7 +524,7 @@ int phase_hip_fstep_create(phase_hip_fstep **out, phase_hip_mesh *m, double rho,
     for (DBuf<double> *b : {&s->pC, &s->m}) b->alloc(N);
     for (DBuf<double> *b : {&s->uF, &s->u0F, &s->gpF}) b->alloc(2 * (size_t)F);
     s->pF.alloc(F);
-    s->maxBuf.alloc(1);
+    s->maxBuf.alloc(1 + kMaxBlocks);
     for (DBuf<double> *b : {&s->uC, &s->pC, &s->gpC, &s->gpF}) b->zero(st);
     std::vector<int> ubc(uBcType, uBcType + F), pbc(F, PHASE_HIP_BC_NORMAL_GRADIENT);
     if (pBcType) pbc.assign(pBcType, pBcType + F);
@@ -641,7 +658,7 @@ int phase_hip_simple_create(phase_hip_simple **out, phase_hip_mesh *m, double rh
     s->df.alloc(F);
     s->rdf.alloc(F);
     s->zeroF.alloc(F);
-    s->maxBuf.alloc(1);
+    s->maxBuf.alloc(1 + kMaxBlocks);
     for (DBuf<double> *b : {&s->uC, &s->pC, &s->pF, &s->zeroF, &s->pp}) b->zero(st);
     std::vector<int> ubc(uBcType, uBcType + F), pbc(F, PHASE_HIP_BC_NORMAL_GRADIENT);
     s->uBc.upload(ubc, st);
@@ -815,7 +832,8 @@ int guardedMP(phase_hip_multiphase *s, Fn &&f) {
 }
 
 double mpMaxAbs(phase_hip_multiphase *s, const double *a, int n) {
-  hipLaunchKernelGGL(kMaxAbs, dim3(1), dim3(256), 0, s->mesh->stream, n, a, s->maxBuf.p);
+  hipLaunchKernelGGL(kMaxAbs, dim3(maxAbsBlocks(n)), dim3(256), 0, s->mesh->stream, n, a, s->maxBuf.p);
+  hipLaunchKernelGGL(kMaxAbsFinish, dim3(1), dim3(256), 0, s->mesh->stream, maxAbsBlocks(n), s->maxBuf.p);
   double v = 0.;
   PH_CHECK(hipMemcpyAsync(&v, s->maxBuf.p, sizeof(double), hipMemcpyDeviceToHost, s->mesh->stream));
   PH_CHECK(hipStreamSynchronize(s->mesh->stream));
@@ -998,7 +1016,7 @@ int phase_hip_multiphase_create(phase_hip_multiphase **out, phase_hip_mesh *m, d
       b->alloc(2 * (size_t)N);
     for (DBuf<double> *b : {&s->ggF, &s->grF, &s->sgF, &s->uF, &s->u0F, &s->u1F, &s->gpF, &s->fstF})
       b->alloc(2 * (size_t)F);
-    s->maxBuf.alloc(1);
+    s->maxBuf.alloc(1 + kMaxBlocks);
     for (DBuf<double> *b : {&s->uC, &s->pC, &s->gpC, &s->gpF, &s->gF, &s->rhoF, &s->muF, &s->fstC, &s->fstF})
       b->zero(st);
     std::vector<int> ubc(uBcType, uBcType + F), pbc(F, PHASE_HIP_BC_NORMAL_GRADIENT),
