@@ -368,7 +368,8 @@ void fp64Loop(phase_hip_solver *s, double b2, int total) {
     // r = b - A x (true residual), r0 = r
     hipLaunchKernelGGL((kResidual<double>), dim3(spmvGrid(s)), dim3(kRedBlock), 0, s->stream, n,
                        s->ord.nSlices, s->dSliceOff.p, s->dSellCol.p, s->dSell64.p, s->dX.p, s->dB.p, K.r.p,
-                       K.r0.p, sc, s->dPartials.p, s->dCounter.p);
+                       K.r0.p, s->dPartials.p);
+    hipLaunchKernelGGL(kFinishResidual, dim3(1), dim3(kFinBlock), 0, s->stream, sc, s->dPartials.p, spmvGrid(s));
     hipLaunchKernelGGL((kBicgInit<double>), dim3(gridFor(n, 256, kElemCap)), dim3(256), 0, s->stream, n, sc,
                        K.p.p, K.v.p, (double *)nullptr, tol2, s->maxIters - total, 0.0);
     if (s->cg) cgStart<double>(s, iluDev<double>(s, s->dFac64.p), K, sc);
@@ -471,7 +472,9 @@ void solveMixed(phase_hip_solver *s) {
   auto trueResidual = [&]() {
     hipLaunchKernelGGL((kResidual<double>), dim3(spmvGrid(s)), dim3(kRedBlock), 0, s->stream, n,
                        s->ord.nSlices, s->dSliceOff.p, s->dSellCol.p, s->dSell64.p, s->dX.p, s->dB.p,
-                       s->dR64.p, (double *)nullptr, outer, s->dPartials.p, s->dCounter.p);
+                       s->dR64.p, (double *)nullptr, s->dPartials.p);
+    hipLaunchKernelGGL(kFinishResidual, dim3(1), dim3(kFinBlock), 0, s->stream, outer, s->dPartials.p,
+                       spmvGrid(s));
     pullScalars(s);
     const double rr = s->hSc[1].rr;
     if (!(rr <= 1e4 * bestRR)) {  // NaN included

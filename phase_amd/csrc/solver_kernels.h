@@ -364,8 +364,7 @@ template <class T>
 __global__ __launch_bounds__(kRedBlock) void kResidual(int n, int nSlices, const int *__restrict__ sliceOff,
                                                        const int *__restrict__ col, const T *__restrict__ val,
                                                        const T *__restrict__ x, const T *__restrict__ b,
-                                                       T *__restrict__ r, T *__restrict__ r0, Scalars *sc,
-                                                       double *partials, unsigned *counter) {
+                                                       T *__restrict__ r, T *__restrict__ r0, double *partials) {
   const int tile = xcdTile(blockIdx.x, gridDim.x);
   const int slice = tile * (kRedBlock / 64) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
@@ -378,7 +377,14 @@ __global__ __launch_bounds__(kRedBlock) void kResidual(int n, int nSlices, const
     if (r0) r0[row] = ri;
     d[0] = (double)ri * (double)ri;
   }
-  if (reduceLast<1>(d, partials, counter, tile) && threadIdx.x == 0) sc->rr = d[0];
+  blockPartials<1>(d, partials, tile);
+}
+
+// rr = (r, r) of the residual above
+static __global__ __launch_bounds__(kFinBlock) void kFinishResidual(Scalars *sc, const double *partials, int nb) {
+  double d[1];
+  finishPartials<1, kFinBlock>(partials, nb, d);
+  if (threadIdx.x == 0) sc->rr = d[0];
 }
 
 // *out = (v, v)
