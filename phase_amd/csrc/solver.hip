@@ -321,8 +321,9 @@ void pullScalars(phase_hip_solver *s) {
 
 double normB2(phase_hip_solver *s) {
   const int n = s->rank;
-  hipLaunchKernelGGL((kNorm2<double>), dim3(gridFor(n, kRedBlock, kRedCap)), dim3(kRedBlock), 0, s->stream,
-                     n, s->dB.p, s->dMisc.p, s->dPartials.p, s->dCounter.p);
+  const int g = gridFor(n, kRedBlock, kRedCap);
+  hipLaunchKernelGGL((kNorm2<double>), dim3(g), dim3(kRedBlock), 0, s->stream, n, s->dB.p, s->dPartials.p);
+  hipLaunchKernelGGL(kFinishNorm2, dim3(1), dim3(kFinBlock), 0, s->stream, s->dPartials.p, g, s->dMisc.p);
   double b2 = 0.;
   PH_CHECK(hipMemcpyAsync(&b2, s->dMisc.p, sizeof(double), hipMemcpyDeviceToHost, s->stream));
   PH_CHECK(hipStreamSynchronize(s->stream));
@@ -505,7 +506,9 @@ void solveMixed(phase_hip_solver *s) {
     hipLaunchKernelGGL((kConvert2<double, float>), dim3(gridFor(n, 256, kElemCap)), dim3(256), 0, s->stream, n,
                        s->dR64.p, K.r.p, K.r0.p);
     hipLaunchKernelGGL((kNorm2<float>), dim3(gridFor(n, kRedBlock, kRedCap)), dim3(kRedBlock), 0, s->stream,
-                       n, K.r.p, &inner->rr, s->dPartials.p, s->dCounter.p);
+                       n, K.r.p, s->dPartials.p);
+    hipLaunchKernelGGL(kFinishNorm2, dim3(1), dim3(kFinBlock), 0, s->stream, s->dPartials.p,
+                       gridFor(n, kRedBlock, kRedCap), &inner->rr);
     hipLaunchKernelGGL((kBicgInit<float>), dim3(gridFor(n, 256, kElemCap)), dim3(256), 0, s->stream, n, inner,
                        K.p.p, K.v.p, (float *)nullptr, tol2, s->maxIters - total, 0.0);
     if (s->cg) cgStart<float>(s, iluDev<float>(s, s->dFac32.p), K, inner);

@@ -389,12 +389,17 @@ static __global__ __launch_bounds__(kFinBlock) void kFinishResidual(Scalars *sc,
 
 // *out = (v, v)
 template <class T>
-__global__ __launch_bounds__(kRedBlock) void kNorm2(int n, const T *__restrict__ v, double *out,
-                                                    double *partials, unsigned *counter) {
+__global__ __launch_bounds__(kRedBlock) void kNorm2(int n, const T *__restrict__ v, double *partials) {
   double d[1] = {0.};
   for (int i = blockIdx.x * kRedBlock + threadIdx.x; i < n; i += gridDim.x * kRedBlock)
     d[0] += (double)v[i] * (double)v[i];
-  if (reduceLast<1>(d, partials, counter, blockIdx.x) && threadIdx.x == 0) *out = d[0];
+  blockPartials<1>(d, partials, blockIdx.x);
+}
+
+static __global__ __launch_bounds__(kFinBlock) void kFinishNorm2(const double *partials, int nb, double *out) {
+  double d[1];
+  finishPartials<1, kFinBlock>(partials, nb, d);
+  if (threadIdx.x == 0) *out = d[0];
 }
 
 // ---------------------------------------------------------------- BiCGStab vector steps
