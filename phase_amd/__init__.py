@@ -19,17 +19,24 @@ interfaces:
   ScalarGradient, interpolateFaces on device fields.
 * HipFractionalStep -- FractionalStep::solve, the reference's projection solver
   (src/2D/Unstructured/Solvers/FractionalStep.cpp:36-117), over the above.
+* HipMultiphase -- FractionalStepMultiphase::solve (CICSAM volume fraction,
+  properties, gravity source, CELESTE surface tension, momentum, pressure)
+  (src/2D/Unstructured/Solvers/FractionalStepMultiphase.cpp:58-218).
+* HipGhostCellIB -- GhostCellImmersedBoundary for circular bodies (cell types,
+  bcs / velocityBcs rows, applyHydrodynamicForce); with
+  HipFractionalStep.immersed_boundary it gives FractionalStepGCIB::solve.
 * HipSimple -- a SIMPLE outer loop over the above (the README's uEqn/pCorrEqn).
 
 There is no CPU fallback: loading fails loudly when the library is absent.
 """
 from .hip import (BC_FIXED, BC_NORMAL_GRADIENT, BC_OUTFLOW, BC_PARTIAL_SLIP, BC_SYMMETRY,
-                  MODE_FP64, MODE_LEARNED, MODE_MIXED, DeviceArray, HipEquation, HipFractionalStep, HipMesh,
-                  HipSimple, HipSparseMatrixSolver, PhaseHipError, device_count, library_path, load_library)
+                  MODE_FP64, MODE_LEARNED, MODE_MIXED, DeviceArray, HipEquation, HipFractionalStep,
+                  HipGhostCellIB, HipMesh, HipMultiphase, HipSimple, HipSparseMatrixSolver, PhaseHipError,
+                  device_count, library_path, load_library)
 
 __all__ = [
     "BC_FIXED", "BC_NORMAL_GRADIENT", "BC_OUTFLOW", "BC_PARTIAL_SLIP", "BC_SYMMETRY",
-    "MODE_FP64", "MODE_LEARNED", "MODE_MIXED", "DeviceArray", "HipEquation", "HipFractionalStep", "HipMesh", "HipSimple",
-    "HipSparseMatrixSolver",
+    "MODE_FP64", "MODE_LEARNED", "MODE_MIXED", "DeviceArray", "HipEquation", "HipFractionalStep",
+    "HipGhostCellIB", "HipMultiphase", "HipMesh", "HipSimple", "HipSparseMatrixSolver",
     "PhaseHipError", "device_count", "library_path", "load_library",
 ]
