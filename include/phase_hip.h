@@ -390,8 +390,8 @@ int phase_hip_mesh_synchronize(phase_hip_mesh *m);
 /* ------------------------------------------------------------------ ghost-cell immersed boundary
  * GhostCellImmersedBoundary (src/2D/Unstructured/FiniteVolume/ImmersedBoundary/
  * GhostCellImmersedBoundary.cpp:12-90 with its BcStencil, FixedBcStencil and
- * NormalGradientBcStencil) for circular objects (cx, cy, r per object, in
- * order): updateCells classifies the cells (0 fluid, 1 ghost: inside with a
+ * NormalGradientBcStencil) for circles (cx, cy, r per object, in order) or,
+ * through phase_hip_gcib_create_shapes, circles and boxes: updateCells classifies the cells (0 fluid, 1 ghost: inside with a
  * face or node neighbour outside, 2 solid) and the ghost stencils are built
  * once (image point, the four cells of the nearest mesh node, their bilinear
  * interpolation; the node needs four cells, as on quad meshes).
@@ -406,8 +406,13 @@ int phase_hip_mesh_synchronize(phase_hip_mesh *m);
  * phase_hip_gcib_cell_types copies the types (and owners, host only) to host
  * memory, or the types to device memory when onDevice is nonzero. */
 typedef struct phase_hip_gcib phase_hip_gcib;
+enum { PHASE_HIP_SHAPE_CIRCLE = 0, PHASE_HIP_SHAPE_BOX = 1 };
 
 int phase_hip_gcib_create(phase_hip_gcib **ib, phase_hip_mesh *m, int nCircles, const double *circles);
+/* Bodies of Shape2D::CIRCLE (geometry cx, cy, r, unused) and Shape2D::BOX
+ * (lower x, lower y, upper x, upper y; Box.cpp) in order, 4 numbers each. */
+int phase_hip_gcib_create_shapes(phase_hip_gcib **ib, phase_hip_mesh *m, int nBodies, const int *kinds,
+                                 const double *geometry);
 int phase_hip_gcib_destroy(phase_hip_gcib *ib);
 const char *phase_hip_gcib_last_error(const phase_hip_gcib *ib);
 int phase_hip_gcib_cell_types(const phase_hip_gcib *ib, int *types, int *owner, int onDevice);

@@ -1,21 +1,23 @@
 """TEST INFRASTRUCTURE ONLY.  The ghost-cell immersed-boundary rows
 (src/2D/Unstructured/FiniteVolume/ImmersedBoundary/GhostCellImmersedBoundary.cpp,
 GhostCellImmersedBoundaryBcStencil.cpp, ...FixedBcStencil.cpp,
-...NormalGradientBcStencil.cpp) for circular objects, restated on the CPU
+...NormalGradientBcStencil.cpp) for circular and box objects, restated on the CPU
 with numpy and the oracle's CrsEquation; the checker for the device rows.
 Only tests/ may use it.
 
 Cells (GhostCellImmersedBoundary::updateCells, :12-31): per object, in
-object order, the domain cells whose centroid lies strictly inside the circle
-are solid; those with a face or diagonal (node) neighbour outside the circle
+object order, the domain cells whose centroid lies strictly inside the shape
+are solid; those with a face or diagonal (node) neighbour outside the shape
 (ImmersedBoundaryObject::internalPerimeterCells with diagonals,
 ImmersedBoundaryObject.tpp:5-35) are the ghost (ib) cells; both leave the
 fluid cells.
 
 Stencil of a ghost cell c (BcStencil, GhostCellImmersedBoundaryBcStencil.cpp:3-70):
-boundary point bp = centre + R (x_c - centre) / |x_c - centre|
-(Circle::nearestIntersect), image point ip = 2 bp - x_c, wall normal
-nw = (centre - bp) / |centre - bp|; the four cells of the mesh node nearest
+boundary point bp = nearestIntersect(x_c) (circle: centre + R (x_c - centre) /
+|x_c - centre|; box: the nearest point of its edges, Box.cpp), image point
+ip = 2 bp - x_c, wall normal nw = nearestEdgeUnitNormal(bp) (circle:
+(centre - bp) / |centre - bp|; box: the unit normal of the nearest edge, turned
+towards the centroid); the four cells of the mesh node nearest
 to ip (FiniteVolumeGrid2D::findNearestNode, lowest node id on a tie; the
 reference takes its R-tree's answer) with the bilinear interpolation
 Ainv = inverse of the rows (x y, x, y, 1) of their centroids.
@@ -52,7 +54,9 @@ FLUID, GHOST, SOLID = 0, 1, 2
 
 
 class GhostCellOracle:
-    def __init__(self, mesh, xy, cptr, cind, circles):
+    def __init__(self, mesh, xy, cptr, cind, circles, boxes=()):
+        """circles: (cx, cy, r) each; boxes: (lower x, lower y, upper x, upper
+        y) each (Box.cpp); bodies are the circles, then the boxes."""
         self.m = mesh
         N = mesh.n_cells
         xy = np.asarray(xy, float).reshape(-1, 2)
@@ -79,26 +83,90 @@ class GhostCellOracle:
         self.types = np.zeros(N, np.int32)
         self.owner = np.full(N, -1, np.int32)
         self.circles = [tuple(map(float, c)) for c in circles]
-        for o, (cx, cy, r) in enumerate(self.circles):
-            d = self.cc - np.array([cx, cy])
-            inside = d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] < r * r  # Circle::isInside
+        self.bodies = [("circle",) + c for c in self.circles] + [("box",) + tuple(map(float, b)) for b in boxes]
+        for o, body in enumerate(self.bodies):
+            inside = self._inside(body, self.cc)
             for c in np.where(inside & (self.types == FLUID))[0]:
                 ghost = any(not inside[k] for k in links[c])
                 self.types[c] = GHOST if ghost else SOLID
                 self.owner[c] = o
         self.stencils = {}
         for c in np.where(self.types == GHOST)[0]:
-            self.stencils[int(c)] = self._stencil(int(c), self.circles[self.owner[c]])
+            self.stencils[int(c)] = self._stencil(int(c), self.bodies[self.owner[c]])
 
-    def _stencil(self, c, circle):
-        cx, cy, r = circle
+    # shape queries: Circle.cpp (isInside, nearestIntersect), Box.cpp
+    # (isInside, nearestIntersect, nearestEdge, edges), and
+    # ImmersedBoundaryObject::nearestEdgeUnitNormal (ImmersedBoundaryObject.cpp:76-90)
+    @staticmethod
+    def _inside(body, pts):
+        if body[0] == "circle":
+            d = pts - np.array(body[1:3])
+            return d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] < body[3] * body[3]
+        lx, ly, ux, uy = body[1:]
+        return (pts[:, 0] < ux) & (pts[:, 0] > lx) & (pts[:, 1] < uy) & (pts[:, 1] > ly)
+
+    @staticmethod
+    def _centre(body):
+        if body[0] == "circle":
+            return np.array(body[1:3])
+        return (np.array(body[1:3]) + np.array(body[3:5])) / 2.
+
+    @staticmethod
+    def _box_edges(body):
+        lx, ly, ux, uy = body[1:]
+        A = [np.array(p) for p in ((lx, ly), (ux, ly), (ux, uy), (lx, uy))]
+        return [(A[k], A[(k + 1) % 4]) for k in range(4)]
+
+    def _nearest_intersect(self, body, p):
+        if body[0] == "circle":
+            c = np.array(body[1:3])
+            d = p - c
+            return d / np.sqrt(d[0] * d[0] + d[1] * d[1]) * body[3] + c
+        best, bd = None, np.inf
+        for a, b in self._box_edges(body):
+            t, r = b - a, p - a
+            dd = r[0] * t[0] + r[1] * t[1]
+            ms = t[0] * t[0] + t[1] * t[1]
+            if dd >= 0. and dd <= ms:  # LineSegment2D::isBounded
+                rxc = (a + (dd * t) / ms) - p
+            else:
+                ea, eb = a - p, b - p
+                rxc = (a if ea[0] * ea[0] + ea[1] * ea[1] < eb[0] * eb[0] + eb[1] * eb[1] else b) - p
+            d2 = rxc[0] * rxc[0] + rxc[1] * rxc[1]
+            if d2 < bd:
+                best, bd = p + rxc, d2
+        return best
+
+    def _wall_normal(self, body, bp):
+        if body[0] == "circle":
+            w = np.array(body[1:3]) - bp
+            return w / np.sqrt(w[0] * w[0] + w[1] * w[1])
+        best, bd = None, np.inf
+        for a, b in self._box_edges(body):
+            t, r = b - a, bp - a
+            dd = r[0] * t[0] + r[1] * t[1]
+            ms = t[0] * t[0] + t[1] * t[1]
+            if dd >= 0. and dd <= ms:
+                q = r - (dd * t) / ms
+                d2 = q[0] * q[0] + q[1] * q[1]
+            else:
+                ea, eb = a - bp, b - bp
+                d2 = min(ea[0] * ea[0] + ea[1] * ea[1], eb[0] * eb[0] + eb[1] * eb[1])
+            if d2 < bd:
+                best, bd = (a, b), d2
+        a, b = best
+        t = b - a
+        n = np.array([t[1], -t[0]])
+        mid = (a + b) / 2.
+        cen = self._centre(body) - mid
+        u = n / np.sqrt(n[0] * n[0] + n[1] * n[1])
+        return u if n[0] * cen[0] + n[1] * cen[1] > 0. else -u
+
+    def _stencil(self, c, body):
         xc = self.cc[c]
-        centre = np.array([cx, cy])
-        d = xc - centre
-        bp = d / np.sqrt(d[0] * d[0] + d[1] * d[1]) * r + centre
+        bp = self._nearest_intersect(body, xc)
         ip = 2. * bp - xc
-        w = centre - bp
-        nw = w / np.sqrt(w[0] * w[0] + w[1] * w[1])
+        nw = self._wall_normal(body, bp)
         dn = self.xy - ip
         node = int(np.argmin(dn[:, 0] * dn[:, 0] + dn[:, 1] * dn[:, 1]))
         cells = list(self.node_cells[node])
@@ -128,7 +196,8 @@ class GhostCellOracle:
         u = np.asarray(u_cell, float).reshape(-1, 2)
         p = np.asarray(p_cell, float)
         out, points = [], []
-        for o, (cx, cy, r) in enumerate(self.circles):
+        for o, body in enumerate(self.bodies):
+            cx, cy = self._centre(body)
             pts = []
             for c in np.where((self.types == GHOST) & (self.owner == o))[0]:
                 st = self.stencils[int(c)]

@@ -1,14 +1,14 @@
 // Ghost-cell immersed boundary rows (src/2D/Unstructured/FiniteVolume/ImmersedBoundary/
 // GhostCellImmersedBoundary.cpp:12-90, GhostCellImmersedBoundaryBcStencil.cpp:3-70,
 // GhostCellImmersedBoundaryFixedBcStencil.cpp:17-22,
-// GhostCellImmersedBoundaryNormalGradientBcStencil.cpp:22-42) for circular objects.
+// GhostCellImmersedBoundaryNormalGradientBcStencil.cpp:22-42) for circle and box bodies.
 //
 // Set-up, host, once per mesh and object set (GhostCellImmersedBoundary::updateCells):
 //   per object in order, the remaining domain cells whose centroid lies
-//   strictly inside the circle are solid; those with a face or node neighbour
+//   strictly inside its shape are solid; those with a face or node neighbour
 //   outside it (internalPerimeterCells with diagonals) are ghost cells;
-//   ghost-cell stencil: bp = centre + R (x_c - centre) / |x_c - centre|,
-//   ip = 2 bp - x_c, nw = (centre - bp) / |centre - bp|, the four cells of the
+//   ghost-cell stencil: bp = nearestIntersect(x_c), ip = 2 bp - x_c,
+//   nw = nearestEdgeUnitNormal(bp) (the Body queries below), the four cells of the
 //   mesh node nearest to ip (lowest node id on a tie) and Ainv, the inverse of
 //   their bilinear rows (x y, x, y, 1) (Gauss-Jordan with partial pivoting;
 //   the reference calls LAPACK getrf/getri).
@@ -42,7 +42,8 @@ struct phase_hip_gcib {
   phase_hip_mesh *mesh = nullptr;
   int N = 0, nObj = 0;
   std::vector<int> types, owner;
-  std::vector<double> circles;
+  std::vector<int> kind;      // per body: PHASE_HIP_SHAPE_CIRCLE or _BOX
+  std::vector<double> geom;   // 4 per body: cx, cy, r, 0 or lower x, lower y, upper x, upper y
   std::vector<GhostPoint> points;
   DBuf<int> dGather;     // 9 cells per ghost point: cell, ipCells, bpCells
   DBuf<double> dValues;  // p, u.x, u.y at those cells
@@ -91,6 +92,112 @@ bool inverse4(const double *a, double *inv) {
     for (int j = 0; j < 4; ++j) inv[4 * i + j] = m[i][4 + j];
   return true;
 }
+
+// Shape queries of the immersed bodies: Circle.cpp (isInside, nearestIntersect),
+// Box.cpp (isInside, nearestIntersect, nearestEdge, edges) and
+// ImmersedBoundaryObject::nearestEdgeUnitNormal (ImmersedBoundaryObject.cpp:76-90).
+struct Body {
+  int kind;
+  double g[4];
+  bool inside(double x, double y) const {
+    if (kind == PHASE_HIP_SHAPE_CIRCLE) {
+      const double dx = x - g[0], dy = y - g[1];
+      return dx * dx + dy * dy < g[2] * g[2];
+    }
+    return x < g[2] && x > g[0] && y < g[3] && y > g[1];
+  }
+  void centre(double &x, double &y) const {
+    if (kind == PHASE_HIP_SHAPE_CIRCLE) {
+      x = g[0];
+      y = g[1];
+    } else {
+      x = (g[0] + g[2]) / 2.;
+      y = (g[1] + g[3]) / 2.;
+    }
+  }
+  void edge(int k, double *a, double *b) const {  // Box::edges order
+    const double px[4] = {g[0], g[2], g[2], g[0]}, py[4] = {g[1], g[1], g[3], g[3]};
+    a[0] = px[k];
+    a[1] = py[k];
+    b[0] = px[(k + 1) % 4];
+    b[1] = py[(k + 1) % 4];
+  }
+  void nearestIntersect(double px, double py, double &bx, double &by) const {
+    if (kind == PHASE_HIP_SHAPE_CIRCLE) {
+      const double dx = px - g[0], dy = py - g[1], dm = std::sqrt(dx * dx + dy * dy);
+      bx = dx / dm * g[2] + g[0];
+      by = dy / dm * g[2] + g[1];
+      return;
+    }
+    double best = INFINITY;
+    for (int k = 0; k < 4; ++k) {
+      double a[2], b[2];
+      edge(k, a, b);
+      const double tx = b[0] - a[0], ty = b[1] - a[1], rx = px - a[0], ry = py - a[1];
+      const double d = rx * tx + ry * ty, ms = tx * tx + ty * ty;
+      double qx, qy;
+      if (d >= 0. && d <= ms) {  // LineSegment2D::isBounded
+        qx = (a[0] + (d * tx) / ms) - px;
+        qy = (a[1] + (d * ty) / ms) - py;
+      } else {
+        const double ax = a[0] - px, ay = a[1] - py, cx = b[0] - px, cy = b[1] - py;
+        const bool useA = ax * ax + ay * ay < cx * cx + cy * cy;
+        qx = (useA ? a[0] : b[0]) - px;
+        qy = (useA ? a[1] : b[1]) - py;
+      }
+      const double d2 = qx * qx + qy * qy;
+      if (d2 < best) {
+        best = d2;
+        bx = px + qx;
+        by = py + qy;
+      }
+    }
+  }
+  void wallNormal(double bx, double by, double &nx, double &ny) const {
+    if (kind == PHASE_HIP_SHAPE_CIRCLE) {
+      const double wx = g[0] - bx, wy = g[1] - by, wm = std::sqrt(wx * wx + wy * wy);
+      nx = wx / wm;
+      ny = wy / wm;
+      return;
+    }
+    double best = INFINITY, ea[2] = {0., 0.}, eb[2] = {0., 0.};
+    for (int k = 0; k < 4; ++k) {
+      double a[2], b[2];
+      edge(k, a, b);
+      const double tx = b[0] - a[0], ty = b[1] - a[1], rx = bx - a[0], ry = by - a[1];
+      const double d = rx * tx + ry * ty, ms = tx * tx + ty * ty;
+      double d2;
+      if (d >= 0. && d <= ms) {
+        const double qx = rx - (d * tx) / ms, qy = ry - (d * ty) / ms;
+        d2 = qx * qx + qy * qy;
+      } else {
+        const double ax = a[0] - bx, ay = a[1] - by, cx = b[0] - bx, cy = b[1] - by;
+        d2 = std::min(ax * ax + ay * ay, cx * cx + cy * cy);
+      }
+      if (d2 < best) {
+        best = d2;
+        ea[0] = a[0];
+        ea[1] = a[1];
+        eb[0] = b[0];
+        eb[1] = b[1];
+      }
+    }
+    const double tx = eb[0] - ea[0], ty = eb[1] - ea[1];
+    const double n0 = ty, n1 = -tx;  // LineSegment2D::norm = (B - A).normalVec()
+    double cx, cy;
+    centre(cx, cy);
+    const double mx = cx - (ea[0] + eb[0]) / 2., my = cy - (ea[1] + eb[1]) / 2.;
+    const double m = std::sqrt(n0 * n0 + n1 * n1);
+    const double ux = n0 / m, uy = n1 / m;
+    if (n0 * mx + n1 * my > 0.) {
+      nx = ux;
+      ny = uy;
+    } else {
+      nx = -ux;
+      ny = -uy;
+    }
+  }
+};
 
 // p, u.x, u.y at the gathered cells
 __global__ void kGatherPU(int m, const int *__restrict__ idx, const double *__restrict__ p,
@@ -154,8 +261,10 @@ int phase_hip_gcib_destroy(phase_hip_gcib *ib) {
 
 const char *phase_hip_gcib_last_error(const phase_hip_gcib *ib) { return ib ? ib->err.c_str() : "null object"; }
 
-int phase_hip_gcib_create(phase_hip_gcib **out, phase_hip_mesh *m, int nCircles, const double *circles) {
-  if (!out || !m || nCircles < 0 || (nCircles > 0 && !circles)) return PHASE_HIP_EINVAL;
+int phase_hip_gcib_create_shapes(phase_hip_gcib **out, phase_hip_mesh *m, int nBodies, const int *kinds,
+                                 const double *geometry) {
+  if (!out || !m || nBodies < 0 || (nBodies > 0 && (!kinds || !geometry))) return PHASE_HIP_EINVAL;
+  const int nCircles = nBodies;
   *out = nullptr;
   auto *ib = new phase_hip_gcib();
   try {
@@ -181,13 +290,27 @@ int phase_hip_gcib_create(phase_hip_gcib **out, phase_hip_mesh *m, int nCircles,
     ib->types.assign(N, 0);
     ib->owner.assign(N, -1);
     std::vector<char> inside(N);
-    for (int o = 0; o < nCircles; ++o) {
-      const double cx = circles[3 * o], cy = circles[3 * o + 1], r = circles[3 * o + 2];
-      if (!(r > 0.)) fail(PHASE_HIP_EINVAL, "circle radius must be positive");
-      for (int c = 0; c < N; ++c) {
-        const double dx = cc[2 * c] - cx, dy = cc[2 * c + 1] - cy;
-        inside[c] = dx * dx + dy * dy < r * r;  // Circle::isInside
+    std::vector<Body> bodies(nBodies);
+    for (int o = 0; o < nBodies; ++o) {
+      Body &bd = bodies[o];
+      bd.kind = kinds[o];
+      std::copy(geometry + 4 * o, geometry + 4 * o + 4, bd.g);
+      if (bd.kind == PHASE_HIP_SHAPE_CIRCLE) {
+        if (!(bd.g[2] > 0.)) fail(PHASE_HIP_EINVAL, "circle radius must be positive");
+      } else if (bd.kind == PHASE_HIP_SHAPE_BOX) {
+        // Box::Box orders the corners
+        if (bd.g[2] < bd.g[0]) std::swap(bd.g[0], bd.g[2]);
+        if (bd.g[3] < bd.g[1]) std::swap(bd.g[1], bd.g[3]);
+        if (!(bd.g[2] > bd.g[0]) || !(bd.g[3] > bd.g[1])) fail(PHASE_HIP_EINVAL, "empty box");
+      } else {
+        fail(PHASE_HIP_EINVAL, "body shapes are PHASE_HIP_SHAPE_CIRCLE or PHASE_HIP_SHAPE_BOX");
       }
+      ib->kind.push_back(bd.kind);
+      ib->geom.insert(ib->geom.end(), bd.g, bd.g + 4);
+    }
+    for (int o = 0; o < nBodies; ++o) {
+      const Body &bd = bodies[o];
+      for (int c = 0; c < N; ++c) inside[c] = bd.inside(cc[2 * c], cc[2 * c + 1]);
       for (int c = 0; c < N; ++c) {
         if (!inside[c] || ib->types[c] != 0) continue;
         bool ghost = false;
@@ -244,14 +367,11 @@ int phase_hip_gcib_create(phase_hip_gcib **out, phase_hip_mesh *m, int nCircles,
     for (int c = 0; c < N; ++c) {
       if (ib->types[c] != 1) continue;
       const int o = ib->owner[c];
-      const double cx = circles[3 * o], cy = circles[3 * o + 1], r = circles[3 * o + 2];
       const double xc = cc[2 * c], yc = cc[2 * c + 1];
-      // Circle::nearestIntersect, ImmersedBoundaryObject::nearestEdgeUnitNormal
-      const double dx = xc - cx, dy = yc - cy, dm = std::sqrt(dx * dx + dy * dy);
-      const double bpx = dx / dm * r + cx, bpy = dy / dm * r + cy;
+      double bpx = 0., bpy = 0., nwx = 0., nwy = 0.;
+      bodies[o].nearestIntersect(xc, yc, bpx, bpy);
       const double ipx = 2. * bpx - xc, ipy = 2. * bpy - yc;
-      const double wx = cx - bpx, wy = cy - bpy, wm = std::sqrt(wx * wx + wy * wy);
-      const double nwx = wx / wm, nwy = wy / wm;
+      bodies[o].wallNormal(bpx, bpy, nwx, nwy);
       const int node = nearestNode(ipx, ipy);
       const std::vector<int> &cells = nodeCells[node];
       if (cells.size() != 4)
@@ -327,7 +447,6 @@ int phase_hip_gcib_create(phase_hip_gcib **out, phase_hip_mesh *m, int nCircles,
         ngVals.push_back(1. / l);
       }
     }
-    ib->circles.assign(circles, circles + 3 * (size_t)nCircles);
     std::vector<int> gather;
     for (const GhostPoint &g : ib->points) {
       gather.push_back(g.cell);
@@ -354,6 +473,14 @@ int phase_hip_gcib_create(phase_hip_gcib **out, phase_hip_mesh *m, int nCircles,
   return PHASE_HIP_OK;
 }
 
+int phase_hip_gcib_create(phase_hip_gcib **out, phase_hip_mesh *m, int nCircles, const double *circles) {
+  if (nCircles < 0 || (nCircles > 0 && !circles)) return PHASE_HIP_EINVAL;
+  std::vector<int> kinds(nCircles, PHASE_HIP_SHAPE_CIRCLE);
+  std::vector<double> geom(4 * (size_t)nCircles, 0.);
+  for (int o = 0; o < nCircles; ++o) std::copy(circles + 3 * o, circles + 3 * o + 3, geom.begin() + 4 * o);
+  return phase_hip_gcib_create_shapes(out, m, nCircles, kinds.data(), geom.data());
+}
+
 int phase_hip_gcib_hydrodynamic_force(phase_hip_gcib *ib, double rho, double mu, const double *uCell,
                                       const double *pCell, double gx, double gy, double *force) {
   if (!ib || !uCell || !pCell || !force) return PHASE_HIP_EINVAL;
@@ -373,7 +500,9 @@ int phase_hip_gcib_hydrodynamic_force(phase_hip_gcib *ib, double rho, double mu,
       double angle, x, y, pr, sh;
     };
     for (int o = 0; o < ib->nObj; ++o) {
-      const double cx = ib->circles[3 * o], cy = ib->circles[3 * o + 1];
+      Body bd{ib->kind[o], {ib->geom[4 * o], ib->geom[4 * o + 1], ib->geom[4 * o + 2], ib->geom[4 * o + 3]}};
+      double cx, cy;
+      bd.centre(cx, cy);
       std::vector<Stress> pts;
       for (size_t k = 0; k < ib->points.size(); ++k) {
         const GhostPoint &g = ib->points[k];

@@ -120,6 +120,7 @@ _SIGS = {
     "phase_hip_fstep_fields": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "phase_hip_fstep_immersed_boundary": (ctypes.c_int, [_vp, _vp, _f64p, ctypes.c_int, ctypes.c_double]),
     "phase_hip_gcib_create": (ctypes.c_int, [ctypes.POINTER(_vp), _vp, ctypes.c_int, _f64p]),
+    "phase_hip_gcib_create_shapes": (ctypes.c_int, [ctypes.POINTER(_vp), _vp, ctypes.c_int, _i32p, _f64p]),
     "phase_hip_gcib_destroy": (ctypes.c_int, [_vp]),
     "phase_hip_gcib_last_error": (ctypes.c_char_p, [_vp]),
     "phase_hip_gcib_cell_types": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int]),
@@ -544,11 +545,20 @@ class HipGhostCellIB:
     """GhostCellImmersedBoundary for circular objects ((cx, cy, r) each):
     cell types (0 fluid, 1 ghost, 2 solid) and the boundary rows."""
 
-    def __init__(self, mesh, circles):
+    def __init__(self, mesh, circles, boxes=()):
+        """circles: (cx, cy, r) each; boxes: (lower x, lower y, upper x,
+        upper y) each; bodies are the circles, then the boxes."""
         h = _vp()
-        c = _f64(np.asarray(circles, float).reshape(-1))
-        self._n_bodies = len(c) // 3
-        rc = load_library().phase_hip_gcib_create(ctypes.byref(h), mesh._h, len(c) // 3, c)
+        circles, boxes = list(circles), list(boxes)
+        self._n_bodies = len(circles) + len(boxes)
+        if boxes:
+            kinds = _i32([0] * len(circles) + [1] * len(boxes))
+            geom = _f64([list(map(float, c)) + [0.0] for c in circles] + [list(map(float, b)) for b in boxes])
+            rc = load_library().phase_hip_gcib_create_shapes(ctypes.byref(h), mesh._h, self._n_bodies, kinds,
+                                                             geom.reshape(-1))
+        else:
+            c = _f64(np.asarray(circles, float).reshape(-1))
+            rc = load_library().phase_hip_gcib_create(ctypes.byref(h), mesh._h, len(c) // 3, c)
         if rc != 0:
             raise PhaseHipError(rc, "ghostCellCreate", "creation failed")
         self._h = h

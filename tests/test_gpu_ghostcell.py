@@ -157,3 +157,38 @@ def test_hydrodynamic_force_matches_oracle(oracle, hip, case):
     got = dg.hydrodynamic_force(1.3, 0.02, hip.DeviceArray(u.reshape(-1)), hip.DeviceArray(p), g=(0.0, -9.81))
     assert got.shape == ref.shape
     assert np.abs(got - ref).max() <= 1e-10 * max(1.0, np.abs(ref).max())
+
+
+@pytest.mark.parametrize("kind", ["fixed", "ng", "velocity", "force"])
+def test_box_and_circle_bodies_match_oracle(oracle, hip, kind):
+    """A circle and a box (Box.cpp) together: cell types identical, rows with
+    identical columns and sources and values within 1e-12, forces within 1e-10."""
+    from oracle.ghostcell_oracle import GhostCellOracle
+    xy, cptr, cind = G.rectilinear(44, 40)
+    om, hm = oracle.Mesh(xy, cptr, cind), hip.HipMesh(xy, cptr, cind)
+    circles, boxes = [(0.3, 0.35, 0.15)], [(0.55, 0.42, 0.83, 0.71)]
+    og = GhostCellOracle(om, xy, cptr, cind, circles, boxes=boxes)
+    dg = hip.HipGhostCellIB(hm, circles, boxes=boxes)
+    t, o = dg.cell_types()
+    assert np.array_equal(t, og.types) and np.array_equal(o, og.owner)
+    assert set(np.unique(o[t > 0])) == {0, 1}
+    if kind == "force":
+        _, cc = om.cells()
+        u = np.stack([np.sin(3 * cc[:, 0]) * cc[:, 1], cc[:, 0] ** 2 - cc[:, 1]], -1)
+        p = 1.0 + cc[:, 0] - 2.0 * cc[:, 1] ** 2
+        ref, _ = og.hydrodynamic_force(1.0, 0.01, u, p)
+        got = dg.hydrodynamic_force(1.0, 0.01, hip.DeviceArray(u.reshape(-1)), hip.DeviceArray(p))
+        assert np.abs(got - ref).max() <= 1e-10 * max(1.0, np.abs(ref).max())
+        return
+    if kind == "velocity":
+        U = [(0.2, 0.0), (0.0, -0.4)]
+        ref = og.velocity_bcs(U)
+        he = hip.HipEquation(hm, 2).gcib_velocity_bcs(dg, U)
+    else:
+        bc = oracle.BC_FIXED if kind == "fixed" else oracle.BC_NORMAL_GRADIENT
+        ref = og.bcs(bc, 0.25)
+        he = hip.HipEquation(hm, 1).gcib_bcs(dg, bc, 0.25)
+    rp, ci, v, rhs = he.export()
+    orp, oci, ov, orhs = ref.get()
+    assert np.array_equal(rp, orp) and np.array_equal(ci, oci) and np.array_equal(rhs, orhs)
+    assert np.abs(v - ov).max() <= 1e-12 * max(1.0, np.abs(ov).max())

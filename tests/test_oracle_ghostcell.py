@@ -124,3 +124,42 @@ def test_hydrodynamic_force_closed_forms(oracle, gc):
         nw = st["nw"]
         tn = T @ nw
         assert abs(sh - 0.1 * (tn[0] * -nw[1] + tn[1] * nw[0])) <= 1e-12
+
+
+def test_box_bodies(oracle, gc):
+    """A box body (Box.cpp): boundary points on its edges, wall normals the
+    inward edge normals; FIXED rows exact for linear fields everywhere and
+    NORMAL_GRADIENT rows where the boundary point is a foot point (at a corner
+    the image direction is not the edge normal, as in the reference)."""
+    n = 40
+    xy, cptr, cind = G.rectilinear(n, n)
+    m = oracle.Mesh(xy, cptr, cind)
+    _, cc = m.cells()
+    box = (0.31, 0.27, 0.62, 0.58)
+    g = gc(m, xy, cptr, cind, [], boxes=[box])
+    inside = (cc[:, 0] > box[0]) & (cc[:, 0] < box[2]) & (cc[:, 1] > box[1]) & (cc[:, 1] < box[3])
+    assert np.array_equal(g.types > 0, inside)
+    ghost = np.where(g.types == 1)[0]
+    a = np.array([0.3, -1.2])
+    lin = lambda x: 0.7 + x @ a
+    N = m.n_cells
+    A_fix, _ = _rows(g.bcs(oracle.BC_FIXED, 0.0), N)
+    A_ng, _ = _rows(g.bcs(oracle.BC_NORMAL_GRADIENT, 0.0), N)
+    yf, yn = A_fix @ lin(cc), A_ng @ lin(cc)
+    corners = 0
+    for c in ghost:
+        st = g.stencils[c]
+        bp, nw = st["bp"], st["nw"]
+        on_x = bp[0] in (box[0], box[2])
+        on_y = bp[1] in (box[1], box[3])
+        assert on_x or on_y
+        assert abs(abs(nw[0]) + abs(nw[1]) - 1.0) < 1e-15  # an axis direction
+        assert abs(yf[c] - lin(bp)) <= 1e-11
+        if on_x and on_y:
+            corners += 1
+            continue
+        assert abs(yn[c] - a @ nw) <= 1e-11
+        # the inward normal points from bp towards the box's inside
+        assert (on_x and nw[0] * (0.5 * (box[0] + box[2]) - bp[0]) > 0) or \
+               (on_y and nw[1] * (0.5 * (box[1] + box[3]) - bp[1]) > 0)
+    assert corners <= 4
