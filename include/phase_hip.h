@@ -418,6 +418,11 @@ const char *phase_hip_gcib_last_error(const phase_hip_gcib *ib);
 int phase_hip_gcib_cell_types(const phase_hip_gcib *ib, int *types, int *owner, int onDevice);
 int phase_hip_eq_gcib_bcs(phase_hip_equation *e, const phase_hip_gcib *ib, int bcType, double refValue);
 int phase_hip_eq_gcib_velocity_bcs(phase_hip_equation *e, const phase_hip_gcib *ib, const double *objectVelocity);
+/* velocityBcs(u) for moving bodies, 3 numbers each (vel.x, vel.y, omega): the
+ * body velocity at a point is Motion::velocity(pt) = vel + omega (pt -
+ * centroid).tangentVec() (Motion.h:31-33), taken at bp for the ghost rows and
+ * at the cell centroid for the solid rows. */
+int phase_hip_eq_gcib_motion_bcs(phase_hip_equation *e, const phase_hip_gcib *ib, const double *objectMotion);
 /* GhostCellImmersedBoundary::applyHydrodynamicForce(rho, mu, u, p, g) --
  * GhostCellImmersedBoundary.cpp:94-146: per body, pressure (bpValue(p) +
  * rho g.bp) and wall shear (mu (grad u(bp) . nw) . nw.tangentVec()) at the

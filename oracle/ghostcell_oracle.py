@@ -244,15 +244,22 @@ class GhostCellOracle:
                     e.add_rhs(c, -ref_value)
         return e
 
-    def velocity_bcs(self, velocities):
-        """GhostCellImmersedBoundary::velocityBcs for translating objects
-        (velocity per object): Vector2D rows in the block layout."""
+    def velocity_bcs(self, velocities, omegas=None):
+        """GhostCellImmersedBoundary::velocityBcs: Vector2D rows in the block
+        layout.  Body velocity at a point (Motion::velocity, Motion.h:31-33):
+        vel + omega (pt - pos).tangentVec(), pos the body's centroid; the
+        ghost rows take it at bp, the solid rows at the cell centroid."""
         N = self.m.n_cells
         e = O.Eqn.create(2 * N, 5)
         for c in range(N):
             if self.types[c] == FLUID:
                 continue
-            u = velocities[self.owner[c]]
+            o = self.owner[c]
+            v = velocities[o]
+            w = 0.0 if omegas is None else float(omegas[o])
+            pt = self.stencils[c]["bp"] if self.types[c] == GHOST else self.cc[c]
+            d = pt - self._centre(self.bodies[o])
+            u = (v[0] + (-d[1]) * w, v[1] + d[0] * w)
             if self.types[c] == GHOST:
                 cols, vals = self.stencils[c]["fixed"]
                 for q in range(2):

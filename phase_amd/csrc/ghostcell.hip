@@ -624,15 +624,51 @@ int phase_hip_eq_gcib_bcs(phase_hip_equation *e, const phase_hip_gcib *ib, int b
   return ghostRows(e, ib, 1, fixed, src);
 }
 
-int phase_hip_eq_gcib_velocity_bcs(phase_hip_equation *e, const phase_hip_gcib *ib, const double *objectVelocity) {
-  if (!ib || !e || (ib->nObj > 0 && !objectVelocity)) return PHASE_HIP_EINVAL;
+int phase_hip_eq_gcib_motion_bcs(phase_hip_equation *e, const phase_hip_gcib *ib, const double *objectMotion) {
+  if (!ib || !e || (ib->nObj > 0 && !objectMotion)) return PHASE_HIP_EINVAL;
+  // Motion::velocity(pt) = vel + omega (pt - pos).tangentVec() (Motion.h:31-33),
+  // pos the body's centroid: at bp for the ghost rows, the centroid for the solid rows
+  std::vector<double> bpOf(2 * (size_t)ib->N, 0.);
+  for (const GhostPoint &g : ib->points) {
+    bpOf[2 * g.cell] = g.bp[0];
+    bpOf[2 * g.cell + 1] = g.bp[1];
+  }
+  std::vector<double> cc;
+  {
+    phase_hip_mesh *m = ib->mesh;
+    cc.resize(2 * (size_t)ib->N);
+    try {
+      PH_CHECK(hipMemcpy(cc.data(), m->dCC.p, cc.size() * sizeof(double), hipMemcpyDeviceToHost));
+    } catch (const HipError &x) {
+      e->err = x.msg;
+      return x.code;
+    }
+  }
   std::vector<double> src(2 * (size_t)ib->N, 0.);
   for (int c = 0; c < ib->N; ++c)
     if (ib->types[c] != 0) {
-      src[2 * c] = -objectVelocity[2 * ib->owner[c]];
-      src[2 * c + 1] = -objectVelocity[2 * ib->owner[c] + 1];
+      const int o = ib->owner[c];
+      Body bd{ib->kind[o], {ib->geom[4 * o], ib->geom[4 * o + 1], ib->geom[4 * o + 2], ib->geom[4 * o + 3]}};
+      double px, py;
+      bd.centre(px, py);
+      const double x = ib->types[c] == 1 ? bpOf[2 * c] : cc[2 * c];
+      const double y = ib->types[c] == 1 ? bpOf[2 * c + 1] : cc[2 * c + 1];
+      const double w = objectMotion[3 * o + 2];
+      const double ux = objectMotion[3 * o] + -(y - py) * w, uy = objectMotion[3 * o + 1] + (x - px) * w;
+      src[2 * c] = -ux;
+      src[2 * c + 1] = -uy;
     }
   return ghostRows(e, ib, 2, true, src);
+}
+
+int phase_hip_eq_gcib_velocity_bcs(phase_hip_equation *e, const phase_hip_gcib *ib, const double *objectVelocity) {
+  if (!ib || !e || (ib->nObj > 0 && !objectVelocity)) return PHASE_HIP_EINVAL;
+  std::vector<double> motion(3 * (size_t)ib->nObj, 0.);
+  for (int o = 0; o < ib->nObj; ++o) {
+    motion[3 * o] = objectVelocity[2 * o];
+    motion[3 * o + 1] = objectVelocity[2 * o + 1];
+  }
+  return phase_hip_eq_gcib_motion_bcs(e, ib, motion.data());
 }
 
 }  // extern "C"

@@ -128,6 +128,7 @@ _SIGS = {
                                                           ctypes.c_double, ctypes.c_double, _vp]),
     "phase_hip_eq_gcib_bcs": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_double]),
     "phase_hip_eq_gcib_velocity_bcs": (ctypes.c_int, [_vp, _vp, _f64p]),
+    "phase_hip_eq_gcib_motion_bcs": (ctypes.c_int, [_vp, _vp, _f64p]),
     "phase_hip_multiphase_create": (ctypes.c_int, [ctypes.POINTER(_vp), _vp] + [ctypes.c_double] * 6 +
                                     [_i32p, _f64p, _vp, _vp, _vp, _f64p, ctypes.c_int]),
     "phase_hip_multiphase_destroy": (ctypes.c_int, [_vp]),
@@ -431,10 +432,16 @@ class HipEquation:
         self._check(load_library().phase_hip_eq_gcib_bcs(self._h, ib._h, int(bc_type), float(ref_value)), "bcs")
         return self
 
-    def gcib_velocity_bcs(self, ib, velocities):
-        """GhostCellImmersedBoundary::velocityBcs(u) for translating objects."""
-        v = _f64(np.asarray(velocities, float).reshape(-1))
-        self._check(load_library().phase_hip_eq_gcib_velocity_bcs(self._h, ib._h, v), "velocityBcs")
+    def gcib_velocity_bcs(self, ib, velocities, omegas=None):
+        """GhostCellImmersedBoundary::velocityBcs(u): body velocities (and
+        angular velocities about their centroids)."""
+        v = np.asarray(velocities, float).reshape(-1, 2)
+        if omegas is None:
+            rc = load_library().phase_hip_eq_gcib_velocity_bcs(self._h, ib._h, _f64(v.reshape(-1)))
+        else:
+            m = _f64(np.c_[v, np.asarray(omegas, float)].reshape(-1))
+            rc = load_library().phase_hip_eq_gcib_motion_bcs(self._h, ib._h, m)
+        self._check(rc, "velocityBcs")
         return self
 
     def __iadd__(self, other):

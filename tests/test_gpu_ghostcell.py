@@ -180,10 +180,10 @@ def test_box_and_circle_bodies_match_oracle(oracle, hip, kind):
         got = dg.hydrodynamic_force(1.0, 0.01, hip.DeviceArray(u.reshape(-1)), hip.DeviceArray(p))
         assert np.abs(got - ref).max() <= 1e-10 * max(1.0, np.abs(ref).max())
         return
-    if kind == "velocity":
-        U = [(0.2, 0.0), (0.0, -0.4)]
-        ref = og.velocity_bcs(U)
-        he = hip.HipEquation(hm, 2).gcib_velocity_bcs(dg, U)
+    if kind == "velocity":  # moving bodies: translation and rotation
+        U, W = [(0.2, 0.0), (0.0, -0.4)], [0.8, -1.3]
+        ref = og.velocity_bcs(U, omegas=W)
+        he = hip.HipEquation(hm, 2).gcib_velocity_bcs(dg, U, omegas=W)
     else:
         bc = oracle.BC_FIXED if kind == "fixed" else oracle.BC_NORMAL_GRADIENT
         ref = og.bcs(bc, 0.25)

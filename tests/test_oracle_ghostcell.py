@@ -163,3 +163,24 @@ def test_box_bodies(oracle, gc):
         assert (on_x and nw[0] * (0.5 * (box[0] + box[2]) - bp[0]) > 0) or \
                (on_y and nw[1] * (0.5 * (box[1] + box[3]) - bp[1]) > 0)
     assert corners <= 4
+
+
+def test_rotating_body_rows(oracle, gc):
+    """Moving bodies (Motion::velocity = vel + omega (pt - pos).tangentVec()):
+    the rigid motion field of each body, linear in x, satisfies its own ghost
+    and solid rows (the FIXED stencil is exact for linear fields)."""
+    n = 32
+    xy, cptr, cind = G.rectilinear(n, n)
+    m = oracle.Mesh(xy, cptr, cind)
+    N = m.n_cells
+    _, cc = m.cells()
+    g = gc(m, xy, cptr, cind, [(0.3, 0.3, 0.15)], boxes=[(0.5, 0.55, 0.8, 0.85)])
+    V, W = [(0.2, -0.1), (0.0, 0.3)], [1.5, -0.7]
+    A, rhs = _rows(g.velocity_bcs(V, omegas=W), 2 * N)
+    for o, body in enumerate(g.bodies):
+        pos = g._centre(body)
+        d = cc - pos
+        field = np.c_[V[o][0] - d[:, 1] * W[o], V[o][1] + d[:, 0] * W[o]]
+        y = A @ np.r_[field[:, 0], field[:, 1]] + rhs
+        rows = np.where((g.types > 0) & (g.owner == o))[0]
+        assert np.abs(y[rows]).max() <= 1e-12 and np.abs(y[N + rows]).max() <= 1e-12
