@@ -12,6 +12,14 @@ One time step (src/2D/Unstructured/Solvers/FractionalStep.cpp:36-117):
 With no FIXED pressure face the pressure matrix is singular and one cell is
 pinned to 0, as on the device.
 
+Pinned by tests/test_oracle_fstep.py: a pressure-driven Poiseuille channel
+(steady profile G y (H - y) / 2 mu to second order, exactly linear pressure),
+a viscous shear mode decaying at the Crank-Nicolson rate of the theta = 0.5
+viscous term, and a closed box at rest staying at rest.  `plant` (tests only)
+plants one mistake so that the tests can show each pin catches it:
+"drop_restore" skips u += dt grad p, "flip_src" adds src(grad p) instead of
+subtracting it, "theta1" makes the viscous term fully implicit.
+
 With an immersed boundary this is FractionalStepGCIB::solve
 (src/2D/Unstructured/Solvers/FractionalStepGCIB.cpp:14-56): the fv:: terms
 cover the fluid cells only (their other rows removed), the ghost-cell rows of
@@ -35,11 +43,12 @@ def _csr(rp, ci, v):
 
 
 class FractionalStepOracle:
-    def __init__(self, mesh, rho, mu, u_bc, u_face0, p_bc=None, p_face0=None, pin=0, immersed=None):
+    def __init__(self, mesh, rho, mu, u_bc, u_face0, p_bc=None, p_face0=None, pin=0, immersed=None, plant=None):
         """immersed: None or (GhostCellOracle, body velocities, pressure bc type
         on the bodies, its reference value)."""
         self.m, self.rho, self.mu, self.pin = mesh, rho, mu, pin
         self.ib = immersed
+        self.plant = plant
         N, F = mesh.n_cells, mesh.n_faces
         self.ubc = np.asarray(u_bc, np.int32)
         self.pbc = (np.full(F, O.BC_NORMAL_GRADIENT, np.int32) if p_bc is None else np.asarray(p_bc, np.int32))
@@ -78,10 +87,14 @@ class FractionalStepOracle:
         if ib is not None:
             E.remove_cells(N, other)
             E += g.velocity_bcs(vel)
-        L = O.laplacian(m, 0, ncomp=2, gamma=self.mu / self.rho, theta=0.5, bc_type=self.ubc,
+        theta = 1.0 if self.plant == "theta1" else 0.5
+        L = O.laplacian(m, 0, ncomp=2, gamma=self.mu / self.rho, theta=theta, bc_type=self.ubc,
                         phi_face=self.uF.reshape(-1), phi0_cell=u0C.reshape(-1), phi0_face=u0F.reshape(-1))
         if ib is None:
-            L -= O.src_src(m, self.gpC.reshape(-1), ncomp=2)
+            if self.plant == "flip_src":
+                L += O.src_src(m, self.gpC.reshape(-1), ncomp=2)
+            else:
+                L -= O.src_src(m, self.gpC.reshape(-1), ncomp=2)
         else:
             L.remove_cells(N, other)
             src = O.src_src(m, (self.gpC / self.rho).reshape(-1), ncomp=2)
@@ -90,7 +103,8 @@ class FractionalStepOracle:
         E.eq(L)
         self.uC = self._solve(E).reshape(2, N).T.copy()
         if ib is None:
-            self.uC = self.uC + dt * self.gpC
+            if self.plant != "drop_restore":
+                self.uC = self.uC + dt * self.gpC
         else:
             self.uC[fluid] = self.uC[fluid] + (dt / self.rho) * self.gpC[fluid]
         self.uF = O.interpolate_faces(m, self.uC.reshape(-1), self.uF.reshape(-1), bc_type=self.ubc,

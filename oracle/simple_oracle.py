@@ -19,6 +19,14 @@ PISO (Issa 1986) uses the same pieces per time step: the momentum predictor
 with fv::ddt(rho, u, dt) and no relaxation, then several pressure corrections
 (alpha_p = 1), each from the latest velocity and pressure gradient.
 
+Pinned by tests/test_oracle_fstep.py: SIMPLE and PISO marched to steady state
+on the Re = 100 cavity land on the centreline data of Ghia, Ghia & Shin
+(1982), and a PISO transient of the impulsively started cavity converges (first
+order in dt) to the FractionalStep transient of the same flow, whose own
+restatement is pinned by closed forms.  `plant` (tests only) plants one mistake
+so that the tests can show each pin catches it: "flip_src" adds src(grad p)
+instead of subtracting it, "ddt_half" halves the ddt coefficient.
+
 Face averages use Face::distanceWeight (Face.cpp:60-64).  Elementwise
 arithmetic is written in the device kernels' order, so only the linear solves
 (direct here, BiCGStab there) differ.
@@ -45,9 +53,10 @@ def _diag_of(rp, ci, v, row):
 
 
 class SimpleOracle:
-    def __init__(self, mesh, rho, mu, alpha_u, alpha_p, u_bc, u_boundary, pin=0):
+    def __init__(self, mesh, rho, mu, alpha_u, alpha_p, u_bc, u_boundary, pin=0, plant=None):
         self.m = mesh
         self.rho, self.mu, self.au, self.ap, self.pin = rho, mu, alpha_u, alpha_p, pin
+        self.plant = plant
         N, F = mesh.n_cells, mesh.n_faces
         f = mesh.faces()
         self.fl, self.fr = f["lcell"], f["rcell"]
@@ -106,13 +115,16 @@ class SimpleOracle:
                   phi_face=self.uwall.reshape(-1), phi0_cell=self.uC.reshape(-1), phi0_face=self.uwall.reshape(-1),
                   ncomp=2)
         if dt > 0.0:
-            T = O.ddt(m, dt, self.u0C.reshape(-1), ncomp=2)
+            T = O.ddt(m, 2.0 * dt if self.plant == "ddt_half" else dt, self.u0C.reshape(-1), ncomp=2)
             T += E
             E = T
         if self.rho != 1.0:
             E.scale(self.rho)
         L = O.laplacian(m, 1, ncomp=2, gamma=self.mu, bc_type=self.ubc, phi_face=self.uwall.reshape(-1))
-        L -= gsrc
+        if self.plant == "flip_src":
+            L += gsrc
+        else:
+            L -= gsrc
         E.eq(L)
         if alpha_u < 1.0:
             E.relax(alpha_u, self.uC.T.reshape(-1).copy())
