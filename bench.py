@@ -545,7 +545,6 @@ def main():
     ap.add_argument("--sigma", type=float, default=1.0)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--inner-tol", type=float, default=0.1)
-    ap.add_argument("--sweep", default="colour")
     ap.add_argument("--block-rows", type=int, default=32, help="rows per ILU aggregate (blocks pack aggregates to 64)")
     ap.add_argument("--ordering", default="geometric", help="ILU aggregates: geometric (bricks) or graph")
     ap.add_argument("--modes", default="fp64,fp64-cg,mixed,mixed-cg,learned",
@@ -602,7 +601,7 @@ def main():
         # "<precision>" runs BiCGStab, "<precision>-cg" preconditioned CG
         prec, _, kry = mode.partition("-")
         s = P.HipSparseMatrixSolver(0, tolerance=args.tol, maxIters=50000, mode=prec,
-                                    innerTolerance=args.inner_tol, sweep=args.sweep, blockRows=args.block_rows,
+                                    innerTolerance=args.inner_tol, blockRows=args.block_rows,
                                     solver=kry or "bicgstab", ordering=args.ordering)
 
         def step():

@@ -97,8 +97,6 @@ int phase_hip_solver_destroy(phase_hip_solver *s);
 /* SparseMatrixSolver::setup(ptree) -- TrilinosBelosSparseMatrixSolver.cpp:44-86.
  * Keys: "tolerance" (1e-8), "maxIters" (500), "mode" (fp64|mixed|learned),
  * "innerTolerance" (mixed: residual reduction between FP64 refinement points, 0.1),
- * "sweep" (ILU sweeps: "colour" = one launch per block colour, "syncfree" =
- * one launch with device-scope completion flags),
  * "blockRows" (rows per ILU aggregate, <= 64, default 32; aggregates of one
  * colour are packed into blocks of up to 64 rows), "fallbackRatio" (learned: minimum
  * per-iteration residual reduction before falling back, 0.5).  Unknown keys

@@ -38,7 +38,6 @@ struct phase_hip_solver {
   double fallbackRatio = 0.5;
   int maxOuter = 100;
   int chunk = 8;  // iterations enqueued between host checks
-  int syncFree = 0;  // ILU sweeps: 0 = one launch per block colour, 1 = one sync-free launch
   int geometric = 1; // cell centroids known: brick aggregates in three colours (else graph aggregates)
   int cg = 0;        // Krylov method: 0 = BiCGStab, 1 = preconditioned CG (symmetric systems)
   int unionPattern = 0;  // keep the stored pattern when a new one lies inside it (its other entries 0)
@@ -52,7 +51,7 @@ struct phase_hip_solver {
   phase::DBuf<phase::BlockInfo> dBlocks;
   phase::DBuf<phase::RowInfo> dRows;
   phase::DBuf<phase::LaneRec> dLanes;
-  phase::DBuf<int> dPred, dSucc, dLCol, dUCol, dUpdPtr, dUpdSrc, dUpdDst;
+  phase::DBuf<int> dLCol, dUCol, dUpdPtr, dUpdSrc, dUpdDst;
   phase::DBuf<int> dSliceOff, dSellCol, dSellSrc, dFacSrc, dPerm;
 
   // values: caller slots (FP64), SpMV copies, ILU factors
@@ -68,10 +67,7 @@ struct phase_hip_solver {
   phase::KrylovVecs<float> k32;
 
   // synchronisation and reductions
-  phase::DBuf<int> dFlags, dErr;
-  phase::DBuf<unsigned long long> dTicket;
-  unsigned long long ticketBase = 0;
-  int epoch = 0;
+  phase::DBuf<int> dErr;  // bit 2: zero pivot in the factorisation
   phase::DBuf<double> dPartials;
   phase::DBuf<unsigned> dCounter;
   phase::DBuf<phase::Scalars> dSc;  // [0] inner run, [1] outer residual, [2] misc

@@ -57,8 +57,6 @@ IluDev<T> iluDev(phase_hip_solver *s, T *fac) {
   L.blocks = s->dBlocks.p;
   L.rows = s->dRows.p;
   L.lanes = s->dLanes.p;
-  L.pred = s->dPred.p;
-  L.succ = s->dSucc.p;
   L.lCol = s->dLCol.p;
   L.uCol = s->dUCol.p;
   L.fac = fac;
@@ -72,30 +70,6 @@ IluDev<T> iluDev(phase_hip_solver *s, T *fac) {
 // Scalars slot that is never set done: sweeps run outside a Krylov iteration
 const Scalars *idleScalars(phase_hip_solver *s) { return s->dSc.p + 3; }
 
-SyncDev syncDev(phase_hip_solver *s) { return SyncDev{s->dFlags.p, s->dTicket.p, s->dErr.p}; }
-
-// Sweep launchers: one launch per block colour (forward colours ascending,
-// backward descending), or one sync-free launch taking nb tickets and a
-// fresh epoch.
-SweepCtl syncFreeCtl(phase_hip_solver *s) {
-  SweepCtl c;
-  c.syncFree = 1;
-  c.blockBase = 0;
-  c.ticketBase = s->ticketBase;
-  s->ticketBase += (unsigned long long)s->ord.nBlocks;
-  c.epoch = ++s->epoch;
-  return c;
-}
-
-SweepCtl colourCtl(int base) {
-  SweepCtl c;
-  c.syncFree = 0;
-  c.blockBase = base;
-  c.ticketBase = 0;
-  c.epoch = 0;
-  return c;
-}
-
 // Calls f with the ordering's sweep register width as a compile-time constant.
 template <class F>
 void byWidth(const phase_hip_solver *s, F &&f) {
@@ -103,21 +77,18 @@ void byWidth(const phase_hip_solver *s, F &&f) {
   else f(std::integral_constant<int, 8>());
 }
 
+// Sweep launchers: one launch per block colour (forward colours ascending,
+// backward descending).
 template <class T, int PRO>
 void launchL(phase_hip_solver *s, const IluDev<T> &L, const Scalars *sc, const T *in, T *y, T *pv, const T *r,
              const T *v, double *xs = nullptr) {
   byWidth(s, [&](auto Wc) {
     constexpr int W = decltype(Wc)::value;
-    if (s->syncFree) {
-      hipLaunchKernelGGL((kLSolve<T, PRO, W>), dim3(s->ord.nBlocks), dim3(64), 0, s->stream, L, syncDev(s),
-                         syncFreeCtl(s), sc, in, y, pv, r, v, xs);
-      return;
-    }
     const std::vector<int> &cp = s->ord.colourPtr;
     for (int c = 0; c + 1 < (int)cp.size(); ++c)
       if (cp[c + 1] > cp[c])
-        hipLaunchKernelGGL((kLSolve<T, PRO, W>), dim3(cp[c + 1] - cp[c]), dim3(64), 0, s->stream, L, syncDev(s),
-                           colourCtl(cp[c]), sc, in, y, pv, r, v, xs);
+        hipLaunchKernelGGL((kLSolve<T, PRO, W>), dim3(cp[c + 1] - cp[c]), dim3(64), 0, s->stream, L, cp[c], sc, in,
+                           y, pv, r, v, xs);
   });
 }
 
@@ -125,59 +96,44 @@ template <class T>
 void launchU(phase_hip_solver *s, const IluDev<T> &L, const Scalars *sc, const T *y, T *x) {
   byWidth(s, [&](auto Wc) {
     constexpr int W = decltype(Wc)::value;
-    if (s->syncFree) {
-      hipLaunchKernelGGL((kUSolve<T, W>), dim3(s->ord.nBlocks), dim3(64), 0, s->stream, L, syncDev(s),
-                         syncFreeCtl(s), sc, y, x);
-      return;
-    }
     const std::vector<int> &cp = s->ord.colourPtr;
     for (int c = (int)cp.size() - 2; c >= 0; --c)
       if (cp[c + 1] > cp[c])
-        hipLaunchKernelGGL((kUSolve<T, W>), dim3(cp[c + 1] - cp[c]), dim3(64), 0, s->stream, L, syncDev(s),
-                           colourCtl(cp[c]), sc, y, x);
+        hipLaunchKernelGGL((kUSolve<T, W>), dim3(cp[c + 1] - cp[c]), dim3(64), 0, s->stream, L, cp[c], sc, y, x);
   });
 }
 
-// out = U^{-1} L^{-1} (right-hand side formed by PRO); per-colour mode fuses
-// the last forward colour with the first backward colour.
+// out = U^{-1} L^{-1} (right-hand side formed by PRO); the last forward colour
+// and the first backward colour run fused in one launch.
 template <class T, int PRO>
 void launchLU(phase_hip_solver *s, const IluDev<T> &L, const Scalars *sc, const T *in, T *y, T *pv, const T *r,
               const T *v, T *out, double *xs = nullptr) {
   const std::vector<int> &cp = s->ord.colourPtr;
   const int C = (int)cp.size() - 1;
-  if (s->syncFree || C < 1) {
-    launchL<T, PRO>(s, L, sc, in, y, pv, r, v, xs);
-    launchU<T>(s, L, sc, y, out);
-    return;
-  }
+  if (C < 1) return;
   byWidth(s, [&](auto Wc) {
     constexpr int W = decltype(Wc)::value;
     for (int c = 0; c + 1 < C; ++c)
       if (cp[c + 1] > cp[c])
-        hipLaunchKernelGGL((kLSolve<T, PRO, W>), dim3(cp[c + 1] - cp[c]), dim3(64), 0, s->stream, L, syncDev(s),
-                           colourCtl(cp[c]), sc, in, y, pv, r, v, xs);
+        hipLaunchKernelGGL((kLSolve<T, PRO, W>), dim3(cp[c + 1] - cp[c]), dim3(64), 0, s->stream, L, cp[c], sc, in,
+                           y, pv, r, v, xs);
     if (cp[C] > cp[C - 1])
-      hipLaunchKernelGGL((kLUSolve<T, PRO, W>), dim3(cp[C] - cp[C - 1]), dim3(64), 0, s->stream, L, syncDev(s),
-                         colourCtl(cp[C - 1]), sc, in, (const T *)y, pv, r, v, out, xs);
+      hipLaunchKernelGGL((kLUSolve<T, PRO, W>), dim3(cp[C] - cp[C - 1]), dim3(64), 0, s->stream, L, cp[C - 1], sc,
+                         in, (const T *)y, pv, r, v, out, xs);
     for (int c = C - 2; c >= 0; --c)
       if (cp[c + 1] > cp[c])
-        hipLaunchKernelGGL((kUSolve<T, W>), dim3(cp[c + 1] - cp[c]), dim3(64), 0, s->stream, L, syncDev(s),
-                           colourCtl(cp[c]), sc, (const T *)y, out);
+        hipLaunchKernelGGL((kUSolve<T, W>), dim3(cp[c + 1] - cp[c]), dim3(64), 0, s->stream, L, cp[c], sc,
+                           (const T *)y, out);
   });
 }
 
 template <class T>
 void launchFactor(phase_hip_solver *s, const IluDev<T> &L) {
-  if (s->syncFree) {
-    hipLaunchKernelGGL((kIluFactor<T>), dim3(s->ord.nBlocks), dim3(64), 0, s->stream, L, s->dUpdPtr.p, s->dUpdSrc.p,
-                       s->dUpdDst.p, syncDev(s), syncFreeCtl(s));
-    return;
-  }
   const std::vector<int> &cp = s->ord.colourPtr;
   for (int c = 0; c + 1 < (int)cp.size(); ++c)
     if (cp[c + 1] > cp[c])
       hipLaunchKernelGGL((kIluFactor<T>), dim3(cp[c + 1] - cp[c]), dim3(64), 0, s->stream, L, s->dUpdPtr.p,
-                         s->dUpdSrc.p, s->dUpdDst.p, syncDev(s), colourCtl(cp[c]));
+                         s->dUpdSrc.p, s->dUpdDst.p, s->dErr.p, cp[c]);
 }
 
 int spmvGrid(phase_hip_solver *s) { return std::max(1, (s->ord.nSlices + 3) / 4); }
@@ -187,10 +143,6 @@ void checkDeviceErr(phase_hip_solver *s) {
   int e = 0;
   PH_CHECK(hipMemcpyAsync(&e, s->dErr.p, sizeof(int), hipMemcpyDeviceToHost, s->stream));
   PH_CHECK(hipStreamSynchronize(s->stream));
-  if (e & 1) {
-    PH_CHECK(hipMemsetAsync(s->dErr.p, 0, sizeof(int), s->stream));
-    fail(PHASE_HIP_EDEVICE, "ILU sweep dependency wait timed out");
-  }
   if (e & 2) {
     PH_CHECK(hipMemsetAsync(s->dErr.p, 0, sizeof(int), s->stream));
     fail(PHASE_HIP_EBREAKDOWN, "zero pivot in ILU(0) factorisation");
@@ -549,12 +501,6 @@ void allocCommon(phase_hip_solver *s) {
   const int n = s->rank;
   s->dB.alloc(n);
   s->dX.alloc(n);
-  s->dFlags.alloc(std::max(1, s->ord.nBlocks));
-  s->dFlags.zero(s->stream);
-  s->epoch = 0;
-  s->dTicket.alloc(1);
-  s->dTicket.zero(s->stream);
-  s->ticketBase = 0;
   s->dErr.alloc(1);
   s->dErr.zero(s->stream);
   const int maxGrid = std::max(spmvGrid(s), kRedCap);
@@ -581,8 +527,6 @@ void solverSetPattern(phase_hip_solver *s, int n, const int *rowPtr, const int *
   s->dBlocks.upload(o.blocks, st);
   s->dRows.upload(o.rows, st);
   s->dLanes.upload(o.lanes, st);
-  s->dPred.upload(o.pred.empty() ? std::vector<int>{0} : o.pred, st);
-  s->dSucc.upload(o.succ.empty() ? std::vector<int>{0} : o.succ, st);
   s->dLCol.upload(o.lCol.empty() ? std::vector<int>{-1} : o.lCol, st);
   s->dUCol.upload(o.uCol.empty() ? std::vector<int>{-1} : o.uCol, st);
   s->dUpdPtr.upload(o.updPtr, st);
@@ -888,11 +832,8 @@ int phase_hip_solver_setup(phase_hip_solver *s, const char *key, const char *val
       if (lv == "exact") s->unionPattern = 0;
       else if (lv == "union") s->unionPattern = 1;
       else fail(PHASE_HIP_EINVAL, "unknown pattern policy '" + v + "' (exact or union)");
-    } else if (k == "sweep") {
-      if (lv == "colour" || lv == "color" || lv == "0") s->syncFree = 0;
-      else if (lv == "syncfree" || lv == "1") s->syncFree = 1;
-      else fail(PHASE_HIP_EINVAL, "unknown sweep '" + v + "'");
     }
+    // other keys are ignored, as the reference's setup(ptree) reads only the keys it knows
   });
 }
 
@@ -1317,20 +1258,20 @@ int phase_hip_solver_time_kernels(phase_hip_solver *s, int reps, double *ms, int
           o3[0] = timeIt([&] {
                     for (int c = C - 2; c >= 0; --c)
                       hipLaunchKernelGGL((kUSolve<T, W>), dim3(cp[c + 1] - cp[c]), dim3(64), 0, s->stream, L,
-                                         syncDev(s), colourCtl(cp[c]), noSc, (const T *)y, x);
+                                         cp[c], noSc, (const T *)y, x);
                   }) / (C - 1);
           o3[2] = timeIt([&] {
                     for (int c = 0; c + 1 < C; ++c)
                       hipLaunchKernelGGL((kLSolve<T, kProNone, W>), dim3(cp[c + 1] - cp[c]), dim3(64), 0,
-                                         s->stream, L, syncDev(s), colourCtl(cp[c]), noSc, in, y, (T *)nullptr,
-                                         (const T *)nullptr, (const T *)nullptr, (double *)nullptr);
+                                         s->stream, L, cp[c], noSc, in, y, (T *)nullptr, (const T *)nullptr,
+                                         (const T *)nullptr, (double *)nullptr);
                   }) / (C - 1);
         }
         if (C >= 1)
           o3[1] = timeIt([&] {
             hipLaunchKernelGGL((kLUSolve<T, kProNone, W>), dim3(cp[C] - cp[C - 1]), dim3(64), 0, s->stream, L,
-                               syncDev(s), colourCtl(cp[C - 1]), noSc, in, (const T *)y, (T *)nullptr,
-                               (const T *)nullptr, (const T *)nullptr, x, (double *)nullptr);
+                               cp[C - 1], noSc, in, (const T *)y, (T *)nullptr, (const T *)nullptr,
+                               (const T *)nullptr, x, (double *)nullptr);
           });
       });
     };

@@ -44,47 +44,10 @@ struct IluDev {
   const BlockInfo *blocks;
   const RowInfo *rows;
   const LaneRec *lanes;
-  const int *pred, *succ, *lCol, *uCol;
+  const int *lCol, *uCol;
   T *fac;
   int nb, nLSlots, nUSlots, n;
 };
-
-struct SyncDev {
-  int *flags;
-  unsigned long long *ticket;
-  int *err;
-};
-
-__device__ __forceinline__ int ldRelaxed(const int *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void stRelease(int *p, int v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Bounded wait for a block's completion flag: every wait ends (after about a
-// second at worst, flagging an error) so a wrong dependency can never hang.
-// The poll is a relaxed device-coherent load; callers follow the wait with one
-// agent-scope acquire fence, so the cache invalidation happens once per block
-// rather than once per poll.
-__device__ __forceinline__ void waitFlag(const int *f, int epoch, int *err) {
-  for (int it = 0; it < (1 << 24); ++it) {
-    if (ldRelaxed(f) >= epoch) return;
-    __builtin_amdgcn_s_sleep(1);
-  }
-  atomicOr(err, 1);
-}
-
-// Take the next block of a sweep.  Tickets are handed out in launch order, so
-// every block a ticket holder waits on belongs to an earlier ticket holder that
-// is already running: the sweep cannot deadlock, whatever the occupancy.
-__device__ __forceinline__ long long takeTicket(unsigned long long *ticket,
-                                                unsigned long long base) {
-  __shared__ unsigned long long sTicket;
-  if (threadIdx.x == 0) sTicket = atomicAdd(ticket, 1ULL);
-  __syncthreads();
-  return (long long)(sTicket - base);
-}
 
 __device__ __forceinline__ double waveSum(double v) {
 #pragma unroll
@@ -652,56 +615,23 @@ __global__ void kNormalise(int n, const double *__restrict__ r64, T *__restrict_
 }
 
 // ---------------------------------------------------------------- ILU(0)
-// How a sweep launch maps workgroups to ILU blocks.
-//  * per colour (syncFree == 0): one launch per block colour; blocks of one
-//    colour share no coupling, so block = blockBase + xcdTile(blockIdx.x)
-//    needs no waiting -- kernel boundaries order the colours.
-//  * sync-free (syncFree == 1): one launch for the whole sweep; blocks take
-//    tickets in order and wait on completion flags of their predecessor
-//    blocks (device-scope release/acquire).
-struct SweepCtl {
-  int syncFree;
-  int blockBase;
-  unsigned long long ticketBase;
-  int epoch;
-};
-
-// Block of this workgroup, or -1 (sync-free tickets beyond the sweep).
-__device__ __forceinline__ int sweepBlock(const SweepCtl &c, const SyncDev &sy, int nb, bool reverse) {
-  if (!c.syncFree) return c.blockBase + xcdTile(blockIdx.x, gridDim.x);
-  const long long bi = takeTicket(sy.ticket, c.ticketBase);
-  if (bi < 0 || bi >= nb) return -1;
-  return reverse ? nb - 1 - (int)bi : (int)bi;
-}
-
-__device__ __forceinline__ void sweepWait(const SweepCtl &c, const SyncDev &sy, const int *list, int beg, int end) {
-  if (c.syncFree) {
-    if ((int)threadIdx.x < end - beg) waitFlag(&sy.flags[list[beg + threadIdx.x]], c.epoch, sy.err);
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    __syncthreads();
-  }
-}
-
-__device__ __forceinline__ void sweepPublish(const SweepCtl &c, const SyncDev &sy, int b) {
-  if (c.syncFree) {
-    __syncthreads();
-    if (threadIdx.x == 0) stRelease(&sy.flags[b], c.epoch);
-  }
-}
+// A sweep or factorisation launch covers the blocks of one colour: blocks of
+// one colour share no coupling, so block = blockBase + xcdTile(blockIdx.x)
+// needs no waiting -- kernel boundaries order the colours.
+__device__ __forceinline__ int colourBlock(int blockBase) { return blockBase + xcdTile(blockIdx.x, gridDim.x); }
 
 // Factorisation, IKJ order per row (the oracle's orc_ilu0): for each L entry
 // (i,k) in ascending k: l = a_ik / u_kk, then a_ij -= l * u_kj over U(k) in
-// ascending j.  Rows of earlier blocks are final (earlier colour launch, or
-// flag set); rows of the same block are final after their level.
+// ascending j.  Rows of earlier blocks are final (earlier colour launch); rows
+// of the same block are final after their level.  A zero pivot sets bit 2 of
+// *err.
 template <class T>
 __global__ __launch_bounds__(64) void kIluFactor(IluDev<T> L, const int *__restrict__ updPtr,
                                                  const int *__restrict__ updSrc,
-                                                 const int *__restrict__ updDst, SyncDev sy, SweepCtl ctl) {
-  const int b = sweepBlock(ctl, sy, L.nb, false);
-  if (b < 0) return;
+                                                 const int *__restrict__ updDst, int *err, int blockBase) {
+  const int b = colourBlock(blockBase);
   const BlockInfo B = L.blocks[b];
   const int lane = threadIdx.x;
-  sweepWait(ctl, sy, L.pred, B.predBeg, B.predEnd);
   const bool act = lane < B.nRows;
   const int row = B.rowStart + lane;
   const RowInfo ri = act ? L.rows[row] : RowInfo{};
@@ -719,11 +649,10 @@ __global__ __launch_bounds__(64) void kIluFactor(IluDev<T> L, const int *__restr
           fac[dst] = fma(-lik, fac[updSrc[u]], fac[dst]);
         }
       }
-      if (fac[dbase + row] == (T)0) atomicOr(sy.err, 2);
+      if (fac[dbase + row] == (T)0) atomicOr(err, 2);
     }
     __syncthreads();
   }
-  sweepPublish(ctl, sy, b);
 }
 
 // A sweep launch is latency-bound: every block runs one wavefront and a
@@ -749,8 +678,7 @@ __global__ void kInvDiag(int n, T *fac, int dbase) {
 // Accumulation order (earlier columns first) is that of the oracle's forward
 // substitution, so the result is bit-identical to it.
 template <class T, int PRO, int W>
-__device__ __forceinline__ T lBody(const IluDev<T> &L, const SyncDev &sy, const SweepCtl &ctl, const BlockInfo &B,
-                                   bool act, int row, unsigned info, int lane, const Scalars *sc,
+__device__ __forceinline__ T lBody(const IluDev<T> &L, const BlockInfo &B, bool act, int row, unsigned info, int lane, const Scalars *sc,
                                    const T *__restrict__ in, const T *__restrict__ y, const T *__restrict__ pv,
                                    const T *__restrict__ r, const T *__restrict__ v, T *ylds, T &pro,
                                    const double *__restrict__ xs, double &xo) {
@@ -806,7 +734,6 @@ __device__ __forceinline__ T lBody(const IluDev<T> &L, const SyncDev &sy, const 
   // keep every load above in flight before the gathers below are addressed
   // (otherwise each column load is waited for in turn)
   asm volatile("" ::: "memory");
-  sweepWait(ctl, sy, L.pred, B.predBeg, B.predEnd);
   T oy[W];
 #pragma unroll
   for (int q = 0; q < W; ++q) oy[q] = q < nOff ? y[oc[q]] : (T)0;
@@ -877,10 +804,9 @@ __device__ __forceinline__ void uLoad(const IluDev<T> &L, const BlockInfo &B, bo
 }
 
 template <class T, int W>
-__device__ __forceinline__ T uSolve(const IluDev<T> &L, const SyncDev &sy, const SweepCtl &ctl, const BlockInfo &B,
-                                    bool act, int lane, T acc, const T *__restrict__ x, T *xlds, const UPre<T, W> &u) {
+__device__ __forceinline__ T uSolve(const IluDev<T> &L, const BlockInfo &B, bool act, int lane, T acc,
+                                    const T *__restrict__ x, T *xlds, const UPre<T, W> &u) {
   asm volatile("" ::: "memory");
-  sweepWait(ctl, sy, L.succ, B.succBeg, B.succEnd);
   T ox[W];
 #pragma unroll
   for (int q = 0; q < W; ++q) ox[q] = q < u.nOff ? x[u.oc[q]] : (T)0;
@@ -907,13 +833,12 @@ __device__ __forceinline__ T uSolve(const IluDev<T> &L, const SyncDev &sy, const
 // Forward sweep y = L^{-1} b (unit L).  After convergence (sc->done) every
 // lane is idle and the launch only walks the block's levels.
 template <class T, int PRO, int W = 8>
-__global__ __launch_bounds__(64) void kLSolve(IluDev<T> L, SyncDev sy, SweepCtl ctl, const Scalars *sc,
+__global__ __launch_bounds__(64) void kLSolve(IluDev<T> L, int blockBase, const Scalars *sc,
                                               const T *__restrict__ in, T *__restrict__ y, T *__restrict__ pv,
                                               const T *__restrict__ r, const T *__restrict__ v,
                                               double *__restrict__ xs) {
   __shared__ T ylds[64];
-  const int b = sweepBlock(ctl, sy, L.nb, false);
-  if (b < 0) return;
+  const int b = colourBlock(blockBase);
   const int lane = threadIdx.x;
   const int done = sc->done;
   const BlockInfo B = L.blocks[b];
@@ -921,23 +846,20 @@ __global__ __launch_bounds__(64) void kLSolve(IluDev<T> L, SyncDev sy, SweepCtl 
   const bool act = rec.row >= 0 && !done;
   T pro;
   double xo = 0.;
-  const T acc = lBody<T, PRO, W>(L, sy, ctl, B, act, rec.row, rec.info, lane, sc, in, y, pv, r, v, ylds, pro,
-                                 xs, xo);
+  const T acc = lBody<T, PRO, W>(L, B, act, rec.row, rec.info, lane, sc, in, y, pv, r, v, ylds, pro, xs, xo);
   if (act) {
     if (PRO != kProNone) pv[rec.row] = pro;
     if (PRO == kProCgR) xs[rec.row] = xo;
     y[rec.row] = acc;
   }
-  sweepPublish(ctl, sy, b);
 }
 
 // Backward sweep x = U^{-1} y (colours in reverse order).
 template <class T, int W = 8>
-__global__ __launch_bounds__(64) void kUSolve(IluDev<T> L, SyncDev sy, SweepCtl ctl, const Scalars *sc,
+__global__ __launch_bounds__(64) void kUSolve(IluDev<T> L, int blockBase, const Scalars *sc,
                                               const T *__restrict__ y, T *__restrict__ x) {
   __shared__ T xlds[64];
-  const int b = sweepBlock(ctl, sy, L.nb, true);
-  if (b < 0) return;
+  const int b = colourBlock(blockBase);
   const int lane = threadIdx.x;
   const int done = sc->done;
   const BlockInfo B = L.blocks[b];
@@ -946,25 +868,24 @@ __global__ __launch_bounds__(64) void kUSolve(IluDev<T> L, SyncDev sy, SweepCtl 
   const T y0 = y[act ? rec.row : 0];
   UPre<T, W> u;
   uLoad<T, W>(L, B, act, rec.row, rec.info, lane, u);
-  const T acc = uSolve<T, W>(L, sy, ctl, B, act, lane, act ? y0 : (T)0, x, xlds, u);
+  const T acc = uSolve<T, W>(L, B, act, lane, act ? y0 : (T)0, x, xlds, u);
   if (act) x[rec.row] = acc;
-  sweepPublish(ctl, sy, b);
 }
 
-// Last colour of a preconditioner apply (per-colour launches only): blocks of
+// Last colour of a preconditioner apply: blocks of
 // the last colour have no later-coloured neighbours, so the forward and the
 // backward sweep of such a block run back to back in one launch.  The U
 // entries load after the forward levels: issued up front they overlap the
 // forward sweep's loads but double the registers, and the lower occupancy
 // costs more (a second round of blocks per launch) than the round trip.
 template <class T, int PRO, int W = 8>
-__global__ __launch_bounds__(64) void kLUSolve(IluDev<T> L, SyncDev sy, SweepCtl ctl, const Scalars *sc,
+__global__ __launch_bounds__(64) void kLUSolve(IluDev<T> L, int blockBase, const Scalars *sc,
                                                const T *__restrict__ in, const T *__restrict__ y,
                                                T *__restrict__ pv, const T *__restrict__ r,
                                                const T *__restrict__ v, T *__restrict__ x, double *__restrict__ xs) {
   __shared__ T ylds[64];
   __shared__ T xlds[64];
-  const int b = ctl.blockBase + xcdTile(blockIdx.x, gridDim.x);
+  const int b = colourBlock(blockBase);
   const int lane = threadIdx.x;
   const int done = sc->done;
   const BlockInfo B = L.blocks[b];
@@ -972,11 +893,10 @@ __global__ __launch_bounds__(64) void kLUSolve(IluDev<T> L, SyncDev sy, SweepCtl
   const bool act = rec.row >= 0 && !done;
   T pro;
   double xo = 0.;
-  const T yv = lBody<T, PRO, W>(L, sy, ctl, B, act, rec.row, rec.info, lane, sc, in, y, pv, r, v, ylds, pro, xs,
-                                xo);
+  const T yv = lBody<T, PRO, W>(L, B, act, rec.row, rec.info, lane, sc, in, y, pv, r, v, ylds, pro, xs, xo);
   UPre<T, W> u;
   uLoad<T, W>(L, B, act, rec.row, rec.info, lane, u);
-  const T acc = uSolve<T, W>(L, sy, ctl, B, act, lane, yv, x, xlds, u);
+  const T acc = uSolve<T, W>(L, B, act, lane, yv, x, xlds, u);
   if (act) {
     if (PRO != kProNone) pv[rec.row] = pro;
     if (PRO == kProCgR) xs[rec.row] = xo;
