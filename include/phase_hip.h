@@ -425,7 +425,11 @@ int phase_hip_eq_gcib_motion_bcs(phase_hip_equation *e, const phase_hip_gcib *ib
  * GhostCellImmersedBoundary.cpp:94-146: per body, pressure (bpValue(p) +
  * rho g.bp) and wall shear (mu (grad u(bp) . nw) . nw.tangentVec()) at the
  * boundary points, sorted by angle and integrated edge by edge; uCell
- * (interleaved) and pCell are device fields, force (2 per body) host. */
+ * (interleaved) and pCell are device fields, force (2 per body) host.
+ * Deliberate difference: each body integrates its own boundary points only.
+ * The reference keeps one stress list across its body loop (:100-108), so its
+ * force on body k also integrates the points of bodies 0..k-1; with one body
+ * the two agree. */
 int phase_hip_gcib_hydrodynamic_force(phase_hip_gcib *ib, double rho, double mu, const double *uCell,
                                       const double *pCell, double gx, double gy, double *force);
 
@@ -458,6 +462,11 @@ int phase_hip_fstep_fields(phase_hip_fstep *s, double *uCell, double *pCell, dou
  * divergence on the fluid cells only. */
 int phase_hip_fstep_immersed_boundary(phase_hip_fstep *s, const phase_hip_gcib *ib, const double *objectVelocity,
                                       int pBcType, double pRefValue);
+/* The same for moving bodies, 3 numbers each (vel.x, vel.y, omega): the
+ * velocity rows are velocityBcs(u) with Motion::velocity (Motion.h:31-33), as
+ * phase_hip_eq_gcib_motion_bcs builds them. */
+int phase_hip_fstep_immersed_boundary_motion(phase_hip_fstep *s, const phase_hip_gcib *ib, const double *objectMotion,
+                                             int pBcType, double pRefValue);
 
 /* ------------------------------------------------------------------ multiphase fractional step
  * FractionalStepMultiphase::solve(timeStep) -- src/2D/Unstructured/Solvers/

@@ -45,7 +45,7 @@ def _csr(rp, ci, v):
 class FractionalStepOracle:
     def __init__(self, mesh, rho, mu, u_bc, u_face0, p_bc=None, p_face0=None, pin=0, immersed=None, plant=None):
         """immersed: None or (GhostCellOracle, body velocities, pressure bc type
-        on the bodies, its reference value)."""
+        on the bodies, its reference value[, body angular velocities])."""
         self.m, self.rho, self.mu, self.pin = mesh, rho, mu, pin
         self.ib = immersed
         self.plant = plant
@@ -77,7 +77,8 @@ class FractionalStepOracle:
         m, N, F = self.m, self.m.n_cells, self.m.n_faces
         ib = self.ib
         if ib is not None:
-            g, vel, pbc_ib, pref = ib
+            g, vel, pbc_ib, pref = ib[:4]
+            omegas = ib[4] if len(ib) > 4 else None
             other = (g.types != 0).astype(np.int32)
             fluid = g.types == 0
         u0C, u0F = self.uC.copy(), self.uF.copy()
@@ -86,7 +87,7 @@ class FractionalStepOracle:
                    phi_face=self.uF.reshape(-1), phi0_cell=u0C.reshape(-1), phi0_face=u0F.reshape(-1), ncomp=2)
         if ib is not None:
             E.remove_cells(N, other)
-            E += g.velocity_bcs(vel)
+            E += g.velocity_bcs(vel, omegas=omegas)
         theta = 1.0 if self.plant == "theta1" else 0.5
         L = O.laplacian(m, 0, ncomp=2, gamma=self.mu / self.rho, theta=theta, bc_type=self.ubc,
                         phi_face=self.uF.reshape(-1), phi0_cell=u0C.reshape(-1), phi0_face=u0F.reshape(-1))

@@ -44,7 +44,12 @@ rho dot(bp, g), and the wall shear mu dot(dot(grad u(bp), nw), nw.tangentVec())
 with grad u from the bilinear fit over the four cells of the node nearest to
 bp (BcStencil::bpGrad); the points sorted by their angle about the centre;
 force = sum over consecutive pairs of -(p_a + p_b)/2 (x_b - x_a).normalVec()
-+ (s_a + s_b)/2 (x_b - x_a).
++ (s_a + s_b)/2 (x_b - x_a).  Each body integrates its own points only: the
+reference keeps one stress list across the body loop (declared before it,
+never cleared, :100-108), so with two or more bodies its force on body k also
+integrates the points of bodies 0..k-1.  That is a reference bug, fixed here
+and on the device on purpose; the two-body closed form in
+tests/test_oracle_ghostcell.py pins the per-body force.
 """
 import numpy as np
 

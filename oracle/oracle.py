@@ -25,6 +25,14 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
+def use_library(path):
+    """Load the restatement from `path` instead (the sanitizer build); call
+    before the first oracle function."""
+    global _LIB_PATH, _lib
+    _LIB_PATH = path
+    _lib = None
+
+
 def _load():
     if not os.path.exists(_LIB_PATH):
         build()

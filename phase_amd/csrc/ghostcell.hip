@@ -45,6 +45,7 @@ struct phase_hip_gcib {
   std::vector<int> kind;      // per body: PHASE_HIP_SHAPE_CIRCLE or _BOX
   std::vector<double> geom;   // 4 per body: cx, cy, r, 0 or lower x, lower y, upper x, upper y
   std::vector<GhostPoint> points;
+  std::vector<double> cc;     // cell centroids (2 per cell), host copy taken at creation
   DBuf<int> dGather;     // 9 cells per ghost point: cell, ipCells, bpCells
   DBuf<double> dValues;  // p, u.x, u.y at those cells
   DBuf<int> dTypes, dStencil, dFixCols, dNgCols;  // per cell: stencil index or -1; 5 columns per stencil
@@ -274,7 +275,9 @@ int phase_hip_gcib_create_shapes(phase_hip_gcib **out, phase_hip_mesh *m, int nB
     ib->mesh = m;
     ib->N = N;
     ib->nObj = nCircles;
-    const std::vector<double> cc = fetch(m->dCC, 2 * (size_t)N, st), xy = fetch(m->dXY, 2 * (size_t)m->nNodes, st);
+    ib->cc = fetch(m->dCC, 2 * (size_t)N, st);
+    const std::vector<double> &cc = ib->cc;
+    const std::vector<double> xy = fetch(m->dXY, 2 * (size_t)m->nNodes, st);
     const std::vector<int> cptr = fetch(m->dCptr, N + 1, st);
     const std::vector<int> cind = fetch(m->dCind, cptr[N], st);
     std::vector<std::vector<int>> nodeCells(m->nNodes);
@@ -633,17 +636,7 @@ int phase_hip_eq_gcib_motion_bcs(phase_hip_equation *e, const phase_hip_gcib *ib
     bpOf[2 * g.cell] = g.bp[0];
     bpOf[2 * g.cell + 1] = g.bp[1];
   }
-  std::vector<double> cc;
-  {
-    phase_hip_mesh *m = ib->mesh;
-    cc.resize(2 * (size_t)ib->N);
-    try {
-      PH_CHECK(hipMemcpy(cc.data(), m->dCC.p, cc.size() * sizeof(double), hipMemcpyDeviceToHost));
-    } catch (const HipError &x) {
-      e->err = x.msg;
-      return x.code;
-    }
-  }
+  const std::vector<double> &cc = ib->cc;
   std::vector<double> src(2 * (size_t)ib->N, 0.);
   for (int c = 0; c < ib->N; ++c)
     if (ib->types[c] != 0) {

@@ -558,15 +558,21 @@ int phase_hip_fstep_setup(phase_hip_fstep *s, const char *key, const char *value
   });
 }
 
-int phase_hip_fstep_immersed_boundary(phase_hip_fstep *s, const phase_hip_gcib *ib, const double *objectVelocity,
-                                      int pBcType, double pRefValue) {
+}  // extern "C"
+
+namespace {
+// FractionalStepGCIB from the next step on: the velocity rows come from
+// `velocityRows` (translating or moving bodies), the pressure rows from bcs(p).
+template <class VelocityRows>
+int attachImmersedBoundary(phase_hip_fstep *s, const phase_hip_gcib *ib, VelocityRows &&velocityRows, int pBcType,
+                           double pRefValue) {
   return guardedF(s, [&] {
     if (!ib) fail(PHASE_HIP_EINVAL, "null immersed boundary");
     phase_hip_mesh *m = s->mesh;
     PH_CHECK(hipSetDevice(m->device));
     if (!s->ibU) ck(phase_hip_equation_create(&s->ibU, m, 2), "velocityBcs", "");
     if (!s->ibP) ck(phase_hip_equation_create(&s->ibP, m, 1), "bcs", "");
-    ck(phase_hip_eq_gcib_velocity_bcs(s->ibU, ib, objectVelocity), "velocityBcs", s->ibU->err.c_str());
+    ck(velocityRows(s->ibU), "velocityBcs", s->ibU->err.c_str());
     ck(phase_hip_eq_gcib_bcs(s->ibP, ib, pBcType, pRefValue), "bcs", s->ibP->err.c_str());
     s->ibTypes.alloc(s->N);
     ck(phase_hip_gcib_cell_types(ib, s->ibTypes.p, nullptr, 1), "cell types", "");
@@ -574,6 +580,23 @@ int phase_hip_fstep_immersed_boundary(phase_hip_fstep *s, const phase_hip_gcib *
     PH_CHECK(hipStreamSynchronize(m->stream));
     s->immersed = true;
   });
+}
+}  // namespace
+
+extern "C" {
+
+int phase_hip_fstep_immersed_boundary(phase_hip_fstep *s, const phase_hip_gcib *ib, const double *objectVelocity,
+                                      int pBcType, double pRefValue) {
+  return attachImmersedBoundary(
+      s, ib, [&](phase_hip_equation *e) { return phase_hip_eq_gcib_velocity_bcs(e, ib, objectVelocity); }, pBcType,
+      pRefValue);
+}
+
+int phase_hip_fstep_immersed_boundary_motion(phase_hip_fstep *s, const phase_hip_gcib *ib, const double *objectMotion,
+                                             int pBcType, double pRefValue) {
+  return attachImmersedBoundary(
+      s, ib, [&](phase_hip_equation *e) { return phase_hip_eq_gcib_motion_bcs(e, ib, objectMotion); }, pBcType,
+      pRefValue);
 }
 
 int phase_hip_fstep_advance(phase_hip_fstep *s, double dt, int nSteps, double *history) {

@@ -482,12 +482,14 @@ void solveMixed(phase_hip_solver *s) {
     if (diverged || it == 0) break;
   }
   if (diverged || !(s->error <= s->tol)) {
-    if ((diverged || total >= s->maxIters) && bestRR < HUGE_VAL) {
-      // restore the best iterate; finish in FP64 if the budget allows
+    // not converged in FP32: restore the best iterate and, while the budget
+    // allows, finish in FP64 from it (divergence, a breakdown right after a
+    // restart, or the refinement-pass limit all end up here)
+    if (bestRR < HUGE_VAL && (diverged || bestRR < s->error * s->error * b2)) {
       PH_CHECK(hipMemcpyAsync(s->dX.p, s->dXBest.p, n * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
       s->error = std::sqrt(bestRR / b2);
     }
-    if (diverged) {
+    if (total < s->maxIters) {
       ++s->fallbacks;
       fp64Loop(s, b2, total);
       return;

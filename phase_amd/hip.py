@@ -118,6 +118,7 @@ _SIGS = {
     "phase_hip_fstep_setup": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_char_p]),
     "phase_hip_fstep_advance": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, _vp]),
     "phase_hip_fstep_fields": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    "phase_hip_fstep_immersed_boundary_motion": (ctypes.c_int, [_vp, _vp, _f64p, ctypes.c_int, ctypes.c_double]),
     "phase_hip_fstep_immersed_boundary": (ctypes.c_int, [_vp, _vp, _f64p, ctypes.c_int, ctypes.c_double]),
     "phase_hip_gcib_create": (ctypes.c_int, [ctypes.POINTER(_vp), _vp, ctypes.c_int, _f64p]),
     "phase_hip_gcib_create_shapes": (ctypes.c_int, [ctypes.POINTER(_vp), _vp, ctypes.c_int, _i32p, _f64p]),
@@ -514,13 +515,19 @@ class HipFractionalStep:
     def setup(self, key, value):
         self._check(load_library().phase_hip_fstep_setup(self._h, key.encode(), str(value).encode()), "setup")
 
-    def immersed_boundary(self, ib, velocities, p_bc, p_ref=0.0):
+    def immersed_boundary(self, ib, velocities, p_bc, p_ref=0.0, omegas=None):
         """FractionalStepGCIB: the ghost-cell rows of `ib` (HipGhostCellIB) in
-        both equations from the next step on."""
+        both equations from the next step on; `omegas` (one per body) makes
+        the bodies rotate about their centroids."""
         self._ib = ib  # the driver keeps using its cell types and rows
-        v = _f64(np.asarray(velocities, float).reshape(-1))
-        self._check(load_library().phase_hip_fstep_immersed_boundary(self._h, ib._h, v, int(p_bc), float(p_ref)),
-                    "immersedBoundary")
+        vel = np.asarray(velocities, float).reshape(-1, 2)
+        if omegas is None:
+            self._check(load_library().phase_hip_fstep_immersed_boundary(self._h, ib._h, _f64(vel.reshape(-1)),
+                                                                         int(p_bc), float(p_ref)), "immersedBoundary")
+        else:
+            motion = _f64(np.c_[vel, np.asarray(omegas, float).reshape(-1)].reshape(-1))
+            self._check(load_library().phase_hip_fstep_immersed_boundary_motion(self._h, ib._h, motion, int(p_bc),
+                                                                                float(p_ref)), "immersedBoundary")
 
     def advance(self, dt, n=1):
         """n time steps; returns an (n, 3) history: momentum and pressure

@@ -99,12 +99,14 @@ def test_invalid_inputs(hip, case):
         hip.HipGhostCellIB(hip.HipMesh(xy, cptr, cind), [(0.5, 0.5, 0.2)])
 
 
-def test_fractional_step_past_a_cylinder(oracle, hip):
+@pytest.mark.parametrize("omega", [None, 2.0])
+def test_fractional_step_past_a_cylinder(oracle, hip, omega):
     """FractionalStepGCIB on the device: a channel with inflow u = 1, a fixed
     cylinder of radius 0.25 (ghost-cell velocity and NORMAL_GRADIENT pressure
     rows), outlet p = 0; four steps against the restatement with direct solves:
-    velocities and pressure within 1e-7, solid cells at rest, fluid faces
-    divergence-free."""
+    velocities and pressure within 1e-7, solid cells at the body's velocity
+    (at rest, or spinning at omega about its centre: Motion::velocity rows in
+    the time step), fluid faces divergence-free."""
     from oracle.fstep_oracle import FractionalStepOracle
     from oracle.ghostcell_oracle import GhostCellOracle
     xy, cptr, cind = G.rectilinear(48, 32, width=3.0, height=2.0)
@@ -121,15 +123,19 @@ def test_fractional_step_past_a_cylinder(oracle, hip):
     pbc = np.where(right, oracle.BC_FIXED, oracle.BC_NORMAL_GRADIENT).astype(np.int32)
     circles = [(1.0, 1.0, 0.25)]
     og = GhostCellOracle(om, xy, cptr, cind, circles)
+    omegas = None if omega is None else [omega]
     ref = FractionalStepOracle(om, 1.0, 0.01, ubc, ub, p_bc=pbc, pin=-1,
-                               immersed=(og, [(0.0, 0.0)], oracle.BC_NORMAL_GRADIENT, 0.0))
+                               immersed=(og, [(0.0, 0.0)], oracle.BC_NORMAL_GRADIENT, 0.0, omegas))
     dg = hip.HipGhostCellIB(hm, circles)
     dev = hip.HipFractionalStep(hm, 1.0, 0.01, ubc, ub, p_bc=pbc, pin_cell=-1)
-    dev.immersed_boundary(dg, [(0.0, 0.0)], hip.BC_NORMAL_GRADIENT, 0.0)
+    dev.immersed_boundary(dg, [(0.0, 0.0)], hip.BC_NORMAL_GRADIENT, 0.0, omegas=omegas)
     for k in ("u", "p"):
         dev.setup(k + ".tolerance", 1e-12)
         dev.setup(k + ".maxIters", 20000)
     solid = og.types == 2
+    _, cc = om.cells()
+    w = 0.0 if omega is None else omega
+    u_body = np.c_[-(cc[:, 1] - 1.0) * w, (cc[:, 0] - 1.0) * w]
     for step in range(4):
         ref.step(0.01)
         hist = dev.advance(0.01)
@@ -139,7 +145,7 @@ def test_fractional_step_past_a_cylinder(oracle, hip):
         assert np.abs(u - ref.uC).max() <= 1e-7 * su
         assert np.abs(uf - ref.uF).max() <= 1e-7 * su
         assert np.abs(p - ref.pC).max() <= 1e-7 * sp_
-        assert np.abs(u[solid]).max() <= 1e-9 * su
+        assert np.abs(u[solid] - u_body[solid]).max() <= 1e-9 * su
         assert hist[0, 2] <= 1e-9 * np.abs(uf).max()
 
 

@@ -175,3 +175,37 @@ def test_variable_density_pressure(hip, oracle):
         # the system is ill-conditioned by the density jump: the solution error
         # is bounded by kappa * 1e-8, far looser than the residual
         assert np.linalg.norm(x - xs) <= 1e-3 * np.linalg.norm(xs)
+
+
+@pytest.mark.parametrize("mode,solver", [("fp64", "bicgstab"), ("mixed", "cg")])
+def test_repeated_solves_are_bit_identical(hip, tri_case, mode, solver):
+    """Every reduction sums per-block partials in a fixed order, so the same
+    solve twice gives the same iterates: identical x bits and iteration
+    counts, on one solver and on a fresh one."""
+    rp, ci, v, b = tri_case
+    out = []
+    for fresh in (False, False, True):
+        if not out or fresh:
+            s = _solver(hip, mode=mode, tolerance=1e-8, maxIters=5000, solver=solver)
+            s.set(rp, ci, v)
+        s.set_rhs(b)
+        s.set_guess(np.zeros(len(b)))
+        s.solve()
+        out.append((s.x(), s.n_iters(), s.stats()["outer_passes"]))
+    for x, it, passes in out[1:]:
+        assert np.array_equal(x, out[0][0]) and it == out[0][1] and passes == out[0][2]
+
+
+def test_mixed_unconverged_finishes_in_fp64(hip, tri_case):
+    """A mixed solve that stops above the tolerance with iterations left (here
+    forced from the host: no FP32 refinement pass allowed) continues in FP64
+    from its best iterate instead of returning unconverged."""
+    rp, ci, v, b = tri_case
+    A = H.compact_sorted(rp, ci, v)
+    for solver in ("bicgstab", "cg"):
+        s = _solver(hip, mode="mixed", tolerance=1e-8, maxIters=5000, solver=solver, maxOuter=0)
+        s.set(rp, ci, v)
+        s.set_rhs(b)
+        assert s.solve() <= 1e-8
+        assert s.stats()["fallbacks"] == 1
+        assert np.linalg.norm(b - A @ s.x()) <= 1.5e-8 * np.linalg.norm(b)

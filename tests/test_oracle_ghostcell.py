@@ -184,3 +184,27 @@ def test_rotating_body_rows(oracle, gc):
         y = A @ np.r_[field[:, 0], field[:, 1]] + rhs
         rows = np.where((g.types > 0) & (g.owner == o))[0]
         assert np.abs(y[rows]).max() <= 1e-12 and np.abs(y[N + rows]).max() <= 1e-12
+
+
+def test_two_body_force_is_per_body(oracle, gc):
+    """With two bodies a linear pressure a + b.x gives each body -b times its
+    own boundary polygon's area.  This restatement (and the device) integrate
+    each body over its own boundary points; the reference's
+    applyHydrodynamicForce (GhostCellImmersedBoundary.cpp:97-141) keeps one
+    stress list across its body loop, so its force on the second body would
+    also integrate the first body's points -- a reference bug this build does
+    not reproduce."""
+    n = 96
+    xy, cptr, cind = G.rectilinear(n, n)
+    m = oracle.Mesh(xy, cptr, cind)
+    N = m.n_cells
+    _, cc = m.cells()
+    g = gc(m, xy, cptr, cind, [(0.3, 0.35, 0.15), (0.68, 0.62, 0.2)])
+    b = np.array([-0.9, 1.7])
+    f, pts = g.hydrodynamic_force(1.0, 0.05, np.zeros((N, 2)), 2.0 + cc @ b)
+    assert f.shape == (2, 2)
+    for o in range(2):
+        poly = np.array([t[1] for t in pts[o]])
+        area = 0.5 * np.sum(poly[:, 0] * np.roll(poly[:, 1], -1) - np.roll(poly[:, 0], -1) * poly[:, 1])
+        assert abs(area - np.pi * [0.15, 0.2][o] ** 2) <= 0.05 * area  # each body's own outline
+        assert np.abs(f[o] + b * area).max() <= 1e-12
