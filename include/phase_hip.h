@@ -152,6 +152,17 @@ int phase_hip_solver_export(phase_hip_solver *s, int *nnz, int *rowPtr, int *col
  * FP64 when single == 0, FP32 values and vectors otherwise. */
 int phase_hip_solver_spmv(phase_hip_solver *s, int single, const double *x, double *y);
 
+/* phase_hip_solver_spmv without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  The
+ * operand and product are staged in a fixed 32 MiB device array that the
+ * library holds on each GPU it runs on; ranks above 2,097,152 rows are
+ * rejected (PHASE_HIP_EINVAL), and single != 0 needs the FP32 copy of the
+ * matrix that a mixed solve or phase_hip_solver_spmv(single = 1) builds
+ * (PHASE_HIP_ESTATE otherwise).  Every call on one GPU shares that array:
+ * calls that overlap on one GPU (several host threads) race. */
+int phase_hip_solver_spmv_noalloc(phase_hip_solver *s, int single, const double *x, double *y);
+
 /* The ILU(0) factor computed on the device, in SOLVER order (rows and columns
  * permuted), compact CSR with sorted columns: L strictly below the diagonal
  * (unit diagonal implied), U on and above.  perm[k] = original row of solver

@@ -9,6 +9,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 
@@ -30,6 +31,32 @@ inline int gridFor(long n, int block, int cap) {
 
 constexpr int kElemCap = 4096;
 constexpr int kRedCap = 1024;
+
+// Scratch of the allocation-free SpMV (phase_hip_solver_spmv_noalloc): the
+// operand and the product in solver order, kNoAllocMaxRank rows each, in FP64
+// (the FP32 path uses the same bytes).  32 MiB that every GPU the library runs
+// on holds until the process ends.  Host code never passes `scratch` itself
+// to a kernel or a copy: it uses the device address that
+// hipGetSymbolAddress gave for the current device (gScratchPtr), set up when
+// a solver is created on that device.
+constexpr int kNoAllocMaxRank = 1 << 21;
+__device__ __align__(256) double scratch[2 * (size_t)kNoAllocMaxRank];
+constexpr int kMaxDevices = 64;
+std::mutex gScratchMutex;
+double *gScratchPtr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (which must be `device`),
+// looked up once per device.
+double *scratchAddress(int device) {
+  if (device < 0 || device >= kMaxDevices) fail(PHASE_HIP_EINVAL, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(gScratchMutex);
+  if (!gScratchPtr[device]) {
+    void *p = nullptr;
+    PH_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    gScratchPtr[device] = static_cast<double *>(p);
+  }
+  return gScratchPtr[device];
+}
 
 template <class F>
 int guarded(phase_hip_solver *s, F &&f) {
@@ -769,6 +796,7 @@ int phase_hip_solver_create(phase_hip_solver **out, int device) {
   s->device = device;
   int rc = guarded(s, [&] {
     setDevice(s);
+    scratchAddress(device);  // the allocation-free entry points' scratch on this device
     PH_CHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     PH_CHECK(hipEventCreate(&s->ev0));
     PH_CHECK(hipEventCreate(&s->ev1));
@@ -1078,6 +1106,45 @@ int phase_hip_solver_spmv(phase_hip_solver *s, int single, const double *x, doub
     }
     PH_CHECK(hipGetLastError());
     PH_CHECK(hipMemcpyAsync(y, s->dR64.p, n * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    PH_CHECK(hipStreamSynchronize(s->stream));
+  });
+}
+
+int phase_hip_solver_spmv_noalloc(phase_hip_solver *s, int single, const double *x, double *y) {
+  return guarded(s, [&] {
+    if (s->rank > kNoAllocMaxRank)
+      fail(PHASE_HIP_EINVAL, "rank " + std::to_string(s->rank) + " exceeds the allocation-free scratch (" +
+                                 std::to_string(kNoAllocMaxRank) + " rows)");
+    if (!s->havePattern || !s->haveValues) fail(PHASE_HIP_ESTATE, "no system");
+    if (single && !s->sell32Valid)
+      fail(PHASE_HIP_ESTATE, "no FP32 copy of the matrix yet (a mixed solve or an FP32 spmv builds it)");
+    if (!x || !y) fail(PHASE_HIP_EINVAL, "null vector");
+    setDevice(s);
+    double *sp = gScratchPtr[s->device];
+    if (!sp) fail(PHASE_HIP_ESTATE, "scratch not set up on this device");
+    const int n = s->rank;
+    // xs: operand in solver order; ys: first the operand in the caller's order,
+    // then the product; the product in the caller's order goes back to xs
+    double *xs = sp, *ys = sp + kNoAllocMaxRank;
+    PH_CHECK(hipMemcpyAsync(ys, x, n * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    if (!single) {
+      hipLaunchKernelGGL((kPermuteIn<double, double>), dim3(gridFor(n, 256, kElemCap)), dim3(256), 0, s->stream, n,
+                         s->dPerm.p, ys, xs, 1.0);
+      hipLaunchKernelGGL((kSpmv<double>), dim3(spmvGrid(s)), dim3(kRedBlock), 0, s->stream, n, s->ord.nSlices,
+                         s->dSliceOff.p, s->dSellCol.p, s->dSell64.p, xs, ys);
+      hipLaunchKernelGGL((kPermuteOut<double>), dim3(gridFor(n, 256, kElemCap)), dim3(256), 0, s->stream, n,
+                         s->dPerm.p, ys, xs);
+    } else {
+      float *xf = reinterpret_cast<float *>(xs), *yf = reinterpret_cast<float *>(ys);
+      hipLaunchKernelGGL((kPermuteIn<double, float>), dim3(gridFor(n, 256, kElemCap)), dim3(256), 0, s->stream, n,
+                         s->dPerm.p, ys, xf, 1.0);
+      hipLaunchKernelGGL((kSpmv<float>), dim3(spmvGrid(s)), dim3(kRedBlock), 0, s->stream, n, s->ord.nSlices,
+                         s->dSliceOff.p, s->dSellCol.p, s->dSell32.p, xf, yf);
+      hipLaunchKernelGGL((kPermuteOut<float>), dim3(gridFor(n, 256, kElemCap)), dim3(256), 0, s->stream, n,
+                         s->dPerm.p, yf, xs);
+    }
+    PH_CHECK(hipGetLastError());
+    PH_CHECK(hipMemcpyAsync(y, xs, n * sizeof(double), hipMemcpyDeviceToHost, s->stream));
     PH_CHECK(hipStreamSynchronize(s->stream));
   });
 }

@@ -46,6 +46,7 @@ _SIGS = {
     "phase_hip_solver_stats": (ctypes.c_int, [_vp, _ip, _ip, _ip, _dp]),
     "phase_hip_solver_export": (ctypes.c_int, [_vp, _ip, _vp, _vp, _vp, _vp]),
     "phase_hip_solver_spmv": (ctypes.c_int, [_vp, ctypes.c_int, _f64p, _f64p]),
+    "phase_hip_solver_spmv_noalloc": (ctypes.c_int, [_vp, ctypes.c_int, _f64p, _f64p]),
     "phase_hip_solver_ilu_export": (ctypes.c_int, [_vp, _ip, _vp, _vp, _vp, _vp]),
     "phase_hip_solver_time_kernels": (ctypes.c_int, [_vp, ctypes.c_int, _f64p, ctypes.c_int]),
     "phase_hip_solver_kernel_bytes": (ctypes.c_int, [_vp, _f64p, ctypes.c_int]),
