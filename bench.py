@@ -45,9 +45,11 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_sweeps_pcorr_poisson_1M.m
 
 
 def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` in the committed counter summary's
-    table (the "HBM bytes per launch" column of the row that names it), or
-    None."""
+    """HBM bytes per launch of `kernel` (a trace name such as
+    "phase::kLSolve<float, 3, 4>") in the committed counter summary's table:
+    the "HBM bytes per launch" column of the row whose kernel or template
+    column names it, or None."""
+    want = "`%s`" % kernel.replace("phase::", "")
     try:
         col = None
         with open(TRAFFIC_FILE) as f:
@@ -55,10 +57,36 @@ def measured_traffic(kernel):
                 cells = [c.strip() for c in line.split("|")]
                 if "HBM bytes per launch" in cells:
                     col = cells.index("HBM bytes per launch")
-                elif col is not None and len(cells) > col and cells[1] == "`%s`" % kernel:
+                elif col is not None and len(cells) > col and want in cells[1:3]:
                     return int(cells[col].replace(",", ""))
     except (OSError, ValueError):
         return None
+    return None
+
+
+TRACE_FILE = os.path.join(ROOT, "profiles", "kernels_pcorr_poisson_1M.md")
+# kernel names of a rocprofv3 trace -> entries of time_kernels() that time the same launches
+TRACE_KEYS = {
+    "phase::kLSolve<float, 3, 4>": "lsolve32cg_launch", "phase::kLUSolve<float, 3, 4>": "lusolve32cg_launch",
+    "phase::kLSolve<float, 0, 4>": "lsolve32_launch", "phase::kLUSolve<float, 0, 4>": "lusolve32_launch",
+    "phase::kUSolve<float, 4>": "usolve32_launch",
+    "phase::kLSolve<double, 3, 4>": "lsolve64cg_launch", "phase::kLUSolve<double, 3, 4>": "lusolve64cg_launch",
+    "phase::kLSolve<double, 0, 4>": "lsolve64_launch", "phase::kLUSolve<double, 0, 4>": "lusolve64_launch",
+    "phase::kUSolve<double, 4>": "usolve64_launch", "phase::kSpmv<double>": "spmv64",
+}
+
+
+def dominant_from_trace():
+    """Kernel name of the first data row of the first table in the committed
+    kernel-trace summary of the headline method (rows sorted by total time)."""
+    try:
+        with open(TRACE_FILE) as f:
+            for line in f:
+                cells = [c.strip() for c in line.split("|")]
+                if len(cells) > 2 and cells[1].startswith("`phase::"):
+                    return cells[1].strip("`")
+    except OSError:
+        pass
     return None
 
 
@@ -79,6 +107,15 @@ def clean(v):
     if isinstance(v, float):
         return float("%.4g" % v) if np.isfinite(v) else None
     return v
+
+
+def solver_kw(mode):
+    """Solver set-up of a method name "<precision>[-cg][-mg]": precision fp64,
+    mixed or learned; -cg preconditioned CG (Belos "CG") instead of BiCGStab;
+    -mg the aggregation multigrid preconditioner instead of ILU(0) alone."""
+    parts = mode.split("-")
+    return dict(mode=parts[0], solver="cg" if "cg" in parts[1:] else "bicgstab",
+                preconditioner="multigrid" if "mg" in parts[1:] else "ilu")
 
 
 def build_problem(n_cells, k, sigma):
@@ -131,7 +168,7 @@ def cpu_baseline(xy, cptr, cind, bc, g, b, iters_target):
                        "%d iterations; %s" % (t_asm, iters_target, text))
 
 
-def train_network(s, mesh, d_g, d_bc, d_pf, args, max_pairs=256, capture_solves=None):
+def train_network(s, mesh, d_g, d_bc, d_pf, args, kw, max_pairs=256, capture_solves=None):
     """The learned part's setup, outside the timed steps: FP64 solves of earlier
     'time steps' (right-hand sides from seeds 100, 101, ...) emit (residual,
     correction) pairs; the network trains on them with the device kernels."""
@@ -139,7 +176,11 @@ def train_network(s, mesh, d_g, d_bc, d_pf, args, max_pairs=256, capture_solves=
 
     t0 = time.perf_counter()
     nsolves = args.capture_solves if capture_solves is None else capture_solves
+    # pairs come from the plain solver (FP64 BiCGStab + ILU(0)), whose long
+    # iteration histories hold many of them
     s.setup("mode", "fp64")
+    s.setup("solver", "bicgstab")
+    s.setup("preconditioner", "ilu")
     for t in range(nsolves):
         s.assemble_laplacian_device(mesh, 3, gamma_face=d_g, bc_type=d_bc, phi_face=d_pf)
         if t == 0:
@@ -153,7 +194,8 @@ def train_network(s, mesh, d_g, d_bc, d_pf, args, max_pairs=256, capture_solves=
     first = s.learned_train(50, lr=args.lr, seed=1)
     loss = s.learned_train(args.train_steps, lr=args.lr, seed=2)
     t2 = time.perf_counter()
-    s.setup("mode", "learned")
+    for k, v in kw.items():
+        s.setup(k, v)
     log("learned: %d pairs from %d FP64 solves (%.1fs), %d Adam steps (%.1fs), loss %.3f -> %.3f"
         % (pairs, nsolves, t1 - t0, args.train_steps + 50, t2 - t1, first, loss))
     return dict(train_pairs=pairs, train_steps=args.train_steps + 50, capture_s=round(t1 - t0, 2),
@@ -170,9 +212,8 @@ def host_path(args, csr, method):
 
     rp, ci, v, rhs = csr
     b = -rhs  # setRhs(-rhs_), FiniteVolumeEquation.tpp:73
-    prec, _, kry = method.partition("-")
-    s = P.HipSparseMatrixSolver(0, tolerance=args.tol, maxIters=50000, mode=prec, innerTolerance=args.inner_tol,
-                                blockRows=args.block_rows, solver=kry or "bicgstab")
+    s = P.HipSparseMatrixSolver(0, tolerance=args.tol, maxIters=50000, innerTolerance=args.inner_tol,
+                                blockRows=args.block_rows, **solver_kw(method))
     s.set(rp, ci, v)
     s.set_rhs(b)
     s.solve(raise_on_noconv=False)
@@ -246,11 +287,11 @@ def learned_case(args):
     d_g, d_bc, d_pf, d_b = (P.DeviceArray(g), P.DeviceArray(bc), P.DeviceArray(np.zeros(mesh.n_faces)),
                             P.DeviceArray(b))
     out = dict(workload="pcorr-poisson-delaunay-sigma2", cells=mesh.n_cells, edge_points=k, sigma=2.0)
-    for mode in ("fp64", "mixed", "learned", "fp64-cg", "mixed-cg"):
-        prec, _, kry = mode.partition("-")
-        s = P.HipSparseMatrixSolver(0, tolerance=args.tol, maxIters=100000, mode=prec,
-                                    innerTolerance=args.inner_tol, blockRows=args.block_rows,
-                                    solver=kry or "bicgstab", ordering=args.ordering)
+    for mode in args.c4_modes.split(","):
+        kw = solver_kw(mode)
+        prec = kw["mode"]
+        s = P.HipSparseMatrixSolver(0, tolerance=args.tol, maxIters=100000, innerTolerance=args.inner_tol,
+                                    blockRows=args.block_rows, ordering=args.ordering, **kw)
 
         def step():
             s.assemble_laplacian_device(mesh, 3, gamma_face=d_g, bc_type=d_bc, phi_face=d_pf)
@@ -264,7 +305,7 @@ def learned_case(args):
             per_solve = max(1, out["fp64"]["iters"] // every)
             a2 = argparse.Namespace(**vars(args))
             a2.capture_every = every
-            info = train_network(s, mesh, d_g, d_bc, d_pf, a2, max_pairs=args.c4_pairs,
+            info = train_network(s, mesh, d_g, d_bc, d_pf, a2, kw, max_pairs=args.c4_pairs,
                                  capture_solves=max(1, -(-args.c4_pairs // per_solve)))
         step()
         t0 = time.perf_counter()
@@ -312,10 +353,9 @@ def bubble_case(args):
     d_g, d_bc, d_pf, d_b = (P.DeviceArray(gamma), P.DeviceArray(bc), P.DeviceArray(np.zeros(mesh.n_faces)),
                             P.DeviceArray(rhs))
     out = dict(workload="rising-bubble-pressure-1x2-quads", cells=mesh.n_cells, density_ratio=1000.0)
-    for mode in ("fp64", "mixed", "fp64-cg", "mixed-cg"):
-        prec, _, kry = mode.partition("-")
-        s = P.HipSparseMatrixSolver(0, tolerance=args.tol, maxIters=100000, mode=prec, blockRows=args.block_rows,
-                                    solver=kry or "bicgstab", ordering=args.ordering)
+    for mode in ("fp64", "fp64-cg", "mixed-cg", "fp64-cg-mg", "mixed-cg-mg"):
+        s = P.HipSparseMatrixSolver(0, tolerance=args.tol, maxIters=100000, blockRows=args.block_rows,
+                                    ordering=args.ordering, **solver_kw(mode))
 
         def step():
             s.assemble_laplacian_device(mesh, 3, gamma_face=d_g, bc_type=d_bc, phi_face=d_pf)
@@ -367,6 +407,7 @@ def multiphase_case(args):
         d.setup(k + ".maxIters", 20000)
         d.setup(k + ".mode", args.multiphase_mode)
     d.setup("p.solver", "cg")
+    d.setup("p.preconditioner", args.multiphase_precond)
     d.surface_tension(24.5, 2.1 * h, "pow8")
     setup_s = time.perf_counter() - t0
     dt = 1e-4
@@ -377,6 +418,7 @@ def multiphase_case(args):
     u, p, uf, g = d.fields()
     out = dict(workload="rising-bubble-multiphase-step-1x2-quads", cells=mesh.n_cells, ms_per_step=round(ms, 2),
                steps=args.multiphase_steps, setup_s=round(setup_s, 2), mode=args.multiphase_mode,
+               pressure_preconditioner=args.multiphase_precond,
                iters_gamma_u_p=[float(x) for x in hist.mean(0)[:3]], max_div=float("%.3g" % hist[:, 3].max()),
                max_u=float("%.4g" % np.abs(u).max()))
     log("multiphase: %.2f ms/step (setup %.1f s), iterations gamma/u/p %s, max div %.2e" %
@@ -416,8 +458,8 @@ def ib_case(args):
     ibq = P.HipEquation(mesh, 1)
     out = dict(workload="cylinder-ghost-cell-pressure-30x20-quads", cells=mesh.n_cells,
                ghost_cells=int((types == 1).sum()), solid_cells=int((types == 2).sum()))
-    for mode in ("fp64", "mixed"):
-        s = P.HipSparseMatrixSolver(0, tolerance=args.tol, maxIters=100000, mode=mode, solver="bicgstab")
+    for mode in ("fp64", "mixed", "fp64-mg", "mixed-mg"):
+        s = P.HipSparseMatrixSolver(0, tolerance=args.tol, maxIters=100000, **solver_kw(mode))
 
         def step():
             nonlocal eq
@@ -473,6 +515,7 @@ def ib_flow_case(args):
         d.setup(k + ".tolerance", args.tol)
         d.setup(k + ".maxIters", 100000)
         d.setup(k + ".mode", args.ib_flow_mode)
+    d.setup("p.preconditioner", args.ib_flow_precond)
     setup_s = time.perf_counter() - t0
     d.advance(2e-3, 2)
     t0 = time.perf_counter()
@@ -482,6 +525,7 @@ def ib_flow_case(args):
     force = ib.hydrodynamic_force(1.0, 0.005, P.DeviceArray(u.reshape(-1)), P.DeviceArray(p))[0]
     out = dict(workload="cylinder-ghost-cell-flow-step-30x20-quads", cells=mesh.n_cells, ms_per_step=round(ms, 2),
                steps=args.ib_flow_steps, setup_s=round(setup_s, 2), mode=args.ib_flow_mode,
+               pressure_preconditioner=args.ib_flow_precond,
                iters_u_p=[float(x) for x in hist.mean(0)[:2]], max_div=float("%.3g" % hist[:, 2].max()),
                force=[float("%.5g" % x) for x in force])
     log("ib flow: %.2f ms/step (setup %.1f s), iterations u/p %s, max div %.2e, force %s" %
@@ -547,8 +591,9 @@ def main():
     ap.add_argument("--inner-tol", type=float, default=0.1)
     ap.add_argument("--block-rows", type=int, default=32, help="rows per ILU aggregate (blocks pack aggregates to 64)")
     ap.add_argument("--ordering", default="geometric", help="ILU aggregates: geometric (bricks) or graph")
-    ap.add_argument("--modes", default="fp64,fp64-cg,mixed,mixed-cg,learned",
-                    help="precision fp64|mixed|learned, with -cg for preconditioned CG instead of BiCGStab")
+    ap.add_argument("--modes", default="fp64,fp64-cg,mixed-cg,fp64-mg,fp64-cg-mg,mixed-cg-mg,learned-cg-mg",
+                    help="precision fp64|mixed|learned, with -cg for preconditioned CG instead of BiCGStab and "
+                         "-mg for the multigrid preconditioner instead of ILU(0) alone")
     ap.add_argument("--capture-solves", type=int, default=4)
     ap.add_argument("--capture-every", type=int, default=100)
     ap.add_argument("--train-steps", type=int, default=600)
@@ -567,10 +612,14 @@ def main():
     ap.add_argument("--ib-cells", type=int, default=1_000_000,
                     help="C5 ghost-cell pressure system at this many cells (0 skips)")
     ap.add_argument("--multiphase-mode", default="mixed", help="solver mode of the multiphase driver's solves")
+    ap.add_argument("--multiphase-precond", default="multigrid", help="pressure preconditioner (ilu or multigrid)")
+    ap.add_argument("--ib-flow-precond", default="multigrid", help="pressure preconditioner (ilu or multigrid)")
     ap.add_argument("--copy-probe-mb", type=int, default=0, help="run one copy kernel of this size (counter scale)")
     ap.add_argument("--no-host-path", dest="host_path", action="store_false",
                     help="skip the host-buffer (PCIe-inclusive) solve")
-    ap.add_argument("--c4-cells", type=int, default=0, help="learned case, sigma 2, at this many cells (0 skips)")
+    ap.add_argument("--c4-cells", type=int, default=1_000_000,
+                    help="learned case, sigma 2, at this many cells (0 skips)")
+    ap.add_argument("--c4-modes", default="fp64,fp64-cg,mixed-cg,fp64-cg-mg,mixed-cg-mg,learned-cg-mg")
     ap.add_argument("--spmv-cells", type=int, default=0, help="SpMV bandwidth point at this many cells (0 skips)")
     ap.add_argument("--c4-pairs", type=int, default=2000, help="training pairs for the learned case")
     args = ap.parse_args()
@@ -598,11 +647,10 @@ def main():
     kernels = None
     nnz = None
     for mode in args.modes.split(","):
-        # "<precision>" runs BiCGStab, "<precision>-cg" preconditioned CG
-        prec, _, kry = mode.partition("-")
-        s = P.HipSparseMatrixSolver(0, tolerance=args.tol, maxIters=50000, mode=prec,
-                                    innerTolerance=args.inner_tol, blockRows=args.block_rows,
-                                    solver=kry or "bicgstab", ordering=args.ordering)
+        kw = solver_kw(mode)
+        prec = kw["mode"]
+        s = P.HipSparseMatrixSolver(0, tolerance=args.tol, maxIters=50000, innerTolerance=args.inner_tol,
+                                    blockRows=args.block_rows, ordering=args.ordering, **kw)
 
         def step():
             s.assemble_laplacian_device(mesh, 3, gamma_face=d_g, bc_type=d_bc, phi_face=d_pf)
@@ -612,7 +660,7 @@ def main():
 
         train_info = None
         if prec == "learned":
-            train_info = train_network(s, mesh, d_g, d_bc, d_pf, args)
+            train_info = train_network(s, mesh, d_g, d_bc, d_pf, args, kw)
         for _ in range(args.warmup):
             step()
         t0 = time.perf_counter()
@@ -646,24 +694,17 @@ def main():
     # headline: the fastest method that reached the FP64 goal
     ok = {k: v for k, v in results.items() if v["relres"] <= args.tol}
     head = min(ok, key=lambda k: ok[k]["ms_per_step"]) if ok else list(results)[0]
-    # dominant kernel of one iteration of the headline method: per apply C-1
-    # kLSolve, one kLUSolve and C-1 kUSolve launches (FP32 sweeps for mixed /
-    # learned), per SpMV one SpMV launch with FP64 coefficients; BiCGStab
-    # applies and multiplies twice per iteration, CG once, so the shares are
-    # the same
-    C = ordinfo["colours"]
-    w = "32" if head.partition("-")[0] in ("mixed", "learned") else "64"
-    share = {"usolve%s_launch" % w: (C - 1) * kernels["usolve%s_launch" % w],
-             "lsolve%s_launch" % w: (C - 1) * kernels["lsolve%s_launch" % w],
-             "lusolve%s_launch" % w: kernels["lusolve%s_launch" % w],
-             "spmv64": kernels["spmv64"]}
-    dom = max(share, key=share.get)
-    T = "double" if w == "64" else "float"
-    names = {"usolve%s_launch" % w: "kUSolve<%s>" % T, "lsolve%s_launch" % w: "kLSolve<%s>" % T,
-             "lusolve%s_launch" % w: "kLUSolve<%s>" % T, "spmv64": "kSpmv<double>"}
-    roof = dict(kernel=names[dom], bound="hbm", achieved=round(gbs[dom], 1), peak=PEAK_HBM_GBS, unit="GB/s",
-                frac=round(gbs[dom] / PEAK_HBM_GBS, 4), traffic=measured_traffic(names[dom]),
-                algorithmic_bytes=round(kbytes[dom]), avg_launch_ms=round(kernels[dom], 5))
+    # dominant kernel of the headline solve: the top row (largest share of
+    # kernel time) of the committed kernel trace of the headline method; its
+    # launches are timed live above with HIP events
+    dom_name = dominant_from_trace()
+    key = TRACE_KEYS.get(dom_name)
+    if key is None or key not in kernels:
+        dom_name, key = "phase::kSpmv<double>", "spmv64"
+    roof = dict(kernel=dom_name, bound="hbm", achieved=round(gbs[key], 1), peak=PEAK_HBM_GBS, unit="GB/s",
+                frac=round(gbs[key] / PEAK_HBM_GBS, 4), traffic=measured_traffic(dom_name),
+                algorithmic_bytes=round(kbytes[key]), avg_launch_ms=round(kernels[key], 5),
+                timing_key=key, selected_from=os.path.relpath(TRACE_FILE, ROOT))
 
     out = {
         "metric": METRIC,

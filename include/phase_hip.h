@@ -95,12 +95,23 @@ int phase_hip_solver_create(phase_hip_solver **s, int device);
 int phase_hip_solver_destroy(phase_hip_solver *s);
 
 /* SparseMatrixSolver::setup(ptree) -- TrilinosBelosSparseMatrixSolver.cpp:44-86.
- * Keys: "tolerance" (1e-8), "maxIters" (500), "mode" (fp64|mixed|learned),
+ * Keys: "tolerance" (1e-8), "maxIters" (500), "solver" (BICGSTAB | CG, the
+ * reference's Belos names), "mode" (fp64|mixed|learned),
  * "innerTolerance" (mixed: residual reduction between FP64 refinement points, 0.1),
- * "blockRows" (rows per ILU aggregate, <= 64, default 32; aggregates of one
- * colour are packed into blocks of up to 64 rows), "fallbackRatio" (learned: minimum
- * per-iteration residual reduction before falling back, 0.5).  Unknown keys
- * are ignored, as the reference ignores ptree keys it does not read. */
+ * "maxOuter" (mixed: FP64 refinement passes, 100), "chunk" (iterations queued
+ * between host checks, 8), "blockRows" (rows per ILU aggregate, <= 64,
+ * default 32; aggregates of one colour are packed into blocks of up to 64
+ * rows), "ordering" (geometric | graph aggregates), "pattern" (exact | union),
+ * "fallbackRatio" (learned: minimum per-iteration residual reduction before
+ * falling back, 0.5),
+ * "preconditioner" (ILU = Ifpack2 RILUK(0) as the reference's Belos backend,
+ * the default; MULTIGRID = aggregation multigrid with ILU(0) smoothing, the
+ * role of the reference's MueLu backend, TrilinosMueluSparseMatrixSolver.cpp:26-35),
+ * "multigridCycle" (K | V | auto: K cycles with flexible CG, V cycles with
+ * BiCGStab), "multigridCoarseRows" (rows per aggregate below the finest
+ * level, 16), "multigridDenseRows" (a level of at most this many rows is
+ * inverted densely, 256, at most 512).  Unknown keys are ignored, as the
+ * reference ignores ptree keys it does not read. */
 int phase_hip_solver_setup(phase_hip_solver *s, const char *key, const char *value);
 
 /* SparseMatrixSolver::setRank(int) -- SparseMatrixSolver.h:23 */
@@ -141,6 +152,10 @@ int phase_hip_solver_stats(const phase_hip_solver *s, int *innerIters, int *oute
  * one wavefront each) and block colours (= sweep launches per triangular
  * solve).  No counterpart in the reference. */
 int phase_hip_solver_ordering_info(const phase_hip_solver *s, int *nBlocks, int *nColours);
+/* Rows of each level of the multigrid hierarchy, finest first (the last level
+ * is inverted densely); nLevels = 0 before the first multigrid solve of a
+ * pattern.  No counterpart in the reference. */
+int phase_hip_solver_multigrid_info(const phase_hip_solver *s, int *nLevels, int *sizes, int cap);
 
 /* Export the system held on the device, in the ORIGINAL row/column numbering
  * and compact CSR form (empty slots dropped), each row in the solver's slot
@@ -180,7 +195,9 @@ int phase_hip_solver_ilu_export(phase_hip_solver *s, int *nnz, int *perm, int *r
  * (kLUSolve, FP64), 10 one forward-sweep launch (kLSolve, FP64, mean as 8),
  * 11 / 12 the FP64 / FP32 SpMV cache-cold (a 1 GiB copy before each launch
  * evicts L2 and the last-level cache; the events bracket the one launch),
- * 13..15 as 8..10 with the FP32 factor (mixed mode's sweeps).
+ * 13..15 as 8..10 with the FP32 factor (mixed mode's sweeps), 16 / 17 the
+ * forward-sweep and fused launches in the form CG runs them (the prologue
+ * also moves x += alpha p and r -= alpha q), FP64, 18 / 19 the same in FP32.
  * Benchmark support; no counterpart in the reference. */
 int phase_hip_solver_time_kernels(phase_hip_solver *s, int reps, double *ms, int nms);
 

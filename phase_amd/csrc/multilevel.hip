@@ -1,0 +1,593 @@
+// Aggregation multigrid preconditioner: hierarchy, numeric set-up and the
+// cycle.  The mathematics is written out in multilevel.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "multilevel.h"
+#include "solver.h"
+#include "solver_kernels.h"
+#include "solver_launch.h"
+
+namespace phase {
+
+namespace {
+
+// Device scalars of the K cycle's two flexible CG iterations at one level.
+struct MlScalars {
+  double pq1, alpha1, gamma, alpha2;
+};
+
+template <class T>
+struct MlVecs {
+  // r: right-hand side, e: solution of this level's coarse solve; t, w, y:
+  // cycle work; z1, q1, r1, z2: the K cycle's iterations
+  DBuf<T> r, e, t, w, y, z1, q1, r1, z2;
+  void alloc(int n) {
+    for (DBuf<T> *b : {&r, &e, &t, &w, &y, &z1, &q1, &r1, &z2}) b->alloc(n);
+  }
+  // the finest level uses the cycle work vectors only
+  void allocCycle(int n) {
+    for (DBuf<T> *b : {&t, &w, &y}) b->alloc(n);
+  }
+};
+
+}  // namespace
+
+struct MlLevel {
+  int n = 0;
+  phase_hip_solver *sol = nullptr;  // nested solver; null for the dense last level
+  // maps from the finer level
+  DBuf<int> galPtr, galSrc;         // coarse slot q = sum of finer caller slots galSrc[galPtr[q] ..]
+  DBuf<int> restRow;                // this level's solver row -> aggregate of the finer level (dense: identity)
+  DBuf<int> finerAggPtr;            // finer solver rows of each aggregate
+  DBuf<int> pIdx;                   // finer solver row -> this level's solver row
+  int nFiner = 0, nSlots = 0;
+  // dense last level
+  DBuf<int> cRowPtr, cColInd;
+  DBuf<double> cVals, dense, inv;
+  MlVecs<double> v64;
+  MlVecs<float> v32;
+  DBuf<MlScalars> msc;
+  DBuf<double> partials;
+};
+
+struct Multilevel {
+  std::vector<MlLevel> lv;  // lv[0]: first coarse level
+  bool valuesValid = false, singleValid = false;
+  MlVecs<double> f64;  // fine-level work vectors (t, w, y used)
+  MlVecs<float> f32;
+  std::vector<int> sizes;
+};
+
+namespace {
+
+template <class T>
+MlVecs<T> &vecs(MlLevel &L);
+template <>
+MlVecs<double> &vecs<double>(MlLevel &L) {
+  return L.v64;
+}
+template <>
+MlVecs<float> &vecs<float>(MlLevel &L) {
+  return L.v32;
+}
+template <class T>
+MlVecs<T> &fineVecs(Multilevel &M);
+template <>
+MlVecs<double> &fineVecs<double>(Multilevel &M) {
+  return M.f64;
+}
+template <>
+MlVecs<float> &fineVecs<float>(Multilevel &M) {
+  return M.f32;
+}
+// the ILU(0) factor a level applies in precision T
+template <class T>
+T *facOf(phase_hip_solver *s);
+template <>
+double *facOf<double>(phase_hip_solver *s) {
+  return s->dFac64.p;
+}
+template <>
+float *facOf<float>(phase_hip_solver *s) {
+  return s->dFac32.p;
+}
+
+// ---------------------------------------------------------------- kernels
+// coarse[q] = sum of fine[src[ptr[q] .. ptr[q+1])] in list order
+__global__ void kGalerkin(int m, const int *__restrict__ ptr, const int *__restrict__ src,
+                          const double *__restrict__ fine, double *__restrict__ coarse) {
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < m; q += gridDim.x * blockDim.x) {
+    double acc = 0.;
+    for (int k = ptr[q]; k < ptr[q + 1]; ++k) acc += fine[src[k]];
+    coarse[q] = acc;
+  }
+}
+
+// dense row-major n x n from CSR (one thread per row)
+__global__ void kDenseFill(int n, const int *__restrict__ rowPtr, const int *__restrict__ colInd,
+                           const double *__restrict__ vals, double *__restrict__ a) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    for (int j = 0; j < n; ++j) a[(size_t)i * n + j] = 0.;
+    for (int k = rowPtr[i]; k < rowPtr[i + 1]; ++k) a[(size_t)i * n + colInd[k]] += vals[k];
+  }
+}
+
+// inv = a^-1 by Gauss-Jordan elimination with partial pivoting, one
+// workgroup (a is destroyed).  A zero pivot sets bit 4 of *err.
+constexpr int kDenseMax = 512;
+constexpr int kGjBlock = 1024;
+__global__ __launch_bounds__(kGjBlock) void kDenseInverse(int n, double *__restrict__ a, double *__restrict__ inv,
+                                                          int *err) {
+  __shared__ double fcol[kDenseMax], prow[kDenseMax], pinv[kDenseMax];
+  __shared__ double sval[kGjBlock / 64];
+  __shared__ int sidx[kGjBlock / 64];
+  __shared__ int sPivot;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int t = tid; t < n * n; t += kGjBlock) inv[t] = (t / n == t % n) ? 1. : 0.;
+  __syncthreads();
+  for (int k = 0; k < n; ++k) {
+    // pivot: largest |a[i][k]|, i >= k (lowest i on a tie)
+    double best = -1.;
+    int bi = k;
+    for (int i = k + tid; i < n; i += kGjBlock) {
+      const double v = fabs(a[(size_t)i * n + k]);
+      if (v > best) {
+        best = v;
+        bi = i;
+      }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      const double ov = __shfl_xor(best, o, 64);
+      const int oi = __shfl_xor(bi, o, 64);
+      if (ov > best || (ov == best && oi < bi)) {
+        best = ov;
+        bi = oi;
+      }
+    }
+    if (lane == 0) {
+      sval[wave] = best;
+      sidx[wave] = bi;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double b = sval[0];
+      int p = sidx[0];
+      for (int w = 1; w < kGjBlock / 64; ++w)
+        if (sval[w] > b || (sval[w] == b && sidx[w] < p)) {
+          b = sval[w];
+          p = sidx[w];
+        }
+      if (!(b > 0.)) atomicOr(err, 4);
+      sPivot = p;
+    }
+    __syncthreads();
+    const int p = sPivot;
+    // swap rows k and p, then scale row k by 1 / pivot (kept in LDS)
+    for (int j = tid; j < n; j += kGjBlock) {
+      const double ak = a[(size_t)k * n + j], ap = a[(size_t)p * n + j];
+      const double ik = inv[(size_t)k * n + j], ip = inv[(size_t)p * n + j];
+      prow[j] = ap;
+      pinv[j] = ip;
+      if (p != k) {
+        a[(size_t)p * n + j] = ak;
+        inv[(size_t)p * n + j] = ik;
+      }
+    }
+    __syncthreads();
+    const double d = prow[k];
+    const double rd = d != 0. ? 1. / d : 0.;
+    __syncthreads();  // every thread has the pivot before the row is scaled
+    for (int j = tid; j < n; j += kGjBlock) {
+      prow[j] *= rd;
+      pinv[j] *= rd;
+    }
+    for (int i = tid; i < n; i += kGjBlock) fcol[i] = i == k ? 0. : a[(size_t)i * n + k];
+    __syncthreads();
+    for (int j = tid; j < n; j += kGjBlock) {
+      a[(size_t)k * n + j] = prow[j];
+      inv[(size_t)k * n + j] = pinv[j];
+    }
+    // eliminate column k from every other row
+    for (int t = tid; t < n * n; t += kGjBlock) {
+      const int i = t / n, j = t - i * n;
+      const double f = fcol[i];
+      if (f != 0.) {
+        a[t] = fma(-f, prow[j], a[t]);
+        inv[t] = fma(-f, pinv[j], inv[t]);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// e = inv r (one wavefront per row, FP64 products)
+template <class T>
+__global__ __launch_bounds__(256) void kDenseApply(int n, const double *__restrict__ inv, const T *__restrict__ r,
+                                                   T *__restrict__ e, const Scalars *sc) {
+  if (sc->done) return;
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (row >= n) return;
+  double acc = 0.;
+  for (int j = lane; j < n; j += 64) acc = fma(inv[(size_t)row * n + j], (double)r[j], acc);
+  acc = waveSum(acc);
+  if (lane == 0) e[row] = (T)acc;
+}
+
+// t = b - A x (SELL-64, one row per lane)
+template <class T, class TV>
+__global__ __launch_bounds__(kRedBlock) void kMlResid(int n, int nSlices, const int *__restrict__ sliceOff,
+                                                      const int *__restrict__ col, const TV *__restrict__ val,
+                                                      const T *__restrict__ x, const T *__restrict__ b,
+                                                      T *__restrict__ t, const Scalars *sc) {
+  if (sc->done) return;
+  const int tile = xcdTile(blockIdx.x, gridDim.x);
+  const int slice = tile * (kRedBlock / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int row = slice * 64 + lane;
+  if (slice >= nSlices || row >= n) return;
+  t[row] = b[row] - sellRow<T, TV>(slice, lane, sliceOff, col, val, x);
+}
+
+// rc[k] = sum of t over the finer rows of aggregate restRow[k] (identity when null)
+template <class T>
+__global__ void kMlRestrict(int nc, const int *__restrict__ restRow, const int *__restrict__ aggPtr,
+                            const T *__restrict__ t, T *__restrict__ rc, const Scalars *sc) {
+  if (sc->done) return;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nc; k += gridDim.x * blockDim.x) {
+    const int a = restRow ? restRow[k] : k;
+    double acc = 0.;
+    for (int i = aggPtr[a]; i < aggPtr[a + 1]; ++i) acc += (double)t[i];
+    rc[k] = (T)acc;
+  }
+}
+
+// e[i] += ec[pIdx[i]]
+template <class T>
+__global__ void kMlProlongAdd(int n, const int *__restrict__ pIdx, const T *__restrict__ ec, T *__restrict__ e,
+                              const Scalars *sc) {
+  if (sc->done) return;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) e[i] += ec[pIdx[i]];
+}
+
+// e += w
+template <class T>
+__global__ void kMlAdd(int n, const T *__restrict__ w, T *__restrict__ e, const Scalars *sc) {
+  if (sc->done) return;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) e[i] += w[i];
+}
+
+// q = A p ; partials of (p, q) and (p, r)
+template <class T, class TV>
+__global__ __launch_bounds__(kRedBlock) void kMlSpmvDot2(int n, int nSlices, const int *__restrict__ sliceOff,
+                                                         const int *__restrict__ col, const TV *__restrict__ val,
+                                                         const T *__restrict__ p, T *__restrict__ q,
+                                                         const T *__restrict__ r, const Scalars *sc,
+                                                         double *partials) {
+  if (sc->done) return;
+  const int tile = xcdTile(blockIdx.x, gridDim.x);
+  const int slice = tile * (kRedBlock / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int row = slice * 64 + lane;
+  double d[2] = {0., 0.};
+  if (slice < nSlices && row < n) {
+    const T y = sellRow<T, TV>(slice, lane, sliceOff, col, val, p);
+    q[row] = y;
+    d[0] = (double)p[row] * (double)y;
+    d[1] = (double)p[row] * (double)r[row];
+  }
+  blockPartials<2>(d, partials, tile);
+}
+
+// alpha = (p, r) / (p, q) of the first (which == 1) or second iteration
+static __global__ __launch_bounds__(kFinBlock) void kMlFinishAlpha(const Scalars *sc, MlScalars *m, int which,
+                                                                   const double *partials, int nb) {
+  if (sc->done) return;
+  double d[2];
+  finishPartials<2, kFinBlock>(partials, nb, d);
+  if (threadIdx.x != 0) return;
+  const double alpha = (d[0] != 0. && isfinite(d[0]) && isfinite(d[1])) ? d[1] / d[0] : 0.;
+  if (which == 1) {
+    m->pq1 = d[0];
+    m->alpha1 = alpha;
+  } else {
+    m->alpha2 = alpha;
+  }
+}
+
+// x = alpha1 z1 ; r1 = r0 - alpha1 q1
+template <class T>
+__global__ void kMlFirst(int n, const Scalars *sc, const MlScalars *m, const T *__restrict__ z1,
+                         const T *__restrict__ q1, const T *__restrict__ r0, T *__restrict__ x, T *__restrict__ r1) {
+  if (sc->done) return;
+  const T a = (T)m->alpha1;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    x[i] = a * z1[i];
+    r1[i] = r0[i] - a * q1[i];
+  }
+}
+
+// partials of (z2, q1)
+template <class T>
+__global__ __launch_bounds__(kRedBlock) void kMlDotZQ(int n, const Scalars *sc, const T *__restrict__ z2,
+                                                      const T *__restrict__ q1, double *partials) {
+  if (sc->done) return;
+  double d[1] = {0.};
+  for (int i = blockIdx.x * kRedBlock + threadIdx.x; i < n; i += gridDim.x * kRedBlock)
+    d[0] += (double)z2[i] * (double)q1[i];
+  blockPartials<1>(d, partials, blockIdx.x);
+}
+
+// gamma = (z2, q1) / (z1, q1) ; z2 -= gamma z1 in every block (the same fixed-order sum)
+template <class T>
+__global__ __launch_bounds__(kRedBlock) void kMlDirection(int n, const Scalars *sc, const MlScalars *m,
+                                                          const double *partials, int nb, const T *__restrict__ z1,
+                                                          T *__restrict__ z2) {
+  if (sc->done) return;
+  double d[1];
+  finishPartials<1>(partials, nb, d);
+  __shared__ double sGamma;
+  if (threadIdx.x == 0) {
+    const double pq = m->pq1;
+    sGamma = (pq != 0. && isfinite(d[0])) ? d[0] / pq : 0.;
+  }
+  __syncthreads();
+  const T g = (T)sGamma;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) z2[i] -= g * z1[i];
+}
+
+// x += alpha2 p2
+template <class T>
+__global__ void kMlSecond(int n, const Scalars *sc, const MlScalars *m, const T *__restrict__ p2,
+                          T *__restrict__ x) {
+  if (sc->done) return;
+  const T a = (T)m->alpha2;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] += a * p2[i];
+}
+
+// ---------------------------------------------------------------- host side
+phase_hip_solver *makeNested(phase_hip_solver *s) {
+  auto *c = new phase_hip_solver();
+  c->device = s->device;
+  c->stream = s->stream;  // borrowed: every level runs on the parent's stream
+  c->blockRows = s->mlCoarseRows;
+  c->geometric = s->geometric;
+  return c;
+}
+
+void freeNested(phase_hip_solver *c) {
+  if (!c) return;
+  c->stream = nullptr;  // not ours to destroy
+  phase_hip_solver_destroy(c);
+}
+
+int elemGrid(long n) { return gridFor(n, 256, kElemCap); }
+
+template <class T>
+void cycle(phase_hip_solver *s, Multilevel &M, int l, const Scalars *sc, const T *r, T *e);
+
+// e = (approximate) A_l^-1 r at coarse level l
+template <class T>
+void coarseSolve(phase_hip_solver *s, Multilevel &M, int l, const Scalars *sc, const T *r, T *e) {
+  MlLevel &L = M.lv[l];
+  hipStream_t st = s->stream;
+  if (!L.sol) {
+    hipLaunchKernelGGL((kDenseApply<T>), dim3((L.n + 3) / 4), dim3(256), 0, st, L.n, L.inv.p, r, e, sc);
+    return;
+  }
+  if (!s->mlKCycle) {
+    cycle<T>(s, M, l, sc, r, e);
+    return;
+  }
+  // two flexible CG iterations preconditioned by the cycle (K cycle)
+  MlVecs<T> &V = vecs<T>(L);
+  phase_hip_solver *c = L.sol;
+  const int n = L.n, gs = spmvGrid(c), gu = gridFor(n, kRedBlock, kRedCap);
+  cycle<T>(s, M, l, sc, r, V.z1.p);
+  hipLaunchKernelGGL((kMlSpmvDot2<T, double>), dim3(gs), dim3(kRedBlock), 0, st, n, c->ord.nSlices, c->dSliceOff.p,
+                     c->dSellCol.p, c->dSell64.p, (const T *)V.z1.p, V.q1.p, r, sc, L.partials.p);
+  hipLaunchKernelGGL(kMlFinishAlpha, dim3(1), dim3(kFinBlock), 0, st, sc, L.msc.p, 1, L.partials.p, gs);
+  hipLaunchKernelGGL((kMlFirst<T>), dim3(elemGrid(n)), dim3(256), 0, st, n, sc, L.msc.p, V.z1.p, V.q1.p, r, e,
+                     V.r1.p);
+  cycle<T>(s, M, l, sc, V.r1.p, V.z2.p);
+  hipLaunchKernelGGL((kMlDotZQ<T>), dim3(gu), dim3(kRedBlock), 0, st, n, sc, V.z2.p, V.q1.p, L.partials.p);
+  hipLaunchKernelGGL((kMlDirection<T>), dim3(gridFor(n, 4 * kRedBlock, kRedCap)), dim3(kRedBlock), 0, st, n, sc,
+                     L.msc.p, L.partials.p, gu, V.z1.p, V.z2.p);
+  hipLaunchKernelGGL((kMlSpmvDot2<T, double>), dim3(gs), dim3(kRedBlock), 0, st, n, c->ord.nSlices, c->dSliceOff.p,
+                     c->dSellCol.p, c->dSell64.p, (const T *)V.z2.p, V.q1.p, V.r1.p, sc, L.partials.p);
+  hipLaunchKernelGGL(kMlFinishAlpha, dim3(1), dim3(kFinBlock), 0, st, sc, L.msc.p, 2, L.partials.p, gs);
+  hipLaunchKernelGGL((kMlSecond<T>), dim3(elemGrid(n)), dim3(256), 0, st, n, sc, L.msc.p, V.z2.p, e);
+}
+
+// Restriction of t (finer solver order) into level l's right-hand side, the
+// coarse solve, and the prolongation added to z (finer solver order).
+template <class T>
+void coarseCorrection(phase_hip_solver *s, Multilevel &M, int l, const Scalars *sc, const T *t, T *z) {
+  MlLevel &L = M.lv[l];
+  MlVecs<T> &V = vecs<T>(L);
+  hipStream_t st = s->stream;
+  hipLaunchKernelGGL((kMlRestrict<T>), dim3(elemGrid(L.n)), dim3(256), 0, st, L.n, L.sol ? L.restRow.p : nullptr,
+                     L.finerAggPtr.p, t, V.r.p, sc);
+  coarseSolve<T>(s, M, l, sc, V.r.p, V.e.p);
+  hipLaunchKernelGGL((kMlProlongAdd<T>), dim3(elemGrid(L.nFiner)), dim3(256), 0, st, L.nFiner, L.pIdx.p,
+                     (const T *)V.e.p, z, sc);
+}
+
+// Smoothing, coarse correction and smoothing at level `lvl` (nested solver c,
+// its work vectors V): z = B r.  When `pre` is false the pre-smoothing has
+// already put M^-1 r into z.
+template <class T>
+void smoothCorrectSmooth(phase_hip_solver *s, Multilevel &M, phase_hip_solver *c, MlVecs<T> &V, int next,
+                         const Scalars *sc, const T *r, T *z, bool pre) {
+  hipStream_t st = s->stream;
+  const int n = c->rank;
+  const IluDev<T> L = iluDev<T>(c, facOf<T>(c));
+  if (pre) launchLU<T, kProNone>(c, L, sc, r, V.y.p, (T *)nullptr, (const T *)nullptr, (const T *)nullptr, z);
+  const int gs = spmvGrid(c);
+  hipLaunchKernelGGL((kMlResid<T, double>), dim3(gs), dim3(kRedBlock), 0, st, n, c->ord.nSlices, c->dSliceOff.p,
+                     c->dSellCol.p, c->dSell64.p, (const T *)z, r, V.t.p, sc);
+  coarseCorrection<T>(s, M, next, sc, V.t.p, z);
+  hipLaunchKernelGGL((kMlResid<T, double>), dim3(gs), dim3(kRedBlock), 0, st, n, c->ord.nSlices, c->dSliceOff.p,
+                     c->dSellCol.p, c->dSell64.p, (const T *)z, r, V.t.p, sc);
+  launchLU<T, kProNone>(c, L, sc, V.t.p, V.y.p, (T *)nullptr, (const T *)nullptr, (const T *)nullptr, V.w.p);
+  hipLaunchKernelGGL((kMlAdd<T>), dim3(elemGrid(n)), dim3(256), 0, st, n, (const T *)V.w.p, z, sc);
+}
+
+template <class T>
+void cycle(phase_hip_solver *s, Multilevel &M, int l, const Scalars *sc, const T *r, T *e) {
+  MlLevel &L = M.lv[l];
+  smoothCorrectSmooth<T>(s, M, L.sol, vecs<T>(L), l + 1, sc, r, e, true);
+}
+
+}  // namespace
+
+template <class T>
+void mlFinishApply(phase_hip_solver *s, const Scalars *sc, const T *rhs, T *out) {
+  Multilevel &M = *s->ml;
+  if (M.lv.empty()) return;  // a system no larger than the dense limit: ILU(0) alone
+  smoothCorrectSmooth<T>(s, M, s, fineVecs<T>(M), 0, sc, rhs, out, false);
+}
+template void mlFinishApply<double>(phase_hip_solver *, const Scalars *, const double *, double *);
+template void mlFinishApply<float>(phase_hip_solver *, const Scalars *, const float *, float *);
+
+void mlFree(phase_hip_solver *s) {
+  if (!s->ml) return;
+  for (MlLevel &L : s->ml->lv) freeNested(L.sol);
+  delete s->ml;
+  s->ml = nullptr;
+}
+
+void mlValuesChanged(phase_hip_solver *s) {
+  if (s->ml) s->ml->valuesValid = s->ml->singleValid = false;
+}
+
+std::vector<int> mlLevelSizes(const phase_hip_solver *s) { return s->ml ? s->ml->sizes : std::vector<int>(); }
+
+void mlEnsureHierarchy(phase_hip_solver *s) {
+  if (s->ml) return;
+  if (!s->havePattern) fail(PHASE_HIP_ESTATE, "no pattern");
+  auto *M = new Multilevel();
+  try {
+    const Ordering *fine = &s->ord;
+    std::vector<double> xy = s->cellXY;
+    int nFiner = s->rank;
+    M->sizes.push_back(nFiner);
+    hipStream_t st = s->stream;
+    for (int level = 0; nFiner > s->mlDenseRows; ++level) {
+      CoarseLevel c;
+      buildCoarse(*fine, xy.empty() ? nullptr : xy.data(), c);
+      if (c.n >= nFiner) fail(PHASE_HIP_EINVAL, "multigrid: aggregation did not coarsen level " + std::to_string(level));
+      M->lv.emplace_back();
+      MlLevel &L = M->lv.back();
+      L.n = c.n;
+      L.nFiner = nFiner;
+      L.nSlots = (int)c.colInd.size();
+      L.galPtr.upload(c.galPtr, st);
+      L.galSrc.upload(c.galSrc.empty() ? std::vector<int>{0} : c.galSrc, st);
+      L.finerAggPtr.upload(fine->aggPtr, st);
+      std::vector<int> pIdx(nFiner);
+      bool dense = c.n <= s->mlDenseRows;
+      if (!dense) {
+        L.sol = makeNested(s);
+        // coarse rows couple to more rows than fine ones: when a row of an
+        // aggregate would couple to more rows of its own block than the sweep
+        // kernels take, smaller aggregates are taken at this level; a level
+        // no ordering fits is inverted densely when it is small enough
+        for (int br = s->mlCoarseRows;; br /= 2) {
+          L.sol->blockRows = br;
+          try {
+            solverSetPattern(L.sol, c.n, c.rowPtr.data(), c.colInd.data(), c.xy.empty() ? nullptr : c.xy.data());
+            break;
+          } catch (const HipError &e) {
+            if (br > 2 && e.code == PHASE_HIP_EINVAL) continue;
+            if (e.code != PHASE_HIP_EINVAL || c.n > kDenseMax) throw;
+            freeNested(L.sol);
+            L.sol = nullptr;
+            dense = true;
+            break;
+          }
+        }
+      }
+      if (dense) {
+        if (c.n > kDenseMax) fail(PHASE_HIP_EINVAL, "multigrid: dense level too large");
+        L.cRowPtr.upload(c.rowPtr, st);
+        L.cColInd.upload(c.colInd, st);
+        L.cVals.alloc(std::max(1, L.nSlots));
+        L.dense.alloc((size_t)c.n * c.n);
+        L.inv.alloc((size_t)c.n * c.n);
+        for (int i = 0; i < nFiner; ++i) pIdx[i] = c.aggOfRow[i];
+      } else {
+        L.restRow.upload(L.sol->ord.perm, st);
+        for (int i = 0; i < nFiner; ++i) pIdx[i] = L.sol->ord.iperm[c.aggOfRow[i]];
+        L.msc.alloc(1);
+        L.partials.alloc(2 * (size_t)std::max(spmvGrid(L.sol), kRedCap));
+      }
+      L.pIdx.upload(pIdx, st);
+      M->sizes.push_back(c.n);
+      if (!L.sol) break;
+      fine = &L.sol->ord;
+      xy = c.xy;
+      nFiner = c.n;
+    }
+    PH_CHECK(hipStreamSynchronize(st));
+  } catch (...) {
+    for (MlLevel &L : M->lv) freeNested(L.sol);
+    delete M;
+    throw;
+  }
+  s->ml = M;
+}
+
+void mlEnsureValues(phase_hip_solver *s, bool single) {
+  Multilevel &M = *s->ml;
+  hipStream_t st = s->stream;
+  if (!M.valuesValid) {
+    const double *finer = s->dVals.p;
+    for (MlLevel &L : M.lv) {
+      if (L.sol) {
+        hipLaunchKernelGGL(kGalerkin, dim3(elemGrid(L.nSlots)), dim3(256), 0, st, L.nSlots, L.galPtr.p, L.galSrc.p,
+                           finer, L.sol->dVals.p);
+        solverValuesChanged(L.sol);
+        ensureFactor64(L.sol);
+        finer = L.sol->dVals.p;
+      } else {
+        hipLaunchKernelGGL(kGalerkin, dim3(elemGrid(L.nSlots)), dim3(256), 0, st, L.nSlots, L.galPtr.p, L.galSrc.p,
+                           finer, L.cVals.p);
+        hipLaunchKernelGGL(kDenseFill, dim3(elemGrid(L.n)), dim3(256), 0, st, L.n, L.cRowPtr.p, L.cColInd.p,
+                           L.cVals.p, L.dense.p);
+        hipLaunchKernelGGL(kDenseInverse, dim3(1), dim3(kGjBlock), 0, st, L.n, L.dense.p, L.inv.p, s->dErr.p);
+        PH_CHECK(hipGetLastError());
+        int e = 0;
+        PH_CHECK(hipMemcpyAsync(&e, s->dErr.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        PH_CHECK(hipStreamSynchronize(st));
+        if (e & 4) {
+          PH_CHECK(hipMemsetAsync(s->dErr.p, 0, sizeof(int), st));
+          fail(PHASE_HIP_EBREAKDOWN, "multigrid: singular coarsest matrix");
+        }
+      }
+    }
+    M.valuesValid = true;
+    M.singleValid = false;
+  }
+  // work vectors: FP64 always (a mixed solve may finish in FP64), FP32 for mixed solves
+  if (M.lv.empty()) return;
+  const int n = s->rank;
+  M.f64.allocCycle(n);
+  for (MlLevel &L : M.lv) L.v64.alloc(L.n);
+  if (single) {
+    M.f32.allocCycle(n);
+    for (MlLevel &L : M.lv) L.v32.alloc(L.n);
+    if (!M.singleValid) {
+      for (MlLevel &L : M.lv)
+        if (L.sol) ensureSingle(L.sol);
+      M.singleValid = true;
+    }
+  }
+  PH_CHECK(hipGetLastError());
+}
+
+}  // namespace phase

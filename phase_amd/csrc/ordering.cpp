@@ -461,6 +461,7 @@ void buildOrdering(Ordering &o, int n, const int *rowPtr, const int *colInd, int
   std::stable_sort(border.begin(), border.end(), [&](int a, int c) { return colour[a] < colour[c]; });
   std::vector<std::vector<int>> packs;
   std::vector<int> packColour;
+  std::vector<int> aggSize;  // aggregates in pack order, i.e. in solver order
   for (int ci = 0; ci < nb; ++ci) {
     const int a = border[ci];
     if (packs.empty() || packColour.back() != colour[a] || packs.back().size() + members[a].size() > (size_t)kLanes) {
@@ -468,7 +469,10 @@ void buildOrdering(Ordering &o, int n, const int *rowPtr, const int *colInd, int
       packColour.push_back(colour[a]);
     }
     packs.back().insert(packs.back().end(), members[a].begin(), members[a].end());
+    aggSize.push_back((int)members[a].size());
   }
+  o.aggPtr.assign(aggSize.size() + 1, 0);
+  for (size_t a = 0; a < aggSize.size(); ++a) o.aggPtr[a + 1] = o.aggPtr[a] + aggSize[a];
   nb = (int)packs.size();
   o.perm.clear();
   o.perm.reserve(n);
@@ -548,10 +552,12 @@ void buildOrdering(Ordering &o, int n, const int *rowPtr, const int *colInd, int
         }
       }
       if (nL > 255 || nU > 255) fail(PHASE_HIP_EINVAL, "row too long for the ILU layout");
-      if (nU - nUin > 8)
-        fail(PHASE_HIP_EINVAL, "row couples to more than 8 rows of later ILU blocks");
-      if (nL - nLoff > 8 || nUin > 8)
-        fail(PHASE_HIP_EINVAL, "row couples to more than 8 rows of its own ILU block");
+      // the lane record holds these counts in 4 bits (the sweep kernels keep
+      // up to 8 in registers and read the rest from the factor)
+      if (nU - nUin > 15)
+        fail(PHASE_HIP_EINVAL, "row couples to more than 15 rows of later ILU blocks");
+      if (nL - nLoff > 15 || nUin > 15)
+        fail(PHASE_HIP_EINVAL, "row couples to more than 15 rows of its own ILU block");
       if (nLoff > 63) fail(PHASE_HIP_EINVAL, "row couples to more than 63 rows of earlier ILU blocks");
       maxRegs = std::max({maxRegs, nL - nLoff, nLoff, nUin, nU - nUin});
       o.rows[k].nL = (uint8_t)nL;
@@ -689,6 +695,60 @@ void buildOrdering(Ordering &o, int n, const int *rowPtr, const int *colInd, int
     o.updSrc[fillPos[slot]] = src;
     o.updDst[fillPos[slot]++] = dst;
   });
+}
+
+}  // namespace phase
+
+namespace phase {
+
+void buildCoarse(const Ordering &fine, const double *fineXY, CoarseLevel &c) {
+  const int n = fine.n;
+  const int na = (int)fine.aggPtr.size() - 1;
+  if (na < 1 || fine.aggPtr[na] != n) fail(PHASE_HIP_EINVAL, "ordering has no aggregates");
+  c = CoarseLevel();
+  c.n = na;
+  c.aggOfRow.assign(n, 0);
+  for (int a = 0; a < na; ++a)
+    for (int k = fine.aggPtr[a]; k < fine.aggPtr[a + 1]; ++k) c.aggOfRow[k] = a;
+  // coarse entry (A, B) of every finer compact entry, with its caller slot
+  struct Ent {
+    int col, slot;
+  };
+  std::vector<std::vector<Ent>> rows(na);
+  for (int i = 0; i < n; ++i) {
+    const int A = c.aggOfRow[fine.iperm[i]];
+    for (int j = fine.cRowPtr[i]; j < fine.cRowPtr[i + 1]; ++j)
+      rows[A].push_back({c.aggOfRow[fine.iperm[fine.cCol[j]]], fine.compactSrc[j]});
+  }
+  c.rowPtr.assign(na + 1, 0);
+  c.galPtr.clear();
+  for (int A = 0; A < na; ++A) {
+    auto &r = rows[A];
+    std::sort(r.begin(), r.end(),
+              [](const Ent &x, const Ent &y) { return x.col < y.col || (x.col == y.col && x.slot < y.slot); });
+    for (size_t q = 0; q < r.size(); ++q) {
+      if (q == 0 || r[q].col != r[q - 1].col) {
+        c.colInd.push_back(r[q].col);
+        c.galPtr.push_back((int)c.galSrc.size());
+      }
+      c.galSrc.push_back(r[q].slot);
+    }
+    c.rowPtr[A + 1] = (int)c.colInd.size();
+  }
+  c.galPtr.push_back((int)c.galSrc.size());
+  if (fineXY) {
+    c.xy.assign(2 * (size_t)na, 0.);
+    for (int a = 0; a < na; ++a) {
+      double sx = 0., sy = 0.;
+      for (int k = fine.aggPtr[a]; k < fine.aggPtr[a + 1]; ++k) {
+        sx += fineXY[2 * fine.perm[k]];
+        sy += fineXY[2 * fine.perm[k] + 1];
+      }
+      const double m = fine.aggPtr[a + 1] - fine.aggPtr[a];
+      c.xy[2 * a] = sx / m;
+      c.xy[2 * a + 1] = sy / m;
+    }
+  }
 }
 
 }  // namespace phase

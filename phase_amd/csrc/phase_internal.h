@@ -105,6 +105,9 @@ struct Ordering {
   int nBlocks = 0, nColours = 0;
   int regWidth = 8;                  // sweep kernels' per-row register width (4 when every count fits)
   std::vector<int> colourPtr;         // blocks of colour c: [colourPtr[c], colourPtr[c+1])
+  // aggregates (the rows a block packs, one aggregate at a time): aggregate a
+  // holds solver rows [aggPtr[a], aggPtr[a+1]), numbered in solver order
+  std::vector<int> aggPtr;
   std::vector<BlockInfo> blocks;
   std::vector<int> pred, succ;
   std::vector<RowInfo> rows;          // per solver row
@@ -126,6 +129,21 @@ struct Ordering {
 // a scalar equation on a 2-D mesh gets three-colour brick aggregates.
 void buildOrdering(Ordering &o, int n, const int *rowPtr, const int *colInd, int blockRows,
                    const double *cellXY = nullptr);
+
+// One coarser level of the aggregation multigrid hierarchy, built from the
+// aggregates of a finer level's ordering (unsmoothed aggregation: the
+// prolongation P has one 1 per row, in the column of the row's aggregate, and
+// the coarse matrix is the Galerkin product P^T A P).  Coarse row a is
+// aggregate a of the finer ordering.
+struct CoarseLevel {
+  int n = 0;
+  std::vector<int> rowPtr, colInd;   // coarse pattern (compact CSR, ascending columns)
+  std::vector<double> xy;            // aggregate centroids (interleaved), empty without finer centroids
+  std::vector<int> galPtr, galSrc;   // coarse slot q = sum of finer caller slots galSrc[galPtr[q] .. galPtr[q+1])
+  std::vector<int> aggOfRow;         // finer solver row -> coarse row
+};
+// fineXY: the finer level's centroids in its caller numbering, or null.
+void buildCoarse(const Ordering &fine, const double *fineXY, CoarseLevel &c);
 
 }  // namespace phase
 

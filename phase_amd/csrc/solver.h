@@ -20,6 +20,7 @@ struct KrylovVecs {
 
 struct MeshBinding;  // mesh.hip: link -> slot maps of an assembled operator
 struct GnnState;     // gnn.hip: learned correction network
+struct Multilevel;   // multilevel.hip: aggregation multigrid hierarchy
 
 }  // namespace phase
 
@@ -41,10 +42,16 @@ struct phase_hip_solver {
   int geometric = 1; // cell centroids known: brick aggregates in three colours (else graph aggregates)
   int cg = 0;        // Krylov method: 0 = BiCGStab, 1 = preconditioned CG (symmetric systems)
   int unionPattern = 0;  // keep the stored pattern when a new one lies inside it (its other entries 0)
+  int precond = 0;       // 0 = ILU(0), 1 = aggregation multigrid with ILU(0) smoothing
+  int mlKCycleSetting = -1;  // multigrid coarse solves: 1 = K cycle, 0 = V cycle, -1 = K for CG, V for BiCGStab
+  int mlKCycle = 0;          // the choice in force for the current solve
+  int mlCoarseRows = 16; // rows per aggregate below the finest level
+  int mlDenseRows = 256; // a level of at most this many rows is inverted densely
 
   // pattern and ordering
   int rank = 0;
   bool havePattern = false;
+  std::vector<double> cellXY;  // row centroids of the pattern (brick aggregates), empty when unknown
   std::vector<int> patRowPtr, patColInd;
   phase::Ordering ord;
 
@@ -86,6 +93,9 @@ struct phase_hip_solver {
   // pattern of the last equation handed over by phase_hip_solver_set_equation
   phase::DBuf<int> dEqCols, dEqFlag;
   int eqStride = 0;
+
+  // multigrid hierarchy of the current pattern (multilevel.hip)
+  phase::Multilevel *ml = nullptr;
 
   // mesh-assembled operator (mesh.hip)
   phase::MeshBinding *binding = nullptr;

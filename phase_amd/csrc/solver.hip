@@ -14,23 +14,16 @@
 #include <type_traits>
 
 #include "gnn.h"
+#include "multilevel.h"
 #include "solver.h"
 #include "solver_kernels.h"
+#include "solver_launch.h"
 
 using namespace phase;
 
 namespace {
 
 thread_local std::string gLastCreateError;
-
-inline int gridFor(long n, int block, int cap) {
-  long g = (n + block - 1) / block;
-  if (g < 1) g = 1;
-  return (int)std::min<long>(g, cap);
-}
-
-constexpr int kElemCap = 4096;
-constexpr int kRedCap = 1024;
 
 // Scratch of the allocation-free SpMV (phase_hip_solver_spmv_noalloc): the
 // operand and the product in solver order, kNoAllocMaxRank rows each, in FP64
@@ -74,175 +67,43 @@ int guarded(phase_hip_solver *s, F &&f) {
 }
 
 void setDevice(phase_hip_solver *s) { PH_CHECK(hipSetDevice(s->device)); }
-
-// entries of phase_hip_solver_time_kernels / phase_hip_solver_kernel_bytes
-constexpr int kTimedKernels = 16;
-
-template <class T>
-IluDev<T> iluDev(phase_hip_solver *s, T *fac) {
-  IluDev<T> L;
-  L.blocks = s->dBlocks.p;
-  L.rows = s->dRows.p;
-  L.lanes = s->dLanes.p;
-  L.lCol = s->dLCol.p;
-  L.uCol = s->dUCol.p;
-  L.fac = fac;
-  L.nb = s->ord.nBlocks;
-  L.nLSlots = s->ord.nLSlots;
-  L.nUSlots = s->ord.nUSlots;
-  L.n = s->rank;
-  return L;
-}
-
-// Scalars slot that is never set done: sweeps run outside a Krylov iteration
-const Scalars *idleScalars(phase_hip_solver *s) { return s->dSc.p + 3; }
-
-// Calls f with the ordering's sweep register width as a compile-time constant.
-template <class F>
-void byWidth(const phase_hip_solver *s, F &&f) {
-  if (s->ord.regWidth == 4) f(std::integral_constant<int, 4>());
-  else f(std::integral_constant<int, 8>());
-}
-
-// Sweep launchers: one launch per block colour (forward colours ascending,
-// backward descending).
-template <class T, int PRO>
-void launchL(phase_hip_solver *s, const IluDev<T> &L, const Scalars *sc, const T *in, T *y, T *pv, const T *r,
-             const T *v, double *xs = nullptr) {
-  byWidth(s, [&](auto Wc) {
-    constexpr int W = decltype(Wc)::value;
-    const std::vector<int> &cp = s->ord.colourPtr;
-    for (int c = 0; c + 1 < (int)cp.size(); ++c)
-      if (cp[c + 1] > cp[c])
-        hipLaunchKernelGGL((kLSolve<T, PRO, W>), dim3(cp[c + 1] - cp[c]), dim3(64), 0, s->stream, L, cp[c], sc, in,
-                           y, pv, r, v, xs);
-  });
-}
-
-template <class T>
-void launchU(phase_hip_solver *s, const IluDev<T> &L, const Scalars *sc, const T *y, T *x) {
-  byWidth(s, [&](auto Wc) {
-    constexpr int W = decltype(Wc)::value;
-    const std::vector<int> &cp = s->ord.colourPtr;
-    for (int c = (int)cp.size() - 2; c >= 0; --c)
-      if (cp[c + 1] > cp[c])
-        hipLaunchKernelGGL((kUSolve<T, W>), dim3(cp[c + 1] - cp[c]), dim3(64), 0, s->stream, L, cp[c], sc, y, x);
-  });
-}
-
-// out = U^{-1} L^{-1} (right-hand side formed by PRO); the last forward colour
-// and the first backward colour run fused in one launch.
-template <class T, int PRO>
-void launchLU(phase_hip_solver *s, const IluDev<T> &L, const Scalars *sc, const T *in, T *y, T *pv, const T *r,
-              const T *v, T *out, double *xs = nullptr) {
-  const std::vector<int> &cp = s->ord.colourPtr;
-  const int C = (int)cp.size() - 1;
-  if (C < 1) return;
-  byWidth(s, [&](auto Wc) {
-    constexpr int W = decltype(Wc)::value;
-    for (int c = 0; c + 1 < C; ++c)
-      if (cp[c + 1] > cp[c])
-        hipLaunchKernelGGL((kLSolve<T, PRO, W>), dim3(cp[c + 1] - cp[c]), dim3(64), 0, s->stream, L, cp[c], sc, in,
-                           y, pv, r, v, xs);
-    if (cp[C] > cp[C - 1])
-      hipLaunchKernelGGL((kLUSolve<T, PRO, W>), dim3(cp[C] - cp[C - 1]), dim3(64), 0, s->stream, L, cp[C - 1], sc,
-                         in, (const T *)y, pv, r, v, out, xs);
-    for (int c = C - 2; c >= 0; --c)
-      if (cp[c + 1] > cp[c])
-        hipLaunchKernelGGL((kUSolve<T, W>), dim3(cp[c + 1] - cp[c]), dim3(64), 0, s->stream, L, cp[c], sc,
-                           (const T *)y, out);
-  });
-}
-
-template <class T>
-void launchFactor(phase_hip_solver *s, const IluDev<T> &L) {
-  const std::vector<int> &cp = s->ord.colourPtr;
-  for (int c = 0; c + 1 < (int)cp.size(); ++c)
-    if (cp[c + 1] > cp[c])
-      hipLaunchKernelGGL((kIluFactor<T>), dim3(cp[c + 1] - cp[c]), dim3(64), 0, s->stream, L, s->dUpdPtr.p,
-                         s->dUpdSrc.p, s->dUpdDst.p, s->dErr.p, cp[c]);
-}
-
-int spmvGrid(phase_hip_solver *s) { return std::max(1, (s->ord.nSlices + 3) / 4); }
-
-
-void checkDeviceErr(phase_hip_solver *s) {
-  int e = 0;
-  PH_CHECK(hipMemcpyAsync(&e, s->dErr.p, sizeof(int), hipMemcpyDeviceToHost, s->stream));
-  PH_CHECK(hipStreamSynchronize(s->stream));
-  if (e & 2) {
-    PH_CHECK(hipMemsetAsync(s->dErr.p, 0, sizeof(int), s->stream));
-    fail(PHASE_HIP_EBREAKDOWN, "zero pivot in ILU(0) factorisation");
-  }
-}
-
-// fac = [L slots | U slots | diagonal (n) | inverse diagonal (n)]
-size_t facSize(const phase_hip_solver *s) { return (size_t)s->ord.nLSlots + s->ord.nUSlots + 2 * (size_t)s->rank; }
-
-template <class T>
-void factorize(phase_hip_solver *s, DBuf<T> &fac) {
-  const int m = (int)(fac.n - s->rank);  // gathered entries: L, U, diagonal
-  hipLaunchKernelGGL((kGather<T>), dim3(gridFor(m, 256, kElemCap)), dim3(256), 0, s->stream, m,
-                     s->dFacSrc.p, s->dVals.p, fac.p);
-  launchFactor<T>(s, iluDev<T>(s, fac.p));
-  hipLaunchKernelGGL((kInvDiag<T>), dim3(gridFor(s->rank, 256, kElemCap)), dim3(256), 0, s->stream, s->rank, fac.p,
-                     s->ord.nLSlots + s->ord.nUSlots);
-  PH_CHECK(hipGetLastError());
-}
-
-void ensureFactor64(phase_hip_solver *s) {
-  if (s->fac64Valid) return;
-  s->dFac64.alloc(facSize(s));
-  factorize<double>(s, s->dFac64);
-  checkDeviceErr(s);
-  s->fac64Valid = true;
-}
-
-}  // namespace
-
-namespace phase {
-void solverEnsureSingle(phase_hip_solver *s);
-}
-
-namespace {
-
-void ensureSingle(phase_hip_solver *s) {
-  if (!s->sell32Valid) {
-    s->dSell32.alloc(s->dSell64.n);
-    const int m = (int)s->dSell64.n;
-    hipLaunchKernelGGL((kConvert<double, float>), dim3(gridFor(m, 256, kElemCap)), dim3(256), 0, s->stream,
-                       m, s->dSell64.p, s->dSell32.p);
-    s->sell32Valid = true;
-  }
-  if (!s->fac32Valid) {
-    // factor in FP64, store and apply in FP32: rounding the finished factor
-    // keeps the FP64 pivots, where an FP32 factorisation of an ill-conditioned
-    // matrix loses them to cancellation
-    ensureFactor64(s);
-    const int m = (int)s->dFac64.n;
-    s->dFac32.alloc(m);
-    hipLaunchKernelGGL((kConvert<double, float>), dim3(gridFor(m, 256, kElemCap)), dim3(256), 0, s->stream, m,
-                       s->dFac64.p, s->dFac32.p);
-    PH_CHECK(hipGetLastError());
-    s->fac32Valid = true;
-  }
-}
-
 template <class T>
 void iluApply(phase_hip_solver *s, DBuf<T> &fac, const T *in, T *tmp, T *out, const Scalars *sc) {
   IluDev<T> L = iluDev<T>(s, fac.p);
   launchLU<T, kProNone>(s, L, sc, in, tmp, (T *)nullptr, (const T *)nullptr, (const T *)nullptr, out);
 }
 
-// rho' = (r, z), beta = rho' / rho ; p = z + beta p  (z in ph)
+// The multigrid preconditioner is in use for this solve.
+bool useMl(const phase_hip_solver *s) { return s->precond == 1 && s->ml; }
+// Its coarse solves are K cycles: the preconditioner varies from apply to
+// apply, and CG runs in its flexible form.
+bool flexible(const phase_hip_solver *s) { return useMl(s) && s->mlKCycle == 1; }
+
+// Rest of a preconditioner apply whose fine pre-smoothing (the sweep that
+// may also run a Krylov prologue) has written M^-1 rhs into out.
+template <class T>
+void finishApply(phase_hip_solver *s, const Scalars *sc, const T *rhs, T *out) {
+  if (useMl(s)) mlFinishApply<T>(s, sc, rhs, out);
+}
+
+// rho' = (r, z), beta ; p = z + beta p  (z in ph)
 template <class T>
 void cgBeta(phase_hip_solver *s, KrylovVecs<T> &K, Scalars *sc) {
   const int n = s->rank;
   const int gu = gridFor(n, kRedBlock, kRedCap);
-  hipLaunchKernelGGL((kDotRZ<T>), dim3(gu), dim3(kRedBlock), 0, s->stream, n, sc, K.r.p, K.ph.p, s->dPartials.p);
   // each direction block re-reads the gu partials: keep the grid near gu
-  hipLaunchKernelGGL((kCgDirectionBeta<T>), dim3(gridFor(n, 4 * kRedBlock, kRedCap)), dim3(kRedBlock), 0, s->stream,
-                     n, sc, s->dPartials.p, gu, K.ph.p, K.p.p);
+  const int gd = gridFor(n, 4 * kRedBlock, kRedCap);
+  if (flexible(s)) {
+    hipLaunchKernelGGL((kDotRZ<T, true>), dim3(gu), dim3(kRedBlock), 0, s->stream, n, sc, K.r.p, K.ph.p, K.v.p,
+                       s->dPartials.p);
+    hipLaunchKernelGGL((kCgDirectionBeta<T, true>), dim3(gd), dim3(kRedBlock), 0, s->stream, n, sc, s->dPartials.p,
+                       gu, K.ph.p, K.p.p);
+  } else {
+    hipLaunchKernelGGL((kDotRZ<T>), dim3(gu), dim3(kRedBlock), 0, s->stream, n, sc, K.r.p, K.ph.p, K.v.p,
+                       s->dPartials.p);
+    hipLaunchKernelGGL((kCgDirectionBeta<T>), dim3(gd), dim3(kRedBlock), 0, s->stream, n, sc, s->dPartials.p, gu,
+                       K.ph.p, K.p.p);
+  }
 }
 
 // Start of the CG recurrence from r (p is zero): z = M^{-1} r, p = z.
@@ -250,6 +111,7 @@ template <class T>
 void cgStart(phase_hip_solver *s, const IluDev<T> &L, KrylovVecs<T> &K, Scalars *sc) {
   launchLU<T, kProNone>(s, L, sc, (const T *)K.r.p, K.y.p, (T *)nullptr, (const T *)nullptr, (const T *)nullptr,
                         K.ph.p);
+  finishApply<T>(s, sc, K.r.p, K.ph.p);
   cgBeta<T>(s, K, sc);
 }
 
@@ -260,6 +122,7 @@ void iterate(phase_hip_solver *s, KrylovVecs<T> &K, DBuf<T> &fac, const TV *sell
   IluDev<T> L = iluDev<T>(s, fac.p);
   // p = r + beta (p - omega v) ; ph = M^{-1} p
   launchLU<T, kProP>(s, L, sc, (const T *)nullptr, K.y.p, K.p.p, K.r.p, K.v.p, K.ph.p);
+  finishApply<T>(s, sc, K.p.p, K.ph.p);
   // v = A ph ; alpha
   const int gs = spmvGrid(s);
   hipLaunchKernelGGL((kSpmvDotV<T, TV>), dim3(gs), dim3(kRedBlock), 0, s->stream, n, s->ord.nSlices, s->dSliceOff.p,
@@ -267,6 +130,7 @@ void iterate(phase_hip_solver *s, KrylovVecs<T> &K, DBuf<T> &fac, const TV *sell
   hipLaunchKernelGGL(kFinishV, dim3(1), dim3(kFinBlock), 0, s->stream, sc, s->dPartials.p, gs);
   // s = r - alpha v ; sh = M^{-1} s
   launchLU<T, kProS>(s, L, sc, (const T *)nullptr, K.y.p, K.s.p, K.r.p, K.v.p, K.sh.p);
+  finishApply<T>(s, sc, K.s.p, K.sh.p);
   // t = A sh ; omega
   hipLaunchKernelGGL((kSpmvDotT<T, TV>), dim3(gs), dim3(kRedBlock), 0, s->stream, n, s->ord.nSlices, s->dSliceOff.p,
                      s->dSellCol.p, sell, K.sh.p, K.t.p, K.s.p, sc, s->dPartials.p);
@@ -290,6 +154,7 @@ void iterateCg(phase_hip_solver *s, KrylovVecs<T> &K, DBuf<T> &fac, const TV *se
                      s->dSliceOff.p, s->dSellCol.p, sell, K.p.p, K.v.p, K.r.p, sc, s->dPartials.p);
   hipLaunchKernelGGL(kFinishCgQ, dim3(1), dim3(kFinBlock), 0, s->stream, sc, s->dPartials.p, gs);
   launchLU<T, kProCgR>(s, L, sc, (const T *)K.p.p, K.y.p, K.r.p, (const T *)nullptr, (const T *)K.v.p, K.ph.p, x);
+  finishApply<T>(s, sc, K.r.p, K.ph.p);
   cgBeta<T>(s, K, sc);
 }
 
@@ -424,8 +289,11 @@ bool learnedStep(phase_hip_solver *s, double r2) {
 // the FP64 true residual b - A x (a reliable update: FP64 iterative refinement
 // that keeps the Krylov recurrence).  Convergence is judged on the FP64 true
 // residual only.  The iterate with the smallest true residual is kept; if the
-// true residual grows a hundredfold past it or stops being finite, the solve
-// falls back to FP64 BiCGStab from that iterate.
+// true residual grows a hundredfold past it, stops being finite, or fails to
+// halve over kStallPasses refinement points in a row, or the FP32 recursive
+// residual makes no new low for twice the longest stretch between refinement
+// points so far (at least kStallMinIters iterations; FP32 stagnation), the
+// solve falls back to FP64 from that iterate.
 void solveMixed(phase_hip_solver *s) {
   const int n = s->rank;
   ensureSingle(s);
@@ -448,6 +316,8 @@ void solveMixed(phase_hip_solver *s) {
   const double delta2 = s->innerTol * s->innerTol;
   double bestRR = HUGE_VAL;
   bool diverged = false;
+  constexpr int kStallPasses = 4, kStallMinIters = 800;
+  int stall = 0, longestGap = 0;
   // true residual into dR64 / outer->rr; keeps the best iterate
   auto trueResidual = [&]() {
     hipLaunchKernelGGL((kResidual<double>), dim3(spmvGrid(s)), dim3(kRedBlock), 0, s->stream, n,
@@ -458,6 +328,11 @@ void solveMixed(phase_hip_solver *s) {
     pullScalars(s);
     const double rr = s->hSc[1].rr;
     if (!(rr <= 1e4 * bestRR)) {  // NaN included
+      diverged = true;
+      return rr;
+    }
+    stall = rr < 0.25 * bestRR ? 0 : stall + 1;  // ||r|| at least halved, or not
+    if (stall >= kStallPasses) {
       diverged = true;
       return rr;
     }
@@ -492,16 +367,30 @@ void solveMixed(phase_hip_solver *s) {
                        K.p.p, K.v.p, (float *)nullptr, tol2, s->maxIters - total, 0.0);
     if (s->cg) cgStart<float>(s, iluDev<float>(s, s->dFac32.p), K, inner);
     pullScalars(s);
-    double lastRU = s->hSc[0].rr;
+    double lastRU = s->hSc[0].rr, minRU = lastRU;
+    int lastUpdate = 0, minAt = 0;
     int it = runKrylov<float, double, double>(s, K, s->dFac32, s->dSell64.p, s->dX.p, s->maxIters - total,
-                                              [&](int) {
-      if (!(s->hSc[0].rr < delta2 * lastRU)) return true;
+                                              [&](int sofar) {
+      if (!(s->hSc[0].rr < delta2 * lastRU)) {
+        // the FP32 recurrence makes no new low for a long stretch before the
+        // next refinement point: stagnation, as when the true residual stops halving
+        if (s->hSc[0].rr < minRU) {
+          minRU = s->hSc[0].rr;
+          minAt = sofar;
+        } else if (sofar - minAt > std::max(kStallMinIters, 2 * longestGap)) {
+          diverged = true;
+          return false;
+        }
+        return true;
+      }
+      longestGap = std::max(longestGap, sofar - lastUpdate);
+      lastUpdate = minAt = sofar;
       // reliable update: r = b - A x in FP64, handed to the FP32 recurrence
       trueResidual();
       if (diverged) return false;
       hipLaunchKernelGGL((kReplaceResidual<float>), dim3(gridFor(n, 256, kElemCap)), dim3(256), 0, s->stream,
                          n, s->dR64.p, K.r.p, inner, outer);
-      lastRU = s->hSc[1].rr;
+      lastRU = minRU = s->hSc[1].rr;
       ++s->outerPasses;
       return true;
     });
@@ -550,6 +439,9 @@ void solverSetPattern(phase_hip_solver *s, int n, const int *rowPtr, const int *
   setDevice(s);
   if (s->rank != 0 && n != s->rank) fail(PHASE_HIP_EINVAL, "pattern size differs from setRank");
   buildOrdering(s->ord, n, rowPtr, colInd, s->blockRows, s->geometric ? cellXY : nullptr);
+  mlFree(s);
+  if (cellXY && s->geometric) s->cellXY.assign(cellXY, cellXY + 2 * (size_t)n);
+  else s->cellXY.clear();
   s->rank = n;
   const Ordering &o = s->ord;
   hipStream_t st = s->stream;
@@ -762,6 +654,7 @@ void solverValuesChanged(phase_hip_solver *s) {
   PH_CHECK(hipGetLastError());
   s->haveValues = true;
   s->fac64Valid = s->fac32Valid = s->sell32Valid = false;
+  mlValuesChanged(s);
 }
 
 }  // namespace phase
@@ -816,6 +709,7 @@ int phase_hip_solver_destroy(phase_hip_solver *s) {
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   solverFreeBinding(s);
   gnnFree(s);
+  mlFree(s);
   s->dBlocks.release();
   if (s->hSc) (void)hipHostFree(s->hSc);
   if (s->ev0) (void)hipEventDestroy(s->ev0);
@@ -858,6 +752,24 @@ int phase_hip_solver_setup(phase_hip_solver *s, const char *key, const char *val
       if (g2 < 0) fail(PHASE_HIP_EINVAL, "unknown ordering '" + v + "' (geometric or graph)");
       if (g2 != s->geometric && s->havePattern) s->havePattern = false;  // rebuild on next set
       s->geometric = g2;
+    } else if (k == "preconditioner") {
+      int pc = lv == "ilu" || lv == "riluk" || lv == "ilu0" ? 0 : lv == "multigrid" || lv == "amg" ? 1 : -1;
+      if (pc < 0) fail(PHASE_HIP_EINVAL, "unknown preconditioner '" + v + "' (ILU or MULTIGRID)");
+      s->precond = pc;
+    } else if (k == "multigridCycle") {
+      int kc = lv == "k" ? 1 : lv == "v" ? 0 : lv == "auto" ? -1 : -2;
+      if (kc < -1) fail(PHASE_HIP_EINVAL, "unknown multigrid cycle '" + v + "' (K, V or auto)");
+      s->mlKCycleSetting = kc;
+    } else if (k == "multigridCoarseRows") {
+      const int r = std::stoi(v);
+      if (r < 2 || r > 64) fail(PHASE_HIP_EINVAL, "multigridCoarseRows must be in [2,64]");
+      if (r != s->mlCoarseRows) mlFree(s);
+      s->mlCoarseRows = r;
+    } else if (k == "multigridDenseRows") {
+      const int r = std::stoi(v);
+      if (r < 1 || r > 512) fail(PHASE_HIP_EINVAL, "multigridDenseRows must be in [1,512]");
+      if (r != s->mlDenseRows) mlFree(s);
+      s->mlDenseRows = r;
     } else if (k == "pattern") {
       if (lv == "exact") s->unionPattern = 0;
       else if (lv == "union") s->unionPattern = 1;
@@ -973,6 +885,12 @@ int phase_hip_solver_solve(phase_hip_solver *s, double *relres) {
     // the previous solution is the initial guess unless setGuess() gave one.
     if (!s->haveGuess && !s->haveSolution) PH_CHECK(hipMemsetAsync(s->dX.p, 0, s->rank * sizeof(double), s->stream));
     PH_CHECK(hipEventRecord(s->ev0, s->stream));
+    if (s->precond == 1) {
+      // hierarchy once per pattern, coarse matrices and factors once per values
+      mlEnsureHierarchy(s);
+      mlEnsureValues(s, s->mode != PHASE_HIP_MODE_FP64);
+      s->mlKCycle = s->mlKCycleSetting >= 0 ? s->mlKCycleSetting : (s->cg ? 1 : 0);
+    }
     s->fallbacks = 0;
     s->learnedSteps = 0;
     if (s->mode == PHASE_HIP_MODE_FP64) solveFp64(s);
@@ -1047,6 +965,15 @@ int phase_hip_solver_ordering_info(const phase_hip_solver *s, int *nBlocks, int 
   if (!s || !s->havePattern) return PHASE_HIP_ESTATE;
   if (nBlocks) *nBlocks = s->ord.nBlocks;
   if (nColours) *nColours = s->ord.nColours;
+  return PHASE_HIP_OK;
+}
+
+int phase_hip_solver_multigrid_info(const phase_hip_solver *s, int *nLevels, int *sizes, int cap) {
+  if (!s) return PHASE_HIP_EINVAL;
+  const std::vector<int> v = mlLevelSizes(s);
+  if (nLevels) *nLevels = (int)v.size();
+  if (sizes)
+    for (int i = 0; i < std::min(cap, (int)v.size()); ++i) sizes[i] = v[i];
   return PHASE_HIP_OK;
 }
 
@@ -1252,6 +1179,21 @@ void kernelBytes(const Ordering &o, double *out) {
       o3[1] = sweep(b0, o.nBlocks, sz, false) + sweep(b0, o.nBlocks, sz, true) -
               (o.nBlocks - b0) * (sizeof(BlockInfo) + 64.0 * sizeof(LaneRec)) - 2 * rows * sz;
     }
+    // the CG forms (prologue kProCgR: r read and written, q read, x read and
+    // written in FP64, on top of the plain sweep), FP64 in 16..17, FP32 in 18..19:
+    // [L launch, fused LU launch]
+    double *oc = sz == 8 ? out + 16 : out + 18;
+    oc[0] = oc[1] = 0.;
+    if (C >= 2) {
+      double rows = 0.;
+      for (int b = 0; b < o.colourPtr[C - 1]; ++b) rows += o.blocks[b].nRows;
+      oc[0] = o3[2] + rows / (C - 1) * (3 * sz + 16);
+    }
+    if (C >= 1) {
+      double rows = 0.;
+      for (int b = o.colourPtr[C - 1]; b < o.nBlocks; ++b) rows += o.blocks[b].nRows;
+      oc[1] = o3[1] + rows * (3 * sz + 16);
+    }
   }
 }
 
@@ -1315,12 +1257,19 @@ int phase_hip_solver_time_kernels(phase_hip_solver *s, int reps, double *ms, int
     // in 8..10, FP32 in 13..15
     const std::vector<int> &cp = s->ord.colourPtr;
     const int C = (int)cp.size() - 1;
-    auto perLaunch = [&](auto *fac, auto &K, double *o3) {
+    s->dR64.alloc(n);
+    PH_CHECK(hipMemcpyAsync(s->dR64.p, s->dB.p, n * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+    auto perLaunch = [&](auto *fac, auto &K, double *o3, double *oc) {
       using T = std::remove_pointer_t<decltype(fac)>;
       const IluDev<T> L = iluDev<T>(s, fac);
       const T *in = K.ph.p;
       T *y = K.y.p, *x = K.sh.p;
       o3[0] = o3[1] = o3[2] = 0.;
+      oc[0] = oc[1] = 0.;
+      // the CG forms read p, r, q and x (alpha = 0 on the idle scalars: nothing moves)
+      for (T *v : {K.p.p, K.r.p, K.v.p})
+        hipLaunchKernelGGL((kConvert<double, T>), dim3(gridFor(n, 256, kElemCap)), dim3(256), 0, s->stream, n,
+                           s->dB.p, v);
       byWidth(s, [&](auto Wc) {
         constexpr int W = decltype(Wc)::value;
         if (C >= 2) {
@@ -1342,11 +1291,24 @@ int phase_hip_solver_time_kernels(phase_hip_solver *s, int reps, double *ms, int
                                cp[C - 1], noSc, in, (const T *)y, (T *)nullptr, (const T *)nullptr,
                                (const T *)nullptr, x, (double *)nullptr);
           });
+        if (C >= 2)
+          oc[0] = timeIt([&] {
+                    for (int c = 0; c + 1 < C; ++c)
+                      hipLaunchKernelGGL((kLSolve<T, kProCgR, W>), dim3(cp[c + 1] - cp[c]), dim3(64), 0,
+                                         s->stream, L, cp[c], noSc, (const T *)K.p.p, y, K.r.p, (const T *)nullptr,
+                                         (const T *)K.v.p, s->dR64.p);
+                  }) / (C - 1);
+        if (C >= 1)
+          oc[1] = timeIt([&] {
+            hipLaunchKernelGGL((kLUSolve<T, kProCgR, W>), dim3(cp[C] - cp[C - 1]), dim3(64), 0, s->stream, L,
+                               cp[C - 1], noSc, (const T *)K.p.p, (const T *)y, K.r.p, (const T *)nullptr,
+                               (const T *)K.v.p, x, s->dR64.p);
+          });
       });
     };
-    perLaunch(s->dFac64.p, s->k64, out + 8);
+    perLaunch(s->dFac64.p, s->k64, out + 8, out + 16);
     ensureSingle(s);  // the FP32 copy of the FP64 factor, as mixed solves use it
-    perLaunch(s->dFac32.p, s->k32, out + 13);
+    perLaunch(s->dFac32.p, s->k32, out + 13, out + 18);
     // cache-cold SpMV: before each timed launch a 1 GiB copy evicts the matrix
     // and vectors from L2 and from the 256 MB last-level cache, and the events
     // bracket the one launch

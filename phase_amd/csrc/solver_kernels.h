@@ -492,35 +492,53 @@ static __global__ __launch_bounds__(kFinBlock) void kFinishCgQ(Scalars *sc, cons
   }
 }
 
-// partials of (r, z)
-template <class T>
+// partials of (r, z), and with FLEX also of (z, v): v still holds A p of the
+// previous direction
+template <class T, bool FLEX = false>
 __global__ __launch_bounds__(kRedBlock) void kDotRZ(int n, const Scalars *sc, const T *__restrict__ r,
-                                                    const T *__restrict__ z, double *partials) {
+                                                    const T *__restrict__ z, const T *__restrict__ v,
+                                                    double *partials) {
   if (sc->done) return;
-  double d[1] = {0.};
-  for (int i = blockIdx.x * kRedBlock + threadIdx.x; i < n; i += gridDim.x * kRedBlock)
+  double d[FLEX ? 2 : 1];
+  d[0] = 0.;
+  if (FLEX) d[FLEX ? 1 : 0] = 0.;
+  for (int i = blockIdx.x * kRedBlock + threadIdx.x; i < n; i += gridDim.x * kRedBlock) {
     d[0] += (double)r[i] * (double)z[i];
-  blockPartials<1>(d, partials, blockIdx.x);
+    if (FLEX) d[FLEX ? 1 : 0] += (double)z[i] * (double)v[i];
+  }
+  blockPartials<FLEX ? 2 : 1>(d, partials, blockIdx.x);
 }
 
-// rho' = (r, z), beta = rho' / rho and p = z + beta p in one launch: every
-// block adds up the (r, z) partials itself, in the same fixed order, so all
-// blocks hold the same beta; block 0 records beta and rho' (as rhoNext: rho
-// itself is still being read by the other blocks; the next alpha takes it
-// over).  At the start p is zero.
-template <class T>
+// rho' = (r, z), beta and p = z + beta p in one launch: every block adds up
+// the partials itself, in the same fixed order, so all blocks hold the same
+// beta; block 0 records beta and rho' (as rhoNext: rho itself is still being
+// read by the other blocks; the next alpha takes it over).  At the start p
+// and v are zero.  Plain CG: beta = rho' / rho.  Flexible CG (a preconditioner
+// that varies from apply to apply, Notay 2000): beta = -(z, A p) / (p, A p)
+// with (p, A p) from the last alpha (sc->rv), which keeps p A-orthogonal to the
+// previous direction.
+template <class T, bool FLEX = false>
 __global__ __launch_bounds__(kRedBlock) void kCgDirectionBeta(int n, Scalars *sc, const double *partials, int nb,
                                                               const T *__restrict__ z, T *__restrict__ p) {
   if (sc->done) return;
-  double d[1];
-  finishPartials<1>(partials, nb, d);
+  double d[FLEX ? 2 : 1];
+  finishPartials<FLEX ? 2 : 1>(partials, nb, d);
   __shared__ double sBeta;
   __shared__ int sBad;
   if (threadIdx.x == 0) {
     const double rhoNew = d[0], rho = sc->rho;
-    const bool bad = !isfinite(rhoNew) || rhoNew == 0. || rho == 0.;
+    double beta;
+    bool bad;
+    if (FLEX) {
+      const double zq = d[FLEX ? 1 : 0], pq = sc->rv;
+      beta = zq == 0. ? 0. : -zq / pq;
+      bad = !isfinite(rhoNew) || rhoNew == 0. || !isfinite(beta);
+    } else {
+      bad = !isfinite(rhoNew) || rhoNew == 0. || rho == 0.;
+      beta = bad ? 0. : rhoNew / rho;
+    }
     sBad = bad;
-    sBeta = bad ? 0. : rhoNew / rho;
+    sBeta = bad ? 0. : beta;
     if (blockIdx.x == 0) {
       if (bad) {
         sc->done = kDoneBreakdown;
@@ -749,6 +767,13 @@ __device__ __forceinline__ T lBody(const IluDev<T> &L, const BlockInfo &B, bool 
 #pragma unroll
       for (int q = 0; q < W; ++q)
         if (q < nIn) acc = fma(-lv[q], ylds[lc[q] - B.rowStart], acc);
+      if constexpr (W == 8) {
+        // rows with more in-block entries than registers (coarse multigrid levels)
+        for (int e = W; e < nIn; ++e) {
+          const int s = B.lOff + ((nOff + e) << 6) + lane;
+          acc = fma(-L.fac[s], ylds[L.lCol[s] - B.rowStart], acc);
+        }
+      }
       ylds[lane] = acc;
     }
     __syncthreads();
@@ -817,11 +842,25 @@ __device__ __forceinline__ T uSolve(const IluDev<T> &L, const BlockInfo &B, bool
 #pragma unroll
   for (int q = 0; q < W; ++q)
     if (q < u.nOff) soff = fma(u.ou[q], ox[q], soff);
+  const T *facU = L.fac + L.nLSlots;
+  if constexpr (W == 8) {
+    // rows with more off-block entries than registers (coarse multigrid levels)
+    for (int e = W; e < u.nOff; ++e) {
+      const int s = B.uOff + ((u.nIn + e) << 6) + lane;
+      soff = fma(facU[s], x[L.uCol[s]], soff);
+    }
+  }
   for (int l = 0; l < B.nULev; ++l) {
     if (act && u.ulev == l) {
 #pragma unroll
       for (int q = 0; q < W; ++q)
         if (q < u.nIn) acc = fma(-u.uv[q], xlds[u.uc[q] - B.rowStart], acc);
+      if constexpr (W == 8) {
+        for (int e = W; e < u.nIn; ++e) {
+          const int s = B.uOff + (e << 6) + lane;
+          acc = fma(-facU[s], xlds[L.uCol[s] - B.rowStart], acc);
+        }
+      }
       acc = (acc - soff) * u.d;
       xlds[lane] = acc;
     }

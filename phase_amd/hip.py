@@ -46,6 +46,7 @@ _SIGS = {
     "phase_hip_solver_stats": (ctypes.c_int, [_vp, _ip, _ip, _ip, _dp]),
     "phase_hip_solver_export": (ctypes.c_int, [_vp, _ip, _vp, _vp, _vp, _vp]),
     "phase_hip_solver_spmv": (ctypes.c_int, [_vp, ctypes.c_int, _f64p, _f64p]),
+    "phase_hip_solver_multigrid_info": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int), _vp, ctypes.c_int]),
     "phase_hip_solver_spmv_noalloc": (ctypes.c_int, [_vp, ctypes.c_int, _f64p, _f64p]),
     "phase_hip_solver_ilu_export": (ctypes.c_int, [_vp, _ip, _vp, _vp, _vp, _vp]),
     "phase_hip_solver_time_kernels": (ctypes.c_int, [_vp, ctypes.c_int, _f64p, ctypes.c_int]),
@@ -846,6 +847,14 @@ class HipSparseMatrixSolver:
                     "export")
         return rp, ci, v, rhs
 
+    def multigrid_levels(self):
+        """Rows of each multigrid level, finest first (empty before a multigrid solve)."""
+        n = ctypes.c_int(0)
+        sizes = np.zeros(32, np.int32)
+        self._check(load_library().phase_hip_solver_multigrid_info(self._h, ctypes.byref(n), _p(sizes), 32),
+                    "multigrid_info")
+        return sizes[:n.value].tolist()
+
     def spmv(self, x, single=False):
         y = np.empty(self.rank)
         self._check(load_library().phase_hip_solver_spmv(self._h, int(bool(single)), _f64(x), y), "spmv")
@@ -864,7 +873,8 @@ class HipSparseMatrixSolver:
 
     KERNEL_NAMES = ("spmv64", "spmv32", "lsweep64", "usweep64", "lsweep32", "usweep32", "factor64", "factor32",
                     "usolve64_launch", "lusolve64_launch", "lsolve64_launch", "spmv64_cold", "spmv32_cold",
-                    "usolve32_launch", "lusolve32_launch", "lsolve32_launch")
+                    "usolve32_launch", "lusolve32_launch", "lsolve32_launch", "lsolve64cg_launch",
+                    "lusolve64cg_launch", "lsolve32cg_launch", "lusolve32cg_launch")
 
     def time_kernels(self, reps=20):
         """Mean device ms of each timed kernel or sweep (HIP events)."""

@@ -7,6 +7,7 @@
 // its bounds and every dependency against the schedule (a predecessor block
 // must already be finished, an in-block dependency must sit at a lower level).
 // The results are compared with the oracle by tests/test_ordering_emul.py.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -51,6 +52,9 @@ extern "C" int emul_run(int n, const int *rowPtr, const int *colInd, const doubl
     }
     info[4] = maxLev;
     info[5] = maxULev;
+    int maxIn = 0;  // widest in-block L / in-block U / off-block U count of a row
+    for (const RowInfo &r : o.rows) maxIn = std::max({maxIn, r.nL - r.nLoff, (int)r.nUin, r.nU - r.nUin});
+    info[6] = maxIn;
     for (int k = 0; k < n; ++k) perm[k] = o.perm[k];
     // --- value gathers (kGather)
     const size_t nSell = o.sellCol.size();
@@ -234,7 +238,7 @@ extern "C" int emul_run(int n, const int *rowPtr, const int *colInd, const doubl
         const RowInfo &ri = o.rows[row];
         acc[lane] = y[row];
         // later-block terms: one sum, formed before the levels (kUSolve)
-        REQ(ri.nU - ri.nUin <= 8, "too many off-block U entries");
+        REQ(ri.nU - ri.nUin <= 15, "too many off-block U entries");
         for (int e = ri.nUin; e < ri.nU; ++e) {
           const int s = uslot(B, e, lane);
           REQ(o.uCol[s] >= B.rowStart + B.nRows && rowDone[o.uCol[s]], "bwd off-block value not ready");
@@ -265,5 +269,67 @@ extern "C" int emul_run(int n, const int *rowPtr, const int *colInd, const doubl
   } catch (const Bad &e) {
     std::snprintf(err, errLen, "%s", e.msg.c_str());
     return 1;
+  }
+}
+
+// Aggregation hierarchy of the multigrid preconditioner, built with the
+// product's host code (buildOrdering per level, buildCoarse between levels)
+// until a level has at most denseMax rows.  Writes, per level l < last: n,
+// perm[n], the aggregate count na, aggPtr[na + 1], then the coarse pattern
+// rowPtr[na + 1], nnz, colInd[nnz], galPtr[nnz + 1], galSrc[galPtr[nnz]];
+// the file ends with the last level's n and a 0.  All int32.
+extern "C" int emul_hierarchy(int n, const int *rowPtr, const int *colInd, const double *cellXY, int fineRows,
+                              int coarseRows, int denseMax, const char *outPath, char *err, int errLen) {
+  try {
+    FILE *f = std::fopen(outPath, "wb");
+    REQ(f != nullptr, "cannot open output");
+    auto put = [&](const int *p, size_t k) { std::fwrite(p, sizeof(int), k, f); };
+    std::vector<int> rp(rowPtr, rowPtr + n + 1), ci(colInd, colInd + rowPtr[n]);
+    std::vector<double> xy;
+    if (cellXY) xy.assign(cellXY, cellXY + 2 * (size_t)n);
+    int level = 0;
+    while (n > denseMax) {
+      Ordering o;
+      CoarseLevel c;
+      try {
+        buildOrdering(o, n, rp.data(), ci.data(), level == 0 ? fineRows : coarseRows, xy.empty() ? nullptr : xy.data());
+        buildCoarse(o, xy.empty() ? nullptr : xy.data(), c);
+      } catch (const HipError &e) {
+        std::fclose(f);
+        throw Bad{"level " + std::to_string(level) + ": " + e.msg};
+      }
+      REQ(c.n < n, "aggregation did not coarsen");
+      // every finer compact entry lands in exactly one coarse slot
+      std::vector<int> seen(rp[n], 0);
+      for (int s : c.galSrc) {
+        REQ(s >= 0 && s < rp[n] && ci[s] >= 0, "Galerkin source out of range");
+        ++seen[s];
+      }
+      for (int s = 0; s < rp[n]; ++s) REQ(seen[s] == (ci[s] >= 0 ? 1 : 0), "Galerkin map does not cover the entries once");
+      put(&n, 1);
+      put(o.perm.data(), n);
+      const int na = c.n;
+      put(&na, 1);
+      put(o.aggPtr.data(), na + 1);
+      put(c.rowPtr.data(), na + 1);
+      const int nnz = (int)c.colInd.size();
+      put(&nnz, 1);
+      put(c.colInd.data(), nnz);
+      put(c.galPtr.data(), nnz + 1);
+      put(c.galSrc.data(), c.galSrc.size());
+      n = na;
+      rp = c.rowPtr;
+      ci = c.colInd;
+      xy = c.xy;
+      ++level;
+    }
+    put(&n, 1);
+    const int zero = 0;
+    put(&zero, 1);
+    std::fclose(f);
+    return level;
+  } catch (const Bad &b) {
+    std::snprintf(err, errLen, "%s", b.msg.c_str());
+    return -1;
   }
 }

@@ -137,3 +137,89 @@ def test_emul_rejects_out_of_range(emul):
     rc = emul.emul_run(n, rp, ci, v, 64, perm, frp, fci, fv, 16, z, z.copy(), z.copy(), z, z.copy(), info, err, 256,
                        None)
     assert rc == 1 and b"out of range" in err.value
+
+
+def _hierarchy(emul, rp, ci, xy, fine_rows, coarse_rows, dense_max, path):
+    n = len(rp) - 1
+    err = ctypes.create_string_buffer(256)
+    xyp = None if xy is None else np.ascontiguousarray(xy, np.float64).ctypes.data_as(ctypes.c_void_p)
+    emul.emul_hierarchy.restype = ctypes.c_int
+    rc = emul.emul_hierarchy(ctypes.c_int(n), np.ascontiguousarray(rp, np.int32).ctypes.data_as(ctypes.c_void_p),
+                             np.ascontiguousarray(ci, np.int32).ctypes.data_as(ctypes.c_void_p), xyp,
+                             ctypes.c_int(fine_rows), ctypes.c_int(coarse_rows), ctypes.c_int(dense_max),
+                             str(path).encode(), err, ctypes.c_int(256))
+    assert rc >= 0, err.value.decode()
+    d = np.fromfile(str(path), np.int32)
+    o, levels = 0, []
+    for _ in range(rc):
+        n = d[o]; o += 1
+        perm = d[o:o + n]; o += n
+        na = d[o]; o += 1
+        agg_ptr = d[o:o + na + 1]; o += na + 1
+        crp = d[o:o + na + 1]; o += na + 1
+        nnz = d[o]; o += 1
+        cci = d[o:o + nnz]; o += nnz
+        gp = d[o:o + nnz + 1]; o += nnz + 1
+        gs = d[o:o + gp[-1]]; o += gp[-1]
+        levels.append(dict(n=n, perm=perm, aggPtr=agg_ptr, crp=crp, cci=cci, gp=gp, gs=gs))
+    last = d[o]
+    return levels, last
+
+
+@pytest.mark.parametrize("geometric", [True, False])
+def test_emul_multigrid_hierarchy_is_galerkin(oracle, emul, tmp_path, geometric):
+    """The multigrid hierarchy the product builds (ordering aggregates, then
+    buildCoarse per level): every level coarsens, the last is within the dense
+    limit, aggregates are contiguous solver-row ranges covering every row once,
+    and the coarse matrices summed through the Galerkin lists equal P^T A P
+    with P the aggregate indicator (computed here with scipy)."""
+    import scipy.sparse as sp
+
+    xy, cptr, cind = H.delaunay_case(20000, 64, seed=12)
+    m, (rp, ci, v), b, g, bc = H.pressure_system(oracle, xy, cptr, cind, sigma=1.0)
+    _, cc = m.cells()
+    levels, last = _hierarchy(emul, rp, ci, cc if geometric else None, 32, 16, 64, tmp_path / "h.bin")
+    assert len(levels) >= 2 and last <= 64
+    vals, lrp, lci = v, rp, ci
+    for L in levels:
+        n = L["n"]
+        perm = L["perm"]
+        assert sorted(perm.tolist()) == list(range(n))
+        ap = L["aggPtr"]
+        assert ap[0] == 0 and ap[-1] == n and np.all(np.diff(ap) > 0)
+        na = len(ap) - 1
+        assert na < n
+        iperm = np.empty(n, np.int64)
+        iperm[perm] = np.arange(n)
+        agg_of_solver_row = np.repeat(np.arange(na), np.diff(ap))
+        P = sp.csr_matrix((np.ones(n), (np.arange(n), agg_of_solver_row[iperm])), shape=(n, na))
+        rows = np.repeat(np.arange(n), np.diff(lrp))
+        keep = lci >= 0
+        A = sp.csr_matrix((vals[keep], (rows[keep], lci[keep])), shape=(n, n))
+        Ac = (P.T @ A @ P).tocsr()
+        Ac.sort_indices()
+        cvals = np.array([vals[L["gs"][L["gp"][q]:L["gp"][q + 1]]].sum() for q in range(len(L["cci"]))])
+        got = sp.csr_matrix((cvals, L["cci"], L["crp"]), shape=(na, na))
+        got.sort_indices()
+        assert np.array_equal(got.indptr, Ac.indptr) and np.array_equal(got.indices, Ac.indices)
+        assert np.allclose(got.data, Ac.data, rtol=1e-12, atol=1e-12 * np.abs(Ac.data).max())
+        vals, lrp, lci = cvals, L["crp"], L["cci"]
+
+
+def test_emul_wide_coarse_rows(oracle, emul, tmp_path):
+    """Coarse multigrid matrices from graph aggregates have rows that couple to
+    more rows of their own or of later ILU blocks than the sweep kernels keep
+    in registers (8); the layout still reproduces the oracle's SpMV, ILU(0)
+    factor and sweeps on such a matrix."""
+    xy, cptr, cind = H.delaunay_case(250000, 128, seed=14)
+    m, (rp, ci, v), b, g, bc = H.pressure_system(oracle, xy, cptr, cind, sigma=1.0)
+    levels, last = _hierarchy(emul, rp, ci, None, 32, 16, 256, tmp_path / "h.bin")
+    widest = 0
+    vals, lrp, lci = v, rp, ci
+    for L in levels:
+        cvals = np.array([vals[L["gs"][L["gp"][q]:L["gp"][q + 1]]].sum() for q in range(len(L["cci"]))])
+        vals, lrp, lci = cvals, L["crp"].astype(np.int32), L["cci"].astype(np.int32)
+        if len(lrp) - 1 > 256:
+            r = _check(oracle, emul, lrp, lci, vals, block_rows=16, seed=3)
+            widest = max(widest, int(r["info"][6]))
+    assert widest > 8
