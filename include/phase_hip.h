@@ -108,9 +108,11 @@ int phase_hip_solver_destroy(phase_hip_solver *s);
  * the default; MULTIGRID = aggregation multigrid with ILU(0) smoothing, the
  * role of the reference's MueLu backend, TrilinosMueluSparseMatrixSolver.cpp:26-35),
  * "multigridCycle" (K | V | auto: K cycles with flexible CG, V cycles with
- * BiCGStab), "multigridCoarseRows" (rows per aggregate below the finest
- * level, 16), "multigridDenseRows" (a level of at most this many rows is
- * inverted densely, 256, at most 512).  Unknown keys are ignored, as the
+ * BiCGStab), "multigridKLevels" (coarse levels, from the first, whose solves
+ * are K cycles; V cycles below; 1), "multigridFinePostSmoothing" (1 | 0),
+ * "multigridCoarseRows" (rows per aggregate below the finest level, 16),
+ * "multigridDenseRows" (a level of at most this many rows is inverted
+ * densely, 256, at most 512).  Unknown keys are ignored, as the
  * reference ignores ptree keys it does not read. */
 int phase_hip_solver_setup(phase_hip_solver *s, const char *key, const char *value);
 

@@ -19,8 +19,9 @@
 //   z = M^-1 r;  r_c = P^T (r - A z);  e_c = coarse solve(r_c);  z += P e_c;
 //   z += M^-1 (r - A z)
 // The coarse solve is the next level's apply ("V" cycle) or two flexible CG
-// iterations preconditioned by it ("K" cycle, Notay & Vassilevski 2008); the
-// last level multiplies by its dense inverse.  The K cycle makes B
+// iterations preconditioned by it ("K" cycle, Notay & Vassilevski 2008; by
+// default on the first coarse level only, V cycles below); the last level
+// multiplies by its dense inverse.  The K cycle makes B
 // nonlinear, so the outer CG runs in its flexible form (Notay 2000: beta =
 // -(z, A p_old) / (p_old, A p_old)).
 #pragma once

@@ -283,28 +283,31 @@ __global__ __launch_bounds__(kRedBlock) void kMlSpmvDot2(int n, int nSlices, con
   blockPartials<2>(d, partials, tile);
 }
 
-// alpha = (p, r) / (p, q) of the first (which == 1) or second iteration
-static __global__ __launch_bounds__(kFinBlock) void kMlFinishAlpha(const Scalars *sc, MlScalars *m, int which,
-                                                                   const double *partials, int nb) {
-  if (sc->done) return;
+// alpha = (p, r) / (p, q) from the partials, summed in the same fixed order
+// in every block (every block holds the same alpha)
+__device__ __forceinline__ double mlAlpha(const double *partials, int nb, double &pq) {
   double d[2];
-  finishPartials<2, kFinBlock>(partials, nb, d);
-  if (threadIdx.x != 0) return;
-  const double alpha = (d[0] != 0. && isfinite(d[0]) && isfinite(d[1])) ? d[1] / d[0] : 0.;
-  if (which == 1) {
-    m->pq1 = d[0];
-    m->alpha1 = alpha;
-  } else {
-    m->alpha2 = alpha;
+  finishPartials<2>(partials, nb, d);
+  __shared__ double sA, sPq;
+  if (threadIdx.x == 0) {
+    sPq = d[0];
+    sA = (d[0] != 0. && isfinite(d[0]) && isfinite(d[1])) ? d[1] / d[0] : 0.;
   }
+  __syncthreads();
+  pq = sPq;
+  return sA;
 }
 
-// x = alpha1 z1 ; r1 = r0 - alpha1 q1
+// first iteration: alpha1; x = alpha1 z1 ; r1 = r0 - alpha1 q1 (block 0 keeps (z1, q1))
 template <class T>
-__global__ void kMlFirst(int n, const Scalars *sc, const MlScalars *m, const T *__restrict__ z1,
-                         const T *__restrict__ q1, const T *__restrict__ r0, T *__restrict__ x, T *__restrict__ r1) {
+__global__ __launch_bounds__(kRedBlock) void kMlFirst(int n, const Scalars *sc, MlScalars *m,
+                                                      const double *partials, int nb, const T *__restrict__ z1,
+                                                      const T *__restrict__ q1, const T *__restrict__ r0,
+                                                      T *__restrict__ x, T *__restrict__ r1) {
   if (sc->done) return;
-  const T a = (T)m->alpha1;
+  double pq;
+  const T a = (T)mlAlpha(partials, nb, pq);
+  if (blockIdx.x == 0 && threadIdx.x == 0) m->pq1 = pq;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     x[i] = a * z1[i];
     r1[i] = r0[i] - a * q1[i];
@@ -340,12 +343,13 @@ __global__ __launch_bounds__(kRedBlock) void kMlDirection(int n, const Scalars *
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) z2[i] -= g * z1[i];
 }
 
-// x += alpha2 p2
+// second iteration: alpha2 ; x += alpha2 p2
 template <class T>
-__global__ void kMlSecond(int n, const Scalars *sc, const MlScalars *m, const T *__restrict__ p2,
-                          T *__restrict__ x) {
+__global__ __launch_bounds__(kRedBlock) void kMlSecond(int n, const Scalars *sc, const double *partials, int nb,
+                                                       const T *__restrict__ p2, T *__restrict__ x) {
   if (sc->done) return;
-  const T a = (T)m->alpha2;
+  double pq;
+  const T a = (T)mlAlpha(partials, nb, pq);
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] += a * p2[i];
 }
 
@@ -379,7 +383,7 @@ void coarseSolve(phase_hip_solver *s, Multilevel &M, int l, const Scalars *sc, c
     hipLaunchKernelGGL((kDenseApply<T>), dim3((L.n + 3) / 4), dim3(256), 0, st, L.n, L.inv.p, r, e, sc);
     return;
   }
-  if (!s->mlKCycle) {
+  if (!s->mlKCycle || l >= s->mlKLevels) {
     cycle<T>(s, M, l, sc, r, e);
     return;
   }
@@ -390,17 +394,17 @@ void coarseSolve(phase_hip_solver *s, Multilevel &M, int l, const Scalars *sc, c
   cycle<T>(s, M, l, sc, r, V.z1.p);
   hipLaunchKernelGGL((kMlSpmvDot2<T, double>), dim3(gs), dim3(kRedBlock), 0, st, n, c->ord.nSlices, c->dSliceOff.p,
                      c->dSellCol.p, c->dSell64.p, (const T *)V.z1.p, V.q1.p, r, sc, L.partials.p);
-  hipLaunchKernelGGL(kMlFinishAlpha, dim3(1), dim3(kFinBlock), 0, st, sc, L.msc.p, 1, L.partials.p, gs);
-  hipLaunchKernelGGL((kMlFirst<T>), dim3(elemGrid(n)), dim3(256), 0, st, n, sc, L.msc.p, V.z1.p, V.q1.p, r, e,
-                     V.r1.p);
+  // each update block re-reads the gs partials: keep the grid near gs
+  const int gd = gridFor(n, 4 * kRedBlock, kRedCap);
+  hipLaunchKernelGGL((kMlFirst<T>), dim3(gd), dim3(kRedBlock), 0, st, n, sc, L.msc.p, L.partials.p, gs, V.z1.p,
+                     V.q1.p, r, e, V.r1.p);
   cycle<T>(s, M, l, sc, V.r1.p, V.z2.p);
   hipLaunchKernelGGL((kMlDotZQ<T>), dim3(gu), dim3(kRedBlock), 0, st, n, sc, V.z2.p, V.q1.p, L.partials.p);
-  hipLaunchKernelGGL((kMlDirection<T>), dim3(gridFor(n, 4 * kRedBlock, kRedCap)), dim3(kRedBlock), 0, st, n, sc,
-                     L.msc.p, L.partials.p, gu, V.z1.p, V.z2.p);
+  hipLaunchKernelGGL((kMlDirection<T>), dim3(gd), dim3(kRedBlock), 0, st, n, sc, L.msc.p, L.partials.p, gu,
+                     V.z1.p, V.z2.p);
   hipLaunchKernelGGL((kMlSpmvDot2<T, double>), dim3(gs), dim3(kRedBlock), 0, st, n, c->ord.nSlices, c->dSliceOff.p,
                      c->dSellCol.p, c->dSell64.p, (const T *)V.z2.p, V.q1.p, V.r1.p, sc, L.partials.p);
-  hipLaunchKernelGGL(kMlFinishAlpha, dim3(1), dim3(kFinBlock), 0, st, sc, L.msc.p, 2, L.partials.p, gs);
-  hipLaunchKernelGGL((kMlSecond<T>), dim3(elemGrid(n)), dim3(256), 0, st, n, sc, L.msc.p, V.z2.p, e);
+  hipLaunchKernelGGL((kMlSecond<T>), dim3(gd), dim3(kRedBlock), 0, st, n, sc, L.partials.p, gs, V.z2.p, e);
 }
 
 // Restriction of t (finer solver order) into level l's right-hand side, the
@@ -422,7 +426,7 @@ void coarseCorrection(phase_hip_solver *s, Multilevel &M, int l, const Scalars *
 // already put M^-1 r into z.
 template <class T>
 void smoothCorrectSmooth(phase_hip_solver *s, Multilevel &M, phase_hip_solver *c, MlVecs<T> &V, int next,
-                         const Scalars *sc, const T *r, T *z, bool pre) {
+                         const Scalars *sc, const T *r, T *z, bool pre, bool post = true) {
   hipStream_t st = s->stream;
   const int n = c->rank;
   const IluDev<T> L = iluDev<T>(c, facOf<T>(c));
@@ -431,6 +435,7 @@ void smoothCorrectSmooth(phase_hip_solver *s, Multilevel &M, phase_hip_solver *c
   hipLaunchKernelGGL((kMlResid<T, double>), dim3(gs), dim3(kRedBlock), 0, st, n, c->ord.nSlices, c->dSliceOff.p,
                      c->dSellCol.p, c->dSell64.p, (const T *)z, r, V.t.p, sc);
   coarseCorrection<T>(s, M, next, sc, V.t.p, z);
+  if (!post) return;
   hipLaunchKernelGGL((kMlResid<T, double>), dim3(gs), dim3(kRedBlock), 0, st, n, c->ord.nSlices, c->dSliceOff.p,
                      c->dSellCol.p, c->dSell64.p, (const T *)z, r, V.t.p, sc);
   launchLU<T, kProNone>(c, L, sc, V.t.p, V.y.p, (T *)nullptr, (const T *)nullptr, (const T *)nullptr, V.w.p);
@@ -449,7 +454,7 @@ template <class T>
 void mlFinishApply(phase_hip_solver *s, const Scalars *sc, const T *rhs, T *out) {
   Multilevel &M = *s->ml;
   if (M.lv.empty()) return;  // a system no larger than the dense limit: ILU(0) alone
-  smoothCorrectSmooth<T>(s, M, s, fineVecs<T>(M), 0, sc, rhs, out, false);
+  smoothCorrectSmooth<T>(s, M, s, fineVecs<T>(M), 0, sc, rhs, out, false, s->mlFinePost != 0);
 }
 template void mlFinishApply<double>(phase_hip_solver *, const Scalars *, const double *, double *);
 template void mlFinishApply<float>(phase_hip_solver *, const Scalars *, const float *, float *);

@@ -46,6 +46,8 @@ struct phase_hip_solver {
   int mlKCycleSetting = -1;  // multigrid coarse solves: 1 = K cycle, 0 = V cycle, -1 = K for CG, V for BiCGStab
   int mlKCycle = 0;          // the choice in force for the current solve
   int mlCoarseRows = 16; // rows per aggregate below the finest level
+  int mlKLevels = 1;     // coarse levels (from the first) whose solves are K cycles; V cycles below
+  int mlFinePost = 1;    // post-smoothing on the finest level (0: pre-smoothing only)
   int mlDenseRows = 256; // a level of at most this many rows is inverted densely
 
   // pattern and ordering

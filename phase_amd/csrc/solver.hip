@@ -75,9 +75,10 @@ void iluApply(phase_hip_solver *s, DBuf<T> &fac, const T *in, T *tmp, T *out, co
 
 // The multigrid preconditioner is in use for this solve.
 bool useMl(const phase_hip_solver *s) { return s->precond == 1 && s->ml; }
-// Its coarse solves are K cycles: the preconditioner varies from apply to
-// apply, and CG runs in its flexible form.
-bool flexible(const phase_hip_solver *s) { return useMl(s) && s->mlKCycle == 1; }
+// Its coarse solves are K cycles (the preconditioner varies from apply to
+// apply) or the finest level is smoothed before the correction only (the
+// preconditioner is not symmetric): CG runs in its flexible form.
+bool flexible(const phase_hip_solver *s) { return useMl(s) && (s->mlKCycle == 1 || !s->mlFinePost); }
 
 // Rest of a preconditioner apply whose fine pre-smoothing (the sweep that
 // may also run a Krylov prologue) has written M^-1 rhs into out.
@@ -760,6 +761,12 @@ int phase_hip_solver_setup(phase_hip_solver *s, const char *key, const char *val
       int kc = lv == "k" ? 1 : lv == "v" ? 0 : lv == "auto" ? -1 : -2;
       if (kc < -1) fail(PHASE_HIP_EINVAL, "unknown multigrid cycle '" + v + "' (K, V or auto)");
       s->mlKCycleSetting = kc;
+    } else if (k == "multigridKLevels") {
+      const int r = std::stoi(v);
+      if (r < 0) fail(PHASE_HIP_EINVAL, "multigridKLevels must be >= 0");
+      s->mlKLevels = r;
+    } else if (k == "multigridFinePostSmoothing") {
+      s->mlFinePost = (lv == "1" || lv == "true" || lv == "yes") ? 1 : 0;
     } else if (k == "multigridCoarseRows") {
       const int r = std::stoi(v);
       if (r < 2 || r > 64) fail(PHASE_HIP_EINVAL, "multigridCoarseRows must be in [2,64]");
