@@ -620,7 +620,8 @@ def main():
     ap.add_argument("--c4-cells", type=int, default=1_000_000,
                     help="learned case, sigma 2, at this many cells (0 skips)")
     ap.add_argument("--c4-modes", default="fp64,fp64-cg,mixed-cg,fp64-cg-mg,mixed-cg-mg,learned-cg-mg")
-    ap.add_argument("--spmv-cells", type=int, default=0, help="SpMV bandwidth point at this many cells (0 skips)")
+    ap.add_argument("--spmv-cells", type=int, default=16_000_000,
+                    help="SpMV bandwidth point beyond the last-level cache at this many cells (0 skips)")
     ap.add_argument("--c4-pairs", type=int, default=2000, help="training pairs for the learned case")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -225,12 +225,14 @@ __device__ __forceinline__ T sellRow(int slice, int lane, const int *__restrict_
   for (int e0 = 0; e0 < K; e0 += kSellBatch) {
     int c[kSellBatch];
     TV a[kSellBatch];
+    // the matrix streams through once per product: non-temporal loads keep it
+    // from evicting the gathered x lines from L2
 #pragma unroll
     for (int q = 0; q < kSellBatch; ++q)
       if (e0 + q < K) {
         const int idx = o + ((e0 + q) << 6) + lane;
-        c[q] = col[idx];
-        a[q] = val[idx];
+        c[q] = __builtin_nontemporal_load(col + idx);
+        a[q] = __builtin_nontemporal_load(val + idx);
       }
     T xv[kSellBatch];
 #pragma unroll
