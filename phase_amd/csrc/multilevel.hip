@@ -521,6 +521,7 @@ void mlEnsureHierarchy(phase_hip_solver *s) {
       bool dense = c.n <= s->mlDenseRows;
       if (!dense) {
         L.sol = makeNested(s);
+        L.sol->blockJacobi = level >= s->mlBlockJacobiFrom;
         // coarse rows couple to more rows than fine ones: when a row of an
         // aggregate would couple to more rows of its own block than the sweep
         // kernels take, smaller aggregates are taken at this level; a level

@@ -335,7 +335,8 @@ static bool brickAggregates(int n, const Graph &g, const double *xy, int target,
   }
 }
 
-void buildOrdering(Ordering &o, int n, const int *rowPtr, const int *colInd, int blockRows, const double *cellXY) {
+void buildOrdering(Ordering &o, int n, const int *rowPtr, const int *colInd, int blockRows, const double *cellXY,
+                   bool blockJacobi) {
   if (n <= 0) fail(PHASE_HIP_EINVAL, "rank must be positive");
   if (blockRows < 1 || blockRows > kLanes) fail(PHASE_HIP_EINVAL, "blockRows must be in [1,64]");
   o = Ordering();
@@ -367,6 +368,35 @@ void buildOrdering(Ordering &o, int n, const int *rowPtr, const int *colInd, int
   int nColours = 0;
   if (cellXY && brickAggregates(n, g, cellXY, blockRows, rpos, blk, members, colour)) {
     nColours = 3;
+  } else if (blockJacobi) {
+    // breadth-first aggregates as below; no colouring (one colour)
+    std::vector<int> queue, nbs;
+    for (int s0 : rorder) {
+      if (blk[s0] >= 0) continue;
+      const int b = (int)members.size();
+      members.emplace_back();
+      auto &mem = members.back();
+      mem.push_back(s0);
+      blk[s0] = b;
+      queue.assign(1, s0);
+      size_t head = 0;
+      while (head < queue.size() && (int)mem.size() < blockRows) {
+        const int u = queue[head++];
+        nbs.clear();
+        for (int k = g.ptr[u]; k < g.ptr[u + 1]; ++k)
+          if (blk[g.adj[k]] < 0) nbs.push_back(g.adj[k]);
+        std::sort(nbs.begin(), nbs.end(), [&](int a, int c) { return rpos[a] < rpos[c]; });
+        for (int w : nbs) {
+          if ((int)mem.size() >= blockRows) break;
+          blk[w] = b;
+          mem.push_back(w);
+          queue.push_back(w);
+        }
+      }
+      std::sort(mem.begin(), mem.end(), [&](int a, int c) { return rpos[a] < rpos[c]; });
+    }
+    colour.assign(members.size(), 0);
+    nColours = 1;
   } else {
     // 3. breadth-first aggregation into blocks of <= blockRows rows
     std::vector<int> queue, nbs;
@@ -447,6 +477,10 @@ void buildOrdering(Ordering &o, int n, const int *rowPtr, const int *colInd, int
       --nColours;
     }
   }
+  if (blockJacobi) {
+    colour.assign(members.size(), 0);
+    nColours = 1;
+  }
   int nb = (int)members.size();
   o.nColours = nColours;
 
@@ -524,7 +558,13 @@ void buildOrdering(Ordering &o, int n, const int *rowPtr, const int *colInd, int
   for (int k = 0; k < n; ++k) {
     int i = o.perm[k];
     auto &r = srow[k];
-    for (int j = o.cRowPtr[i]; j < o.cRowPtr[i + 1]; ++j) r.push_back({o.iperm[o.cCol[j]], o.compactSrc[j]});
+    const BlockInfo &Bk = o.blocks[blockOfRow[k]];
+    for (int j = o.cRowPtr[i]; j < o.cRowPtr[i + 1]; ++j) {
+      const int c = o.iperm[o.cCol[j]];
+      // block Jacobi: the factor keeps the couplings inside the row's block only
+      if (blockJacobi && (c < Bk.rowStart || c >= Bk.rowStart + Bk.nRows)) continue;
+      r.push_back({c, o.compactSrc[j]});
+    }
     std::sort(r.begin(), r.end(), [](const Ent &a, const Ent &b) { return a.col < b.col; });
     // merge duplicate columns (possible only in hand-made patterns): keep first
     std::vector<Ent> u;

@@ -127,8 +127,10 @@ struct Ordering {
 
 // cellXY (optional): centroid (x, y) of each row's cell, interleaved; with it
 // a scalar equation on a 2-D mesh gets three-colour brick aggregates.
+// blockJacobi: one colour, and the ILU(0) factor of the block-diagonal part
+// only (couplings between blocks dropped): one sweep launch per apply.
 void buildOrdering(Ordering &o, int n, const int *rowPtr, const int *colInd, int blockRows,
-                   const double *cellXY = nullptr);
+                   const double *cellXY = nullptr, bool blockJacobi = false);
 
 // One coarser level of the aggregation multigrid hierarchy, built from the
 // aggregates of a finer level's ordering (unsmoothed aggregation: the

@@ -42,12 +42,14 @@ struct phase_hip_solver {
   int geometric = 1; // cell centroids known: brick aggregates in three colours (else graph aggregates)
   int cg = 0;        // Krylov method: 0 = BiCGStab, 1 = preconditioned CG (symmetric systems)
   int unionPattern = 0;  // keep the stored pattern when a new one lies inside it (its other entries 0)
+  int blockJacobi = 0;   // ILU(0) of the block-diagonal part only, one colour (coarse multigrid levels)
   int precond = 0;       // 0 = ILU(0), 1 = aggregation multigrid with ILU(0) smoothing
   int mlKCycleSetting = -1;  // multigrid coarse solves: 1 = K cycle, 0 = V cycle, -1 = K for CG, V for BiCGStab
   int mlKCycle = 0;          // the choice in force for the current solve
   int mlCoarseRows = 16; // rows per aggregate below the finest level
-  int mlKLevels = 1;     // coarse levels (from the first) whose solves are K cycles; V cycles below
+  int mlKLevels = 2;     // coarse levels (from the first) whose solves are K cycles; V cycles below
   int mlFinePost = 1;    // post-smoothing on the finest level (0: pre-smoothing only)
+  int mlBlockJacobiFrom = 1;   // coarse levels from this one (0 = the first) smooth with block-Jacobi ILU(0)
   int mlDenseRows = 256; // a level of at most this many rows is inverted densely
 
   // pattern and ordering

@@ -439,7 +439,7 @@ namespace phase {
 void solverSetPattern(phase_hip_solver *s, int n, const int *rowPtr, const int *colInd, const double *cellXY) {
   setDevice(s);
   if (s->rank != 0 && n != s->rank) fail(PHASE_HIP_EINVAL, "pattern size differs from setRank");
-  buildOrdering(s->ord, n, rowPtr, colInd, s->blockRows, s->geometric ? cellXY : nullptr);
+  buildOrdering(s->ord, n, rowPtr, colInd, s->blockRows, s->geometric ? cellXY : nullptr, s->blockJacobi != 0);
   mlFree(s);
   if (cellXY && s->geometric) s->cellXY.assign(cellXY, cellXY + 2 * (size_t)n);
   else s->cellXY.clear();
@@ -765,6 +765,11 @@ int phase_hip_solver_setup(phase_hip_solver *s, const char *key, const char *val
       const int r = std::stoi(v);
       if (r < 0) fail(PHASE_HIP_EINVAL, "multigridKLevels must be >= 0");
       s->mlKLevels = r;
+    } else if (k == "multigridBlockJacobiFrom") {
+      const int r = std::stoi(v);
+      if (r < 0) fail(PHASE_HIP_EINVAL, "multigridBlockJacobiFrom must be >= 0");
+      if (r != s->mlBlockJacobiFrom) mlFree(s);
+      s->mlBlockJacobiFrom = r;
     } else if (k == "multigridFinePostSmoothing") {
       s->mlFinePost = (lv == "1" || lv == "true" || lv == "yes") ? 1 : 0;
     } else if (k == "multigridCoarseRows") {

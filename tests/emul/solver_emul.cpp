@@ -28,6 +28,13 @@ struct Bad {
   } while (0)
 }  // namespace
 
+namespace {
+bool gBlockJacobi = false;
+}
+// The next emul_run builds block-Jacobi orderings (one colour, factor of the
+// block-diagonal part).
+extern "C" void emul_set_block_jacobi(int on) { gBlockJacobi = on != 0; }
+
 extern "C" int emul_run(int n, const int *rowPtr, const int *colInd, const double *vals, int blockRows,
                         int *perm, int *facRowPtr, int *facCol, double *facVal, int cap, const double *rhsIn,
                         double *y, double *x, const double *xin, double *spmvOut, int *info, char *err,
@@ -35,7 +42,7 @@ extern "C" int emul_run(int n, const int *rowPtr, const int *colInd, const doubl
   try {
     Ordering o;
     try {
-      buildOrdering(o, n, rowPtr, colInd, blockRows, cellXY);
+      buildOrdering(o, n, rowPtr, colInd, blockRows, cellXY, gBlockJacobi);
     } catch (const HipError &e) {
       throw Bad{e.msg};
     }

@@ -223,3 +223,38 @@ def test_emul_wide_coarse_rows(oracle, emul, tmp_path):
             r = _check(oracle, emul, lrp, lci, vals, block_rows=16, seed=3)
             widest = max(widest, int(r["info"][6]))
     assert widest > 8
+
+
+def test_emul_block_jacobi_ordering(oracle, emul):
+    """The block-Jacobi ordering of coarse multigrid levels: one colour, the
+    ILU(0) factor of the block-diagonal part only (couplings between blocks
+    dropped), the SpMV still over the whole matrix; the factor and both sweeps
+    equal the oracle's on the block-diagonal restriction of the permuted
+    matrix."""
+    import scipy.sparse as sp
+
+    xy, cptr, cind = H.delaunay_case(3000, 24, seed=15)
+    m, (rp, ci, v), b, g, bc = H.pressure_system(oracle, xy, cptr, cind, sigma=2.0)
+    n = len(rp) - 1
+    emul.emul_set_block_jacobi.restype = None
+    emul.emul_set_block_jacobi(1)
+    try:
+        perm = _run(emul, rp, ci, v, np.zeros(n), np.zeros(n), 16)["perm"]
+        rng = np.random.default_rng(4)
+        x_orig, b_solver = rng.standard_normal(n), rng.standard_normal(n)
+        r = _run(emul, rp, ci, v, b_solver, x_orig[perm], 16)
+    finally:
+        emul.emul_set_block_jacobi(0)
+    assert r["info"][1] == 1
+    assert np.array_equal(r["spmv"], oracle.spmv(rp, ci, v, x_orig)[perm])
+    A = H.permuted(H.compact_sorted(rp, ci, v), perm)
+    pat = sp.csr_matrix((np.ones(len(r["fci"])), r["fci"], r["frp"]), shape=(n, n))
+    Ab = A.multiply(pat).tocsr()
+    Ab.sort_indices()
+    assert np.array_equal(Ab.indptr, r["frp"]) and np.array_equal(Ab.indices, r["fci"])
+    assert Ab.nnz < A.nnz  # couplings between blocks were dropped
+    fac = oracle.ilu0(Ab.indptr.astype(np.int32), Ab.indices.astype(np.int32), Ab.data)
+    assert np.array_equal(r["fv"], fac)
+    y, x = oracle.lu_solve(Ab.indptr, Ab.indices, fac, b_solver)
+    assert np.array_equal(r["y"], y)
+    assert np.linalg.norm(r["x"] - x) <= 1e-12 * np.linalg.norm(x)
