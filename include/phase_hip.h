@@ -112,7 +112,7 @@ int phase_hip_solver_destroy(phase_hip_solver *s);
  * are K cycles; V cycles below; 2), "multigridBlockJacobiFrom" (coarse
  * levels from this index on, 0 being the first, smooth with the ILU(0) of
  * their block-diagonal part in one launch instead of the block-coloured
- * ILU(0); 1), "multigridFinePostSmoothing" (1 | 0),
+ * ILU(0); 0), "multigridFinePostSmoothing" (1 | 0),
  * "multigridCoarseRows" (rows per aggregate below the finest level, 16),
  * "multigridDenseRows" (a level of at most this many rows is inverted
  * densely, 256, at most 512).  Unknown keys are ignored, as the

@@ -21,9 +21,10 @@
 // The coarse solve is the next level's apply ("V" cycle) or two flexible CG
 // iterations preconditioned by it ("K" cycle, Notay & Vassilevski 2008; by
 // default on the first two coarse levels, V cycles below); the last level
-// multiplies by its dense inverse.  From the second coarse level on, M is by
-// default the ILU(0) of the block-diagonal part (one launch per apply: on
-// these small levels every launch costs more than its arithmetic).  The K cycle makes B
+// multiplies by its dense inverse.  On the coarse levels M is by default the
+// ILU(0) of the block-diagonal part, one aggregate per block (one launch per
+// apply: on these small levels every launch costs more than its arithmetic,
+// and a coloured sweep takes five).  The K cycle makes B
 // nonlinear, so the outer CG runs in its flexible form (Notay 2000: beta =
 // -(z, A p_old) / (p_old, A p_old)).
 #pragma once

@@ -233,34 +233,20 @@ __global__ __launch_bounds__(kRedBlock) void kMlResid(int n, int nSlices, const 
   t[row] = b[row] - sellRow<T, TV>(slice, lane, sliceOff, col, val, x);
 }
 
-// rc[k] = sum over the finer rows i of aggregate restRow[k] (identity when
-// null) of the residual b_i - (A x)_i: the restriction of the residual without
-// storing it.  One wavefront per coarse row (an aggregate has at most 64 rows):
-// lane j takes row aggPtr[a] + j, in FP64, and the wavefront sums in a fixed
-// order.
-template <class T, class TV>
-__global__ __launch_bounds__(256) void kMlResidRestrict(int nc, const int *__restrict__ restRow,
-                                                        const int *__restrict__ aggPtr,
-                                                        const int *__restrict__ sliceOff,
-                                                        const int *__restrict__ col, const TV *__restrict__ val,
-                                                        const T *__restrict__ x, const T *__restrict__ b,
-                                                        T *__restrict__ rc, const Scalars *sc) {
+// rc[k] = sum of t over the finer rows of aggregate restRow[k] (identity when
+// null).  One wavefront per coarse row (an aggregate has at most 64 rows,
+// consecutive in the finer solver order): lane j reads row aggPtr[a] + j, a
+// coalesced read, and the wavefront sums in a fixed order in FP64.
+template <class T>
+__global__ __launch_bounds__(256) void kMlRestrict(int nc, const int *__restrict__ restRow,
+                                                   const int *__restrict__ aggPtr, const T *__restrict__ t,
+                                                   T *__restrict__ rc, const Scalars *sc) {
   if (sc->done) return;
   const int k = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (k >= nc) return;
   const int a = restRow ? restRow[k] : k;
   const int row = aggPtr[a] + lane;
-  double d = 0.;
-  if (row < aggPtr[a + 1]) {
-    const int sl = row >> 6, ln = row & 63;
-    const int o = sliceOff[sl], K = (sliceOff[sl + 1] - o) >> 6;
-    double acc = 0.;
-    for (int e = 0; e < K; ++e) {
-      const int c = col[o + (e << 6) + ln];
-      if (c >= 0) acc = fma((double)val[o + (e << 6) + ln], (double)x[c], acc);
-    }
-    d = (double)b[row] - acc;
-  }
+  double d = row < aggPtr[a + 1] ? (double)t[row] : 0.;
   d = waveSum(d);
   if (lane == 0) rc[k] = (T)d;
 }
@@ -426,17 +412,15 @@ void coarseSolve(phase_hip_solver *s, Multilevel &M, int l, const Scalars *sc, c
   hipLaunchKernelGGL((kMlSecond<T>), dim3(gd), dim3(kRedBlock), 0, st, n, sc, L.partials.p, gs, V.z2.p, e);
 }
 
-// Restriction of the residual r - A z (finer level c, solver order) into level
-// l's right-hand side, the coarse solve, and the prolongation added to z.
+// Restriction of the residual t (finer solver order) into level l's
+// right-hand side, the coarse solve, and the prolongation added to z.
 template <class T>
-void coarseCorrection(phase_hip_solver *s, Multilevel &M, int l, phase_hip_solver *c, const Scalars *sc, const T *r,
-                      T *z) {
+void coarseCorrection(phase_hip_solver *s, Multilevel &M, int l, const Scalars *sc, const T *t, T *z) {
   MlLevel &L = M.lv[l];
   MlVecs<T> &V = vecs<T>(L);
   hipStream_t st = s->stream;
-  hipLaunchKernelGGL((kMlResidRestrict<T, double>), dim3((L.n + 3) / 4), dim3(256), 0, st, L.n,
-                     L.sol ? L.restRow.p : nullptr, L.finerAggPtr.p, c->dSliceOff.p, c->dSellCol.p, c->dSell64.p,
-                     (const T *)z, r, V.r.p, sc);
+  hipLaunchKernelGGL((kMlRestrict<T>), dim3((L.n + 3) / 4), dim3(256), 0, st, L.n, L.sol ? L.restRow.p : nullptr,
+                     L.finerAggPtr.p, t, V.r.p, sc);
   coarseSolve<T>(s, M, l, sc, V.r.p, V.e.p);
   hipLaunchKernelGGL((kMlProlongAdd<T>), dim3(elemGrid(L.nFiner)), dim3(256), 0, st, L.nFiner, L.pIdx.p,
                      (const T *)V.e.p, z, sc);
@@ -453,7 +437,9 @@ void smoothCorrectSmooth(phase_hip_solver *s, Multilevel &M, phase_hip_solver *c
   const IluDev<T> L = iluDev<T>(c, facOf<T>(c));
   if (pre) launchLU<T, kProNone>(c, L, sc, r, V.y.p, (T *)nullptr, (const T *)nullptr, (const T *)nullptr, z);
   const int gs = spmvGrid(c);
-  coarseCorrection<T>(s, M, next, c, sc, r, z);
+  hipLaunchKernelGGL((kMlResid<T, double>), dim3(gs), dim3(kRedBlock), 0, st, n, c->ord.nSlices, c->dSliceOff.p,
+                     c->dSellCol.p, c->dSell64.p, (const T *)z, r, V.t.p, sc);
+  coarseCorrection<T>(s, M, next, sc, V.t.p, z);
   if (!post) return;
   hipLaunchKernelGGL((kMlResid<T, double>), dim3(gs), dim3(kRedBlock), 0, st, n, c->ord.nSlices, c->dSliceOff.p,
                      c->dSellCol.p, c->dSell64.p, (const T *)z, r, V.t.p, sc);

@@ -498,7 +498,10 @@ void buildOrdering(Ordering &o, int n, const int *rowPtr, const int *colInd, int
   std::vector<int> aggSize;  // aggregates in pack order, i.e. in solver order
   for (int ci = 0; ci < nb; ++ci) {
     const int a = border[ci];
-    if (packs.empty() || packColour.back() != colour[a] || packs.back().size() + members[a].size() > (size_t)kLanes) {
+    // block Jacobi keeps one aggregate per block: couplings between the rows of
+    // packed aggregates would lengthen the block's dependency levels
+    if (packs.empty() || packColour.back() != colour[a] || blockJacobi ||
+        packs.back().size() + members[a].size() > (size_t)kLanes) {
       packs.emplace_back();
       packColour.push_back(colour[a]);
     }

@@ -49,7 +49,7 @@ struct phase_hip_solver {
   int mlCoarseRows = 16; // rows per aggregate below the finest level
   int mlKLevels = 2;     // coarse levels (from the first) whose solves are K cycles; V cycles below
   int mlFinePost = 1;    // post-smoothing on the finest level (0: pre-smoothing only)
-  int mlBlockJacobiFrom = 1;   // coarse levels from this one (0 = the first) smooth with block-Jacobi ILU(0)
+  int mlBlockJacobiFrom = 0;   // coarse levels from this one (0 = the first) smooth with block-Jacobi ILU(0)
   int mlDenseRows = 256; // a level of at most this many rows is inverted densely
 
   // pattern and ordering
