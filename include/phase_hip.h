@@ -477,7 +477,14 @@ int phase_hip_gcib_hydrodynamic_force(phase_hip_gcib *ib, double rho, double mu,
  * uFace0 / pFace0: initial face values (boundary values of FIXED faces);
  * pinCell >= 0 fixes p at that cell (needed when no pressure face is FIXED),
  * -1 for none.  history (optional, 3 per step): momentum and pressure
- * iterations, FractionalStep::maxDivergenceError of the corrected faces. */
+ * iterations (summed over the passes), FractionalStep::maxDivergenceError of
+ * the corrected faces.  Setup keys u.<k> and p.<k> go to the two solvers;
+ * numPressureCorrections = n (1 to 10, default 1) runs the momentum solve,
+ * pressure solve and velocity correction n times per step from the same
+ * old-time fields, each later pass with the latest face velocities and
+ * pressure gradient.  The reference's case files carry this key
+ * (Examples/RisingBubble/case/case.info:16) but its fractional-step solvers
+ * make one pass, which is the default here. */
 typedef struct phase_hip_fstep phase_hip_fstep;
 
 int phase_hip_fstep_create(phase_hip_fstep **s, phase_hip_mesh *m, double rho, double mu, const int *uBcType,
@@ -517,8 +524,11 @@ int phase_hip_fstep_immersed_boundary_motion(phase_hip_fstep *s, const phase_hip
  * and gammaBcType (NULL: zero gradient everywhere); uFace0, pFace0 boundary
  * values; gammaCell0 the initial volume fraction; pinCell as for
  * phase_hip_fstep_create.  Setup keys gamma.<k>, u.<k>, p.<k> go to the three
- * solvers.  history (optional, 4 per step): gamma, momentum and pressure
- * iterations, max |src::div(u_f)| after the correction. */
+ * solvers; numPressureCorrections as for phase_hip_fstep_setup (the passes
+ * repeat the momentum, pressure and correction stages, not the volume
+ * fraction).  history (optional, 4 per step): gamma, momentum and pressure
+ * iterations (summed over the passes), max |src::div(u_f)| after the
+ * correction. */
 typedef struct phase_hip_multiphase phase_hip_multiphase;
 
 int phase_hip_multiphase_create(phase_hip_multiphase **s, phase_hip_mesh *m, double rho1, double rho2, double mu1,

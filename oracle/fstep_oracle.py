@@ -18,7 +18,14 @@ a viscous shear mode decaying at the Crank-Nicolson rate of the theta = 0.5
 viscous term, and a closed box at rest staying at rest.  `plant` (tests only)
 plants one mistake so that the tests can show each pin catches it:
 "drop_restore" skips u += dt grad p, "flip_src" adds src(grad p) instead of
-subtracting it, "theta1" makes the viscous term fully implicit.
+subtracting it, "theta1" makes the viscous term fully implicit,
+"save_per_corrector" takes the old-time fields again before every pass of a
+step with several pressure correctors (the step then advances one dt per pass).
+
+With `correctors` = n > 1 everything after savePreviousTimeStep runs n times
+per step: each later pass assembles the momentum equation from the same
+old-time fields with the latest face velocities and pressure gradient and
+projects again (the device driver's numPressureCorrections).
 
 With an immersed boundary this is FractionalStepGCIB::solve
 (src/2D/Unstructured/Solvers/FractionalStepGCIB.cpp:14-56): the fv:: terms
@@ -73,7 +80,18 @@ class FractionalStepOracle:
             b[pin] = 0.0
         return spla.spsolve(A.tocsc(), b)
 
-    def step(self, dt):
+    def step(self, dt, correctors=1):
+        """One time step; `correctors` passes of momentum, pressure and
+        velocity correction from the same old-time fields (the device driver's
+        numPressureCorrections)."""
+        u0C, u0F = self.uC.copy(), self.uF.copy()
+        for _ in range(correctors):
+            if self.plant == "save_per_corrector":
+                u0C, u0F = self.uC.copy(), self.uF.copy()
+            div = self._pass(dt, u0C, u0F)
+        return div
+
+    def _pass(self, dt, u0C, u0F):
         m, N, F = self.m, self.m.n_cells, self.m.n_faces
         ib = self.ib
         if ib is not None:
@@ -81,7 +99,6 @@ class FractionalStepOracle:
             omegas = ib[4] if len(ib) > 4 else None
             other = (g.types != 0).astype(np.int32)
             fluid = g.types == 0
-        u0C, u0F = self.uC.copy(), self.uF.copy()
         E = O.ddt(m, dt, u0C.reshape(-1), ncomp=2)
         E += O.div(m, self.uF.reshape(-1), u0F.reshape(-1), theta=0.0, bc_type=self.ubc,
                    phi_face=self.uF.reshape(-1), phi0_cell=u0C.reshape(-1), phi0_face=u0F.reshape(-1), ncomp=2)

@@ -23,6 +23,9 @@ is given, else 0:
   solvePEqn (:152-163): laplacian(dt / rho, p) == src::div(u); boundary faces;
       gradP_f; gradP = faceToCell(rho, rho) of gradP_f
   correctVelocity (:165-173): u -= dt / rho gradP (cells and faces)
+With `correctors` = n > 1, solveUEqn (after the one shift of the old-time
+fields), solvePEqn and correctVelocity run n times per step, as the device
+driver's numPressureCorrections does.
 
 Elementwise arithmetic is written in the device kernels' order, so only the
 linear solves (direct here, BiCGStab there) differ.  Each operator used here
@@ -110,7 +113,9 @@ class MultiphaseOracle:
             self.fstC = O.face_to_cell(m, self.fstF.reshape(-1), self.rhoC, self.rhoF)
             self.st_fields = r
 
-    def step(self, dt):
+    def step(self, dt, correctors=1):
+        """One time step; `correctors` passes of solveUEqn (after the one
+        shift of the old-time fields), solvePEqn and correctVelocity."""
         m = self.m
         N, F = m.n_cells, m.n_faces
         # solveGammaEqn
@@ -126,6 +131,13 @@ class MultiphaseOracle:
         # solveUEqn
         self.u1C, self.u0C = self.u0C, self.uC.copy()
         self.u1F, self.u0F = self.u0F, self.uF.copy()
+        for _ in range(correctors):
+            div = self._pass(dt)
+        return div
+
+    def _pass(self, dt):
+        m = self.m
+        N = m.n_cells
         self.gpC = O.face_to_cell(m, self.gpF.reshape(-1), self.rhoC, self.rho0F)
         a = O.ddt(m, dt, self.u0C.reshape(-1), ncomp=2)
         a.scale_cells(N, self.rhoC)

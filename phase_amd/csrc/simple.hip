@@ -24,6 +24,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -179,6 +180,17 @@ void ck(int rc, const char *what, const char *detail) {
   if (rc != PHASE_HIP_OK) fail(rc, std::string(what) + ": " + (detail ? detail : ""));
 }
 
+// The value of the numPressureCorrections key (the name the reference's case
+// files use, Examples/RisingBubble/case/case.info:16): 1 to 10 passes of
+// momentum predictor, pressure solve and velocity correction per time step.
+int parseCorrections(const char *value) {
+  char *end = nullptr;
+  const long n = std::strtol(value, &end, 10);
+  if (end == value || *end != '\0' || n < 1 || n > 10)
+    fail(PHASE_HIP_EINVAL, "numPressureCorrections takes an integer from 1 to 10");
+  return (int)n;
+}
+
 template <class F>
 int guardedS(phase_hip_simple *s, F &&f) {
   if (!s) return PHASE_HIP_EINVAL;
@@ -331,6 +343,13 @@ void pisoStep(phase_hip_simple *s, double dt, int nCorr, double *hist) {
 // add the ghost and solid rows, the momentum source is src(grad p / rho),
 // u += dt / rho grad p and u -= dt grad p touch the fluid cells only, and
 // src::div and the divergence check cover the fluid cells.
+//
+// Setup key numPressureCorrections = n (default 1, the reference's single
+// pass) repeats everything after savePreviousTimeStep n times per step: each
+// later pass assembles the momentum equation with the same old-time fields
+// but the latest face velocities and pressure gradient, then projects again
+// (the fractional-step form of PISO's repeated correctors).  The reference's
+// case files carry the key; its fractional-step solvers make one pass.
 struct phase_hip_fstep {
   phase_hip_mesh *mesh = nullptr;
   int N = 0, F = 0, pin = 0;
@@ -344,6 +363,7 @@ struct phase_hip_fstep {
   DBuf<int> ibTypes;
   DBuf<double> gq;
   phase_hip_equation *ibU = nullptr, *ibP = nullptr;
+  int nCorr = 1;  // numPressureCorrections
   std::string err;
 };
 
@@ -402,75 +422,80 @@ void timeStep(phase_hip_fstep *s, double dt, double *hist) {
   // savePreviousTimeStep
   PH_CHECK(hipMemcpyAsync(s->u0C.p, s->uC.p, 2 * (size_t)N * sizeof(double), hipMemcpyDeviceToDevice, st));
   PH_CHECK(hipMemcpyAsync(s->u0F.p, s->uF.p, 2 * (size_t)F * sizeof(double), hipMemcpyDeviceToDevice, st));
-  // solveUEqn (FractionalStep.cpp:79-92)
-  ck(phase_hip_eq_ddt(s->uEq, dt, s->u0C.p), "fv::ddt", s->uEq->err.c_str());
-  ck(phase_hip_eq_div(s->tmp, s->uF.p, s->u0F.p, 0.0, s->uBc.p, s->uF.p, s->u0C.p, s->u0F.p), "fv::div",
-     s->tmp->err.c_str());
-  ck(phase_hip_eq_add(s->uEq, s->tmp, 1), "operator+", s->uEq->err.c_str());
   const bool ib = s->immersed;
-  if (ib) {  // + ib_.velocityBcs(u_), the fv:: rows on the fluid cells
-    ck(phase_hip_eq_remove_cells(s->uEq, s->ibTypes.p), "remove", s->uEq->err.c_str());
-    ck(phase_hip_eq_add(s->uEq, s->ibU, 1), "operator+", s->uEq->err.c_str());
+  int uIters = 0, pIters = 0;
+  for (int corr = 0; corr < s->nCorr; ++corr) {
+    // solveUEqn (FractionalStep.cpp:79-92)
+    ck(phase_hip_eq_ddt(s->uEq, dt, s->u0C.p), "fv::ddt", s->uEq->err.c_str());
+    ck(phase_hip_eq_div(s->tmp, s->uF.p, s->u0F.p, 0.0, s->uBc.p, s->uF.p, s->u0C.p, s->u0F.p), "fv::div",
+       s->tmp->err.c_str());
+    ck(phase_hip_eq_add(s->uEq, s->tmp, 1), "operator+", s->uEq->err.c_str());
+    if (ib) {  // + ib_.velocityBcs(u_), the fv:: rows on the fluid cells
+      ck(phase_hip_eq_remove_cells(s->uEq, s->ibTypes.p), "remove", s->uEq->err.c_str());
+      ck(phase_hip_eq_add(s->uEq, s->ibU, 1), "operator+", s->uEq->err.c_str());
+    }
+    ck(phase_hip_eq_laplacian(s->tmp, 0, s->mu / s->rho, nullptr, nullptr, 0.5, s->uBc.p, s->uF.p, s->u0C.p, s->u0F.p),
+       "fv::laplacian", s->tmp->err.c_str());
+    if (ib) {  // - src::src(gradP_ / rho_) over the fluid cells
+      ck(phase_hip_eq_remove_cells(s->tmp, s->ibTypes.p), "remove", s->tmp->err.c_str());
+      hipLaunchKernelGGL(kDivide, dim3(g256(2L * N)), dim3(256), 0, st, 2L * N, s->rho, s->gpC.p, s->gq.p);
+      ck(phase_hip_field_src_src(m, 2, s->gq.p, s->gsrc.p), "src::src", m->err.c_str());
+      hipLaunchKernelGGL(kZeroNonFluid, dim3(g256(N)), dim3(256), 0, st, N, 2, s->ibTypes.p, s->gsrc.p);
+    } else {
+      ck(phase_hip_field_src_src(m, 2, s->gpC.p, s->gsrc.p), "src::src", m->err.c_str());
+    }
+    ck(phase_hip_eq_sub_vector(s->tmp, s->gsrc.p), "operator-", s->tmp->err.c_str());
+    ck(phase_hip_eq_add(s->uEq, s->tmp, -1), "operator==", s->uEq->err.c_str());
+    hipLaunchKernelGGL(kToBlocks, dim3(g256(N)), dim3(256), 0, st, N, s->uC.p, s->uB.p);
+    ck(phase_hip_solver_set_equation(s->su, s->uEq), "uEqn set", s->su->err.c_str());
+    ck(phase_hip_solver_set_guess_device(s->su, s->uB.p), "uEqn guess", s->su->err.c_str());
+    double err = 0.;
+    int rc = phase_hip_solver_solve(s->su, &err);
+    if (rc != PHASE_HIP_ENOCONV) ck(rc, "uEqn solve", s->su->err.c_str());
+    ck(phase_hip_solver_get_x_device(s->su, s->uB.p), "uEqn x", s->su->err.c_str());
+    hipLaunchKernelGGL(kFromBlocks, dim3(g256(N)), dim3(256), 0, st, N, s->uB.p, s->uC.p);
+    if (ib)
+      hipLaunchKernelGGL(kFluidAxpy, dim3(g256(N)), dim3(256), 0, st, N, dt / s->rho, s->ibTypes.p, s->gpC.p, s->uC.p);
+    else
+      hipLaunchKernelGGL(kAxpy, dim3(g256(2L * N)), dim3(256), 0, st, 2L * N, dt, s->gpC.p, s->uC.p);
+    ck(phase_hip_field_interpolate_faces(m, 2, s->uC.p, s->uBc.p, s->uF.p), "interpolateFaces", m->err.c_str());
+    // solvePEqn (FractionalStep.cpp:94-106; FractionalStepGCIB.cpp:44-56)
+    ck(phase_hip_eq_laplacian(s->pEq, 1, ib ? dt / s->rho : dt, nullptr, nullptr, 1.0, s->pBc.p, s->pF.p, nullptr,
+                              nullptr),
+       "fv::laplacian", s->pEq->err.c_str());
+    if (ib) {  // + ib_.bcs(p_)
+      ck(phase_hip_eq_remove_cells(s->pEq, s->ibTypes.p), "remove", s->pEq->err.c_str());
+      ck(phase_hip_eq_add(s->pEq, s->ibP, 1), "operator+", s->pEq->err.c_str());
+    }
+    ck(phase_hip_field_src_div(m, s->uF.p, s->m.p), "src::div", m->err.c_str());
+    if (ib) hipLaunchKernelGGL(kZeroNonFluid, dim3(g256(N)), dim3(256), 0, st, N, 1, s->ibTypes.p, s->m.p);
+    ck(phase_hip_eq_sub_vector(s->pEq, s->m.p), "operator==", s->pEq->err.c_str());
+    if (s->pin >= 0)  // all-Neumann pressure: one cell fixed at 0
+      hipLaunchKernelGGL(kPinRow, dim3(1), dim3(64), 0, st, s->pin, s->pEq->stride, s->pEq->cols.p, s->pEq->vals.p,
+                         s->pEq->cnt.p, s->pEq->rhs.p);
+    ck(phase_hip_solver_set_equation(s->sp, s->pEq), "pEqn set", s->sp->err.c_str());
+    ck(phase_hip_solver_set_guess_device(s->sp, s->pC.p), "pEqn guess", s->sp->err.c_str());
+    rc = phase_hip_solver_solve(s->sp, &err);
+    if (rc != PHASE_HIP_ENOCONV) ck(rc, "pEqn solve", s->sp->err.c_str());
+    ck(phase_hip_solver_get_x_device(s->sp, s->pC.p), "pEqn x", s->sp->err.c_str());
+    ck(phase_hip_field_interpolate_faces(m, 1, s->pC.p, s->pBc.p, s->pF.p), "setBoundaryFaces", m->err.c_str());
+    ck(phase_hip_field_grad(m, s->pC.p, s->pF.p, s->gpF.p, s->gpC.p), "grad p", m->err.c_str());
+    // correctVelocity (FractionalStep.cpp:108-117)
+    if (ib)
+      hipLaunchKernelGGL(kFluidAxpy, dim3(g256(N)), dim3(256), 0, st, N, -dt, s->ibTypes.p, s->gpC.p, s->uC.p);
+    else
+      hipLaunchKernelGGL(kAxpy, dim3(g256(2L * N)), dim3(256), 0, st, 2L * N, -dt, s->gpC.p, s->uC.p);
+    hipLaunchKernelGGL(kAxpy, dim3(g256(2L * F)), dim3(256), 0, st, 2L * F, -dt, s->gpF.p, s->uF.p);
+    PH_CHECK(hipGetLastError());
+    uIters += phase_hip_solver_n_iters(s->su);
+    pIters += phase_hip_solver_n_iters(s->sp);
   }
-  ck(phase_hip_eq_laplacian(s->tmp, 0, s->mu / s->rho, nullptr, nullptr, 0.5, s->uBc.p, s->uF.p, s->u0C.p, s->u0F.p),
-     "fv::laplacian", s->tmp->err.c_str());
-  if (ib) {  // - src::src(gradP_ / rho_) over the fluid cells
-    ck(phase_hip_eq_remove_cells(s->tmp, s->ibTypes.p), "remove", s->tmp->err.c_str());
-    hipLaunchKernelGGL(kDivide, dim3(g256(2L * N)), dim3(256), 0, st, 2L * N, s->rho, s->gpC.p, s->gq.p);
-    ck(phase_hip_field_src_src(m, 2, s->gq.p, s->gsrc.p), "src::src", m->err.c_str());
-    hipLaunchKernelGGL(kZeroNonFluid, dim3(g256(N)), dim3(256), 0, st, N, 2, s->ibTypes.p, s->gsrc.p);
-  } else {
-    ck(phase_hip_field_src_src(m, 2, s->gpC.p, s->gsrc.p), "src::src", m->err.c_str());
-  }
-  ck(phase_hip_eq_sub_vector(s->tmp, s->gsrc.p), "operator-", s->tmp->err.c_str());
-  ck(phase_hip_eq_add(s->uEq, s->tmp, -1), "operator==", s->uEq->err.c_str());
-  hipLaunchKernelGGL(kToBlocks, dim3(g256(N)), dim3(256), 0, st, N, s->uC.p, s->uB.p);
-  ck(phase_hip_solver_set_equation(s->su, s->uEq), "uEqn set", s->su->err.c_str());
-  ck(phase_hip_solver_set_guess_device(s->su, s->uB.p), "uEqn guess", s->su->err.c_str());
-  double err = 0.;
-  int rc = phase_hip_solver_solve(s->su, &err);
-  if (rc != PHASE_HIP_ENOCONV) ck(rc, "uEqn solve", s->su->err.c_str());
-  ck(phase_hip_solver_get_x_device(s->su, s->uB.p), "uEqn x", s->su->err.c_str());
-  hipLaunchKernelGGL(kFromBlocks, dim3(g256(N)), dim3(256), 0, st, N, s->uB.p, s->uC.p);
-  if (ib)
-    hipLaunchKernelGGL(kFluidAxpy, dim3(g256(N)), dim3(256), 0, st, N, dt / s->rho, s->ibTypes.p, s->gpC.p, s->uC.p);
-  else
-    hipLaunchKernelGGL(kAxpy, dim3(g256(2L * N)), dim3(256), 0, st, 2L * N, dt, s->gpC.p, s->uC.p);
-  ck(phase_hip_field_interpolate_faces(m, 2, s->uC.p, s->uBc.p, s->uF.p), "interpolateFaces", m->err.c_str());
-  // solvePEqn (FractionalStep.cpp:94-106; FractionalStepGCIB.cpp:44-56)
-  ck(phase_hip_eq_laplacian(s->pEq, 1, ib ? dt / s->rho : dt, nullptr, nullptr, 1.0, s->pBc.p, s->pF.p, nullptr,
-                            nullptr),
-     "fv::laplacian", s->pEq->err.c_str());
-  if (ib) {  // + ib_.bcs(p_)
-    ck(phase_hip_eq_remove_cells(s->pEq, s->ibTypes.p), "remove", s->pEq->err.c_str());
-    ck(phase_hip_eq_add(s->pEq, s->ibP, 1), "operator+", s->pEq->err.c_str());
-  }
-  ck(phase_hip_field_src_div(m, s->uF.p, s->m.p), "src::div", m->err.c_str());
-  if (ib) hipLaunchKernelGGL(kZeroNonFluid, dim3(g256(N)), dim3(256), 0, st, N, 1, s->ibTypes.p, s->m.p);
-  ck(phase_hip_eq_sub_vector(s->pEq, s->m.p), "operator==", s->pEq->err.c_str());
-  if (s->pin >= 0)  // all-Neumann pressure: one cell fixed at 0
-    hipLaunchKernelGGL(kPinRow, dim3(1), dim3(64), 0, st, s->pin, s->pEq->stride, s->pEq->cols.p, s->pEq->vals.p,
-                       s->pEq->cnt.p, s->pEq->rhs.p);
-  ck(phase_hip_solver_set_equation(s->sp, s->pEq), "pEqn set", s->sp->err.c_str());
-  ck(phase_hip_solver_set_guess_device(s->sp, s->pC.p), "pEqn guess", s->sp->err.c_str());
-  rc = phase_hip_solver_solve(s->sp, &err);
-  if (rc != PHASE_HIP_ENOCONV) ck(rc, "pEqn solve", s->sp->err.c_str());
-  ck(phase_hip_solver_get_x_device(s->sp, s->pC.p), "pEqn x", s->sp->err.c_str());
-  ck(phase_hip_field_interpolate_faces(m, 1, s->pC.p, s->pBc.p, s->pF.p), "setBoundaryFaces", m->err.c_str());
-  ck(phase_hip_field_grad(m, s->pC.p, s->pF.p, s->gpF.p, s->gpC.p), "grad p", m->err.c_str());
-  // correctVelocity (FractionalStep.cpp:108-117)
-  if (ib)
-    hipLaunchKernelGGL(kFluidAxpy, dim3(g256(N)), dim3(256), 0, st, N, -dt, s->ibTypes.p, s->gpC.p, s->uC.p);
-  else
-    hipLaunchKernelGGL(kAxpy, dim3(g256(2L * N)), dim3(256), 0, st, 2L * N, -dt, s->gpC.p, s->uC.p);
-  hipLaunchKernelGGL(kAxpy, dim3(g256(2L * F)), dim3(256), 0, st, 2L * F, -dt, s->gpF.p, s->uF.p);
-  PH_CHECK(hipGetLastError());
   if (hist) {
     // FractionalStep::maxDivergenceError (FractionalStep.cpp:119-135) on the corrected faces
     ck(phase_hip_field_src_div(m, s->uF.p, s->m.p), "src::div", m->err.c_str());
     if (ib) hipLaunchKernelGGL(kZeroNonFluid, dim3(g256(N)), dim3(256), 0, st, N, 1, s->ibTypes.p, s->m.p);
-    hist[0] = phase_hip_solver_n_iters(s->su);
-    hist[1] = phase_hip_solver_n_iters(s->sp);
+    hist[0] = uIters;
+    hist[1] = pIters;
     hist[2] = maxAbsF(s, s->m.p, N);
   }
 }
@@ -550,10 +575,14 @@ int phase_hip_fstep_setup(phase_hip_fstep *s, const char *key, const char *value
   return guardedF(s, [&] {
     if (!key || !value) fail(PHASE_HIP_EINVAL, "null key/value");
     std::string k(key);
+    if (k == "numPressureCorrections") {
+      s->nCorr = parseCorrections(value);
+      return;
+    }
     phase_hip_solver *v = nullptr;
     if (k.rfind("u.", 0) == 0) v = s->su;
     else if (k.rfind("p.", 0) == 0) v = s->sp;
-    else fail(PHASE_HIP_EINVAL, "keys are u.<solver key> or p.<solver key>");
+    else fail(PHASE_HIP_EINVAL, "keys are numPressureCorrections, u.<solver key> or p.<solver key>");
     ck(phase_hip_solver_setup(v, k.c_str() + 2, value), "setup", v->err.c_str());
   });
 }
@@ -756,6 +785,8 @@ int phase_hip_simple_fields(phase_hip_simple *s, double *uCell, double *pCell, d
 //   solvePEqn (:152-163): laplacian(dt / rho, p) == src::div(u); boundary
 //     faces; grad_f p; gradP = faceToCell(rho, rho)
 //   correctVelocity (:165-173): u -= dt / rho gradP (cells and every face)
+// numPressureCorrections = n repeats solveUEqn (after u1, u0 = u0, u),
+// solvePEqn and correctVelocity n times per step, as in phase_hip_fstep.
 struct phase_hip_multiphase {
   phase_hip_mesh *mesh = nullptr;
   int N = 0, F = 0, pin = -1;
@@ -768,6 +799,7 @@ struct phase_hip_multiphase {
   DBuf<double> fstF, fstC;  // surface tension (0 unless turned on)
   DBuf<int> uBc, pBc, gBc;
   phase::CelesteModel *st = nullptr;
+  int nCorr = 1;  // numPressureCorrections
   std::string err;
 };
 
@@ -924,49 +956,52 @@ void mpStep(phase_hip_multiphase *s, double dt, double *hist) {
   copyD(s->u0C.p, s->uC.p, 2 * (size_t)N, st);
   copyD(s->u1F.p, s->u0F.p, 2 * (size_t)F, st);
   copyD(s->u0F.p, s->uF.p, 2 * (size_t)F, st);
-  ck(phase_hip_field_face_to_cell(m, s->gpF.p, s->rhoC.p, s->rho0F.p, s->gpC.p), "gradP faceToCell", m->err.c_str());
-  ck(phase_hip_eq_ddt(s->uEq, dt, s->u0C.p), "fv::ddt", s->uEq->err.c_str());
-  ck(phase_hip_eq_scale_cells(s->uEq, s->rhoC.p), "rho *", s->uEq->err.c_str());
-  ck(phase_hip_eq_dive(s->tmp, s->u0F.p, s->u1F.p, 0.5, s->uBc.p, s->u0C.p, s->u1C.p, s->u0F.p, s->u1F.p), "fv::dive",
-     s->tmp->err.c_str());
-  ck(phase_hip_eq_scale_cells(s->tmp, s->rhoC.p), "rho *", s->tmp->err.c_str());
-  ck(phase_hip_eq_add(s->uEq, s->tmp, 1), "operator+", s->uEq->err.c_str());
-  ck(phase_hip_eq_laplacian(s->tmp, 2, 0., s->muF.p, s->mu0F.p, 0.5, s->uBc.p, s->uF.p, s->u0C.p, s->u0F.p),
-     "fv::laplacian", s->tmp->err.c_str());
-  // + src::src(fst + sg - gradP) is -= src::src(gradP - (fst + sg))
-  hipLaunchKernelGGL(kMomentumSource, dim3(g256(2L * N)), dim3(256), 0, st, 2L * N, s->gpC.p, s->fstC.p, s->sgC.p,
-                     s->srcT.p);
-  ck(phase_hip_field_src_src(m, 2, s->srcT.p, s->gsrc.p), "src::src", m->err.c_str());
-  ck(phase_hip_eq_sub_vector(s->tmp, s->gsrc.p), "operator+", s->tmp->err.c_str());
-  ck(phase_hip_eq_add(s->uEq, s->tmp, -1), "operator==", s->uEq->err.c_str());
-  hipLaunchKernelGGL(kToBlocks, dim3(g256(N)), dim3(256), 0, st, N, s->uC.p, s->uB.p);
-  const int uIt = solveInto(s->su, s->uEq, s->uB.p, "uEqn");
-  hipLaunchKernelGGL(kFromBlocks, dim3(g256(N)), dim3(256), 0, st, N, s->uB.p, s->uC.p);
-  hipLaunchKernelGGL(kDtOverRhoUpdate, dim3(g256(2L * N)), dim3(256), 0, st, 2L * N, dt, 1, s->rhoC.p, s->gpC.p,
-                     s->uC.p);
-  hipLaunchKernelGGL(kMultiphaseFaceVelocity, dim3(g256(F)), dim3(256), 0, st, F, dt, m->dFL.p, m->dFR.p, m->dVol.p,
-                     m->dSf.p, s->uBc.p, s->uC.p, s->rhoC.p, s->rhoF.p, s->fstC.p, s->fstF.p, s->sgC.p, s->sgF.p,
-                     s->uF.p);
-  PH_CHECK(hipGetLastError());
-  // solvePEqn
-  hipLaunchKernelGGL(kScalarOver, dim3(g256(F)), dim3(256), 0, st, F, dt, s->rhoF.p, s->mob.p);
-  ck(phase_hip_eq_laplacian(s->pEq, 3, 0., s->mob.p, nullptr, 1.0, s->pBc.p, s->pF.p, nullptr, nullptr),
-     "fv::laplacian", s->pEq->err.c_str());
-  ck(phase_hip_field_src_div(m, s->uF.p, s->m.p), "src::div", m->err.c_str());
-  ck(phase_hip_eq_sub_vector(s->pEq, s->m.p), "operator==", s->pEq->err.c_str());
-  if (s->pin >= 0)
-    hipLaunchKernelGGL(kPinRow, dim3(1), dim3(64), 0, st, s->pin, s->pEq->stride, s->pEq->cols.p, s->pEq->vals.p,
-                       s->pEq->cnt.p, s->pEq->rhs.p);
-  const int pIt = solveInto(s->sp, s->pEq, s->pC.p, "pEqn");
-  ck(phase_hip_field_interpolate_faces(m, 1, s->pC.p, s->pBc.p, s->pF.p), "setBoundaryFaces", m->err.c_str());
-  ck(phase_hip_field_grad(m, s->pC.p, s->pF.p, s->gpF.p, s->gpC.p), "grad p", m->err.c_str());
-  ck(phase_hip_field_face_to_cell(m, s->gpF.p, s->rhoC.p, s->rhoF.p, s->gpC.p), "gradP faceToCell", m->err.c_str());
-  // correctVelocity
-  hipLaunchKernelGGL(kDtOverRhoUpdate, dim3(g256(2L * N)), dim3(256), 0, st, 2L * N, dt, 0, s->rhoC.p, s->gpC.p,
-                     s->uC.p);
-  hipLaunchKernelGGL(kDtOverRhoUpdate, dim3(g256(2L * F)), dim3(256), 0, st, 2L * F, dt, 0, s->rhoF.p, s->gpF.p,
-                     s->uF.p);
-  PH_CHECK(hipGetLastError());
+  int uIt = 0, pIt = 0;
+  for (int corr = 0; corr < s->nCorr; ++corr) {
+    ck(phase_hip_field_face_to_cell(m, s->gpF.p, s->rhoC.p, s->rho0F.p, s->gpC.p), "gradP faceToCell", m->err.c_str());
+    ck(phase_hip_eq_ddt(s->uEq, dt, s->u0C.p), "fv::ddt", s->uEq->err.c_str());
+    ck(phase_hip_eq_scale_cells(s->uEq, s->rhoC.p), "rho *", s->uEq->err.c_str());
+    ck(phase_hip_eq_dive(s->tmp, s->u0F.p, s->u1F.p, 0.5, s->uBc.p, s->u0C.p, s->u1C.p, s->u0F.p, s->u1F.p), "fv::dive",
+       s->tmp->err.c_str());
+    ck(phase_hip_eq_scale_cells(s->tmp, s->rhoC.p), "rho *", s->tmp->err.c_str());
+    ck(phase_hip_eq_add(s->uEq, s->tmp, 1), "operator+", s->uEq->err.c_str());
+    ck(phase_hip_eq_laplacian(s->tmp, 2, 0., s->muF.p, s->mu0F.p, 0.5, s->uBc.p, s->uF.p, s->u0C.p, s->u0F.p),
+       "fv::laplacian", s->tmp->err.c_str());
+    // + src::src(fst + sg - gradP) is -= src::src(gradP - (fst + sg))
+    hipLaunchKernelGGL(kMomentumSource, dim3(g256(2L * N)), dim3(256), 0, st, 2L * N, s->gpC.p, s->fstC.p, s->sgC.p,
+                       s->srcT.p);
+    ck(phase_hip_field_src_src(m, 2, s->srcT.p, s->gsrc.p), "src::src", m->err.c_str());
+    ck(phase_hip_eq_sub_vector(s->tmp, s->gsrc.p), "operator+", s->tmp->err.c_str());
+    ck(phase_hip_eq_add(s->uEq, s->tmp, -1), "operator==", s->uEq->err.c_str());
+    hipLaunchKernelGGL(kToBlocks, dim3(g256(N)), dim3(256), 0, st, N, s->uC.p, s->uB.p);
+    uIt += solveInto(s->su, s->uEq, s->uB.p, "uEqn");
+    hipLaunchKernelGGL(kFromBlocks, dim3(g256(N)), dim3(256), 0, st, N, s->uB.p, s->uC.p);
+    hipLaunchKernelGGL(kDtOverRhoUpdate, dim3(g256(2L * N)), dim3(256), 0, st, 2L * N, dt, 1, s->rhoC.p, s->gpC.p,
+                       s->uC.p);
+    hipLaunchKernelGGL(kMultiphaseFaceVelocity, dim3(g256(F)), dim3(256), 0, st, F, dt, m->dFL.p, m->dFR.p, m->dVol.p,
+                       m->dSf.p, s->uBc.p, s->uC.p, s->rhoC.p, s->rhoF.p, s->fstC.p, s->fstF.p, s->sgC.p, s->sgF.p,
+                       s->uF.p);
+    PH_CHECK(hipGetLastError());
+    // solvePEqn
+    hipLaunchKernelGGL(kScalarOver, dim3(g256(F)), dim3(256), 0, st, F, dt, s->rhoF.p, s->mob.p);
+    ck(phase_hip_eq_laplacian(s->pEq, 3, 0., s->mob.p, nullptr, 1.0, s->pBc.p, s->pF.p, nullptr, nullptr),
+       "fv::laplacian", s->pEq->err.c_str());
+    ck(phase_hip_field_src_div(m, s->uF.p, s->m.p), "src::div", m->err.c_str());
+    ck(phase_hip_eq_sub_vector(s->pEq, s->m.p), "operator==", s->pEq->err.c_str());
+    if (s->pin >= 0)
+      hipLaunchKernelGGL(kPinRow, dim3(1), dim3(64), 0, st, s->pin, s->pEq->stride, s->pEq->cols.p, s->pEq->vals.p,
+                         s->pEq->cnt.p, s->pEq->rhs.p);
+    pIt += solveInto(s->sp, s->pEq, s->pC.p, "pEqn");
+    ck(phase_hip_field_interpolate_faces(m, 1, s->pC.p, s->pBc.p, s->pF.p), "setBoundaryFaces", m->err.c_str());
+    ck(phase_hip_field_grad(m, s->pC.p, s->pF.p, s->gpF.p, s->gpC.p), "grad p", m->err.c_str());
+    ck(phase_hip_field_face_to_cell(m, s->gpF.p, s->rhoC.p, s->rhoF.p, s->gpC.p), "gradP faceToCell", m->err.c_str());
+    // correctVelocity
+    hipLaunchKernelGGL(kDtOverRhoUpdate, dim3(g256(2L * N)), dim3(256), 0, st, 2L * N, dt, 0, s->rhoC.p, s->gpC.p,
+                       s->uC.p);
+    hipLaunchKernelGGL(kDtOverRhoUpdate, dim3(g256(2L * F)), dim3(256), 0, st, 2L * F, dt, 0, s->rhoF.p, s->gpF.p,
+                       s->uF.p);
+    PH_CHECK(hipGetLastError());
+  }
   if (hist) {
     // FractionalStep::maxDivergenceError on the corrected faces
     ck(phase_hip_field_src_div(m, s->uF.p, s->m.p), "src::div", m->err.c_str());
@@ -1076,12 +1111,18 @@ int phase_hip_multiphase_setup(phase_hip_multiphase *s, const char *key, const c
   return guardedMP(s, [&] {
     if (!key || !value) fail(PHASE_HIP_EINVAL, "null key/value");
     std::string k(key);
+    if (k == "numPressureCorrections") {
+      s->nCorr = parseCorrections(value);
+      return;
+    }
     phase_hip_solver *v = nullptr;
     size_t skip = 2;
     if (k.rfind("u.", 0) == 0) v = s->su;
     else if (k.rfind("p.", 0) == 0) v = s->sp;
     else if (k.rfind("gamma.", 0) == 0) v = s->sg, skip = 6;
-    else fail(PHASE_HIP_EINVAL, "keys are gamma.<solver key>, u.<solver key> or p.<solver key>");
+    else
+      fail(PHASE_HIP_EINVAL,
+           "keys are numPressureCorrections, gamma.<solver key>, u.<solver key> or p.<solver key>");
     ck(phase_hip_solver_setup(v, k.c_str() + skip, value), "setup", v->err.c_str());
   });
 }

@@ -99,14 +99,15 @@ def test_invalid_inputs(hip, case):
         hip.HipGhostCellIB(hip.HipMesh(xy, cptr, cind), [(0.5, 0.5, 0.2)])
 
 
-@pytest.mark.parametrize("omega", [None, 2.0])
-def test_fractional_step_past_a_cylinder(oracle, hip, omega):
+@pytest.mark.parametrize("omega,correctors", [(None, 1), (2.0, 1), (None, 2)])
+def test_fractional_step_past_a_cylinder(oracle, hip, omega, correctors):
     """FractionalStepGCIB on the device: a channel with inflow u = 1, a fixed
     cylinder of radius 0.25 (ghost-cell velocity and NORMAL_GRADIENT pressure
     rows), outlet p = 0; four steps against the restatement with direct solves:
     velocities and pressure within 1e-7, solid cells at the body's velocity
     (at rest, or spinning at omega about its centre: Motion::velocity rows in
-    the time step), fluid faces divergence-free."""
+    the time step), fluid faces divergence-free; one or two pressure-corrector
+    passes per step."""
     from oracle.fstep_oracle import FractionalStepOracle
     from oracle.ghostcell_oracle import GhostCellOracle
     xy, cptr, cind = G.rectilinear(48, 32, width=3.0, height=2.0)
@@ -132,12 +133,13 @@ def test_fractional_step_past_a_cylinder(oracle, hip, omega):
     for k in ("u", "p"):
         dev.setup(k + ".tolerance", 1e-12)
         dev.setup(k + ".maxIters", 20000)
+    dev.setup("numPressureCorrections", correctors)
     solid = og.types == 2
     _, cc = om.cells()
     w = 0.0 if omega is None else omega
     u_body = np.c_[-(cc[:, 1] - 1.0) * w, (cc[:, 0] - 1.0) * w]
     for step in range(4):
-        ref.step(0.01)
+        ref.step(0.01, correctors)
         hist = dev.advance(0.01)
         u, p, uf = dev.fields()
         print("step", step, "iters", hist[0, :2], "div", hist[0, 2], flush=True)

@@ -32,8 +32,8 @@ def _case(oracle, kind):
     return xy, cptr, cind, om, ubc, pbc, gam
 
 
-@pytest.mark.parametrize("kind", ["rect", "tri"])
-def test_multiphase_steps_match_oracle(oracle, hip, kind):
+@pytest.mark.parametrize("kind,correctors", [("rect", 1), ("tri", 1), ("rect", 2)])
+def test_multiphase_steps_match_oracle(oracle, hip, kind, correctors):
     from oracle.multiphase_oracle import MultiphaseOracle
     xy, cptr, cind, om, ubc, pbc, gam = _case(oracle, kind)
     F = om.n_faces
@@ -45,9 +45,10 @@ def test_multiphase_steps_match_oracle(oracle, hip, kind):
     for k in ("gamma", "u", "p"):
         dev.setup(k + ".tolerance", 1e-12)
         dev.setup(k + ".maxIters", 5000)
+    dev.setup("numPressureCorrections", correctors)
     dt = 2e-3
     for step in range(3):
-        ref.step(dt)
+        ref.step(dt, correctors)
         hist = dev.advance(dt)
         u, p, uf, g = dev.fields()
         print("step", step, "iters", hist[0, :3], "div", hist[0, 3], flush=True)
@@ -70,6 +71,9 @@ def test_multiphase_rejects_bad_input(hip):
     d = hip.HipMultiphase(hm, 1.0, 2.0, 1.0, 1.0, (0, -1), ubc, np.zeros(2 * F), np.zeros(hm.n_cells), pin_cell=0)
     with pytest.raises(hip.PhaseHipError):
         d.setup("q.tolerance", 1e-8)
+    for bad in ("0", "11", "2x", ""):
+        with pytest.raises(hip.PhaseHipError):
+            d.setup("numPressureCorrections", bad)
     with pytest.raises(hip.PhaseHipError):
         d.advance(-1.0)
 

@@ -76,10 +76,12 @@ def test_simple_cavity_re100_ghia(hip, oracle):
     assert np.abs(vv - GHIA_V).max() <= 0.02
 
 
-def test_fractional_step_matches_oracle(hip, oracle):
+@pytest.mark.parametrize("correctors", [1, 2])
+def test_fractional_step_matches_oracle(hip, oracle, correctors):
     """FractionalStep::solve on the device: time steps agree with the CPU
     restatement (oracle/fstep_oracle.py, direct solves) and the corrected
-    face velocities are discretely divergence-free."""
+    face velocities are discretely divergence-free; with one pass per step
+    (the reference's) and with two pressure-corrector passes."""
     from oracle.fstep_oracle import FractionalStepOracle
 
     om, hm, ubc, ub = cavity(oracle, hip, 16)
@@ -87,9 +89,10 @@ def test_fractional_step_matches_oracle(hip, oracle):
     dev = hip.HipFractionalStep(hm, 1.0, 0.01, ubc, ub, pin_cell=0)
     for k in ("u.tolerance", "p.tolerance"):
         dev.setup(k, "1e-13")
+    dev.setup("numPressureCorrections", correctors)
     for _ in range(3):
         for _ in range(4):
-            ref.step(0.01)
+            ref.step(0.01, correctors)
         hist = dev.advance(0.01, 4)
         u, p, uf = dev.fields()
         scale = np.abs(ref.uC).max()

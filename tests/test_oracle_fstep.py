@@ -16,10 +16,16 @@ device drivers they check:
   centreline data of Ghia, Ghia & Shin (1982), Table I/II.
 * PISO transient of the impulsively started cavity: converges (first order in
   dt) to the FractionalStep transient of the same flow.
+* FractionalStep with several pressure-corrector passes per step: the
+  Poiseuille and shear-mode pins hold unchanged (each pass starts from the
+  same old-time fields), and on the impulsively started cavity the passes of
+  one step contract to a fixed point (the change from pass to pass shrinks at
+  least fourfold), as a fixed-point iteration of one implicit step must.
 
 Each restatement takes a `plant` argument that plants one mistake (dropping
 u += dt grad p, the sign of src(grad p), the viscous theta, the ddt
-coefficient); the tests show the pins above catch every one of them.
+coefficient, old-time fields taken again before every corrector pass); the
+tests show the pins above catch every one of them.
 """
 import numpy as np
 import pytest
@@ -48,7 +54,7 @@ def _channel(O, nx, ny, L, H):
     return m, pid
 
 
-def poiseuille(O, ny, plant=None, L=2.0, H=1.0, p_in=1.0, mu=1.0, dt=0.02, steps=200):
+def poiseuille(O, ny, plant=None, L=2.0, H=1.0, p_in=1.0, mu=1.0, dt=0.02, steps=200, correctors=1):
     """Relative velocity error, max |v| and max pressure error at steady state."""
     m, pid = _channel(O, 2 * ny, ny, L, H)
     F = m.n_faces
@@ -59,7 +65,7 @@ def poiseuille(O, ny, plant=None, L=2.0, H=1.0, p_in=1.0, mu=1.0, dt=0.02, steps
     ref = FractionalStepOracle(m, 1.0, mu, ubc, np.zeros((F, 2)), p_bc=pbc, p_face0=pF, pin=-1, plant=plant)
     with np.errstate(all="ignore"):
         for _ in range(steps):
-            ref.step(dt)
+            ref.step(dt, correctors)
     _, cc = m.cells()
     u_ex = p_in / L / (2.0 * mu) * cc[:, 1] * (H - cc[:, 1])
     p_ex = p_in * (1.0 - cc[:, 0] / L)
@@ -67,7 +73,7 @@ def poiseuille(O, ny, plant=None, L=2.0, H=1.0, p_in=1.0, mu=1.0, dt=0.02, steps
             np.abs(ref.pC - p_ex).max())
 
 
-def shear_decay(O, ny, dt, plant=None, mu=0.1, T=1.0, L=2.0):
+def shear_decay(O, ny, dt, plant=None, mu=0.1, T=1.0, L=2.0, correctors=1):
     """Error of the decaying mode sin(pi y) exp(-nu pi^2 T), relative to its amplitude."""
     m, pid = _channel(O, 8, ny, L, 1.0)
     F = m.n_faces
@@ -80,7 +86,7 @@ def shear_decay(O, ny, dt, plant=None, mu=0.1, T=1.0, L=2.0):
                                  ncomp=2).reshape(F, 2)
     with np.errstate(all="ignore"):
         for _ in range(int(round(T / dt))):
-            ref.step(dt)
+            ref.step(dt, correctors)
     amp = np.exp(-mu * np.pi ** 2 * T)
     return np.abs(ref.uC[:, 0] - amp * np.sin(np.pi * cc[:, 1])).max() / amp
 
@@ -187,3 +193,42 @@ def test_piso_transient_converges_to_fractional_step(oracle):
 def test_piso_pins_catch_planted_mistakes(oracle, plant):
     d = piso_vs_fstep(oracle, plant)
     assert not _piso_transient_ok(d), (plant, d)
+
+
+def corrector_contraction(O, plant=None, n=16, dt=0.02, passes=4):
+    """Changes of u from pass to pass within one step of the impulsively
+    started cavity (after 10 ordinary steps)."""
+    om, ubc, ub = _cavity(O, n)
+    ref = FractionalStepOracle(om, 1.0, 0.01, ubc, ub, pin=0, plant=plant)
+    for _ in range(10):
+        ref.step(dt)
+    state = (ref.uC.copy(), ref.uF.copy(), ref.pC.copy(), ref.pF.copy(), ref.gpC.copy(), ref.gpF.copy())
+    out = []
+    for k in range(1, passes + 1):
+        ref.uC, ref.uF, ref.pC, ref.pF, ref.gpC, ref.gpF = (a.copy() for a in state)
+        ref.step(dt, correctors=k)
+        out.append(ref.uC.copy())
+    return [np.abs(b - a).max() for a, b in zip(out, out[1:])]
+
+
+def _contraction_ok(d):
+    return d[0] > 0.0 and d[1] <= 0.25 * d[0] and d[2] <= 0.25 * d[1]
+
+
+def test_fstep_corrector_passes_keep_the_pins(oracle):
+    e8, e16 = poiseuille(oracle, 8, correctors=2), poiseuille(oracle, 16, correctors=2)
+    assert _poiseuille_ok(e8, e16), (e8, e16)
+    a, b = shear_decay(oracle, 16, 0.1, correctors=2), shear_decay(oracle, 32, 0.05, correctors=2)
+    assert _shear_ok(a, b), (a, b)
+
+
+def test_fstep_corrector_passes_contract(oracle):
+    d = corrector_contraction(oracle)
+    assert _contraction_ok(d), d
+
+
+def test_fstep_corrector_pins_catch_planted_mistake(oracle):
+    a, b = shear_decay(oracle, 16, 0.1, "save_per_corrector", correctors=2), \
+        shear_decay(oracle, 32, 0.05, "save_per_corrector", correctors=2)
+    d = corrector_contraction(oracle, "save_per_corrector")
+    assert not _shear_ok(a, b) and not _contraction_ok(d), (a, b, d)
