@@ -23,11 +23,11 @@ struct MlScalars {
 
 template <class T>
 struct MlVecs {
-  // r: right-hand side, e: solution of this level's coarse solve; t, w, y:
+  // r: right-hand side, e: solution of this level's coarse solve; t, w, y, u:
   // cycle work; z1, q1, r1, z2: the K cycle's iterations
-  DBuf<T> r, e, t, w, y, z1, q1, r1, z2;
+  DBuf<T> r, e, t, w, y, u, z1, q1, r1, z2;
   void alloc(int n) {
-    for (DBuf<T> *b : {&r, &e, &t, &w, &y, &z1, &q1, &r1, &z2}) b->alloc(n);
+    for (DBuf<T> *b : {&r, &e, &t, &w, &y, &u, &z1, &q1, &r1, &z2}) b->alloc(n);
   }
   // the finest level uses the cycle work vectors only
   void allocCycle(int n) {
@@ -205,6 +205,131 @@ __global__ __launch_bounds__(kGjBlock) void kDenseInverse(int n, double *__restr
   }
 }
 
+// The same inverse with the matrix in registers, for n <= 32 RB: the 1024
+// threads form a 32 x 32 grid and thread (tr, tc) holds the entries (tr + 32 a,
+// tc + 32 b), a, b < RB.  In-place Gauss-Jordan with partial pivoting and no
+// row exchanges: step k takes as pivot the unused row p_k with the largest
+// |a_ik| (lowest i on a tie) and eliminates column k from every other row,
+// replacing it by the column of the inverse being built.  That is the
+// unpivoted in-place inversion of B = P A (B row k = A row p_k), so physical
+// row p_k ends holding row k of B^-1 = A^-1 P^T: inv[k][p_j] = R[p_k][j].
+// Two barriers per step (column k published; pivot row published); the
+// column and row buffers alternate between steps, so a step never overwrites
+// what a slower thread of the previous step still reads.
+template <int RB>
+__global__ __launch_bounds__(kGjBlock) void kDenseInverseReg(int n, const double *__restrict__ a0,
+                                                             double *__restrict__ inv, int *err) {
+  constexpr int S = 32 * RB;
+  __shared__ double fcol[2][S], prow[2][S];
+  __shared__ int pivOf[S], kOf[S], used[S];
+  const int tid = threadIdx.x, tr = tid >> 5, tc = tid & 31, lane = tid & 63;
+  double A[RB][RB];
+#pragma unroll
+  for (int a = 0; a < RB; ++a)
+#pragma unroll
+    for (int b = 0; b < RB; ++b) {
+      const int i = tr + 32 * a, j = tc + 32 * b;
+      A[a][b] = (i < n && j < n) ? a0[(size_t)i * n + j] : 0.;
+    }
+  for (int i = tid; i < S; i += kGjBlock) {
+    used[i] = 0;
+    pivOf[i] = 0;
+    kOf[i] = 0;
+  }
+  __syncthreads();
+  for (int k = 0; k < n; ++k) {
+    const int buf = k & 1, kb = k >> 5, kc = k & 31;
+    if (tc == kc) {
+#pragma unroll
+      for (int a = 0; a < RB; ++a) {
+        double v = 0.;
+#pragma unroll
+        for (int b = 0; b < RB; ++b)
+          if (b == kb) v = A[a][b];
+        const int i = tr + 32 * a;
+        if (i < n) fcol[buf][i] = v;
+      }
+    }
+    __syncthreads();
+    // every wavefront finds the same pivot (a NaN entry is taken, so some
+    // unused row always is, and flagged below)
+    double best = -1.;
+    int bi = n;
+    for (int i = lane; i < n; i += 64)
+      if (!used[i]) {
+        const double v = fabs(fcol[buf][i]);
+        if (!(v <= best)) {
+          best = v;
+          bi = i;
+        }
+      }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const double ov = __shfl_xor(best, o, 64);
+      const int oi = __shfl_xor(bi, o, 64);
+      if (ov > best || (ov == best && oi < bi)) {
+        best = ov;
+        bi = oi;
+      }
+    }
+    const int p = bi < n ? bi : 0;
+    const double pv = fcol[buf][p];
+    const double pivinv = pv != 0. ? 1. / pv : 0.;
+    const int pb = p >> 5, pr = p & 31;
+    if (tr == pr) {
+#pragma unroll
+      for (int b = 0; b < RB; ++b) {
+        double v = 0.;
+#pragma unroll
+        for (int a = 0; a < RB; ++a)
+          if (a == pb) v = A[a][b];
+        const int j = tc + 32 * b;
+        if (j < n) prow[buf][j] = (j == k ? 1. : v) * pivinv;
+      }
+    }
+    __syncthreads();
+    if (tid == 0) {
+      // after the barrier: every wavefront has finished this step's search
+      used[p] = 1;
+      pivOf[k] = p;
+      kOf[p] = k;
+      if (!(fabs(pv) > 0.)) atomicOr(err, 4);
+    }
+#pragma unroll
+    for (int a = 0; a < RB; ++a) {
+      const int i = tr + 32 * a;
+      const double f = i < n ? fcol[buf][i] : 0.;
+#pragma unroll
+      for (int b = 0; b < RB; ++b) {
+        const int j = tc + 32 * b;
+        if (i < n && j < n) {
+          const double rs = prow[buf][j];
+          A[a][b] = i == p ? rs : fma(-f, rs, j == k ? 0. : A[a][b]);
+        }
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int a = 0; a < RB; ++a)
+#pragma unroll
+    for (int b = 0; b < RB; ++b) {
+      const int i = tr + 32 * a, j = tc + 32 * b;
+      if (i < n && j < n) inv[(size_t)kOf[i] * n + pivOf[j]] = A[a][b];
+    }
+}
+
+// Largest n the register Gauss-Jordan takes (RB = 6: 36 entries per thread).
+constexpr int kDenseRegMax = 192;
+
+void launchDenseInverse(hipStream_t st, int n, double *dense, double *inv, int *err) {
+  if (n <= 64) hipLaunchKernelGGL(kDenseInverseReg<2>, dim3(1), dim3(kGjBlock), 0, st, n, dense, inv, err);
+  else if (n <= 128) hipLaunchKernelGGL(kDenseInverseReg<4>, dim3(1), dim3(kGjBlock), 0, st, n, dense, inv, err);
+  else if (n <= kDenseRegMax)
+    hipLaunchKernelGGL(kDenseInverseReg<6>, dim3(1), dim3(kGjBlock), 0, st, n, dense, inv, err);
+  else hipLaunchKernelGGL(kDenseInverse, dim3(1), dim3(kGjBlock), 0, st, n, dense, inv, err);
+}
+
 // e = inv r (one wavefront per row, FP64 products)
 template <class T>
 __global__ __launch_bounds__(256) void kDenseApply(int n, const double *__restrict__ inv, const T *__restrict__ r,
@@ -259,11 +384,80 @@ __global__ void kMlProlongAdd(int n, const int *__restrict__ pIdx, const T *__re
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) e[i] += ec[pIdx[i]];
 }
 
-// e += w
-template <class T>
-__global__ void kMlAdd(int n, const T *__restrict__ w, T *__restrict__ e, const Scalars *sc) {
+// Row `row` of A x in SELL-64 for a lane whose slice need not be the rest of
+// the wavefront's: the products and their order are sellRow's (x[c] comes
+// from gx(c)), so the result is bit-identical to it.
+template <class T, class TV, class GX>
+__device__ __forceinline__ T sellRowAt(int row, const int *__restrict__ sliceOff, const int *__restrict__ col,
+                                       const TV *__restrict__ val, GX &&gx) {
+  using A = typename std::conditional<(sizeof(TV) > sizeof(T)), TV, T>::type;
+  const int slice = row >> 6, lane = row & 63;
+  const int o = sliceOff[slice];
+  const int K = (sliceOff[slice + 1] - o) >> 6;
+  A acc = 0;
+  for (int e0 = 0; e0 < K; e0 += 4) {
+    int c[4];
+    TV a[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (e0 + q < K) {
+        const int idx = o + ((e0 + q) << 6) + lane;
+        c[q] = col[idx];
+        a[q] = val[idx];
+      }
+    T xv[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (e0 + q < K) xv[q] = gx(c[q] >= 0 ? c[q] : 0);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (e0 + q < K && c[q] >= 0) acc = fma((A)a[q], (A)xv[q], acc);
+  }
+  return (T)acc;
+}
+
+// Coarse levels (launch-bound), kMlResid and kMlRestrict in one launch:
+// rc[k] = sum over the rows of aggregate restRow[k] of (b - A x), one
+// wavefront per coarse row; each row's residual is rounded to T before the
+// FP64 sum, as the two kernels do.
+template <class T, class TV>
+__global__ __launch_bounds__(256) void kMlResidRestrict(int nc, const int *__restrict__ restRow,
+                                                        const int *__restrict__ aggPtr,
+                                                        const int *__restrict__ sliceOff,
+                                                        const int *__restrict__ col, const TV *__restrict__ val,
+                                                        const T *__restrict__ x, const T *__restrict__ b,
+                                                        T *__restrict__ rc, const Scalars *sc) {
   if (sc->done) return;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) e[i] += w[i];
+  const int k = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (k >= nc) return;
+  const int a = restRow ? restRow[k] : k;
+  const int row = aggPtr[a] + lane;
+  double d = 0.;
+  if (row < aggPtr[a + 1]) {
+    const T t = b[row] - sellRowAt<T, TV>(row, sliceOff, col, val, [&](int c) { return x[c]; });
+    d = (double)t;
+  }
+  d = waveSum(d);
+  if (lane == 0) rc[k] = (T)d;
+}
+
+// Coarse levels, kMlProlongAdd and kMlResid in one launch:
+// u = z + P ec and t = b - A u, where A's products read z + P ec at the
+// gathered columns (the sums kMlProlongAdd would store) and z stays as it is.
+template <class T, class TV>
+__global__ __launch_bounds__(kRedBlock) void kMlResidProlong(int n, int nSlices, const int *__restrict__ sliceOff,
+                                                             const int *__restrict__ col, const TV *__restrict__ val,
+                                                             const int *__restrict__ pIdx,
+                                                             const T *__restrict__ ec, const T *__restrict__ z,
+                                                             const T *__restrict__ b, T *__restrict__ t,
+                                                             T *__restrict__ u, const Scalars *sc) {
+  if (sc->done) return;
+  const int slice = blockIdx.x * (kRedBlock / 64) + (threadIdx.x >> 6);
+  const int row = slice * 64 + (threadIdx.x & 63);
+  if (slice >= nSlices || row >= n) return;
+  const T y = sellRowAt<T, TV>(row, sliceOff, col, val, [&](int c) { return (T)(z[c] + ec[pIdx[c]]); });
+  t[row] = b[row] - y;
+  u[row] = z[row] + ec[pIdx[row]];
 }
 
 // q = A p ; partials of (p, q) and (p, r)
@@ -429,6 +623,12 @@ void coarseCorrection(phase_hip_solver *s, Multilevel &M, int l, const Scalars *
 // Smoothing, coarse correction and smoothing at level `lvl` (nested solver c,
 // its work vectors V): z = B r.  When `pre` is false the pre-smoothing has
 // already put M^-1 r into z.
+//
+// On the coarse levels (c nested), where every launch costs more than its
+// arithmetic, the residual and its restriction run as one launch, and so do
+// the prolongation and the post-smoothing residual (into the work vector u);
+// at every level the post-smoothing sweep adds its result into z as it
+// finishes each row.  The arithmetic is that of the separate kernels.
 template <class T>
 void smoothCorrectSmooth(phase_hip_solver *s, Multilevel &M, phase_hip_solver *c, MlVecs<T> &V, int next,
                          const Scalars *sc, const T *r, T *z, bool pre, bool post = true) {
@@ -437,14 +637,28 @@ void smoothCorrectSmooth(phase_hip_solver *s, Multilevel &M, phase_hip_solver *c
   const IluDev<T> L = iluDev<T>(c, facOf<T>(c));
   if (pre) launchLU<T, kProNone>(c, L, sc, r, V.y.p, (T *)nullptr, (const T *)nullptr, (const T *)nullptr, z);
   const int gs = spmvGrid(c);
+  if (c != s && post) {
+    MlLevel &N = M.lv[next];
+    MlVecs<T> &NV = vecs<T>(N);
+    hipLaunchKernelGGL((kMlResidRestrict<T, double>), dim3((N.n + 3) / 4), dim3(256), 0, st, N.n,
+                       N.sol ? N.restRow.p : nullptr, N.finerAggPtr.p, c->dSliceOff.p, c->dSellCol.p,
+                       c->dSell64.p, (const T *)z, r, NV.r.p, sc);
+    coarseSolve<T>(s, M, next, sc, NV.r.p, NV.e.p);
+    hipLaunchKernelGGL((kMlResidProlong<T, double>), dim3(gs), dim3(kRedBlock), 0, st, n, c->ord.nSlices,
+                       c->dSliceOff.p, c->dSellCol.p, c->dSell64.p, N.pIdx.p, (const T *)NV.e.p, (const T *)z, r,
+                       V.t.p, V.u.p, sc);
+    launchLU<T, kProNone>(c, L, sc, V.t.p, V.y.p, (T *)nullptr, (const T *)nullptr, (const T *)nullptr, V.w.p,
+                          nullptr, (const T *)V.u.p, z);
+    return;
+  }
   hipLaunchKernelGGL((kMlResid<T, double>), dim3(gs), dim3(kRedBlock), 0, st, n, c->ord.nSlices, c->dSliceOff.p,
                      c->dSellCol.p, c->dSell64.p, (const T *)z, r, V.t.p, sc);
   coarseCorrection<T>(s, M, next, sc, V.t.p, z);
   if (!post) return;
   hipLaunchKernelGGL((kMlResid<T, double>), dim3(gs), dim3(kRedBlock), 0, st, n, c->ord.nSlices, c->dSliceOff.p,
                      c->dSellCol.p, c->dSell64.p, (const T *)z, r, V.t.p, sc);
-  launchLU<T, kProNone>(c, L, sc, V.t.p, V.y.p, (T *)nullptr, (const T *)nullptr, (const T *)nullptr, V.w.p);
-  hipLaunchKernelGGL((kMlAdd<T>), dim3(elemGrid(n)), dim3(256), 0, st, n, (const T *)V.w.p, z, sc);
+  launchLU<T, kProNone>(c, L, sc, V.t.p, V.y.p, (T *)nullptr, (const T *)nullptr, (const T *)nullptr, V.w.p,
+                        nullptr, (const T *)z, z);
 }
 
 template <class T>
@@ -574,7 +788,7 @@ void mlEnsureValues(phase_hip_solver *s, bool single) {
                            finer, L.cVals.p);
         hipLaunchKernelGGL(kDenseFill, dim3(elemGrid(L.n)), dim3(256), 0, st, L.n, L.cRowPtr.p, L.cColInd.p,
                            L.cVals.p, L.dense.p);
-        hipLaunchKernelGGL(kDenseInverse, dim3(1), dim3(kGjBlock), 0, st, L.n, L.dense.p, L.inv.p, s->dErr.p);
+        launchDenseInverse(st, L.n, L.dense.p, L.inv.p, s->dErr.p);
         PH_CHECK(hipGetLastError());
         int e = 0;
         PH_CHECK(hipMemcpyAsync(&e, s->dErr.p, sizeof(int), hipMemcpyDeviceToHost, st));

@@ -1288,7 +1288,7 @@ int phase_hip_solver_time_kernels(phase_hip_solver *s, int reps, double *ms, int
           o3[0] = timeIt([&] {
                     for (int c = C - 2; c >= 0; --c)
                       hipLaunchKernelGGL((kUSolve<T, W>), dim3(cp[c + 1] - cp[c]), dim3(64), 0, s->stream, L,
-                                         cp[c], noSc, (const T *)y, x);
+                                         cp[c], noSc, (const T *)y, x, (const T *)nullptr, (T *)nullptr);
                   }) / (C - 1);
           o3[2] = timeIt([&] {
                     for (int c = 0; c + 1 < C; ++c)
@@ -1301,7 +1301,7 @@ int phase_hip_solver_time_kernels(phase_hip_solver *s, int reps, double *ms, int
           o3[1] = timeIt([&] {
             hipLaunchKernelGGL((kLUSolve<T, kProNone, W>), dim3(cp[C] - cp[C - 1]), dim3(64), 0, s->stream, L,
                                cp[C - 1], noSc, in, (const T *)y, (T *)nullptr, (const T *)nullptr,
-                               (const T *)nullptr, x, (double *)nullptr);
+                               (const T *)nullptr, x, (double *)nullptr, (const T *)nullptr, (T *)nullptr);
           });
         if (C >= 2)
           oc[0] = timeIt([&] {
@@ -1314,7 +1314,7 @@ int phase_hip_solver_time_kernels(phase_hip_solver *s, int reps, double *ms, int
           oc[1] = timeIt([&] {
             hipLaunchKernelGGL((kLUSolve<T, kProCgR, W>), dim3(cp[C] - cp[C - 1]), dim3(64), 0, s->stream, L,
                                cp[C - 1], noSc, (const T *)K.p.p, (const T *)y, K.r.p, (const T *)nullptr,
-                               (const T *)K.v.p, x, s->dR64.p);
+                               (const T *)K.v.p, x, s->dR64.p, (const T *)nullptr, (T *)nullptr);
           });
       });
     };

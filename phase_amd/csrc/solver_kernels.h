@@ -895,10 +895,14 @@ __global__ __launch_bounds__(64) void kLSolve(IluDev<T> L, int blockBase, const 
   }
 }
 
-// Backward sweep x = U^{-1} y (colours in reverse order).
+// Backward sweep x = U^{-1} y (colours in reverse order).  With addTo set,
+// the finished x of each row is also added to addFrom's value into addTo
+// (z = z + M^-1 t of a multigrid post-smoothing, without a launch of its own;
+// addFrom may be addTo).
 template <class T, int W = 8>
 __global__ __launch_bounds__(64) void kUSolve(IluDev<T> L, int blockBase, const Scalars *sc,
-                                              const T *__restrict__ y, T *__restrict__ x) {
+                                              const T *__restrict__ y, T *__restrict__ x,
+                                              const T *addFrom, T *addTo) {
   __shared__ T xlds[64];
   const int b = colourBlock(blockBase);
   const int lane = threadIdx.x;
@@ -910,7 +914,10 @@ __global__ __launch_bounds__(64) void kUSolve(IluDev<T> L, int blockBase, const 
   UPre<T, W> u;
   uLoad<T, W>(L, B, act, rec.row, rec.info, lane, u);
   const T acc = uSolve<T, W>(L, B, act, lane, act ? y0 : (T)0, x, xlds, u);
-  if (act) x[rec.row] = acc;
+  if (act) {
+    x[rec.row] = acc;
+    if (addTo) addTo[rec.row] = addFrom[rec.row] + acc;
+  }
 }
 
 // Last colour of a preconditioner apply: blocks of
@@ -923,7 +930,8 @@ template <class T, int PRO, int W = 8>
 __global__ __launch_bounds__(64) void kLUSolve(IluDev<T> L, int blockBase, const Scalars *sc,
                                                const T *__restrict__ in, const T *__restrict__ y,
                                                T *__restrict__ pv, const T *__restrict__ r,
-                                               const T *__restrict__ v, T *__restrict__ x, double *__restrict__ xs) {
+                                               const T *__restrict__ v, T *__restrict__ x, double *__restrict__ xs,
+                                               const T *addFrom, T *addTo) {
   __shared__ T ylds[64];
   __shared__ T xlds[64];
   const int b = colourBlock(blockBase);
@@ -942,6 +950,7 @@ __global__ __launch_bounds__(64) void kLUSolve(IluDev<T> L, int blockBase, const
     if (PRO != kProNone) pv[rec.row] = pro;
     if (PRO == kProCgR) xs[rec.row] = xo;
     x[rec.row] = acc;
+    if (addTo) addTo[rec.row] = addFrom[rec.row] + acc;
   }
 }
 

@@ -74,15 +74,17 @@ void launchU(phase_hip_solver *s, const IluDev<T> &L, const Scalars *sc, const T
     const std::vector<int> &cp = s->ord.colourPtr;
     for (int c = (int)cp.size() - 2; c >= 0; --c)
       if (cp[c + 1] > cp[c])
-        hipLaunchKernelGGL((kUSolve<T, W>), dim3(cp[c + 1] - cp[c]), dim3(64), 0, s->stream, L, cp[c], sc, y, x);
+        hipLaunchKernelGGL((kUSolve<T, W>), dim3(cp[c + 1] - cp[c]), dim3(64), 0, s->stream, L, cp[c], sc, y, x,
+                           (const T *)nullptr, (T *)nullptr);
   });
 }
 
 // out = U^{-1} L^{-1} (right-hand side formed by PRO); the last forward colour
-// and the first backward colour run fused in one launch.
+// and the first backward colour run fused in one launch.  With addTo set,
+// addTo = addFrom + out as well (row by row, in the launch that finishes it).
 template <class T, int PRO>
 void launchLU(phase_hip_solver *s, const IluDev<T> &L, const Scalars *sc, const T *in, T *y, T *pv, const T *r,
-              const T *v, T *out, double *xs = nullptr) {
+              const T *v, T *out, double *xs = nullptr, const T *addFrom = nullptr, T *addTo = nullptr) {
   const std::vector<int> &cp = s->ord.colourPtr;
   const int C = (int)cp.size() - 1;
   if (C < 1) return;
@@ -94,11 +96,11 @@ void launchLU(phase_hip_solver *s, const IluDev<T> &L, const Scalars *sc, const 
                            y, pv, r, v, xs);
     if (cp[C] > cp[C - 1])
       hipLaunchKernelGGL((kLUSolve<T, PRO, W>), dim3(cp[C] - cp[C - 1]), dim3(64), 0, s->stream, L, cp[C - 1], sc,
-                         in, (const T *)y, pv, r, v, out, xs);
+                         in, (const T *)y, pv, r, v, out, xs, addFrom, addTo);
     for (int c = C - 2; c >= 0; --c)
       if (cp[c + 1] > cp[c])
         hipLaunchKernelGGL((kUSolve<T, W>), dim3(cp[c + 1] - cp[c]), dim3(64), 0, s->stream, L, cp[c], sc,
-                           (const T *)y, out);
+                           (const T *)y, out, addFrom, addTo);
   });
 }
 
