@@ -161,6 +161,12 @@ int phase_hip_solver_ordering_info(const phase_hip_solver *s, int *nBlocks, int 
  * is inverted densely); nLevels = 0 before the first multigrid solve of a
  * pattern.  No counterpart in the reference. */
 int phase_hip_solver_multigrid_info(const phase_hip_solver *s, int *nLevels, int *sizes, int cap);
+/* The densely inverted last level of the multigrid hierarchy after a solve:
+ * its Galerkin matrix and the inverse the cycle applies, n x n row-major in
+ * the same row order (n must equal the last entry of
+ * phase_hip_solver_multigrid_info).  For checking; no counterpart in the
+ * reference. */
+int phase_hip_solver_multigrid_coarsest(phase_hip_solver *s, int n, double *matrix, double *inverse);
 
 /* Export the system held on the device, in the ORIGINAL row/column numbering
  * and compact CSR form (empty slots dropped), each row in the solver's slot

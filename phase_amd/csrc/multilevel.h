@@ -55,5 +55,8 @@ template <class T>
 void mlFinishApply(phase_hip_solver *s, const Scalars *sc, const T *rhs, T *out);
 // Level sizes (fine first) for reporting.
 std::vector<int> mlLevelSizes(const phase_hip_solver *s);
+// The densely inverted last level: its matrix and inverse, n x n row-major
+// (n = the last level size), copied to host memory.
+void mlCoarsest(phase_hip_solver *s, int n, double *matrix, double *inverse);
 
 }  // namespace phase

@@ -40,6 +40,9 @@ struct MlVecs {
 struct MlLevel {
   int n = 0;
   phase_hip_solver *sol = nullptr;  // nested solver; null for the dense last level
+  bool jacobi = false;              // damped Jacobi smoothing (dinv) instead of the ILU(0) factor
+  DBuf<double> dinv64;              // omega / a_ii in solver order
+  DBuf<float> dinv32;
   // maps from the finer level
   DBuf<int> galPtr, galSrc;         // coarse slot q = sum of finer caller slots galSrc[galPtr[q] ..]
   DBuf<int> restRow;                // this level's solver row -> aggregate of the finer level (dense: identity)
@@ -84,6 +87,16 @@ MlVecs<double> &fineVecs<double>(Multilevel &M) {
 template <>
 MlVecs<float> &fineVecs<float>(Multilevel &M) {
   return M.f32;
+}
+template <class T>
+const T *dinvOf(const MlLevel &L);
+template <>
+const double *dinvOf<double>(const MlLevel &L) {
+  return L.dinv64.p;
+}
+template <>
+const float *dinvOf<float>(const MlLevel &L) {
+  return L.dinv32.p;
 }
 // the ILU(0) factor a level applies in precision T
 template <class T>
@@ -552,6 +565,64 @@ __global__ __launch_bounds__(kRedBlock) void kMlSecond(int n, const Scalars *sc,
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] += a * p2[i];
 }
 
+// dinv[i] = omega / a_ii (the diagonal found in the row's SELL-64 entries)
+__global__ void kMlDiagInv(int n, const int *__restrict__ sliceOff, const int *__restrict__ col,
+                           const double *__restrict__ val, double omega, double *__restrict__ dinv) {
+  for (int row = blockIdx.x * blockDim.x + threadIdx.x; row < n; row += gridDim.x * blockDim.x) {
+    const int slice = row >> 6, lane = row & 63;
+    const int o = sliceOff[slice], K = (sliceOff[slice + 1] - o) >> 6;
+    double d = 0.;
+    for (int e = 0; e < K; ++e)
+      if (col[o + (e << 6) + lane] == row) d += val[o + (e << 6) + lane];
+    dinv[row] = d != 0. ? omega / d : 0.;
+  }
+}
+
+// Jacobi pre-smoothing, residual and restriction in one launch (one
+// wavefront per coarse row): z = dinv r on the aggregate's rows (stored),
+// rc[k] = sum over them of r - A z, with A's products reading dinv_j r_j.
+template <class T, class TV>
+__global__ __launch_bounds__(256) void kMlJacResidRestrict(int nc, const int *__restrict__ restRow,
+                                                           const int *__restrict__ aggPtr,
+                                                           const int *__restrict__ sliceOff,
+                                                           const int *__restrict__ col, const TV *__restrict__ val,
+                                                           const T *__restrict__ dinv, const T *__restrict__ r,
+                                                           T *__restrict__ z, T *__restrict__ rc,
+                                                           const Scalars *sc) {
+  if (sc->done) return;
+  const int k = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (k >= nc) return;
+  const int a = restRow ? restRow[k] : k;
+  const int row = aggPtr[a] + lane;
+  double d = 0.;
+  if (row < aggPtr[a + 1]) {
+    z[row] = dinv[row] * r[row];
+    const T t = r[row] - sellRowAt<T, TV>(row, sliceOff, col, val, [&](int c) { return dinv[c] * r[c]; });
+    d = (double)t;
+  }
+  d = waveSum(d);
+  if (lane == 0) rc[k] = (T)d;
+}
+
+// Prolongation, residual and Jacobi post-smoothing in one launch:
+// u = z + P ec; out = u + dinv (r - A u) (z and out distinct).
+template <class T, class TV>
+__global__ __launch_bounds__(kRedBlock) void kMlJacResidProlong(int n, int nSlices, const int *__restrict__ sliceOff,
+                                                                const int *__restrict__ col,
+                                                                const TV *__restrict__ val,
+                                                                const int *__restrict__ pIdx,
+                                                                const T *__restrict__ ec, const T *__restrict__ z,
+                                                                const T *__restrict__ r, const T *__restrict__ dinv,
+                                                                T *__restrict__ out, const Scalars *sc) {
+  if (sc->done) return;
+  const int slice = blockIdx.x * (kRedBlock / 64) + (threadIdx.x >> 6);
+  const int row = slice * 64 + (threadIdx.x & 63);
+  if (slice >= nSlices || row >= n) return;
+  const T y = sellRowAt<T, TV>(row, sliceOff, col, val, [&](int c) { return (T)(z[c] + ec[pIdx[c]]); });
+  const T u = z[row] + ec[pIdx[row]];
+  out[row] = u + dinv[row] * (r[row] - y);
+}
+
 // ---------------------------------------------------------------- host side
 phase_hip_solver *makeNested(phase_hip_solver *s) {
   auto *c = new phase_hip_solver();
@@ -664,7 +735,23 @@ void smoothCorrectSmooth(phase_hip_solver *s, Multilevel &M, phase_hip_solver *c
 template <class T>
 void cycle(phase_hip_solver *s, Multilevel &M, int l, const Scalars *sc, const T *r, T *e) {
   MlLevel &L = M.lv[l];
-  smoothCorrectSmooth<T>(s, M, L.sol, vecs<T>(L), l + 1, sc, r, e, true);
+  if (!L.jacobi) {
+    smoothCorrectSmooth<T>(s, M, L.sol, vecs<T>(L), l + 1, sc, r, e, true);
+    return;
+  }
+  // damped Jacobi smoothing: two launches and the coarse solve
+  phase_hip_solver *c = L.sol;
+  MlVecs<T> &V = vecs<T>(L);
+  MlLevel &N = M.lv[l + 1];
+  MlVecs<T> &NV = vecs<T>(N);
+  hipStream_t st = s->stream;
+  hipLaunchKernelGGL((kMlJacResidRestrict<T, double>), dim3((N.n + 3) / 4), dim3(256), 0, st, N.n,
+                     N.sol ? N.restRow.p : nullptr, N.finerAggPtr.p, c->dSliceOff.p, c->dSellCol.p, c->dSell64.p,
+                     dinvOf<T>(L), r, V.w.p, NV.r.p, sc);
+  coarseSolve<T>(s, M, l + 1, sc, NV.r.p, NV.e.p);
+  hipLaunchKernelGGL((kMlJacResidProlong<T, double>), dim3(spmvGrid(c)), dim3(kRedBlock), 0, st, c->rank,
+                     c->ord.nSlices, c->dSliceOff.p, c->dSellCol.p, c->dSell64.p, N.pIdx.p, (const T *)NV.e.p,
+                     (const T *)V.w.p, r, dinvOf<T>(L), e, sc);
 }
 
 }  // namespace
@@ -690,6 +777,21 @@ void mlValuesChanged(phase_hip_solver *s) {
 }
 
 std::vector<int> mlLevelSizes(const phase_hip_solver *s) { return s->ml ? s->ml->sizes : std::vector<int>(); }
+
+void mlCoarsest(phase_hip_solver *s, int n, double *matrix, double *inverse) {
+  if (!s->ml || s->ml->lv.empty() || !s->ml->valuesValid) fail(PHASE_HIP_ESTATE, "no multigrid hierarchy");
+  MlLevel &L = s->ml->lv.back();
+  if (L.sol) fail(PHASE_HIP_ESTATE, "the last level is not dense");
+  if (n != L.n) fail(PHASE_HIP_EINVAL, "the last level has " + std::to_string(L.n) + " rows");
+  hipStream_t st = s->stream;
+  // the matrix again from its CSR values (the global-memory inverse overwrites it)
+  hipLaunchKernelGGL(kDenseFill, dim3(elemGrid(L.n)), dim3(256), 0, st, L.n, L.cRowPtr.p, L.cColInd.p, L.cVals.p,
+                     L.dense.p);
+  const size_t bytes = (size_t)L.n * L.n * sizeof(double);
+  if (matrix) PH_CHECK(hipMemcpyAsync(matrix, L.dense.p, bytes, hipMemcpyDeviceToHost, st));
+  if (inverse) PH_CHECK(hipMemcpyAsync(inverse, L.inv.p, bytes, hipMemcpyDeviceToHost, st));
+  PH_CHECK(hipStreamSynchronize(st));
+}
 
 void mlEnsureHierarchy(phase_hip_solver *s) {
   if (s->ml) return;
@@ -722,6 +824,7 @@ void mlEnsureHierarchy(phase_hip_solver *s) {
       if (!dense) {
         L.sol = makeNested(s);
         L.sol->blockJacobi = level >= s->mlBlockJacobiFrom;
+        L.jacobi = level >= s->mlJacobiFrom;
         // coarse rows couple to more rows than fine ones: when a row of an
         // aggregate would couple to more rows of its own block than the sweep
         // kernels take, smaller aggregates are taken at this level; a level
@@ -781,7 +884,13 @@ void mlEnsureValues(phase_hip_solver *s, bool single) {
         hipLaunchKernelGGL(kGalerkin, dim3(elemGrid(L.nSlots)), dim3(256), 0, st, L.nSlots, L.galPtr.p, L.galSrc.p,
                            finer, L.sol->dVals.p);
         solverValuesChanged(L.sol);
-        ensureFactor64(L.sol);
+        if (L.jacobi) {
+          L.dinv64.alloc(L.n);
+          hipLaunchKernelGGL(kMlDiagInv, dim3(elemGrid(L.n)), dim3(256), 0, st, L.n, L.sol->dSliceOff.p,
+                             L.sol->dSellCol.p, L.sol->dSell64.p, s->mlJacobiWeight, L.dinv64.p);
+        } else {
+          ensureFactor64(L.sol);
+        }
         finer = L.sol->dVals.p;
       } else {
         hipLaunchKernelGGL(kGalerkin, dim3(elemGrid(L.nSlots)), dim3(256), 0, st, L.nSlots, L.galPtr.p, L.galSrc.p,
@@ -812,7 +921,13 @@ void mlEnsureValues(phase_hip_solver *s, bool single) {
     for (MlLevel &L : M.lv) L.v32.alloc(L.n);
     if (!M.singleValid) {
       for (MlLevel &L : M.lv)
-        if (L.sol) ensureSingle(L.sol);
+        if (L.jacobi) {
+          L.dinv32.alloc(L.n);
+          hipLaunchKernelGGL((kConvert<double, float>), dim3(elemGrid(L.n)), dim3(256), 0, st, L.n, L.dinv64.p,
+                             L.dinv32.p);
+        } else if (L.sol) {
+          ensureSingle(L.sol);
+        }
       M.singleValid = true;
     }
   }

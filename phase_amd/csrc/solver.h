@@ -51,6 +51,8 @@ struct phase_hip_solver {
   int mlFinePost = 1;    // post-smoothing on the finest level (0: pre-smoothing only)
   int mlBlockJacobiFrom = 0;   // coarse levels from this one (0 = the first) smooth with block-Jacobi ILU(0)
   int mlDenseRows = 256; // a level of at most this many rows is inverted densely
+  int mlJacobiFrom = 0;  // coarse levels from this one (0 = all) smooth with damped Jacobi instead of ILU(0)
+  double mlJacobiWeight = 2.0 / 3.0;  // the damping weight
 
   // pattern and ordering
   int rank = 0;

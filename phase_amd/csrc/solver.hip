@@ -770,6 +770,16 @@ int phase_hip_solver_setup(phase_hip_solver *s, const char *key, const char *val
       if (r < 0) fail(PHASE_HIP_EINVAL, "multigridBlockJacobiFrom must be >= 0");
       if (r != s->mlBlockJacobiFrom) mlFree(s);
       s->mlBlockJacobiFrom = r;
+    } else if (k == "multigridJacobiFrom") {
+      const int r = std::stoi(v);
+      if (r < 0) fail(PHASE_HIP_EINVAL, "multigridJacobiFrom must be >= 0");
+      if (r != s->mlJacobiFrom) mlFree(s);
+      s->mlJacobiFrom = r;
+    } else if (k == "multigridJacobiWeight") {
+      const double w = std::stod(v);
+      if (!(w > 0. && w <= 1.5)) fail(PHASE_HIP_EINVAL, "multigridJacobiWeight must be in (0, 1.5]");
+      if (w != s->mlJacobiWeight) mlValuesChanged(s);
+      s->mlJacobiWeight = w;
     } else if (k == "multigridFinePostSmoothing") {
       s->mlFinePost = (lv == "1" || lv == "true" || lv == "yes") ? 1 : 0;
     } else if (k == "multigridCoarseRows") {
@@ -987,6 +997,13 @@ int phase_hip_solver_multigrid_info(const phase_hip_solver *s, int *nLevels, int
   if (sizes)
     for (int i = 0; i < std::min(cap, (int)v.size()); ++i) sizes[i] = v[i];
   return PHASE_HIP_OK;
+}
+
+int phase_hip_solver_multigrid_coarsest(phase_hip_solver *s, int n, double *matrix, double *inverse) {
+  return guarded(s, [&] {
+    setDevice(s);
+    mlCoarsest(s, n, matrix, inverse);
+  });
 }
 
 int phase_hip_solver_export(phase_hip_solver *s, int *nnz, int *rowPtr, int *colInd, double *vals,

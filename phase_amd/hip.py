@@ -47,6 +47,7 @@ _SIGS = {
     "phase_hip_solver_export": (ctypes.c_int, [_vp, _ip, _vp, _vp, _vp, _vp]),
     "phase_hip_solver_spmv": (ctypes.c_int, [_vp, ctypes.c_int, _f64p, _f64p]),
     "phase_hip_solver_multigrid_info": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int), _vp, ctypes.c_int]),
+    "phase_hip_solver_multigrid_coarsest": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp]),
     "phase_hip_solver_spmv_noalloc": (ctypes.c_int, [_vp, ctypes.c_int, _f64p, _f64p]),
     "phase_hip_solver_ilu_export": (ctypes.c_int, [_vp, _ip, _vp, _vp, _vp, _vp]),
     "phase_hip_solver_time_kernels": (ctypes.c_int, [_vp, ctypes.c_int, _f64p, ctypes.c_int]),
@@ -854,6 +855,16 @@ class HipSparseMatrixSolver:
         self._check(load_library().phase_hip_solver_multigrid_info(self._h, ctypes.byref(n), _p(sizes), 32),
                     "multigrid_info")
         return sizes[:n.value].tolist()
+
+    def multigrid_coarsest(self):
+        """The densely inverted last multigrid level: (matrix, inverse), both
+        n x n in the same row order."""
+        n = self.multigrid_levels()[-1]
+        a = np.zeros((n, n))
+        inv = np.zeros((n, n))
+        self._check(load_library().phase_hip_solver_multigrid_coarsest(self._h, n, _p(a), _p(inv)),
+                    "multigrid_coarsest")
+        return a, inv
 
     def spmv(self, x, single=False):
         y = np.empty(self.rank)

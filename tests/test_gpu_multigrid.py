@@ -3,8 +3,9 @@
 true relative residual 1e-8 and agrees with a direct solve (scipy splu) within
 cond(A) * 1.5e-8; the hierarchy coarsens down to the dense limit; the
 preconditioner cuts the iteration count against ILU(0) alone; K and V cycles,
-nested ILU levels and the dense last level are all exercised; repeated solves
-are bit-identical."""
+damped-Jacobi and ILU(0) smoothing of the coarse levels, nested levels and the
+dense last level (register and global-memory inverses) are all exercised;
+repeated solves are bit-identical."""
 import numpy as np
 import pytest
 import scipy.sparse.linalg as spla
@@ -67,12 +68,35 @@ def test_multigrid_matches_direct(hip, request, which, mode, solver, dense):
     assert 3 * s.n_iters() <= ilu.n_iters(), (s.n_iters(), ilu.n_iters())
 
 
+@pytest.mark.parametrize("smoother", ["jacobi", "ilu"])
 @pytest.mark.parametrize("cycle", ["K", "V"])
-def test_multigrid_cycles_converge(hip, sys2, cycle):
+def test_multigrid_cycles_converge(hip, sys2, cycle, smoother):
     rp, ci, v, b, A, kappa, x_ref = sys2
+    kw = dict(multigridJacobiFrom=64) if smoother == "ilu" else {}
     s, rel, x = _solve(hip, rp, ci, v, b, mode="fp64", solver="cg", preconditioner="multigrid",
-                       multigridCycle=cycle, multigridDenseRows=16, multigridCoarseRows=4)
+                       multigridCycle=cycle, multigridDenseRows=16, multigridCoarseRows=4, **kw)
     _check(A, b, x, x_ref, kappa, rel)
+
+
+# (fine aggregate rows, coarse aggregate rows, dense limit) -> last level of
+# 26, 74, 176 and 254 rows on this mesh (CPU hierarchy of the same orderings)
+@pytest.mark.parametrize("fine,coarse,dense,lo,hi", [(32, 16, 60, 1, 64), (32, 4, 120, 65, 128),
+                                                     (48, 16, 256, 129, 192), (32, 16, 256, 193, 256)])
+def test_multigrid_dense_inverse(hip, sys1, fine, coarse, dense, lo, hi):
+    """The last level's inverse, from each register layout of the Gauss-Jordan
+    kernel (up to 64, 128 and 192 rows) and from the global-memory one (above
+    192): times the last level's Galerkin matrix it gives the identity to
+    rounding, and the solve converges."""
+    rp, ci, v, b, A, kappa, x_ref = sys1
+    s, rel, x = _solve(hip, rp, ci, v, b, mode="fp64", solver="cg", preconditioner="multigrid",
+                       blockRows=fine, multigridDenseRows=dense, multigridCoarseRows=coarse)
+    _check(A, b, x, x_ref, kappa, rel)
+    m, inv = s.multigrid_coarsest()
+    n = len(m)
+    assert lo <= n <= hi and np.abs(m).max() > 0, n
+    k = np.linalg.cond(m)
+    assert np.abs(inv @ m - np.eye(n)).max() <= 1e-13 * k, (n, k)
+    print("dense level", n, "cond %.3g" % k, flush=True)
 
 
 def test_multigrid_device_assembly_brick_aggregates(hip, oracle):
@@ -142,7 +166,8 @@ def test_multigrid_graph_aggregates_wide_coarse_rows(hip, oracle):
     m, (rp, ci, v), b, g, bc = H.pressure_system(oracle, xy, cptr, cind, sigma=1.0)
     A = H.compact_sorted(rp, ci, v)
     for mode in ("fp64", "mixed"):
-        s, rel, x = _solve(hip, rp, ci, v, b, mode=mode, solver="cg", preconditioner="multigrid")
+        s, rel, x = _solve(hip, rp, ci, v, b, mode=mode, solver="cg", preconditioner="multigrid",
+                           multigridJacobiFrom=64)  # ILU(0) sweeps on every coarse level
         assert rel <= 1e-8
         assert np.linalg.norm(b - A @ x) <= 1.5e-8 * np.linalg.norm(b)
         assert len(s.multigrid_levels()) >= 4
