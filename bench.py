@@ -73,6 +73,7 @@ TRACE_KEYS = {
     "phase::kLSolve<double, 3, 4>": "lsolve64cg_launch", "phase::kLUSolve<double, 3, 4>": "lusolve64cg_launch",
     "phase::kLSolve<double, 0, 4>": "lsolve64_launch", "phase::kLUSolve<double, 0, 4>": "lusolve64_launch",
     "phase::kUSolve<double, 4>": "usolve64_launch", "phase::kSpmv<double>": "spmv64",
+    "phase::kMlResid<float, double>": "mlresid32", "phase::kMlResid<double, double>": "mlresid64",
 }
 
 

@@ -208,7 +208,9 @@ int phase_hip_solver_ilu_export(phase_hip_solver *s, int *nnz, int *perm, int *r
  * evicts L2 and the last-level cache; the events bracket the one launch),
  * 13..15 as 8..10 with the FP32 factor (mixed mode's sweeps), 16 / 17 the
  * forward-sweep and fused launches in the form CG runs them (the prologue
- * also moves x += alpha p and r -= alpha q), FP64, 18 / 19 the same in FP32.
+ * also moves x += alpha p and r -= alpha q), FP64, 18 / 19 the same in FP32,
+ * 20 / 21 the multigrid cycle's fine-level residual t = b - A x (FP64 matrix
+ * values; FP32 / FP64 vectors).
  * Benchmark support; no counterpart in the reference. */
 int phase_hip_solver_time_kernels(phase_hip_solver *s, int reps, double *ms, int nms);
 

@@ -55,6 +55,10 @@ template <class T>
 void mlFinishApply(phase_hip_solver *s, const Scalars *sc, const T *rhs, T *out);
 // Level sizes (fine first) for reporting.
 std::vector<int> mlLevelSizes(const phase_hip_solver *s);
+// One launch of the fine level's residual kernel t = b - A x (FP64 matrix
+// values, vectors in T), as each multigrid apply runs it; for kernel timing.
+template <class T>
+void mlResidLaunch(phase_hip_solver *s, const T *x, const T *b, T *t);
 // The densely inverted last level: its matrix and inverse, n x n row-major
 // (n = the last level size), copied to host memory.
 void mlCoarsest(phase_hip_solver *s, int n, double *matrix, double *inverse);

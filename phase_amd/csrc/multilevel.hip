@@ -55,7 +55,7 @@ struct MlLevel {
   MlVecs<double> v64;
   MlVecs<float> v32;
   DBuf<MlScalars> msc;
-  DBuf<double> partials;
+  DBuf<double> partials, partials2;
 };
 
 struct Multilevel {
@@ -605,22 +605,100 @@ __global__ __launch_bounds__(256) void kMlJacResidRestrict(int nc, const int *__
 }
 
 // Prolongation, residual and Jacobi post-smoothing in one launch:
-// u = z + P ec; out = u + dinv (r - A u) (z and out distinct).
-template <class T, class TV>
+// u = z + P ec; out = u + dinv (r - A u) (z and out distinct).  With DOT, also
+// the per-block partials of (out, q) (the K cycle's (z2, q1)).
+template <class T, class TV, bool DOT>
 __global__ __launch_bounds__(kRedBlock) void kMlJacResidProlong(int n, int nSlices, const int *__restrict__ sliceOff,
                                                                 const int *__restrict__ col,
                                                                 const TV *__restrict__ val,
                                                                 const int *__restrict__ pIdx,
                                                                 const T *__restrict__ ec, const T *__restrict__ z,
                                                                 const T *__restrict__ r, const T *__restrict__ dinv,
-                                                                T *__restrict__ out, const Scalars *sc) {
+                                                                T *__restrict__ out, const Scalars *sc,
+                                                                const T *__restrict__ q, double *partials) {
   if (sc->done) return;
   const int slice = blockIdx.x * (kRedBlock / 64) + (threadIdx.x >> 6);
   const int row = slice * 64 + (threadIdx.x & 63);
-  if (slice >= nSlices || row >= n) return;
-  const T y = sellRowAt<T, TV>(row, sliceOff, col, val, [&](int c) { return (T)(z[c] + ec[pIdx[c]]); });
-  const T u = z[row] + ec[pIdx[row]];
-  out[row] = u + dinv[row] * (r[row] - y);
+  double d[1] = {0.};
+  if (slice < nSlices && row < n) {
+    const T y = sellRowAt<T, TV>(row, sliceOff, col, val, [&](int c) { return (T)(z[c] + ec[pIdx[c]]); });
+    const T u = z[row] + ec[pIdx[row]];
+    const T o = u + dinv[row] * (r[row] - y);
+    out[row] = o;
+    if (DOT) d[0] = (double)o * (double)q[row];
+  }
+  if (DOT) blockPartials<1>(d, partials, blockIdx.x);
+}
+
+// The K cycle on a Jacobi level, second apply, first launch: kMlFirst's work
+// (alpha1 = (z1, r) / (z1, q1) from kMlSpmvDot2's partials, summed in every
+// block; e = alpha1 z1; r1 = r - alpha1 q1, stored) and kMlJacResidRestrict on
+// r1, whose products form r1 at the gathered columns.  The same arithmetic as
+// the two kernels.
+template <class T, class TV>
+__global__ __launch_bounds__(256) void kMlJacFirstRestrict(int nc, const int *__restrict__ restRow,
+                                                           const int *__restrict__ aggPtr,
+                                                           const int *__restrict__ sliceOff,
+                                                           const int *__restrict__ col, const TV *__restrict__ val,
+                                                           const T *__restrict__ dinv, const T *__restrict__ r,
+                                                           const T *__restrict__ q1, const T *__restrict__ z1,
+                                                           const double *partials, int nb, MlScalars *m,
+                                                           T *__restrict__ r1, T *__restrict__ e, T *__restrict__ z,
+                                                           T *__restrict__ rc, const Scalars *sc) {
+  if (sc->done) return;
+  double pq;
+  const T a = (T)mlAlpha(partials, nb, pq);
+  if (blockIdx.x == 0 && threadIdx.x == 0) m->pq1 = pq;
+  const int k = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (k >= nc) return;
+  const int ag = restRow ? restRow[k] : k;
+  const int row = aggPtr[ag] + lane;
+  double d = 0.;
+  if (row < aggPtr[ag + 1]) {
+    const T rr = r[row] - a * q1[row];
+    r1[row] = rr;
+    e[row] = a * z1[row];
+    z[row] = dinv[row] * rr;
+    const T t = rr - sellRowAt<T, TV>(row, sliceOff, col, val, [&](int c) { return dinv[c] * (r[c] - a * q1[c]); });
+    d = (double)t;
+  }
+  d = waveSum(d);
+  if (lane == 0) rc[k] = (T)d;
+}
+
+// The K cycle's direction on a Jacobi level: gamma = (z2, q1) / (z1, q1) (the
+// partials of the second apply, summed in every block), p2 = z2 - gamma z1
+// (stored), and the partials of (p2, A p2) and (p2, r1), A p2 formed from p2
+// at the gathered columns (kMlDirection and kMlSpmvDot2 in one launch).
+template <class T, class TV>
+__global__ __launch_bounds__(kRedBlock) void kMlJacDirection(int n, int nSlices, const int *__restrict__ sliceOff,
+                                                             const int *__restrict__ col, const TV *__restrict__ val,
+                                                             const Scalars *sc, const MlScalars *m,
+                                                             const double *partialsIn, int nb,
+                                                             const T *__restrict__ z1, const T *__restrict__ z2,
+                                                             const T *__restrict__ r1, T *__restrict__ p2,
+                                                             double *partialsOut) {
+  if (sc->done) return;
+  double dz[1];
+  finishPartials<1>(partialsIn, nb, dz);
+  __shared__ double sGamma;
+  if (threadIdx.x == 0) {
+    const double pq = m->pq1;
+    sGamma = (pq != 0. && isfinite(dz[0])) ? dz[0] / pq : 0.;
+  }
+  __syncthreads();
+  const T g = (T)sGamma;
+  const int slice = blockIdx.x * (kRedBlock / 64) + (threadIdx.x >> 6);
+  const int row = slice * 64 + (threadIdx.x & 63);
+  double d[2] = {0., 0.};
+  if (slice < nSlices && row < n) {
+    const T p = z2[row] - g * z1[row];
+    p2[row] = p;
+    const T y = sellRowAt<T, TV>(row, sliceOff, col, val, [&](int c) { return (T)(z2[c] - g * z1[c]); });
+    d[0] = (double)p * (double)y;
+    d[1] = (double)p * (double)r1[row];
+  }
+  blockPartials<2>(d, partialsOut, blockIdx.x);
 }
 
 // ---------------------------------------------------------------- host side
@@ -661,11 +739,36 @@ void coarseSolve(phase_hip_solver *s, Multilevel &M, int l, const Scalars *sc, c
   MlVecs<T> &V = vecs<T>(L);
   phase_hip_solver *c = L.sol;
   const int n = L.n, gs = spmvGrid(c), gu = gridFor(n, kRedBlock, kRedCap);
+  // each update block re-reads the gs partials: keep the grid near gs
+  const int gd = gridFor(n, 4 * kRedBlock, kRedCap);
+  if (L.jacobi) {
+    // on a Jacobi level the K cycle's updates ride on the cycle's launches:
+    // first apply, A z1 with its dots, second apply (r1 formed in its first
+    // launch, (z2, q1) summed in its last), the direction with A p2 and its
+    // dots, the update of e
+    MlLevel &N = M.lv[l + 1];
+    MlVecs<T> &NV = vecs<T>(N);
+    cycle<T>(s, M, l, sc, r, V.z1.p);
+    hipLaunchKernelGGL((kMlSpmvDot2<T, double>), dim3(gs), dim3(kRedBlock), 0, st, n, c->ord.nSlices,
+                       c->dSliceOff.p, c->dSellCol.p, c->dSell64.p, (const T *)V.z1.p, V.q1.p, r, sc, L.partials.p);
+    hipLaunchKernelGGL((kMlJacFirstRestrict<T, double>), dim3((N.n + 3) / 4), dim3(256), 0, st, N.n,
+                       N.sol ? N.restRow.p : nullptr, N.finerAggPtr.p, c->dSliceOff.p, c->dSellCol.p, c->dSell64.p,
+                       dinvOf<T>(L), r, (const T *)V.q1.p, (const T *)V.z1.p, (const double *)L.partials.p, gs,
+                       L.msc.p, V.r1.p, e, V.w.p, NV.r.p, sc);
+    coarseSolve<T>(s, M, l + 1, sc, NV.r.p, NV.e.p);
+    hipLaunchKernelGGL((kMlJacResidProlong<T, double, true>), dim3(gs), dim3(kRedBlock), 0, st, n, c->ord.nSlices,
+                       c->dSliceOff.p, c->dSellCol.p, c->dSell64.p, N.pIdx.p, (const T *)NV.e.p, (const T *)V.w.p,
+                       (const T *)V.r1.p, dinvOf<T>(L), V.z2.p, sc, (const T *)V.q1.p, L.partials2.p);
+    hipLaunchKernelGGL((kMlJacDirection<T, double>), dim3(gs), dim3(kRedBlock), 0, st, n, c->ord.nSlices,
+                       c->dSliceOff.p, c->dSellCol.p, c->dSell64.p, sc, (const MlScalars *)L.msc.p,
+                       (const double *)L.partials2.p, gs, (const T *)V.z1.p, (const T *)V.z2.p, (const T *)V.r1.p,
+                       V.t.p, L.partials.p);
+    hipLaunchKernelGGL((kMlSecond<T>), dim3(gd), dim3(kRedBlock), 0, st, n, sc, L.partials.p, gs, V.t.p, e);
+    return;
+  }
   cycle<T>(s, M, l, sc, r, V.z1.p);
   hipLaunchKernelGGL((kMlSpmvDot2<T, double>), dim3(gs), dim3(kRedBlock), 0, st, n, c->ord.nSlices, c->dSliceOff.p,
                      c->dSellCol.p, c->dSell64.p, (const T *)V.z1.p, V.q1.p, r, sc, L.partials.p);
-  // each update block re-reads the gs partials: keep the grid near gs
-  const int gd = gridFor(n, 4 * kRedBlock, kRedCap);
   hipLaunchKernelGGL((kMlFirst<T>), dim3(gd), dim3(kRedBlock), 0, st, n, sc, L.msc.p, L.partials.p, gs, V.z1.p,
                      V.q1.p, r, e, V.r1.p);
   cycle<T>(s, M, l, sc, V.r1.p, V.z2.p);
@@ -749,9 +852,9 @@ void cycle(phase_hip_solver *s, Multilevel &M, int l, const Scalars *sc, const T
                      N.sol ? N.restRow.p : nullptr, N.finerAggPtr.p, c->dSliceOff.p, c->dSellCol.p, c->dSell64.p,
                      dinvOf<T>(L), r, V.w.p, NV.r.p, sc);
   coarseSolve<T>(s, M, l + 1, sc, NV.r.p, NV.e.p);
-  hipLaunchKernelGGL((kMlJacResidProlong<T, double>), dim3(spmvGrid(c)), dim3(kRedBlock), 0, st, c->rank,
+  hipLaunchKernelGGL((kMlJacResidProlong<T, double, false>), dim3(spmvGrid(c)), dim3(kRedBlock), 0, st, c->rank,
                      c->ord.nSlices, c->dSliceOff.p, c->dSellCol.p, c->dSell64.p, N.pIdx.p, (const T *)NV.e.p,
-                     (const T *)V.w.p, r, dinvOf<T>(L), e, sc);
+                     (const T *)V.w.p, r, dinvOf<T>(L), e, sc, (const T *)nullptr, (double *)nullptr);
 }
 
 }  // namespace
@@ -777,6 +880,14 @@ void mlValuesChanged(phase_hip_solver *s) {
 }
 
 std::vector<int> mlLevelSizes(const phase_hip_solver *s) { return s->ml ? s->ml->sizes : std::vector<int>(); }
+
+template <class T>
+void mlResidLaunch(phase_hip_solver *s, const T *x, const T *b, T *t) {
+  hipLaunchKernelGGL((kMlResid<T, double>), dim3(spmvGrid(s)), dim3(kRedBlock), 0, s->stream, s->rank,
+                     s->ord.nSlices, s->dSliceOff.p, s->dSellCol.p, s->dSell64.p, x, b, t, idleScalars(s));
+}
+template void mlResidLaunch<double>(phase_hip_solver *, const double *, const double *, double *);
+template void mlResidLaunch<float>(phase_hip_solver *, const float *, const float *, float *);
 
 void mlCoarsest(phase_hip_solver *s, int n, double *matrix, double *inverse) {
   if (!s->ml || s->ml->lv.empty() || !s->ml->valuesValid) fail(PHASE_HIP_ESTATE, "no multigrid hierarchy");
@@ -857,6 +968,7 @@ void mlEnsureHierarchy(phase_hip_solver *s) {
         for (int i = 0; i < nFiner; ++i) pIdx[i] = L.sol->ord.iperm[c.aggOfRow[i]];
         L.msc.alloc(1);
         L.partials.alloc(2 * (size_t)std::max(spmvGrid(L.sol), kRedCap));
+        L.partials2.alloc((size_t)std::max(spmvGrid(L.sol), kRedCap));
       }
       L.pIdx.upload(pIdx, st);
       M->sizes.push_back(c.n);

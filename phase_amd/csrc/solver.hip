@@ -1224,6 +1224,9 @@ void kernelBytes(const Ordering &o, double *out) {
       oc[1] = o3[1] + rows * (3 * sz + 16);
     }
   }
+  // multigrid residual t = b - A x: the FP64 matrix, x gathered, b read, t written
+  out[20] = o.nnz * 12.0 + 3 * n * 4 + (o.nSlices + 1) * 4.0;
+  out[21] = o.nnz * 12.0 + 3 * n * 8 + (o.nSlices + 1) * 4.0;
 }
 
 }  // namespace
@@ -1338,6 +1341,9 @@ int phase_hip_solver_time_kernels(phase_hip_solver *s, int reps, double *ms, int
     perLaunch(s->dFac64.p, s->k64, out + 8, out + 16);
     ensureSingle(s);  // the FP32 copy of the FP64 factor, as mixed solves use it
     perLaunch(s->dFac32.p, s->k32, out + 13, out + 18);
+    // the multigrid cycle's fine-level residual t = b - A x (FP64 values), FP32 in 20, FP64 in 21
+    out[20] = timeIt([&] { mlResidLaunch<float>(s, s->k32.ph.p, s->k32.p.p, s->k32.t.p); });
+    out[21] = timeIt([&] { mlResidLaunch<double>(s, s->k64.ph.p, s->k64.p.p, s->k64.t.p); });
     // cache-cold SpMV: before each timed launch a 1 GiB copy evicts the matrix
     // and vectors from L2 and from the 256 MB last-level cache, and the events
     // bracket the one launch

@@ -24,7 +24,7 @@ constexpr int kRedCap = 1024;
 
 
 // entries of phase_hip_solver_time_kernels / phase_hip_solver_kernel_bytes
-constexpr int kTimedKernels = 20;
+constexpr int kTimedKernels = 22;
 
 template <class T>
 IluDev<T> iluDev(phase_hip_solver *s, T *fac) {

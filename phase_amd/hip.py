@@ -885,7 +885,7 @@ class HipSparseMatrixSolver:
     KERNEL_NAMES = ("spmv64", "spmv32", "lsweep64", "usweep64", "lsweep32", "usweep32", "factor64", "factor32",
                     "usolve64_launch", "lusolve64_launch", "lsolve64_launch", "spmv64_cold", "spmv32_cold",
                     "usolve32_launch", "lusolve32_launch", "lsolve32_launch", "lsolve64cg_launch",
-                    "lusolve64cg_launch", "lsolve32cg_launch", "lusolve32cg_launch")
+                    "lusolve64cg_launch", "lsolve32cg_launch", "lusolve32cg_launch", "mlresid32", "mlresid64")
 
     def time_kernels(self, reps=20):
         """Mean device ms of each timed kernel or sweep (HIP events)."""
