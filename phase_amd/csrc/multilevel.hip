@@ -226,16 +226,17 @@ __global__ __launch_bounds__(kGjBlock) void kDenseInverse(int n, double *__restr
 // replacing it by the column of the inverse being built.  That is the
 // unpivoted in-place inversion of B = P A (B row k = A row p_k), so physical
 // row p_k ends holding row k of B^-1 = A^-1 P^T: inv[k][p_j] = R[p_k][j].
-// Two barriers per step (column k published; pivot row published); the
-// column and row buffers alternate between steps, so a step never overwrites
-// what a slower thread of the previous step still reads.
+// Two barriers per step (column k and the pivot candidates published; pivot
+// row published); the column, candidate and row buffers alternate between
+// steps, so a step never overwrites what a slower thread of the previous step
+// still reads.
 template <int RB>
 __global__ __launch_bounds__(kGjBlock) void kDenseInverseReg(int n, const double *__restrict__ a0,
                                                              double *__restrict__ inv, int *err) {
   constexpr int S = 32 * RB;
-  __shared__ double fcol[2][S], prow[2][S];
-  __shared__ int pivOf[S], kOf[S], used[S];
-  const int tid = threadIdx.x, tr = tid >> 5, tc = tid & 31, lane = tid & 63;
+  __shared__ double fcol[2][S], prow[2][S], candV[2][32];
+  __shared__ int candI[2][32], pivOf[S], kOf[S], used[S];
+  const int tid = threadIdx.x, tr = tid >> 5, tc = tid & 31;
   double A[RB][RB];
 #pragma unroll
   for (int a = 0; a < RB; ++a)
@@ -253,6 +254,11 @@ __global__ __launch_bounds__(kGjBlock) void kDenseInverseReg(int n, const double
   for (int k = 0; k < n; ++k) {
     const int buf = k & 1, kb = k >> 5, kc = k & 31;
     if (tc == kc) {
+      // the 32 threads holding column k publish it, each with its best pivot
+      // candidate among its unused rows (largest |a_ik|, lowest i on a tie; a
+      // NaN is taken, so some unused row always is, and flagged below)
+      double best = -1.;
+      int bi = n;
 #pragma unroll
       for (int a = 0; a < RB; ++a) {
         double v = 0.;
@@ -260,34 +266,40 @@ __global__ __launch_bounds__(kGjBlock) void kDenseInverseReg(int n, const double
         for (int b = 0; b < RB; ++b)
           if (b == kb) v = A[a][b];
         const int i = tr + 32 * a;
-        if (i < n) fcol[buf][i] = v;
-      }
-    }
-    __syncthreads();
-    // every wavefront finds the same pivot (a NaN entry is taken, so some
-    // unused row always is, and flagged below)
-    double best = -1.;
-    int bi = n;
-    for (int i = lane; i < n; i += 64)
-      if (!used[i]) {
-        const double v = fabs(fcol[buf][i]);
-        if (!(v <= best)) {
-          best = v;
-          bi = i;
+        if (i < n) {
+          fcol[buf][i] = v;
+          const double av = fabs(v);
+          if (!used[i] && !(av <= best)) {
+            best = av;
+            bi = i;
+          }
         }
       }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      const double ov = __shfl_xor(best, o, 64);
-      const int oi = __shfl_xor(bi, o, 64);
-      if (ov > best || (ov == best && oi < bi)) {
-        best = ov;
-        bi = oi;
+      candV[buf][tr] = best;
+      candI[buf][tr] = bi;
+    }
+    __syncthreads();
+    // every thread reduces the 32 candidates in the same order
+    double best = -1.;
+    int bi = n;
+    for (int q = 0; q < 32; ++q) {
+      const double v = candV[buf][q];
+      const int i = candI[buf][q];
+      if (i < n && (!(v <= best) || (v == best && i < bi))) {
+        best = v;
+        bi = i;
       }
     }
     const int p = bi < n ? bi : 0;
     const double pv = fcol[buf][p];
     const double pivinv = pv != 0. ? 1. / pv : 0.;
+    if (tid == 0) {
+      // before the barrier: the next step's column owners read used[]
+      used[p] = 1;
+      pivOf[k] = p;
+      kOf[p] = k;
+      if (!(fabs(pv) > 0.)) atomicOr(err, 4);
+    }
     const int pb = p >> 5, pr = p & 31;
     if (tr == pr) {
 #pragma unroll
@@ -301,13 +313,6 @@ __global__ __launch_bounds__(kGjBlock) void kDenseInverseReg(int n, const double
       }
     }
     __syncthreads();
-    if (tid == 0) {
-      // after the barrier: every wavefront has finished this step's search
-      used[p] = 1;
-      pivOf[k] = p;
-      kOf[p] = k;
-      if (!(fabs(pv) > 0.)) atomicOr(err, 4);
-    }
 #pragma unroll
     for (int a = 0; a < RB; ++a) {
       const int i = tr + 32 * a;
