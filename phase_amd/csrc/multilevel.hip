@@ -234,6 +234,8 @@ template <int RB>
 __global__ __launch_bounds__(kGjBlock) void kDenseInverseReg(int n, const double *__restrict__ a0,
                                                              double *__restrict__ inv, int *err) {
   constexpr int S = 32 * RB;
+  // entries past n of the column and row buffers stay 0, so the update needs
+  // no bounds tests: padding entries of A stay 0 or unchanged and are never stored
   __shared__ double fcol[2][S], prow[2][S], candV[2][32];
   __shared__ int candI[2][32], pivOf[S], kOf[S], used[S];
   const int tid = threadIdx.x, tr = tid >> 5, tc = tid & 31;
@@ -249,39 +251,40 @@ __global__ __launch_bounds__(kGjBlock) void kDenseInverseReg(int n, const double
     used[i] = 0;
     pivOf[i] = 0;
     kOf[i] = 0;
+    fcol[0][i] = fcol[1][i] = prow[0][i] = prow[1][i] = 0.;
   }
   __syncthreads();
   for (int k = 0; k < n; ++k) {
     const int buf = k & 1, kb = k >> 5, kc = k & 31;
-    if (tc == kc) {
-      // the 32 threads holding column k publish it, each with its best pivot
-      // candidate among its unused rows (largest |a_ik|, lowest i on a tie; a
-      // NaN is taken, so some unused row always is, and flagged below)
-      double best = -1.;
-      int bi = n;
+    // the 32 threads holding column k publish it, each with its best pivot
+    // candidate among its unused rows (largest |a_ik|, lowest i on a tie; a
+    // NaN is taken, so some unused row always is, and flagged below); b == kb
+    // is the same for every thread, so only the branch taken is issued
 #pragma unroll
-      for (int a = 0; a < RB; ++a) {
-        double v = 0.;
+    for (int b = 0; b < RB; ++b)
+      if (b == kb && tc == kc) {
+        double best = -1.;
+        int bi = n;
 #pragma unroll
-        for (int b = 0; b < RB; ++b)
-          if (b == kb) v = A[a][b];
-        const int i = tr + 32 * a;
-        if (i < n) {
-          fcol[buf][i] = v;
-          const double av = fabs(v);
-          if (!used[i] && !(av <= best)) {
-            best = av;
-            bi = i;
+        for (int a = 0; a < RB; ++a) {
+          const int i = tr + 32 * a;
+          if (i < n) {
+            fcol[buf][i] = A[a][b];
+            const double av = fabs(A[a][b]);
+            if (!used[i] && !(av <= best)) {
+              best = av;
+              bi = i;
+            }
           }
         }
+        candV[buf][tr] = best;
+        candI[buf][tr] = bi;
       }
-      candV[buf][tr] = best;
-      candI[buf][tr] = bi;
-    }
     __syncthreads();
     // every thread reduces the 32 candidates in the same order
     double best = -1.;
     int bi = n;
+#pragma unroll 8
     for (int q = 0; q < 32; ++q) {
       const double v = candV[buf][q];
       const int i = candI[buf][q];
@@ -300,32 +303,42 @@ __global__ __launch_bounds__(kGjBlock) void kDenseInverseReg(int n, const double
       kOf[p] = k;
       if (!(fabs(pv) > 0.)) atomicOr(err, 4);
     }
+    // pivot row, scaled, with 1 / pivot in column k (row k of the inverse so far)
     const int pb = p >> 5, pr = p & 31;
-    if (tr == pr) {
 #pragma unroll
-      for (int b = 0; b < RB; ++b) {
-        double v = 0.;
+    for (int a = 0; a < RB; ++a)
+      if (a == pb && tr == pr) {
 #pragma unroll
-        for (int a = 0; a < RB; ++a)
-          if (a == pb) v = A[a][b];
-        const int j = tc + 32 * b;
-        if (j < n) prow[buf][j] = (j == k ? 1. : v) * pivinv;
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int a = 0; a < RB; ++a) {
-      const int i = tr + 32 * a;
-      const double f = i < n ? fcol[buf][i] : 0.;
-#pragma unroll
-      for (int b = 0; b < RB; ++b) {
-        const int j = tc + 32 * b;
-        if (i < n && j < n) {
-          const double rs = prow[buf][j];
-          A[a][b] = i == p ? rs : fma(-f, rs, j == k ? 0. : A[a][b]);
+        for (int b = 0; b < RB; ++b) {
+          const int j = tc + 32 * b;
+          if (j < n) prow[buf][j] = A[a][b] * pivinv;
         }
+        if (tc == kc) prow[buf][k] = pivinv;
       }
-    }
+    __syncthreads();
+    // column k becomes the inverse's column: a_ik = 0, then every row takes
+    // a_ij -= a_ik_old * prow_j; the pivot row is then set to prow
+#pragma unroll
+    for (int b = 0; b < RB; ++b)
+      if (b == kb && tc == kc) {
+#pragma unroll
+        for (int a = 0; a < RB; ++a) A[a][b] = 0.;
+      }
+    double rs[RB], f[RB];
+#pragma unroll
+    for (int b = 0; b < RB; ++b) rs[b] = prow[buf][tc + 32 * b];
+#pragma unroll
+    for (int a = 0; a < RB; ++a) f[a] = fcol[buf][tr + 32 * a];
+#pragma unroll
+    for (int a = 0; a < RB; ++a)
+#pragma unroll
+      for (int b = 0; b < RB; ++b) A[a][b] = fma(-f[a], rs[b], A[a][b]);
+#pragma unroll
+    for (int a = 0; a < RB; ++a)
+      if (a == pb && tr == pr) {
+#pragma unroll
+        for (int b = 0; b < RB; ++b) A[a][b] = rs[b];
+      }
   }
   __syncthreads();
 #pragma unroll
@@ -337,15 +350,124 @@ __global__ __launch_bounds__(kGjBlock) void kDenseInverseReg(int n, const double
     }
 }
 
-// Largest n the register Gauss-Jordan takes (RB = 6: 36 entries per thread).
+// The same register layout without pivoting: step k eliminates with the
+// diagonal a_kk, so the column and the row of the step are known before any
+// search and one barrier per step publishes both.  Gauss-Jordan without
+// pivoting is stable for the diagonally dominant and the symmetric positive
+// definite matrices the pressure equations coarsen to; a pivot below 1e-14 of
+// the largest entry, or not finite, sets bit 8 of *err and the caller inverts
+// again with pivoting.
+template <int RB>
+__global__ __launch_bounds__(kGjBlock) void kDenseInverseDiag(int n, const double *__restrict__ a0,
+                                                              double *__restrict__ inv, int *err) {
+  constexpr int S = 32 * RB;
+  __shared__ double fcol[2][S], prow[2][S], smax[kGjBlock / 64];
+  const int tid = threadIdx.x, tr = tid >> 5, tc = tid & 31, lane = tid & 63, wave = tid >> 6;
+  double A[RB][RB];
+  double m = 0.;
+#pragma unroll
+  for (int a = 0; a < RB; ++a)
+#pragma unroll
+    for (int b = 0; b < RB; ++b) {
+      const int i = tr + 32 * a, j = tc + 32 * b;
+      A[a][b] = (i < n && j < n) ? a0[(size_t)i * n + j] : 0.;
+      m = fmax(m, fabs(A[a][b]));
+    }
+  for (int i = tid; i < S; i += kGjBlock) fcol[0][i] = fcol[1][i] = prow[0][i] = prow[1][i] = 0.;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  if (lane == 0) smax[wave] = m;
+  __syncthreads();
+  double amax = 0.;
+  for (int w = 0; w < kGjBlock / 64; ++w) amax = fmax(amax, smax[w]);
+  const double tiny = 1e-14 * amax;
+  for (int k = 0; k < n; ++k) {
+    const int buf = k & 1, kb = k >> 5, kc = k & 31;
+    // column k and row k as they stand (k >> 5 is the same for every thread:
+    // only the branch taken is issued)
+#pragma unroll
+    for (int b = 0; b < RB; ++b)
+      if (b == kb && tc == kc) {
+#pragma unroll
+        for (int a = 0; a < RB; ++a) {
+          const int i = tr + 32 * a;
+          if (i < n) fcol[buf][i] = A[a][b];
+        }
+      }
+#pragma unroll
+    for (int a = 0; a < RB; ++a)
+      if (a == kb && tr == kc) {
+#pragma unroll
+        for (int b = 0; b < RB; ++b) {
+          const int j = tc + 32 * b;
+          if (j < n) prow[buf][j] = A[a][b];
+        }
+      }
+    __syncthreads();
+    const double pv = fcol[buf][k];
+    const double pivinv = pv != 0. ? 1. / pv : 0.;
+    if (tid == 0 && !(fabs(pv) > tiny)) atomicOr(err, 8);
+    double rs[RB], f[RB];
+#pragma unroll
+    for (int b = 0; b < RB; ++b) rs[b] = (b == kb && tc == kc) ? pivinv : prow[buf][tc + 32 * b] * pivinv;
+#pragma unroll
+    for (int a = 0; a < RB; ++a) f[a] = fcol[buf][tr + 32 * a];
+#pragma unroll
+    for (int b = 0; b < RB; ++b)
+      if (b == kb && tc == kc) {
+#pragma unroll
+        for (int a = 0; a < RB; ++a) A[a][b] = 0.;
+      }
+#pragma unroll
+    for (int a = 0; a < RB; ++a)
+#pragma unroll
+      for (int b = 0; b < RB; ++b) A[a][b] = fma(-f[a], rs[b], A[a][b]);
+#pragma unroll
+    for (int a = 0; a < RB; ++a)
+      if (a == kb && tr == kc) {
+#pragma unroll
+        for (int b = 0; b < RB; ++b) A[a][b] = rs[b];
+      }
+  }
+#pragma unroll
+  for (int a = 0; a < RB; ++a)
+#pragma unroll
+    for (int b = 0; b < RB; ++b) {
+      const int i = tr + 32 * a, j = tc + 32 * b;
+      if (i < n && j < n) inv[(size_t)i * n + j] = A[a][b];
+    }
+}
+
+// Largest n the register Gauss-Jordan kernels take (RB = 6: 36 entries per thread).
 constexpr int kDenseRegMax = 192;
 
-void launchDenseInverse(hipStream_t st, int n, double *dense, double *inv, int *err) {
+// inv = dense^-1 into `inv`; err bit 4 on a singular matrix.  Up to
+// kDenseRegMax rows, without pivoting first unless `pivot`, and with pivoting
+// when that flags a small pivot (one host synchronisation each).
+void denseInverse(hipStream_t st, int n, double *dense, double *inv, int *err, bool pivot) {
+  auto readErr = [&] {
+    int e = 0;
+    PH_CHECK(hipMemcpyAsync(&e, err, sizeof(int), hipMemcpyDeviceToHost, st));
+    PH_CHECK(hipStreamSynchronize(st));
+    return e;
+  };
+  if (n > kDenseRegMax) {
+    hipLaunchKernelGGL(kDenseInverse, dim3(1), dim3(kGjBlock), 0, st, n, dense, inv, err);
+    PH_CHECK(hipGetLastError());
+    return;
+  }
+  if (!pivot) {
+    if (n <= 64) hipLaunchKernelGGL(kDenseInverseDiag<2>, dim3(1), dim3(kGjBlock), 0, st, n, dense, inv, err);
+    else if (n <= 128) hipLaunchKernelGGL(kDenseInverseDiag<4>, dim3(1), dim3(kGjBlock), 0, st, n, dense, inv, err);
+    else hipLaunchKernelGGL(kDenseInverseDiag<6>, dim3(1), dim3(kGjBlock), 0, st, n, dense, inv, err);
+    PH_CHECK(hipGetLastError());
+    if (!(readErr() & 8)) return;
+    PH_CHECK(hipMemsetAsync(err, 0, sizeof(int), st));
+  }
   if (n <= 64) hipLaunchKernelGGL(kDenseInverseReg<2>, dim3(1), dim3(kGjBlock), 0, st, n, dense, inv, err);
   else if (n <= 128) hipLaunchKernelGGL(kDenseInverseReg<4>, dim3(1), dim3(kGjBlock), 0, st, n, dense, inv, err);
-  else if (n <= kDenseRegMax)
-    hipLaunchKernelGGL(kDenseInverseReg<6>, dim3(1), dim3(kGjBlock), 0, st, n, dense, inv, err);
-  else hipLaunchKernelGGL(kDenseInverse, dim3(1), dim3(kGjBlock), 0, st, n, dense, inv, err);
+  else hipLaunchKernelGGL(kDenseInverseReg<6>, dim3(1), dim3(kGjBlock), 0, st, n, dense, inv, err);
+  PH_CHECK(hipGetLastError());
 }
 
 // e = inv r (one wavefront per row, FP64 products)
@@ -1014,8 +1136,7 @@ void mlEnsureValues(phase_hip_solver *s, bool single) {
                            finer, L.cVals.p);
         hipLaunchKernelGGL(kDenseFill, dim3(elemGrid(L.n)), dim3(256), 0, st, L.n, L.cRowPtr.p, L.cColInd.p,
                            L.cVals.p, L.dense.p);
-        launchDenseInverse(st, L.n, L.dense.p, L.inv.p, s->dErr.p);
-        PH_CHECK(hipGetLastError());
+        denseInverse(st, L.n, L.dense.p, L.inv.p, s->dErr.p, s->mlDensePivot != 0);
         int e = 0;
         PH_CHECK(hipMemcpyAsync(&e, s->dErr.p, sizeof(int), hipMemcpyDeviceToHost, st));
         PH_CHECK(hipStreamSynchronize(st));

@@ -53,6 +53,7 @@ struct phase_hip_solver {
   int mlDenseRows = 256; // a level of at most this many rows is inverted densely
   int mlJacobiFrom = 0;  // coarse levels from this one (0 = all) smooth with damped Jacobi instead of ILU(0)
   double mlJacobiWeight = 2.0 / 3.0;  // the damping weight
+  int mlDensePivot = 0;  // 1: the coarsest inverse always pivots (0: only when a pivot is small)
 
   // pattern and ordering
   int rank = 0;
@@ -80,7 +81,8 @@ struct phase_hip_solver {
   phase::KrylovVecs<float> k32;
 
   // synchronisation and reductions
-  phase::DBuf<int> dErr;  // bit 2: zero pivot in the factorisation
+  phase::DBuf<int> dErr;  // bit 2: zero pivot in the factorisation; 4: singular coarsest multigrid level;
+                          // 8: small pivot in its unpivoted inverse (inverted again with pivoting)
   phase::DBuf<double> dPartials;
   phase::DBuf<unsigned> dCounter;
   phase::DBuf<phase::Scalars> dSc;  // [0] inner run, [1] outer residual, [2] misc

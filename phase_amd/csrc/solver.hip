@@ -780,6 +780,11 @@ int phase_hip_solver_setup(phase_hip_solver *s, const char *key, const char *val
       if (!(w > 0. && w <= 1.5)) fail(PHASE_HIP_EINVAL, "multigridJacobiWeight must be in (0, 1.5]");
       if (w != s->mlJacobiWeight) mlValuesChanged(s);
       s->mlJacobiWeight = w;
+    } else if (k == "multigridDensePivoting") {
+      const int pv = lv == "always" ? 1 : lv == "auto" ? 0 : -1;
+      if (pv < 0) fail(PHASE_HIP_EINVAL, "multigridDensePivoting is AUTO or ALWAYS");
+      if (pv != s->mlDensePivot) mlValuesChanged(s);
+      s->mlDensePivot = pv;
     } else if (k == "multigridFinePostSmoothing") {
       s->mlFinePost = (lv == "1" || lv == "true" || lv == "yes") ? 1 : 0;
     } else if (k == "multigridCoarseRows") {

@@ -80,16 +80,19 @@ def test_multigrid_cycles_converge(hip, sys2, cycle, smoother):
 
 # (fine aggregate rows, coarse aggregate rows, dense limit) -> last level of
 # 26, 74, 176 and 254 rows on this mesh (CPU hierarchy of the same orderings)
+@pytest.mark.parametrize("pivoting", ["auto", "always"])
 @pytest.mark.parametrize("fine,coarse,dense,lo,hi", [(32, 16, 60, 1, 64), (32, 4, 120, 65, 128),
                                                      (48, 16, 256, 129, 192), (32, 16, 256, 193, 256)])
-def test_multigrid_dense_inverse(hip, sys1, fine, coarse, dense, lo, hi):
+def test_multigrid_dense_inverse(hip, sys1, fine, coarse, dense, lo, hi, pivoting):
     """The last level's inverse, from each register layout of the Gauss-Jordan
-    kernel (up to 64, 128 and 192 rows) and from the global-memory one (above
+    kernels (up to 64, 128 and 192 rows; without pivoting, as a pressure
+    system takes, and with pivoting) and from the global-memory one (above
     192): times the last level's Galerkin matrix it gives the identity to
     rounding, and the solve converges."""
     rp, ci, v, b, A, kappa, x_ref = sys1
     s, rel, x = _solve(hip, rp, ci, v, b, mode="fp64", solver="cg", preconditioner="multigrid",
-                       blockRows=fine, multigridDenseRows=dense, multigridCoarseRows=coarse)
+                       blockRows=fine, multigridDenseRows=dense, multigridCoarseRows=coarse,
+                       multigridDensePivoting=pivoting)
     _check(A, b, x, x_ref, kappa, rel)
     m, inv = s.multigrid_coarsest()
     n = len(m)
