@@ -384,8 +384,11 @@ def multiphase_case(args):
     dt = 1e-4; smoothing kernel radius 2.1 cells (the reference's
     RisingBubble example: 0.021 at h = 0.01), pow8; walls u = 0, p = 0 on the
     top.  Each step: volume-fraction solve, properties and surface force,
-    then `args.correctors` passes of momentum solve, pressure solve and
-    corrections (2: the PISO corrector count of the workload)."""
+    then `args.multiphase_correctors` passes of momentum solve, pressure solve
+    and corrections (1, the reference's fractional step: with two passes this
+    step diverges, because dt = 1e-4 is above the capillary time-step limit of
+    these 1e-3 cells, about 5.7e-5, and the pass iteration does not contract
+    there)."""
     from phase_amd import hip as P
     from phase_amd import meshgen as G
 
@@ -410,7 +413,7 @@ def multiphase_case(args):
         d.setup(k + ".mode", args.multiphase_mode)
     d.setup("p.solver", "cg")
     d.setup("p.preconditioner", args.multiphase_precond)
-    d.setup("numPressureCorrections", args.correctors)
+    d.setup("numPressureCorrections", args.multiphase_correctors)
     d.surface_tension(24.5, 2.1 * h, "pow8")
     setup_s = time.perf_counter() - t0
     dt = 1e-4
@@ -421,7 +424,7 @@ def multiphase_case(args):
     u, p, uf, g = d.fields()
     out = dict(workload="rising-bubble-multiphase-step-1x2-quads", cells=mesh.n_cells, ms_per_step=round(ms, 2),
                steps=args.multiphase_steps, setup_s=round(setup_s, 2), mode=args.multiphase_mode,
-               pressure_preconditioner=args.multiphase_precond, pressure_correctors=args.correctors,
+               pressure_preconditioner=args.multiphase_precond, pressure_correctors=args.multiphase_correctors,
                iters_gamma_u_p=[float(x) for x in hist.mean(0)[:3]], max_div=float("%.3g" % hist[:, 3].max()),
                max_u=float("%.4g" % np.abs(u).max()))
     log("multiphase: %.2f ms/step (setup %.1f s), iterations gamma/u/p %s, max div %.2e" %
@@ -492,8 +495,8 @@ def ib_flow_case(args):
     ghost-cell velocity rows and NORMAL_GRADIENT pressure rows, inflow u = (1, 0)
     at x = 0, walls at y = 0 and 20, outlet at x = 30 (u zero gradient, p = 0);
     rho = 1, Re = 200 (mu = 0.005), dt = 2e-3; BiCGStab to 1e-8 for both
-    equations, each warm-started from the previous step, `args.correctors`
-    passes per step.  Reports ms per step and the body force
+    equations, each warm-started from the previous step,
+    `args.ib_flow_correctors` passes per step.  Reports ms per step and the body force
     (applyHydrodynamicForce) after the last step."""
     from phase_amd import hip as P
     from phase_amd import meshgen as G
@@ -520,7 +523,7 @@ def ib_flow_case(args):
         d.setup(k + ".maxIters", 100000)
         d.setup(k + ".mode", args.ib_flow_mode)
     d.setup("p.preconditioner", args.ib_flow_precond)
-    d.setup("numPressureCorrections", args.correctors)
+    d.setup("numPressureCorrections", args.ib_flow_correctors)
     setup_s = time.perf_counter() - t0
     d.advance(2e-3, 2)
     t0 = time.perf_counter()
@@ -530,7 +533,7 @@ def ib_flow_case(args):
     force = ib.hydrodynamic_force(1.0, 0.005, P.DeviceArray(u.reshape(-1)), P.DeviceArray(p))[0]
     out = dict(workload="cylinder-ghost-cell-flow-step-30x20-quads", cells=mesh.n_cells, ms_per_step=round(ms, 2),
                steps=args.ib_flow_steps, setup_s=round(setup_s, 2), mode=args.ib_flow_mode,
-               pressure_preconditioner=args.ib_flow_precond, pressure_correctors=args.correctors,
+               pressure_preconditioner=args.ib_flow_precond, pressure_correctors=args.ib_flow_correctors,
                iters_u_p=[float(x) for x in hist.mean(0)[:2]], max_div=float("%.3g" % hist[:, 2].max()),
                force=[float("%.5g" % x) for x in force])
     log("ib flow: %.2f ms/step (setup %.1f s), iterations u/p %s, max div %.2e, force %s" %
@@ -619,8 +622,11 @@ def main():
     ap.add_argument("--multiphase-mode", default="mixed", help="solver mode of the multiphase driver's solves")
     ap.add_argument("--multiphase-precond", default="multigrid", help="pressure preconditioner (ilu or multigrid)")
     ap.add_argument("--ib-flow-precond", default="multigrid", help="pressure preconditioner (ilu or multigrid)")
-    ap.add_argument("--correctors", type=int, default=2,
-                    help="pressure-corrector passes per time step of the multiphase and immersed-boundary steps")
+    ap.add_argument("--multiphase-correctors", type=int, default=1,
+                    help="pressure-corrector passes per multiphase step (two diverge at this dt and mesh: the "
+                         "surface-tension time-step limit is below dt)")
+    ap.add_argument("--ib-flow-correctors", type=int, default=2,
+                    help="pressure-corrector passes per immersed-boundary flow step")
     ap.add_argument("--copy-probe-mb", type=int, default=0, help="run one copy kernel of this size (counter scale)")
     ap.add_argument("--no-host-path", dest="host_path", action="store_false",
                     help="skip the host-buffer (PCIe-inclusive) solve")
