@@ -105,18 +105,22 @@ int phase_hip_solver_destroy(phase_hip_solver *s);
  * "fallbackRatio" (learned: minimum per-iteration residual reduction before
  * falling back, 0.5),
  * "preconditioner" (ILU = Ifpack2 RILUK(0) as the reference's Belos backend,
- * the default; MULTIGRID = aggregation multigrid with ILU(0) smoothing, the
- * role of the reference's MueLu backend, TrilinosMueluSparseMatrixSolver.cpp:26-35),
+ * the default; MULTIGRID = aggregation multigrid, ILU(0) smoothing on the
+ * finest level and damped Jacobi on the coarse ones, the role of the
+ * reference's MueLu backend, TrilinosMueluSparseMatrixSolver.cpp:26-35),
  * "multigridCycle" (K | V | auto: K cycles with flexible CG, V cycles with
  * BiCGStab), "multigridKLevels" (coarse levels, from the first, whose solves
- * are K cycles; V cycles below; 2), "multigridBlockJacobiFrom" (coarse
- * levels from this index on, 0 being the first, smooth with the ILU(0) of
- * their block-diagonal part in one launch instead of the block-coloured
- * ILU(0); 0), "multigridFinePostSmoothing" (1 | 0),
+ * are K cycles; V cycles below; 2), "multigridJacobiFrom" (coarse levels
+ * from this index on, 0 being the first, smooth with damped Jacobi; the
+ * others with ILU(0); 0), "multigridJacobiWeight" (the damping, 2/3),
+ * "multigridBlockJacobiFrom" (coarse ILU(0) levels from this index on use
+ * the ILU(0) of their block-diagonal part in one launch instead of the
+ * block-coloured ILU(0); 0), "multigridFinePostSmoothing" (1 | 0),
  * "multigridCoarseRows" (rows per aggregate below the finest level, 16),
  * "multigridDenseRows" (a level of at most this many rows is inverted
- * densely, 256, at most 512).  Unknown keys are ignored, as the
- * reference ignores ptree keys it does not read. */
+ * densely, 256, at most 512), "multigridDensePivoting" (AUTO: without
+ * pivoting unless a pivot is small | ALWAYS).  Unknown keys are ignored, as
+ * the reference ignores ptree keys it does not read. */
 int phase_hip_solver_setup(phase_hip_solver *s, const char *key, const char *value);
 
 /* SparseMatrixSolver::setRank(int) -- SparseMatrixSolver.h:23 */
