@@ -111,8 +111,9 @@ int phase_hip_solver_destroy(phase_hip_solver *s);
  * "multigridCycle" (K | V | auto: K cycles with flexible CG, V cycles with
  * BiCGStab), "multigridKLevels" (coarse levels, from the first, whose solves
  * are K cycles; V cycles below; 2), "multigridJacobiFrom" (coarse levels
- * from this index on, 0 being the first, smooth with damped Jacobi; the
- * others with ILU(0); 0), "multigridJacobiWeight" (the damping, 2/3),
+ * from this index on, 0 being the first, smooth with damped Jacobi, the
+ * others with ILU(0); AUTO: all with CG, none with BiCGStab),
+ * "multigridJacobiWeight" (the damping, 2/3),
  * "multigridBlockJacobiFrom" (coarse ILU(0) levels from this index on use
  * the ILU(0) of their block-diagonal part in one launch instead of the
  * block-coloured ILU(0); 0), "multigridFinePostSmoothing" (1 | 0),

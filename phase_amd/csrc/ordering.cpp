@@ -336,7 +336,7 @@ static bool brickAggregates(int n, const Graph &g, const double *xy, int target,
 }
 
 void buildOrdering(Ordering &o, int n, const int *rowPtr, const int *colInd, int blockRows, const double *cellXY,
-                   bool blockJacobi) {
+                   bool blockJacobi, bool iluLimits) {
   if (n <= 0) fail(PHASE_HIP_EINVAL, "rank must be positive");
   if (blockRows < 1 || blockRows > kLanes) fail(PHASE_HIP_EINVAL, "blockRows must be in [1,64]");
   o = Ordering();
@@ -366,7 +366,23 @@ void buildOrdering(Ordering &o, int n, const int *rowPtr, const int *colInd, int
   std::vector<std::vector<int>> members;
   std::vector<int> colour;
   int nColours = 0;
-  if (cellXY && brickAggregates(n, g, cellXY, blockRows, rpos, blk, members, colour)) {
+  const bool bricks = cellXY && brickAggregates(n, g, cellXY, blockRows, rpos, blk, members, colour);
+  if (!bricks) {
+    // rows coupled to no other row (the solid cells of an immersed boundary)
+    // would each be an aggregate of their own, and stay so on every coarser
+    // level: they are grouped blockRows at a time in RCM order instead (they
+    // couple to nothing, so a group of them factors exactly and coarsens to
+    // another such row)
+    std::vector<int> iso;
+    for (int s0 : rorder)
+      if (g.ptr[s0 + 1] == g.ptr[s0]) iso.push_back(s0);
+    for (size_t k = 0; k < iso.size(); k += blockRows) {
+      const int b = (int)members.size();
+      members.emplace_back(iso.begin() + k, iso.begin() + std::min(iso.size(), k + blockRows));
+      for (int r : members.back()) blk[r] = b;
+    }
+  }
+  if (bricks) {
     nColours = 3;
   } else if (blockJacobi) {
     // breadth-first aggregates as below; no colouring (one colour)
@@ -596,12 +612,14 @@ void buildOrdering(Ordering &o, int n, const int *rowPtr, const int *colInd, int
       }
       if (nL > 255 || nU > 255) fail(PHASE_HIP_EINVAL, "row too long for the ILU layout");
       // the lane record holds these counts in 4 bits (the sweep kernels keep
-      // up to 8 in registers and read the rest from the factor)
-      if (nU - nUin > 15)
+      // up to 8 in registers and read the rest from the factor); an ordering
+      // whose ILU(0) is never applied (a Jacobi-smoothed multigrid level)
+      // takes any width, its lane records unused
+      if (iluLimits && nU - nUin > 15)
         fail(PHASE_HIP_EINVAL, "row couples to more than 15 rows of later ILU blocks");
-      if (nL - nLoff > 15 || nUin > 15)
+      if (iluLimits && (nL - nLoff > 15 || nUin > 15))
         fail(PHASE_HIP_EINVAL, "row couples to more than 15 rows of its own ILU block");
-      if (nLoff > 63) fail(PHASE_HIP_EINVAL, "row couples to more than 63 rows of earlier ILU blocks");
+      if (iluLimits && nLoff > 63) fail(PHASE_HIP_EINVAL, "row couples to more than 63 rows of earlier ILU blocks");
       maxRegs = std::max({maxRegs, nL - nLoff, nLoff, nUin, nU - nUin});
       o.rows[k].nL = (uint8_t)nL;
       o.rows[k].nLoff = (uint8_t)nLoff;
@@ -638,7 +656,8 @@ void buildOrdering(Ordering &o, int n, const int *rowPtr, const int *colInd, int
       const int k = B.rowStart + l;
       const RowInfo &r = o.rows[k];
       o.lanes[(size_t)b * kLanes + l] =
-          LaneRec{k, packLane(r.nL - r.nLoff, r.nLoff, r.nUin, r.nU - r.nUin, r.lev, r.ulev)};
+          LaneRec{k, packLane(std::min(r.nL - r.nLoff, 15), std::min((int)r.nLoff, 63), std::min((int)r.nUin, 15),
+                              std::min(r.nU - r.nUin, 15), r.lev, r.ulev)};
     }
   }
   o.lCol.assign(lRun, -1);

@@ -129,8 +129,11 @@ struct Ordering {
 // a scalar equation on a 2-D mesh gets three-colour brick aggregates.
 // blockJacobi: one colour, and the ILU(0) factor of the block-diagonal part
 // only (couplings between blocks dropped): one sweep launch per apply.
+// iluLimits false: the ordering's ILU(0) will not be applied (a multigrid
+// level smoothed by Jacobi), so rows wider than the sweep kernels' lane
+// records are accepted.
 void buildOrdering(Ordering &o, int n, const int *rowPtr, const int *colInd, int blockRows,
-                   const double *cellXY = nullptr, bool blockJacobi = false);
+                   const double *cellXY = nullptr, bool blockJacobi = false, bool iluLimits = true);
 
 // One coarser level of the aggregation multigrid hierarchy, built from the
 // aggregates of a finer level's ordering (unsmoothed aggregation: the

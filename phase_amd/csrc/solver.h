@@ -43,6 +43,7 @@ struct phase_hip_solver {
   int cg = 0;        // Krylov method: 0 = BiCGStab, 1 = preconditioned CG (symmetric systems)
   int unionPattern = 0;  // keep the stored pattern when a new one lies inside it (its other entries 0)
   int blockJacobi = 0;   // ILU(0) of the block-diagonal part only, one colour (coarse multigrid levels)
+  int iluLimits = 1;     // 0: the ordering's ILU(0) is never applied (Jacobi multigrid levels), any row width
   int precond = 0;       // 0 = ILU(0), 1 = aggregation multigrid with ILU(0) smoothing
   int mlKCycleSetting = -1;  // multigrid coarse solves: 1 = K cycle, 0 = V cycle, -1 = K for CG, V for BiCGStab
   int mlKCycle = 0;          // the choice in force for the current solve
@@ -51,7 +52,8 @@ struct phase_hip_solver {
   int mlFinePost = 1;    // post-smoothing on the finest level (0: pre-smoothing only)
   int mlBlockJacobiFrom = 0;   // coarse levels from this one (0 = the first) smooth with block-Jacobi ILU(0)
   int mlDenseRows = 256; // a level of at most this many rows is inverted densely
-  int mlJacobiFrom = 0;  // coarse levels from this one (0 = all) smooth with damped Jacobi instead of ILU(0)
+  int mlJacobiFrom = -1;  // coarse levels from this one (0 = all) smooth with damped Jacobi instead of ILU(0);
+                          // -1: all with CG, none with BiCGStab
   double mlJacobiWeight = 2.0 / 3.0;  // the damping weight
   int mlDensePivot = 0;  // 1: the coarsest inverse always pivots (0: only when a pivot is small)
 

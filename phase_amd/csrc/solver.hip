@@ -175,24 +175,37 @@ double normB2(phase_hip_solver *s) {
   return b2;
 }
 
-// Run BiCGStab chunks until the device says done; returns iterations taken.
+// Run Krylov chunks until the device says done; returns iterations taken.
 // onChunk(iterationsSoFar) runs on the host after every chunk and returns
-// false to stop the recurrence.
+// false to stop the recurrence.  Iterations queued after the one that meets
+// the threshold still launch (each returns at once), so once a rate is known
+// the next chunk is the number of iterations the last chunk's rate of
+// decrease needs to reach the threshold (at most `chunk`): a guess one short
+// costs a host round trip, one long an iteration of empty launches.
 template <class T, class TX, class TV, class OnChunk>
 int runKrylov(phase_hip_solver *s, KrylovVecs<T> &K, DBuf<T> &fac, const TV *sell, TX *x, int budget,
               OnChunk &&onChunk) {
   Scalars *sc = s->dSc.p;  // slot 0
-  int it = 0;
+  int it = 0, chunk = s->chunk, itLast = 0;
+  double rrLast = -1.;
   while (true) {
-    for (int c = 0; c < s->chunk; ++c) {
+    for (int c = 0; c < chunk; ++c) {
       if (s->cg) iterateCg<T, TX, TV>(s, K, fac, sell, x, sc);
       else iterate<T, TX, TV>(s, K, fac, sell, x, sc);
     }
     PH_CHECK(hipGetLastError());
     pullScalars(s);
-    it = s->hSc[0].iters;
-    if (s->hSc[0].done != kDoneRunning || it >= budget) break;
+    const Scalars &h = s->hSc[0];
+    it = h.iters;
+    if (h.done != kDoneRunning || it >= budget) break;
     if (!onChunk(it)) break;
+    chunk = s->chunk;
+    if (rrLast > 0. && h.rr > h.tol2 && h.rr < rrLast && it > itLast) {
+      const double need = std::log(h.tol2 / h.rr) / (std::log(h.rr / rrLast) / (it - itLast));
+      if (need < chunk) chunk = std::max(1, (int)std::ceil(need));
+    }
+    rrLast = h.rr;
+    itLast = it;
   }
   checkDeviceErr(s);
   return it;
@@ -439,7 +452,8 @@ namespace phase {
 void solverSetPattern(phase_hip_solver *s, int n, const int *rowPtr, const int *colInd, const double *cellXY) {
   setDevice(s);
   if (s->rank != 0 && n != s->rank) fail(PHASE_HIP_EINVAL, "pattern size differs from setRank");
-  buildOrdering(s->ord, n, rowPtr, colInd, s->blockRows, s->geometric ? cellXY : nullptr, s->blockJacobi != 0);
+  buildOrdering(s->ord, n, rowPtr, colInd, s->blockRows, s->geometric ? cellXY : nullptr, s->blockJacobi != 0,
+                s->iluLimits != 0);
   mlFree(s);
   if (cellXY && s->geometric) s->cellXY.assign(cellXY, cellXY + 2 * (size_t)n);
   else s->cellXY.clear();
@@ -771,10 +785,9 @@ int phase_hip_solver_setup(phase_hip_solver *s, const char *key, const char *val
       if (r != s->mlBlockJacobiFrom) mlFree(s);
       s->mlBlockJacobiFrom = r;
     } else if (k == "multigridJacobiFrom") {
-      const int r = std::stoi(v);
-      if (r < 0) fail(PHASE_HIP_EINVAL, "multigridJacobiFrom must be >= 0");
-      if (r != s->mlJacobiFrom) mlFree(s);
-      s->mlJacobiFrom = r;
+      const int r = lv == "auto" ? -1 : std::stoi(v);
+      if (r < -1) fail(PHASE_HIP_EINVAL, "multigridJacobiFrom must be AUTO or >= 0");
+      s->mlJacobiFrom = r;  // the hierarchy is rebuilt at the next solve if this changes its levels
     } else if (k == "multigridJacobiWeight") {
       const double w = std::stod(v);
       if (!(w > 0. && w <= 1.5)) fail(PHASE_HIP_EINVAL, "multigridJacobiWeight must be in (0, 1.5]");
