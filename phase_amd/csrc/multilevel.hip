@@ -500,21 +500,29 @@ __global__ __launch_bounds__(kRedBlock) void kMlResid(int n, int nSlices, const 
 }
 
 // rc[k] = sum of t over the finer rows of aggregate restRow[k] (identity when
-// null).  One wavefront per coarse row (an aggregate has at most 64 rows,
-// consecutive in the finer solver order): lane j reads row aggPtr[a] + j, a
-// coalesced read, and the wavefront sums in a fixed order in FP64.
+// null; an aggregate's rows are consecutive in the finer solver order), one
+// thread per coarse row, summing the rows in ascending order in FP64: each
+// lane streams its own contiguous rows.
 template <class T>
-__global__ __launch_bounds__(256) void kMlRestrict(int nc, const int *__restrict__ restRow,
-                                                   const int *__restrict__ aggPtr, const T *__restrict__ t,
-                                                   T *__restrict__ rc, const Scalars *sc) {
+__global__ __launch_bounds__(256) void kMlRestrictRows(int nc, const int *__restrict__ restRow,
+                                                       const int *__restrict__ aggPtr, const T *__restrict__ t,
+                                                       T *__restrict__ rc, const Scalars *sc) {
   if (sc->done) return;
-  const int k = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int k = blockIdx.x * 256 + threadIdx.x;
   if (k >= nc) return;
   const int a = restRow ? restRow[k] : k;
-  const int row = aggPtr[a] + lane;
-  double d = row < aggPtr[a + 1] ? (double)t[row] : 0.;
-  d = waveSum(d);
-  if (lane == 0) rc[k] = (T)d;
+  const int r0 = aggPtr[a], r1 = aggPtr[a + 1];
+  double d = 0.;
+  int i = r0;
+  for (; i + 4 <= r1; i += 4) {
+    const T v0 = t[i], v1 = t[i + 1], v2 = t[i + 2], v3 = t[i + 3];
+    d += (double)v0;
+    d += (double)v1;
+    d += (double)v2;
+    d += (double)v3;
+  }
+  for (; i < r1; ++i) d += (double)t[i];
+  rc[k] = (T)d;
 }
 
 // e[i] += ec[pIdx[i]]
@@ -557,10 +565,10 @@ __device__ __forceinline__ T sellRowAt(int row, const int *__restrict__ sliceOff
   return (T)acc;
 }
 
-// Coarse levels (launch-bound), kMlResid and kMlRestrict in one launch:
+// Coarse levels (launch-bound), kMlResid and the restriction in one launch:
 // rc[k] = sum over the rows of aggregate restRow[k] of (b - A x), one
 // wavefront per coarse row; each row's residual is rounded to T before the
-// FP64 sum, as the two kernels do.
+// FP64 sum (in a wavefront's fixed order).
 template <class T, class TV>
 __global__ __launch_bounds__(256) void kMlResidRestrict(int nc, const int *__restrict__ restRow,
                                                         const int *__restrict__ aggPtr,
@@ -915,8 +923,8 @@ void coarseCorrection(phase_hip_solver *s, Multilevel &M, int l, const Scalars *
   MlLevel &L = M.lv[l];
   MlVecs<T> &V = vecs<T>(L);
   hipStream_t st = s->stream;
-  hipLaunchKernelGGL((kMlRestrict<T>), dim3((L.n + 3) / 4), dim3(256), 0, st, L.n, L.sol ? L.restRow.p : nullptr,
-                     L.finerAggPtr.p, t, V.r.p, sc);
+  hipLaunchKernelGGL((kMlRestrictRows<T>), dim3((L.n + 255) / 256), dim3(256), 0, st, L.n,
+                     L.sol ? L.restRow.p : nullptr, L.finerAggPtr.p, t, V.r.p, sc);
   coarseSolve<T>(s, M, l, sc, V.r.p, V.e.p);
   hipLaunchKernelGGL((kMlProlongAdd<T>), dim3(elemGrid(L.nFiner)), dim3(256), 0, st, L.nFiner, L.pIdx.p,
                      (const T *)V.e.p, z, sc);
