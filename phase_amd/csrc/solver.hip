@@ -159,11 +159,6 @@ void iterateCg(phase_hip_solver *s, KrylovVecs<T> &K, DBuf<T> &fac, const TV *se
   cgBeta<T>(s, K, sc);
 }
 
-void pullScalars(phase_hip_solver *s) {
-  PH_CHECK(hipMemcpyAsync(s->hSc, s->dSc.p, 3 * sizeof(Scalars), hipMemcpyDeviceToHost, s->stream));
-  PH_CHECK(hipStreamSynchronize(s->stream));
-}
-
 double normB2(phase_hip_solver *s) {
   const int n = s->rank;
   const int g = gridFor(n, kRedBlock, kRedCap);
@@ -759,9 +754,14 @@ int phase_hip_solver_setup(phase_hip_solver *s, const char *key, const char *val
     else if (k == "maxOuter") s->maxOuter = std::stoi(v);
     else if (k == "chunk") s->chunk = std::max(1, std::stoi(v));
     else if (k == "solver") {
-      if (lv == "bicgstab") s->cg = 0;
-      else if (lv == "cg") s->cg = 1;
-      else fail(PHASE_HIP_EINVAL, "unknown solver '" + v + "' (BICGSTAB or CG)");
+      if (lv == "bicgstab") s->cg = 0, s->gmres = 0;
+      else if (lv == "cg") s->cg = 1, s->gmres = 0;
+      else if (lv == "gmres") s->cg = 0, s->gmres = 1;
+      else fail(PHASE_HIP_EINVAL, "unknown solver '" + v + "' (BICGSTAB, CG or GMRES)");
+    } else if (k == "gmresRestart") {
+      const int r = std::stoi(v);
+      if (r < 1 || r > kGmMaxRestart) fail(PHASE_HIP_EINVAL, "gmresRestart must be in [1,64]");
+      s->gmresRestart = r;
     } else if (k == "ordering") {
       const int g2 = lv == "geometric" ? 1 : lv == "graph" ? 0 : -1;
       if (g2 < 0) fail(PHASE_HIP_EINVAL, "unknown ordering '" + v + "' (geometric or graph)");
@@ -929,12 +929,25 @@ int phase_hip_solver_solve(phase_hip_solver *s, double *relres) {
       // hierarchy once per pattern, coarse matrices and factors once per values
       mlEnsureHierarchy(s);
       mlEnsureValues(s, s->mode != PHASE_HIP_MODE_FP64);
-      s->mlKCycle = s->mlKCycleSetting >= 0 ? s->mlKCycleSetting : (s->cg ? 1 : 0);
+      s->mlKCycle = s->mlKCycleSetting >= 0 ? s->mlKCycleSetting : (s->cg || s->gmres ? 1 : 0);
     }
     s->fallbacks = 0;
     s->learnedSteps = 0;
-    if (s->mode == PHASE_HIP_MODE_FP64) solveFp64(s);
-    else solveMixed(s);
+    if (s->gmres) {
+      // FP64 whatever the mode
+      const double b2 = normB2(s);
+      s->innerIters = s->nIters = s->outerPasses = 0;
+      if (b2 == 0.) {
+        PH_CHECK(hipMemsetAsync(s->dX.p, 0, s->rank * sizeof(double), s->stream));
+        s->error = 0.;
+      } else {
+        solveGmres(s, b2);
+      }
+    } else if (s->mode == PHASE_HIP_MODE_FP64) {
+      solveFp64(s);
+    } else {
+      solveMixed(s);
+    }
     PH_CHECK(hipEventRecord(s->ev1, s->stream));
     PH_CHECK(hipEventSynchronize(s->ev1));
     float ms = 0.f;
@@ -944,7 +957,7 @@ int phase_hip_solver_solve(phase_hip_solver *s, double *relres) {
     s->haveSolution = true;
     if (relres) *relres = s->error;
     if (!(s->error <= s->tol))
-      fail(PHASE_HIP_ENOCONV, "BiCGStab stopped at relative residual " + std::to_string(s->error) +
+      fail(PHASE_HIP_ENOCONV, "Krylov solve stopped at relative residual " + std::to_string(s->error) +
                                   " after " + std::to_string(s->nIters) + " iterations");
   });
 }

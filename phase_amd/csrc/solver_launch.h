@@ -116,6 +116,12 @@ void launchFactor(phase_hip_solver *s, const IluDev<T> &L) {
 inline int spmvGrid(phase_hip_solver *s) { return std::max(1, (s->ord.nSlices + 3) / 4); }
 
 
+// The three scalar records (Krylov run, outer refinement, misc) to the host.
+inline void pullScalars(phase_hip_solver *s) {
+  PH_CHECK(hipMemcpyAsync(s->hSc, s->dSc.p, 3 * sizeof(Scalars), hipMemcpyDeviceToHost, s->stream));
+  PH_CHECK(hipStreamSynchronize(s->stream));
+}
+
 inline void checkDeviceErr(phase_hip_solver *s) {
   int e = 0;
   PH_CHECK(hipMemcpyAsync(&e, s->dErr.p, sizeof(int), hipMemcpyDeviceToHost, s->stream));
