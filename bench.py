@@ -439,10 +439,11 @@ def ib_case(args):
     FractionalStepGCIB::solvePEqn's laplacian(dt / rho, p) + ib.bcs(p) ==
     src::div(u) (FractionalStepGCIB.cpp:44-46) with rho = 1, dt = 2e-3, the
     laplacian rows on the fluid cells only (remove(cell)), NORMAL_GRADIENT
-    ghost rows on the cylinder, p = 0 on the outlet (x = 30) and zero
-    gradient elsewhere; src::div(u) stands in as a zero-mean N(0,1) field on
-    the fluid cells.  The ghost rows are not symmetric: BiCGStab, plain FP64
-    against mixed precision, to 1e-8."""
+    ghost rows on the cylinder (scaled by 4 dt / rho, to a fluid row's
+    diagonal, as the time-step driver scales them), p = 0 on the outlet
+    (x = 30) and zero gradient elsewhere; src::div(u) stands in as a
+    zero-mean N(0,1) field on the fluid cells.  The ghost rows are not
+    symmetric: BiCGStab, plain FP64 against mixed precision, to 1e-8."""
     from phase_amd import hip as P
     from phase_amd import meshgen as G
 
@@ -471,7 +472,8 @@ def ib_case(args):
             nonlocal eq
             eq.laplacian(1, gamma=2e-3, bc_type=d_bc, phi_face=d_pf)
             eq.remove_cells(d_mask)
-            eq += ibq.gcib_bcs(ib, P.BC_NORMAL_GRADIENT, 0.0)
+            # the ghost rows scaled to a fluid row's diagonal, as the time step does
+            eq += ibq.gcib_bcs(ib, P.BC_NORMAL_GRADIENT, 0.0).scale(4 * 2e-3)
             eq.sub_vector(d_b)
             s.set_equation(eq)
             s.zero_guess()
