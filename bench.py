@@ -628,8 +628,8 @@ def main():
     ap.add_argument("--multiphase-correctors", type=int, default=1,
                     help="pressure-corrector passes per multiphase step (two diverge at this dt and mesh: the "
                          "surface-tension time-step limit is below dt)")
-    ap.add_argument("--ib-flow-correctors", type=int, default=1,
-                    help="pressure-corrector passes per immersed-boundary flow step (two diverged at 4M cells)")
+    ap.add_argument("--ib-flow-correctors", type=int, default=2,
+                    help="pressure-corrector passes per immersed-boundary flow step")
     ap.add_argument("--copy-probe-mb", type=int, default=0, help="run one copy kernel of this size (counter scale)")
     ap.add_argument("--no-host-path", dest="host_path", action="store_false",
                     help="skip the host-buffer (PCIe-inclusive) solve")
