@@ -241,6 +241,12 @@ int phase_hip_solver_kernel_bytes(phase_hip_solver *s, double *bytes, int nb);
  * maxPairs pairs in device memory (2 * maxPairs * rank floats). */
 int phase_hip_learned_capture(phase_hip_solver *s, int everyIters, int maxPairs);
 int phase_hip_learned_n_pairs(const phase_hip_solver *s, int *count);
+/* Checkpoint of the captured pairs: pairs [first, first + count) copied out
+ * in original numbering, pair-major (r and e: count * rank floats each). */
+int phase_hip_learned_pairs(phase_hip_solver *s, int first, int count, float *r, float *e);
+/* Append count pairs in that layout to the store phase_hip_learned_capture set
+ * up (EINVAL beyond its maxPairs; ESTATE without a store). */
+int phase_hip_learned_add_pairs(phase_hip_solver *s, int count, const float *r, const float *e);
 /* `steps` Adam steps (one captured pair each) at learning rate lr; *lastLoss is
  * the mean loss over those steps.  Loss = 1 - cos^2(network output, target
  * correction): only the direction matters, the solver scales the step. */

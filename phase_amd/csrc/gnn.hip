@@ -844,6 +844,60 @@ int phase_hip_learned_n_pairs(const phase_hip_solver *s, int *count) {
   return PHASE_HIP_OK;
 }
 
+// Captured pairs [first, first + count) to the host, original numbering,
+// pair-major (count * rank floats each for r and e).
+int phase_hip_learned_pairs(phase_hip_solver *s, int first, int count, float *r, float *e) {
+  return guardedG(s, [&] {
+    requireSystem(s);
+    PH_CHECK(hipSetDevice(s->device));
+    const int have = s->gnn ? s->gnn->nPairs : 0;
+    if (first < 0 || count < 0 || first + count > have) fail(PHASE_HIP_EINVAL, "pair range outside the captured pairs");
+    if (count > 0 && (!r || !e)) fail(PHASE_HIP_EINVAL, "null pair buffer");
+    GnnState &G = *s->gnn;
+    const int n = s->rank;
+    DBuf<float> tmp;
+    tmp.alloc(2 * (size_t)n);
+    for (int k = 0; k < count; ++k) {
+      const size_t slot = (size_t)(first + k) * n;
+      hipLaunchKernelGGL((kPermuteOut<float, float>), dim3(g1(n)), dim3(256), 0, s->stream, n, s->dPerm.p,
+                         G.pairR.p + slot, tmp.p);
+      hipLaunchKernelGGL((kPermuteOut<float, float>), dim3(g1(n)), dim3(256), 0, s->stream, n, s->dPerm.p,
+                         G.pairE.p + slot, tmp.p + n);
+      PH_CHECK(hipMemcpyAsync(r + (size_t)k * n, tmp.p, n * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+      PH_CHECK(hipMemcpyAsync(e + (size_t)k * n, tmp.p + n, n * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+      PH_CHECK(hipStreamSynchronize(s->stream));
+    }
+  });
+}
+
+// Append count pairs (original numbering, the layout phase_hip_learned_pairs
+// writes) to the capture store set up by phase_hip_learned_capture.
+int phase_hip_learned_add_pairs(phase_hip_solver *s, int count, const float *r, const float *e) {
+  return guardedG(s, [&] {
+    requireSystem(s);
+    PH_CHECK(hipSetDevice(s->device));
+    if (count < 0 || (count > 0 && (!r || !e))) fail(PHASE_HIP_EINVAL, "bad pair count or null pair buffer");
+    if (!s->gnn || s->gnn->maxPairs == 0) fail(PHASE_HIP_ESTATE, "no pair store: call phase_hip_learned_capture first");
+    GnnState &G = *s->gnn;
+    if (!G.pendingSnap.empty()) fail(PHASE_HIP_ESTATE, "a capture is in progress");
+    if (G.nPairs + count > G.maxPairs) fail(PHASE_HIP_EINVAL, "more pairs than the store holds (maxPairs)");
+    const int n = s->rank;
+    DBuf<float> tmp;
+    tmp.alloc(2 * (size_t)n);
+    for (int k = 0; k < count; ++k) {
+      const size_t slot = (size_t)(G.nPairs + k) * n;
+      PH_CHECK(hipMemcpyAsync(tmp.p, r + (size_t)k * n, n * sizeof(float), hipMemcpyHostToDevice, s->stream));
+      PH_CHECK(hipMemcpyAsync(tmp.p + n, e + (size_t)k * n, n * sizeof(float), hipMemcpyHostToDevice, s->stream));
+      hipLaunchKernelGGL((kPermuteIn<float, float>), dim3(g1(n)), dim3(256), 0, s->stream, n, s->dPerm.p, tmp.p,
+                         G.pairR.p + slot, 1.0);
+      hipLaunchKernelGGL((kPermuteIn<float, float>), dim3(g1(n)), dim3(256), 0, s->stream, n, s->dPerm.p, tmp.p + n,
+                         G.pairE.p + slot, 1.0);
+      PH_CHECK(hipStreamSynchronize(s->stream));
+    }
+    G.nPairs += count;
+  });
+}
+
 int phase_hip_learned_train(phase_hip_solver *s, int steps, double lr, int seed, double *lastLoss) {
   return guardedG(s, [&] {
     requireSystem(s);

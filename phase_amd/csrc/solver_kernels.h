@@ -191,11 +191,11 @@ __global__ void kPermuteIn(int n, const int *__restrict__ perm, const TI *__rest
 }
 
 // out[perm[k]] = in[k] (solver -> original order)
-template <class T>
+template <class T, class TO = double>
 __global__ void kPermuteOut(int n, const int *__restrict__ perm, const T *__restrict__ in,
-                            double *__restrict__ out) {
+                            TO *__restrict__ out) {
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
-    out[perm[k]] = (double)in[k];
+    out[perm[k]] = (TO)in[k];
 }
 
 template <class TI, class TO>

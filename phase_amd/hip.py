@@ -74,6 +74,8 @@ _SIGS = {
     "phase_hip_equation_export_padded": (ctypes.c_int, [_vp, _ip, _vp, _vp, _vp, _vp]),
     "phase_hip_learned_capture": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "phase_hip_learned_n_pairs": (ctypes.c_int, [_vp, _ip]),
+    "phase_hip_learned_pairs": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp]),
+    "phase_hip_learned_add_pairs": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp]),
     "phase_hip_learned_train": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp]),
     "phase_hip_learned_params": (ctypes.c_int, [_vp, _vp, _ip]),
     "phase_hip_learned_set_params": (ctypes.c_int, [_vp, _vp, ctypes.c_int]),
@@ -907,6 +909,25 @@ class HipSparseMatrixSolver:
         c = ctypes.c_int()
         self._check(load_library().phase_hip_learned_n_pairs(self._h, ctypes.byref(c)), "pairs")
         return c.value
+
+    def learned_pairs(self, first=0, count=None):
+        """Captured (residual, correction) pairs, original numbering: two
+        float32 arrays of shape (count, rank)."""
+        if count is None:
+            count = self.learned_n_pairs() - first
+        n = self.rank
+        r = np.empty((count, n), np.float32)
+        e = np.empty((count, n), np.float32)
+        self._check(load_library().phase_hip_learned_pairs(self._h, int(first), int(count), _p(r), _p(e)), "pairs")
+        return r, e
+
+    def learned_add_pairs(self, r, e):
+        """Append pairs in the layout learned_pairs() returns."""
+        r = np.ascontiguousarray(r, dtype=np.float32)
+        e = np.ascontiguousarray(e, dtype=np.float32)
+        if r.shape != e.shape or r.ndim != 2 or r.shape[1] != self.rank:
+            raise ValueError("pairs must be two (count, rank) arrays")
+        self._check(load_library().phase_hip_learned_add_pairs(self._h, r.shape[0], _p(r), _p(e)), "addPairs")
 
     def learned_train(self, steps, lr=1e-3, seed=0):
         loss = ctypes.c_double()

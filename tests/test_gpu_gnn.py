@@ -136,3 +136,52 @@ def test_fallback_on_useless_network(hip, system):
     s.set_guess(np.zeros(n))
     rr = s.solve()
     assert rr <= 1e-8 and s.stats()["fallbacks"] == 1
+
+
+def test_pair_checkpoint_round_trip(hip, system):
+    """Captured pairs leave the device in original numbering and load into
+    another solver bit for bit: the residual is the one of the captured
+    iterate (A e points along r, e being x_final - x_k), and training on the
+    reloaded pairs gives the same loss as on the originals."""
+    rp, ci, v, b = system
+    n = len(b)
+    A = H.compact_sorted(rp, ci, v)
+    s = hip.HipSparseMatrixSolver(0, tolerance=1e-8, maxIters=20000, mode="fp64")
+    s.set(rp, ci, v)
+    s.learned_capture(25, 40)
+    rng = np.random.default_rng(9)
+    for _ in range(3):
+        bb = rng.standard_normal(n)
+        s.set_rhs(bb - bb.mean())
+        s.set_guess(np.zeros(n))
+        s.solve()
+    npairs = s.learned_n_pairs()
+    assert npairs >= 6
+    r, e = s.learned_pairs()
+    assert r.shape == (npairs, n) and e.shape == (npairs, n)
+    # A e_k = r_k - r_final with ||r_final|| <= 1e-8 ||b||: checked on the
+    # pairs whose residual is well above that
+    rn = np.linalg.norm(r, axis=1)
+    big = np.flatnonzero(rn >= 1e-4 * rn.max())
+    assert len(big) >= 3
+    for k in big:
+        ae = A @ e[k].astype(np.float64)
+        cos = ae @ r[k] / (np.linalg.norm(ae) * rn[k])
+        assert cos > 0.999
+    r2, e2 = s.learned_pairs(2, 3)
+    assert np.array_equal(r2, r[2:5]) and np.array_equal(e2, e[2:5])
+
+    t = hip.HipSparseMatrixSolver(0, tolerance=1e-8, maxIters=20000, mode="fp64")
+    t.set(rp, ci, v)
+    t.learned_capture(0, npairs)
+    t.learned_add_pairs(r[:1], e[:1])
+    t.learned_add_pairs(r[1:], e[1:])
+    assert t.learned_n_pairs() == npairs
+    rt, et = t.learned_pairs()
+    assert np.array_equal(rt, r) and np.array_equal(et, e)
+    t.learned_set_params(s.learned_params())
+    assert t.learned_train(50, lr=0.0, seed=3) == s.learned_train(50, lr=0.0, seed=3)
+    with pytest.raises(hip.PhaseHipError):
+        t.learned_add_pairs(r[:1], e[:1])           # store full
+    with pytest.raises(hip.PhaseHipError):
+        s.learned_pairs(npairs - 1, 2)              # beyond the captured pairs
