@@ -111,12 +111,13 @@ def clean(v):
 
 
 def solver_kw(mode):
-    """Solver set-up of a method name "<precision>[-cg][-mg]": precision fp64,
-    mixed or learned; -cg preconditioned CG (Belos "CG") instead of BiCGStab;
-    -mg the aggregation multigrid preconditioner instead of ILU(0) alone."""
+    """Solver set-up of a method name "<precision>[-cg|-gmres][-mg]": precision
+    fp64, mixed or learned; -cg preconditioned CG (Belos "CG") and -gmres
+    flexible restarted GMRES (FP64) instead of BiCGStab; -mg the aggregation
+    multigrid preconditioner instead of ILU(0) alone."""
     parts = mode.split("-")
-    return dict(mode=parts[0], solver="cg" if "cg" in parts[1:] else "bicgstab",
-                preconditioner="multigrid" if "mg" in parts[1:] else "ilu")
+    solver = "cg" if "cg" in parts[1:] else "gmres" if "gmres" in parts[1:] else "bicgstab"
+    return dict(mode=parts[0], solver=solver, preconditioner="multigrid" if "mg" in parts[1:] else "ilu")
 
 
 def build_problem(n_cells, k, sigma):
@@ -306,8 +307,10 @@ def learned_case(args):
             per_solve = max(1, out["fp64"]["iters"] // every)
             a2 = argparse.Namespace(**vars(args))
             a2.capture_every = every
+            # one solve more than the count suggests: a solve's snapshots fall at
+            # multiples of `every` and the last may not complete a pair
             info = train_network(s, mesh, d_g, d_bc, d_pf, a2, kw, max_pairs=args.c4_pairs,
-                                 capture_solves=max(1, -(-args.c4_pairs // per_solve)))
+                                 capture_solves=1 + max(1, -(-args.c4_pairs // per_solve)))
         step()
         t0 = time.perf_counter()
         rel = step()
@@ -345,7 +348,10 @@ def bubble_case(args):
     f = mesh.faces()
     _, _, fc, _ = mesh.geometry()
     a, bn = xy[f["lnode"]], xy[f["rnode"]]
-    top = (f["rcell"] < 0) & (a[:, 1] == 2.0) & (bn[:, 1] == 2.0)
+    # the mesh's own top row of nodes (i * (2 / n) need not round to 2.0)
+    ytop = xy[:, 1].max()
+    top = (f["rcell"] < 0) & (a[:, 1] == ytop) & (bn[:, 1] == ytop)
+    assert top.any(), "no top boundary faces"
     bc = np.where(top, P.BC_FIXED, P.BC_NORMAL_GRADIENT).astype(np.int32)
     inside = (fc[:, 0] - 0.5) ** 2 + (fc[:, 1] - 0.5) ** 2 < 0.25 ** 2
     rho_f = np.where(inside, 1.0, 1000.0)
@@ -399,7 +405,10 @@ def multiphase_case(args):
     f = mesh.faces()
     _, cc, _, _ = mesh.geometry()
     a, bn = xy[f["lnode"]], xy[f["rnode"]]
-    top = (f["rcell"] < 0) & (a[:, 1] == 2.0) & (bn[:, 1] == 2.0)
+    # the mesh's own top row of nodes (i * (2 / n) need not round to 2.0)
+    ytop = xy[:, 1].max()
+    top = (f["rcell"] < 0) & (a[:, 1] == ytop) & (bn[:, 1] == ytop)
+    assert top.any(), "no top boundary faces"
     pbc = np.where(top, P.BC_FIXED, P.BC_NORMAL_GRADIENT).astype(np.int32)
     ubc = np.full(mesh.n_faces, P.BC_FIXED, np.int32)
     r = np.hypot(cc[:, 0] - 0.5, cc[:, 1] - 0.5)
@@ -439,11 +448,12 @@ def ib_case(args):
     FractionalStepGCIB::solvePEqn's laplacian(dt / rho, p) + ib.bcs(p) ==
     src::div(u) (FractionalStepGCIB.cpp:44-46) with rho = 1, dt = 2e-3, the
     laplacian rows on the fluid cells only (remove(cell)), NORMAL_GRADIENT
-    ghost rows on the cylinder (scaled by 4 dt / rho, to a fluid row's
+    ghost rows on the cylinder (scaled by -4 dt h / rho, to a fluid row's
     diagonal, as the time-step driver scales them), p = 0 on the outlet
     (x = 30) and zero gradient elsewhere; src::div(u) stands in as a
     zero-mean N(0,1) field on the fluid cells.  The ghost rows are not
-    symmetric: BiCGStab, plain FP64 against mixed precision, to 1e-8."""
+    symmetric: BiCGStab (plain FP64 against mixed precision) and flexible
+    GMRES, to 1e-8."""
     from phase_amd import hip as P
     from phase_amd import meshgen as G
 
@@ -453,7 +463,12 @@ def ib_case(args):
     mesh = P.HipMesh(xy, cptr, cind)
     f = mesh.faces()
     a, bn = xy[f["lnode"]], xy[f["rnode"]]
-    outlet = (f["rcell"] < 0) & (a[:, 0] == 30.0) & (bn[:, 0] == 30.0)
+    # the mesh's own last column of nodes: i * (30 / nx) need not round to
+    # 30.0 (at nx = 1732 it does not), and without outlet faces the pressure
+    # system would be all-Neumann, singular
+    xr = xy[:, 0].max()
+    outlet = (f["rcell"] < 0) & (a[:, 0] == xr) & (bn[:, 0] == xr)
+    assert outlet.any(), "no outlet faces"
     bc = np.where(outlet, P.BC_FIXED, P.BC_NORMAL_GRADIENT).astype(np.int32)
     ib = P.HipGhostCellIB(mesh, [(10.0, 10.0, 0.5)])
     types, _ = ib.cell_types()
@@ -463,17 +478,20 @@ def ib_case(args):
     ib.cell_types_device(d_mask)
     eq = P.HipEquation(mesh, 1)
     ibq = P.HipEquation(mesh, 1)
+    h = np.sqrt(30.0 * 20.0 / mesh.n_cells)  # the ghost cells' size (uniform quads)
     out = dict(workload="cylinder-ghost-cell-pressure-30x20-quads", cells=mesh.n_cells,
                ghost_cells=int((types == 1).sum()), solid_cells=int((types == 2).sum()))
-    for mode in ("fp64", "mixed", "fp64-mg", "mixed-mg"):
+    for mode in ("fp64", "mixed", "fp64-mg", "mixed-mg", "fp64-gmres-mg"):
         s = P.HipSparseMatrixSolver(0, tolerance=args.tol, maxIters=100000, **solver_kw(mode))
 
         def step():
             nonlocal eq
             eq.laplacian(1, gamma=2e-3, bc_type=d_bc, phi_face=d_pf)
             eq.remove_cells(d_mask)
-            # the ghost rows scaled to a fluid row's diagonal, as the time step does
-            eq += ibq.gcib_bcs(ib, P.BC_NORMAL_GRADIENT, 0.0).scale(4 * 2e-3)
+            # the ghost rows scaled to a fluid row's diagonal in size and sign
+            # (-4 dt h / rho; their coefficients are of order 1 / h), as the
+            # time step does
+            eq += ibq.gcib_bcs(ib, P.BC_NORMAL_GRADIENT, 0.0).scale(-4 * 2e-3 * h)
             eq.sub_vector(d_b)
             s.set_equation(eq)
             s.zero_guess()
@@ -511,7 +529,9 @@ def ib_flow_case(args):
     a, bn = xy[f["lnode"]], xy[f["rnode"]]
     bnd = f["rcell"] < 0
     left = bnd & (a[:, 0] == 0.0) & (bn[:, 0] == 0.0)
-    right = bnd & (a[:, 0] == 30.0) & (bn[:, 0] == 30.0)
+    xr = xy[:, 0].max()  # the mesh's own outlet column (see ib_case)
+    right = bnd & (a[:, 0] == xr) & (bn[:, 0] == xr)
+    assert right.any(), "no outlet faces"
     ubc = np.where(right, P.BC_NORMAL_GRADIENT, P.BC_FIXED).astype(np.int32)
     ub = np.zeros((mesh.n_faces, 2))
     ub[left, 0] = 1.0
@@ -525,6 +545,7 @@ def ib_flow_case(args):
         d.setup(k + ".maxIters", 100000)
         d.setup(k + ".mode", args.ib_flow_mode)
     d.setup("p.preconditioner", args.ib_flow_precond)
+    d.setup("p.solver", args.ib_flow_psolver)
     d.setup("numPressureCorrections", args.ib_flow_correctors)
     setup_s = time.perf_counter() - t0
     d.advance(2e-3, 2)
@@ -536,7 +557,7 @@ def ib_flow_case(args):
     force = ib.hydrodynamic_force(1.0, 0.005, P.DeviceArray(u.reshape(-1)), P.DeviceArray(p))[0]
     out = dict(workload="cylinder-ghost-cell-flow-step-30x20-quads", cells=mesh.n_cells, ms_per_step=round(ms, 2),
                steps=args.ib_flow_steps, setup_s=round(setup_s, 2), mode=args.ib_flow_mode,
-               pressure_preconditioner=args.ib_flow_precond, pressure_correctors=args.ib_flow_correctors,
+               pressure_preconditioner=args.ib_flow_precond, pressure_solver=args.ib_flow_psolver, pressure_correctors=args.ib_flow_correctors,
                iters_u_p=[float(x) for x in hist.mean(0)[:2]], max_div=float("%.3g" % hist[:, 2].max()),
                force=[float("%.5g" % x) for x in force])
     log("ib flow: %.2f ms/step (setup %.1f s), iterations u/p %s, max div %.2e, force %s" %
@@ -625,6 +646,7 @@ def main():
     ap.add_argument("--multiphase-mode", default="mixed", help="solver mode of the multiphase driver's solves")
     ap.add_argument("--multiphase-precond", default="multigrid", help="pressure preconditioner (ilu or multigrid)")
     ap.add_argument("--ib-flow-precond", default="multigrid", help="pressure preconditioner (ilu or multigrid)")
+    ap.add_argument("--ib-flow-psolver", default="gmres", help="pressure Krylov method (bicgstab or gmres)")
     ap.add_argument("--multiphase-correctors", type=int, default=1,
                     help="pressure-corrector passes per multiphase step (two diverge at this dt and mesh: the "
                          "surface-tension time-step limit is below dt)")

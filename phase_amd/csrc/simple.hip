@@ -364,6 +364,8 @@ struct phase_hip_fstep {
   DBuf<double> gq;
   phase_hip_equation *ibU = nullptr, *ibP = nullptr;
   double ibPScale = 1.;  // factor the ghost pressure rows carry (see timeStep)
+  double ibH = 1.;       // the ghost cells' mean size sqrt(volume)
+  int ibPBc = PHASE_HIP_BC_NORMAL_GRADIENT;
   int nCorr = 1;  // numPressureCorrections
   std::string err;
 };
@@ -465,13 +467,16 @@ void timeStep(phase_hip_fstep *s, double dt, double *hist) {
                               nullptr),
        "fv::laplacian", s->pEq->err.c_str());
     if (ib) {  // + ib_.bcs(p_)
-      // The ghost and solid rows are constraints of order one while the fluid
-      // rows scale with dt / rho: scaled to about a fluid row's diagonal (4 dt /
-      // rho on cells of aspect ratio near one) they have the same solution, and
-      // the Krylov solve and the multigrid coarsening converge several times
-      // faster (ghost-cell pressure system at 1M cells: 627 -> 166 BiCGStab
-      // iterations with multigrid; at 4M, 5,156 -> 1,002)
-      const double want = 4.0 * dt / s->rho;
+      // The ghost and solid rows are constraints independent of dt / rho, and
+      // a NORMAL_GRADIENT ghost row's coefficients are of order 1 / h, while a
+      // fluid row's diagonal is about -4 dt / rho (cells of aspect ratio near
+      // one) with positive neighbours.  Scaled to that size and sign (-4 dt h /
+      // rho for NORMAL_GRADIENT rows, -4 dt / rho for FIXED ones) they have
+      // the same solution, and the Krylov solve and the multigrid coarsening
+      // converge several times faster: with BiCGStab and multigrid at 1M cells
+      // 627 iterations unscaled, 166 at +4 dt / rho, 71 scaled so; at 4M cells
+      // 5,156, 1,002 and 164.
+      const double want = -4.0 * dt / s->rho * (s->ibPBc == PHASE_HIP_BC_NORMAL_GRADIENT ? s->ibH : 1.0);
       if (want != s->ibPScale) {
         ck(phase_hip_eq_scale(s->ibP, want / s->ibPScale), "operator*", s->ibP->err.c_str());
         s->ibPScale = want;
@@ -616,8 +621,23 @@ int attachImmersedBoundary(phase_hip_fstep *s, const phase_hip_gcib *ib, Velocit
     ck(velocityRows(s->ibU), "velocityBcs", s->ibU->err.c_str());
     ck(phase_hip_eq_gcib_bcs(s->ibP, ib, pBcType, pRefValue), "bcs", s->ibP->err.c_str());
     s->ibPScale = 1.;
+    s->ibPBc = pBcType;
     s->ibTypes.alloc(s->N);
     ck(phase_hip_gcib_cell_types(ib, s->ibTypes.p, nullptr, 1), "cell types", "");
+    {
+      std::vector<int> types(s->N);
+      std::vector<double> vol(s->N);
+      ck(phase_hip_gcib_cell_types(ib, types.data(), nullptr, 0), "cell types", "");
+      ck(phase_hip_mesh_geometry(m, vol.data(), nullptr, nullptr, nullptr), "geometry", m->err.c_str());
+      double hs = 0.;
+      long ng = 0;
+      for (int c = 0; c < s->N; ++c)
+        if (types[c] == 1) {
+          hs += std::sqrt(vol[c]);
+          ++ng;
+        }
+      s->ibH = ng ? hs / ng : 1.;
+    }
     s->gq.alloc(2 * (size_t)s->N);
     PH_CHECK(hipStreamSynchronize(m->stream));
     s->immersed = true;
