@@ -109,10 +109,18 @@ __global__ void kRhieChow(int nFaces, const int *__restrict__ fl, const int *__r
 }
 
 // row `pin` of a scalar equation becomes x_pin = 0
+// Row `pin` becomes a_pp p_pin = 0 with the row's own diagonal a_pp (1 when
+// it has none): the same solution as a unit row.  Under ILU(0) the Krylov
+// iterates are the same either way (the pin component of every residual stays
+// 0); the multigrid's Galerkin sums add the row into its aggregate, where a
+// row of the laplacian's sign and size belongs.
 __global__ void kPinRow(int pin, int stride, int *cols, double *vals, int *cnt, double *rhs) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
+    double d = 0.;
+    for (int j = 0; j < cnt[pin]; ++j)
+      if (cols[(size_t)pin * stride + j] == pin) d += vals[(size_t)pin * stride + j];
     cols[(size_t)pin * stride] = pin;
-    vals[(size_t)pin * stride] = 1.;
+    vals[(size_t)pin * stride] = d != 0. ? d : 1.;
     for (int j = 1; j < stride; ++j) {
       cols[(size_t)pin * stride + j] = -1;
       vals[(size_t)pin * stride + j] = 0.;
