@@ -105,15 +105,17 @@ def test_invalid_inputs(hip, case):
         hip.HipGhostCellIB(hip.HipMesh(xy, cptr, cind), [(0.5, 0.5, 0.2)])
 
 
-@pytest.mark.parametrize("omega,correctors", [(None, 1), (2.0, 1), (None, 2)])
-def test_fractional_step_past_a_cylinder(oracle, hip, omega, correctors):
+@pytest.mark.parametrize("omega,correctors,psolver", [(None, 1, {}), (2.0, 1, {}), (None, 2, {}),
+                                                     (None, 2, {"solver": "gmres", "preconditioner": "multigrid"})])
+def test_fractional_step_past_a_cylinder(oracle, hip, omega, correctors, psolver):
     """FractionalStepGCIB on the device: a channel with inflow u = 1, a fixed
     cylinder of radius 0.25 (ghost-cell velocity and NORMAL_GRADIENT pressure
     rows), outlet p = 0; four steps against the restatement with direct solves:
     velocities and pressure within 1e-7, solid cells at the body's velocity
     (at rest, or spinning at omega about its centre: Motion::velocity rows in
     the time step), fluid faces divergence-free; one or two pressure-corrector
-    passes per step."""
+    passes per step; the pressure by BiCGStab + ILU(0), or by flexible GMRES +
+    multigrid on the scaled ghost rows."""
     from oracle.fstep_oracle import FractionalStepOracle
     from oracle.ghostcell_oracle import GhostCellOracle
     xy, cptr, cind = G.rectilinear(48, 32, width=3.0, height=2.0)
@@ -140,6 +142,8 @@ def test_fractional_step_past_a_cylinder(oracle, hip, omega, correctors):
         dev.setup(k + ".tolerance", 1e-12)
         dev.setup(k + ".maxIters", 20000)
     dev.setup("numPressureCorrections", correctors)
+    for k, v in psolver.items():
+        dev.setup("p." + k, v)
     solid = og.types == 2
     _, cc = om.cells()
     w = 0.0 if omega is None else omega
