@@ -452,8 +452,8 @@ def ib_case(args):
     diagonal, as the time-step driver scales them), p = 0 on the outlet
     (x = 30) and zero gradient elsewhere; src::div(u) stands in as a
     zero-mean N(0,1) field on the fluid cells.  The ghost rows are not
-    symmetric: BiCGStab (plain FP64 against mixed precision) and flexible
-    GMRES, to 1e-8."""
+    symmetric: BiCGStab and flexible GMRES, plain FP64 against mixed
+    precision, to 1e-8."""
     from phase_amd import hip as P
     from phase_amd import meshgen as G
 
@@ -481,7 +481,7 @@ def ib_case(args):
     h = np.sqrt(30.0 * 20.0 / mesh.n_cells)  # the ghost cells' size (uniform quads)
     out = dict(workload="cylinder-ghost-cell-pressure-30x20-quads", cells=mesh.n_cells,
                ghost_cells=int((types == 1).sum()), solid_cells=int((types == 2).sum()))
-    for mode in ("fp64", "mixed", "fp64-mg", "mixed-mg", "fp64-gmres-mg"):
+    for mode in ("fp64", "mixed", "fp64-mg", "mixed-mg", "fp64-gmres-mg", "mixed-gmres-mg"):
         s = P.HipSparseMatrixSolver(0, tolerance=args.tol, maxIters=100000, **solver_kw(mode))
 
         def step():

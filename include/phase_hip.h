@@ -96,10 +96,13 @@ int phase_hip_solver_destroy(phase_hip_solver *s);
 
 /* SparseMatrixSolver::setup(ptree) -- TrilinosBelosSparseMatrixSolver.cpp:44-86.
  * Keys: "tolerance" (1e-8), "maxIters" (500), "solver" (BICGSTAB | CG |
- * GMRES, the reference's Belos names; GMRES is flexible, restarted and FP64
- * whatever the mode, the default of the reference's MueLu backend,
- * TrilinosMueluSparseMatrixSolver.cpp:47), "gmresRestart" (basis vectors
- * per GMRES cycle, 1 to 64, 30), "mode" (fp64|mixed|learned),
+ * GMRES, the reference's Belos names; GMRES is flexible and restarted, the
+ * default of the reference's MueLu backend,
+ * TrilinosMueluSparseMatrixSolver.cpp:47; in mode mixed or learned its
+ * cycles run in FP32 from the FP64 residual, each until its estimate drops
+ * 1e5-fold, continuing in FP64 when they stall; no network under GMRES),
+ * "gmresRestart" (basis vectors per GMRES cycle, 1 to 64, 30), "mode"
+ * (fp64|mixed|learned),
  * "innerTolerance" (mixed: residual reduction between FP64 refinement points, 0.1),
  * "maxOuter" (mixed: FP64 refinement passes, 100), "chunk" (iterations queued
  * between host checks, 8), "blockRows" (rows per ILU aggregate, <= 64,

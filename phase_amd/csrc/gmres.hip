@@ -15,16 +15,27 @@
 //     Givens rotations of column j; |g_j+1| is the residual norm
 //   y = H^-1 g (upper triangular), x += sum y_j z_j; restart from x
 //
-// FP64 throughout.  The Hessenberg column, the rotations and the residual
-// estimate live on the device, so an Arnoldi step needs no host round trip;
-// the host checks the step count and the convergence flag after every chunk
-// of steps, and every launch after convergence returns at once (except the
-// SpMV).  A cycle that ends on the estimate is followed by the true residual,
-// which decides.
+// In mode fp64 everything is FP64.  In mode mixed the basis V and the
+// preconditioned vectors Z are FP32, and so are the preconditioner (the FP32
+// copy of the ILU(0) factor, the multigrid cycle in FP32) and the products'
+// vectors (FP64 matrix values, rows accumulated in FP64); the dot products,
+// the Hessenberg matrix, the rotations and the solution stay FP64.  Every cycle
+// starts from the FP64 true residual, so restarts are the refinement points:
+// a mixed cycle stops once its estimate has dropped by kGmMixedDrop (or meets
+// the tolerance).  When kGmMixedStall cycles in a row either make no new best
+// true residual or leave it well above their own estimate without halving it
+// (the FP32 basis no longer carries the correction), or a cycle breaks down
+// at its first step, the solve continues in FP64 from the best iterate.
+// The Hessenberg column, the rotations and the residual estimate live on the
+// device, so an Arnoldi step needs no host round trip; the host checks the
+// step count and the convergence flag after every chunk of steps, and every
+// launch after convergence returns at once (except the SpMV).  A cycle that
+// ends on the estimate is followed by the true residual, which decides.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 #include "multilevel.h"
 #include "solver.h"
@@ -36,10 +47,13 @@ namespace phase {
 namespace {
 
 constexpr int kGmGroup = 8;
+constexpr double kGmMixedDrop = 1e-5;  // a mixed cycle's reduction of the residual estimate
+constexpr int kGmMixedStall = 3;
 
 // partials of (w, V_j) for the kGmGroup vectors j = 8 blockIdx.y + q < k
-__global__ __launch_bounds__(kRedBlock) void kGmDots(int n, int k, const double *__restrict__ V,
-                                                     const double *__restrict__ w, double *partials,
+template <class T>
+__global__ __launch_bounds__(kRedBlock) void kGmDots(int n, int k, const T *__restrict__ V,
+                                                     const T *__restrict__ w, double *partials,
                                                      const Scalars *sc) {
   if (sc->done) return;
   const int g0 = blockIdx.y * kGmGroup;
@@ -47,10 +61,10 @@ __global__ __launch_bounds__(kRedBlock) void kGmDots(int n, int k, const double 
 #pragma unroll
   for (int q = 0; q < kGmGroup; ++q) d[q] = 0.;
   for (int i = blockIdx.x * kRedBlock + threadIdx.x; i < n; i += gridDim.x * kRedBlock) {
-    const double wi = w[i];
+    const double wi = (double)w[i];
 #pragma unroll
     for (int q = 0; q < kGmGroup; ++q)
-      if (g0 + q < k) d[q] = fma(V[(size_t)(g0 + q) * n + i], wi, d[q]);
+      if (g0 + q < k) d[q] = fma((double)V[(size_t)(g0 + q) * n + i], wi, d[q]);
   }
   blockPartials<kGmGroup>(d, partials + (size_t)g0 * gridDim.x, blockIdx.x);
 }
@@ -72,10 +86,11 @@ __global__ __launch_bounds__(kFinBlock) void kGmDotsFinish(int k, int G, const d
   }
 }
 
-// w -= sum_{j<k} h_j V_j; with NORM the partials of (w, w)
-template <bool NORM>
-__global__ __launch_bounds__(kRedBlock) void kGmAxpy(int n, int k, const double *__restrict__ V,
-                                                     const double *__restrict__ hcur, double *__restrict__ w,
+// w -= sum_{j<k} h_j V_j (in FP64, stored in T); with NORM the partials of
+// (w, w) of the stored w
+template <class T, bool NORM>
+__global__ __launch_bounds__(kRedBlock) void kGmAxpy(int n, int k, const T *__restrict__ V,
+                                                     const double *__restrict__ hcur, T *__restrict__ w,
                                                      double *partials, const Scalars *sc) {
   if (sc->done) return;
   __shared__ double h[kGmMaxRestart + 1];
@@ -83,10 +98,11 @@ __global__ __launch_bounds__(kRedBlock) void kGmAxpy(int n, int k, const double 
   __syncthreads();
   double d[1] = {0.};
   for (int i = blockIdx.x * kRedBlock + threadIdx.x; i < n; i += gridDim.x * kRedBlock) {
-    double v = w[i];
-    for (int j = 0; j < k; ++j) v = fma(-h[j], V[(size_t)j * n + i], v);
-    w[i] = v;
-    if (NORM) d[0] = fma(v, v, d[0]);
+    double v = (double)w[i];
+    for (int j = 0; j < k; ++j) v = fma(-h[j], (double)V[(size_t)j * n + i], v);
+    const T vt = (T)v;
+    w[i] = vt;
+    if (NORM) d[0] = fma((double)vt, (double)vt, d[0]);
   }
   if (NORM) blockPartials<1>(d, partials, blockIdx.x);
 }
@@ -125,10 +141,18 @@ __global__ __launch_bounds__(kFinBlock) void kGmGivens(int j, int ld, const doub
 }
 
 // v *= *scale
-__global__ void kGmScale(int n, double *__restrict__ v, const double *scale, const Scalars *sc) {
+template <class T>
+__global__ void kGmScale(int n, T *__restrict__ v, const double *scale, const Scalars *sc) {
   if (sc->done) return;
   const double f = *scale;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) v[i] *= f;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) v[i] = (T)((double)v[i] * f);
+}
+
+// v_0 = r * *scale (the cycle's first basis vector from the FP64 residual)
+template <class T>
+__global__ void kGmFirst(int n, const double *__restrict__ r, const double *scale, T *__restrict__ v) {
+  const double f = *scale;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) v[i] = (T)(r[i] * f);
 }
 
 __global__ void kGmSetTol(double tol2, Scalars *sc) {
@@ -157,95 +181,197 @@ __global__ void kGmSolveY(int k, int ld, const double *H, const double *g, doubl
   }
 }
 
-// x += sum_{j<k} y_j Z_j
-__global__ __launch_bounds__(kRedBlock) void kGmUpdateX(int n, int k, const double *__restrict__ Z,
+// x += sum_{j<k} y_j Z_j (x in FP64)
+template <class T>
+__global__ __launch_bounds__(kRedBlock) void kGmUpdateX(int n, int k, const T *__restrict__ Z,
                                                         const double *__restrict__ y, double *__restrict__ x) {
   __shared__ double sy[kGmMaxRestart];
   for (int j = threadIdx.x; j < k; j += kRedBlock) sy[j] = y[j];
   __syncthreads();
   for (int i = blockIdx.x * kRedBlock + threadIdx.x; i < n; i += gridDim.x * kRedBlock) {
     double v = x[i];
-    for (int j = 0; j < k; ++j) v = fma(sy[j], Z[(size_t)j * n + i], v);
+    for (int j = 0; j < k; ++j) v = fma(sy[j], (double)Z[(size_t)j * n + i], v);
     x[i] = v;
   }
 }
 
-}  // namespace
+// The Krylov vectors of precision T
+template <class T>
+DBuf<T> &gmBasis(phase_hip_solver *s);
+template <>
+DBuf<double> &gmBasis<double>(phase_hip_solver *s) {
+  return s->gmV;
+}
+template <>
+DBuf<float> &gmBasis<float>(phase_hip_solver *s) {
+  return s->gmV32;
+}
+template <class T>
+DBuf<T> &gmPrec(phase_hip_solver *s);
+template <>
+DBuf<double> &gmPrec<double>(phase_hip_solver *s) {
+  return s->gmZ;
+}
+template <>
+DBuf<float> &gmPrec<float>(phase_hip_solver *s) {
+  return s->gmZ32;
+}
+template <class T>
+KrylovVecs<T> &krylov(phase_hip_solver *s);
+template <>
+KrylovVecs<double> &krylov<double>(phase_hip_solver *s) {
+  return s->k64;
+}
+template <>
+KrylovVecs<float> &krylov<float>(phase_hip_solver *s) {
+  return s->k32;
+}
+template <class T>
+T *factor(phase_hip_solver *s);
+template <>
+double *factor<double>(phase_hip_solver *s) {
+  return s->dFac64.p;
+}
+template <>
+float *factor<float>(phase_hip_solver *s) {
+  return s->dFac32.p;
+}
 
-void solveGmres(phase_hip_solver *s, double b2) {
+// GMRES cycles with Krylov vectors in T from the current x until the FP64
+// true residual meets the tolerance, `budget` steps are spent or (T = float)
+// the cycles stall; returns the steps taken.  *stalled tells the last apart.
+template <class T>
+int gmresCycles(phase_hip_solver *s, double b2, int budget, bool *stalled) {
+  constexpr bool kMixed = std::is_same<T, float>::value;
   const int n = s->rank, m = s->gmresRestart, ld = m + 1;
   hipStream_t st = s->stream;
-  ensureFactor64(s);
-  const IluDev<double> L = iluDev<double>(s, s->dFac64.p);
+  const IluDev<T> L = iluDev<T>(s, factor<T>(s));
   const int G = gridFor(n, kRedBlock, kRedCap);
   const int groups = (m + 1 + kGmGroup - 1) / kGmGroup;
-  s->gmV.alloc((size_t)(m + 1) * n);
-  s->gmZ.alloc((size_t)m * n);
+  gmBasis<T>(s).alloc((size_t)(m + 1) * n);
+  gmPrec<T>(s).alloc((size_t)m * n);
   s->gmH.alloc((size_t)ld * m);
   // small: cs (m), sn (m), g (m + 1), y (m), hcur (m + 1), vscale (1)
   s->gmSmall.alloc((size_t)5 * (m + 1) + 1);
   s->gmPart.alloc((size_t)groups * kGmGroup * G + G);
-  s->k64.alloc(n);
-  double *V = s->gmV.p, *Z = s->gmZ.p, *H = s->gmH.p;
+  krylov<T>(s).alloc(n);
+  s->dR64.alloc(n);
+  T *V = gmBasis<T>(s).p, *Z = gmPrec<T>(s).p;
+  double *H = s->gmH.p;
   double *cs = s->gmSmall.p, *sn = cs + (m + 1), *g = sn + (m + 1), *y = g + (m + 1), *hcur = y + (m + 1);
   double *vscale = hcur + (m + 1);
   double *dotPart = s->gmPart.p, *normPart = dotPart + (size_t)groups * kGmGroup * G;
   Scalars *sc = s->dSc.p;
-  hipLaunchKernelGGL(kGmSetTol, dim3(1), dim3(64), 0, st, s->tol * s->tol * b2, sc);
+  const double tol2 = s->tol * s->tol * b2;
   const int ge = gridFor(n, 256, kElemCap);
-  int total = 0;
+  int total = 0, stall = 0;
+  double bestRR = HUGE_VAL, startRR = -1., estRR = -1.;
+  *stalled = false;
   while (true) {
-    // true residual into v_0, then the cycle's start
+    // FP64 true residual; it decides convergence
     hipLaunchKernelGGL((kResidual<double>), dim3(spmvGrid(s)), dim3(kRedBlock), 0, st, n, s->ord.nSlices,
-                       s->dSliceOff.p, s->dSellCol.p, s->dSell64.p, s->dX.p, s->dB.p, V, (double *)nullptr,
-                       s->dPartials.p);
+                       s->dSliceOff.p, s->dSellCol.p, s->dSell64.p, s->dX.p, s->dB.p, s->dR64.p,
+                       (double *)nullptr, s->dPartials.p);
     hipLaunchKernelGGL(kFinishResidual, dim3(1), dim3(kFinBlock), 0, st, sc, s->dPartials.p, spmvGrid(s));
-    hipLaunchKernelGGL(kGmStart, dim3(1), dim3(64), 0, st, m, g, vscale, sc);
-    hipLaunchKernelGGL(kGmScale, dim3(ge), dim3(256), 0, st, n, V, (const double *)vscale, (const Scalars *)sc);
     pullScalars(s);
-    s->error = std::sqrt(s->hSc[0].rr / b2);
+    const double rr = s->hSc[0].rr;
+    s->error = std::sqrt(rr / b2);
     ++s->outerPasses;
-    if (s->hSc[0].done != kDoneRunning || total >= s->maxIters) break;
+    if (!(rr > tol2) || total >= budget) break;
+    if (kMixed) {
+      // FP32 trouble: no new best, or a cycle that did not halve the true
+      // residual while its own estimate claimed ten times more
+      if (startRR > 0.) {
+        const bool lost = rr > 0.25 * startRR && rr > 100. * estRR;
+        stall = lost || !(rr < bestRR) ? stall + 1 : 0;
+      }
+      if (rr < bestRR) {
+        bestRR = rr;
+        PH_CHECK(hipMemcpyAsync(s->dXBest.p, s->dX.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+      }
+      if (stall >= kGmMixedStall) {
+        *stalled = true;
+        break;
+      }
+      startRR = rr;
+    }
+    // the cycle's target: the tolerance, or (mixed) the FP32 cycle's drop
+    hipLaunchKernelGGL(kGmSetTol, dim3(1), dim3(64), 0, st,
+                       kMixed ? std::max(tol2, kGmMixedDrop * kGmMixedDrop * rr) : tol2, sc);
+    hipLaunchKernelGGL(kGmStart, dim3(1), dim3(64), 0, st, m, g, vscale, sc);
+    hipLaunchKernelGGL((kGmFirst<T>), dim3(ge), dim3(256), 0, st, n, (const double *)s->dR64.p,
+                       (const double *)vscale, V);
     int j = 0, k = 0;
-    while (j < m && total + j < s->maxIters) {
-      const int steps = std::min({s->chunk, m - j, s->maxIters - total - j});
+    while (j < m && total + j < budget) {
+      const int steps = std::min({s->chunk, m - j, budget - total - j});
       for (int q = 0; q < steps; ++q, ++j) {
-        double *vj = V + (size_t)j * n, *zj = Z + (size_t)j * n, *w = V + (size_t)(j + 1) * n;
-        launchLU<double, kProNone>(s, L, sc, vj, s->k64.y.p, (double *)nullptr, (const double *)nullptr,
-                                   (const double *)nullptr, zj);
-        if (s->precond == 1) mlFinishApply<double>(s, sc, vj, zj);
-        hipLaunchKernelGGL((kSpmv<double>), dim3(spmvGrid(s)), dim3(kRedBlock), 0, st, n, s->ord.nSlices,
-                           s->dSliceOff.p, s->dSellCol.p, s->dSell64.p, (const double *)zj, w);
+        T *vj = V + (size_t)j * n, *zj = Z + (size_t)j * n, *w = V + (size_t)(j + 1) * n;
+        launchLU<T, kProNone>(s, L, sc, vj, krylov<T>(s).y.p, (T *)nullptr, (const T *)nullptr,
+                              (const T *)nullptr, zj);
+        if (s->precond == 1) mlFinishApply<T>(s, sc, vj, zj);
+        hipLaunchKernelGGL((kSpmv<T, double>), dim3(spmvGrid(s)), dim3(kRedBlock), 0, st, n, s->ord.nSlices,
+                           s->dSliceOff.p, s->dSellCol.p, s->dSell64.p, (const T *)zj, w);
         for (int pass = 0; pass < 2; ++pass) {
-          hipLaunchKernelGGL(kGmDots, dim3(G, (j + 1 + kGmGroup - 1) / kGmGroup), dim3(kRedBlock), 0, st, n,
-                             j + 1, (const double *)V, (const double *)w, dotPart, (const Scalars *)sc);
+          hipLaunchKernelGGL((kGmDots<T>), dim3(G, (j + 1 + kGmGroup - 1) / kGmGroup), dim3(kRedBlock), 0, st, n,
+                             j + 1, (const T *)V, (const T *)w, dotPart, (const Scalars *)sc);
           hipLaunchKernelGGL(kGmDotsFinish, dim3(1), dim3(kFinBlock), 0, st, j + 1, G, (const double *)dotPart,
                              hcur, H + (size_t)j * ld, pass, (const Scalars *)sc);
           if (pass == 0)
-            hipLaunchKernelGGL((kGmAxpy<false>), dim3(G), dim3(kRedBlock), 0, st, n, j + 1, (const double *)V,
+            hipLaunchKernelGGL((kGmAxpy<T, false>), dim3(G), dim3(kRedBlock), 0, st, n, j + 1, (const T *)V,
                                (const double *)hcur, w, normPart, (const Scalars *)sc);
           else
-            hipLaunchKernelGGL((kGmAxpy<true>), dim3(G), dim3(kRedBlock), 0, st, n, j + 1, (const double *)V,
+            hipLaunchKernelGGL((kGmAxpy<T, true>), dim3(G), dim3(kRedBlock), 0, st, n, j + 1, (const T *)V,
                                (const double *)hcur, w, normPart, (const Scalars *)sc);
         }
         hipLaunchKernelGGL(kGmGivens, dim3(1), dim3(kFinBlock), 0, st, j, ld, (const double *)normPart, G, H, cs,
                            sn, g, vscale, sc);
-        hipLaunchKernelGGL(kGmScale, dim3(ge), dim3(256), 0, st, n, w, (const double *)vscale,
+        hipLaunchKernelGGL((kGmScale<T>), dim3(ge), dim3(256), 0, st, n, w, (const double *)vscale,
                            (const Scalars *)sc);
       }
       PH_CHECK(hipGetLastError());
       pullScalars(s);
       k = s->hSc[0].iters;
+      estRR = s->hSc[0].rr;
       if (s->hSc[0].done != kDoneRunning) break;
     }
-    if (s->hSc[0].done == kDoneBreakdown && k == 0) break;
+    if (s->hSc[0].done == kDoneBreakdown) {
+      // the column of the step that broke down is not finite: keep the steps
+      // before it; with none (FP32: a factor or vector out of FP32 range) the
+      // solve ends here, or continues in FP64
+      k -= 1;
+      if (k <= 0) {
+        *stalled = kMixed;
+        break;
+      }
+    }
     // x += Z y over the k steps of this cycle, then restart from the true residual
     hipLaunchKernelGGL(kGmSolveY, dim3(1), dim3(64), 0, st, k, ld, (const double *)H, (const double *)g, y);
-    hipLaunchKernelGGL(kGmUpdateX, dim3(G), dim3(kRedBlock), 0, st, n, k, (const double *)Z, (const double *)y,
+    hipLaunchKernelGGL((kGmUpdateX<T>), dim3(G), dim3(kRedBlock), 0, st, n, k, (const T *)Z, (const double *)y,
                        s->dX.p);
     PH_CHECK(hipGetLastError());
-    // the next pass's true residual decides convergence (kGmStart resets the flag)
     total += k;
+  }
+  if (*stalled && bestRR < HUGE_VAL) {
+    PH_CHECK(hipMemcpyAsync(s->dX.p, s->dXBest.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    s->error = std::sqrt(bestRR / b2);
+  }
+  return total;
+}
+
+}  // namespace
+
+void solveGmres(phase_hip_solver *s, double b2) {
+  int total = 0;
+  bool stalled = false;
+  if (s->mode != PHASE_HIP_MODE_FP64) {
+    ensureSingle(s);
+    s->dXBest.alloc(s->rank);
+    total = gmresCycles<float>(s, b2, s->maxIters, &stalled);
+    if (stalled) ++s->fallbacks;
+  }
+  if (s->mode == PHASE_HIP_MODE_FP64 || stalled) {
+    ensureFactor64(s);
+    total += gmresCycles<double>(s, b2, s->maxIters - total, &stalled);
   }
   checkDeviceErr(s);
   s->innerIters = total;

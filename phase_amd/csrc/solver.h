@@ -44,9 +44,10 @@ struct phase_hip_solver {
   int chunk = 8;  // iterations enqueued between host checks
   int geometric = 1; // cell centroids known: brick aggregates in three colours (else graph aggregates)
   int cg = 0;        // Krylov method: 0 = BiCGStab, 1 = preconditioned CG (symmetric systems)
-  int gmres = 0;     // 1: flexible restarted GMRES (gmres.hip), FP64
+  int gmres = 0;     // 1: flexible restarted GMRES (gmres.hip)
   int gmresRestart = 30;
   phase::DBuf<double> gmV, gmZ, gmH, gmSmall, gmPart;  // GMRES basis, preconditioned basis, Hessenberg, scalars
+  phase::DBuf<float> gmV32, gmZ32;                    // the same two in mode mixed
   int unionPattern = 0;  // keep the stored pattern when a new one lies inside it (its other entries 0)
   int blockJacobi = 0;   // ILU(0) of the block-diagonal part only, one colour (coarse multigrid levels)
   int iluLimits = 1;     // 0: the ordering's ILU(0) is never applied (Jacobi multigrid levels), any row width
@@ -135,6 +136,7 @@ void solverEnsureSingle(phase_hip_solver *s);
 // the forward-sweep result).  Instantiated for float and double.
 template <class T>
 void iluApplyDev(phase_hip_solver *s, T *fac, const T *in, T *tmp, T *out);
-// Flexible restarted GMRES in FP64 from the current x (gmres.hip); b2 = ||b||^2.
+// Flexible restarted GMRES from the current x (gmres.hip), FP64 or FP32 cycles
+// restarted from the FP64 residual; b2 = ||b||^2.
 void solveGmres(phase_hip_solver *s, double b2);
 }  // namespace phase

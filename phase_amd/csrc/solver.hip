@@ -934,7 +934,6 @@ int phase_hip_solver_solve(phase_hip_solver *s, double *relres) {
     s->fallbacks = 0;
     s->learnedSteps = 0;
     if (s->gmres) {
-      // FP64 whatever the mode
       const double b2 = normB2(s);
       s->innerIters = s->nIters = s->outerPasses = 0;
       if (b2 == 0.) {

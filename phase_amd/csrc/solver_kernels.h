@@ -245,10 +245,10 @@ __device__ __forceinline__ T sellRow(int slice, int lane, const int *__restrict_
   return (T)acc;
 }
 
-// y = A x
-template <class T>
+// y = A x (matrix values in TV; rows accumulate in the wider of T and TV)
+template <class T, class TV = T>
 __global__ __launch_bounds__(kRedBlock) void kSpmv(int n, int nSlices, const int *__restrict__ sliceOff,
-                                                   const int *__restrict__ col, const T *__restrict__ val,
+                                                   const int *__restrict__ col, const TV *__restrict__ val,
                                                    const T *__restrict__ x, T *__restrict__ y) {
   const int tile = xcdTile(blockIdx.x, gridDim.x);
   const int slice = tile * (kRedBlock / 64) + (threadIdx.x >> 6);

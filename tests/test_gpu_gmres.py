@@ -5,7 +5,9 @@ between applications: the flexible form is what makes that valid), restart
 lengths from 5 to 64 (30 and 64 with ILU(0) alone), the FP64 true relative residual reaches the goal and the
 solution agrees with a direct solve (scipy splu) within cond(A) * 1.5e-8; the
 restarted form keeps the residual it reports; repeated solves are
-bit-identical; bad restart lengths are rejected."""
+bit-identical; bad restart lengths are rejected.  Mode mixed (FP32 cycles
+restarted from the FP64 residual) reaches the same FP64 goal, and continues
+in FP64 when its FP32 cycles cannot run."""
 import numpy as np
 import pytest
 import scipy.sparse.linalg as spla
@@ -36,16 +38,30 @@ def _solve(hip, rp, ci, v, b, **kw):
 
 # GMRES(5) with ILU(0) alone stagnates on this system (so does a textbook
 # FGMRES(5) with scipy's ILU(0): 3.9e-2 after 20,000 iterations)
-@pytest.mark.parametrize("prec,restart", [("ilu", 30), ("ilu", 64), ("multigrid", 5), ("multigrid", 30),
-                                          ("multigrid", 64)])
-def test_gmres_matches_direct(hip, sys1, prec, restart):
+@pytest.mark.parametrize("prec,restart,mode", [("ilu", 30, "fp64"), ("ilu", 64, "fp64"), ("multigrid", 5, "fp64"),
+                                               ("multigrid", 30, "fp64"), ("multigrid", 64, "fp64"),
+                                               ("ilu", 30, "mixed"), ("multigrid", 5, "mixed"),
+                                               ("multigrid", 30, "mixed")])
+def test_gmres_matches_direct(hip, sys1, prec, restart, mode):
     rp, ci, v, b, A, kappa, x_ref = sys1
-    s, rel, x = _solve(hip, rp, ci, v, b, preconditioner=prec, gmresRestart=restart)
+    s, rel, x = _solve(hip, rp, ci, v, b, preconditioner=prec, gmresRestart=restart, mode=mode)
+    assert s.stats()["fallbacks"] == 0
     assert rel <= 1e-8
     true = np.linalg.norm(b - A @ x) / np.linalg.norm(b)
     assert true <= 1.5e-8 and abs(true - rel) <= 1e-3 * rel + 1e-14
     assert np.linalg.norm(x - x_ref) <= kappa * 1.5e-8 * np.linalg.norm(x_ref)
-    print(prec, restart, "iterations", s.n_iters(), flush=True)
+    print(prec, restart, mode, "iterations", s.n_iters(), flush=True)
+
+
+def test_mixed_gmres_continues_in_fp64(hip, sys1):
+    """A system scaled by 1e40 is the same system in FP64, but its FP32
+    factor overflows: the FP32 cycles break down at once and the solve
+    continues in FP64 to the same goal, counting one fallback."""
+    rp, ci, v, b, A, kappa, x_ref = sys1
+    for prec in ("ilu", "multigrid"):
+        s, rel, x = _solve(hip, rp, ci, v * 1e40, b * 1e40, preconditioner=prec, mode="mixed")
+        assert rel <= 1e-8 and s.stats()["fallbacks"] == 1
+        assert np.linalg.norm(b - A @ x) <= 1.5e-8 * np.linalg.norm(b)
 
 
 def test_gmres_iteration_counts(hip, sys1):
@@ -61,9 +77,10 @@ def test_gmres_iteration_counts(hip, sys1):
 
 def test_gmres_repeatable_and_rejects_bad_restart(hip, sys1):
     rp, ci, v, b, A, kappa, x_ref = sys1
-    s1, _, x1 = _solve(hip, rp, ci, v, b, preconditioner="multigrid")
-    s2, _, x2 = _solve(hip, rp, ci, v, b, preconditioner="multigrid")
-    assert np.array_equal(x1, x2) and s1.n_iters() == s2.n_iters()
+    for mode in ("fp64", "mixed"):
+        s1, _, x1 = _solve(hip, rp, ci, v, b, preconditioner="multigrid", mode=mode)
+        s2, _, x2 = _solve(hip, rp, ci, v, b, preconditioner="multigrid", mode=mode)
+        assert np.array_equal(x1, x2) and s1.n_iters() == s2.n_iters()
     for bad in (0, 65):
         with pytest.raises(hip.PhaseHipError):
             hip.HipSparseMatrixSolver(0, solver="gmres", gmresRestart=bad)
