@@ -481,7 +481,7 @@ def ib_case(args):
     h = np.sqrt(30.0 * 20.0 / mesh.n_cells)  # the ghost cells' size (uniform quads)
     out = dict(workload="cylinder-ghost-cell-pressure-30x20-quads", cells=mesh.n_cells,
                ghost_cells=int((types == 1).sum()), solid_cells=int((types == 2).sum()))
-    for mode in ("fp64", "mixed", "fp64-mg", "mixed-mg", "fp64-gmres-mg", "mixed-gmres-mg"):
+    for mode in args.ib_modes.split(","):
         s = P.HipSparseMatrixSolver(0, tolerance=args.tol, maxIters=100000, **solver_kw(mode))
 
         def step():
@@ -546,6 +546,8 @@ def ib_flow_case(args):
         d.setup(k + ".mode", args.ib_flow_mode)
     d.setup("p.preconditioner", args.ib_flow_precond)
     d.setup("p.solver", args.ib_flow_psolver)
+    if args.ib_flow_pmode:
+        d.setup("p.mode", args.ib_flow_pmode)
     d.setup("numPressureCorrections", args.ib_flow_correctors)
     setup_s = time.perf_counter() - t0
     d.advance(2e-3, 2)
@@ -556,7 +558,7 @@ def ib_flow_case(args):
     u, p, _ = d.fields()
     force = ib.hydrodynamic_force(1.0, 0.005, P.DeviceArray(u.reshape(-1)), P.DeviceArray(p))[0]
     out = dict(workload="cylinder-ghost-cell-flow-step-30x20-quads", cells=mesh.n_cells, ms_per_step=round(ms, 2),
-               steps=args.ib_flow_steps, setup_s=round(setup_s, 2), mode=args.ib_flow_mode,
+               steps=args.ib_flow_steps, setup_s=round(setup_s, 2), mode=args.ib_flow_mode, pressure_mode=args.ib_flow_pmode or args.ib_flow_mode,
                pressure_preconditioner=args.ib_flow_precond, pressure_solver=args.ib_flow_psolver, pressure_correctors=args.ib_flow_correctors,
                iters_u_p=[float(x) for x in hist.mean(0)[:2]], max_div=float("%.3g" % hist[:, 2].max()),
                force=[float("%.5g" % x) for x in force])
@@ -647,6 +649,9 @@ def main():
     ap.add_argument("--multiphase-precond", default="multigrid", help="pressure preconditioner (ilu or multigrid)")
     ap.add_argument("--ib-flow-precond", default="multigrid", help="pressure preconditioner (ilu or multigrid)")
     ap.add_argument("--ib-flow-psolver", default="gmres", help="pressure Krylov method (bicgstab or gmres)")
+    ap.add_argument("--ib-flow-pmode", default="mixed", help="pressure precision (fp64 or mixed)")
+    ap.add_argument("--ib-modes", default="fp64,mixed,fp64-mg,mixed-mg,fp64-gmres-mg,mixed-gmres-mg",
+                    help="methods of the ghost-cell pressure case")
     ap.add_argument("--multiphase-correctors", type=int, default=1,
                     help="pressure-corrector passes per multiphase step (two diverge at this dt and mesh: the "
                          "surface-tension time-step limit is below dt)")
