@@ -110,6 +110,9 @@ def clean(v):
     return v
 
 
+FALLBACK_RATIO = 0.5  # set from --fallback-ratio
+
+
 def solver_kw(mode):
     """Solver set-up of a method name "<precision>[-cg|-gmres][-mg]": precision
     fp64, mixed or learned; -cg preconditioned CG (Belos "CG") and -gmres
@@ -117,7 +120,10 @@ def solver_kw(mode):
     multigrid preconditioner instead of ILU(0) alone."""
     parts = mode.split("-")
     solver = "cg" if "cg" in parts[1:] else "gmres" if "gmres" in parts[1:] else "bicgstab"
-    return dict(mode=parts[0], solver=solver, preconditioner="multigrid" if "mg" in parts[1:] else "ilu")
+    kw = dict(mode=parts[0], solver=solver, preconditioner="multigrid" if "mg" in parts[1:] else "ilu")
+    if parts[0] == "learned":
+        kw["fallbackRatio"] = FALLBACK_RATIO
+    return kw
 
 
 def build_problem(n_cells, k, sigma):
@@ -183,6 +189,7 @@ def train_network(s, mesh, d_g, d_bc, d_pf, args, kw, max_pairs=256, capture_sol
     s.setup("mode", "fp64")
     s.setup("solver", "bicgstab")
     s.setup("preconditioner", "ilu")
+    s.setup("learnedObjective", args.learned_objective)
     for t in range(nsolves):
         s.assemble_laplacian_device(mesh, 3, gamma_face=d_g, bc_type=d_bc, phi_face=d_pf)
         if t == 0:
@@ -200,7 +207,8 @@ def train_network(s, mesh, d_g, d_bc, d_pf, args, kw, max_pairs=256, capture_sol
         s.setup(k, v)
     log("learned: %d pairs from %d FP64 solves (%.1fs), %d Adam steps (%.1fs), loss %.3f -> %.3f"
         % (pairs, nsolves, t1 - t0, args.train_steps + 50, t2 - t1, first, loss))
-    return dict(train_pairs=pairs, train_steps=args.train_steps + 50, capture_s=round(t1 - t0, 2),
+    return dict(train_pairs=pairs, train_steps=args.train_steps + 50, objective=args.learned_objective,
+                capture_s=round(t1 - t0, 2),
                 train_s=round(t2 - t1, 2), loss_first=round(first, 4), loss_last=round(loss, 4))
 
 
@@ -666,7 +674,13 @@ def main():
     ap.add_argument("--spmv-cells", type=int, default=16_000_000,
                     help="SpMV bandwidth point beyond the last-level cache at this many cells (0 skips)")
     ap.add_argument("--c4-pairs", type=int, default=2000, help="training pairs for the learned case")
+    ap.add_argument("--fallback-ratio", type=float, default=0.5,
+                    help="learned modes: the residual ratio a network step must reach to be kept")
+    ap.add_argument("--learned-objective", default="residual",
+                    help="the network's training loss: residual (what the solver's step keeps) or error")
     args = ap.parse_args()
+    global FALLBACK_RATIO
+    FALLBACK_RATIO = args.fallback_ratio
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported (the path would shard by "
               "cell partition with one halo exchange per SpMV)" % args.gpus, file=sys.stderr)

@@ -109,7 +109,10 @@ int phase_hip_solver_destroy(phase_hip_solver *s);
  * default 32; aggregates of one colour are packed into blocks of up to 64
  * rows), "ordering" (geometric | graph aggregates), "pattern" (exact | union),
  * "fallbackRatio" (learned: minimum per-iteration residual reduction before
- * falling back, 0.5),
+ * falling back, 0.5), "learnedObjective" (the network's training loss:
+ * RESIDUAL, the default, the part of the pair's residual that the
+ * minimal-residual step of the output leaves; ERROR, the direction of the
+ * captured correction),
  * "preconditioner" (ILU = Ifpack2 RILUK(0) as the reference's Belos backend,
  * the default; MULTIGRID = aggregation multigrid, ILU(0) smoothing on the
  * finest level and damped Jacobi on the coarse ones, the role of the

@@ -22,6 +22,11 @@ Loss on a captured pair (r, e), with t_i = e_i sd_i / s:
 i.e. one minus the squared cosine between output and target.  The solver
 applies the correction through a minimal-residual multiple alpha * e, so only
 its direction matters; the loss is invariant to the output's scale and sign.
+Residual objective (setup "learnedObjective" RESIDUAL): with the correction's
+direction c_i = out_i / sd_i and q = A c,
+    loss = 1 - (q . r)^2 / (|q|^2 |r|^2)
+the part of ||r||^2 that the minimal-residual step alpha c leaves; for a
+symmetric A, dloss/dout_i = (A dloss/dq)_i / sd_i.
 """
 import numpy as np
 
@@ -86,14 +91,26 @@ def forward(Ah, X, p):
     return out, cache
 
 
-def loss_and_grad(Ah, X, p, target):
+def loss_and_grad(Ah, X, p, target, residual=None):
+    """Error objective against `target`; with residual = (A, sd, r) the
+    residual objective instead (A symmetric)."""
     out, c = forward(Ah, X, p)
     N = len(out)
-    a = out @ target
-    pp = out @ out
-    q = target @ target
-    loss = 1.0 - a * a / (pp * q)
-    dout = (2.0 * a / (pp * q)) * (a * out / pp - target)
+    if residual is None:
+        a = out @ target
+        pp = out @ out
+        q = target @ target
+        loss = 1.0 - a * a / (pp * q)
+        dout = (2.0 * a / (pp * q)) * (a * out / pp - target)
+    else:
+        A, sd, r = residual
+        qv = A @ (out / sd)
+        a = qv @ r
+        pp = qv @ qv
+        rr = r @ r
+        loss = 1.0 - a * a / (pp * rr)
+        dq = (2.0 * a / (pp * rr)) * (a * qv / pp - r)
+        dout = (A.T @ dq) / sd
     g = {}
     hL = c["hs"][-1]
     g["w_out"] = hL.T @ dout

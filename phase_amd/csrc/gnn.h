@@ -44,6 +44,7 @@ struct GnnState {
   DBuf<__bf16> wT;     // per layer [64 out][128 k] (forward)
   DBuf<__bf16> wN;     // per layer [128 k][64 out] (backward data)
   DBuf<float> sd, rowsum, X, out, dOut;
+  DBuf<float> resC, resQ, resDq, resG;  // residual objective: out / sd, A c, dloss/dq, A dloss/dq
   DBuf<__bf16> h;      // (L+1) x n x 64
   DBuf<__bf16> agg, pre;  // L x n x 64
   DBuf<__bf16> pre0, dpre;

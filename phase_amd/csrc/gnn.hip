@@ -10,7 +10,11 @@
 //   h0   = relu(X W_in + b_in)
 //   h_l  = h_{l-1} + relu([h_{l-1}, A_hat h_{l-1}] Wsm + mean(h_{l-1}) Wg + b)
 //   out  = h_L w_out + b_out,   correction e_i = out_i s / sd_i
-//   loss = mean_i (out_i - e*_i sd_i / s)^2 for a captured pair (r, e*).
+// Loss on a captured pair (r, e*), with t_i = e*_i sd_i / s:
+//   error objective (default):  1 - (out . t)^2 / (|out|^2 |t|^2)
+//   residual objective:         1 - (q . r)^2 / (|q|^2 |r|^2),  q = A (out / sd),
+//     the part of ||r||^2 the minimal-residual step leaves (A symmetric:
+//     dloss/dout = (A dloss/dq) / sd).
 //
 // Dense layers run on BF16 MFMA (v_mfma_f32_16x16x32_bf16, FP32 accumulate):
 // activations are stored as BF16 rows of 64 features, weights as BF16 copies
@@ -296,6 +300,31 @@ __global__ void kGnnDout(int n, const float *__restrict__ out, const float *__re
   for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < (long)n * H; k += (long)gridDim.x * blockDim.x) {
     const int i = (int)(k / H), f = (int)(k % H);
     const float d = c1 * (c2 * out[i] - t[i]);
+    if (f == 0) dout[i] = d;
+    dh[k] = d * wo[f];
+  }
+}
+
+// residual objective: c = out / sd
+__global__ void kGnnResC(int n, const float *__restrict__ out, const float *__restrict__ sd, float *__restrict__ c) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) c[i] = out[i] / sd[i];
+}
+
+// residual objective: dq_i = 2a/(p rr) (a q_i / p - r_i), a = q.r, p = |q|^2, rr = |r|^2
+__global__ void kGnnResDq(int n, const float *__restrict__ q, const float *__restrict__ r,
+                          const double *__restrict__ scal, float *__restrict__ dq) {
+  const double a = scal[2], p = scal[3], rr = scal[4];
+  const bool ok = p > 0. && rr > 0.;
+  const float c1 = ok ? (float)(2.0 * a / (p * rr)) : 0.f, c2 = ok ? (float)(a / p) : 0.f;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dq[i] = c1 * (c2 * q[i] - r[i]);
+}
+
+// residual objective: dout_i = g_i / sd_i (g = A dq) ; dh_L = dout w_out^T
+__global__ void kGnnResDout(int n, const float *__restrict__ g, const float *__restrict__ sd,
+                            const float *__restrict__ wo, float *__restrict__ dout, float *__restrict__ dh) {
+  for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < (long)n * H; k += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(k / H), f = (int)(k % H);
+    const float d = g[i] / sd[i];
     if (f == 0) dout[i] = d;
     dh[k] = d * wo[f];
   }
@@ -651,14 +680,31 @@ void gnnForward(phase_hip_solver *s) {
   PH_CHECK(hipGetLastError());
 }
 
-// Cosine loss of the last forward against target t; dout and dh_L for backward.
-void gnnLoss(phase_hip_solver *s, const float *target) {
+// Loss of the last forward: against target t (error objective) or on the
+// residual r of the pair (residual objective, with the FP32 copy of A);
+// dout and dh_L for backward.
+void gnnLoss(phase_hip_solver *s, const float *target, const float *r) {
   GnnState &G = *s->gnn;
   const int n = s->rank;
-  hipLaunchKernelGGL(kGnnLossCos, dim3(gridFor1(n)), dim3(kRedBlock), 0, s->stream, n, G.out.p, target, G.scal.p,
-                     s->dPartials.p, s->dCounter.p);
-  hipLaunchKernelGGL(kGnnDout, dim3(g1((long)n * H)), dim3(256), 0, s->stream, n, G.out.p, target,
-                     G.params.p + G.lay.outW(), G.scal.p, G.dOut.p, G.dh.p);
+  if (s->learnedObjective == 1) {
+    for (DBuf<float> *b : {&G.resC, &G.resQ, &G.resDq, &G.resG}) b->alloc(n);
+    const int gs = std::max(1, (s->ord.nSlices + 3) / 4);
+    hipLaunchKernelGGL(kGnnResC, dim3(g1(n)), dim3(256), 0, s->stream, n, G.out.p, G.sd.p, G.resC.p);
+    hipLaunchKernelGGL((kSpmv<float>), dim3(gs), dim3(kRedBlock), 0, s->stream, n, s->ord.nSlices, s->dSliceOff.p,
+                       s->dSellCol.p, s->dSell32.p, (const float *)G.resC.p, G.resQ.p);
+    hipLaunchKernelGGL(kGnnLossCos, dim3(gridFor1(n)), dim3(kRedBlock), 0, s->stream, n, G.resQ.p, r, G.scal.p,
+                       s->dPartials.p, s->dCounter.p);
+    hipLaunchKernelGGL(kGnnResDq, dim3(g1(n)), dim3(256), 0, s->stream, n, G.resQ.p, r, G.scal.p, G.resDq.p);
+    hipLaunchKernelGGL((kSpmv<float>), dim3(gs), dim3(kRedBlock), 0, s->stream, n, s->ord.nSlices, s->dSliceOff.p,
+                       s->dSellCol.p, s->dSell32.p, (const float *)G.resDq.p, G.resG.p);
+    hipLaunchKernelGGL(kGnnResDout, dim3(g1((long)n * H)), dim3(256), 0, s->stream, n, G.resG.p, G.sd.p,
+                       G.params.p + G.lay.outW(), G.dOut.p, G.dh.p);
+  } else {
+    hipLaunchKernelGGL(kGnnLossCos, dim3(gridFor1(n)), dim3(kRedBlock), 0, s->stream, n, G.out.p, target, G.scal.p,
+                       s->dPartials.p, s->dCounter.p);
+    hipLaunchKernelGGL(kGnnDout, dim3(g1((long)n * H)), dim3(256), 0, s->stream, n, G.out.p, target,
+                       G.params.p + G.lay.outW(), G.scal.p, G.dOut.p, G.dh.p);
+  }
   PH_CHECK(hipGetLastError());
 }
 
@@ -711,7 +757,7 @@ void gnnTrainStep(phase_hip_solver *s, int pair, float lr) {
   gnnFeatures(s, r, K.sh.p);
   hipLaunchKernelGGL(kGnnTarget, dim3(g1(n)), dim3(256), 0, s->stream, n, e, G.sd.p, G.scal.p, K.s.p);
   gnnForward(s);
-  gnnLoss(s, K.s.p);
+  gnnLoss(s, K.s.p, r);
   gnnBackward(s);
   ++G.step;
   const float b1 = 0.9f, b2 = 0.999f;
@@ -974,7 +1020,7 @@ int phase_hip_learned_eval(phase_hip_solver *s, const double *r, const double *z
                        K.s.p, 1.0);
     gnnFeatures(s, K.t.p, K.sh.p);
     gnnForward(s);
-    gnnLoss(s, K.s.p);
+    gnnLoss(s, K.s.p, K.t.p);
     gnnBackward(s);
     hipLaunchKernelGGL((kConvert<float, double>), dim3(g1(n)), dim3(256), 0, s->stream, n, G.out.p, tmp.p);
     hipLaunchKernelGGL((kPermuteOut<double>), dim3(g1(n)), dim3(256), 0, s->stream, n, s->dPerm.p, tmp.p, tmp.p + n);

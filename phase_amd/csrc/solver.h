@@ -105,6 +105,7 @@ struct phase_hip_solver {
   // learned correction (gnn.hip) and its FP64 work vectors
   phase::GnnState *gnn = nullptr;
   phase::DBuf<double> dE64, dQ64;
+  int learnedObjective = 1;  // training loss: 0 the error's direction, 1 the residual the step leaves
   int lastCaptureIter = 0;
 
   // pattern of the last equation handed over by phase_hip_solver_set_equation

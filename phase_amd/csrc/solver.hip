@@ -798,6 +798,10 @@ int phase_hip_solver_setup(phase_hip_solver *s, const char *key, const char *val
       if (pv < 0) fail(PHASE_HIP_EINVAL, "multigridDensePivoting is AUTO or ALWAYS");
       if (pv != s->mlDensePivot) mlValuesChanged(s);
       s->mlDensePivot = pv;
+    } else if (k == "learnedObjective") {
+      const int o = lv == "error" ? 0 : lv == "residual" ? 1 : -1;
+      if (o < 0) fail(PHASE_HIP_EINVAL, "learnedObjective is ERROR or RESIDUAL");
+      s->learnedObjective = o;
     } else if (k == "multigridFinePostSmoothing") {
       s->mlFinePost = (lv == "1" || lv == "true" || lv == "yes") ? 1 : 0;
     } else if (k == "multigridCoarseRows") {

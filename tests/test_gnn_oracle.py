@@ -2,6 +2,7 @@
 mathematics: a hand-computed two-node forward pass, and central finite
 differences of the loss for every parameter block."""
 import numpy as np
+import pytest
 import scipy.sparse as sp
 
 from oracle import gnn_oracle as N
@@ -30,7 +31,8 @@ def test_forward_two_nodes_by_hand():
     assert np.allclose(out, [2.0 - 3.0 + 0.25, 0.0 - 6.0 + 0.25])
 
 
-def test_backward_matches_finite_differences():
+@pytest.mark.parametrize("objective", ["error", "residual"])
+def test_backward_matches_finite_differences(objective):
     rng = np.random.default_rng(3)
     n = 40
     M = sp.random(n, n, density=0.1, random_state=4)
@@ -45,7 +47,8 @@ def test_backward_matches_finite_differences():
         if k.startswith("b"):
             p[k] = rng.standard_normal(p[k].shape) * 0.1
     target = rng.standard_normal(n)
-    loss, g, _ = N.loss_and_grad(Ah, X, p, target)
+    res = (A.tocsr(), sd, r) if objective == "residual" else None
+    loss, g, _ = N.loss_and_grad(Ah, X, p, target, res)
     eps = 1e-6
     for k in p:
         flat = p[k].reshape(-1)
@@ -53,9 +56,9 @@ def test_backward_matches_finite_differences():
         for i in idx:
             old = flat[i]
             flat[i] = old + eps
-            lp = N.loss_and_grad(Ah, X, p, target)[0]
+            lp = N.loss_and_grad(Ah, X, p, target, res)[0]
             flat[i] = old - eps
-            lm = N.loss_and_grad(Ah, X, p, target)[0]
+            lm = N.loss_and_grad(Ah, X, p, target, res)[0]
             flat[i] = old
             fd = (lp - lm) / (2 * eps)
             an = g[k].reshape(-1)[i]

@@ -59,10 +59,11 @@ def rel(a, b):
     return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
 
 
-def test_forward_backward_match_fp64_oracle(hip, system):
+@pytest.mark.parametrize("objective", ["error", "residual"])
+def test_forward_backward_match_fp64_oracle(hip, system, objective):
     rp, ci, v, b = system
     A = H.compact_sorted(rp, ci, v)
-    s = hip.HipSparseMatrixSolver(0)
+    s = hip.HipSparseMatrixSolver(0, learnedObjective=objective)
     s.set(rp, ci, v)
     rng = np.random.default_rng(0)
     p = NO.init_params(hidden=64, layers=6, seed=3)
@@ -78,7 +79,8 @@ def test_forward_backward_match_fp64_oracle(hip, system):
     Ah, sd, rowsum = NO.scaled_graph(A)
     X, sc = NO.features(r, z, sd, rowsum)
     # the target is given in network units (out space)
-    loss_ref, g_ref, out_ref = NO.loss_and_grad(Ah, X, p, target)
+    res = (A, sd, r) if objective == "residual" else None
+    loss_ref, g_ref, out_ref = NO.loss_and_grad(Ah, X, p, target, res)
     out, loss, grad = s.learned_eval(r, z, target)
     assert rel(out, out_ref) < 3e-2
     assert abs(loss - loss_ref) <= 5e-2 * loss_ref
@@ -187,7 +189,8 @@ def test_pair_checkpoint_round_trip(hip, system):
         s.learned_pairs(npairs - 1, 2)              # beyond the captured pairs
 
 
-def test_what_training_gains_on_held_out_pairs(hip, system):
+@pytest.mark.parametrize("objective", ["error", "residual"])
+def test_what_training_gains_on_held_out_pairs(hip, system, objective):
     """What the trained network gains over the method without training, on
     (residual, correction) pairs of right-hand sides it never saw: its
     correction lines up far better with the true error (cos^2, the training
@@ -197,13 +200,15 @@ def test_what_training_gains_on_held_out_pairs(hip, system):
     stays near 1: the error is dominated by smooth modes, which leave little
     trace in the residual, so a correction aligned with the error removes
     almost none of the residual, and the solver falls back to ILU(0)
-    refinement."""
+    refinement.  Trained with the residual objective instead, the network
+    is asked for exactly that residual, and the held-out residual it leaves
+    falls."""
     rp, ci, v, b = system
     n = len(b)
     A = H.compact_sorted(rp, ci, v)
 
     def pairs(seed, solves, every):
-        s = hip.HipSparseMatrixSolver(0, tolerance=1e-8, maxIters=20000, mode="fp64")
+        s = hip.HipSparseMatrixSolver(0, tolerance=1e-8, maxIters=20000, mode="fp64", learnedObjective=objective)
         s.set(rp, ci, v)
         s.learned_capture(every, 400)
         rng = np.random.default_rng(seed)
@@ -232,7 +237,21 @@ def test_what_training_gains_on_held_out_pairs(hip, system):
     before = scores(train)
     train.learned_train(600, lr=1e-3, seed=1)
     after = scores(train)
-    print("held-out pairs %d: cos^2 %.3f -> %.3f, residual left by the step %.3f -> %.3f (fallback ratio 0.5)"
-          % (len(r_h), before[0], after[0], before[1], after[1]), flush=True)
-    assert after[0] > before[0] + 0.2
-    assert 0.0 <= after[1] <= 1.0
+    print("%s objective, held-out pairs %d: cos^2 %.3f -> %.3f, residual left by the step %.3f -> %.3f "
+          "(fallback ratio 0.5)" % (objective, len(r_h), before[0], after[0], before[1], after[1]), flush=True)
+    if objective == "error":
+        assert after[0] > before[0] + 0.2
+        assert 0.0 <= after[1] <= 1.0
+    else:
+        assert after[1] < before[1]
+    # the trained network inside a solve: learned steps kept, iterations against mixed
+    it = {}
+    for mode in ("mixed", "learned"):
+        train.setup("mode", mode)
+        bb = np.random.default_rng(123).standard_normal(n)
+        train.set_rhs(bb - bb.mean())
+        train.set_guess(np.zeros(n))
+        assert train.solve() <= 1e-8
+        it[mode] = (train.n_iters(), train.stats()["fallbacks"])
+    print("%s objective: iterations (mixed, learned) %s, learned fallbacks %d"
+          % (objective, (it["mixed"][0], it["learned"][0]), it["learned"][1]), flush=True)
