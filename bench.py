@@ -433,14 +433,14 @@ def multiphase_case(args):
     d.setup("numPressureCorrections", args.multiphase_correctors)
     d.surface_tension(24.5, 2.1 * h, "pow8")
     setup_s = time.perf_counter() - t0
-    dt = 1e-4
+    dt = args.multiphase_dt
     d.advance(dt, 2)
     t0 = time.perf_counter()
     hist = d.advance(dt, args.multiphase_steps)
     ms = 1e3 * (time.perf_counter() - t0) / args.multiphase_steps
     u, p, uf, g = d.fields()
     out = dict(workload="rising-bubble-multiphase-step-1x2-quads", cells=mesh.n_cells, ms_per_step=round(ms, 2),
-               steps=args.multiphase_steps, setup_s=round(setup_s, 2), mode=args.multiphase_mode,
+               steps=args.multiphase_steps, dt=args.multiphase_dt, setup_s=round(setup_s, 2), mode=args.multiphase_mode,
                pressure_preconditioner=args.multiphase_precond, pressure_correctors=args.multiphase_correctors,
                iters_gamma_u_p=[float(x) for x in hist.mean(0)[:3]], max_div=float("%.3g" % hist[:, 3].max()),
                max_u=float("%.4g" % np.abs(u).max()))
@@ -647,6 +647,7 @@ def main():
     ap.add_argument("--multiphase-cells", type=int, default=2_000_000,
                     help="C3 whole multiphase time steps with surface tension (0 skips)")
     ap.add_argument("--multiphase-steps", type=int, default=5)
+    ap.add_argument("--multiphase-dt", type=float, default=1e-4)
     ap.add_argument("--ib-flow-cells", type=int, default=1_000_000,
                     help="C5 FractionalStepGCIB time steps at this many cells (0 skips)")
     ap.add_argument("--ib-flow-steps", type=int, default=3)
