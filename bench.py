@@ -113,6 +113,13 @@ def clean(v):
 FALLBACK_RATIO = 0.5  # set from --fallback-ratio
 
 
+def apply_setup(s, args):
+    """The extra solver setup of --setup (key=value,...)."""
+    for item in filter(None, args.setup.split(",")):
+        key, value = item.split("=", 1)
+        s.setup(key, value)
+
+
 def solver_kw(mode):
     """Solver set-up of a method name "<precision>[-cg|-gmres][-mg]": precision
     fp64, mixed or learned; -cg preconditioned CG (Belos "CG") and -gmres
@@ -302,6 +309,7 @@ def learned_case(args):
         prec = kw["mode"]
         s = P.HipSparseMatrixSolver(0, tolerance=args.tol, maxIters=100000, innerTolerance=args.inner_tol,
                                     blockRows=args.block_rows, ordering=args.ordering, **kw)
+        apply_setup(s, args)
 
         def step():
             s.assemble_laplacian_device(mesh, 3, gamma_face=d_g, bc_type=d_bc, phi_face=d_pf)
@@ -368,9 +376,10 @@ def bubble_case(args):
     d_g, d_bc, d_pf, d_b = (P.DeviceArray(gamma), P.DeviceArray(bc), P.DeviceArray(np.zeros(mesh.n_faces)),
                             P.DeviceArray(rhs))
     out = dict(workload="rising-bubble-pressure-1x2-quads", cells=mesh.n_cells, density_ratio=1000.0)
-    for mode in ("fp64", "fp64-cg", "mixed-cg", "fp64-cg-mg", "mixed-cg-mg"):
+    for mode in args.bubble_modes.split(","):
         s = P.HipSparseMatrixSolver(0, tolerance=args.tol, maxIters=100000, blockRows=args.block_rows,
                                     ordering=args.ordering, **solver_kw(mode))
+        apply_setup(s, args)
 
         def step():
             s.assemble_laplacian_device(mesh, 3, gamma_face=d_g, bc_type=d_bc, phi_face=d_pf)
@@ -632,6 +641,9 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--inner-tol", type=float, default=0.1)
     ap.add_argument("--block-rows", type=int, default=32, help="rows per ILU aggregate (blocks pack aggregates to 64)")
+    ap.add_argument("--setup", default="",
+                    help="extra solver setup of the C2, C3-pressure and C4 methods: key=value,...")
+    ap.add_argument("--bubble-modes", default="fp64,fp64-cg,mixed-cg,fp64-cg-mg,mixed-cg-mg")
     ap.add_argument("--ordering", default="geometric", help="ILU aggregates: geometric (bricks) or graph")
     ap.add_argument("--modes", default="fp64,fp64-cg,mixed-cg,fp64-mg,fp64-cg-mg,mixed-cg-mg,learned-cg-mg",
                     help="precision fp64|mixed|learned, with -cg for preconditioned CG instead of BiCGStab and "
@@ -710,6 +722,7 @@ def main():
         prec = kw["mode"]
         s = P.HipSparseMatrixSolver(0, tolerance=args.tol, maxIters=50000, innerTolerance=args.inner_tol,
                                     blockRows=args.block_rows, ordering=args.ordering, **kw)
+        apply_setup(s, args)
 
         def step():
             s.assemble_laplacian_device(mesh, 3, gamma_face=d_g, bc_type=d_bc, phi_face=d_pf)

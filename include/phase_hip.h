@@ -122,7 +122,7 @@ int phase_hip_solver_destroy(phase_hip_solver *s);
  * are K cycles; V cycles below; 2), "multigridJacobiFrom" (coarse levels
  * from this index on, 0 being the first, smooth with damped Jacobi, the
  * others with ILU(0); AUTO: all with CG, none with BiCGStab),
- * "multigridJacobiWeight" (the damping, 2/3),
+ * "multigridJacobiWeight" (the damping, 0.8),
  * "multigridBlockJacobiFrom" (coarse ILU(0) levels from this index on use
  * the ILU(0) of their block-diagonal part in one launch instead of the
  * block-coloured ILU(0); 0), "multigridFinePostSmoothing" (1 | 0),

@@ -61,7 +61,7 @@ struct phase_hip_solver {
   int mlDenseRows = 256; // a level of at most this many rows is inverted densely
   int mlJacobiFrom = -1;  // coarse levels from this one (0 = all) smooth with damped Jacobi instead of ILU(0);
                           // -1: all with CG, none with BiCGStab
-  double mlJacobiWeight = 2.0 / 3.0;  // the damping weight
+  double mlJacobiWeight = 0.8;  // the damping weight (0.8: fewer iterations than 2/3 on every benchmarked system)
   int mlDensePivot = 0;  // 1: the coarsest inverse always pivots (0: only when a pivot is small)
 
   // pattern and ordering
