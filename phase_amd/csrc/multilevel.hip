@@ -500,29 +500,25 @@ __global__ __launch_bounds__(kRedBlock) void kMlResid(int n, int nSlices, const 
 }
 
 // rc[k] = sum of t over the finer rows of aggregate restRow[k] (identity when
-// null; an aggregate's rows are consecutive in the finer solver order), one
-// thread per coarse row, summing the rows in ascending order in FP64: each
-// lane streams its own contiguous rows.
+// null; an aggregate's rows are consecutive in the finer solver order), in
+// FP64: 16 lanes per coarse row, lane l summing rows l, l + 16, ... in order,
+// then a fixed xor tree over the 16 lanes (one or two load rounds for an
+// aggregate of about 32 rows, where a thread per aggregate took seven).
 template <class T>
 __global__ __launch_bounds__(256) void kMlRestrictRows(int nc, const int *__restrict__ restRow,
                                                        const int *__restrict__ aggPtr, const T *__restrict__ t,
                                                        T *__restrict__ rc, const Scalars *sc) {
   if (sc->done) return;
-  const int k = blockIdx.x * 256 + threadIdx.x;
-  if (k >= nc) return;
-  const int a = restRow ? restRow[k] : k;
-  const int r0 = aggPtr[a], r1 = aggPtr[a + 1];
+  const int k = blockIdx.x * 16 + (threadIdx.x >> 4), l = threadIdx.x & 15;
   double d = 0.;
-  int i = r0;
-  for (; i + 4 <= r1; i += 4) {
-    const T v0 = t[i], v1 = t[i + 1], v2 = t[i + 2], v3 = t[i + 3];
-    d += (double)v0;
-    d += (double)v1;
-    d += (double)v2;
-    d += (double)v3;
+  if (k < nc) {
+    const int a = restRow ? restRow[k] : k;
+    const int r1 = aggPtr[a + 1];
+    for (int i = aggPtr[a] + l; i < r1; i += 16) d += (double)t[i];
   }
-  for (; i < r1; ++i) d += (double)t[i];
-  rc[k] = (T)d;
+#pragma unroll
+  for (int o = 8; o > 0; o >>= 1) d += __shfl_xor(d, o, 16);
+  if (k < nc && l == 0) rc[k] = (T)d;
 }
 
 // e[i] += ec[pIdx[i]]
@@ -923,7 +919,7 @@ void coarseCorrection(phase_hip_solver *s, Multilevel &M, int l, const Scalars *
   MlLevel &L = M.lv[l];
   MlVecs<T> &V = vecs<T>(L);
   hipStream_t st = s->stream;
-  hipLaunchKernelGGL((kMlRestrictRows<T>), dim3((L.n + 255) / 256), dim3(256), 0, st, L.n,
+  hipLaunchKernelGGL((kMlRestrictRows<T>), dim3((L.n + 15) / 16), dim3(256), 0, st, L.n,
                      L.sol ? L.restRow.p : nullptr, L.finerAggPtr.p, t, V.r.p, sc);
   coarseSolve<T>(s, M, l, sc, V.r.p, V.e.p);
   hipLaunchKernelGGL((kMlProlongAdd<T>), dim3(elemGrid(L.nFiner)), dim3(256), 0, st, L.nFiner, L.pIdx.p,
