@@ -58,11 +58,13 @@ def test_rows_match_oracle(oracle, hip, case, kind):
 
 @pytest.mark.parametrize("solver", [{}, {"solver": "gmres"}, {"solver": "gmres", "preconditioner": "multigrid"},
                                     {"solver": "gmres", "gmresRestart": 8},
-                                    {"preconditioner": "multigrid"}])
+                                    {"preconditioner": "multigrid"}, {"mode": "mixed"},
+                                    {"mode": "mixed", "solver": "gmres", "preconditioner": "multigrid"}])
 def test_poisson_around_hole_on_device(oracle, hip, case, solver):
     """The ghost-cell rows make the system non-symmetric: BiCGStab (the
     reference's default) and flexible GMRES (its multigrid backend's), with
-    ILU(0) or the multigrid cycle, all agree with the direct solve."""
+    ILU(0) or the multigrid cycle, in FP64 or with FP32 inner iterations
+    inside FP64 refinement, all agree with the direct solve."""
     om, hm, og, dg = case
     N, F = om.n_cells, om.n_faces
     f = om.faces()
