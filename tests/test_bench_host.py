@@ -55,3 +55,9 @@ def test_roofline_kernel_and_traffic_from_committed_profiles():
     # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE passes: within
     # 0.8-1.3 of the kernel's 60,037,928 algorithmic bytes
     assert 0.8 * 60037928 < traffic < 1.3 * 60037928
+
+
+def test_metric_is_the_baseline_metric():
+    import json
+    with open(os.path.join(ROOT, "BASELINE.json")) as f:
+        assert bench.METRIC == json.load(f)["metric"]
